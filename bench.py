@@ -1,0 +1,221 @@
+"""Benchmark: two-flavour (Nf=2) Wilson-fermion HMC trajectories per second on a
+1024x1024 lattice on one MI355X, mixed-precision MD solves beside the plain
+FP64 form on identical random streams, plus the roofline position of the
+dominant kernel and the CPU oracle on the host cores.
+
+Workload (one "step" = one HMC trajectory): compact U(1) gauge action + Nf=2
+Wilson fermions, beta = 4.0, m0 = 0.2, leapfrog ns = 10, eps = 0.1, CG on
+(gamma5 Dw)^2 to |r|^2 < tol |b|^2, start from a hot configuration (link
+angles uniform in [-pi, pi), seed 1234) thermalised by pure-gauge OMF4 HMC.
+
+    python bench.py [--steps K] [--warmup W] [--L 1024] [--tol 1e-16]
+
+Prints ONE JSON line (rank 0).  Only N = 1 GPU is supported.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "dav-qed2d_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "HMC traj/s at 1024² Nf=2 (mixed vs FP64, same ΔH); Dw GB/s vs HBM peak"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per lattice site of each kernel class (each logical field
+# access counted once; FP64 complex = 16 B, FP32 complex = 8 B, 2 spin
+# components or 2 link directions per site).
+BYTES_PER_SITE = {
+    # p = r + beta p_old (reads r, p_old), t = Q p (reads U), writes p, t
+    "cg_k1_f64": 32 * 5, "cg_k1_f32": 16 * 5,
+    # Ap = Q t + mu^2 p (reads t, U, p), x += a p, r -= a Ap (x, r read+write)
+    "cg_k2_f64": 32 * 7, "cg_k2_f32": 16 * 7,
+    # so = Q si: si, U read, so written
+    "q_f64": 96,
+    # mom += eps (F_gauge + F_fermion(X, Q X)): U, X, Y read, mom read+write
+    "force_mom": 128,
+    # U <- exp(i eps p) U: U read+write, p read
+    "updt_fld": 80,
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--L", type=int, default=1024)
+    ap.add_argument("--beta", type=float, default=4.0)
+    ap.add_argument("--m0", type=float, default=0.2)
+    ap.add_argument("--ns", type=int, default=10)
+    ap.add_argument("--eps", type=float, default=0.1)
+    ap.add_argument("--tol", type=float, default=1e-16)
+    ap.add_argument("--therm", type=int, default=20)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile", type=int, default=4)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import qed2d as Q
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    L = args.L
+    prm = Q.LattParm((L, L), args.beta)
+    Q.lib.qed2d_ctx_set_tile(prm.ctx(), args.tile)
+    seed = 1234
+
+    # ---- start configuration: hot start + pure-gauge thermalisation --------
+    U = Q.spinor(prm)
+    Q.hot_start(U, prm, seed)
+    Q.seed(seed, chain=1)
+    t0 = time.time()
+    for _ in range(args.therm):
+        Q.HMC(U, 0.1, 10, [], prm, None)
+    torch.cuda.synchronize()
+    log(f"thermalised {args.therm} pure-gauge trajectories in {time.time() - t0:.2f}s, "
+        f"plaquette {Q.Plaquette(U, prm):.6f}")
+    U0 = U.clone()
+
+    def run(precision, label):
+        U.copy_(U0)
+        Q.seed(seed, chain=0)
+        acc = []
+        infos = []
+        for _ in range(args.warmup):
+            Q.HMC(U, args.m0, args.eps, args.ns, acc, 100000, args.tol, prm, None,
+                  precision=precision)
+        torch.cuda.synchronize()
+        Q.profile_enable(prm, True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            infos.append(Q.HMC(U, args.m0, args.eps, args.ns, acc, 100000, args.tol, prm, None,
+                               precision=precision))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        prof = Q.profile_read(prm)
+        Q.profile_enable(prm, False)
+        it = sum(i.cg_iters for i in infos) / len(infos)
+        log(f"{label}: {args.steps} traj in {dt:.3f}s = {args.steps / dt:.3f} traj/s, "
+            f"{it:.0f} CG it/traj, acc {sum(i.accepted for i in infos)}/{len(infos)}")
+        for k, (n, ms) in prof.items():
+            if n:
+                log(f"   {k:16s} {n:7d} launches {ms:9.2f} ms  avg {1000 * ms / n:8.2f} us")
+        return dt, infos, prof
+
+    dt_mx, inf_mx, prof_mx = run(Q.PREC_MIXED, "mixed")
+    res = {}
+    if not args.no_fp64:
+        dt_64, inf_64, prof_64 = run(Q.PREC_F64, "fp64")
+        dmax = max(abs(a.dH - b.dH) for a, b in zip(inf_mx, inf_64))
+        res["fp64_traj_per_s"] = args.steps / dt_64
+        res["speedup_vs_fp64"] = dt_64 / dt_mx
+        res["max_abs_dH_diff_vs_fp64"] = dmax
+        res["accept_identical_vs_fp64"] = [a.accepted for a in inf_mx] == [b.accepted for b in inf_64]
+        res["cg_iters_per_traj_fp64"] = sum(i.cg_iters for i in inf_64) / len(inf_64)
+    res["cg_iters_per_traj_mixed"] = sum(i.cg_iters for i in inf_mx) / len(inf_mx)
+    res["dH_mixed"] = [round(i.dH, 10) for i in inf_mx]
+    res["acceptance"] = sum(i.accepted for i in inf_mx) / len(inf_mx)
+
+    # ---- roofline of the dominant kernel of the timed mixed run -------------
+    V = L * L
+    dom = max((k for k in prof_mx if k in BYTES_PER_SITE), key=lambda k: prof_mx[k][1])
+    n, ms = prof_mx[dom]
+    avg_s = ms / n / 1000.0
+    achieved = BYTES_PER_SITE[dom] * V / avg_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": dom, "avg_us": round(avg_s * 1e6, 2),
+            "bytes_per_site": BYTES_PER_SITE[dom], "sites": V}
+    kern = {}
+    for k, (n, ms) in prof_mx.items():
+        if n and k in BYTES_PER_SITE:
+            a = BYTES_PER_SITE[k] * V / (ms / n / 1000.0) / 1e9
+            kern[k] = {"avg_us": round(1000 * ms / n, 2), "GBs": round(a, 1), "launches": n}
+    res["kernels_mixed"] = kern
+
+    # ---- CPU oracle on the host cores (bounded sample) ----------------------
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, U0, prm)
+
+    ms_step = 1000.0 * dt_mx / args.steps
+    out = {
+        "metric": METRIC,
+        "value": round(args.steps / dt_mx, 4),
+        "unit": "traj/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 (FP32 CG iterations, FP64 reliable updates, forces, H)",
+        "data": "synthetic",
+        "config": {"workload": f"Nf=2 HMC {L}x{L} beta={args.beta} m0={args.m0} leapfrog "
+                               f"ns={args.ns} eps={args.eps} CG tol={args.tol} (squared "
+                               f"relative); hot start seed {seed} + {args.therm} pure-gauge "
+                               f"OMF4 trajectories",
+                   "lattice": [L, L], "precision": "mixed", "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    out.update(res)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(args, U0, prm):
+    """FP64 CPU oracle (C + OpenMP) on this host: one MD force evaluation of
+    the same workload (CG solve from x = 0, Q X, fermion + gauge force,
+    momentum update) and one link update, timed and scaled to trajectories/s
+    with the leapfrog's ns + 1 force evaluations and ns link updates."""
+    import numpy as np
+    from oracle import qed2d_oracle as O
+
+    import qed2d as Q
+
+    L = args.L
+    U = U0.cpu().numpy().copy()
+    mom = O.momenta(L, L, 1234, 0, 0)
+    X = O.pseudofermion(L, L, 1234, 0, 0)
+    F = O.gamm5Dw(U, X, args.m0)
+    p = O.latt(L, L, args.beta)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    g5 = np.zeros_like(X)
+    t = time.perf_counter()
+    it = O.lib.orc_update_momenta_nf2(O._ptr(mom), O._ptr(U), O._ptr(X), O._ptr(F), O._ptr(g5),
+                                      args.m0, args.eps, 100000, args.tol, O.C.byref(p))
+    t_force = time.perf_counter() - t
+    t = time.perf_counter()
+    O.lib.orc_updt_fld(O._ptr(U), O._ptr(mom), args.eps, L, L)
+    t_updt = time.perf_counter() - t
+    per_traj = (args.ns + 1) * t_force + args.ns * t_updt
+    log(f"cpu oracle: force eval {t_force:.2f}s ({it} CG it), link update {t_updt:.3f}s, "
+        f"{threads} threads -> {1.0 / per_traj:.5f} traj/s")
+    return {"value": round(1.0 / per_traj, 6), "unit": "traj/s", "cores": threads,
+            "kind": "port",
+            "sample": f"one MD force evaluation ({it} FP64 CG iterations + forces + momentum "
+                      f"update) and one link update at {L}x{L}, scaled by the leapfrog's "
+                      f"{args.ns + 1} force evaluations and {args.ns} link updates per "
+                      f"trajectory"}
+
+
+if __name__ == "__main__":
+    main()

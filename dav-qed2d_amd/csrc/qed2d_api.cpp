@@ -1,0 +1,1359 @@
+// qed2d_api.cpp -- context management, solver / MD / HMC orchestration and
+// the C ABI of libqed2d_hip (declared in include/qed2d.h).
+//
+// The host side only sequences kernels on the context's stream.  Solvers
+// keep their scalars on the device (SolverState) and the host looks at them
+// once per chunk of iterations, so a CG iteration costs two kernel launches
+// and no synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qed2d.h"
+#include "qed2d_internal.h"
+
+using namespace qed2d;
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+enum SpinorSlot {
+    S_R, S_P0, S_P1, S_T, S_TMP, S_TMP2, S_UCP, S_X, S_F, S_G5DX, S_AUX, S_PB, S_PBA, S_PBL,
+    S_NSLOTS
+};
+enum FloatSlot { FS_R, FS_P0, FS_P1, FS_T, FS_X, FS_U, FS_NSLOTS };
+enum RealSlot { R_MOM, R_FACC, R_NSLOTS };
+
+struct qed2d_ctx {
+    Latt L;
+    double beta = 0.0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int sync = 1;
+    int rows = 4;
+    std::string err;
+    double2 *spin[S_NSLOTS] = {};
+    float2 *fspin[FS_NSLOTS] = {};
+    double *real[R_NSLOTS] = {};
+    std::vector<double2 *> flav;  // RHMC pseudofermions, one per flavour
+    double *part = nullptr;
+    long part_len = 0;
+    SolverState *state = nullptr;
+    RedOut *red = nullptr;
+    SolverState *hstate = nullptr;  // pinned
+    RedOut *hred = nullptr;         // pinned
+    // kernel-class timing (qed2d_profile_*)
+    bool prof = false;
+    std::vector<hipEvent_t> evpool;
+    struct Pending { int cls; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    double prof_ms[QED2D_PROF_NCLASS] = {};
+    long prof_n[QED2D_PROF_NCLASS] = {};
+};
+
+namespace {
+
+int fail(qed2d_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((c), QED2D_EHIP, "%s: %s", #call, hipGetErrorString(e_));         \
+    } while (0)
+
+int launched(qed2d_ctx *c)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, QED2D_EHIP, "kernel launch: %s", hipGetErrorString(e));
+    return QED2D_OK;
+}
+
+void prof_harvest(qed2d_ctx *c);
+
+int finish(qed2d_ctx *c)
+{
+    int rc = launched(c);
+    if (rc) return rc;
+    if (c->sync) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        prof_harvest(c);
+    }
+    return QED2D_OK;
+}
+
+long spinor_elems(const qed2d_ctx *c) { return 2 * c->L.V; }  // complex numbers
+
+int get_spin(qed2d_ctx *c, int slot, double2 **out)
+{
+    if (!c->spin[slot]) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(double2) * spinor_elems(c)) != hipSuccess)
+            return fail(c, QED2D_ENOMEM, "hipMalloc of a %ld-byte spinor failed",
+                        (long)(sizeof(double2) * spinor_elems(c)));
+        c->spin[slot] = (double2 *)p;
+    }
+    *out = c->spin[slot];
+    return QED2D_OK;
+}
+
+int get_fspin(qed2d_ctx *c, int slot, float2 **out)
+{
+    if (!c->fspin[slot]) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(float2) * spinor_elems(c)) != hipSuccess)
+            return fail(c, QED2D_ENOMEM, "hipMalloc of an FP32 spinor failed");
+        c->fspin[slot] = (float2 *)p;
+    }
+    *out = c->fspin[slot];
+    return QED2D_OK;
+}
+
+int get_real(qed2d_ctx *c, int slot, double **out)
+{
+    if (!c->real[slot]) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(double) * 2 * c->L.V) != hipSuccess)
+            return fail(c, QED2D_ENOMEM, "hipMalloc of a real field failed");
+        c->real[slot] = (double *)p;
+    }
+    *out = c->real[slot];
+    return QED2D_OK;
+}
+
+#define TRY(x)                      \
+    do {                            \
+        int rc_ = (x);              \
+        if (rc_ != QED2D_OK) return rc_; \
+    } while (0)
+
+// ---- kernel-class timing ---------------------------------------------------
+hipEvent_t prof_event(qed2d_ctx *c)
+{
+    if (c->evpool.empty()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    hipEvent_t e = c->evpool.back();
+    c->evpool.pop_back();
+    return e;
+}
+
+// Opens a timed region of class cls; returns the start event (or null).
+hipEvent_t prof_begin(qed2d_ctx *c)
+{
+    if (!c->prof) return nullptr;
+    hipEvent_t a = prof_event(c);
+    if (a && hipEventRecord(a, c->stream) != hipSuccess) return nullptr;
+    return a;
+}
+
+void prof_end(qed2d_ctx *c, int cls, hipEvent_t a)
+{
+    if (!c->prof || !a) return;
+    hipEvent_t b = prof_event(c);
+    if (!b || hipEventRecord(b, c->stream) != hipSuccess) return;
+    c->pending.push_back({cls, a, b});
+}
+
+// Called after the stream has been synchronised: fold finished timings in.
+void prof_harvest(qed2d_ctx *c)
+{
+    for (auto &p : c->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            c->prof_ms[p.cls] += ms;
+            c->prof_n[p.cls] += 1;
+        }
+        c->evpool.push_back(p.a);
+        c->evpool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+#define PROF(c, cls, launch)                 \
+    do {                                     \
+        hipEvent_t ev_ = prof_begin(c);      \
+        launch;                              \
+        prof_end((c), (cls), ev_);           \
+    } while (0)
+
+int read_state(qed2d_ctx *c)
+{
+    HIPCHK(c, hipMemcpyAsync(c->hstate, c->state, sizeof(SolverState), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_harvest(c);
+    return QED2D_OK;
+}
+
+int read_red(qed2d_ctx *c)
+{
+    HIPCHK(c, hipMemcpyAsync(c->hred, c->red, sizeof(RedOut), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_harvest(c);
+    return QED2D_OK;
+}
+
+// Host double-double helpers (same TwoSum as the device).
+struct hdd {
+    double hi = 0.0, lo = 0.0;
+    void add(double b)
+    {
+        dd a{hi, lo};
+        a = dd_add(a, b);
+        hi = a.hi;
+        lo = a.lo;
+    }
+    void add(double bh, double bl)
+    {
+        dd a{hi, lo}, b{bh, bl};
+        a = dd_add(a, b);
+        hi = a.hi;
+        lo = a.lo;
+    }
+    double value() const { return hi + lo; }
+};
+
+// ---------------------------------------------------------------------------
+// elementary operators
+// ---------------------------------------------------------------------------
+void q64(qed2d_ctx *c, const double *U, const double *si, double *so, double am0)
+{
+    PROF(c, QED2D_PROF_Q_F64,
+         launch_q_apply_f64(c->L, (const double2 *)U, (const double2 *)si, (double2 *)so, am0,
+                            c->rows, c->stream));
+}
+
+// so = Q^2 si (+ mu^2 si), tmp = Q si
+void qsq64(qed2d_ctx *c, const double *U, const double *si, double *so, double *tmp, double am0,
+           double mu2)
+{
+    q64(c, U, si, tmp, am0);
+    launch_q_shift_f64(c->L, (const double2 *)U, (const double2 *)tmp, (const double2 *)si,
+                       (double2 *)so, am0, mu2, c->rows, c->stream);
+}
+
+int dot_host(qed2d_ctx *c, const double *a, const double *b, double *re, double *im)
+{
+    launch_dot((const double2 *)a, (const double2 *)b, spinor_elems(c), c->red, c->part,
+               c->stream);
+    TRY(launched(c));
+    TRY(read_red(c));
+    *re = c->hred->v[0];
+    if (im) *im = c->hred->v[1];
+    return QED2D_OK;
+}
+
+int norm2_host(qed2d_ctx *c, const double *a, double *n2)
+{
+    launch_norm2((const double2 *)a, spinor_elems(c), c->red, c->part, c->stream);
+    TRY(launched(c));
+    TRY(read_red(c));
+    *n2 = c->hred->v[0];
+    return QED2D_OK;
+}
+
+// Sum Re P, sum arg P, sum p^2 as double-doubles.
+int observables(qed2d_ctx *c, const double *U, const double *mom, hdd *sp, hdd *sq, hdd *sm)
+{
+    launch_observables(c->L, (const double2 *)U, mom, nullptr, nullptr, c->red, c->part,
+                       c->stream);
+    TRY(launched(c));
+    TRY(read_red(c));
+    if (sp) { *sp = hdd(); sp->add(c->hred->v[0], c->hred->v[1]); }
+    if (sq) { *sq = hdd(); sq->add(c->hred->v[2], c->hred->v[3]); }
+    if (sm) { *sm = hdd(); sm->add(c->hred->v[4], c->hred->v[5]); }
+    return QED2D_OK;
+}
+
+// H = sum p^2 / 2 + beta (V - sum Re P)   (src/QED2dHMC.jl:198-215), returned
+// as a double-double so that differences of two H stay exact to ~1e-16 |H|.
+int hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, hdd *H)
+{
+    hdd sp, sm;
+    TRY(observables(c, U, mom, &sp, nullptr, &sm));
+    // beta (V - sum ReP): form V - sumReP in double-double first
+    hdd g;
+    g.add((double)c->L.V);
+    g.add(-sp.hi, -sp.lo);
+    double gh = g.hi, gl = g.lo;
+    // multiply by beta with an FMA-exact product of the high part
+    double ph = c->beta * gh;
+    double pe = std::fma(c->beta, gh, -ph);
+    hdd h;
+    h.add(0.5 * sm.hi, 0.5 * sm.lo);
+    h.add(ph, pe + c->beta * gl);
+    *H = h;
+    return QED2D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CG drivers
+// ---------------------------------------------------------------------------
+struct SolveArgs {
+    double am0 = 0.0, mu2 = 0.0, eps = 0.0;
+    int maxiter = 0, guess = 0;
+};
+
+int upload_state(qed2d_ctx *c, const SolveArgs &a)
+{
+    SolverState &s = *c->hstate;
+    std::memset(&s, 0, sizeof s);
+    s.mu2 = a.mu2;
+    s.eps = a.eps;
+    s.maxiter = a.maxiter;
+    s.delta2 = 1e-2;  // reliable update once |r| has dropped 10x since the last one
+    HIPCHK(c, hipMemcpyAsync(c->state, c->hstate, sizeof(SolverState), hipMemcpyHostToDevice,
+                             c->stream));
+    return QED2D_OK;
+}
+
+// Number of iterations to queue before looking at the state again: an
+// estimate of what is left from the observed convergence rate.
+int next_chunk(const SolverState &s, double prev_err, int prev_iter, int done_iters)
+{
+    int chunk = 8;
+    if (done_iters > 0 && prev_err > 0.0 && s.err > 0.0 && s.iter > prev_iter && s.err < prev_err) {
+        double rate = std::log(s.err / prev_err) / (double)(s.iter - prev_iter);
+        double left = std::log(s.tol / s.err) / rate;
+        if (left > 0.0 && left < 1e6) chunk = (int)std::ceil(left);
+    }
+    if (chunk < 4) chunk = 4;
+    if (chunk > 64) chunk = 64;
+    return chunk;
+}
+
+int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const SolveArgs &a,
+           int *iters)
+{
+    double2 *r, *P[2], *t;
+    TRY(get_spin(c, S_R, &r));
+    TRY(get_spin(c, S_P0, &P[0]));
+    TRY(get_spin(c, S_P1, &P[1]));
+    TRY(get_spin(c, S_T, &t));
+    TRY(upload_state(c, a));
+    const Latt &L = c->L;
+    hipStream_t st = c->stream;
+    if (a.guess) {
+        q64(c, U, (const double *)x, (double *)t, a.am0);
+        launch_cg_guess_f64(L, (const double2 *)U, (const double2 *)b, (const double2 *)x, t, r,
+                            P[0], a.am0, c->state, c->part, c->rows, st);
+    } else {
+        launch_cg_init_f64(L, (const double2 *)b, (double2 *)x, r, P[0], c->state, c->part, st);
+    }
+    TRY(launched(c));
+    TRY(read_state(c));
+    int it = 1;
+    double prev_err = c->hstate->norm;
+    int prev_iter = 0;
+    while (c->hstate->status == 0) {
+        int chunk = next_chunk(*c->hstate, prev_err, prev_iter, it - 1);
+        prev_err = (it == 1) ? c->hstate->norm : c->hstate->err;
+        prev_iter = c->hstate->iter;
+        for (int j = 0; j < chunk && it <= a.maxiter; ++j, ++it) {
+            PROF(c, QED2D_PROF_CG_K1_F64,
+                 launch_cg_k1_f64(L, (const double2 *)U, r, P[(it - 1) & 1], P[it & 1], t,
+                                  a.am0, c->state, c->part, c->rows, st));
+            PROF(c, QED2D_PROF_CG_K2_F64,
+                 launch_cg_k2_f64(L, (const double2 *)U, t, P[it & 1], (double2 *)x, r, a.am0,
+                                  c->state, c->part, c->rows, 0, st));
+        }
+        TRY(launched(c));
+        TRY(read_state(c));
+        if (it > a.maxiter && c->hstate->status == 0) break;
+    }
+    *iters = c->hstate->iter;
+    if (c->hstate->status == 1) return QED2D_OK;
+    return fail(c, QED2D_ENOCONV, "CG not converged after %d iterations (|r|^2 = %g, tol = %g)",
+                a.maxiter, c->hstate->err, c->hstate->tol);
+}
+
+int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const SolveArgs &a,
+             int *iters)
+{
+    float2 *rs, *P[2], *ts, *xs, *Us;
+    double2 *t;
+    TRY(get_fspin(c, FS_R, &rs));
+    TRY(get_fspin(c, FS_P0, &P[0]));
+    TRY(get_fspin(c, FS_P1, &P[1]));
+    TRY(get_fspin(c, FS_T, &ts));
+    TRY(get_fspin(c, FS_X, &xs));
+    TRY(get_fspin(c, FS_U, &Us));
+    TRY(get_spin(c, S_T, &t));
+    TRY(upload_state(c, a));
+    const Latt &L = c->L;
+    hipStream_t st = c->stream;
+    const float am0f = (float)a.am0;
+    launch_to_f32((const double2 *)U, Us, 2 * L.V, st);
+    if (a.guess) {
+        HIPCHK(c, hipMemsetAsync(xs, 0, sizeof(float2) * 2 * L.V, st));
+        q64(c, U, x, (double *)t, a.am0);
+        launch_ru_residual(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x, rs,
+                           P[0], a.am0, c->state, c->part, c->rows, 1, st);
+    } else {
+        HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double2) * 2 * L.V, st));
+        launch_cg_init_mixed(L, (const double2 *)b, rs, xs, P[0], c->state, c->part, st);
+    }
+    TRY(launched(c));
+    TRY(read_state(c));
+    int it = 1;
+    double prev_err = c->hstate->norm;
+    int prev_iter = 0;
+    while (c->hstate->status == 0) {
+        if (c->hstate->need_ru) {
+            hipEvent_t ev = prof_begin(c);
+            launch_ru_accum(L, (double2 *)x, xs, c->state, st);
+            launch_q_apply_f64(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->rows, st);
+            launch_ru_residual(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
+                               rs, nullptr, a.am0, c->state, c->part, c->rows, 0, st);
+            prof_end(c, QED2D_PROF_RELIABLE, ev);
+            TRY(launched(c));
+            TRY(read_state(c));
+            continue;
+        }
+        int chunk = next_chunk(*c->hstate, prev_err, prev_iter, it - 1);
+        prev_err = (it == 1) ? c->hstate->norm : c->hstate->err;
+        prev_iter = c->hstate->iter;
+        for (int j = 0; j < chunk; ++j, ++it) {
+            PROF(c, QED2D_PROF_CG_K1_F32,
+                 launch_cg_k1_f32(L, Us, rs, P[(it - 1) & 1], P[it & 1], ts, am0f, c->state,
+                                  c->part, c->rows, st));
+            PROF(c, QED2D_PROF_CG_K2_F32,
+                 launch_cg_k2_f32(L, Us, ts, P[it & 1], xs, rs, am0f, c->state, c->part, c->rows,
+                                  1, st));
+        }
+        TRY(launched(c));
+        TRY(read_state(c));
+    }
+    *iters = c->hstate->iter;
+    if (c->hstate->status == 1) return QED2D_OK;
+    return fail(c, QED2D_ENOCONV,
+                "mixed-precision CG not converged after %d iterations (|r|^2 = %g, tol = %g)",
+                a.maxiter, c->hstate->err, c->hstate->tol);
+}
+
+int cg_any(qed2d_ctx *c, double *x, const double *U, const double *b, double am0, double mu,
+           int op, int maxiter, double eps, int guess, int precision, int *iters)
+{
+    SolveArgs a;
+    a.am0 = am0;
+    a.mu2 = (op == QED2D_OP_SQR_MUSQ) ? mu * mu : 0.0;
+    a.eps = eps;
+    a.maxiter = maxiter;
+    a.guess = guess;
+    int dummy = 0;
+    if (!iters) iters = &dummy;
+    if (precision == QED2D_PREC_MIXED) return cg_mixed(c, x, U, b, a, iters);
+    return cg_f64(c, x, U, b, a, iters);
+}
+
+// ---------------------------------------------------------------------------
+// MD
+// ---------------------------------------------------------------------------
+const double OMF_R1 = 0.08398315262876693;
+const double OMF_R2 = 0.2539785108410595;
+const double OMF_R3 = 0.6822365335719091;
+const double OMF_R4 = -0.03230286765269967;
+
+void updt(qed2d_ctx *c, double *U, const double *mom, double eps)
+{
+    PROF(c, QED2D_PROF_UPDT_FLD, launch_updt_fld(c->L, (double2 *)U, mom, eps, c->stream));
+}
+
+// update_momenta! for Nf=2 (src/QED2dHMC.jl:452-481)
+int update_momenta_nf2(qed2d_ctx *c, double *mom, const double *U, double *X, const double *F,
+                       double *g5DX, double am0, double eps, int maxiter, double tol,
+                       int precision, int *iters)
+{
+    int it = 0;
+    TRY(cg_any(c, X, U, F, am0, 0.0, QED2D_OP_SQR_MSQ, maxiter, tol, 0, precision, &it));
+    *iters += it;
+    q64(c, U, X, g5DX, am0);
+    PROF(c, QED2D_PROF_FORCE_MOM,
+         launch_force_mom(c->L, (const double2 *)U, (const double2 *)X, (const double2 *)g5DX,
+                          nullptr, mom, c->beta, eps, 1.0, c->stream));
+    return launched(c);
+}
+
+int md_nf2(qed2d_ctx *c, int integrator, double *mom, double *U, double *X, const double *F,
+           double *g5DX, double am0, double eps, int ns, int maxiter, double tol, int precision,
+           int *iters)
+{
+    *iters = 0;
+    if (integrator == QED2D_OMF4) {
+        // src/QED2dHMC.jl:307-422
+        const double r1 = OMF_R1, r2 = OMF_R2, r3 = OMF_R3, r4 = OMF_R4;
+        const double r5 = 0.5 - r1 - r3, r6 = 1.0 - 2.0 * (r2 + r4);
+        const double pc[6] = {r1, r3, r5, r5, r3, r1};
+        const double uc[5] = {r2, r4, r6, r4, r2};
+        for (int i = 0; i < ns; ++i) {
+            for (int s = 0; s < 6; ++s) {
+                TRY(update_momenta_nf2(c, mom, U, X, F, g5DX, am0, pc[s] * eps, maxiter, tol,
+                                       precision, iters));
+                if (s < 5) updt(c, U, mom, eps * uc[s]);
+            }
+        }
+        return launched(c);
+    }
+    // leapfrog, src/QED2dHMC.jl:426-449
+    TRY(update_momenta_nf2(c, mom, U, X, F, g5DX, am0, 0.5 * eps, maxiter, tol, precision,
+                           iters));
+    for (int i = 0; i < ns - 1; ++i) {
+        updt(c, U, mom, eps);
+        TRY(update_momenta_nf2(c, mom, U, X, F, g5DX, am0, eps, maxiter, tol, precision, iters));
+    }
+    updt(c, U, mom, eps);
+    TRY(update_momenta_nf2(c, mom, U, X, F, g5DX, am0, 0.5 * eps, maxiter, tol, precision,
+                           iters));
+    return launched(c);
+}
+
+void gauge_mom(qed2d_ctx *c, double *mom, const double *U, double coef)
+{
+    launch_force_mom(c->L, (const double2 *)U, nullptr, nullptr, nullptr, mom, c->beta, coef, 0.0,
+                     c->stream);
+}
+
+// OMF4! pure gauge (src/QED2dHMC.jl:239-305)
+int omf4_pure(qed2d_ctx *c, double *mom, double *U, double eps, int ns)
+{
+    const double r1 = OMF_R1, r2 = OMF_R2, r3 = OMF_R3, r4 = OMF_R4;
+    const double r5 = 0.5 - r1 - r3, r6 = 1.0 - 2.0 * (r2 + r4);
+    const double pc[6] = {r1, r3, r5, r5, r3, r1};
+    const double uc[5] = {r2, r4, r6, r4, r2};
+    for (int i = 0; i < ns; ++i) {
+        for (int s = 0; s < 6; ++s) {
+            gauge_mom(c, mom, U, pc[s] * eps);
+            if (s < 5) updt(c, U, mom, eps * uc[s]);
+        }
+    }
+    return launched(c);
+}
+
+// Leapfrog for the pure gauge theory (stage order of src/QED2dHMC.jl:426-449).
+int leapfrog_pure(qed2d_ctx *c, double *mom, double *U, double eps, int ns)
+{
+    gauge_mom(c, mom, U, 0.5 * eps);
+    for (int i = 0; i < ns - 1; ++i) {
+        updt(c, U, mom, eps);
+        gauge_mom(c, mom, U, eps);
+    }
+    updt(c, U, mom, eps);
+    gauge_mom(c, mom, U, 0.5 * eps);
+    return launched(c);
+}
+
+int metropolis(qed2d_ctx *c, double hini, double hfin, int qzero, const double *U, uint64_t seed,
+               uint32_t chain, uint32_t traj, int *accepted)
+{
+    double pacc;
+    bool veto = false;
+    if (qzero) {
+        hdd sq;
+        TRY(observables(c, U, nullptr, nullptr, &sq, nullptr));
+        veto = std::fabs(sq.value() / (2.0 * M_PI)) > 0.1;
+    }
+    pacc = veto ? 0.0 : std::exp(-(hfin - hini));
+    if (pacc < 1.0) {
+        double r = qed2d_metropolis_uniform(seed, chain, traj);
+        *accepted = (pacc > r) ? 1 : 0;
+    } else {
+        *accepted = 1;
+    }
+    return QED2D_OK;
+}
+
+void key_of(uint64_t seed, uint32_t chain, uint32_t *k0, uint32_t *k1)
+{
+    *k0 = (uint32_t)seed;
+    *k1 = (uint32_t)(seed >> 32) ^ chain;
+}
+
+// RHMC force (src/QED2dHMC.jl:516-557)
+int update_momenta_rhmc(qed2d_ctx *c, double *mom, const double *U, double *X,
+                        const std::vector<double2 *> &F, double *g5DX, const double *am0, int nf,
+                        const int *npole, const double *const *mu, const double *const *rho,
+                        double eps, int maxiter, double tol)
+{
+    double *facc = nullptr;
+    TRY(get_real(c, R_FACC, &facc));
+    bool first = true;
+    for (int j = 0; j < nf; ++j) {
+        int guess = 0;
+        for (int i = 0; i < npole[j]; ++i) {
+            int it = 0;
+            TRY(cg_any(c, X, U, (const double *)F[j], am0[j], mu[j][i], QED2D_OP_SQR_MUSQ, maxiter,
+                       tol, guess, QED2D_PREC_F64, &it));
+            guess = 1;
+            q64(c, U, X, g5DX, am0[j]);
+            launch_force_acc(c->L, (const double2 *)U, (const double2 *)X, (const double2 *)g5DX,
+                             facc, rho[j][i], first ? 1 : 0, c->stream);
+            first = false;
+        }
+    }
+    launch_force_mom(c->L, (const double2 *)U, nullptr, nullptr, first ? nullptr : facc, mom,
+                     c->beta, eps, 0.0, c->stream);
+    return launched(c);
+}
+
+int leapfrog_rhmc(qed2d_ctx *c, double *mom, double *U, double *X, const std::vector<double2 *> &F,
+                  double *g5DX, const double *am0, int nf, const int *npole,
+                  const double *const *mu, const double *const *rho, double eps, int ns,
+                  int maxiter, double tol)
+{
+    TRY(update_momenta_rhmc(c, mom, U, X, F, g5DX, am0, nf, npole, mu, rho, 0.5 * eps, maxiter,
+                            tol));
+    for (int i = 0; i < ns - 1; ++i) {
+        updt(c, U, mom, eps);
+        TRY(update_momenta_rhmc(c, mom, U, X, F, g5DX, am0, nf, npole, mu, rho, eps, maxiter,
+                                tol));
+    }
+    updt(c, U, mom, eps);
+    return update_momenta_rhmc(c, mom, U, X, F, g5DX, am0, nf, npole, mu, rho, 0.5 * eps,
+                               maxiter, tol);
+}
+
+int multicg_impl(qed2d_ctx *c, double *so, const double *U, const double *si, double am0, int n,
+                 const double *mu, const double *rho, int maxiter, double eps)
+{
+    double2 *aux;
+    TRY(get_spin(c, S_AUX, &aux));
+    const long n2 = spinor_elems(c);
+    launch_copy((double2 *)so, (const double2 *)si, n2, c->stream);
+    for (int j = 0; j < n; ++j) {
+        int it;
+        TRY(cg_any(c, (double *)aux, U, si, am0, mu[j], QED2D_OP_SQR_MUSQ, maxiter, eps, 0,
+                   QED2D_PREC_F64, &it));
+        launch_scale_add((double2 *)so, 1.0, aux, rho[j], n2, c->stream);
+    }
+    return launched(c);
+}
+
+int gen_pf_impl(qed2d_ctx *c, double *F, const double *U, const double *X, double am0, int n,
+                const double *mu, const double *nu, int maxiter, double tol)
+{
+    double2 *aux;
+    TRY(get_spin(c, S_TMP2, &aux));
+    const long n2 = spinor_elems(c);
+    hipStream_t st = c->stream;
+    launch_copy((double2 *)F, (const double2 *)X, n2, st);
+    for (int i = 0; i < n; ++i) {
+        launch_copy(aux, (const double2 *)F, n2, st);
+        q64(c, U, (const double *)aux, F, am0);
+        launch_axpy_c((double2 *)F, aux, 0.0, mu[i], n2, st);
+        launch_copy(aux, (const double2 *)F, n2, st);
+        int it;
+        TRY(cg_any(c, F, U, (const double *)aux, am0, nu[i], QED2D_OP_SQR_MUSQ, maxiter, tol, 0,
+                   QED2D_PREC_F64, &it));
+        launch_copy(aux, (const double2 *)F, n2, st);
+        q64(c, U, (const double *)aux, F, am0);
+        launch_axpy_c((double2 *)F, aux, 0.0, -nu[i], n2, st);
+    }
+    return launched(c);
+}
+
+bool overlaps(const void *a, const void *b) { return a && b && a == b; }
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int qed2d_ctx_create(qed2d_ctx **out, int L1, int L2, double beta, int bc_flags, void *stream)
+{
+    if (!out) return QED2D_EINVAL;
+    *out = nullptr;
+    if (L1 < 2 || L2 < 2 || (long)L1 * L2 > (1L << 31)) return QED2D_EINVAL;
+    if (bc_flags & ~QED2D_BC_APERIODIC_T) return QED2D_EINVAL;
+    qed2d_ctx *c = new qed2d_ctx();
+    c->L.L1 = L1;
+    c->L.L2 = L2;
+    c->L.V = (long)L1 * L2;
+    c->L.bc = bc_flags;
+    c->beta = beta;
+    c->stream = (hipStream_t)stream;
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return QED2D_EHIP; }
+    // partial sums: the largest marching grid (rows = 1) needs 2 doubles per
+    // block, the observables 6 per block of at most 1024.
+    MarchGrid g = march_grid(c->L, 1);
+    c->part_len = 2L * g.nblocks() + 8 * 1024 + 64;
+    bool ok = hipMalloc((void **)&c->part, sizeof(double) * c->part_len) == hipSuccess &&
+              hipMalloc((void **)&c->state, sizeof(SolverState)) == hipSuccess &&
+              hipMalloc((void **)&c->red, sizeof(RedOut)) == hipSuccess &&
+              hipHostMalloc((void **)&c->hstate, sizeof(SolverState), hipHostMallocDefault) ==
+                  hipSuccess &&
+              hipHostMalloc((void **)&c->hred, sizeof(RedOut), hipHostMallocDefault) == hipSuccess;
+    if (!ok) { qed2d_ctx_destroy(c); return QED2D_ENOMEM; }
+    // tickets start at zero; every kernel that takes one resets it.
+    ok = hipMemsetAsync(c->state, 0, sizeof(SolverState), c->stream) == hipSuccess &&
+         hipMemsetAsync(c->red, 0, sizeof(RedOut), c->stream) == hipSuccess &&
+         hipStreamSynchronize(c->stream) == hipSuccess;
+    if (!ok) { qed2d_ctx_destroy(c); return QED2D_EHIP; }
+    *out = c;
+    return QED2D_OK;
+}
+
+void qed2d_ctx_destroy(qed2d_ctx *c)
+{
+    if (!c) return;
+    (void)(c->stream ? hipStreamSynchronize(c->stream) : hipDeviceSynchronize());
+    for (auto &p : c->spin) if (p) (void)hipFree(p);
+    for (auto &p : c->fspin) if (p) (void)hipFree(p);
+    for (auto &p : c->real) if (p) (void)hipFree(p);
+    for (auto p : c->flav) if (p) (void)hipFree(p);
+    if (c->part) (void)hipFree(c->part);
+    if (c->state) (void)hipFree(c->state);
+    if (c->red) (void)hipFree(c->red);
+    if (c->hstate) (void)hipHostFree(c->hstate);
+    if (c->hred) (void)hipHostFree(c->hred);
+    for (auto &p : c->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : c->evpool) (void)hipEventDestroy(e);
+    delete c;
+}
+
+const char *qed2d_last_error(const qed2d_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int qed2d_ctx_set_beta(qed2d_ctx *c, double beta)
+{
+    if (!c) return QED2D_EINVAL;
+    c->beta = beta;
+    return QED2D_OK;
+}
+
+int qed2d_ctx_set_sync(qed2d_ctx *c, int sync)
+{
+    if (!c) return QED2D_EINVAL;
+    c->sync = sync ? 1 : 0;
+    return QED2D_OK;
+}
+
+int qed2d_ctx_set_tile(qed2d_ctx *c, int rows)
+{
+    if (!c) return QED2D_EINVAL;
+    if (rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16)
+        return fail(c, QED2D_EINVAL, "tile rows must be 1, 2, 4, 8 or 16 (got %d)", rows);
+    c->rows = rows;
+    return QED2D_OK;
+}
+
+int qed2d_synchronize(qed2d_ctx *c)
+{
+    if (!c) return QED2D_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return QED2D_OK;
+}
+
+#define CHECK_CTX(c) \
+    if (!(c)) return QED2D_EINVAL
+#define NEED(c, p)                                                         \
+    if (!(p)) return fail((c), QED2D_EINVAL, "%s: null pointer %s", __func__, #p)
+#define NOALIAS(c, a, b)                                                            \
+    if (overlaps((a), (b)))                                                         \
+        return fail((c), QED2D_EINVAL, "%s: %s aliases %s", __func__, #a, #b)
+
+int qed2d_gamm5Dw(qed2d_ctx *c, double *so, const double *U, const double *si, double am0)
+{
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, U); NEED(c, si);
+    NOALIAS(c, so, si); NOALIAS(c, so, U);
+    q64(c, U, si, so, am0);
+    return finish(c);
+}
+
+int qed2d_gamm5Dw_f32(qed2d_ctx *c, float *so, const float *U, const float *si, float am0)
+{
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, U); NEED(c, si);
+    NOALIAS(c, so, si);
+    launch_q_apply_f32(c->L, (const float2 *)U, (const float2 *)si, (float2 *)so, am0, c->rows,
+                       c->stream);
+    return finish(c);
+}
+
+int qed2d_gamm5Dw_mrhs(qed2d_ctx *c, double *so, const double *U, const double *si, int nrhs,
+                       double am0)
+{
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, U); NEED(c, si);
+    NOALIAS(c, so, si);
+    if (nrhs < 1) return fail(c, QED2D_EINVAL, "nrhs must be >= 1");
+    launch_q_apply_mrhs_f64(c->L, (const double2 *)U, (const double2 *)si, (double2 *)so, nrhs,
+                            am0, c->stream);
+    return finish(c);
+}
+
+int qed2d_gamm5Dw_mrhs_f32(qed2d_ctx *c, float *so, const float *U, const float *si, int nrhs,
+                           float am0)
+{
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, U); NEED(c, si);
+    NOALIAS(c, so, si);
+    if (nrhs < 1) return fail(c, QED2D_EINVAL, "nrhs must be >= 1");
+    launch_q_apply_mrhs_f32(c->L, (const float2 *)U, (const float2 *)si, (float2 *)so, nrhs, am0,
+                            c->stream);
+    return finish(c);
+}
+
+int qed2d_gamm5(qed2d_ctx *c, double *so, const double *si)
+{
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, si);
+    launch_gamm5(c->L, (const double2 *)si, (double2 *)so, c->stream);
+    return finish(c);
+}
+
+int qed2d_Dw(qed2d_ctx *c, double *so, const double *U, const double *si, double am0)
+{
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, U); NEED(c, si);
+    NOALIAS(c, so, si);
+    double2 *tmp;
+    TRY(get_spin(c, S_TMP, &tmp));
+    q64(c, U, si, (double *)tmp, am0);
+    launch_gamm5(c->L, tmp, (double2 *)so, c->stream);
+    return finish(c);
+}
+
+int qed2d_gamm5Dw_sqr(qed2d_ctx *c, double *so, const double *U, const double *si, double am0)
+{
+    return qed2d_gamm5Dw_sqr_musq(c, so, nullptr, U, si, am0, 0.0);
+}
+
+int qed2d_gamm5Dw_sqr_msq(qed2d_ctx *c, double *so, double *tmp, const double *U,
+                          const double *si, double am0)
+{
+    return qed2d_gamm5Dw_sqr_musq(c, so, tmp, U, si, am0, 0.0);
+}
+
+int qed2d_gamm5Dw_sqr_musq(qed2d_ctx *c, double *so, double *tmp, const double *U,
+                           const double *si, double am0, double mu)
+{
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, U); NEED(c, si);
+    NOALIAS(c, so, si); NOALIAS(c, tmp, si); NOALIAS(c, tmp, so);
+    if (!tmp) {
+        double2 *t;
+        TRY(get_spin(c, S_TMP, &t));
+        tmp = (double *)t;
+    }
+    qsq64(c, U, si, so, tmp, am0, mu * mu);
+    return finish(c);
+}
+
+int qed2d_gamm5Dw_sqr_sqr(qed2d_ctx *c, double *so, const double *U, const double *si,
+                          double am0)
+{
+    return qed2d_gamm5Dw_sqr_sqr_musq(c, so, nullptr, U, si, am0, 0.0);
+}
+
+int qed2d_gamm5Dw_sqr_sqr_musq(qed2d_ctx *c, double *so, double *tmp, const double *U,
+                               const double *si, double am0, double mu)
+{
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, U); NEED(c, si);
+    NOALIAS(c, so, si); NOALIAS(c, tmp, si); NOALIAS(c, tmp, so);
+    double2 *t1, *t2;
+    TRY(get_spin(c, S_TMP, &t1));
+    TRY(get_spin(c, S_TMP2, &t2));
+    if (tmp) t1 = (double2 *)tmp;
+    q64(c, U, si, (double *)t1, am0);
+    q64(c, U, (const double *)t1, (double *)t2, am0);
+    q64(c, U, (const double *)t2, (double *)t1, am0);
+    launch_q_shift_f64(c->L, (const double2 *)U, t1, (const double2 *)si, (double2 *)so, am0,
+                       mu * mu, c->rows, c->stream);
+    return finish(c);
+}
+
+int qed2d_tr_dQwdU(qed2d_ctx *c, double *frc, const double *U, const double *X,
+                   const double *g5DwX)
+{
+    CHECK_CTX(c);
+    NEED(c, frc); NEED(c, U); NEED(c, X); NEED(c, g5DwX);
+    launch_tr_dQwdU(c->L, (const double2 *)U, (const double2 *)X, (const double2 *)g5DwX, frc,
+                    c->stream);
+    return finish(c);
+}
+
+int qed2d_krnl_force(qed2d_ctx *c, double *frc1, double *frc2, const double *U)
+{
+    CHECK_CTX(c);
+    NEED(c, frc1); NEED(c, frc2); NEED(c, U);
+    NOALIAS(c, frc1, frc2);
+    launch_krnl_force(c->L, (const double2 *)U, c->beta, frc1, frc2, c->stream);
+    return finish(c);
+}
+
+int qed2d_updt_fld(qed2d_ctx *c, double *U, const double *mom, double eps)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, mom);
+    updt(c, U, mom, eps);
+    return finish(c);
+}
+
+int qed2d_force_mom_update(qed2d_ctx *c, double *mom, const double *U, const double *X,
+                           const double *g5DwX, double eps)
+{
+    CHECK_CTX(c);
+    NEED(c, mom); NEED(c, U);
+    if (X && !g5DwX) return fail(c, QED2D_EINVAL, "g5DwX required with X");
+    launch_force_mom(c->L, (const double2 *)U, (const double2 *)X, (const double2 *)g5DwX,
+                     nullptr, mom, c->beta, eps, 1.0, c->stream);
+    return finish(c);
+}
+
+int qed2d_plaquette_field(qed2d_ctx *c, double *plx, const double *U)
+{
+    CHECK_CTX(c);
+    NEED(c, plx); NEED(c, U);
+    launch_observables(c->L, (const double2 *)U, nullptr, plx, nullptr, c->red, c->part,
+                       c->stream);
+    return finish(c);
+}
+
+int qed2d_qtop_field(qed2d_ctx *c, double *plx, const double *U)
+{
+    CHECK_CTX(c);
+    NEED(c, plx); NEED(c, U);
+    launch_observables(c->L, (const double2 *)U, nullptr, nullptr, plx, c->red, c->part,
+                       c->stream);
+    return finish(c);
+}
+
+int qed2d_action(qed2d_ctx *c, const double *U, double *S)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, S);
+    hdd sp;
+    TRY(observables(c, U, nullptr, &sp, nullptr, nullptr));
+    hdd g;
+    g.add((double)c->L.V);
+    g.add(-sp.hi, -sp.lo);
+    *S = c->beta * g.value();
+    return QED2D_OK;
+}
+
+int qed2d_plaquette(qed2d_ctx *c, const double *U, double *P)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, P);
+    hdd sp;
+    TRY(observables(c, U, nullptr, &sp, nullptr, nullptr));
+    *P = sp.value() / (double)c->L.V;
+    return QED2D_OK;
+}
+
+int qed2d_qtop(qed2d_ctx *c, const double *U, double *Q)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, Q);
+    hdd sq;
+    TRY(observables(c, U, nullptr, nullptr, &sq, nullptr));
+    *Q = sq.value() / (2.0 * M_PI);
+    return QED2D_OK;
+}
+
+int qed2d_hamiltonian(qed2d_ctx *c, const double *mom, const double *U, double *H)
+{
+    CHECK_CTX(c);
+    NEED(c, mom); NEED(c, U); NEED(c, H);
+    hdd h;
+    TRY(hamiltonian_dd(c, mom, U, &h));
+    *H = h.value();
+    return QED2D_OK;
+}
+
+int qed2d_dot(qed2d_ctx *c, const double *a, const double *b, double *re, double *im)
+{
+    CHECK_CTX(c);
+    NEED(c, a); NEED(c, b); NEED(c, re);
+    return dot_host(c, a, b, re, im);
+}
+
+int qed2d_norm2(qed2d_ctx *c, const double *a, double *n2)
+{
+    CHECK_CTX(c);
+    NEED(c, a); NEED(c, n2);
+    return norm2_host(c, a, n2);
+}
+
+int qed2d_cg(qed2d_ctx *c, double *so, const double *U, const double *si, double am0, double mu,
+             int op, int maxiter, double eps, int so_as_guess, int precision, int *iters)
+{
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, U); NEED(c, si);
+    NOALIAS(c, so, si);
+    if (op != QED2D_OP_SQR_MSQ && op != QED2D_OP_SQR_MUSQ)
+        return fail(c, QED2D_EINVAL, "unknown operator %d", op);
+    if (precision != QED2D_PREC_F64 && precision != QED2D_PREC_MIXED)
+        return fail(c, QED2D_EINVAL, "unknown precision %d", precision);
+    if (maxiter < 1) return fail(c, QED2D_EINVAL, "maxiter must be >= 1");
+    return cg_any(c, so, U, si, am0, mu, op, maxiter, eps, so_as_guess, precision, iters);
+}
+
+int qed2d_multicg(qed2d_ctx *c, double *so, const double *U, const double *si, double am0,
+                  int n, const double *mu, const double *rho, int maxiter, double eps)
+{
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, U); NEED(c, si);
+    NOALIAS(c, so, si);
+    if (n < 0 || (n > 0 && (!mu || !rho))) return fail(c, QED2D_EINVAL, "bad pole arrays");
+    TRY(multicg_impl(c, so, U, si, am0, n, mu, rho, maxiter, eps));
+    return finish(c);
+}
+
+int qed2d_R(qed2d_ctx *c, double *so, const double *U, const double *si, double am0, int n,
+            const double *mu, const double *rho, double A, double r_b, int maxiter, double eps)
+{
+    TRY(qed2d_multicg(c, so, U, si, am0, n, mu, rho, maxiter, eps));
+    // so .= r_b^(-1) * A * so  (src/QED2dRHMC.jl:268)
+    const double f = std::pow(r_b, -1.0) * A;
+    launch_scale_add((double2 *)so, f, (const double2 *)so, 0.0, spinor_elems(c), c->stream);
+    return finish(c);
+}
+
+int qed2d_generate_pseudofermion(qed2d_ctx *c, double *F, const double *U, const double *X,
+                                 double am0, int n, const double *mu, const double *nu,
+                                 int maxiter, double tol)
+{
+    CHECK_CTX(c);
+    NEED(c, F); NEED(c, U); NEED(c, X);
+    NOALIAS(c, F, X);
+    if (n < 0 || (n > 0 && (!mu || !nu))) return fail(c, QED2D_EINVAL, "bad pole arrays");
+    TRY(gen_pf_impl(c, F, U, X, am0, n, mu, nu, maxiter, tol));
+    return finish(c);
+}
+
+int qed2d_power_method(qed2d_ctx *c, const double *U, const double *b0, double am0, int iter,
+                       double *lambda_min, double *lambda_max)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, b0); NEED(c, lambda_min); NEED(c, lambda_max);
+    double2 *b, *ba, *bl, *tmp;
+    TRY(get_spin(c, S_PB, &b));
+    TRY(get_spin(c, S_PBA, &ba));
+    TRY(get_spin(c, S_PBL, &bl));
+    TRY(get_spin(c, S_TMP, &tmp));
+    const long n2 = spinor_elems(c);
+    hipStream_t st = c->stream;
+    const double shift = 2.0;
+    launch_copy(b, (const double2 *)b0, n2, st);
+    double re, im;
+    // src/QED2dRHMC.jl:144-149
+    for (int i = 0; i < iter; ++i) {
+        qsq64(c, U, (const double *)b, (double *)ba, (double *)tmp, am0, 0.0);
+        launch_scale_add(ba, 1.0, b, shift, n2, st);
+        TRY(dot_host(c, (const double *)ba, (const double *)ba, &re, &im));
+        launch_scale_add(b, 0.0, ba, 1.0 / re, n2, st);
+    }
+    qsq64(c, U, (const double *)b, (double *)ba, (double *)tmp, am0, 0.0);
+    launch_scale_add(ba, 1.0, b, shift, n2, st);
+    double num, den;
+    TRY(dot_host(c, (const double *)b, (const double *)ba, &num, &im));
+    TRY(dot_host(c, (const double *)b, (const double *)b, &den, &im));
+    const double lmax = num / den - shift;
+    // src/QED2dRHMC.jl:157-166 (Q^2 applied to a copy; the reference aliases)
+    for (int i = 0; i < iter; ++i) {
+        launch_copy(bl, b, n2, st);
+        qsq64(c, U, (const double *)bl, (double *)b, (double *)tmp, am0, 0.0);
+        launch_scale_add(b, 1.0, bl, -lmax, n2, st);
+        TRY(dot_host(c, (const double *)b, (const double *)b, &re, &im));
+        launch_scale_add(b, 1.0 / re, b, 0.0, n2, st);
+    }
+    qsq64(c, U, (const double *)b, (double *)ba, (double *)tmp, am0, 0.0);
+    launch_scale_add(ba, 1.0, b, -lmax, n2, st);
+    TRY(dot_host(c, (const double *)b, (const double *)ba, &num, &im));
+    TRY(dot_host(c, (const double *)b, (const double *)b, &den, &im));
+    *lambda_min = num / den + lmax;
+    *lambda_max = lmax;
+    return QED2D_OK;
+}
+
+int qed2d_fill_momenta(qed2d_ctx *c, double *mom, uint64_t seed, uint32_t chain, uint32_t traj)
+{
+    CHECK_CTX(c);
+    NEED(c, mom);
+    uint32_t k0, k1;
+    key_of(seed, chain, &k0, &k1);
+    launch_fill_momenta(c->L, mom, k0, k1, traj, c->stream);
+    return finish(c);
+}
+
+int qed2d_fill_pseudofermion(qed2d_ctx *c, double *X, uint64_t seed, uint32_t chain,
+                             uint32_t traj, uint32_t flavour)
+{
+    CHECK_CTX(c);
+    NEED(c, X);
+    if (flavour >= 16) return fail(c, QED2D_EINVAL, "flavour index must be < 16");
+    uint32_t k0, k1;
+    key_of(seed, chain, &k0, &k1);
+    launch_fill_pseudofermion(c->L, (double2 *)X, k0, k1, traj, flavour, c->stream);
+    return finish(c);
+}
+
+int qed2d_hot_start(qed2d_ctx *c, double *U, uint64_t seed, uint32_t chain)
+{
+    CHECK_CTX(c);
+    NEED(c, U);
+    uint32_t k0, k1;
+    key_of(seed, chain, &k0, &k1);
+    launch_hot_start(c->L, (double2 *)U, k0, k1, c->stream);
+    return finish(c);
+}
+
+double qed2d_metropolis_uniform(uint64_t seed, uint32_t chain, uint32_t traj)
+{
+    uint32_t k0, k1;
+    key_of(seed, chain, &k0, &k1);
+    uint32_t ctr[4] = {0u, 0u, kPurposeMetro, traj};
+    philox4x32_10(ctr, k0, k1);
+    return u53(ctr[0], ctr[1]);
+}
+
+int qed2d_omf4_pure(qed2d_ctx *c, double *mom, double *U, double eps, int ns)
+{
+    CHECK_CTX(c);
+    NEED(c, mom); NEED(c, U);
+    TRY(omf4_pure(c, mom, U, eps, ns));
+    return finish(c);
+}
+
+int qed2d_md_nf2(qed2d_ctx *c, int integrator, double *mom, double *U, double *X, const double *F,
+                 double *g5DX, double am0, double eps, int ns, int maxiter, double tol,
+                 int precision, int *cg_iters)
+{
+    CHECK_CTX(c);
+    NEED(c, mom); NEED(c, U); NEED(c, X); NEED(c, F); NEED(c, g5DX);
+    NOALIAS(c, X, F);
+    int it = 0;
+    TRY(md_nf2(c, integrator, mom, U, X, F, g5DX, am0, eps, ns, maxiter, tol, precision, &it));
+    if (cg_iters) *cg_iters = it;
+    return finish(c);
+}
+
+int qed2d_hmc_pure(qed2d_ctx *c, double *U, double eps, int ns, int integrator, int qzero,
+                   uint64_t seed, uint32_t chain, uint32_t traj, double *hini, double *hfin,
+                   int *accepted)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, hini); NEED(c, hfin); NEED(c, accepted);
+    double2 *Ucp;
+    double *mom;
+    TRY(get_spin(c, S_UCP, &Ucp));
+    TRY(get_real(c, R_MOM, &mom));
+    const long n2 = spinor_elems(c);
+    launch_copy(Ucp, (const double2 *)U, n2, c->stream);
+    uint32_t k0, k1;
+    key_of(seed, chain, &k0, &k1);
+    launch_fill_momenta(c->L, mom, k0, k1, traj, c->stream);
+    hdd h0, h1;
+    TRY(hamiltonian_dd(c, mom, U, &h0));
+    if (integrator == QED2D_OMF4) TRY(omf4_pure(c, mom, U, eps, ns));
+    else TRY(leapfrog_pure(c, mom, U, eps, ns));
+    TRY(hamiltonian_dd(c, mom, U, &h1));
+    *hini = h0.value();
+    *hfin = h1.value();
+    TRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
+    if (!*accepted) launch_copy((double2 *)U, Ucp, n2, c->stream);
+    return finish(c);
+}
+
+int qed2d_hmc_nf2(qed2d_ctx *c, double *U, double am0, double eps, int ns, int integrator,
+                  int maxiter, double tol, int qzero, int precision, double acc_tol,
+                  uint64_t seed, uint32_t chain, uint32_t traj, double *hini, double *hfin,
+                  int *accepted, int *cg_iters)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, hini); NEED(c, hfin); NEED(c, accepted);
+    double2 *Ucp, *X, *F, *g5DX;
+    double *mom;
+    TRY(get_spin(c, S_UCP, &Ucp));
+    TRY(get_spin(c, S_X, &X));
+    TRY(get_spin(c, S_F, &F));
+    TRY(get_spin(c, S_G5DX, &g5DX));
+    TRY(get_real(c, R_MOM, &mom));
+    const long n2 = spinor_elems(c);
+    hipStream_t st = c->stream;
+    launch_copy(Ucp, (const double2 *)U, n2, st);
+    uint32_t k0, k1;
+    key_of(seed, chain, &k0, &k1);
+    launch_fill_momenta(c->L, mom, k0, k1, traj, st);
+    launch_fill_pseudofermion(c->L, X, k0, k1, traj, 0, st);
+    // hini = H + X^dag X  (src/QED2dHMC.jl:74)
+    hdd h0, h1;
+    TRY(hamiltonian_dd(c, mom, U, &h0));
+    double xx;
+    TRY(norm2_host(c, (const double *)X, &xx));
+    h0.add(xx);
+    q64(c, U, (const double *)X, (double *)F, am0);  // F = Q X  (:76-78)
+    int it = 0;
+    int rc = md_nf2(c, integrator, mom, U, (double *)X, (const double *)F, (double *)g5DX, am0,
+                    eps, ns, maxiter, tol, precision, &it);
+    if (cg_iters) *cg_iters = it;
+    if (rc != QED2D_OK) {
+        std::string msg = c->err;
+        launch_copy((double2 *)U, Ucp, n2, st);
+        (void)hipStreamSynchronize(st);
+        c->err = msg;
+        *accepted = 0;
+        return rc;
+    }
+    TRY(hamiltonian_dd(c, mom, U, &h1));
+    if (acc_tol <= 0.0) {
+        // hfin = H + Re X^dag F with X the last MD solve  (:82)
+        double re, im;
+        TRY(dot_host(c, (const double *)X, (const double *)F, &re, &im));
+        h1.add(re);
+    } else {
+        int ait = 0;
+        TRY(cg_any(c, (double *)X, U, (const double *)F, am0, 0.0, QED2D_OP_SQR_MSQ, maxiter,
+                   acc_tol, 1, QED2D_PREC_F64, &ait));
+        q64(c, U, (const double *)X, (double *)g5DX, am0);
+        double re, im, yy;
+        TRY(dot_host(c, (const double *)X, (const double *)F, &re, &im));
+        TRY(norm2_host(c, (const double *)g5DX, &yy));
+        h1.add(2.0 * re);
+        h1.add(-yy);
+    }
+    *hini = h0.value();
+    *hfin = h1.value();
+    TRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
+    if (!*accepted) launch_copy((double2 *)U, Ucp, n2, st);
+    return finish(c);
+}
+
+int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int *npole,
+                   const double *const *mu, const double *const *nu, const double *const *rho,
+                   double eps, int ns, int maxiter, double tol, int qzero,
+                   int reversibility_test, uint64_t seed, uint32_t chain, uint32_t traj,
+                   double *hini, double *hfin, int *accepted)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, hini); NEED(c, hfin); NEED(c, accepted);
+    if (nf < 0 || nf > 16 || (nf > 0 && (!am0 || !npole || !mu || !nu || !rho)))
+        return fail(c, QED2D_EINVAL, "bad flavour arrays");
+    double2 *Ucp, *X, *g5DX;
+    double *mom;
+    TRY(get_spin(c, S_UCP, &Ucp));
+    TRY(get_spin(c, S_X, &X));
+    TRY(get_spin(c, S_G5DX, &g5DX));
+    TRY(get_real(c, R_MOM, &mom));
+    const long n2 = spinor_elems(c);
+    while ((int)c->flav.size() < nf) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(double2) * n2) != hipSuccess)
+            return fail(c, QED2D_ENOMEM, "hipMalloc of a pseudofermion failed");
+        c->flav.push_back((double2 *)p);
+    }
+    hipStream_t st = c->stream;
+    launch_copy(Ucp, (const double2 *)U, n2, st);
+    uint32_t k0, k1;
+    key_of(seed, chain, &k0, &k1);
+    launch_fill_momenta(c->L, mom, k0, k1, traj, st);
+    hdd h0, h1;
+    TRY(hamiltonian_dd(c, mom, U, &h0));
+    std::vector<double2 *> F(c->flav.begin(), c->flav.begin() + nf);
+    for (int j = 0; j < nf; ++j) {
+        // src/QED2dHMC.jl:136-144
+        launch_fill_pseudofermion(c->L, X, k0, k1, traj, (uint32_t)j, st);
+        double xx;
+        TRY(norm2_host(c, (const double *)X, &xx));
+        h0.add(xx);
+        TRY(gen_pf_impl(c, (double *)F[j], U, (const double *)X, am0[j], npole[j], mu[j], nu[j],
+                        maxiter, tol));
+    }
+    int rc = leapfrog_rhmc(c, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu, rho,
+                           eps, ns, maxiter, tol);
+    if (rc != QED2D_OK) {
+        std::string msg = c->err;
+        launch_copy((double2 *)U, Ucp, n2, st);
+        (void)hipStreamSynchronize(st);
+        c->err = msg;
+        *accepted = 0;
+        return rc;
+    }
+    TRY(hamiltonian_dd(c, mom, U, &h1));
+    for (int j = 0; j < nf; ++j) {
+        // src/QED2dHMC.jl:150-156
+        TRY(multicg_impl(c, (double *)X, U, (const double *)F[j], am0[j], npole[j], mu[j], rho[j],
+                         maxiter, tol));
+        double re, im;
+        TRY(dot_host(c, (const double *)X, (const double *)F[j], &re, &im));
+        h1.add(re);
+    }
+    *hini = h0.value();
+    *hfin = h1.value();
+    if (reversibility_test) {
+        // src/QED2dHMC.jl:159-164
+        launch_negate(mom, 2 * c->L.V, st);
+        TRY(leapfrog_rhmc(c, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu, rho, eps,
+                          ns, maxiter, tol));
+        *accepted = -1;
+        return finish(c);
+    }
+    TRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
+    if (!*accepted) launch_copy((double2 *)U, Ucp, n2, st);
+    return finish(c);
+}
+
+int qed2d_profile_enable(qed2d_ctx *c, int on)
+{
+    CHECK_CTX(c);
+    c->prof = on != 0;
+    return QED2D_OK;
+}
+
+int qed2d_profile_reset(qed2d_ctx *c)
+{
+    CHECK_CTX(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_harvest(c);
+    for (int k = 0; k < QED2D_PROF_NCLASS; ++k) {
+        c->prof_ms[k] = 0.0;
+        c->prof_n[k] = 0;
+    }
+    return QED2D_OK;
+}
+
+int qed2d_profile_read(qed2d_ctx *c, int cls, long *count, double *total_ms)
+{
+    CHECK_CTX(c);
+    if (cls < 0 || cls >= QED2D_PROF_NCLASS || !count || !total_ms) return QED2D_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_harvest(c);
+    *count = c->prof_n[cls];
+    *total_ms = c->prof_ms[cls];
+    return QED2D_OK;
+}
+
+int qed2d_unfold(qed2d_ctx *c, double *Uout, const double *Uin, int reflect)
+{
+    CHECK_CTX(c);
+    NEED(c, Uout); NEED(c, Uin);
+    NOALIAS(c, Uout, Uin);
+    launch_unfold(c->L, (const double2 *)Uin, (double2 *)Uout, reflect ? 1 : 0, c->stream);
+    return finish(c);
+}
+
+}  // extern "C"

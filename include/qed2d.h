@@ -1,0 +1,245 @@
+/*
+ * qed2d.h -- C ABI of libqed2d_hip, the MI355X (gfx950) implementation of the
+ * QED2d molecular-dynamics hot path of dalbandea/Dav-QED2d (Julia module
+ * QED2d).  Each entry point names the reference function it replaces.
+ *
+ * Fields
+ *   All field pointers are DEVICE pointers in the reference's own layout,
+ *   the Julia column-major arrays:
+ *     spinor, link   ComplexF64[L1,L2,2]  -> double[2*2*V], index
+ *                    2*((c*L2 + i2)*L1 + i1) + {0 re, 1 im}
+ *     momentum/force Float64[L1,L2,2]     -> double[2*V]
+ *     site density   Float64[L1,L2]       -> double[V]
+ *   A CuArray/HIP buffer of the reference can therefore be passed as is.
+ *
+ * Contexts
+ *   A context binds one lattice (L1, L2, beta, fermion boundary flags) to one
+ *   HIP stream on the current device and owns every scratch buffer; it is the
+ *   counterpart of the reference's LattParm (src/QED2d.jl:5-8).  The
+ *   reference's KernelParm (src/QED2d.jl:10-13) has no counterpart: launch
+ *   shapes are chosen by the library.  A context is not re-entrant; use one
+ *   context per host thread.
+ *
+ * Calls are synchronous by default (results visible when the call returns),
+ * as the reference's CUDA.@sync launches.  qed2d_ctx_set_sync(ctx, 0) makes
+ * pure kernel launches asynchronous on the context's stream.
+ *
+ * Errors: every function returns QED2D_OK (0) or a negative status;
+ * qed2d_last_error(ctx) gives the message.  The reference throws
+ * (src/QED2dSolver.jl:40-43,99-102; src/QED2dRHMC.jl:12-14).
+ *
+ * Aliasing: an output may not alias an input of the same call
+ * (QED2D_EINVAL), except where stated.  The reference's only aliased call
+ * (src/QED2dRHMC.jl:159) is a data race there.
+ */
+#ifndef QED2D_H
+#define QED2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QED2D_OK 0
+#define QED2D_EINVAL (-1)   /* bad argument (sizes, aliasing, null pointers) */
+#define QED2D_EHIP (-2)     /* HIP runtime error */
+#define QED2D_ENOCONV (-3)  /* solver hit maxiter without converging */
+#define QED2D_ENOMEM (-4)   /* device allocation failed */
+
+/* Fermion boundary flag: antiperiodic across i2 = L2 -> 1 (direction 2,
+ * "time").  The reference is periodic in both directions
+ * (src/QED2dDirac.jl:7-11); 0 reproduces it. */
+#define QED2D_BC_APERIODIC_T 1
+
+/* Solver operator selectors: the function the reference passes as `A`. */
+#define QED2D_OP_SQR_MSQ 0   /* gamm5Dw_sqr_msq : Q^2          (src/QED2dDirac.jl:69-81) */
+#define QED2D_OP_SQR_MUSQ 1  /* gamm5Dw_sqr_musq: Q^2 + mu^2   (src/QED2dDirac.jl:83-95) */
+
+/* Solver precisions. */
+#define QED2D_PREC_F64 0     /* FP64 throughout (the reference) */
+#define QED2D_PREC_MIXED 1   /* FP32 iterations + FP64 reliable updates, same FP64 stopping rule */
+
+/* Integrators (src/integrators/integrators.jl:1-3). */
+#define QED2D_LEAPFROG 0
+#define QED2D_OMF4 1
+
+typedef struct qed2d_ctx qed2d_ctx;
+
+/* ---- context ------------------------------------------------------------
+ * LattParm(iL, beta) (src/QED2d.jl:5-8) + the fermion BC flag. `stream` is a
+ * hipStream_t (NULL = default stream). */
+int qed2d_ctx_create(qed2d_ctx **ctx, int L1, int L2, double beta, int bc_flags, void *stream);
+void qed2d_ctx_destroy(qed2d_ctx *ctx);
+const char *qed2d_last_error(const qed2d_ctx *ctx);
+int qed2d_ctx_set_beta(qed2d_ctx *ctx, double beta);
+int qed2d_ctx_set_sync(qed2d_ctx *ctx, int sync);
+/* Rows marched per thread by the stencil kernels (1, 2, 4, 8 or 16). */
+int qed2d_ctx_set_tile(qed2d_ctx *ctx, int rows);
+int qed2d_synchronize(qed2d_ctx *ctx);
+
+/* ---- Dirac operator -------------------------------------------------------- */
+/* gamm5Dw(so, U, si, am0, prm) kernel: so = gamma5 Dw si (src/QED2dDirac.jl:2-33) */
+int qed2d_gamm5Dw(qed2d_ctx *ctx, double *so, const double *U, const double *si, double am0);
+/* FP32 form of the same operator (float2 fields, same layout). */
+int qed2d_gamm5Dw_f32(qed2d_ctx *ctx, float *so, const float *U, const float *si, float am0);
+/* nrhs right-hand sides sharing U; RHS j at si + j*4V doubles. */
+int qed2d_gamm5Dw_mrhs(qed2d_ctx *ctx, double *so, const double *U, const double *si, int nrhs,
+                       double am0);
+int qed2d_gamm5Dw_mrhs_f32(qed2d_ctx *ctx, float *so, const float *U, const float *si, int nrhs,
+                           float am0);
+/* gamm5(so, si, am0, prm) kernel (src/QED2dDirac.jl:130-146) */
+int qed2d_gamm5(qed2d_ctx *ctx, double *so, const double *si);
+/* Dw(so, U, si, am0, prm, kprm) (src/QED2dDirac.jl:157-169) */
+int qed2d_Dw(qed2d_ctx *ctx, double *so, const double *U, const double *si, double am0);
+/* gamm5Dw_sqr(so, U, si, am0, prm, kprm) (src/QED2dDirac.jl:35-46) */
+int qed2d_gamm5Dw_sqr(qed2d_ctx *ctx, double *so, const double *U, const double *si, double am0);
+/* gamm5Dw_sqr_msq(so, tmp, U, si, am0, prm, kprm) (src/QED2dDirac.jl:69-81); tmp may be NULL */
+int qed2d_gamm5Dw_sqr_msq(qed2d_ctx *ctx, double *so, double *tmp, const double *U,
+                          const double *si, double am0);
+/* gamm5Dw_sqr_musq(so, tmp, U, si, am0, mu, prm, kprm) (src/QED2dDirac.jl:83-95) */
+int qed2d_gamm5Dw_sqr_musq(qed2d_ctx *ctx, double *so, double *tmp, const double *U,
+                           const double *si, double am0, double mu);
+/* gamm5Dw_sqr_sqr(so, U, si, am0, prm, kprm) (src/QED2dDirac.jl:48-67) */
+int qed2d_gamm5Dw_sqr_sqr(qed2d_ctx *ctx, double *so, const double *U, const double *si,
+                          double am0);
+/* gamm5Dw_sqr_sqr_musq(so, tmp, U, si, am0, mu, prm, kprm) (src/QED2dDirac.jl:97-117) */
+int qed2d_gamm5Dw_sqr_sqr_musq(qed2d_ctx *ctx, double *so, double *tmp, const double *U,
+                               const double *si, double am0, double mu);
+
+/* ---- forces, link update, observables -------------------------------------- */
+/* tr_dQwdU(frc, U, X, g5DwX, prm) kernel (src/QED2dDirac.jl:172-199) */
+int qed2d_tr_dQwdU(qed2d_ctx *ctx, double *frc, const double *U, const double *X,
+                   const double *g5DwX);
+/* krnl_force!(frc1, frc2, U, prm) kernel (src/QED2dAction.jl:34-53) */
+int qed2d_krnl_force(qed2d_ctx *ctx, double *frc1, double *frc2, const double *U);
+/* updt_fld!(U, mom, eps) kernel (src/QED2dAction.jl:55-64) */
+int qed2d_updt_fld(qed2d_ctx *ctx, double *U, const double *mom, double eps);
+/* mom .+= eps (frc1 + frc2 + frc) fused with the force kernels
+ * (src/QED2dHMC.jl:463-478): gauge force, plus the Nf=2 fermion force of
+ * (X, g5DwX) when X is not NULL. */
+int qed2d_force_mom_update(qed2d_ctx *ctx, double *mom, const double *U, const double *X,
+                           const double *g5DwX, double eps);
+/* plaquette!(plx, U, prm) kernel (src/QED2dAction.jl:2-16) */
+int qed2d_plaquette_field(qed2d_ctx *ctx, double *plx, const double *U);
+/* qtop!(plx, U, prm) kernel (src/QED2dAction.jl:18-32) */
+int qed2d_qtop_field(qed2d_ctx *ctx, double *plx, const double *U);
+/* Action(U, prm, kprm) (src/QED2dHMC.jl:206-215) */
+int qed2d_action(qed2d_ctx *ctx, const double *U, double *S);
+/* Plaquette(U, prm, kprm) (src/QED2dHMC.jl:217-226) */
+int qed2d_plaquette(qed2d_ctx *ctx, const double *U, double *P);
+/* Qtop(U, prm, kprm) (src/QED2dHMC.jl:228-237) */
+int qed2d_qtop(qed2d_ctx *ctx, const double *U, double *Q);
+/* Hamiltonian(mom, U, prm, kprm) (src/QED2dHMC.jl:198-203) */
+int qed2d_hamiltonian(qed2d_ctx *ctx, const double *mom, const double *U, double *H);
+/* CUDA.dot(a, b) and CUDA.mapreduce(abs2, +, a) on spinors (src/QED2dSolver.jl:13,21) */
+int qed2d_dot(qed2d_ctx *ctx, const double *a, const double *b, double *re, double *im);
+int qed2d_norm2(qed2d_ctx *ctx, const double *a, double *n2);
+
+/* ---- solvers ------------------------------------------------------------------
+ * CG(so, U, si, am0, maxiter, eps, A, prm, kprm)            (src/QED2dSolver.jl:3-45)
+ * CG(so, U, si, am0, mu, maxiter, eps, A, prm, kprm;
+ *    so_as_guess)                                          (src/QED2dSolver.jl:49-105)
+ * op selects A; stop when |r|^2 < eps |b|^2 (eps is a SQUARED relative
+ * residual, as in the reference).  precision QED2D_PREC_MIXED runs FP32
+ * iterations with FP64 reliable updates and stops on the same FP64 true
+ * residual.  *iters receives the iteration count (0 when the guess already
+ * satisfies the tolerance, as the reference).  Returns QED2D_ENOCONV after
+ * maxiter iterations without convergence. */
+int qed2d_cg(qed2d_ctx *ctx, double *so, const double *U, const double *si, double am0,
+             double mu, int op, int maxiter, double eps, int so_as_guess, int precision,
+             int *iters);
+/* MultiCG(so, U, si, am0, maxiter, eps, A, rprm, prm, kprm) (src/QED2dRHMC.jl:281-290):
+ * so = (1 + sum_j rho_j (Q^2 + mu_j^2)^-1) si. */
+int qed2d_multicg(qed2d_ctx *ctx, double *so, const double *U, const double *si, double am0,
+                  int n, const double *mu, const double *rho, int maxiter, double eps);
+/* R(so, U, si, am0, maxiter, eps, A, rprm, prm, kprm) (src/QED2dRHMC.jl:261-271) */
+int qed2d_R(qed2d_ctx *ctx, double *so, const double *U, const double *si, double am0, int n,
+            const double *mu, const double *rho, double A, double r_b, int maxiter, double eps);
+/* generate_pseudofermion!(F, U, X, am0, CGmaxiter, CGtol, prm, kprm, rprm)
+ * (src/QED2dRHMC.jl:298-319) */
+int qed2d_generate_pseudofermion(qed2d_ctx *ctx, double *F, const double *U, const double *X,
+                                 double am0, int n, const double *mu, const double *nu,
+                                 int maxiter, double tol);
+/* power_method(U, am0, prm, kprm; iter) (src/QED2dRHMC.jl:135-170); b0 is the
+ * starting vector (the reference draws it at :137). */
+int qed2d_power_method(qed2d_ctx *ctx, const double *U, const double *b0, double am0, int iter,
+                       double *lambda_min, double *lambda_max);
+
+/* ---- Zolotarev coefficients (host only) ---------------------------------------
+ * get_rhmc_params(n, r_a, r_b) (src/QED2dRHMC.jl:93-110) -> RHMCParm fields
+ * (src/QED2d.jl:15-26).  rho, mu, nu hold n doubles each. */
+int qed2d_zolotarev(int n, double r_a, double r_b, double *eps, double *A, double *delta,
+                    double *rho, double *mu, double *nu);
+
+/* ---- random fields -------------------------------------------------------------
+ * Counter-based Philox4x32-10 streams keyed by (seed, chain) and indexed by
+ * trajectory; they replace CUDA.randn / rand() at src/QED2dHMC.jl:25,42,69-70,
+ * 95,124,138 (CURAND streams are not reproducible outside CUDA). */
+int qed2d_fill_momenta(qed2d_ctx *ctx, double *mom, uint64_t seed, uint32_t chain,
+                       uint32_t traj);
+int qed2d_fill_pseudofermion(qed2d_ctx *ctx, double *X, uint64_t seed, uint32_t chain,
+                             uint32_t traj, uint32_t flavour);
+int qed2d_hot_start(qed2d_ctx *ctx, double *U, uint64_t seed, uint32_t chain);
+double qed2d_metropolis_uniform(uint64_t seed, uint32_t chain, uint32_t traj);
+
+/* ---- molecular dynamics ---------------------------------------------------------
+ * OMF4!(mom, U, eps, ns, prm, kprm)                 (src/QED2dHMC.jl:239-305) */
+int qed2d_omf4_pure(qed2d_ctx *ctx, double *mom, double *U, double eps, int ns);
+/* leapfrog!(mom, U, X, F, g5DX, am0, eps, ns, ...)  (src/QED2dHMC.jl:426-449)
+ * OMF4!(mom, U, X, F, g5DX, am0, eps, ns, ...)      (src/QED2dHMC.jl:307-422)
+ * X receives the last solve; g5DX the last Q X.  *cg_iters: total. */
+int qed2d_md_nf2(qed2d_ctx *ctx, int integrator, double *mom, double *U, double *X,
+                 const double *F, double *g5DX, double am0, double eps, int ns, int maxiter,
+                 double tol, int precision, int *cg_iters);
+
+/* ---- HMC trajectories --------------------------------------------------------------
+ * HMC!(U, eps, ns, acc, prm, kprm; integrator, qzero)         (src/QED2dHMC.jl:19-60) */
+int qed2d_hmc_pure(qed2d_ctx *ctx, double *U, double eps, int ns, int integrator, int qzero,
+                   uint64_t seed, uint32_t chain, uint32_t traj, double *hini, double *hfin,
+                   int *accepted);
+/* HMC!(U, am0, eps, ns, acc, CGmaxiter, tol, prm, kprm; integrator, qzero)
+ * (src/QED2dHMC.jl:64-113).  precision selects the MD-force solver.
+ * acc_tol <= 0: the acceptance step reuses the last MD solve, as the
+ * reference (:82).  acc_tol > 0: the acceptance step re-solves in FP64 to
+ * that squared relative residual, starting from the last MD solve, and
+ * evaluates S_f = 2 Re(X^dag F) - |Q X|^2 (error second order in the
+ * residual). */
+int qed2d_hmc_nf2(qed2d_ctx *ctx, double *U, double am0, double eps, int ns, int integrator,
+                  int maxiter, double tol, int qzero, int precision, double acc_tol,
+                  uint64_t seed, uint32_t chain, uint32_t traj, double *hini, double *hfin,
+                  int *accepted, int *cg_iters);
+/* HMC!(U, am0::Array, eps, ns, acc, CGmaxiter, tol, prm, kprm, rprm::Array;
+ *      qzero, reversibility_test)                            (src/QED2dHMC.jl:116-195)
+ * nf flavours; flavour j has npole[j] poles mu[j][], nu[j][], rho[j][]. */
+int qed2d_hmc_rhmc(qed2d_ctx *ctx, double *U, const double *am0, int nf, const int *npole,
+                   const double *const *mu, const double *const *nu, const double *const *rho,
+                   double eps, int ns, int maxiter, double tol, int qzero,
+                   int reversibility_test, uint64_t seed, uint32_t chain, uint32_t traj,
+                   double *hini, double *hfin, int *accepted);
+
+/* ---- measurement -----------------------------------------------------------------------
+ * HIP-event timing of the library's own kernels, per kernel class, on the
+ * context's stream (no counterpart in the reference, which only has @time). */
+#define QED2D_PROF_CG_K1_F64 0    /* CG kernel 1, FP64: p = r + beta p, t = Q p, |t|^2 */
+#define QED2D_PROF_CG_K2_F64 1    /* CG kernel 2, FP64: Ap = Q t, x/r update, |r|^2 */
+#define QED2D_PROF_CG_K1_F32 2
+#define QED2D_PROF_CG_K2_F32 3
+#define QED2D_PROF_Q_F64 4        /* plain gamm5Dw apply */
+#define QED2D_PROF_FORCE_MOM 5    /* fused force + momentum update */
+#define QED2D_PROF_UPDT_FLD 6
+#define QED2D_PROF_RELIABLE 7     /* reliable-update kernels of the mixed CG */
+#define QED2D_PROF_OTHER 8
+#define QED2D_PROF_NCLASS 9
+int qed2d_profile_enable(qed2d_ctx *ctx, int on);
+int qed2d_profile_reset(qed2d_ctx *ctx);
+int qed2d_profile_read(qed2d_ctx *ctx, int cls, long *count, double *total_ms);
+
+/* ---- fields ----------------------------------------------------------------------------
+ * unfold_fld! / unfold_reflect! (src/QED2dFields.jl:1-50): Uout is (2L1)x(2L2). */
+int qed2d_unfold(qed2d_ctx *ctx, double *Uout, const double *Uin, int reflect);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,245 @@
+"""GPU parity of the solvers (CG v1/v2, mixed precision, MultiCG, R,
+pseudofermion generation, power method) and of the MD / HMC drivers against
+the FP64 CPU oracle.
+
+Tolerances: solutions are compared through the true FP64 residual and to the
+oracle's solution at the level the stopping rule allows; HMC energies agree to
+1e-9 absolute at 24^2 and the accept/reject sequence is identical."""
+import numpy as np
+import pytest
+
+from dense import random_links, random_spinor
+from gpu_util import dev, host, relerr, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _true_res(O, U, x, b, am0, mu=0.0, bc=0):
+    Ax = O.gamm5Dw_sqr_musq(U, x, am0, mu, bc)
+    return O.norm2(b - Ax) / O.norm2(b)
+
+
+@pytest.mark.parametrize("L1,L2", [(24, 24), (64, 40)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_cg_v1(Q, O, L1, L2, prec):
+    say("CG v1", L1, L2, "prec", prec)
+    rng = np.random.default_rng(L1)
+    U = random_links(L1, L2, rng, unitary=True)
+    b = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 4.0)
+    x = Q.spinor(prm)
+    tmp = Q.spinor(prm)
+    it = Q.CG(x, dev(U), dev(b), 0.2, 1000, 1e-20, Q.gamm5Dw_sqr_msq, prm, Q.KernelParm(),
+              precision=prec)
+    xr, itr, st = O.cg(U, b, 0.2, 1000, 1e-20)
+    assert st == 0
+    assert _true_res(O, U, host(x), b, 0.2) < 1e-20
+    assert relerr(host(x), xr) < 1e-8
+    if prec == 0:
+        assert abs(it - itr) <= 1
+    else:
+        assert it <= 1.3 * itr + 10
+
+
+def test_cg_v2_shift_and_guess(Q, O):
+    say("CG v2")
+    L1, L2 = 32, 32
+    rng = np.random.default_rng(4)
+    U = random_links(L1, L2, rng)
+    b = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 4.0)
+    x = Q.spinor(prm)
+    dU, db = dev(U), dev(b)
+    it = Q.CG(x, dU, db, 0.5, 0.6, 2000, 1e-22, Q.gamm5Dw_sqr_musq, prm, Q.KernelParm())
+    xr, itr, st = O.cg(U, b, 0.5, 2000, 1e-22, mu=0.6, op=1)
+    assert abs(it - itr) <= 1
+    assert relerr(host(x), xr) < 1e-9
+    # warm start from the converged solution: reference returns 0 iterations
+    it2 = Q.CG(x, dU, db, 0.5, 0.6, 2000, 1e-22, Q.gamm5Dw_sqr_musq, prm, Q.KernelParm(),
+               so_as_guess=True)
+    assert it2 == 0
+    # warm start from a different shift's solution converges to the new shift
+    it3 = Q.CG(x, dU, db, 0.5, 0.9, 2000, 1e-22, Q.gamm5Dw_sqr_musq, prm, Q.KernelParm(),
+               so_as_guess=True)
+    xr3, _, _ = O.cg(U, b, 0.5, 2000, 1e-22, mu=0.9, op=1)
+    assert it3 > 0 and relerr(host(x), xr3) < 1e-9
+    # mixed precision with a guess
+    x.zero_()
+    Q.CG(x, dU, db, 0.5, 0.6, 2000, 1e-22, Q.gamm5Dw_sqr_musq, prm, Q.KernelParm(),
+         precision=Q.PREC_MIXED)
+    assert _true_res(O, U, host(x), b, 0.5, 0.6) < 1e-22
+
+
+def test_cg_not_converged_raises(Q):
+    say("CG maxiter")
+    L1, L2 = 24, 24
+    rng = np.random.default_rng(8)
+    prm = Q.LattParm((L1, L2), 4.0)
+    x = Q.spinor(prm)
+    with pytest.raises(Q.CGNotConverged):
+        Q.CG(x, dev(random_links(L1, L2, rng)), dev(random_spinor(L1, L2, rng)), 0.01, 3, 1e-30,
+             Q.gamm5Dw_sqr_msq, prm, Q.KernelParm())
+
+
+def test_cg_zero_rhs(Q):
+    prm = Q.LattParm((16, 16), 4.0)
+    x = Q.spinor(prm) + 1.0
+    it = Q.CG(x, Q.cold_links(prm), Q.spinor(prm), 0.1, 10, 1e-12, Q.gamm5Dw_sqr_msq, prm,
+              Q.KernelParm())
+    assert it == 0 and float(x.abs().max()) == 0.0
+
+
+def test_multicg_R_pseudofermion(Q, O):
+    say("MultiCG / R / generate_pseudofermion")
+    L1, L2 = 16, 16
+    rng = np.random.default_rng(12)
+    U = random_links(L1, L2, rng)
+    X = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 6.05)
+    rp = Q.get_rhmc_params(4, np.sqrt(0.1), np.sqrt(26.0))
+    z = O.zolotarev(4, np.sqrt(0.1), np.sqrt(26.0))
+    assert relerr(rp.rho, z["rho"]) < 1e-14 and relerr(rp.mu, z["mu"]) < 1e-14
+    dU, dX = dev(U), dev(X)
+    so = Q.spinor(prm)
+    Q.MultiCG(so, dU, dX, 0.3, 5000, 1e-24, Q.gamm5Dw_sqr_musq, rp, prm)
+    ref, bad = O.multicg(U, X, 0.3, z["mu"], z["rho"], 5000, 1e-24)
+    assert relerr(host(so), ref) < 1e-10
+    Q.R(so, dU, dX, 0.3, 5000, 1e-24, Q.gamm5Dw_sqr_musq, rp, prm)
+    ref, bad = O.R(U, X, 0.3, z, 5000, 1e-24)
+    assert relerr(host(so), ref) < 1e-10
+    F = Q.spinor(prm)
+    Q.generate_pseudofermion(F, dU, dX, 0.3, 5000, 1e-24, prm, None, rp)
+    Fr, bad = O.generate_pseudofermion(U, X, 0.3, z, 5000, 1e-24)
+    assert relerr(host(F), Fr) < 1e-10
+    # test/rhmc_pf_generation.jl:56-57: X^dag X = phi^dag MultiCG(phi)
+    Q.MultiCG(so, dU, F, 0.3, 5000, 1e-24, Q.gamm5Dw_sqr_musq, rp, prm)
+    assert abs(Q.dot(so, F, prm).real - O.norm2(X)) < 1e-10 * O.norm2(X)
+
+
+def test_power_method(Q, O):
+    say("power method")
+    L1, L2 = 10, 10
+    rng = np.random.default_rng(21)
+    U = random_links(L1, L2, rng)
+    b0 = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 6.05)
+    lmin, lmax = Q.power_method(dev(U), 10.0, prm, iter=300, b0=dev(b0))
+    rmin, rmax = O.power_method(U, b0, 10.0, 300)
+    assert abs(lmax - rmax) < 1e-10 * rmax
+    assert abs(lmin - rmin) < 1e-8 * rmax
+
+
+def test_omf4_pure_gauge(Q, O):
+    say("OMF4 pure gauge")
+    L1, L2 = 32, 32
+    U = O.hot_start(L1, L2, 77)
+    mom = O.momenta(L1, L2, 77, 0, 0)
+    prm = Q.LattParm((L1, L2), 4.0)
+    dU, dm = dev(U), dev(mom)
+    Q.OMF4_pure(dm, dU, 0.1, 10, prm)
+    p = O.latt(L1, L2, 4.0)
+    O.omf4_pure(mom, U, 0.1, 10, p)
+    assert relerr(host(dU), U) < 1e-12
+    assert relerr(host(dm), mom) < 1e-11
+
+
+@pytest.mark.parametrize("integ", [0, 1])
+def test_md_nf2(Q, O, integ):
+    say("MD nf2 integrator", integ)
+    L1, L2 = 16, 16
+    U = O.hot_start(L1, L2, 5)
+    mom = O.momenta(L1, L2, 5, 0, 1)
+    X = O.pseudofermion(L1, L2, 5, 0, 1)
+    F = O.gamm5Dw(U, X, 0.5)
+    prm = Q.LattParm((L1, L2), 4.0)
+    dU, dm, dX, dF, dg = dev(U), dev(mom), Q.spinor(prm), dev(F), Q.spinor(prm)
+    I = Q.OMF4() if integ else Q.Leapfrog()
+    Q.MD_nf2(I, dm, dU, dX, dF, dg, 0.5, 0.05, 4, 5000, 1e-24, prm)
+    p = O.latt(L1, L2, 4.0)
+    Xo = np.zeros_like(X)
+    if integ:
+        O.omf4_nf2(mom, U, Xo, F, 0.5, 0.05, 4, 5000, 1e-24, p)
+    else:
+        O.leapfrog_nf2(mom, U, Xo, F, 0.5, 0.05, 4, 5000, 1e-24, p)
+    assert relerr(host(dU), U) < 1e-10
+    assert relerr(host(dm), mom) < 1e-9
+    assert relerr(host(dX), Xo) < 1e-8
+
+
+def test_hmc_pure_sequence(Q, O):
+    say("HMC pure")
+    L1, L2 = 24, 24
+    prm = Q.LattParm((L1, L2), 4.0)
+    U = O.hot_start(L1, L2, 1234)
+    dU = dev(U)
+    p = O.latt(L1, L2, 4.0)
+    Q.seed(1234)
+    acc = []
+    for t in range(5):
+        info = Q.HMC(dU, 0.1, 10, acc, prm, None)
+        hi, hf, a = O.hmc_pure(U, 0.1, 10, p, 1234, t)
+        assert abs(info.hini - hi) < 1e-9 and abs(info.hfin - hf) < 1e-9
+        assert info.accepted == a
+    assert relerr(host(dU), U) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hmc_nf2_sequence(Q, O, prec):
+    """Config-1 physics at 24^2 (beta 4, m0 0.2, leapfrog ns=10 eps=0.1) with a
+    tight CG tolerance, so that FP64 and mixed MD forces agree with the oracle
+    to the level the energies are compared at."""
+    say("HMC nf2 prec", prec)
+    L1, L2 = 24, 24
+    prm = Q.LattParm((L1, L2), 4.0)
+    U = O.hot_start(L1, L2, 1234)
+    dU = dev(U)
+    p = O.latt(L1, L2, 4.0)
+    Q.seed(1234)
+    acc = []
+    for t in range(4):
+        info = Q.HMC(dU, 0.2, 0.1, 10, acc, 10000, 1e-24, prm, None, precision=prec)
+        hi, hf, a, it, rc = O.hmc_nf2(U, 0.2, 0.1, 10, 10000, 1e-24, p, 1234, t)
+        assert rc == 0
+        assert abs(info.hini - hi) < 1e-9
+        assert abs((info.hfin - info.hini) - (hf - hi)) < 1e-8
+        assert info.accepted == a
+    assert relerr(host(dU), U) < 1e-8
+
+
+def test_hmc_rhmc_reversibility(Q, O):
+    """test/rhmc_hmc_reversibility.jl:41-50 restated: forward + backward MD
+    returns the gauge action to 1e-10."""
+    say("RHMC reversibility")
+    L1, L2 = 10, 10
+    rng = np.random.default_rng(1234)
+    U = random_links(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 6.05)
+    dU = dev(U)
+    b0 = dev(random_spinor(L1, L2, rng))
+    lmin, lmax = Q.power_method(dU, 10.0, prm, iter=1000, b0=b0)
+    rp = Q.get_rhmc_params([5], [np.sqrt(lmin)], [np.sqrt(lmax)])
+    S0 = Q.Action(dU, prm)
+    Q.seed(1234)
+    info = Q.HMC(dU, [10.0], 0.01, 100, [], 10000, 1e-25, prm, None, rp,
+                 reversibility_test=True)
+    assert info.accepted == -1
+    assert abs(Q.Action(dU, prm) - S0) < 1e-10
+
+
+def test_hmc_rhmc_vs_oracle(Q, O):
+    say("RHMC vs oracle")
+    L1, L2 = 12, 12
+    prm = Q.LattParm((L1, L2), 4.0)
+    U = O.hot_start(L1, L2, 99)
+    dU = dev(U)
+    p = O.latt(L1, L2, 4.0)
+    z = O.zolotarev(6, np.sqrt(0.05), np.sqrt(20.0))
+    rp = Q.get_rhmc_params(6, np.sqrt(0.05), np.sqrt(20.0))
+    Q.seed(99)
+    for t in range(2):
+        info = Q.HMC(dU, [0.5, 0.7], 0.1, 5, [], 10000, 1e-24, prm, None, [rp, rp])
+        hi, hf, a, rc = O.hmc_rhmc(U, [0.5, 0.7], [z, z], 0.1, 5, 10000, 1e-24, p, 99, t)
+        assert rc == 0
+        assert abs(info.hini - hi) < 1e-9
+        assert abs(info.dH - (hf - hi)) < 1e-8
+        assert info.accepted == a
