@@ -17,9 +17,11 @@ importing this module without the built library raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import math
 import os
+import weakref
 from dataclasses import dataclass, field
 from typing import List, Sequence
 
@@ -138,6 +140,19 @@ class CGNotConverged(QED2dError):
 # ---------------------------------------------------------------------------
 # parameter types (src/QED2d.jl:5-26, src/integrators/integrators.jl:1-3)
 # ---------------------------------------------------------------------------
+_live = weakref.WeakSet()
+_exiting = False
+
+
+@atexit.register
+def _destroy_all():
+    """Release every context before the HIP runtime tears itself down."""
+    global _exiting
+    for prm in list(_live):
+        prm.close()
+    _exiting = True
+
+
 class LattParm:
     """LattParm(iL, beta) (src/QED2d.jl:5-8); ``bc`` adds the fermion boundary
     flag (0 periodic as the reference, BC_APERIODIC_T antiperiodic in i2)."""
@@ -164,18 +179,22 @@ class LattParm:
                 raise QED2dError(rc, "qed2d_ctx_create failed")
             self._ctx = h
             self._ctx_beta = self.beta
+            _live.add(self)
         if self._ctx_beta != self.beta:
             lib.qed2d_ctx_set_beta(self._ctx, self.beta)
             self._ctx_beta = self.beta
         return self._ctx
 
+    def close(self):
+        if getattr(self, "_ctx", None) is not None and not _exiting:
+            lib.qed2d_ctx_destroy(self._ctx)
+        self._ctx = None
+
     def __del__(self):
-        if getattr(self, "_ctx", None) is not None:
-            try:
-                lib.qed2d_ctx_destroy(self._ctx)
-            except Exception:
-                pass
-            self._ctx = None
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass
