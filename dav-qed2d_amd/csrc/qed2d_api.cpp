@@ -41,6 +41,11 @@ struct qed2d_ctx {
     float2 *fspin[FS_NSLOTS] = {};
     double *real[R_NSLOTS] = {};
     std::vector<double2 *> flav;  // RHMC pseudofermions, one per flavour
+    std::vector<double2 *> shiftbuf;  // multi-shift CG: internal x_i
+    std::vector<double2 *> psbuf;     // multi-shift CG: search directions p_i
+    int multishift = 1;               // MultiCG / RHMC force through multi-shift CG
+    MultiShiftState *ms = nullptr;    // device
+    MultiShiftState *hms = nullptr;   // pinned
     double *part = nullptr;
     long part_len = 0;
     SolverState *state = nullptr;
@@ -463,6 +468,109 @@ int cg_any(qed2d_ctx *c, double *x, const double *U, const double *b, double am0
     return cg_f64(c, x, U, b, a, iters);
 }
 
+// Multi-shift CG: (Q^2 + mu_i^2) x_i = b for all i at once; x[i] are device
+// spinors (may be internal buffers).  Each shift stops on its own
+// |r_i|^2 < eps |b|^2, the reference's per-CG rule (src/QED2dSolver.jl:16,88).
+int grow(qed2d_ctx *c, std::vector<double2 *> &v, int n)
+{
+    while ((int)v.size() < n) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(double2) * spinor_elems(c)) != hipSuccess)
+            return fail(c, QED2D_ENOMEM, "hipMalloc of a multi-shift buffer failed");
+        v.push_back((double2 *)p);
+    }
+    return QED2D_OK;
+}
+
+// nx internal solution buffers, np search-direction buffers, the state.
+int get_shiftbufs(qed2d_ctx *c, int nx, int np)
+{
+    TRY(grow(c, c->shiftbuf, nx));
+    TRY(grow(c, c->psbuf, np));
+    if (!c->ms) {
+        if (hipMalloc((void **)&c->ms, sizeof(MultiShiftState)) != hipSuccess ||
+            hipHostMalloc((void **)&c->hms, sizeof(MultiShiftState), hipHostMallocDefault) !=
+                hipSuccess)
+            return fail(c, QED2D_ENOMEM, "allocation of the multi-shift state failed");
+        HIPCHK(c, hipMemsetAsync(c->ms, 0, sizeof(MultiShiftState), c->stream));
+    }
+    return QED2D_OK;
+}
+
+int read_ms(qed2d_ctx *c)
+{
+    HIPCHK(c, hipMemcpyAsync(c->hms, c->ms, sizeof(MultiShiftState), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_harvest(c);
+    return QED2D_OK;
+}
+
+int multishift_cg(qed2d_ctx *c, double2 *const *x, const double *U, const double *b, double am0,
+                  int n, const double *mu, int maxiter, double eps, int *iters)
+{
+    if (n < 1 || n > kMaxShift)
+        return fail(c, QED2D_EINVAL, "multi-shift CG takes 1..%d shifts (got %d)", kMaxShift, n);
+    TRY(get_shiftbufs(c, 0, n));
+    double2 *r, *P[2], *t;
+    TRY(get_spin(c, S_R, &r));
+    TRY(get_spin(c, S_P0, &P[0]));
+    TRY(get_spin(c, S_P1, &P[1]));
+    TRY(get_spin(c, S_T, &t));
+    MultiShiftState &h = *c->hms;
+    std::memset(&h, 0, sizeof h);
+    double sb = mu[0] * mu[0];
+    for (int k = 1; k < n; ++k) sb = std::min(sb, mu[k] * mu[k]);
+    h.sigma_b = sb;
+    h.eps = eps;
+    h.maxiter = maxiter;
+    h.nshift = n;
+    for (int k = 0; k < n; ++k) {
+        h.s[k] = mu[k] * mu[k] - sb;
+        h.x[k] = x[k];
+        h.ps[k] = c->psbuf[k];
+    }
+    HIPCHK(c, hipMemcpyAsync(c->ms, c->hms, sizeof(MultiShiftState), hipMemcpyHostToDevice,
+                             c->stream));
+    const Latt &L = c->L;
+    hipStream_t st = c->stream;
+    launch_ms_init(L, (const double2 *)b, r, P[0], c->ms, c->part, n, st);
+    TRY(launched(c));
+    TRY(read_ms(c));
+    int it = 1;
+    double prev = c->hms->rr;
+    int prev_it = 0;
+    while (c->hms->status == 0) {
+        int chunk = 8;
+        if (it > 1 && c->hms->rr < prev && c->hms->iter > prev_it) {
+            // estimate from the slowest (base) residual
+            double rate = std::log(c->hms->rr / prev) / (double)(c->hms->iter - prev_it);
+            double left = std::log(c->hms->tol / c->hms->rr) / rate;
+            if (left > 0.0 && left < 1e6) chunk = (int)std::ceil(left);
+        }
+        chunk = std::max(4, std::min(64, chunk));
+        prev = c->hms->rr;
+        prev_it = c->hms->iter;
+        for (int j = 0; j < chunk && it <= maxiter; ++j, ++it) {
+            PROF(c, QED2D_PROF_OTHER,
+                 launch_ms_k1(L, (const double2 *)U, r, P[(it - 1) & 1], P[it & 1], t, am0, c->ms,
+                              c->part, n, c->rows, st));
+            PROF(c, QED2D_PROF_OTHER,
+                 launch_ms_k2(L, (const double2 *)U, t, P[it & 1], r, am0, c->ms, c->part, n,
+                              c->rows, st));
+        }
+        TRY(launched(c));
+        TRY(read_ms(c));
+        if (it > maxiter && c->hms->status == 0) break;
+    }
+    launch_ms_final(L, c->ms, n, st);
+    TRY(launched(c));
+    TRY(read_ms(c));
+    if (iters) *iters = c->hms->iter;
+    if (c->hms->status == 1) return QED2D_OK;
+    return fail(c, QED2D_ENOCONV, "multi-shift CG not converged after %d iterations", maxiter);
+}
+
 // ---------------------------------------------------------------------------
 // MD
 // ---------------------------------------------------------------------------
@@ -594,6 +702,28 @@ int update_momenta_rhmc(qed2d_ctx *c, double *mom, const double *U, double *X,
     double *facc = nullptr;
     TRY(get_real(c, R_FACC, &facc));
     bool first = true;
+    if (c->multishift) {
+        // all poles of a flavour in one multi-shift solve
+        for (int j = 0; j < nf; ++j) {
+            if (npole[j] < 1 || npole[j] > kMaxShift) continue;
+            TRY(get_shiftbufs(c, npole[j], npole[j]));
+            int it = 0;
+            TRY(multishift_cg(c, c->shiftbuf.data(), U, (const double *)F[j], am0[j], npole[j],
+                              mu[j], maxiter, tol, &it));
+            for (int i = 0; i < npole[j]; ++i) {
+                const double *Xi = (const double *)c->shiftbuf[i];
+                q64(c, U, Xi, g5DX, am0[j]);
+                launch_force_acc(c->L, (const double2 *)U, (const double2 *)Xi,
+                                 (const double2 *)g5DX, facc, rho[j][i], first ? 1 : 0,
+                                 c->stream);
+                first = false;
+            }
+        }
+        (void)X;
+        launch_force_mom(c->L, (const double2 *)U, nullptr, nullptr, first ? nullptr : facc, mom,
+                         c->beta, eps, 0.0, c->stream);
+        return launched(c);
+    }
     for (int j = 0; j < nf; ++j) {
         int guess = 0;
         for (int i = 0; i < npole[j]; ++i) {
@@ -635,6 +765,15 @@ int multicg_impl(qed2d_ctx *c, double *so, const double *U, const double *si, do
     double2 *aux;
     TRY(get_spin(c, S_AUX, &aux));
     const long n2 = spinor_elems(c);
+    if (c->multishift && n >= 1 && n <= kMaxShift) {
+        TRY(get_shiftbufs(c, n, n));
+        int it = 0;
+        TRY(multishift_cg(c, c->shiftbuf.data(), U, si, am0, n, mu, maxiter, eps, &it));
+        launch_copy((double2 *)so, (const double2 *)si, n2, c->stream);
+        for (int j = 0; j < n; ++j)
+            launch_scale_add((double2 *)so, 1.0, c->shiftbuf[j], rho[j], n2, c->stream);
+        return launched(c);
+    }
     launch_copy((double2 *)so, (const double2 *)si, n2, c->stream);
     for (int j = 0; j < n; ++j) {
         int it;
@@ -719,6 +858,10 @@ void qed2d_ctx_destroy(qed2d_ctx *c)
     for (auto &p : c->fspin) if (p) (void)hipFree(p);
     for (auto &p : c->real) if (p) (void)hipFree(p);
     for (auto p : c->flav) if (p) (void)hipFree(p);
+    for (auto p : c->shiftbuf) if (p) (void)hipFree(p);
+    for (auto p : c->psbuf) if (p) (void)hipFree(p);
+    if (c->ms) (void)hipFree(c->ms);
+    if (c->hms) (void)hipHostFree(c->hms);
     if (c->part) (void)hipFree(c->part);
     if (c->state) (void)hipFree(c->state);
     if (c->red) (void)hipFree(c->red);
@@ -1314,6 +1457,30 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
     }
     TRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
     if (!*accepted) launch_copy((double2 *)U, Ucp, n2, st);
+    return finish(c);
+}
+
+int qed2d_ctx_set_multishift(qed2d_ctx *c, int on)
+{
+    CHECK_CTX(c);
+    c->multishift = on ? 1 : 0;
+    return QED2D_OK;
+}
+
+int qed2d_multishift_cg(qed2d_ctx *c, double *const *x, const double *U, const double *b,
+                        double am0, int n, const double *mu, int maxiter, double eps, int *iters)
+{
+    CHECK_CTX(c);
+    NEED(c, x); NEED(c, U); NEED(c, b); NEED(c, mu);
+    if (n < 1 || n > kMaxShift)
+        return fail(c, QED2D_EINVAL, "multi-shift CG takes 1..%d shifts (got %d)", kMaxShift, n);
+    std::vector<double2 *> xs(n);
+    for (int k = 0; k < n; ++k) {
+        NEED(c, x[k]);
+        NOALIAS(c, (const double *)x[k], b);
+        xs[k] = (double2 *)x[k];
+    }
+    TRY(multishift_cg(c, xs.data(), U, b, am0, n, mu, maxiter, eps, iters));
     return finish(c);
 }
 

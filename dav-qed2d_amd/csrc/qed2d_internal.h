@@ -33,6 +33,30 @@ struct SolverState {
     unsigned ticket[8];
 };
 
+// Multi-shift CG state (all shifts sigma_i = mu_i^2 of (Q^2 + sigma_i) x_i = b on
+// one Krylov sequence).  Shift i is active 1 while iterating, 2 once converged
+// with its last x update still pending, 0 when finished.
+constexpr int kMaxShift = 32;
+struct MultiShiftState {
+    double rr;          // |r_k|^2
+    double tt, pp;      // |Q p_k|^2, |p_k|^2
+    double alpha_prev;  // alpha_{k-1}
+    double beta;        // beta_k (p_k = r_k + beta_k p_{k-1})
+    double sigma_b;     // base shift (the smallest)
+    double tol, bnorm, eps;
+    int iter, maxiter, status, nshift;
+    double s[kMaxShift];         // sigma_i - sigma_b
+    double zeta[kMaxShift];      // zeta_k
+    double zeta_prev[kMaxShift]; // zeta_{k-1}
+    double alpha_s[kMaxShift];   // alpha^sigma_{k-1} (pending x update)
+    double beta_s[kMaxShift];    // beta^sigma_k
+    int active[kMaxShift];
+    int conv_iter[kMaxShift];
+    double2 *x[kMaxShift];       // solution of shift i
+    double2 *ps[kMaxShift];      // search direction of shift i
+    unsigned ticket[4];
+};
+
 // Scalars of a reduction that the host reads back.
 struct RedOut {
     double v[8];
@@ -93,6 +117,18 @@ void launch_ru_residual(const Latt &L, const double2 *U, const double2 *b, const
                         const double2 *x, float2 *rs, float2 *p0, double am0, SolverState *s,
                         double *part, int rows, int initial, hipStream_t st);
 void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st);
+
+// Multi-shift CG: init (r = b, x_i = 0, p_i = 0, p0 = 0, |b|^2), the two
+// per-iteration kernels and the pending final x updates.
+void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0,
+                    MultiShiftState *s, double *part, int nshift, hipStream_t st);
+void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, const double2 *pold,
+                  double2 *pnew, double2 *t, double am0, MultiShiftState *s, double *part,
+                  int nshift, int rows, hipStream_t st);
+void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p, double2 *r,
+                  double am0, MultiShiftState *s, double *part, int nshift, int rows,
+                  hipStream_t st);
+void launch_ms_final(const Latt &L, MultiShiftState *s, int nshift, hipStream_t st);
 
 // Reductions (deterministic; results in out->v[]).
 void launch_norm2(const double2 *a, long n, RedOut *out, double *part, hipStream_t st);

@@ -930,6 +930,267 @@ void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const SolverState *s
 }
 
 // ===========================================================================
+// Multi-shift CG (Jegerlehner, hep-lat/9612014): the systems
+// (Q^2 + sigma_i) x_i = b share the Krylov space of the base system
+// (Q^2 + sigma_b), whose residual r_k fixes every shifted residual up to a
+// scalar, r^i_k = zeta^i_k r_k.  Per iteration k:
+//   K1: p_k = r_k + beta_k p_{k-1}, t = Q p_k, |t|^2, |p_k|^2, and for each
+//       shift x_i += alpha^i_{k-1} p^i_{k-1}, p^i_k = zeta^i_k r_k + beta^i_k p^i_{k-1}
+//   K2: Ap = Q t + sigma_b p_k, alpha_k = |r_k|^2 / p_k^dag Ap, r_{k+1} = r_k - alpha_k Ap,
+//       |r_{k+1}|^2, then (last block) the zeta / alpha / beta recurrences:
+//   zeta_{k+1} = zeta_k zeta_{k-1} alpha_{k-1} /
+//                (alpha_k beta_k (zeta_{k-1} - zeta_k) + zeta_{k-1} alpha_{k-1} (1 + s alpha_k))
+//   alpha^i_k = alpha_k zeta_{k+1} / zeta_k,   beta^i_{k+1} = beta_{k+1} (zeta_{k+1} / zeta_k)^2
+// with s = sigma_i - sigma_b.  A shift stops once zeta^2 |r|^2 < tol.
+// Algorithmic bytes per site and iteration (FP64): 320 + 128 n_active.
+// ===========================================================================
+struct EpiMS1 {
+    c2<double> *__restrict__ pnew;
+    c2<double> *__restrict__ t;
+    const c2<double> *__restrict__ r;
+    const MultiShiftState *S;
+    int nshift;
+    long V;
+    double tt, pp;
+    __device__ __forceinline__ void one(long i, double2 rv)
+    {
+        for (int k = 0; k < nshift; ++k) {
+            const int a = S->active[k];
+            if (a == 0) continue;
+            double2 *ps = S->ps[k];
+            double2 *x = S->x[k];
+            const double2 pv = ps[i];
+            const double as = S->alpha_s[k];
+            double2 xv = x[i];
+            x[i] = mk<double>(xv.x + as * pv.x, xv.y + as * pv.y);
+            if (a == 1) {
+                const double z = S->zeta[k], bs = S->beta_s[k];
+                ps[i] = mk<double>(z * rv.x + bs * pv.x, z * rv.y + bs * pv.y);
+            }
+        }
+    }
+    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2 c1,
+                                               double2 cc2)
+    {
+        pnew[s] = c1;
+        pnew[V + s] = cc2;
+        t[s] = o1;
+        t[V + s] = o2;
+        tt += (o1.x * o1.x + o1.y * o1.y) + (o2.x * o2.x + o2.y * o2.y);
+        pp += (c1.x * c1.x + c1.y * c1.y) + (cc2.x * cc2.x + cc2.y * cc2.y);
+        one(s, r[s]);
+        one(V + s, r[V + s]);
+    }
+};
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_ms1(Latt L, const double2 *__restrict__ U,
+                                                const double2 *__restrict__ r,
+                                                const double2 *__restrict__ pold,
+                                                double2 *__restrict__ pnew,
+                                                double2 *__restrict__ t, double mass2,
+                                                MultiShiftState *S, double *part, int nshift,
+                                                int bx, int by, int nbx)
+{
+    __shared__ double lds[kBlock / kWave];
+    if (S->status != 0) return;
+    MarchPos p = march_pos(L, bx, by, nbx, R);
+    EpiMS1 epi{pnew, t, r, S, nshift, L.V, 0.0, 0.0};
+    if (p.active) {
+        SrcXpay<double> src{r, pold, S->beta, L.V};
+        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
+    }
+    double a = block_sum(epi.tt, lds);
+    double b = block_sum(epi.pp, lds);
+    if (threadIdx.x == 0) {
+        part[2 * p.blk] = a;
+        part[2 * p.blk + 1] = b;
+    }
+    if (last_block_arrives<2>(&S->ticket[0], gridDim.x)) {
+        double res[2];
+        sum_partials<2>(part, gridDim.x, res, lds);
+        if (threadIdx.x == 0) {
+            S->tt = res[0];
+            S->pp = res[1];
+            for (int k = 0; k < nshift; ++k) {
+                if (S->active[k] == 2) S->active[k] = 0;  // last x update done
+                S->alpha_s[k] = 0.0;
+            }
+            S->ticket[0] = 0u;
+        }
+    }
+}
+
+struct EpiMS2 {
+    double2 *__restrict__ r;
+    const double2 *__restrict__ p;
+    double alpha, sigma;
+    long V;
+    double rr;
+    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2, double2)
+    {
+        const double2 p1 = p[s], p2 = p[V + s];
+        double2 r1 = r[s], r2 = r[V + s];
+        r1 = mk<double>(r1.x - alpha * (o1.x + sigma * p1.x), r1.y - alpha * (o1.y + sigma * p1.y));
+        r2 = mk<double>(r2.x - alpha * (o2.x + sigma * p2.x), r2.y - alpha * (o2.y + sigma * p2.y));
+        r[s] = r1;
+        r[V + s] = r2;
+        rr += (r1.x * r1.x + r1.y * r1.y) + (r2.x * r2.x + r2.y * r2.y);
+    }
+};
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_ms2(Latt L, const double2 *__restrict__ U,
+                                                const double2 *__restrict__ t,
+                                                const double2 *__restrict__ pv,
+                                                double2 *__restrict__ r, double mass2,
+                                                MultiShiftState *S, double *part, int nshift,
+                                                int bx, int by, int nbx)
+{
+    __shared__ double lds[kBlock / kWave];
+    if (S->status != 0) return;
+    MarchPos p = march_pos(L, bx, by, nbx, R);
+    const double alpha = S->rr / (S->tt + S->sigma_b * S->pp);
+    EpiMS2 epi{r, pv, alpha, S->sigma_b, L.V, 0.0};
+    if (p.active) {
+        SrcPlain<double> src{t, L.V};
+        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
+    }
+    double a = block_sum(epi.rr, lds);
+    if (threadIdx.x == 0) part[p.blk] = a;
+    if (last_block_arrives<1>(&S->ticket[1], gridDim.x)) {
+        double res[1];
+        sum_partials<1>(part, gridDim.x, res, lds);
+        if (threadIdx.x == 0) {
+            const double rr_new = res[0];
+            const double beta_k = S->beta, alpha_p = S->alpha_prev;
+            const double beta_n = rr_new / S->rr;
+            int running = 0;
+            for (int k = 0; k < nshift; ++k) {
+                if (S->active[k] != 1) continue;
+                const double z = S->zeta[k], zp = S->zeta_prev[k];
+                const double zn = z * zp * alpha_p /
+                                  (alpha * beta_k * (zp - z) + zp * alpha_p * (1.0 + S->s[k] * alpha));
+                S->alpha_s[k] = alpha * zn / z;
+                S->beta_s[k] = beta_n * (zn / z) * (zn / z);
+                S->zeta_prev[k] = z;
+                S->zeta[k] = zn;
+                if (zn * zn * rr_new < S->tol) {
+                    S->active[k] = 2;
+                    S->conv_iter[k] = S->iter + 1;
+                } else {
+                    running = 1;
+                }
+            }
+            S->alpha_prev = alpha;
+            S->beta = beta_n;
+            S->rr = rr_new;
+            S->iter += 1;
+            if (!running) S->status = 1;
+            else if (S->iter >= S->maxiter) S->status = 2;
+            S->ticket[1] = 0u;
+        }
+    }
+}
+
+// Pending x updates of the shifts that converged in the last iteration.
+__global__ void k_ms_final(MultiShiftState *S, int nshift, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        for (int k = 0; k < nshift; ++k) {
+            if (S->active[k] != 2) continue;
+            const double as = S->alpha_s[k];
+            const double2 pv = S->ps[k][i];
+            double2 xv = S->x[k][i];
+            S->x[k][i] = mk<double>(xv.x + as * pv.x, xv.y + as * pv.y);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ms_init(const double2 *__restrict__ b,
+                                                    double2 *__restrict__ r,
+                                                    double2 *__restrict__ p0, long n,
+                                                    MultiShiftState *S, double *part,
+                                                    int nshift)
+{
+    __shared__ double lds[kBlock / kWave];
+    double acc = 0.0;
+    const double2 z = mk<double>(0.0, 0.0);
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        const double2 v = b[i];
+        r[i] = v;
+        p0[i] = z;
+        for (int k = 0; k < nshift; ++k) {
+            S->x[k][i] = z;
+            S->ps[k][i] = z;
+        }
+        acc += v.x * v.x + v.y * v.y;
+    }
+    double a = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = a;
+    if (last_block_arrives<1>(&S->ticket[2], gridDim.x)) {
+        double res[1];
+        sum_partials<1>(part, gridDim.x, res, lds);
+        if (threadIdx.x == 0) {
+            S->bnorm = res[0];
+            S->rr = res[0];
+            S->tol = S->eps * res[0];
+            S->alpha_prev = 1.0;
+            S->beta = 0.0;
+            S->iter = 0;
+            for (int k = 0; k < nshift; ++k) {
+                S->zeta[k] = 1.0;
+                S->zeta_prev[k] = 1.0;
+                S->alpha_s[k] = 0.0;
+                S->beta_s[k] = 0.0;
+                S->active[k] = 1;
+                S->conv_iter[k] = 0;
+            }
+            S->status = (res[0] == 0.0) ? 1 : 0;
+            if (res[0] == 0.0)
+                for (int k = 0; k < nshift; ++k) S->active[k] = 0;
+            S->ticket[2] = 0u;
+        }
+    }
+}
+
+void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0,
+                    MultiShiftState *s, double *part, int nshift, hipStream_t st)
+{
+    const long n = 2 * L.V;
+    hipLaunchKernelGGL(k_ms_init, dim3(red_blocks(n)), dim3(kBlock), 0, st, b, r, p0, n, s, part,
+                       nshift);
+}
+
+void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, const double2 *pold,
+                  double2 *pnew, double2 *t, double am0, MultiShiftState *s, double *part,
+                  int nshift, int rows, hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    dim3 grid(g.nblocks()), blk(kBlock);
+    DISPATCH_ROWS(rows, k_ms1, grid, blk, 0, st, L, U, r, pold, pnew, t, 2.0 + am0, s, part,
+                  nshift, g.bx, g.by, g.nbx);
+}
+
+void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p, double2 *r,
+                  double am0, MultiShiftState *s, double *part, int nshift, int rows,
+                  hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    dim3 grid(g.nblocks()), blk(kBlock);
+    DISPATCH_ROWS(rows, k_ms2, grid, blk, 0, st, L, U, t, p, r, 2.0 + am0, s, part, nshift, g.bx,
+                  g.by, g.nbx);
+}
+
+void launch_ms_final(const Latt &L, MultiShiftState *s, int nshift, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ms_final, dim3(ew_blocks(2 * L.V)), dim3(kBlock), 0, st, s, nshift,
+                       2 * L.V);
+}
+
+// ===========================================================================
 // Forces and link updates
 // ===========================================================================
 

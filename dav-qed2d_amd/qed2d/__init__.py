@@ -95,6 +95,8 @@ SIGNATURES = {
     "qed2d_hmc_rhmc": (_i, [_p, _p, _pd, _i, _pi, _p, _p, _p, _d, _i, _i, _d, _i, _i, _u64, _u32,
                             _u32, _pd, _pd, _pi]),
     "qed2d_unfold": (_i, [_p, _p, _p, _i]),
+    "qed2d_multishift_cg": (_i, [_p, C.POINTER(_p), _p, _p, _d, _i, _pd, _i, _d, _pi]),
+    "qed2d_ctx_set_multishift": (_i, [_p, _i]),
     "qed2d_profile_enable": (_i, [_p, _i]),
     "qed2d_profile_reset": (_i, [_p]),
     "qed2d_profile_read": (_i, [_p, _i, C.POINTER(C.c_long), _pd]),
@@ -493,6 +495,23 @@ def get_rhmc_params(n_rhmc, r_a_rhmc, r_b_rhmc, reweighting_N=1, reweighting_Tay
 def _arr(v):
     a = (_d * len(v))(*[float(x) for x in v])
     return a
+
+
+def multishift_CG(xs, U, si, am0, mus, maxiter, eps, prm):
+    """(Q^2 + mu_i^2) x_i = si for every shift at once (multi-shift CG).
+    Returns the iteration count of the slowest shift."""
+    n = len(mus)
+    ptrs = (_p * n)(*[_sp(prm, x).value for x in xs])
+    it = _i()
+    _check(prm, lib.qed2d_multishift_cg(prm.ctx(), ptrs, _sp(prm, U), _sp(prm, si), am0, n,
+                                        _arr(mus), int(maxiter), eps, C.byref(it)))
+    return it.value
+
+
+def set_multishift(prm, on):
+    """MultiCG / R / RHMC force through multi-shift CG (default) or through the
+    reference's sequential CG solves."""
+    _check(prm, lib.qed2d_ctx_set_multishift(prm.ctx(), 1 if on else 0))
 
 
 def MultiCG(so, U, si, am0, maxiter, eps, A, rprm, prm, kprm=None):

@@ -153,6 +153,17 @@ int qed2d_cg(qed2d_ctx *ctx, double *so, const double *U, const double *si, doub
  * so = (1 + sum_j rho_j (Q^2 + mu_j^2)^-1) si. */
 int qed2d_multicg(qed2d_ctx *ctx, double *so, const double *U, const double *si, double am0,
                   int n, const double *mu, const double *rho, int maxiter, double eps);
+/* Multi-shift CG: (Q^2 + mu_i^2) x_i = b for i < n <= 32 on one Krylov
+ * sequence (Jegerlehner, hep-lat/9612014).  x is a host array of n device
+ * pointers.  Replaces the n sequential CG solves of MultiCG / R
+ * (src/QED2dRHMC.jl:264-267,284-287) and of the RHMC force
+ * (src/QED2dHMC.jl:528-546); each shift stops on the reference's rule
+ * |r_i|^2 < eps |b|^2. */
+int qed2d_multishift_cg(qed2d_ctx *ctx, double *const *x, const double *U, const double *b,
+                        double am0, int n, const double *mu, int maxiter, double eps, int *iters);
+/* 1 (default): MultiCG, R and the RHMC force use qed2d_multishift_cg; 0: the
+ * reference's sequential (warm-started) CG solves. */
+int qed2d_ctx_set_multishift(qed2d_ctx *ctx, int on);
 /* R(so, U, si, am0, maxiter, eps, A, rprm, prm, kprm) (src/QED2dRHMC.jl:261-271) */
 int qed2d_R(qed2d_ctx *ctx, double *so, const double *U, const double *si, double am0, int n,
             const double *mu, const double *rho, double A, double r_b, int maxiter, double eps);

@@ -243,3 +243,43 @@ def test_hmc_rhmc_vs_oracle(Q, O):
         assert abs(info.hini - hi) < 1e-9
         assert abs(info.dH - (hf - hi)) < 1e-8
         assert info.accepted == a
+
+
+@pytest.mark.parametrize("n", [1, 5, 16])
+def test_multishift_cg_vs_sequential_oracle(Q, O, n):
+    """Multi-shift CG against the oracle's per-shift CG (the reference's
+    MultiCG loop, src/QED2dRHMC.jl:284-287): each x_i to the tolerance."""
+    say("multishift", n)
+    L1, L2 = 32, 24
+    rng = np.random.default_rng(n)
+    U = random_links(L1, L2, rng, unitary=True)
+    b = random_spinor(L1, L2, rng)
+    z = O.zolotarev(n, np.sqrt(0.01), np.sqrt(17.0))
+    prm = Q.LattParm((L1, L2), 4.0)
+    xs = [Q.spinor(prm) for _ in range(n)]
+    it = Q.multishift_CG(xs, dev(U), dev(b), 0.15, list(z["mu"]), 5000, 1e-22, prm)
+    assert it > 0
+    for k in range(n):
+        xr, itr, st = O.cg(U, b, 0.15, 5000, 1e-22, mu=z["mu"][k], op=1)
+        assert _true_res(O, U, host(xs[k]), b, 0.15, z["mu"][k]) < 1e-21
+        assert relerr(host(xs[k]), xr) < 1e-9
+
+
+def test_multicg_multishift_and_sequential_agree(Q, O):
+    say("MultiCG multishift vs sequential")
+    L1, L2 = 24, 24
+    rng = np.random.default_rng(77)
+    U = random_links(L1, L2, rng)
+    X = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 4.0)
+    rp = Q.get_rhmc_params(8, np.sqrt(0.05), np.sqrt(30.0))
+    z = O.zolotarev(8, np.sqrt(0.05), np.sqrt(30.0))
+    dU, dX = dev(U), dev(X)
+    a, b = Q.spinor(prm), Q.spinor(prm)
+    Q.set_multishift(prm, True)
+    Q.MultiCG(a, dU, dX, 0.3, 5000, 1e-24, Q.gamm5Dw_sqr_musq, rp, prm)
+    Q.set_multishift(prm, False)
+    Q.MultiCG(b, dU, dX, 0.3, 5000, 1e-24, Q.gamm5Dw_sqr_musq, rp, prm)
+    Q.set_multishift(prm, True)
+    ref, bad = O.multicg(U, X, 0.3, z["mu"], z["rho"], 5000, 1e-24)
+    assert relerr(host(a), ref) < 1e-10 and relerr(host(b), ref) < 1e-10
