@@ -759,6 +759,36 @@ int leapfrog_rhmc(qed2d_ctx *c, double *mom, double *U, double *X, const std::ve
                                maxiter, tol);
 }
 
+// OMF4 with the RHMC force: a build extension (the reference's MD! has no
+// OMF4 method for RHMC, src/QED2dHMC.jl:12-16); stage order of :307-422.
+int omf4_rhmc(qed2d_ctx *c, double *mom, double *U, double *X, const std::vector<double2 *> &F,
+              double *g5DX, const double *am0, int nf, const int *npole, const double *const *mu,
+              const double *const *rho, double eps, int ns, int maxiter, double tol)
+{
+    const double r1 = OMF_R1, r2 = OMF_R2, r3 = OMF_R3, r4 = OMF_R4;
+    const double r5 = 0.5 - r1 - r3, r6 = 1.0 - 2.0 * (r2 + r4);
+    const double pc[6] = {r1, r3, r5, r5, r3, r1};
+    const double uc[5] = {r2, r4, r6, r4, r2};
+    for (int i = 0; i < ns; ++i) {
+        for (int s = 0; s < 6; ++s) {
+            TRY(update_momenta_rhmc(c, mom, U, X, F, g5DX, am0, nf, npole, mu, rho, pc[s] * eps,
+                                    maxiter, tol));
+            if (s < 5) updt(c, U, mom, eps * uc[s]);
+        }
+    }
+    return launched(c);
+}
+
+int md_rhmc(qed2d_ctx *c, int integrator, double *mom, double *U, double *X,
+            const std::vector<double2 *> &F, double *g5DX, const double *am0, int nf,
+            const int *npole, const double *const *mu, const double *const *rho, double eps,
+            int ns, int maxiter, double tol)
+{
+    if (integrator == QED2D_OMF4)
+        return omf4_rhmc(c, mom, U, X, F, g5DX, am0, nf, npole, mu, rho, eps, ns, maxiter, tol);
+    return leapfrog_rhmc(c, mom, U, X, F, g5DX, am0, nf, npole, mu, rho, eps, ns, maxiter, tol);
+}
+
 int multicg_impl(qed2d_ctx *c, double *so, const double *U, const double *si, double am0, int n,
                  const double *mu, const double *rho, int maxiter, double eps)
 {
@@ -1388,7 +1418,7 @@ int qed2d_hmc_nf2(qed2d_ctx *c, double *U, double am0, double eps, int ns, int i
 
 int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int *npole,
                    const double *const *mu, const double *const *nu, const double *const *rho,
-                   double eps, int ns, int maxiter, double tol, int qzero,
+                   double eps, int ns, int integrator, int maxiter, double tol, int qzero,
                    int reversibility_test, uint64_t seed, uint32_t chain, uint32_t traj,
                    double *hini, double *hfin, int *accepted)
 {
@@ -1426,8 +1456,8 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
         TRY(gen_pf_impl(c, (double *)F[j], U, (const double *)X, am0[j], npole[j], mu[j], nu[j],
                         maxiter, tol));
     }
-    int rc = leapfrog_rhmc(c, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu, rho,
-                           eps, ns, maxiter, tol);
+    int rc = md_rhmc(c, integrator, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu,
+                     rho, eps, ns, maxiter, tol);
     if (rc != QED2D_OK) {
         std::string msg = c->err;
         launch_copy((double2 *)U, Ucp, n2, st);
@@ -1450,14 +1480,40 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
     if (reversibility_test) {
         // src/QED2dHMC.jl:159-164
         launch_negate(mom, 2 * c->L.V, st);
-        TRY(leapfrog_rhmc(c, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu, rho, eps,
-                          ns, maxiter, tol));
+        TRY(md_rhmc(c, integrator, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu,
+                    rho, eps, ns, maxiter, tol));
         *accepted = -1;
         return finish(c);
     }
     TRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
     if (!*accepted) launch_copy((double2 *)U, Ucp, n2, st);
     return finish(c);
+}
+
+int qed2d_save_gauge(qed2d_ctx *c, const double *U, const char *path)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, path);
+    const long n = 2 * c->L.V;
+    std::vector<double> h(2 * n);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(h.data(), U, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+    if (qed2d_write_links_text(h.data(), n, path) != QED2D_OK)
+        return fail(c, QED2D_EINVAL, "cannot write %s", path);
+    return QED2D_OK;
+}
+
+int qed2d_load_gauge(qed2d_ctx *c, double *U, const char *path)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, path);
+    const long n = 2 * c->L.V;
+    std::vector<double> h(2 * n);
+    if (qed2d_read_links_text(h.data(), n, path) != QED2D_OK)
+        return fail(c, QED2D_EINVAL, "cannot read %ld links from %s", n, path);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(U, h.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+    return QED2D_OK;
 }
 
 int qed2d_ctx_set_multishift(qed2d_ctx *c, int on)

@@ -92,11 +92,15 @@ SIGNATURES = {
     "qed2d_hmc_pure": (_i, [_p, _p, _d, _i, _i, _i, _u64, _u32, _u32, _pd, _pd, _pi]),
     "qed2d_hmc_nf2": (_i, [_p, _p, _d, _d, _i, _i, _i, _d, _i, _i, _d, _u64, _u32, _u32, _pd,
                            _pd, _pi, _pi]),
-    "qed2d_hmc_rhmc": (_i, [_p, _p, _pd, _i, _pi, _p, _p, _p, _d, _i, _i, _d, _i, _i, _u64, _u32,
-                            _u32, _pd, _pd, _pi]),
+    "qed2d_hmc_rhmc": (_i, [_p, _p, _pd, _i, _pi, _p, _p, _p, _d, _i, _i, _i, _d, _i, _i, _u64,
+                            _u32, _u32, _pd, _pd, _pi]),
     "qed2d_unfold": (_i, [_p, _p, _p, _i]),
     "qed2d_multishift_cg": (_i, [_p, C.POINTER(_p), _p, _p, _d, _i, _pd, _i, _d, _pi]),
     "qed2d_ctx_set_multishift": (_i, [_p, _i]),
+    "qed2d_save_gauge": (_i, [_p, _p, C.c_char_p]),
+    "qed2d_load_gauge": (_i, [_p, _p, C.c_char_p]),
+    "qed2d_write_links_text": (_i, [_p, C.c_long, C.c_char_p]),
+    "qed2d_read_links_text": (_i, [_p, C.c_long, C.c_char_p]),
     "qed2d_profile_enable": (_i, [_p, _i]),
     "qed2d_profile_reset": (_i, [_p]),
     "qed2d_profile_read": (_i, [_p, _i, C.POINTER(C.c_long), _pd]),
@@ -654,8 +658,8 @@ def HMC(U, *args, integrator=None, qzero=False, reversibility_test=False,
                                       C.byref(acc_o), C.byref(it)))
     elif len(args) == 9:
         am0s, eps, ns, acc, CGmaxiter, tol, prm, kprm, rprms = args
-        if integrator is not None and not isinstance(integrator, Leapfrog):
-            raise TypeError("RHMC supports the Leapfrog integrator only (src/QED2dHMC.jl:12-16)")
+        # OMF4 is an extension: the reference's MD! has no OMF4 method for RHMC.
+        integ = integrator or Leapfrog()
         nf = len(am0s)
         keep = []
         mus = (_p * nf)()
@@ -669,7 +673,7 @@ def HMC(U, *args, integrator=None, qzero=False, reversibility_test=False,
             rhos[j] = C.cast(c, _p)
         npole = (_i * nf)(*[r.n for r in rprms])
         _check(prm, lib.qed2d_hmc_rhmc(prm.ctx(), _sp(prm, U), _arr(am0s), nf, npole, mus, nus,
-                                       rhos, eps, int(ns), int(CGmaxiter), tol,
+                                       rhos, eps, int(ns), integ.code, int(CGmaxiter), tol,
                                        1 if qzero else 0, 1 if reversibility_test else 0,
                                        _rng.seed, _rng.chain, t, C.byref(hi), C.byref(hf),
                                        C.byref(acc_o)))
@@ -679,6 +683,40 @@ def HMC(U, *args, integrator=None, qzero=False, reversibility_test=False,
     if acc_o.value >= 0 and acc is not None:
         acc.append(acc_o.value)
     return TrajInfo(hi.value, hf.value, acc_o.value, it.value, t)
+
+
+# ---------------------------------------------------------------------------
+# src/QED2dLoadSave.jl
+# ---------------------------------------------------------------------------
+def save_gauge(U, file, prm):
+    """save_gauge (src/QED2dLoadSave.jl:1-10)."""
+    _check(prm, lib.qed2d_save_gauge(prm.ctx(), _sp(prm, U), os.fsencode(file)))
+
+
+def load_gauge(U, file, prm):
+    """load_gauge (src/QED2dLoadSave.jl:12-19)."""
+    _check(prm, lib.qed2d_load_gauge(prm.ctx(), _sp(prm, U), os.fsencode(file)))
+
+
+def write_links_text(Uh, file):
+    """Host form: Uh is a numpy complex128 array in the (2, L2, L1) layout."""
+    a = np.ascontiguousarray(Uh, dtype=np.complex128)
+    rc = lib.qed2d_write_links_text(a.ctypes.data_as(_p), a.size, os.fsencode(file))
+    if rc != QED2D_OK:
+        raise QED2dError(rc, f"cannot write {file}")
+
+
+def read_links_text(file, L1, L2):
+    a = np.zeros((2, L2, L1), np.complex128)
+    rc = lib.qed2d_read_links_text(a.ctypes.data_as(_p), a.size, os.fsencode(file))
+    if rc != QED2D_OK:
+        raise QED2dError(rc, f"cannot read {2 * L1 * L2} links from {file}")
+    return a
+
+
+def config_name(prm, am0, n):
+    """File name of main/hmc-nf2.jl:143."""
+    return f"config_{prm.iL[0]}_{prm.iL[1]}_b{prm.beta}_m{am0}_n{n}"
 
 
 # ---------------------------------------------------------------------------

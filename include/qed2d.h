@@ -221,13 +221,24 @@ int qed2d_hmc_nf2(qed2d_ctx *ctx, double *U, double am0, double eps, int ns, int
                   uint64_t seed, uint32_t chain, uint32_t traj, double *hini, double *hfin,
                   int *accepted, int *cg_iters);
 /* HMC!(U, am0::Array, eps, ns, acc, CGmaxiter, tol, prm, kprm, rprm::Array;
- *      qzero, reversibility_test)                            (src/QED2dHMC.jl:116-195)
- * nf flavours; flavour j has npole[j] poles mu[j][], nu[j][], rho[j][]. */
+ *      integrator, qzero, reversibility_test)                (src/QED2dHMC.jl:116-195)
+ * nf flavours; flavour j has npole[j] poles mu[j][], nu[j][], rho[j][].
+ * integrator QED2D_OMF4 is an extension: the reference's MD! has no OMF4
+ * method for RHMC (src/QED2dHMC.jl:12-16). */
 int qed2d_hmc_rhmc(qed2d_ctx *ctx, double *U, const double *am0, int nf, const int *npole,
                    const double *const *mu, const double *const *nu, const double *const *rho,
-                   double eps, int ns, int maxiter, double tol, int qzero,
+                   double eps, int ns, int integrator, int maxiter, double tol, int qzero,
                    int reversibility_test, uint64_t seed, uint32_t chain, uint32_t traj,
                    double *hini, double *hfin, int *accepted);
+
+/* ---- gauge-configuration files -----------------------------------------------------
+ * save_gauge(U, file, prm) / load_gauge(U, file, prm) (src/QED2dLoadSave.jl:1-19):
+ * text, one link per line "re<TAB>im" in column-major order.  U is a device
+ * field; the host-only forms take a host array of nlinks = 2 L1 L2 complex numbers. */
+int qed2d_save_gauge(qed2d_ctx *ctx, const double *U, const char *path);
+int qed2d_load_gauge(qed2d_ctx *ctx, double *U, const char *path);
+int qed2d_write_links_text(const double *U_host, long nlinks, const char *path);
+int qed2d_read_links_text(double *U_host, long nlinks, const char *path);
 
 /* ---- measurement -----------------------------------------------------------------------
  * HIP-event timing of the library's own kernels, per kernel class, on the

@@ -967,6 +967,31 @@ int orc_leapfrog_rhmc(double *mom, double *U, double *X, double *const *F, doubl
 #undef ACC
 }
 
+/* OMF4 with the RHMC force.  The reference has no such method (MD! dispatch,
+ * src/QED2dHMC.jl:12-16); this is a build extension combining the OMF4 stage
+ * order of src/QED2dHMC.jl:307-422 with the RHMC force of :516-557. */
+int orc_omf4_rhmc(double *mom, double *U, double *X, double *const *F, double *g5DX,
+                  const double *am0, int nf, const int *npole, double *const *mu,
+                  double *const *rho, double eps, int ns, int maxiter, double tol,
+                  const orc_latt *p)
+{
+    const double r1 = OMF_R1, r2 = OMF_R2, r3 = OMF_R3, r4 = OMF_R4;
+    const double r5 = 0.5 - r1 - r3, r6 = 1.0 - 2.0 * (r2 + r4);
+    const double pc[6] = {r1, r3, r5, r5, r3, r1};
+    const double uc[5] = {r2, r4, r6, r4, r2};
+    int tot = 0;
+    for (int i = 0; i < ns; ++i) {
+        for (int st = 0; st < 6; ++st) {
+            int it = orc_update_momenta_rhmc(mom, U, X, F, g5DX, am0, nf, npole, mu, rho,
+                                             pc[st] * eps, maxiter, tol, p);
+            if (it < 0) return -1;
+            tot += it;
+            if (st < 5) orc_updt_fld(U, mom, eps * uc[st], p->L1, p->L2);
+        }
+    }
+    return tot;
+}
+
 /* Metropolis step: src/QED2dHMC.jl:31-51 (same for :84-104, :166-186). */
 static int metropolis(double hini, double hfin, int qzero, const double *U, const orc_latt *p,
                       uint64_t seed, uint32_t chain, uint32_t traj)
@@ -1035,10 +1060,21 @@ out:
 }
 
 /* RHMC step, one pseudofermion per flavour: src/QED2dHMC.jl:116-195. */
+static int md_rhmc(int integrator, double *mom, double *U, double *X, double *const *F,
+                   double *g5DX, const double *am0, int nf, const int *npole, double *const *mu,
+                   double *const *rho, double eps, int ns, int maxiter, double tol,
+                   const orc_latt *p)
+{
+    if (integrator == 1)
+        return orc_omf4_rhmc(mom, U, X, F, g5DX, am0, nf, npole, mu, rho, eps, ns, maxiter, tol, p);
+    return orc_leapfrog_rhmc(mom, U, X, F, g5DX, am0, nf, npole, mu, rho, eps, ns, maxiter, tol, p);
+}
+
 int orc_hmc_rhmc(double *U, const double *am0, int nf, const int *npole, double *const *mu,
-                 double *const *nu, double *const *rho, double eps, int ns, int maxiter,
-                 double tol, int qzero, int reversibility_test, uint64_t seed, uint32_t chain,
-                 uint32_t traj, const orc_latt *p, double *hini, double *hfin, int *accepted)
+                 double *const *nu, double *const *rho, double eps, int ns, int integrator,
+                 int maxiter, double tol, int qzero, int reversibility_test, uint64_t seed,
+                 uint32_t chain, uint32_t traj, const orc_latt *p, double *hini, double *hfin,
+                 int *accepted)
 {
     long V = (long)p->L1 * p->L2;
     double *Ucp = (double *)malloc(sizeof(double) * 4 * V);
@@ -1057,7 +1093,7 @@ int orc_hmc_rhmc(double *U, const double *am0, int nf, const int *npole, double 
         bad |= orc_generate_pseudofermion(F[j], U, X, am0[j], npole[j], mu[j], nu[j], maxiter,
                                           tol, p->L1, p->L2, p->bc);
     }
-    if (orc_leapfrog_rhmc(mom, U, X, F, g5DX, am0, nf, npole, mu, rho, eps, ns, maxiter, tol, p) < 0)
+    if (md_rhmc(integrator, mom, U, X, F, g5DX, am0, nf, npole, mu, rho, eps, ns, maxiter, tol, p) < 0)
         bad = 1;
     *hfin = orc_hamiltonian(mom, U, p->beta, p->L1, p->L2);
     for (int j = 0; j < nf; ++j) {
@@ -1069,8 +1105,8 @@ int orc_hmc_rhmc(double *U, const double *am0, int nf, const int *npole, double 
     }
     if (reversibility_test) {
         for (long k = 0; k < 2 * V; ++k) mom[k] = -mom[k];
-        if (orc_leapfrog_rhmc(mom, U, X, F, g5DX, am0, nf, npole, mu, rho, eps, ns, maxiter, tol,
-                              p) < 0)
+        if (md_rhmc(integrator, mom, U, X, F, g5DX, am0, nf, npole, mu, rho, eps, ns, maxiter, tol,
+                    p) < 0)
             bad = 1;
         *accepted = -1;
     } else {

@@ -133,6 +133,11 @@ int orc_leapfrog_rhmc(double *mom, double *U, double *X, double *const *F, doubl
                       double *const *rho, double eps, int ns, int maxiter, double tol,
                       const orc_latt *p);
 
+int orc_omf4_rhmc(double *mom, double *U, double *X, double *const *F, double *g5DX,
+                  const double *am0, int nf, const int *npole, double *const *mu,
+                  double *const *rho, double eps, int ns, int maxiter, double tol,
+                  const orc_latt *p);
+
 /* integrator: 0 leapfrog, 1 OMF4.  Output: hini, hfin, accepted (0/1). */
 int orc_hmc_pure(double *U, double eps, int ns, int integrator, int qzero, uint64_t seed,
                  uint32_t chain, uint32_t traj, const orc_latt *p, double *hini,
@@ -141,9 +146,10 @@ int orc_hmc_nf2(double *U, double am0, double eps, int ns, int integrator, int m
                 double tol, int qzero, uint64_t seed, uint32_t chain, uint32_t traj,
                 const orc_latt *p, double *hini, double *hfin, int *accepted, int *cg_iters);
 int orc_hmc_rhmc(double *U, const double *am0, int nf, const int *npole, double *const *mu,
-                 double *const *nu, double *const *rho, double eps, int ns, int maxiter,
-                 double tol, int qzero, int reversibility_test, uint64_t seed, uint32_t chain,
-                 uint32_t traj, const orc_latt *p, double *hini, double *hfin, int *accepted);
+                 double *const *nu, double *const *rho, double eps, int ns, int integrator,
+                 int maxiter, double tol, int qzero, int reversibility_test, uint64_t seed,
+                 uint32_t chain, uint32_t traj, const orc_latt *p, double *hini, double *hfin,
+                 int *accepted);
 
 /* ---- lattice I/O and master-field growth -------------------------------- */
 void orc_unfold(double *Uout, const double *Uin, int L1, int L2, int reflect);

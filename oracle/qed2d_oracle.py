@@ -89,8 +89,8 @@ _sig("orc_hmc_pure", _i, [_p, _d, _i, _i, _i, _u64, _u32, _u32, C.POINTER(Latt),
                           C.POINTER(_i)])
 _sig("orc_hmc_nf2", _i, [_p, _d, _d, _i, _i, _i, _d, _i, _u64, _u32, _u32, C.POINTER(Latt),
                          _pd, _pd, C.POINTER(_i), C.POINTER(_i)])
-_sig("orc_hmc_rhmc", _i, [_p, _p, _i, _p, _p, _p, _p, _d, _i, _i, _d, _i, _i, _u64, _u32, _u32,
-                          C.POINTER(Latt), _pd, _pd, C.POINTER(_i)])
+_sig("orc_hmc_rhmc", _i, [_p, _p, _i, _p, _p, _p, _p, _d, _i, _i, _i, _d, _i, _i, _u64, _u32,
+                          _u32, C.POINTER(Latt), _pd, _pd, C.POINTER(_i)])
 _sig("orc_update_momenta_nf2", _i, [_p, _p, _p, _p, _p, _d, _d, _i, _d, C.POINTER(Latt)])
 _sig("orc_omf4_pure", None, [_p, _p, _d, _i, C.POINTER(Latt)])
 _sig("orc_leapfrog_pure", None, [_p, _p, _d, _i, C.POINTER(Latt)])
@@ -375,7 +375,7 @@ def hmc_nf2(U, am0, eps, ns, maxiter, tol, p, seed, traj, chain=0, integrator=0,
 
 
 def hmc_rhmc(U, am0s, zs, eps, ns, maxiter, tol, p, seed, traj, chain=0, qzero=0,
-             reversibility_test=0):
+             reversibility_test=0, integrator=0):
     nf = len(am0s)
     am0 = np.ascontiguousarray(am0s, np.float64)
     npole = np.ascontiguousarray([z["n"] for z in zs], np.int32)
@@ -384,8 +384,9 @@ def hmc_rhmc(U, am0s, zs, eps, ns, maxiter, tol, p, seed, traj, chain=0, qzero=0
     nus = (C.c_void_p * nf)(*[keep[3 * j + 1].ctypes.data for j in range(nf)])
     rhos = (C.c_void_p * nf)(*[keep[3 * j + 2].ctypes.data for j in range(nf)])
     hi, hf, acc = _d(), _d(), _i()
-    rc = lib.orc_hmc_rhmc(_ptr(U), _ptr(am0), nf, _ptr(npole), mus, nus, rhos, eps, ns, maxiter,
-                          tol, qzero, reversibility_test, seed, chain, traj, C.byref(p),
+    rc = lib.orc_hmc_rhmc(_ptr(U), _ptr(am0), nf, _ptr(npole), mus, nus, rhos, eps, ns,
+                          integrator, maxiter, tol, qzero, reversibility_test, seed, chain, traj,
+                          C.byref(p),
                           C.byref(hi), C.byref(hf), C.byref(acc))
     return hi.value, hf.value, acc.value, rc
 
