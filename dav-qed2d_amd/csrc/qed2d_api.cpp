@@ -373,10 +373,10 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
         prev_iter = c->hstate->iter;
         for (int j = 0; j < chunk && it <= a.maxiter; ++j, ++it) {
             PROF(c, QED2D_PROF_CG_K1_F64,
-                 launch_cg_k1_f64(L, (const double2 *)U, r, P[(it - 1) & 1], P[it & 1], t,
-                                  a.am0, c->state, c->part, c->rows, st));
+                 launch_cg_k1_f64(L, (const double2 *)U, r, P[0], P[1], t, a.am0, c->state,
+                                  c->part, c->rows, st));
             PROF(c, QED2D_PROF_CG_K2_F64,
-                 launch_cg_k2_f64(L, (const double2 *)U, t, P[it & 1], (double2 *)x, r, a.am0,
+                 launch_cg_k2_f64(L, (const double2 *)U, t, P[0], P[1], (double2 *)x, r, a.am0,
                                   c->state, c->part, c->rows, 0, st));
         }
         TRY(launched(c));
@@ -437,11 +437,11 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
         prev_iter = c->hstate->iter;
         for (int j = 0; j < chunk; ++j, ++it) {
             PROF(c, QED2D_PROF_CG_K1_F32,
-                 launch_cg_k1_f32(L, Us, rs, P[(it - 1) & 1], P[it & 1], ts, am0f, c->state,
-                                  c->part, c->rows, st));
+                 launch_cg_k1_f32(L, Us, rs, P[0], P[1], ts, am0f, c->state, c->part, c->rows,
+                                  st));
             PROF(c, QED2D_PROF_CG_K2_F32,
-                 launch_cg_k2_f32(L, Us, ts, P[it & 1], xs, rs, am0f, c->state, c->part, c->rows,
-                                  1, st));
+                 launch_cg_k2_f32(L, Us, ts, P[0], P[1], xs, rs, am0f, c->state, c->part,
+                                  c->rows, 1, st));
         }
         TRY(launched(c));
         TRY(read_state(c));
@@ -553,10 +553,10 @@ int multishift_cg(qed2d_ctx *c, double2 *const *x, const double *U, const double
         prev_it = c->hms->iter;
         for (int j = 0; j < chunk && it <= maxiter; ++j, ++it) {
             PROF(c, QED2D_PROF_OTHER,
-                 launch_ms_k1(L, (const double2 *)U, r, P[(it - 1) & 1], P[it & 1], t, am0, c->ms,
-                              c->part, n, c->rows, st));
+                 launch_ms_k1(L, (const double2 *)U, r, P[0], P[1], t, am0, c->ms, c->part, n,
+                              c->rows, st));
             PROF(c, QED2D_PROF_OTHER,
-                 launch_ms_k2(L, (const double2 *)U, t, P[it & 1], r, am0, c->ms, c->part, n,
+                 launch_ms_k2(L, (const double2 *)U, t, P[0], P[1], r, am0, c->ms, c->part, n,
                               c->rows, st));
         }
         TRY(launched(c));

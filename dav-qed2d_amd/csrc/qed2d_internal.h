@@ -94,18 +94,20 @@ void launch_cg_init_f64(const Latt &L, const double2 *b, double2 *x, double2 *r,
 void launch_cg_guess_f64(const Latt &L, const double2 *U, const double2 *b, const double2 *x,
                          const double2 *t, double2 *r, double2 *p0, double am0, SolverState *s,
                          double *part, int rows, hipStream_t st);
-void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, const double2 *pold,
-                      double2 *pnew, double2 *t, double am0, SolverState *s, double *part,
+// p0 / p1: the two search-direction buffers, used alternately by iteration
+// parity on the device.
+void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2 *p0,
+                      double2 *p1, double2 *t, double am0, SolverState *s, double *part,
                       int rows, hipStream_t st);
-void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p,
-                      double2 *x, double2 *r, double am0, SolverState *s, double *part,
-                      int rows, int mixed, hipStream_t st);
-void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, const float2 *pold,
-                      float2 *pnew, float2 *t, float am0, SolverState *s, double *part,
-                      int rows, hipStream_t st);
-void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p,
-                      float2 *x, float2 *r, float am0, SolverState *s, double *part, int rows,
-                      int mixed, hipStream_t st);
+void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
+                      const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
+                      double *part, int rows, int mixed, hipStream_t st);
+void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
+                      float2 *t, float am0, SolverState *s, double *part, int rows,
+                      hipStream_t st);
+void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
+                      const float2 *p1, float2 *x, float2 *r, float am0, SolverState *s,
+                      double *part, int rows, int mixed, hipStream_t st);
 // Mixed precision helpers.
 void launch_cg_init_mixed(const Latt &L, const double2 *b, float2 *rs, float2 *xs, float2 *p0,
                           SolverState *s, double *part, hipStream_t st);
@@ -122,12 +124,12 @@ void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st);
 // per-iteration kernels and the pending final x updates.
 void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0,
                     MultiShiftState *s, double *part, int nshift, hipStream_t st);
-void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, const double2 *pold,
-                  double2 *pnew, double2 *t, double am0, MultiShiftState *s, double *part,
-                  int nshift, int rows, hipStream_t st);
-void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p, double2 *r,
-                  double am0, MultiShiftState *s, double *part, int nshift, int rows,
+void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, double2 *p0, double2 *p1,
+                  double2 *t, double am0, MultiShiftState *s, double *part, int nshift, int rows,
                   hipStream_t st);
+void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
+                  const double2 *p1, double2 *r, double am0, MultiShiftState *s, double *part,
+                  int nshift, int rows, hipStream_t st);
 void launch_ms_final(const Latt &L, MultiShiftState *s, int nshift, hipStream_t st);
 
 // Reductions (deterministic; results in out->v[]).

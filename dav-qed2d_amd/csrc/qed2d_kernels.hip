@@ -534,16 +534,22 @@ template <typename T> struct EpiK1 {
     }
 };
 
+// The two search-direction buffers alternate with the device's iteration
+// count: iteration k = iter + 1 reads P[iter & 1] and writes P[(iter + 1) & 1].
+// Selecting them on the device keeps the pairing right when queued
+// iterations are skipped (converged, or waiting for a reliable update).
 template <typename T, int R>
 __global__ void __launch_bounds__(kBlock) k_cg1(Latt L, const c2<T> *__restrict__ U,
-                                                const c2<T> *__restrict__ r,
-                                                const c2<T> *__restrict__ pold,
-                                                c2<T> *__restrict__ pnew, c2<T> *__restrict__ t,
-                                                T mass2, SolverState *S, double *part, int bx,
-                                                int by, int nbx)
+                                                const c2<T> *__restrict__ r, c2<T> *P0,
+                                                c2<T> *P1, c2<T> *__restrict__ t, T mass2,
+                                                SolverState *S, double *part, int bx, int by,
+                                                int nbx)
 {
     __shared__ double lds[kBlock / kWave];
     if (S->status != 0 || S->need_ru != 0) return;
+    const bool odd = (S->iter & 1) != 0;
+    const c2<T> *pold = odd ? P1 : P0;
+    c2<T> *pnew = odd ? P0 : P1;
     MarchPos p = march_pos(L, bx, by, nbx, R);
     EpiK1<T> epi{pnew, t, L.V, 0.0, 0.0};
     if (p.active) {
@@ -594,14 +600,14 @@ template <typename T> struct EpiK2 {
 
 template <typename T, int R>
 __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict__ U,
-                                                const c2<T> *__restrict__ t,
-                                                const c2<T> *__restrict__ pv,
-                                                c2<T> *__restrict__ x, c2<T> *__restrict__ r,
-                                                T mass2, SolverState *S, double *part, int bx,
-                                                int by, int nbx, int mixed)
+                                                const c2<T> *__restrict__ t, const c2<T> *P0,
+                                                const c2<T> *P1, c2<T> *__restrict__ x,
+                                                c2<T> *__restrict__ r, T mass2, SolverState *S,
+                                                double *part, int bx, int by, int nbx, int mixed)
 {
     __shared__ double lds[kBlock / kWave];
     if (S->status != 0 || S->need_ru != 0) return;
+    const c2<T> *pv = (S->iter & 1) ? P0 : P1;
     MarchPos p = march_pos(L, bx, by, nbx, R);
     const double alpha = S->norm / (S->tt + S->mu2 * S->pp);
     EpiK2<T> epi{pv, x, r, (T)alpha, (T)S->mu2, L.V, 0.0};
@@ -645,9 +651,8 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
 }
 
 template <typename T>
-static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, const c2<T> *pold,
-                     c2<T> *pnew, c2<T> *t, T am0, SolverState *S, double *part, int rows,
-                     hipStream_t st)
+static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *pold, c2<T> *pnew,
+                     c2<T> *t, T am0, SolverState *S, double *part, int rows, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
     dim3 grid(g.nblocks()), blk(kBlock);
@@ -662,45 +667,45 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, const c2<T> 
 }
 
 template <typename T>
-static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> *p, c2<T> *x,
-                     c2<T> *r, T am0, SolverState *S, double *part, int rows, int mixed,
-                     hipStream_t st)
+static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> *p0,
+                     const c2<T> *p1, c2<T> *x, c2<T> *r, T am0, SolverState *S, double *part,
+                     int rows, int mixed, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
     dim3 grid(g.nblocks()), blk(kBlock);
     const T m2 = T(2) + am0;
     switch (rows) {
-    case 1: hipLaunchKernelGGL((k_cg2<T, 1>), grid, blk, 0, st, L, U, t, p, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
-    case 2: hipLaunchKernelGGL((k_cg2<T, 2>), grid, blk, 0, st, L, U, t, p, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
-    case 4: hipLaunchKernelGGL((k_cg2<T, 4>), grid, blk, 0, st, L, U, t, p, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
-    case 8: hipLaunchKernelGGL((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
-    default: hipLaunchKernelGGL((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
+    case 1: hipLaunchKernelGGL((k_cg2<T, 1>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
+    case 2: hipLaunchKernelGGL((k_cg2<T, 2>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
+    case 4: hipLaunchKernelGGL((k_cg2<T, 4>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
+    case 8: hipLaunchKernelGGL((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
+    default: hipLaunchKernelGGL((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
     }
 }
 
-void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, const double2 *pold,
-                      double2 *pnew, double2 *t, double am0, SolverState *s, double *part,
+void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2 *p0,
+                      double2 *p1, double2 *t, double am0, SolverState *s, double *part,
                       int rows, hipStream_t st)
 {
-    cg1_impl<double>(L, U, r, pold, pnew, t, am0, s, part, rows, st);
+    cg1_impl<double>(L, U, r, p0, p1, t, am0, s, part, rows, st);
 }
-void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p,
-                      double2 *x, double2 *r, double am0, SolverState *s, double *part,
-                      int rows, int mixed, hipStream_t st)
+void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
+                      const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
+                      double *part, int rows, int mixed, hipStream_t st)
 {
-    cg2_impl<double>(L, U, t, p, x, r, am0, s, part, rows, mixed, st);
+    cg2_impl<double>(L, U, t, p0, p1, x, r, am0, s, part, rows, mixed, st);
 }
-void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, const float2 *pold,
-                      float2 *pnew, float2 *t, float am0, SolverState *s, double *part,
-                      int rows, hipStream_t st)
+void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
+                      float2 *t, float am0, SolverState *s, double *part, int rows,
+                      hipStream_t st)
 {
-    cg1_impl<float>(L, U, r, pold, pnew, t, am0, s, part, rows, st);
+    cg1_impl<float>(L, U, r, p0, p1, t, am0, s, part, rows, st);
 }
-void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p,
-                      float2 *x, float2 *r, float am0, SolverState *s, double *part, int rows,
-                      int mixed, hipStream_t st)
+void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
+                      const float2 *p1, float2 *x, float2 *r, float am0, SolverState *s,
+                      double *part, int rows, int mixed, hipStream_t st)
 {
-    cg2_impl<float>(L, U, t, p, x, r, am0, s, part, rows, mixed, st);
+    cg2_impl<float>(L, U, t, p0, p1, x, r, am0, s, part, rows, mixed, st);
 }
 
 // x = 0, r = b, p0 = 0, |b|^2 -> norm, bnorm, tol  (src/QED2dSolver.jl:10-16)
@@ -985,15 +990,16 @@ struct EpiMS1 {
 
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_ms1(Latt L, const double2 *__restrict__ U,
-                                                const double2 *__restrict__ r,
-                                                const double2 *__restrict__ pold,
-                                                double2 *__restrict__ pnew,
-                                                double2 *__restrict__ t, double mass2,
-                                                MultiShiftState *S, double *part, int nshift,
-                                                int bx, int by, int nbx)
+                                                const double2 *__restrict__ r, double2 *P0,
+                                                double2 *P1, double2 *__restrict__ t,
+                                                double mass2, MultiShiftState *S, double *part,
+                                                int nshift, int bx, int by, int nbx)
 {
     __shared__ double lds[kBlock / kWave];
     if (S->status != 0) return;
+    const bool odd = (S->iter & 1) != 0;
+    const double2 *pold = odd ? P1 : P0;
+    double2 *pnew = odd ? P0 : P1;
     MarchPos p = march_pos(L, bx, by, nbx, R);
     EpiMS1 epi{pnew, t, r, S, nshift, L.V, 0.0, 0.0};
     if (p.active) {
@@ -1041,14 +1047,14 @@ struct EpiMS2 {
 
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_ms2(Latt L, const double2 *__restrict__ U,
-                                                const double2 *__restrict__ t,
-                                                const double2 *__restrict__ pv,
-                                                double2 *__restrict__ r, double mass2,
-                                                MultiShiftState *S, double *part, int nshift,
-                                                int bx, int by, int nbx)
+                                                const double2 *__restrict__ t, const double2 *P0,
+                                                const double2 *P1, double2 *__restrict__ r,
+                                                double mass2, MultiShiftState *S, double *part,
+                                                int nshift, int bx, int by, int nbx)
 {
     __shared__ double lds[kBlock / kWave];
     if (S->status != 0) return;
+    const double2 *pv = (S->iter & 1) ? P0 : P1;
     MarchPos p = march_pos(L, bx, by, nbx, R);
     const double alpha = S->rr / (S->tt + S->sigma_b * S->pp);
     EpiMS2 epi{r, pv, alpha, S->sigma_b, L.V, 0.0};
@@ -1164,24 +1170,24 @@ void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0,
                        nshift);
 }
 
-void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, const double2 *pold,
-                  double2 *pnew, double2 *t, double am0, MultiShiftState *s, double *part,
-                  int nshift, int rows, hipStream_t st)
-{
-    MarchGrid g = march_grid(L, rows);
-    dim3 grid(g.nblocks()), blk(kBlock);
-    DISPATCH_ROWS(rows, k_ms1, grid, blk, 0, st, L, U, r, pold, pnew, t, 2.0 + am0, s, part,
-                  nshift, g.bx, g.by, g.nbx);
-}
-
-void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p, double2 *r,
-                  double am0, MultiShiftState *s, double *part, int nshift, int rows,
+void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, double2 *p0, double2 *p1,
+                  double2 *t, double am0, MultiShiftState *s, double *part, int nshift, int rows,
                   hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
     dim3 grid(g.nblocks()), blk(kBlock);
-    DISPATCH_ROWS(rows, k_ms2, grid, blk, 0, st, L, U, t, p, r, 2.0 + am0, s, part, nshift, g.bx,
-                  g.by, g.nbx);
+    DISPATCH_ROWS(rows, k_ms1, grid, blk, 0, st, L, U, r, p0, p1, t, 2.0 + am0, s, part, nshift,
+                  g.bx, g.by, g.nbx);
+}
+
+void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
+                  const double2 *p1, double2 *r, double am0, MultiShiftState *s, double *part,
+                  int nshift, int rows, hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    dim3 grid(g.nblocks()), blk(kBlock);
+    DISPATCH_ROWS(rows, k_ms2, grid, blk, 0, st, L, U, t, p0, p1, r, 2.0 + am0, s, part, nshift,
+                  g.bx, g.by, g.nbx);
 }
 
 void launch_ms_final(const Latt &L, MultiShiftState *s, int nshift, hipStream_t st)
