@@ -46,14 +46,17 @@ struct qed2d_ctx {
     int multishift = 1;               // MultiCG / RHMC force through multi-shift CG
     MultiShiftState *ms = nullptr;    // device
     MultiShiftState *hms = nullptr;   // pinned
-    double *part = nullptr;
+    double *part = nullptr;  // partial sums of the generic reductions
     long part_len = 0;
+    double *cgpart = nullptr;  // partial sums of the solver kernels
+    CGPartials P{};
     SolverState *state = nullptr;
     RedOut *red = nullptr;
     SolverState *hstate = nullptr;  // pinned
     RedOut *hred = nullptr;         // pinned
     // kernel-class timing (qed2d_profile_*)
-    bool prof = false;
+    int prof = 0;  // 0 off, N: time every N-th launch of each class
+    long prof_seen[QED2D_PROF_NCLASS] = {};
     std::vector<hipEvent_t> evpool;
     struct Pending { int cls; hipEvent_t a, b; };
     std::vector<Pending> pending;
@@ -159,10 +162,12 @@ hipEvent_t prof_event(qed2d_ctx *c)
     return e;
 }
 
-// Opens a timed region of class cls; returns the start event (or null).
-hipEvent_t prof_begin(qed2d_ctx *c)
+// Opens a timed region; returns the start event (or null when this launch is
+// not sampled).
+hipEvent_t prof_begin(qed2d_ctx *c, int cls = QED2D_PROF_OTHER)
 {
     if (!c->prof) return nullptr;
+    if ((c->prof_seen[cls]++ % c->prof) != 0) return nullptr;
     hipEvent_t a = prof_event(c);
     if (a && hipEventRecord(a, c->stream) != hipSuccess) return nullptr;
     return a;
@@ -193,7 +198,7 @@ void prof_harvest(qed2d_ctx *c)
 
 #define PROF(c, cls, launch)                 \
     do {                                     \
-        hipEvent_t ev_ = prof_begin(c);      \
+        hipEvent_t ev_ = prof_begin(c, cls); \
         launch;                              \
         prof_end((c), (cls), ev_);           \
     } while (0)
@@ -323,26 +328,38 @@ int upload_state(qed2d_ctx *c, const SolveArgs &a)
     s.mu2 = a.mu2;
     s.eps = a.eps;
     s.maxiter = a.maxiter;
+    s.guess = a.guess;
     s.delta2 = 1e-2;  // reliable update once |r| has dropped 10x since the last one
     HIPCHK(c, hipMemcpyAsync(c->state, c->hstate, sizeof(SolverState), hipMemcpyHostToDevice,
                              c->stream));
     return QED2D_OK;
 }
 
-// Number of iterations to queue before looking at the state again: an
-// estimate of what is left from the observed convergence rate.
-int next_chunk(const SolverState &s, double prev_err, int prev_iter, int done_iters)
-{
-    int chunk = 8;
-    if (done_iters > 0 && prev_err > 0.0 && s.err > 0.0 && s.iter > prev_iter && s.err < prev_err) {
-        double rate = std::log(s.err / prev_err) / (double)(s.iter - prev_iter);
-        double left = std::log(s.tol / s.err) / rate;
-        if (left > 0.0 && left < 1e6) chunk = (int)std::ceil(left);
+// Iterations to queue before the next look at the state: an estimate of what
+// is left (to convergence, or for the mixed solver to the next reliable
+// update) from the convergence rate seen so far.  A solve stops at most a
+// few queued iterations early; the rest of the chunk returns at once.
+struct ChunkPredictor {
+    double err0 = 0.0;
+    int it0 = -1;
+    int next(double err, int iter, double tol, double ru_target)
+    {
+        int chunk = 8;
+        if (it0 >= 0 && iter > it0 && err > 0.0 && err0 > err) {
+            const double rate = std::log(err / err0) / (double)(iter - it0);  // < 0
+            double target = tol;
+            if (ru_target > target) target = ru_target;
+            const double left = std::log(target / err) / rate;
+            if (left > 0.0 && left < 1e6) chunk = (int)std::ceil(left) + 1;
+            else chunk = 2;
+        }
+        it0 = iter;
+        err0 = err;
+        if (chunk < 2) chunk = 2;
+        if (chunk > 64) chunk = 64;
+        return chunk;
     }
-    if (chunk < 4) chunk = 4;
-    if (chunk > 64) chunk = 64;
-    return chunk;
-}
+};
 
 int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const SolveArgs &a,
            int *iters)
@@ -357,31 +374,28 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
     hipStream_t st = c->stream;
     if (a.guess) {
         q64(c, U, (const double *)x, (double *)t, a.am0);
-        launch_cg_guess_f64(L, (const double2 *)U, (const double2 *)b, (const double2 *)x, t, r,
-                            P[0], a.am0, c->state, c->part, c->rows, st);
+        launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x, r,
+                        nullptr, P[0], nullptr, a.am0, c->state, c->P, 1, 0, c->rows, st);
     } else {
-        launch_cg_init_f64(L, (const double2 *)b, (double2 *)x, r, P[0], c->state, c->part, st);
+        launch_cg_init_f64(L, (const double2 *)b, (double2 *)x, r, P[0], c->P, c->rows, st);
     }
-    TRY(launched(c));
-    TRY(read_state(c));
-    int it = 1;
-    double prev_err = c->hstate->norm;
-    int prev_iter = 0;
-    while (c->hstate->status == 0) {
-        int chunk = next_chunk(*c->hstate, prev_err, prev_iter, it - 1);
-        prev_err = (it == 1) ? c->hstate->norm : c->hstate->err;
-        prev_iter = c->hstate->iter;
-        for (int j = 0; j < chunk && it <= a.maxiter; ++j, ++it) {
+    int k = 1;
+    int chunk = 8;
+    ChunkPredictor pred;
+    for (;;) {
+        for (int j = 0; j < chunk && k <= a.maxiter + 1; ++j, ++k) {
             PROF(c, QED2D_PROF_CG_K1_F64,
-                 launch_cg_k1_f64(L, (const double2 *)U, r, P[0], P[1], t, a.am0, c->state,
-                                  c->part, c->rows, st));
+                 launch_cg_k1_f64(L, (const double2 *)U, r, P[0], P[1], t, a.am0, c->state, c->P,
+                                  k, c->rows, st));
             PROF(c, QED2D_PROF_CG_K2_F64,
                  launch_cg_k2_f64(L, (const double2 *)U, t, P[0], P[1], (double2 *)x, r, a.am0,
-                                  c->state, c->part, c->rows, 0, st));
+                                  c->state, c->P, k, c->rows, st));
         }
         TRY(launched(c));
         TRY(read_state(c));
-        if (it > a.maxiter && c->hstate->status == 0) break;
+        if (c->hstate->status != 0) break;
+        if (k > a.maxiter + 1) break;  // unreachable: K1(maxiter + 1) ends the solve
+        chunk = pred.next(c->hstate->err, c->hstate->iter, c->hstate->tol, 0.0);
     }
     *iters = c->hstate->iter;
     if (c->hstate->status == 1) return QED2D_OK;
@@ -407,44 +421,44 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     const float am0f = (float)a.am0;
     launch_to_f32((const double2 *)U, Us, 2 * L.V, st);
     if (a.guess) {
-        HIPCHK(c, hipMemsetAsync(xs, 0, sizeof(float2) * 2 * L.V, st));
         q64(c, U, x, (double *)t, a.am0);
-        launch_ru_residual(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x, rs,
-                           P[0], a.am0, c->state, c->part, c->rows, 1, st);
+        launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x, nullptr,
+                        rs, nullptr, P[0], a.am0, c->state, c->P, 1, 1, c->rows, st);
+        HIPCHK(c, hipMemsetAsync(xs, 0, sizeof(float2) * 2 * L.V, st));
     } else {
-        HIPCHK(c, hipMemsetAsync(x, 0, sizeof(double2) * 2 * L.V, st));
-        launch_cg_init_mixed(L, (const double2 *)b, rs, xs, P[0], c->state, c->part, st);
+        launch_cg_init_mixed(L, (const double2 *)b, (double2 *)x, rs, xs, P[0], c->P, c->rows, st);
     }
-    TRY(launched(c));
-    TRY(read_state(c));
-    int it = 1;
-    double prev_err = c->hstate->norm;
-    int prev_iter = 0;
-    while (c->hstate->status == 0) {
-        if (c->hstate->need_ru) {
-            hipEvent_t ev = prof_begin(c);
-            launch_ru_accum(L, (double2 *)x, xs, c->state, st);
-            launch_q_apply_f64(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->rows, st);
-            launch_ru_residual(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
-                               rs, nullptr, a.am0, c->state, c->part, c->rows, 0, st);
-            prof_end(c, QED2D_PROF_RELIABLE, ev);
-            TRY(launched(c));
-            TRY(read_state(c));
-            continue;
-        }
-        int chunk = next_chunk(*c->hstate, prev_err, prev_iter, it - 1);
-        prev_err = (it == 1) ? c->hstate->norm : c->hstate->err;
-        prev_iter = c->hstate->iter;
-        for (int j = 0; j < chunk; ++j, ++it) {
+    int k = 1;
+    int chunk = 8;
+    ChunkPredictor pred;
+    for (;;) {
+        for (int j = 0; j < chunk; ++j, ++k) {
             PROF(c, QED2D_PROF_CG_K1_F32,
-                 launch_cg_k1_f32(L, Us, rs, P[0], P[1], ts, am0f, c->state, c->part, c->rows,
+                 launch_cg_k1_f32(L, Us, rs, P[0], P[1], ts, am0f, c->state, c->P, k, c->rows,
                                   st));
             PROF(c, QED2D_PROF_CG_K2_F32,
-                 launch_cg_k2_f32(L, Us, ts, P[0], P[1], xs, rs, am0f, c->state, c->part,
-                                  c->rows, 1, st));
+                 launch_cg_k2_f32(L, Us, ts, P[0], P[1], xs, rs, am0f, c->state, c->P, k, c->rows,
+                                  st));
         }
         TRY(launched(c));
         TRY(read_state(c));
+        if (c->hstate->status != 0) break;
+        if (c->hstate->need_ru) {
+            // K1(k) paused: fold the FP32 partial solution into x, recompute the
+            // true residual in FP64 and run K1(k) again on it.
+            k = c->hstate->iter + 1;
+            hipEvent_t ev = prof_begin(c, QED2D_PROF_RELIABLE);
+            launch_ru_accum(L, (double2 *)x, xs, st);
+            launch_q_apply_f64(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->rows, st);
+            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
+                            nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, k, 2, c->rows,
+                            st);
+            prof_end(c, QED2D_PROF_RELIABLE, ev);
+            chunk = 2;  // K1(k) decides at once whether the true residual converged
+            continue;
+        }
+        const SolverState &h = *c->hstate;
+        chunk = pred.next(h.err, h.iter, h.tol, h.delta2 * h.rmax2[(h.iter + 1) & 1]);
     }
     *iters = c->hstate->iter;
     if (c->hstate->status == 1) return QED2D_OK;
@@ -534,38 +548,24 @@ int multishift_cg(qed2d_ctx *c, double2 *const *x, const double *U, const double
                              c->stream));
     const Latt &L = c->L;
     hipStream_t st = c->stream;
-    launch_ms_init(L, (const double2 *)b, r, P[0], c->ms, c->part, n, st);
-    TRY(launched(c));
-    TRY(read_ms(c));
-    int it = 1;
-    double prev = c->hms->rr;
-    int prev_it = 0;
-    while (c->hms->status == 0) {
-        int chunk = 8;
-        if (it > 1 && c->hms->rr < prev && c->hms->iter > prev_it) {
-            // estimate from the slowest (base) residual
-            double rate = std::log(c->hms->rr / prev) / (double)(c->hms->iter - prev_it);
-            double left = std::log(c->hms->tol / c->hms->rr) / rate;
-            if (left > 0.0 && left < 1e6) chunk = (int)std::ceil(left);
-        }
-        chunk = std::max(4, std::min(64, chunk));
-        prev = c->hms->rr;
-        prev_it = c->hms->iter;
-        for (int j = 0; j < chunk && it <= maxiter; ++j, ++it) {
+    launch_ms_init(L, (const double2 *)b, r, P[0], c->ms, c->P, n, c->rows, st);
+    int k = 1;
+    int chunk = 8;
+    ChunkPredictor pred;
+    for (;;) {
+        for (int j = 0; j < chunk && k <= maxiter + 1; ++j, ++k) {
             PROF(c, QED2D_PROF_OTHER,
-                 launch_ms_k1(L, (const double2 *)U, r, P[0], P[1], t, am0, c->ms, c->part, n,
+                 launch_ms_k1(L, (const double2 *)U, r, P[0], P[1], t, am0, c->ms, c->P, k, n,
                               c->rows, st));
             PROF(c, QED2D_PROF_OTHER,
-                 launch_ms_k2(L, (const double2 *)U, t, P[0], P[1], r, am0, c->ms, c->part, n,
+                 launch_ms_k2(L, (const double2 *)U, t, P[0], P[1], r, am0, c->ms, c->P, k, n,
                               c->rows, st));
         }
         TRY(launched(c));
         TRY(read_ms(c));
-        if (it > maxiter && c->hms->status == 0) break;
+        if (c->hms->status != 0 || k > maxiter + 1) break;
+        chunk = pred.next(c->hms->err, c->hms->iter, c->hms->tol, 0.0);
     }
-    launch_ms_final(L, c->ms, n, st);
-    TRY(launched(c));
-    TRY(read_ms(c));
     if (iters) *iters = c->hms->iter;
     if (c->hms->status == 1) return QED2D_OK;
     return fail(c, QED2D_ENOCONV, "multi-shift CG not converged after %d iterations", maxiter);
@@ -862,16 +862,23 @@ int qed2d_ctx_create(qed2d_ctx **out, int L1, int L2, double beta, int bc_flags,
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; return QED2D_EHIP; }
     // partial sums: the largest marching grid (rows = 1) needs 2 doubles per
     // block, the observables 6 per block of at most 1024.
-    MarchGrid g = march_grid(c->L, 1);
-    c->part_len = 2L * g.nblocks() + 8 * 1024 + 64;
+    // generic reductions: at most 1024 blocks x 6 doubles; solver kernels: per
+    // block of the largest marching grid (rows = 1), 2 x 2 (tt) + 2 (rr) + 1 (b).
+    c->part_len = 8 * 1024;
+    const long nbmax = march_grid(c->L, 1).nblocks();
     bool ok = hipMalloc((void **)&c->part, sizeof(double) * c->part_len) == hipSuccess &&
+              hipMalloc((void **)&c->cgpart, sizeof(double) * 7 * nbmax) == hipSuccess &&
               hipMalloc((void **)&c->state, sizeof(SolverState)) == hipSuccess &&
               hipMalloc((void **)&c->red, sizeof(RedOut)) == hipSuccess &&
               hipHostMalloc((void **)&c->hstate, sizeof(SolverState), hipHostMallocDefault) ==
                   hipSuccess &&
               hipHostMalloc((void **)&c->hred, sizeof(RedOut), hipHostMallocDefault) == hipSuccess;
     if (!ok) { qed2d_ctx_destroy(c); return QED2D_ENOMEM; }
-    // tickets start at zero; every kernel that takes one resets it.
+    c->P.tt[0] = c->cgpart;
+    c->P.tt[1] = c->cgpart + 2 * nbmax;
+    c->P.rr[0] = c->cgpart + 4 * nbmax;
+    c->P.rr[1] = c->cgpart + 5 * nbmax;
+    c->P.b = c->cgpart + 6 * nbmax;
     ok = hipMemsetAsync(c->state, 0, sizeof(SolverState), c->stream) == hipSuccess &&
          hipMemsetAsync(c->red, 0, sizeof(RedOut), c->stream) == hipSuccess &&
          hipStreamSynchronize(c->stream) == hipSuccess;
@@ -893,6 +900,7 @@ void qed2d_ctx_destroy(qed2d_ctx *c)
     if (c->ms) (void)hipFree(c->ms);
     if (c->hms) (void)hipHostFree(c->hms);
     if (c->part) (void)hipFree(c->part);
+    if (c->cgpart) (void)hipFree(c->cgpart);
     if (c->state) (void)hipFree(c->state);
     if (c->red) (void)hipFree(c->red);
     if (c->hstate) (void)hipHostFree(c->hstate);
@@ -1540,10 +1548,12 @@ int qed2d_multishift_cg(qed2d_ctx *c, double *const *x, const double *U, const d
     return finish(c);
 }
 
-int qed2d_profile_enable(qed2d_ctx *c, int on)
+int qed2d_profile_enable(qed2d_ctx *c, int every)
 {
     CHECK_CTX(c);
-    c->prof = on != 0;
+    if (every < 0) return fail(c, QED2D_EINVAL, "sampling period must be >= 0");
+    c->prof = every;
+    for (auto &n : c->prof_seen) n = 0;
     return QED2D_OK;
 }
 

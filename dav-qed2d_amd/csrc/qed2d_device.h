@@ -145,42 +145,6 @@ __device__ __forceinline__ dd wave_sum_dd(dd v)
 }
 
 // ---------------------------------------------------------------------------
-// Grid-level "last block finishes" reduction.  Every block writes its partial
-// sums to part[blockIdx * NV + k], the block that arrives last (atomic
-// ticket) sums all partials in block order and calls fin().  The ticket is
-// reset by the last block, so no memset is needed between launches.
-// ---------------------------------------------------------------------------
-template <int NV>
-__device__ __forceinline__ bool last_block_arrives(unsigned *ticket, int nblocks)
-{
-    __shared__ bool amlast;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned t = atomicAdd(ticket, 1u);
-        amlast = (t == (unsigned)nblocks - 1u);
-    }
-    __syncthreads();
-    return amlast;
-}
-
-// Sum NV partial arrays (stride NV) over nblocks in a fixed order with the
-// whole block; out[k] valid in thread 0.
-template <int NV>
-__device__ __forceinline__ void sum_partials(const double *part, int nblocks, double *out,
-                                             double *lds)
-{
-    for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-            s += __builtin_nontemporal_load(&part[(long)b * NV + k]);
-        }
-        s = block_sum(s, lds);
-        if (threadIdx.x == 0) out[k] = s;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Counter-based random numbers: Philox4x32-10 (Salmon et al., SC'11) and
 // Box-Muller with log / sin / cos evaluated by fixed FMA sequences.  The
 // same sequences are restated in the CPU oracle, so host and device streams

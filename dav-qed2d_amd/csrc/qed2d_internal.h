@@ -9,58 +9,57 @@
 
 namespace qed2d {
 
-// Device-resident solver state.  Kernels read the scalars they need and the
-// last block of each reduction writes the new ones, so a whole CG iteration
-// runs without a host round trip.
+// Device-resident solver state.  No kernel synchronises the grid: each
+// kernel writes one partial sum per block, and the next kernel (after the
+// kernel boundary) sums them in block order, so every block derives the same
+// scalars.  Block 0 of the first CG kernel of iteration k commits the
+// scalars later kernels need into slot k & 1, so that no kernel reads a slot
+// that a block of the same kernel may be writing.
 struct SolverState {
-    double norm;    // |r|^2 entering the current iteration
-    double tol;     // eps * |b|^2
-    double beta;    // coefficient of the next search-direction update
-    double tt;      // |Q p|^2
-    double pp;      // |p|^2
-    double err;     // |r|^2 after the last update
-    double mu2;     // shift mu^2 of the operator Q^2 + mu^2
-    double rmax2;   // largest |r|^2 since the last reliable update
-    double delta2;  // reliable-update trigger (squared residual reduction)
-    double bnorm;   // |b|^2
-    double eps;     // requested squared relative residual
-    int iter;       // iterations completed
+    double norm[2];    // |r_{k-1}|^2, committed by K1(k) into slot k & 1
+    double rmax2[2];   // mixed precision: largest |r|^2 since the last reliable update
+    double tol, bnorm; // committed by K1(1)
+    double eps, mu2, delta2;  // set by the host
+    double err;        // last |r|^2 seen by K1 (host information)
+    int iter;          // iterations completed when K1 last committed
     int maxiter;
-    int status;     // 0 running, 1 converged, 2 iteration limit reached
-    int need_ru;    // mixed precision: a reliable update is due
-    int n_ru;       // reliable updates performed
-    int pad;
-    unsigned ticket[8];
+    int status;        // host information: 0 running, 1 converged, 2 iteration limit
+    int need_ru;       // host information: mixed precision waits for a reliable update
+    int n_ru;          // reliable updates performed
+    int guess;         // the solve started from a caller's guess
+    int true_at;       // K1(true_at) sees a true (FP64) residual norm
+    int stop_k;        // nonzero: K1(stop_k) ended or paused the solve; later kernels return
 };
 
 // Multi-shift CG state (all shifts sigma_i = mu_i^2 of (Q^2 + sigma_i) x_i = b on
-// one Krylov sequence).  Shift i is active 1 while iterating, 2 once converged
-// with its last x update still pending, 0 when finished.
+// one Krylov sequence).  Slots as above: K1(k) handles Krylov index j = k - 1.
 constexpr int kMaxShift = 32;
 struct MultiShiftState {
-    double rr;          // |r_k|^2
-    double tt, pp;      // |Q p_k|^2, |p_k|^2
-    double alpha_prev;  // alpha_{k-1}
-    double beta;        // beta_k (p_k = r_k + beta_k p_{k-1})
-    double sigma_b;     // base shift (the smallest)
-    double tol, bnorm, eps;
-    int iter, maxiter, status, nshift;
-    double s[kMaxShift];         // sigma_i - sigma_b
-    double zeta[kMaxShift];      // zeta_k
-    double zeta_prev[kMaxShift]; // zeta_{k-1}
-    double alpha_s[kMaxShift];   // alpha^sigma_{k-1} (pending x update)
-    double beta_s[kMaxShift];    // beta^sigma_k
-    int active[kMaxShift];
+    double norm[2];                  // |r_j|^2
+    double alpha[2];                 // alpha_{j-1}
+    double beta[2];                  // beta_j (p_j = r_j + beta_j p_{j-1})
+    double zeta[2][kMaxShift];       // zeta^i_j
+    double zeta_prev[2][kMaxShift];  // zeta^i_{j-1}
+    int active[2][kMaxShift];        // shift i still iterating after step j
+    double tol, bnorm, eps, sigma_b, err;
+    double s[kMaxShift];             // sigma_i - sigma_b
+    int iter, maxiter, status, nshift, stop_k, pad;
     int conv_iter[kMaxShift];
-    double2 *x[kMaxShift];       // solution of shift i
-    double2 *ps[kMaxShift];      // search direction of shift i
-    unsigned ticket[4];
+    double2 *x[kMaxShift];           // solution of shift i
+    double2 *ps[kMaxShift];          // search direction of shift i
+};
+
+// Per-block partial sums of the solver kernels.  tt: (|Q p|^2, |p|^2) per block,
+// two slots by iteration parity; rr: |r|^2 per block, two slots; b: |b|^2.
+struct CGPartials {
+    double *tt[2];
+    double *rr[2];
+    double *b;
 };
 
 // Scalars of a reduction that the host reads back.
 struct RedOut {
     double v[8];
-    unsigned ticket[8];
 };
 
 enum { kBlock = 256 };
@@ -87,50 +86,49 @@ void launch_gamm5(const Latt &L, const double2 *si, double2 *so, hipStream_t st)
 void launch_q_shift_f64(const Latt &L, const double2 *U, const double2 *si, const double2 *x,
                         double2 *so, double am0, double mu2, int rows, hipStream_t st);
 
-// CG (T = double or float).  part: per-block partial sums (>= 2*nblocks).
+// ---- CG on Q^2 + mu^2 (all take the marching grid of `rows`) -------------
+// Start from x = 0: x = 0, r = b, p0 = 0; |b|^2 partials into rr[0] and b.
 void launch_cg_init_f64(const Latt &L, const double2 *b, double2 *x, double2 *r, double2 *p0,
-                        SolverState *s, double *part, hipStream_t st);
-// r = b - (Q t + mu^2 x) with t = Q x already computed; p0 = 0; sets norm, bnorm, tol.
-void launch_cg_guess_f64(const Latt &L, const double2 *U, const double2 *b, const double2 *x,
-                         const double2 *t, double2 *r, double2 *p0, double am0, SolverState *s,
-                         double *part, int rows, hipStream_t st);
-// p0 / p1: the two search-direction buffers, used alternately by iteration
-// parity on the device.
+                        const CGPartials &P, int rows, hipStream_t st);
+// Mixed precision start from x = 0: x = 0, rs = float(b), xs = 0, p0 = 0.
+void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *rs, float2 *xs,
+                          float2 *p0, const CGPartials &P, int rows, hipStream_t st);
+// Residual from x with t = Q x already computed: r = b - (Q t + mu^2 x).
+// mode 0: FP64 start from a guess (writes r, zeroes p0, |b|^2 partials);
+// mode 1: mixed start from a guess (writes rs, zeroes p0s, |b|^2 partials);
+// mode 2: mixed reliable update before K1(k) (writes rs).
+// |r|^2 partials go to rr[(k - 1) & 1].
+void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const double2 *t,
+                     const double2 *x, double2 *r, float2 *rs, double2 *p0, float2 *p0s,
+                     double am0, SolverState *s, const CGPartials &P, int k, int mode, int rows,
+                     hipStream_t st);
+// Iteration k: K1 = convergence test, p = r + beta p, t = Q p; K2 = x, r update.
 void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2 *p0,
-                      double2 *p1, double2 *t, double am0, SolverState *s, double *part,
-                      int rows, hipStream_t st);
+                      double2 *p1, double2 *t, double am0, SolverState *s, const CGPartials &P,
+                      int k, int rows, hipStream_t st);
 void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
                       const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
-                      double *part, int rows, int mixed, hipStream_t st);
+                      const CGPartials &P, int k, int rows, hipStream_t st);
 void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
-                      float2 *t, float am0, SolverState *s, double *part, int rows,
+                      float2 *t, float am0, SolverState *s, const CGPartials &P, int k, int rows,
                       hipStream_t st);
 void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
                       const float2 *p1, float2 *x, float2 *r, float am0, SolverState *s,
-                      double *part, int rows, int mixed, hipStream_t st);
-// Mixed precision helpers.
-void launch_cg_init_mixed(const Latt &L, const double2 *b, float2 *rs, float2 *xs, float2 *p0,
-                          SolverState *s, double *part, hipStream_t st);
-void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const SolverState *s,
-                     hipStream_t st);
-// rs = float(b - (Q t + mu^2 x)), t = Q x.  initial != 0: first residual of a
-// solve (beta = 0, sets bnorm and tol); otherwise a reliable update.
-void launch_ru_residual(const Latt &L, const double2 *U, const double2 *b, const double2 *t,
-                        const double2 *x, float2 *rs, float2 *p0, double am0, SolverState *s,
-                        double *part, int rows, int initial, hipStream_t st);
+                      const CGPartials &P, int k, int rows, hipStream_t st);
+// x += double(xs); xs = 0 (a reliable update).
+void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, hipStream_t st);
 void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st);
 
-// Multi-shift CG: init (r = b, x_i = 0, p_i = 0, p0 = 0, |b|^2), the two
-// per-iteration kernels and the pending final x updates.
-void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0,
-                    MultiShiftState *s, double *part, int nshift, hipStream_t st);
+// ---- multi-shift CG --------------------------------------------------------
+// r = b, p0 = 0, x_i = 0, p_i = 0, |b|^2 partials into rr[0].
+void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0, MultiShiftState *s,
+                    const CGPartials &P, int nshift, int rows, hipStream_t st);
 void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, double2 *p0, double2 *p1,
-                  double2 *t, double am0, MultiShiftState *s, double *part, int nshift, int rows,
-                  hipStream_t st);
-void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
-                  const double2 *p1, double2 *r, double am0, MultiShiftState *s, double *part,
+                  double2 *t, double am0, MultiShiftState *s, const CGPartials &P, int k,
                   int nshift, int rows, hipStream_t st);
-void launch_ms_final(const Latt &L, MultiShiftState *s, int nshift, hipStream_t st);
+void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
+                  const double2 *p1, double2 *r, double am0, MultiShiftState *s,
+                  const CGPartials &P, int k, int nshift, int rows, hipStream_t st);
 
 // Reductions (deterministic; results in out->v[]).
 void launch_norm2(const double2 *a, long n, RedOut *out, double *part, hipStream_t st);

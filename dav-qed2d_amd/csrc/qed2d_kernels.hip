@@ -6,7 +6,7 @@
 // vertical neighbours kept in registers by marching each thread up a column
 // of `R` rows, and reductions finished on the device by the last block to
 // arrive so that no kernel boundary needs a host round trip.
-#include "qed2d_internal.h"
+#include "qed2d_march.h"
 
 namespace qed2d {
 
@@ -27,93 +27,6 @@ MarchGrid march_grid(const Latt &L, int rows)
     int rowgroups = (L.L2 + rows - 1) / rows;
     g.nby = (rowgroups + g.by - 1) / g.by;
     return g;
-}
-
-struct MarchPos {
-    int i1, r0;
-    bool active;
-    int blk;  // logical block index (partial-sum slot)
-};
-
-__device__ __forceinline__ MarchPos march_pos(const Latt &L, int bx, int by, int nbx, int rows)
-{
-    MarchPos p;
-    const int nb = gridDim.x;
-    const int b = xcd_remap(blockIdx.x, nb);
-    const int bxi = b % nbx, byi = b / nbx;
-    const int tx = threadIdx.x % bx, ty = threadIdx.x / bx;
-    p.i1 = bxi * bx + tx;
-    p.r0 = (byi * by + ty) * rows;
-    p.active = (p.i1 < L.L1) && (p.r0 < L.L2);
-    p.blk = b;
-    return p;
-}
-
-// Spinor sources for the stencil.
-template <typename T> struct SrcPlain {
-    const c2<T> *__restrict__ s;
-    long V;
-    __device__ __forceinline__ void load(long i, c2<T> &a, c2<T> &b) const
-    {
-        a = s[i];
-        b = s[V + i];
-    }
-};
-
-// p_new = r + beta * p_old, evaluated where it is read (src/QED2dSolver.jl:35).
-template <typename T> struct SrcXpay {
-    const c2<T> *__restrict__ r;
-    const c2<T> *__restrict__ p;
-    T beta;
-    long V;
-    __device__ __forceinline__ void load(long i, c2<T> &a, c2<T> &b) const
-    {
-        c2<T> r0 = r[i], r1 = r[V + i], p0 = p[i], p1 = p[V + i];
-        a = mk<T>(r0.x + beta * p0.x, r0.y + beta * p0.y);
-        b = mk<T>(r1.x + beta * p1.x, r1.y + beta * p1.y);
-    }
-};
-
-// March one thread's column: for rows r0 .. r0+R-1 compute (Q s)(x) and hand
-// it to the epilogue together with s(x).
-template <typename T, int R, typename Src, typename Epi>
-__device__ __forceinline__ void q_march(const Latt &L, const c2<T> *__restrict__ U, T mass2,
-                                        const Src &src, Epi &epi, int i1, int r0)
-{
-    const int L1 = L.L1, L2 = L.L2;
-    const long V = L.V;
-    const int i1p = (i1 + 1 == L1) ? 0 : i1 + 1;
-    const int i1m = (i1 == 0) ? L1 - 1 : i1 - 1;
-    const bool ap = (L.bc & 1) != 0;
-    const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
-    c2<T> dn1, dn2, c1, cc2;
-    src.load((long)rm * L1 + i1, dn1, dn2);
-    src.load((long)r0 * L1 + i1, c1, cc2);
-    c2<T> U2m = U[V + (long)rm * L1 + i1];
-    if (ap && r0 == 0) U2m = cneg<T>(U2m);
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        const int r = r0 + k;
-        if (r >= L2) break;
-        const int rp = (r + 1 == L2) ? 0 : r + 1;
-        const long row = (long)r * L1;
-        const long s = row + i1;
-        c2<T> up1, up2, p11, p12, m11, m12;
-        src.load((long)rp * L1 + i1, up1, up2);
-        src.load(row + i1p, p11, p12);
-        src.load(row + i1m, m11, m12);
-        const c2<T> U1 = U[s], U1m = U[row + i1m], U2 = U[V + s];
-        const c2<T> U2e = (ap && r == L2 - 1) ? cneg<T>(U2) : U2;
-        c2<T> o1, o2;
-        q_site<T>(mass2, c1, cc2, p11, p12, m11, m12, up1, up2, dn1, dn2, U1, U1m, U2e, U2m, o1,
-                  o2);
-        epi(s, o1, o2, c1, cc2);
-        dn1 = c1;
-        dn2 = cc2;
-        c1 = up1;
-        cc2 = up2;
-        U2m = U2;
-    }
 }
 
 // ===========================================================================
@@ -170,14 +83,6 @@ __global__ void __launch_bounds__(kBlock) k_q_shift(Latt L, const double2 *__res
     q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
 }
 
-#define DISPATCH_ROWS(rows, KERNEL, ...)                                              \
-    switch (rows) {                                                                   \
-    case 1: hipLaunchKernelGGL((KERNEL<1>), __VA_ARGS__); break;                      \
-    case 2: hipLaunchKernelGGL((KERNEL<2>), __VA_ARGS__); break;                      \
-    case 4: hipLaunchKernelGGL((KERNEL<4>), __VA_ARGS__); break;                      \
-    case 8: hipLaunchKernelGGL((KERNEL<8>), __VA_ARGS__); break;                      \
-    default: hipLaunchKernelGGL((KERNEL<16>), __VA_ARGS__); break;                    \
-    }
 
 template <typename T>
 static void q_apply_impl(const Latt &L, const c2<T> *U, const c2<T> *si, c2<T> *so, T am0,
@@ -347,21 +252,13 @@ void launch_negate(double *a, long n, hipStream_t st)
     hipLaunchKernelGGL(k_negate, dim3(ew_blocks(n)), dim3(kBlock), 0, st, a, n);
 }
 
-__global__ void k_to_f32(const double2 *__restrict__ in, float2 *__restrict__ out, long n)
-{
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x) {
-        double2 v = in[i];
-        out[i] = mk<float>((float)v.x, (float)v.y);
-    }
-}
-void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_to_f32, dim3(ew_blocks(n)), dim3(kBlock), 0, st, in, out, n);
-}
 
 // ===========================================================================
-// Deterministic reductions: fixed grid, fixed tree, last block finishes.
+// Deterministic reductions without in-kernel grid synchronisation: a
+// producer kernel writes one partial per block, a one-block finishing kernel
+// sums them in block order.  (A grid-wide "last block" finish needs an
+// agent-scope release fence per block, which on a multi-XCD part writes back
+// every dirty L2 line.)
 // ===========================================================================
 constexpr int kRedBlocks = 1024;
 
@@ -373,12 +270,12 @@ static inline int red_blocks(long n)
     return (int)b;
 }
 
-// a^dag b (complex) or |a|^2 when b == nullptr.
+// a^dag b (complex) or |a|^2 when b == nullptr: partials (re, im) per block.
 __global__ void __launch_bounds__(kBlock) k_dot(const double2 *__restrict__ a,
                                                 const double2 *__restrict__ b, long n,
-                                                RedOut *out, double *part)
+                                                double *part)
 {
-    __shared__ double lds[kBlock / kWave];
+    __shared__ double lds[kBlock / kWave + 1];
     double sr = 0.0, si = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
          i += (long)gridDim.x * blockDim.x) {
@@ -391,30 +288,37 @@ __global__ void __launch_bounds__(kBlock) k_dot(const double2 *__restrict__ a,
             sr += x.x * x.x + x.y * x.y;
         }
     }
-    double br = block_sum(sr, lds);
-    double bi = block_sum(si, lds);
+    double br = block_allsum(sr, lds);
+    double bi = block_allsum(si, lds);
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = br;
         part[2 * blockIdx.x + 1] = bi;
     }
-    if (last_block_arrives<2>(&out->ticket[0], gridDim.x)) {
-        double res[2];
-        sum_partials<2>(part, gridDim.x, res, lds);
-        if (threadIdx.x == 0) {
-            out->v[0] = res[0];
-            out->v[1] = res[1];
-            out->ticket[0] = 0u;
-        }
+}
+
+// Sums nv interleaved plain partials: out->v[k] = sum_b part[b * nv + k].
+__global__ void __launch_bounds__(kBlock) k_finish_sum(const double *part, int nblocks, int nv,
+                                                       RedOut *out)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    for (int k = 0; k < nv; ++k) {
+        double s = sum_prev_partials(part, nblocks, nv, k, lds);
+        if (threadIdx.x == 0) out->v[k] = s;
     }
 }
+
 void launch_norm2(const double2 *a, long n, RedOut *out, double *part, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(kBlock), 0, st, a, nullptr, n, out, part);
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kBlock), 0, st, a, nullptr, n, part);
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBlock), 0, st, part, nb, 2, out);
 }
 void launch_dot(const double2 *a, const double2 *b, long n, RedOut *out, double *part,
                 hipStream_t st)
 {
-    hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(kBlock), 0, st, a, b, n, out, part);
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kBlock), 0, st, a, b, n, part);
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBlock), 0, st, part, nb, 2, out);
 }
 
 // Block-wide double-double sum; result in thread 0.
@@ -452,13 +356,15 @@ __device__ __forceinline__ double2 plaq_at(const double2 *__restrict__ U, const 
     return cmul<double>(a, cconj<double>(b));
 }
 
+
+
 // Sum Re P, sum arg P (src/QED2dAction.jl:2-32) and sum p^2 (src/QED2dHMC.jl:200),
-// each as a double-double; optional per-site density outputs.
+// each as a double-double, per block; optional per-site density outputs.
 __global__ void __launch_bounds__(kBlock) k_observables(Latt L, const double2 *__restrict__ U,
                                                         const double *__restrict__ mom,
                                                         double *__restrict__ plx,
                                                         double *__restrict__ qdens,
-                                                        RedOut *out, double *part)
+                                                        double *part)
 {
     __shared__ double lds[2 * (kBlock / kWave)];
     dd sp = {0.0, 0.0}, sq = {0.0, 0.0}, sm = {0.0, 0.0};
@@ -484,718 +390,34 @@ __global__ void __launch_bounds__(kBlock) k_observables(Latt L, const double2 *_
         double *pp = part + 6 * (long)blockIdx.x;
         pp[0] = bp.hi; pp[1] = bp.lo; pp[2] = bq.hi; pp[3] = bq.lo; pp[4] = bm.hi; pp[5] = bm.lo;
     }
-    if (last_block_arrives<6>(&out->ticket[0], gridDim.x)) {
-        for (int k = 0; k < 3; ++k) {
-            dd acc = {0.0, 0.0};
-            for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-                dd t = {__builtin_nontemporal_load(&part[6 * (long)b + 2 * k]),
-                        __builtin_nontemporal_load(&part[6 * (long)b + 2 * k + 1])};
-                acc = dd_add(acc, t);
-            }
-            acc = block_sum_dd(acc, lds);
-            if (threadIdx.x == 0) {
-                out->v[2 * k] = acc.hi;
-                out->v[2 * k + 1] = acc.lo;
-            }
+}
+
+// Double-double finish of nq (hi, lo) partial pairs: out->v[2k], v[2k+1].
+__global__ void __launch_bounds__(kBlock) k_finish_dd(const double *part, int nblocks, int nq,
+                                                      RedOut *out)
+{
+    __shared__ double lds[2 * (kBlock / kWave)];
+    for (int k = 0; k < nq; ++k) {
+        dd acc = {0.0, 0.0};
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+            dd t = {part[(long)b * 2 * nq + 2 * k], part[(long)b * 2 * nq + 2 * k + 1]};
+            acc = dd_add(acc, t);
         }
-        if (threadIdx.x == 0) out->ticket[0] = 0u;
+        acc = block_sum_dd(acc, lds);
+        if (threadIdx.x == 0) {
+            out->v[2 * k] = acc.hi;
+            out->v[2 * k + 1] = acc.lo;
+        }
     }
 }
+
 void launch_observables(const Latt &L, const double2 *U, const double *mom, double *plx,
                         double *qdens, RedOut *out, double *part, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_observables, dim3(red_blocks(L.V)), dim3(kBlock), 0, st, L, U, mom, plx,
-                       qdens, out, part);
+    const int nb = red_blocks(L.V);
+    hipLaunchKernelGGL(k_observables, dim3(nb), dim3(kBlock), 0, st, L, U, mom, plx, qdens, part);
+    hipLaunchKernelGGL(k_finish_dd, dim3(1), dim3(kBlock), 0, st, part, nb, 3, out);
 }
-
-// ===========================================================================
-// Conjugate gradient on Q^2 + mu^2 (src/QED2dSolver.jl:3-105), restructured
-// into two fused kernels per iteration:
-//   K1: p = r + beta p_old (on the fly), t = Q p, |t|^2, |p|^2
-//   K2: Ap = Q t + mu^2 p, alpha = |r|^2 / (|t|^2 + mu^2 |p|^2),
-//       x += alpha p, r -= alpha Ap, |r|^2, convergence test
-// p^dag (Q^2 + mu^2) p = |Q p|^2 + mu^2 |p|^2 because Q is Hermitian, so the
-// reference's dot(p, Ap) needs no separate pass.  The scalars live in
-// SolverState; converged solves turn the remaining queued kernels into no-ops.
-// ===========================================================================
-template <typename T> struct EpiK1 {
-    c2<T> *__restrict__ pnew;
-    c2<T> *__restrict__ t;
-    long V;
-    double tt, pp;
-    __device__ __forceinline__ void operator()(long s, c2<T> o1, c2<T> o2, c2<T> c1, c2<T> cc2)
-    {
-        pnew[s] = c1;
-        pnew[V + s] = cc2;
-        t[s] = o1;
-        t[V + s] = o2;
-        tt += (double)(o1.x * o1.x + o1.y * o1.y) + (double)(o2.x * o2.x + o2.y * o2.y);
-        pp += (double)(c1.x * c1.x + c1.y * c1.y) + (double)(cc2.x * cc2.x + cc2.y * cc2.y);
-    }
-};
-
-// The two search-direction buffers alternate with the device's iteration
-// count: iteration k = iter + 1 reads P[iter & 1] and writes P[(iter + 1) & 1].
-// Selecting them on the device keeps the pairing right when queued
-// iterations are skipped (converged, or waiting for a reliable update).
-template <typename T, int R>
-__global__ void __launch_bounds__(kBlock) k_cg1(Latt L, const c2<T> *__restrict__ U,
-                                                const c2<T> *__restrict__ r, c2<T> *P0,
-                                                c2<T> *P1, c2<T> *__restrict__ t, T mass2,
-                                                SolverState *S, double *part, int bx, int by,
-                                                int nbx)
-{
-    __shared__ double lds[kBlock / kWave];
-    if (S->status != 0 || S->need_ru != 0) return;
-    const bool odd = (S->iter & 1) != 0;
-    const c2<T> *pold = odd ? P1 : P0;
-    c2<T> *pnew = odd ? P0 : P1;
-    MarchPos p = march_pos(L, bx, by, nbx, R);
-    EpiK1<T> epi{pnew, t, L.V, 0.0, 0.0};
-    if (p.active) {
-        SrcXpay<T> src{r, pold, (T)S->beta, L.V};
-        q_march<T, R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
-    double a = block_sum(epi.tt, lds);
-    double b = block_sum(epi.pp, lds);
-    if (threadIdx.x == 0) {
-        part[2 * p.blk] = a;
-        part[2 * p.blk + 1] = b;
-    }
-    if (last_block_arrives<2>(&S->ticket[0], gridDim.x)) {
-        double res[2];
-        sum_partials<2>(part, gridDim.x, res, lds);
-        if (threadIdx.x == 0) {
-            S->tt = res[0];
-            S->pp = res[1];
-            S->ticket[0] = 0u;
-        }
-    }
-}
-
-template <typename T> struct EpiK2 {
-    const c2<T> *__restrict__ p;
-    c2<T> *__restrict__ x;
-    c2<T> *__restrict__ r;
-    T alpha, mu2;
-    long V;
-    double rr;
-    __device__ __forceinline__ void operator()(long s, c2<T> o1, c2<T> o2, c2<T>, c2<T>)
-    {
-        c2<T> p1 = p[s], p2 = p[V + s];
-        c2<T> a1 = mk<T>(o1.x + mu2 * p1.x, o1.y + mu2 * p1.y);
-        c2<T> a2 = mk<T>(o2.x + mu2 * p2.x, o2.y + mu2 * p2.y);
-        c2<T> x1 = x[s], x2 = x[V + s], r1 = r[s], r2 = r[V + s];
-        x1 = mk<T>(x1.x + alpha * p1.x, x1.y + alpha * p1.y);
-        x2 = mk<T>(x2.x + alpha * p2.x, x2.y + alpha * p2.y);
-        r1 = mk<T>(r1.x - alpha * a1.x, r1.y - alpha * a1.y);
-        r2 = mk<T>(r2.x - alpha * a2.x, r2.y - alpha * a2.y);
-        x[s] = x1;
-        x[V + s] = x2;
-        r[s] = r1;
-        r[V + s] = r2;
-        rr += (double)(r1.x * r1.x + r1.y * r1.y) + (double)(r2.x * r2.x + r2.y * r2.y);
-    }
-};
-
-template <typename T, int R>
-__global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict__ U,
-                                                const c2<T> *__restrict__ t, const c2<T> *P0,
-                                                const c2<T> *P1, c2<T> *__restrict__ x,
-                                                c2<T> *__restrict__ r, T mass2, SolverState *S,
-                                                double *part, int bx, int by, int nbx, int mixed)
-{
-    __shared__ double lds[kBlock / kWave];
-    if (S->status != 0 || S->need_ru != 0) return;
-    const c2<T> *pv = (S->iter & 1) ? P0 : P1;
-    MarchPos p = march_pos(L, bx, by, nbx, R);
-    const double alpha = S->norm / (S->tt + S->mu2 * S->pp);
-    EpiK2<T> epi{pv, x, r, (T)alpha, (T)S->mu2, L.V, 0.0};
-    if (p.active) {
-        SrcPlain<T> src{t, L.V};
-        q_march<T, R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
-    double a = block_sum(epi.rr, lds);
-    if (threadIdx.x == 0) part[p.blk] = a;
-    if (last_block_arrives<1>(&S->ticket[1], gridDim.x)) {
-        double res[1];
-        sum_partials<1>(part, gridDim.x, res, lds);
-        if (threadIdx.x == 0) {
-            const double err = res[0];
-            S->err = err;
-            S->iter += 1;
-            if (!mixed) {
-                // src/QED2dSolver.jl:27-37
-                if (err < S->tol) {
-                    S->status = 1;
-                } else {
-                    S->beta = err / S->norm;
-                    S->norm = err;
-                    if (S->iter >= S->maxiter) S->status = 2;
-                }
-            } else {
-                // FP32 iteration: ask for a reliable update once the iterated
-                // residual has dropped by delta since the last one, or looks
-                // converged.
-                if (err < S->tol || err < S->delta2 * S->rmax2 || S->iter >= S->maxiter) {
-                    S->need_ru = 1;
-                } else {
-                    S->beta = err / S->norm;
-                    S->norm = err;
-                    if (err > S->rmax2) S->rmax2 = err;
-                }
-            }
-            S->ticket[1] = 0u;
-        }
-    }
-}
-
-template <typename T>
-static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *pold, c2<T> *pnew,
-                     c2<T> *t, T am0, SolverState *S, double *part, int rows, hipStream_t st)
-{
-    MarchGrid g = march_grid(L, rows);
-    dim3 grid(g.nblocks()), blk(kBlock);
-    const T m2 = T(2) + am0;
-    switch (rows) {
-    case 1: hipLaunchKernelGGL((k_cg1<T, 1>), grid, blk, 0, st, L, U, r, pold, pnew, t, m2, S, part, g.bx, g.by, g.nbx); break;
-    case 2: hipLaunchKernelGGL((k_cg1<T, 2>), grid, blk, 0, st, L, U, r, pold, pnew, t, m2, S, part, g.bx, g.by, g.nbx); break;
-    case 4: hipLaunchKernelGGL((k_cg1<T, 4>), grid, blk, 0, st, L, U, r, pold, pnew, t, m2, S, part, g.bx, g.by, g.nbx); break;
-    case 8: hipLaunchKernelGGL((k_cg1<T, 8>), grid, blk, 0, st, L, U, r, pold, pnew, t, m2, S, part, g.bx, g.by, g.nbx); break;
-    default: hipLaunchKernelGGL((k_cg1<T, 16>), grid, blk, 0, st, L, U, r, pold, pnew, t, m2, S, part, g.bx, g.by, g.nbx); break;
-    }
-}
-
-template <typename T>
-static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> *p0,
-                     const c2<T> *p1, c2<T> *x, c2<T> *r, T am0, SolverState *S, double *part,
-                     int rows, int mixed, hipStream_t st)
-{
-    MarchGrid g = march_grid(L, rows);
-    dim3 grid(g.nblocks()), blk(kBlock);
-    const T m2 = T(2) + am0;
-    switch (rows) {
-    case 1: hipLaunchKernelGGL((k_cg2<T, 1>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
-    case 2: hipLaunchKernelGGL((k_cg2<T, 2>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
-    case 4: hipLaunchKernelGGL((k_cg2<T, 4>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
-    case 8: hipLaunchKernelGGL((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
-    default: hipLaunchKernelGGL((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, part, g.bx, g.by, g.nbx, mixed); break;
-    }
-}
-
-void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2 *p0,
-                      double2 *p1, double2 *t, double am0, SolverState *s, double *part,
-                      int rows, hipStream_t st)
-{
-    cg1_impl<double>(L, U, r, p0, p1, t, am0, s, part, rows, st);
-}
-void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
-                      const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
-                      double *part, int rows, int mixed, hipStream_t st)
-{
-    cg2_impl<double>(L, U, t, p0, p1, x, r, am0, s, part, rows, mixed, st);
-}
-void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
-                      float2 *t, float am0, SolverState *s, double *part, int rows,
-                      hipStream_t st)
-{
-    cg1_impl<float>(L, U, r, p0, p1, t, am0, s, part, rows, st);
-}
-void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
-                      const float2 *p1, float2 *x, float2 *r, float am0, SolverState *s,
-                      double *part, int rows, int mixed, hipStream_t st)
-{
-    cg2_impl<float>(L, U, t, p0, p1, x, r, am0, s, part, rows, mixed, st);
-}
-
-// x = 0, r = b, p0 = 0, |b|^2 -> norm, bnorm, tol  (src/QED2dSolver.jl:10-16)
-__global__ void __launch_bounds__(kBlock) k_cg_init(const double2 *__restrict__ b,
-                                                    double2 *__restrict__ x,
-                                                    double2 *__restrict__ r,
-                                                    double2 *__restrict__ p0, long n,
-                                                    SolverState *S, double *part)
-{
-    __shared__ double lds[kBlock / kWave];
-    double acc = 0.0;
-    const double2 z = mk<double>(0.0, 0.0);
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x) {
-        double2 v = b[i];
-        x[i] = z;
-        r[i] = v;
-        p0[i] = z;
-        acc += v.x * v.x + v.y * v.y;
-    }
-    double a = block_sum(acc, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = a;
-    if (last_block_arrives<1>(&S->ticket[2], gridDim.x)) {
-        double res[1];
-        sum_partials<1>(part, gridDim.x, res, lds);
-        if (threadIdx.x == 0) {
-            S->bnorm = res[0];
-            S->norm = res[0];
-            S->tol = S->eps * res[0];
-            S->rmax2 = res[0];
-            S->beta = 0.0;
-            S->iter = 0;
-            S->need_ru = 0;
-            S->n_ru = 0;
-            // b = 0 has the exact solution x = 0.
-            S->status = (res[0] == 0.0) ? 1 : 0;
-            S->ticket[2] = 0u;
-        }
-    }
-}
-void launch_cg_init_f64(const Latt &L, const double2 *b, double2 *x, double2 *r, double2 *p0,
-                        SolverState *s, double *part, hipStream_t st)
-{
-    const long n = 2 * L.V;
-    hipLaunchKernelGGL(k_cg_init, dim3(red_blocks(n)), dim3(kBlock), 0, st, b, x, r, p0, n, s,
-                       part);
-}
-
-// Residual of a starting guess: r = b - (Q t + mu^2 x), t = Q x
-// (src/QED2dSolver.jl:65-72).  mode 0: FP64 r; mode 1 (mixed, initial) and
-// mode 2 (mixed, reliable update) write the FP32 residual instead.
-struct EpiResid {
-    const double2 *__restrict__ b;
-    const double2 *__restrict__ x;
-    double2 *__restrict__ r;
-    float2 *__restrict__ rs;
-    double2 *__restrict__ p0;
-    float2 *__restrict__ p0s;
-    double mu2;
-    long V;
-    double rr, bb;
-    __device__ __forceinline__ void one(long i, double2 o)
-    {
-        double2 bv = b[i], xv = x[i];
-        double2 rv = mk<double>(bv.x - (o.x + mu2 * xv.x), bv.y - (o.y + mu2 * xv.y));
-        if (r) r[i] = rv;
-        if (rs) rs[i] = mk<float>((float)rv.x, (float)rv.y);
-        if (p0) p0[i] = mk<double>(0.0, 0.0);
-        if (p0s) p0s[i] = mk<float>(0.0f, 0.0f);
-        rr += rv.x * rv.x + rv.y * rv.y;
-        bb += bv.x * bv.x + bv.y * bv.y;
-    }
-    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2, double2)
-    {
-        one(s, o1);
-        one(V + s, o2);
-    }
-};
-
-template <int R>
-__global__ void __launch_bounds__(kBlock) k_resid(Latt L, const double2 *__restrict__ U,
-                                                  const double2 *__restrict__ t,
-                                                  EpiResid epi0, double mass2, SolverState *S,
-                                                  double *part, int bx, int by, int nbx,
-                                                  int mode)
-{
-    __shared__ double lds[kBlock / kWave];
-    if (mode == 2 && (S->status != 0 || S->need_ru == 0)) return;
-    MarchPos p = march_pos(L, bx, by, nbx, R);
-    EpiResid epi = epi0;
-    epi.mu2 = S->mu2;
-    epi.rr = 0.0;
-    epi.bb = 0.0;
-    if (p.active) {
-        SrcPlain<double> src{t, L.V};
-        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
-    double a = block_sum(epi.rr, lds);
-    double b = block_sum(epi.bb, lds);
-    if (threadIdx.x == 0) {
-        part[2 * p.blk] = a;
-        part[2 * p.blk + 1] = b;
-    }
-    if (last_block_arrives<2>(&S->ticket[3], gridDim.x)) {
-        double res[2];
-        sum_partials<2>(part, gridDim.x, res, lds);
-        if (threadIdx.x == 0) {
-            const double r2 = res[0];
-            if (mode != 2) {
-                S->bnorm = res[1];
-                S->tol = S->eps * res[1];
-                S->norm = r2;
-                S->rmax2 = r2;
-                S->beta = 0.0;
-                S->iter = 0;
-                S->need_ru = 0;
-                S->n_ru = 0;
-                S->status = (r2 < S->tol || res[1] == 0.0) ? 1 : 0;
-            } else {
-                S->n_ru += 1;
-                S->need_ru = 0;
-                if (r2 < S->tol) {
-                    S->status = 1;
-                } else {
-                    S->beta = r2 / S->norm;
-                    S->norm = r2;
-                    S->rmax2 = r2;
-                    if (S->iter >= S->maxiter) S->status = 2;
-                }
-            }
-            S->err = r2;
-            S->ticket[3] = 0u;
-        }
-    }
-}
-
-static void resid_impl(const Latt &L, const double2 *U, const double2 *t, EpiResid e,
-                       double am0, SolverState *S, double *part, int rows, int mode,
-                       hipStream_t st)
-{
-    MarchGrid g = march_grid(L, rows);
-    dim3 grid(g.nblocks()), blk(kBlock);
-    DISPATCH_ROWS(rows, k_resid, grid, blk, 0, st, L, U, t, e, 2.0 + am0, S, part, g.bx, g.by,
-                  g.nbx, mode);
-}
-
-void launch_cg_guess_f64(const Latt &L, const double2 *U, const double2 *b, const double2 *x,
-                         const double2 *t, double2 *r, double2 *p0, double am0, SolverState *s,
-                         double *part, int rows, hipStream_t st)
-{
-    EpiResid e{b, x, r, nullptr, p0, nullptr, 0.0, L.V, 0.0, 0.0};
-    resid_impl(L, U, t, e, am0, s, part, rows, 0, st);
-}
-
-void launch_ru_residual(const Latt &L, const double2 *U, const double2 *b, const double2 *t,
-                        const double2 *x, float2 *rs, float2 *p0, double am0, SolverState *s,
-                        double *part, int rows, int initial, hipStream_t st)
-{
-    EpiResid e{b, x, nullptr, rs, nullptr, initial ? p0 : nullptr, 0.0, L.V, 0.0, 0.0};
-    resid_impl(L, U, t, e, am0, s, part, rows, initial ? 1 : 2, st);
-}
-
-// Mixed-precision start from x = 0: rs = float(b), xs = 0, p0 = 0.
-__global__ void __launch_bounds__(kBlock) k_cg_init_mixed(const double2 *__restrict__ b,
-                                                          float2 *__restrict__ rs,
-                                                          float2 *__restrict__ xs,
-                                                          float2 *__restrict__ p0, long n,
-                                                          SolverState *S, double *part)
-{
-    __shared__ double lds[kBlock / kWave];
-    double acc = 0.0;
-    const float2 z = mk<float>(0.0f, 0.0f);
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x) {
-        double2 v = b[i];
-        rs[i] = mk<float>((float)v.x, (float)v.y);
-        xs[i] = z;
-        p0[i] = z;
-        acc += v.x * v.x + v.y * v.y;
-    }
-    double a = block_sum(acc, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = a;
-    if (last_block_arrives<1>(&S->ticket[2], gridDim.x)) {
-        double res[1];
-        sum_partials<1>(part, gridDim.x, res, lds);
-        if (threadIdx.x == 0) {
-            S->bnorm = res[0];
-            S->norm = res[0];
-            S->tol = S->eps * res[0];
-            S->rmax2 = res[0];
-            S->beta = 0.0;
-            S->iter = 0;
-            S->need_ru = 0;
-            S->n_ru = 0;
-            S->status = (res[0] == 0.0) ? 1 : 0;
-            S->ticket[2] = 0u;
-        }
-    }
-}
-void launch_cg_init_mixed(const Latt &L, const double2 *b, float2 *rs, float2 *xs, float2 *p0,
-                          SolverState *s, double *part, hipStream_t st)
-{
-    const long n = 2 * L.V;
-    hipLaunchKernelGGL(k_cg_init_mixed, dim3(red_blocks(n)), dim3(kBlock), 0, st, b, rs, xs, p0,
-                       n, s, part);
-}
-
-// x += double(xs); xs = 0  (a reliable update folds the FP32 partial solution
-// into the FP64 one).
-__global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs, long n,
-                           const SolverState *S)
-{
-    if (S->need_ru == 0 || S->status != 0) return;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x) {
-        float2 d = xs[i];
-        double2 v = x[i];
-        x[i] = mk<double>(v.x + (double)d.x, v.y + (double)d.y);
-        xs[i] = mk<float>(0.0f, 0.0f);
-    }
-}
-void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const SolverState *s,
-                     hipStream_t st)
-{
-    hipLaunchKernelGGL(k_ru_accum, dim3(ew_blocks(2 * L.V)), dim3(kBlock), 0, st, x, xs, 2 * L.V,
-                       s);
-}
-
-// ===========================================================================
-// Multi-shift CG (Jegerlehner, hep-lat/9612014): the systems
-// (Q^2 + sigma_i) x_i = b share the Krylov space of the base system
-// (Q^2 + sigma_b), whose residual r_k fixes every shifted residual up to a
-// scalar, r^i_k = zeta^i_k r_k.  Per iteration k:
-//   K1: p_k = r_k + beta_k p_{k-1}, t = Q p_k, |t|^2, |p_k|^2, and for each
-//       shift x_i += alpha^i_{k-1} p^i_{k-1}, p^i_k = zeta^i_k r_k + beta^i_k p^i_{k-1}
-//   K2: Ap = Q t + sigma_b p_k, alpha_k = |r_k|^2 / p_k^dag Ap, r_{k+1} = r_k - alpha_k Ap,
-//       |r_{k+1}|^2, then (last block) the zeta / alpha / beta recurrences:
-//   zeta_{k+1} = zeta_k zeta_{k-1} alpha_{k-1} /
-//                (alpha_k beta_k (zeta_{k-1} - zeta_k) + zeta_{k-1} alpha_{k-1} (1 + s alpha_k))
-//   alpha^i_k = alpha_k zeta_{k+1} / zeta_k,   beta^i_{k+1} = beta_{k+1} (zeta_{k+1} / zeta_k)^2
-// with s = sigma_i - sigma_b.  A shift stops once zeta^2 |r|^2 < tol.
-// Algorithmic bytes per site and iteration (FP64): 320 + 128 n_active.
-// ===========================================================================
-struct EpiMS1 {
-    c2<double> *__restrict__ pnew;
-    c2<double> *__restrict__ t;
-    const c2<double> *__restrict__ r;
-    const MultiShiftState *S;
-    int nshift;
-    long V;
-    double tt, pp;
-    __device__ __forceinline__ void one(long i, double2 rv)
-    {
-        for (int k = 0; k < nshift; ++k) {
-            const int a = S->active[k];
-            if (a == 0) continue;
-            double2 *ps = S->ps[k];
-            double2 *x = S->x[k];
-            const double2 pv = ps[i];
-            const double as = S->alpha_s[k];
-            double2 xv = x[i];
-            x[i] = mk<double>(xv.x + as * pv.x, xv.y + as * pv.y);
-            if (a == 1) {
-                const double z = S->zeta[k], bs = S->beta_s[k];
-                ps[i] = mk<double>(z * rv.x + bs * pv.x, z * rv.y + bs * pv.y);
-            }
-        }
-    }
-    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2 c1,
-                                               double2 cc2)
-    {
-        pnew[s] = c1;
-        pnew[V + s] = cc2;
-        t[s] = o1;
-        t[V + s] = o2;
-        tt += (o1.x * o1.x + o1.y * o1.y) + (o2.x * o2.x + o2.y * o2.y);
-        pp += (c1.x * c1.x + c1.y * c1.y) + (cc2.x * cc2.x + cc2.y * cc2.y);
-        one(s, r[s]);
-        one(V + s, r[V + s]);
-    }
-};
-
-template <int R>
-__global__ void __launch_bounds__(kBlock) k_ms1(Latt L, const double2 *__restrict__ U,
-                                                const double2 *__restrict__ r, double2 *P0,
-                                                double2 *P1, double2 *__restrict__ t,
-                                                double mass2, MultiShiftState *S, double *part,
-                                                int nshift, int bx, int by, int nbx)
-{
-    __shared__ double lds[kBlock / kWave];
-    if (S->status != 0) return;
-    const bool odd = (S->iter & 1) != 0;
-    const double2 *pold = odd ? P1 : P0;
-    double2 *pnew = odd ? P0 : P1;
-    MarchPos p = march_pos(L, bx, by, nbx, R);
-    EpiMS1 epi{pnew, t, r, S, nshift, L.V, 0.0, 0.0};
-    if (p.active) {
-        SrcXpay<double> src{r, pold, S->beta, L.V};
-        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
-    double a = block_sum(epi.tt, lds);
-    double b = block_sum(epi.pp, lds);
-    if (threadIdx.x == 0) {
-        part[2 * p.blk] = a;
-        part[2 * p.blk + 1] = b;
-    }
-    if (last_block_arrives<2>(&S->ticket[0], gridDim.x)) {
-        double res[2];
-        sum_partials<2>(part, gridDim.x, res, lds);
-        if (threadIdx.x == 0) {
-            S->tt = res[0];
-            S->pp = res[1];
-            for (int k = 0; k < nshift; ++k) {
-                if (S->active[k] == 2) S->active[k] = 0;  // last x update done
-                S->alpha_s[k] = 0.0;
-            }
-            S->ticket[0] = 0u;
-        }
-    }
-}
-
-struct EpiMS2 {
-    double2 *__restrict__ r;
-    const double2 *__restrict__ p;
-    double alpha, sigma;
-    long V;
-    double rr;
-    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2, double2)
-    {
-        const double2 p1 = p[s], p2 = p[V + s];
-        double2 r1 = r[s], r2 = r[V + s];
-        r1 = mk<double>(r1.x - alpha * (o1.x + sigma * p1.x), r1.y - alpha * (o1.y + sigma * p1.y));
-        r2 = mk<double>(r2.x - alpha * (o2.x + sigma * p2.x), r2.y - alpha * (o2.y + sigma * p2.y));
-        r[s] = r1;
-        r[V + s] = r2;
-        rr += (r1.x * r1.x + r1.y * r1.y) + (r2.x * r2.x + r2.y * r2.y);
-    }
-};
-
-template <int R>
-__global__ void __launch_bounds__(kBlock) k_ms2(Latt L, const double2 *__restrict__ U,
-                                                const double2 *__restrict__ t, const double2 *P0,
-                                                const double2 *P1, double2 *__restrict__ r,
-                                                double mass2, MultiShiftState *S, double *part,
-                                                int nshift, int bx, int by, int nbx)
-{
-    __shared__ double lds[kBlock / kWave];
-    if (S->status != 0) return;
-    const double2 *pv = (S->iter & 1) ? P0 : P1;
-    MarchPos p = march_pos(L, bx, by, nbx, R);
-    const double alpha = S->rr / (S->tt + S->sigma_b * S->pp);
-    EpiMS2 epi{r, pv, alpha, S->sigma_b, L.V, 0.0};
-    if (p.active) {
-        SrcPlain<double> src{t, L.V};
-        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
-    double a = block_sum(epi.rr, lds);
-    if (threadIdx.x == 0) part[p.blk] = a;
-    if (last_block_arrives<1>(&S->ticket[1], gridDim.x)) {
-        double res[1];
-        sum_partials<1>(part, gridDim.x, res, lds);
-        if (threadIdx.x == 0) {
-            const double rr_new = res[0];
-            const double beta_k = S->beta, alpha_p = S->alpha_prev;
-            const double beta_n = rr_new / S->rr;
-            int running = 0;
-            for (int k = 0; k < nshift; ++k) {
-                if (S->active[k] != 1) continue;
-                const double z = S->zeta[k], zp = S->zeta_prev[k];
-                const double zn = z * zp * alpha_p /
-                                  (alpha * beta_k * (zp - z) + zp * alpha_p * (1.0 + S->s[k] * alpha));
-                S->alpha_s[k] = alpha * zn / z;
-                S->beta_s[k] = beta_n * (zn / z) * (zn / z);
-                S->zeta_prev[k] = z;
-                S->zeta[k] = zn;
-                if (zn * zn * rr_new < S->tol) {
-                    S->active[k] = 2;
-                    S->conv_iter[k] = S->iter + 1;
-                } else {
-                    running = 1;
-                }
-            }
-            S->alpha_prev = alpha;
-            S->beta = beta_n;
-            S->rr = rr_new;
-            S->iter += 1;
-            if (!running) S->status = 1;
-            else if (S->iter >= S->maxiter) S->status = 2;
-            S->ticket[1] = 0u;
-        }
-    }
-}
-
-// Pending x updates of the shifts that converged in the last iteration.
-__global__ void k_ms_final(MultiShiftState *S, int nshift, long n)
-{
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x) {
-        for (int k = 0; k < nshift; ++k) {
-            if (S->active[k] != 2) continue;
-            const double as = S->alpha_s[k];
-            const double2 pv = S->ps[k][i];
-            double2 xv = S->x[k][i];
-            S->x[k][i] = mk<double>(xv.x + as * pv.x, xv.y + as * pv.y);
-        }
-    }
-}
-
-__global__ void __launch_bounds__(kBlock) k_ms_init(const double2 *__restrict__ b,
-                                                    double2 *__restrict__ r,
-                                                    double2 *__restrict__ p0, long n,
-                                                    MultiShiftState *S, double *part,
-                                                    int nshift)
-{
-    __shared__ double lds[kBlock / kWave];
-    double acc = 0.0;
-    const double2 z = mk<double>(0.0, 0.0);
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x) {
-        const double2 v = b[i];
-        r[i] = v;
-        p0[i] = z;
-        for (int k = 0; k < nshift; ++k) {
-            S->x[k][i] = z;
-            S->ps[k][i] = z;
-        }
-        acc += v.x * v.x + v.y * v.y;
-    }
-    double a = block_sum(acc, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = a;
-    if (last_block_arrives<1>(&S->ticket[2], gridDim.x)) {
-        double res[1];
-        sum_partials<1>(part, gridDim.x, res, lds);
-        if (threadIdx.x == 0) {
-            S->bnorm = res[0];
-            S->rr = res[0];
-            S->tol = S->eps * res[0];
-            S->alpha_prev = 1.0;
-            S->beta = 0.0;
-            S->iter = 0;
-            for (int k = 0; k < nshift; ++k) {
-                S->zeta[k] = 1.0;
-                S->zeta_prev[k] = 1.0;
-                S->alpha_s[k] = 0.0;
-                S->beta_s[k] = 0.0;
-                S->active[k] = 1;
-                S->conv_iter[k] = 0;
-            }
-            S->status = (res[0] == 0.0) ? 1 : 0;
-            if (res[0] == 0.0)
-                for (int k = 0; k < nshift; ++k) S->active[k] = 0;
-            S->ticket[2] = 0u;
-        }
-    }
-}
-
-void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0,
-                    MultiShiftState *s, double *part, int nshift, hipStream_t st)
-{
-    const long n = 2 * L.V;
-    hipLaunchKernelGGL(k_ms_init, dim3(red_blocks(n)), dim3(kBlock), 0, st, b, r, p0, n, s, part,
-                       nshift);
-}
-
-void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, double2 *p0, double2 *p1,
-                  double2 *t, double am0, MultiShiftState *s, double *part, int nshift, int rows,
-                  hipStream_t st)
-{
-    MarchGrid g = march_grid(L, rows);
-    dim3 grid(g.nblocks()), blk(kBlock);
-    DISPATCH_ROWS(rows, k_ms1, grid, blk, 0, st, L, U, r, p0, p1, t, 2.0 + am0, s, part, nshift,
-                  g.bx, g.by, g.nbx);
-}
-
-void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
-                  const double2 *p1, double2 *r, double am0, MultiShiftState *s, double *part,
-                  int nshift, int rows, hipStream_t st)
-{
-    MarchGrid g = march_grid(L, rows);
-    dim3 grid(g.nblocks()), blk(kBlock);
-    DISPATCH_ROWS(rows, k_ms2, grid, blk, 0, st, L, U, t, p0, p1, r, 2.0 + am0, s, part, nshift,
-                  g.bx, g.by, g.nbx);
-}
-
-void launch_ms_final(const Latt &L, MultiShiftState *s, int nshift, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_ms_final, dim3(ew_blocks(2 * L.V)), dim3(kBlock), 0, st, s, nshift,
-                       2 * L.V);
-}
-
 // ===========================================================================
 // Forces and link updates
 // ===========================================================================

@@ -1,0 +1,687 @@
+// qed2d_solvers.hip -- CG, mixed-precision CG and multi-shift CG on
+// Q^2 + mu^2 (reference src/QED2dSolver.jl, src/QED2dRHMC.jl:261-290).
+//
+// One CG iteration k is two kernels, each a single pass of the marching
+// Wilson stencil with the vector algebra fused into it:
+//   K1(k): sum the |r_{k-1}|^2 partials of K2(k-1), convergence test,
+//          p_k = r + beta p_{k-1} evaluated where it is read, t = Q p_k,
+//          partials of |t|^2 and |p_k|^2
+//   K2(k): alpha = |r_{k-1}|^2 / (|t|^2 + mu^2 |p_k|^2), Ap = Q t + mu^2 p_k,
+//          x += alpha p_k, r -= alpha Ap, partials of |r_k|^2
+// p^dag (Q^2 + mu^2) p = |Q p|^2 + mu^2 |p|^2 because Q is Hermitian, so the
+// reference's dot(p, Ap) (src/QED2dSolver.jl:21) needs no pass of its own.
+//
+// Scalars never cross the grid inside a kernel: each kernel sums the
+// previous kernel's per-block partials in block order (identical bits in
+// every block), and block 0 of K1(k) commits what later kernels need into
+// slot k & 1 of the state.  The search direction ping-pongs between two
+// buffers (K1(k) reads P[(k-1) & 1], writes P[k & 1]) so the neighbours a
+// block reads are never overwritten during the kernel.
+#include <type_traits>
+
+#include "qed2d_march.h"
+
+namespace qed2d {
+
+static inline int ew_grid(long n)
+{
+    long b = (n + kBlock - 1) / kBlock;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+template <typename T> __device__ __forceinline__ c2<T> from_d2(double2 v)
+{
+    return mk<T>((T)v.x, (T)v.y);
+}
+
+// ===========================================================================
+// Start of a solve from x = 0 (src/QED2dSolver.jl:10-16)
+// ===========================================================================
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_cg_init(Latt L, const double2 *__restrict__ b,
+                                                    double2 *__restrict__ x,
+                                                    c2<T> *__restrict__ r,
+                                                    c2<T> *__restrict__ p0,
+                                                    float2 *__restrict__ xs, double *Prr,
+                                                    double *Pb, int bx, int by, int nbx,
+                                                    int rows)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    MarchPos p = march_pos(L, bx, by, nbx, rows);
+    double acc = 0.0;
+    if (p.active) {
+        for (int kk = 0; kk < rows; ++kk) {
+            const int row = p.r0 + kk;
+            if (row >= L.L2) break;
+            for (int c = 0; c < 2; ++c) {
+                const long i = c * L.V + (long)row * L.L1 + p.i1;
+                const double2 v = b[i];
+                x[i] = mk<double>(0.0, 0.0);
+                r[i] = from_d2<T>(v);
+                p0[i] = mk<T>(T(0), T(0));
+                if (xs) xs[i] = mk<float>(0.0f, 0.0f);
+                acc += v.x * v.x + v.y * v.y;
+            }
+        }
+    }
+    const double s = block_allsum(acc, lds);
+    if (threadIdx.x == 0) {
+        Prr[p.blk] = s;
+        Pb[p.blk] = s;
+    }
+}
+
+void launch_cg_init_f64(const Latt &L, const double2 *b, double2 *x, double2 *r, double2 *p0,
+                        const CGPartials &P, int rows, hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    hipLaunchKernelGGL(k_cg_init<double>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, x, r, p0,
+                       (float2 *)nullptr, P.rr[0], P.b, g.bx, g.by, g.nbx, rows);
+}
+
+void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *rs, float2 *xs,
+                          float2 *p0, const CGPartials &P, int rows, hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    hipLaunchKernelGGL(k_cg_init<float>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, x, rs, p0,
+                       xs, P.rr[0], P.b, g.bx, g.by, g.nbx, rows);
+}
+
+// ===========================================================================
+// Residual of an FP64 x: r = b - (Q t + mu^2 x), t = Q x  (the guess branch
+// of src/QED2dSolver.jl:65-72, and the reliable update of the mixed solver)
+// ===========================================================================
+struct EpiResid {
+    const double2 *__restrict__ b;
+    const double2 *__restrict__ x;
+    double2 *__restrict__ r;
+    float2 *__restrict__ rs;
+    double2 *__restrict__ p0;
+    float2 *__restrict__ p0s;
+    double mu2;
+    long V;
+    double rr, bb;
+    __device__ __forceinline__ void one(long i, double2 o)
+    {
+        const double2 bv = b[i], xv = x[i];
+        const double2 rv = mk<double>(bv.x - (o.x + mu2 * xv.x), bv.y - (o.y + mu2 * xv.y));
+        if (r) r[i] = rv;
+        if (rs) rs[i] = mk<float>((float)rv.x, (float)rv.y);
+        if (p0) p0[i] = mk<double>(0.0, 0.0);
+        if (p0s) p0s[i] = mk<float>(0.0f, 0.0f);
+        rr += rv.x * rv.x + rv.y * rv.y;
+        bb += bv.x * bv.x + bv.y * bv.y;
+    }
+    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2, double2)
+    {
+        one(s, o1);
+        one(V + s, o2);
+    }
+};
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_cg_resid(Latt L, const double2 *__restrict__ U,
+                                                     const double2 *__restrict__ t,
+                                                     EpiResid e0, double mass2, SolverState *S,
+                                                     double *Prr, double *Pb, int k, int mode,
+                                                     int bx, int by, int nbx)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    MarchPos p = march_pos(L, bx, by, nbx, R);
+    EpiResid epi = e0;
+    epi.mu2 = S->mu2;
+    epi.rr = 0.0;
+    epi.bb = 0.0;
+    if (p.active) {
+        SrcPlain<double> src{t, L.V};
+        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
+    }
+    const double a = block_allsum(epi.rr, lds);
+    const double bb = block_allsum(epi.bb, lds);
+    if (threadIdx.x == 0) {
+        Prr[p.blk] = a;
+        if (mode != 2) Pb[p.blk] = bb;
+    }
+    if (p.blk == 0 && threadIdx.x == 0) {
+        S->true_at = k;  // K1(k) will see an FP64 true residual
+        if (mode == 2) {
+            S->need_ru = 0;
+            S->n_ru += 1;
+            S->stop_k = 0;  // resume: K1(k) runs again
+        }
+    }
+}
+
+void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const double2 *t,
+                     const double2 *x, double2 *r, float2 *rs, double2 *p0, float2 *p0s,
+                     double am0, SolverState *s, const CGPartials &P, int k, int mode, int rows,
+                     hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    EpiResid e{b, x, r, rs, mode == 0 ? p0 : nullptr, mode == 1 ? p0s : nullptr, 0.0, L.V, 0.0,
+               0.0};
+    dim3 grid(g.nblocks()), blk(kBlock);
+    DISPATCH_ROWS(rows, k_cg_resid, grid, blk, 0, st, L, U, t, e, 2.0 + am0, s, P.rr[(k - 1) & 1],
+                  P.b, k, mode, g.bx, g.by, g.nbx);
+}
+
+// x += double(xs); xs = 0
+__global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        const float2 d = xs[i];
+        const double2 v = x[i];
+        x[i] = mk<double>(v.x + (double)d.x, v.y + (double)d.y);
+        xs[i] = mk<float>(0.0f, 0.0f);
+    }
+}
+void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ru_accum, dim3(ew_grid(2 * L.V)), dim3(kBlock), 0, st, x, xs, 2 * L.V);
+}
+
+__global__ void k_to_f32(const double2 *__restrict__ in, float2 *__restrict__ out, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        out[i] = from_d2<float>(in[i]);
+}
+void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_to_f32, dim3(ew_grid(n)), dim3(kBlock), 0, st, in, out, n);
+}
+
+// ===========================================================================
+// CG iteration kernels
+// ===========================================================================
+template <typename T> struct EpiK1 {
+    c2<T> *__restrict__ pnew;
+    c2<T> *__restrict__ t;
+    long V;
+    double tt, pp;
+    __device__ __forceinline__ void operator()(long s, c2<T> o1, c2<T> o2, c2<T> c1, c2<T> cc2)
+    {
+        pnew[s] = c1;
+        pnew[V + s] = cc2;
+        t[s] = o1;
+        t[V + s] = o2;
+        tt += (double)(o1.x * o1.x + o1.y * o1.y) + (double)(o2.x * o2.x + o2.y * o2.y);
+        pp += (double)(c1.x * c1.x + c1.y * c1.y) + (double)(cc2.x * cc2.x + cc2.y * cc2.y);
+    }
+};
+
+// K1(k).  FP64 (T = double): the reference's test |r|^2 < eps |b|^2.  FP32
+// (T = float, the iterations of the mixed solver): the iterated residual asks
+// for a reliable update once it has dropped by delta since the last one, looks
+// converged, or the iteration limit is reached; only an FP64 true residual
+// (true_at == k) ends the solve.
+template <typename T, int R>
+__global__ void __launch_bounds__(kBlock) k_cg1(Latt L, const c2<T> *__restrict__ U,
+                                                const c2<T> *__restrict__ r, c2<T> *P0,
+                                                c2<T> *P1, c2<T> *__restrict__ t, T mass2,
+                                                SolverState *S, double *Ptt, const double *Prr,
+                                                const double *Pb, int nb, int k, int bx, int by,
+                                                int nbx)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const int sk = S->stop_k;
+    if (sk != 0 && sk < k) return;
+    constexpr bool mixed = std::is_same<T, float>::value;
+    const int w = k & 1, rd = (k - 1) & 1;
+    const MarchPos p = march_pos(L, bx, by, nbx, R);
+    const double rr = sum_prev_partials(Prr, nb, 1, 0, lds);
+    const bool first = (k == 1);
+    const double bn = first ? sum_prev_partials(Pb, nb, 1, 0, lds) : S->bnorm;
+    const double tol = first ? S->eps * bn : S->tol;
+    const double prev = first ? 0.0 : S->norm[rd];
+    double rmax = first ? rr : S->rmax2[rd];
+    const bool truer = first || S->true_at == k;
+    int stat = 0, ru = 0;
+    if (first && bn == 0.0 && !S->guess) {
+        stat = 1;  // b = 0: x = 0 exactly
+    } else if (!mixed) {
+        if (rr < tol) stat = 1;                         // src/QED2dSolver.jl:29
+        else if (k - 1 >= S->maxiter) stat = 2;
+    } else if (truer) {
+        if (rr < tol) stat = 1;
+        else {
+            rmax = rr;
+            if (k - 1 >= S->maxiter) stat = 2;
+        }
+    } else {
+        if (rr < tol || rr < S->delta2 * rmax || k - 1 >= S->maxiter) ru = 1;
+        else if (rr > rmax) rmax = rr;
+    }
+    if (p.blk == 0 && threadIdx.x == 0) {
+        S->err = rr;
+        S->iter = k - 1;
+        S->norm[w] = rr;
+        S->rmax2[w] = rmax;
+        if (first) {
+            S->bnorm = bn;
+            S->tol = tol;
+        }
+        if (stat) {
+            S->status = stat;
+            S->stop_k = k;
+        }
+        if (ru) {
+            S->need_ru = 1;
+            S->stop_k = k;
+        }
+    }
+    if (stat || ru) return;
+    const T beta = first ? T(0) : (T)(rr / prev);       // src/QED2dSolver.jl:33-35
+    const c2<T> *pold = rd ? P1 : P0;
+    c2<T> *pnew = w ? P1 : P0;
+    EpiK1<T> epi{pnew, t, L.V, 0.0, 0.0};
+    if (p.active) {
+        SrcXpay<T> src{r, pold, beta, L.V};
+        q_march<T, R>(L, U, mass2, src, epi, p.i1, p.r0);
+    }
+    const double a = block_allsum(epi.tt, lds);
+    const double b = block_allsum(epi.pp, lds);
+    if (threadIdx.x == 0) {
+        Ptt[2 * p.blk] = a;
+        Ptt[2 * p.blk + 1] = b;
+    }
+}
+
+template <typename T> struct EpiK2 {
+    const c2<T> *__restrict__ p;
+    c2<T> *__restrict__ x;
+    c2<T> *__restrict__ r;
+    T alpha, mu2;
+    long V;
+    double rr;
+    __device__ __forceinline__ void one(long i, c2<T> o)
+    {
+        const c2<T> pv = p[i];
+        const c2<T> a = mk<T>(o.x + mu2 * pv.x, o.y + mu2 * pv.y);
+        c2<T> xv = x[i], rv = r[i];
+        xv = mk<T>(xv.x + alpha * pv.x, xv.y + alpha * pv.y);   // src/QED2dSolver.jl:24
+        rv = mk<T>(rv.x - alpha * a.x, rv.y - alpha * a.y);     // src/QED2dSolver.jl:25
+        x[i] = xv;
+        r[i] = rv;
+        rr += (double)(rv.x * rv.x + rv.y * rv.y);
+    }
+    __device__ __forceinline__ void operator()(long s, c2<T> o1, c2<T> o2, c2<T>, c2<T>)
+    {
+        one(s, o1);
+        one(V + s, o2);
+    }
+};
+
+template <typename T, int R>
+__global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict__ U,
+                                                const c2<T> *__restrict__ t, const c2<T> *P0,
+                                                const c2<T> *P1, c2<T> *__restrict__ x,
+                                                c2<T> *__restrict__ r, T mass2, SolverState *S,
+                                                const double *Ptt, double *Prr, int nb, int k,
+                                                int bx, int by, int nbx)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const int sk = S->stop_k;
+    if (sk != 0 && sk <= k) return;
+    const int w = k & 1;
+    const MarchPos p = march_pos(L, bx, by, nbx, R);
+    const double tt = sum_prev_partials(Ptt, nb, 2, 0, lds);
+    const double pp = sum_prev_partials(Ptt, nb, 2, 1, lds);
+    const double alpha = S->norm[w] / (tt + S->mu2 * pp);  // src/QED2dSolver.jl:21-22
+    const c2<T> *pv = w ? P1 : P0;
+    EpiK2<T> epi{pv, x, r, (T)alpha, (T)S->mu2, L.V, 0.0};
+    if (p.active) {
+        SrcPlain<T> src{t, L.V};
+        q_march<T, R>(L, U, mass2, src, epi, p.i1, p.r0);
+    }
+    const double a = block_allsum(epi.rr, lds);
+    if (threadIdx.x == 0) Prr[p.blk] = a;
+}
+
+template <typename T>
+static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c2<T> *p1,
+                     c2<T> *t, T am0, SolverState *S, const CGPartials &P, int k, int rows,
+                     hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    dim3 grid(g.nblocks()), blk(kBlock);
+    const T m2 = T(2) + am0;
+    const int nb = g.nblocks();
+    double *ptt = P.tt[k & 1];
+    const double *prr = P.rr[(k - 1) & 1];
+    switch (rows) {
+    case 1: hipLaunchKernelGGL((k_cg1<T, 1>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nb, k, g.bx, g.by, g.nbx); break;
+    case 2: hipLaunchKernelGGL((k_cg1<T, 2>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nb, k, g.bx, g.by, g.nbx); break;
+    case 4: hipLaunchKernelGGL((k_cg1<T, 4>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nb, k, g.bx, g.by, g.nbx); break;
+    case 8: hipLaunchKernelGGL((k_cg1<T, 8>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nb, k, g.bx, g.by, g.nbx); break;
+    default: hipLaunchKernelGGL((k_cg1<T, 16>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nb, k, g.bx, g.by, g.nbx); break;
+    }
+}
+
+template <typename T>
+static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> *p0,
+                     const c2<T> *p1, c2<T> *x, c2<T> *r, T am0, SolverState *S,
+                     const CGPartials &P, int k, int rows, hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    dim3 grid(g.nblocks()), blk(kBlock);
+    const T m2 = T(2) + am0;
+    const int nb = g.nblocks();
+    const double *ptt = P.tt[k & 1];
+    double *prr = P.rr[k & 1];
+    switch (rows) {
+    case 1: hipLaunchKernelGGL((k_cg2<T, 1>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, nb, k, g.bx, g.by, g.nbx); break;
+    case 2: hipLaunchKernelGGL((k_cg2<T, 2>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, nb, k, g.bx, g.by, g.nbx); break;
+    case 4: hipLaunchKernelGGL((k_cg2<T, 4>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, nb, k, g.bx, g.by, g.nbx); break;
+    case 8: hipLaunchKernelGGL((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, nb, k, g.bx, g.by, g.nbx); break;
+    default: hipLaunchKernelGGL((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, nb, k, g.bx, g.by, g.nbx); break;
+    }
+}
+
+void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2 *p0,
+                      double2 *p1, double2 *t, double am0, SolverState *s, const CGPartials &P,
+                      int k, int rows, hipStream_t st)
+{
+    cg1_impl<double>(L, U, r, p0, p1, t, am0, s, P, k, rows, st);
+}
+void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
+                      const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
+                      const CGPartials &P, int k, int rows, hipStream_t st)
+{
+    cg2_impl<double>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, st);
+}
+void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
+                      float2 *t, float am0, SolverState *s, const CGPartials &P, int k, int rows,
+                      hipStream_t st)
+{
+    cg1_impl<float>(L, U, r, p0, p1, t, am0, s, P, k, rows, st);
+}
+void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
+                      const float2 *p1, float2 *x, float2 *r, float am0, SolverState *s,
+                      const CGPartials &P, int k, int rows, hipStream_t st)
+{
+    cg2_impl<float>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, st);
+}
+
+// ===========================================================================
+// Multi-shift CG (Jegerlehner, hep-lat/9612014).  The systems
+// (Q^2 + sigma_i) x_i = b share the Krylov space of the base system
+// (Q^2 + sigma_b), sigma_b the smallest shift, whose residual r_j fixes every
+// shifted residual up to a scalar, r^i_j = zeta^i_j r_j.  With
+// s = sigma_i - sigma_b and p_j = r_j + beta_j p_{j-1}:
+//   zeta_j    = zeta_{j-1} zeta_{j-2} alpha_{j-2} /
+//               (alpha_{j-1} beta_{j-1} (zeta_{j-2} - zeta_{j-1})
+//                + zeta_{j-2} alpha_{j-2} (1 + s alpha_{j-1}))
+//   alpha^i_{j-1} = alpha_{j-1} zeta_j / zeta_{j-1},  beta^i_j = beta_j (zeta_j / zeta_{j-1})^2
+//   x_i += alpha^i_{j-1} p^i_{j-1},  p^i_j = zeta_j r_j + beta^i_j p^i_{j-1}
+// (zeta_0 = zeta_{-1} = alpha_{-1} = 1, beta_0 = 0).  K1(k) handles j = k - 1
+// and fuses every shift's x and p update into the stencil pass; a shift stops
+// once zeta_j^2 |r_j|^2 < eps |b|^2.  Bytes per site and iteration (FP64):
+// K1 160 + 128 per active shift, K2 160.
+// ===========================================================================
+struct EpiMS1 {
+    double2 *__restrict__ pnew;
+    double2 *__restrict__ t;
+    const double2 *__restrict__ r;
+    const MultiShiftState *S;
+    const double *zj, *as, *bs;
+    const int *upd;
+    int nshift;
+    long V;
+    double tt, pp;
+    __device__ __forceinline__ void shifts(long i, double2 rv)
+    {
+        for (int k = 0; k < nshift; ++k) {
+            const int u = upd[k];
+            if (u == 0) continue;
+            double2 *ps = S->ps[k];
+            double2 *x = S->x[k];
+            const double2 pv = ps[i];
+            const double a = as[k];
+            const double2 xv = x[i];
+            x[i] = mk<double>(xv.x + a * pv.x, xv.y + a * pv.y);
+            if (u == 2) {
+                const double z = zj[k], b = bs[k];
+                ps[i] = mk<double>(z * rv.x + b * pv.x, z * rv.y + b * pv.y);
+            }
+        }
+    }
+    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2 c1,
+                                               double2 cc2)
+    {
+        pnew[s] = c1;
+        pnew[V + s] = cc2;
+        t[s] = o1;
+        t[V + s] = o2;
+        tt += (o1.x * o1.x + o1.y * o1.y) + (o2.x * o2.x + o2.y * o2.y);
+        pp += (c1.x * c1.x + c1.y * c1.y) + (cc2.x * cc2.x + cc2.y * cc2.y);
+        shifts(s, r[s]);
+        shifts(V + s, r[V + s]);
+    }
+};
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_ms1(Latt L, const double2 *__restrict__ U,
+                                                const double2 *__restrict__ r, double2 *P0,
+                                                double2 *P1, double2 *__restrict__ t,
+                                                double mass2, MultiShiftState *S, double *Ptt_w,
+                                                const double *Ptt_rd, const double *Prr_rd,
+                                                int nb, int k, int nshift, int bx, int by,
+                                                int nbx)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    __shared__ double zj_s[kMaxShift], z1_s[kMaxShift], as_s[kMaxShift], bs_s[kMaxShift];
+    __shared__ int upd_s[kMaxShift];
+    __shared__ int anyleft_s;
+    const int sk = S->stop_k;
+    if (sk != 0 && sk < k) return;
+    const int w = k & 1, rd = (k - 1) & 1;
+    const MarchPos p = march_pos(L, bx, by, nbx, R);
+    const double rr = sum_prev_partials(Prr_rd, nb, 1, 0, lds);
+    const bool first = (k == 1);
+    double tol, alpha_jm1, beta_j;
+    if (first) {
+        tol = S->eps * rr;
+        alpha_jm1 = 1.0;
+        beta_j = 0.0;
+    } else {
+        const double tt = sum_prev_partials(Ptt_rd, nb, 2, 0, lds);
+        const double pp = sum_prev_partials(Ptt_rd, nb, 2, 1, lds);
+        tol = S->tol;
+        alpha_jm1 = S->norm[rd] / (tt + S->sigma_b * pp);
+        beta_j = rr / S->norm[rd];
+    }
+    if (threadIdx.x < nshift) {
+        const int i = threadIdx.x;
+        double zj = 1.0, z1 = 1.0, as = 0.0, bs = 0.0;
+        int act = 1, conv = 0;
+        if (first) {
+            conv = (rr == 0.0);
+        } else {
+            act = S->active[rd][i];
+            if (act) {
+                z1 = S->zeta[rd][i];
+                const double z2 = S->zeta_prev[rd][i];
+                const double alpha_jm2 = S->alpha[rd], beta_jm1 = S->beta[rd];
+                zj = z1 * z2 * alpha_jm2 /
+                     (alpha_jm1 * beta_jm1 * (z2 - z1) + z2 * alpha_jm2 * (1.0 + S->s[i] * alpha_jm1));
+                as = alpha_jm1 * zj / z1;
+                bs = beta_j * (zj / z1) * (zj / z1);
+                conv = zj * zj * rr < tol;
+            }
+        }
+        zj_s[i] = zj;
+        z1_s[i] = z1;
+        as_s[i] = as;
+        bs_s[i] = bs;
+        upd_s[i] = act ? (conv ? 1 : 2) : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int any = 0;
+        for (int i = 0; i < nshift; ++i) any |= (upd_s[i] == 2);
+        anyleft_s = any;
+    }
+    __syncthreads();
+    const int anyleft = anyleft_s;
+    const int stat = anyleft ? ((k - 1 >= S->maxiter) ? 2 : 0) : 1;
+    if (p.blk == 0 && threadIdx.x == 0) {
+        S->norm[w] = rr;
+        S->alpha[w] = alpha_jm1;
+        S->beta[w] = beta_j;
+        for (int i = 0; i < nshift; ++i) {
+            S->zeta[w][i] = zj_s[i];
+            S->zeta_prev[w][i] = z1_s[i];
+            S->active[w][i] = (upd_s[i] == 2);
+            if (upd_s[i] == 1) S->conv_iter[i] = k - 1;
+        }
+        S->iter = k - 1;
+        S->err = rr;
+        if (first) {
+            S->tol = tol;
+            S->bnorm = rr;
+        }
+        if (stat) {
+            S->status = stat;
+            S->stop_k = k;
+        }
+    }
+    if (stat == 2) return;
+    if (anyleft) {
+        const double2 *pold = rd ? P1 : P0;
+        double2 *pnew = w ? P1 : P0;
+        EpiMS1 epi{pnew, t, r, S, zj_s, as_s, bs_s, upd_s, nshift, L.V, 0.0, 0.0};
+        if (p.active) {
+            SrcXpay<double> src{r, pold, beta_j, L.V};
+            q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
+        }
+        const double a = block_allsum(epi.tt, lds);
+        const double b = block_allsum(epi.pp, lds);
+        if (threadIdx.x == 0) {
+            Ptt_w[2 * p.blk] = a;
+            Ptt_w[2 * p.blk + 1] = b;
+        }
+    } else if (p.active) {
+        // every shift converged at this step: only the last x updates remain
+        EpiMS1 epi{nullptr, nullptr, r, S, zj_s, as_s, bs_s, upd_s, nshift, L.V, 0.0, 0.0};
+        for (int kk = 0; kk < R; ++kk) {
+            const int row = p.r0 + kk;
+            if (row >= L.L2) break;
+            const long s = (long)row * L.L1 + p.i1;
+            epi.shifts(s, r[s]);
+            epi.shifts(L.V + s, r[L.V + s]);
+        }
+    }
+}
+
+struct EpiMS2 {
+    double2 *__restrict__ r;
+    const double2 *__restrict__ p;
+    double alpha, sigma;
+    long V;
+    double rr;
+    __device__ __forceinline__ void one(long i, double2 o)
+    {
+        const double2 pv = p[i];
+        double2 rv = r[i];
+        rv = mk<double>(rv.x - alpha * (o.x + sigma * pv.x), rv.y - alpha * (o.y + sigma * pv.y));
+        r[i] = rv;
+        rr += rv.x * rv.x + rv.y * rv.y;
+    }
+    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2, double2)
+    {
+        one(s, o1);
+        one(V + s, o2);
+    }
+};
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_ms2(Latt L, const double2 *__restrict__ U,
+                                                const double2 *__restrict__ t, const double2 *P0,
+                                                const double2 *P1, double2 *__restrict__ r,
+                                                double mass2, MultiShiftState *S,
+                                                const double *Ptt_w, double *Prr_w, int nb, int k,
+                                                int bx, int by, int nbx)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const int sk = S->stop_k;
+    if (sk != 0 && sk <= k) return;
+    const int w = k & 1;
+    const MarchPos p = march_pos(L, bx, by, nbx, R);
+    const double tt = sum_prev_partials(Ptt_w, nb, 2, 0, lds);
+    const double pp = sum_prev_partials(Ptt_w, nb, 2, 1, lds);
+    const double alpha = S->norm[w] / (tt + S->sigma_b * pp);
+    EpiMS2 epi{r, w ? P1 : P0, alpha, S->sigma_b, L.V, 0.0};
+    if (p.active) {
+        SrcPlain<double> src{t, L.V};
+        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
+    }
+    const double a = block_allsum(epi.rr, lds);
+    if (threadIdx.x == 0) Prr_w[p.blk] = a;
+}
+
+// r = b, p0 = 0, x_i = 0, p_i = 0, |b|^2 partials.
+__global__ void __launch_bounds__(kBlock) k_ms_init(Latt L, const double2 *__restrict__ b,
+                                                    double2 *__restrict__ r,
+                                                    double2 *__restrict__ p0,
+                                                    const MultiShiftState *S, double *Prr,
+                                                    int nshift, int bx, int by, int nbx, int rows)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    MarchPos p = march_pos(L, bx, by, nbx, rows);
+    double acc = 0.0;
+    const double2 z = mk<double>(0.0, 0.0);
+    if (p.active) {
+        for (int kk = 0; kk < rows; ++kk) {
+            const int row = p.r0 + kk;
+            if (row >= L.L2) break;
+            for (int c = 0; c < 2; ++c) {
+                const long i = c * L.V + (long)row * L.L1 + p.i1;
+                const double2 v = b[i];
+                r[i] = v;
+                p0[i] = z;
+                for (int k = 0; k < nshift; ++k) {
+                    S->x[k][i] = z;
+                    S->ps[k][i] = z;
+                }
+                acc += v.x * v.x + v.y * v.y;
+            }
+        }
+    }
+    const double s = block_allsum(acc, lds);
+    if (threadIdx.x == 0) Prr[p.blk] = s;
+}
+
+void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0, MultiShiftState *s,
+                    const CGPartials &P, int nshift, int rows, hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    hipLaunchKernelGGL(k_ms_init, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, r, p0, s, P.rr[0],
+                       nshift, g.bx, g.by, g.nbx, rows);
+}
+
+void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, double2 *p0, double2 *p1,
+                  double2 *t, double am0, MultiShiftState *s, const CGPartials &P, int k,
+                  int nshift, int rows, hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    dim3 grid(g.nblocks()), blk(kBlock);
+    DISPATCH_ROWS(rows, k_ms1, grid, blk, 0, st, L, U, r, p0, p1, t, 2.0 + am0, s, P.tt[k & 1],
+                  P.tt[(k - 1) & 1], P.rr[(k - 1) & 1], g.nblocks(), k, nshift, g.bx, g.by,
+                  g.nbx);
+}
+
+void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
+                  const double2 *p1, double2 *r, double am0, MultiShiftState *s,
+                  const CGPartials &P, int k, int nshift, int rows, hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    dim3 grid(g.nblocks()), blk(kBlock);
+    DISPATCH_ROWS(rows, k_ms2, grid, blk, 0, st, L, U, t, p0, p1, r, 2.0 + am0, s, P.tt[k & 1],
+                  P.rr[k & 1], g.nblocks(), k, g.bx, g.by, g.nbx);
+}
+
+}  // namespace qed2d

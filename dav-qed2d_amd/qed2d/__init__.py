@@ -115,9 +115,10 @@ PROF_CLASSES = ["cg_k1_f64", "cg_k2_f64", "cg_k1_f32", "cg_k2_f32", "q_f64", "fo
                 "updt_fld", "reliable_update", "other"]
 
 
-def profile_enable(prm, on=True):
-    """Per-kernel-class HIP-event timing on the context's stream."""
-    _check(prm, lib.qed2d_profile_enable(prm.ctx(), 1 if on else 0))
+def profile_enable(prm, on=True, every=1):
+    """Per-kernel-class HIP-event timing on the context's stream; `every`
+    samples one launch in `every` of each class."""
+    _check(prm, lib.qed2d_profile_enable(prm.ctx(), int(every) if on else 0))
     _check(prm, lib.qed2d_profile_reset(prm.ctx()))
 
 

@@ -253,7 +253,8 @@ int qed2d_read_links_text(double *U_host, long nlinks, const char *path);
 #define QED2D_PROF_RELIABLE 7     /* reliable-update kernels of the mixed CG */
 #define QED2D_PROF_OTHER 8
 #define QED2D_PROF_NCLASS 9
-int qed2d_profile_enable(qed2d_ctx *ctx, int on);
+/* every = 0: off; every = N: time every N-th launch of each class. */
+int qed2d_profile_enable(qed2d_ctx *ctx, int every);
 int qed2d_profile_reset(qed2d_ctx *ctx);
 int qed2d_profile_read(qed2d_ctx *ctx, int cls, long *count, double *total_ms);
 
