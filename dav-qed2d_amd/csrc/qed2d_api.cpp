@@ -35,7 +35,8 @@ struct qed2d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int sync = 1;
-    int rows = 4;
+    int rows = 16;   // rows marched per thread, FP64 stencil kernels
+    int rows32 = 4;  // rows marched per thread, FP32 stencil kernels
     std::string err;
     double2 *spin[S_NSLOTS] = {};
     float2 *fspin[FS_NSLOTS] = {};
@@ -434,11 +435,11 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     for (;;) {
         for (int j = 0; j < chunk; ++j, ++k) {
             PROF(c, QED2D_PROF_CG_K1_F32,
-                 launch_cg_k1_f32(L, Us, rs, P[0], P[1], ts, am0f, c->state, c->P, k, c->rows,
+                 launch_cg_k1_f32(L, Us, rs, P[0], P[1], ts, am0f, c->state, c->P, k, c->rows32,
                                   st));
             PROF(c, QED2D_PROF_CG_K2_F32,
-                 launch_cg_k2_f32(L, Us, ts, P[0], P[1], xs, rs, am0f, c->state, c->P, k, c->rows,
-                                  st));
+                 launch_cg_k2_f32(L, Us, ts, P[0], P[1], xs, rs, am0f, c->state, c->P, k,
+                                  c->rows32, st));
         }
         TRY(launched(c));
         TRY(read_state(c));
@@ -938,6 +939,15 @@ int qed2d_ctx_set_tile(qed2d_ctx *c, int rows)
     return QED2D_OK;
 }
 
+int qed2d_ctx_set_tile_f32(qed2d_ctx *c, int rows)
+{
+    if (!c) return QED2D_EINVAL;
+    if (rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16)
+        return fail(c, QED2D_EINVAL, "tile rows must be 1, 2, 4, 8 or 16 (got %d)", rows);
+    c->rows32 = rows;
+    return QED2D_OK;
+}
+
 int qed2d_synchronize(qed2d_ctx *c)
 {
     if (!c) return QED2D_EINVAL;
@@ -967,7 +977,7 @@ int qed2d_gamm5Dw_f32(qed2d_ctx *c, float *so, const float *U, const float *si, 
     CHECK_CTX(c);
     NEED(c, so); NEED(c, U); NEED(c, si);
     NOALIAS(c, so, si);
-    launch_q_apply_f32(c->L, (const float2 *)U, (const float2 *)si, (float2 *)so, am0, c->rows,
+    launch_q_apply_f32(c->L, (const float2 *)U, (const float2 *)si, (float2 *)so, am0, c->rows32,
                        c->stream);
     return finish(c);
 }

@@ -51,10 +51,14 @@ struct MultiShiftState {
 
 // Per-block partial sums of the solver kernels.  tt: (|Q p|^2, |p|^2) per block,
 // two slots by iteration parity; rr: |r|^2 per block, two slots; b: |b|^2.
+// The n* fields record how many blocks the kernel that last filled a slot
+// had (the FP32 pair kernels use a different grid from the FP64 ones); the
+// launch functions keep them up to date in enqueue order.
 struct CGPartials {
     double *tt[2];
     double *rr[2];
     double *b;
+    int ntt[2], nrr[2], nb;
 };
 
 // Scalars of a reduction that the host reads back.
@@ -89,10 +93,10 @@ void launch_q_shift_f64(const Latt &L, const double2 *U, const double2 *si, cons
 // ---- CG on Q^2 + mu^2 (all take the marching grid of `rows`) -------------
 // Start from x = 0: x = 0, r = b, p0 = 0; |b|^2 partials into rr[0] and b.
 void launch_cg_init_f64(const Latt &L, const double2 *b, double2 *x, double2 *r, double2 *p0,
-                        const CGPartials &P, int rows, hipStream_t st);
+                        CGPartials &P, int rows, hipStream_t st);
 // Mixed precision start from x = 0: x = 0, rs = float(b), xs = 0, p0 = 0.
 void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *rs, float2 *xs,
-                          float2 *p0, const CGPartials &P, int rows, hipStream_t st);
+                          float2 *p0, CGPartials &P, int rows, hipStream_t st);
 // Residual from x with t = Q x already computed: r = b - (Q t + mu^2 x).
 // mode 0: FP64 start from a guess (writes r, zeroes p0, |b|^2 partials);
 // mode 1: mixed start from a guess (writes rs, zeroes p0s, |b|^2 partials);
@@ -100,21 +104,21 @@ void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *r
 // |r|^2 partials go to rr[(k - 1) & 1].
 void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const double2 *t,
                      const double2 *x, double2 *r, float2 *rs, double2 *p0, float2 *p0s,
-                     double am0, SolverState *s, const CGPartials &P, int k, int mode, int rows,
+                     double am0, SolverState *s, CGPartials &P, int k, int mode, int rows,
                      hipStream_t st);
 // Iteration k: K1 = convergence test, p = r + beta p, t = Q p; K2 = x, r update.
 void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2 *p0,
-                      double2 *p1, double2 *t, double am0, SolverState *s, const CGPartials &P,
+                      double2 *p1, double2 *t, double am0, SolverState *s, CGPartials &P,
                       int k, int rows, hipStream_t st);
 void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
                       const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
-                      const CGPartials &P, int k, int rows, hipStream_t st);
+                      CGPartials &P, int k, int rows, hipStream_t st);
 void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
-                      float2 *t, float am0, SolverState *s, const CGPartials &P, int k, int rows,
+                      float2 *t, float am0, SolverState *s, CGPartials &P, int k, int rows,
                       hipStream_t st);
 void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
                       const float2 *p1, float2 *x, float2 *r, float am0, SolverState *s,
-                      const CGPartials &P, int k, int rows, hipStream_t st);
+                      CGPartials &P, int k, int rows, hipStream_t st);
 // x += double(xs); xs = 0 (a reliable update).
 void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, hipStream_t st);
 void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st);
@@ -122,13 +126,13 @@ void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st);
 // ---- multi-shift CG --------------------------------------------------------
 // r = b, p0 = 0, x_i = 0, p_i = 0, |b|^2 partials into rr[0].
 void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0, MultiShiftState *s,
-                    const CGPartials &P, int nshift, int rows, hipStream_t st);
+                    CGPartials &P, int nshift, int rows, hipStream_t st);
 void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, double2 *p0, double2 *p1,
-                  double2 *t, double am0, MultiShiftState *s, const CGPartials &P, int k,
+                  double2 *t, double am0, MultiShiftState *s, CGPartials &P, int k,
                   int nshift, int rows, hipStream_t st);
 void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
                   const double2 *p1, double2 *r, double am0, MultiShiftState *s,
-                  const CGPartials &P, int k, int nshift, int rows, hipStream_t st);
+                  CGPartials &P, int k, int nshift, int rows, hipStream_t st);
 
 // Reductions (deterministic; results in out->v[]).
 void launch_norm2(const double2 *a, long n, RedOut *out, double *part, hipStream_t st);

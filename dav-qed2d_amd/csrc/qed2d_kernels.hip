@@ -6,6 +6,8 @@
 // vertical neighbours kept in registers by marching each thread up a column
 // of `R` rows, and reductions finished on the device by the last block to
 // arrive so that no kernel boundary needs a host round trip.
+#include <type_traits>
+
 #include "qed2d_march.h"
 
 namespace qed2d {
@@ -84,10 +86,44 @@ __global__ void __launch_bounds__(kBlock) k_q_shift(Latt L, const double2 *__res
 }
 
 
+struct EpiStore2 {
+    float2 *__restrict__ so;
+    long V;
+    __device__ __forceinline__ void pair(long s, float2 oa1, float2 oa2, float2 ob1, float2 ob2,
+                                         F2x2, F2x2)
+    {
+        st2(so, s, oa1, ob1);
+        st2(so, V + s, oa2, ob2);
+    }
+};
+
+// FP32 apply, two sites per thread along i1.
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_q_apply2(Latt L, const float2 *__restrict__ U,
+                                                     const float2 *__restrict__ si,
+                                                     float2 *__restrict__ so, float mass2, int bx,
+                                                     int by, int nbx)
+{
+    MarchPos p = march_pos2(L, bx, by, nbx, R);
+    if (!p.active) return;
+    SrcPlain2 src{si, L.V};
+    EpiStore2 epi{so, L.V};
+    q_march2<R>(L, U, mass2, src, epi, p.i1, p.r0);
+}
+
 template <typename T>
 static void q_apply_impl(const Latt &L, const c2<T> *U, const c2<T> *si, c2<T> *so, T am0,
                          int rows, hipStream_t st)
 {
+    if constexpr (std::is_same<T, float>::value) {
+        if (pair_ok(L)) {
+            MarchGrid g = march_grid2(L, rows);
+            dim3 grid(g.nblocks()), blk(kBlock);
+            DISPATCH_ROWS(rows, k_q_apply2, grid, blk, 0, st, L, U, si, so, T(2) + am0, g.bx, g.by,
+                          g.nbx);
+            return;
+        }
+    }
     MarchGrid g = march_grid(L, rows);
     const T mass2 = T(2) + am0;
     dim3 grid(g.nblocks()), blk(kBlock);

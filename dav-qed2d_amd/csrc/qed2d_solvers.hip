@@ -74,19 +74,21 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Latt L, const double2 *__res
 }
 
 void launch_cg_init_f64(const Latt &L, const double2 *b, double2 *x, double2 *r, double2 *p0,
-                        const CGPartials &P, int rows, hipStream_t st)
+                        CGPartials &P, int rows, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
     hipLaunchKernelGGL(k_cg_init<double>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, x, r, p0,
                        (float2 *)nullptr, P.rr[0], P.b, g.bx, g.by, g.nbx, rows);
+    P.nrr[0] = P.nb = g.nblocks();
 }
 
 void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *rs, float2 *xs,
-                          float2 *p0, const CGPartials &P, int rows, hipStream_t st)
+                          float2 *p0, CGPartials &P, int rows, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
     hipLaunchKernelGGL(k_cg_init<float>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, x, rs, p0,
                        xs, P.rr[0], P.b, g.bx, g.by, g.nbx, rows);
+    P.nrr[0] = P.nb = g.nblocks();
 }
 
 // ===========================================================================
@@ -156,7 +158,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_resid(Latt L, const double2 *__re
 
 void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const double2 *t,
                      const double2 *x, double2 *r, float2 *rs, double2 *p0, float2 *p0s,
-                     double am0, SolverState *s, const CGPartials &P, int k, int mode, int rows,
+                     double am0, SolverState *s, CGPartials &P, int k, int mode, int rows,
                      hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
@@ -165,6 +167,8 @@ void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const do
     dim3 grid(g.nblocks()), blk(kBlock);
     DISPATCH_ROWS(rows, k_cg_resid, grid, blk, 0, st, L, U, t, e, 2.0 + am0, s, P.rr[(k - 1) & 1],
                   P.b, k, mode, g.bx, g.by, g.nbx);
+    P.nrr[(k - 1) & 1] = g.nblocks();
+    if (mode != 2) P.nb = g.nblocks();
 }
 
 // x += double(xs); xs = 0
@@ -197,6 +201,64 @@ void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st)
 // ===========================================================================
 // CG iteration kernels
 // ===========================================================================
+// Convergence test and scalars at the start of K1(k), identical in every
+// block: sums the |r_{k-1}|^2 partials, decides whether the solve ends
+// (converged / iteration limit) or, for the FP32 iterations of the mixed
+// solver, pauses for a reliable update; block 0 commits the scalars of slot
+// k & 1.  Returns false when the kernel has no work; beta receives
+// |r_{k-1}|^2 / |r_{k-2}|^2 (src/QED2dSolver.jl:33-35).
+template <bool mixed>
+__device__ __forceinline__ bool cg_k1_scalars(SolverState *S, const double *Prr, const double *Pb,
+                                              int nrr, int nbb, int k, int blk, double *lds,
+                                              double &beta)
+{
+    const int w = k & 1, rd = (k - 1) & 1;
+    const double rr = sum_prev_partials(Prr, nrr, 1, 0, lds);
+    const bool first = (k == 1);
+    const double bn = first ? sum_prev_partials(Pb, nbb, 1, 0, lds) : S->bnorm;
+    const double tol = first ? S->eps * bn : S->tol;
+    const double prev = first ? 0.0 : S->norm[rd];
+    double rmax = first ? rr : S->rmax2[rd];
+    const bool truer = first || S->true_at == k;
+    int stat = 0, ru = 0;
+    if (first && bn == 0.0 && !S->guess) {
+        stat = 1;  // b = 0: x = 0 exactly
+    } else if (!mixed) {
+        if (rr < tol) stat = 1;                         // src/QED2dSolver.jl:29
+        else if (k - 1 >= S->maxiter) stat = 2;
+    } else if (truer) {
+        if (rr < tol) stat = 1;
+        else {
+            rmax = rr;
+            if (k - 1 >= S->maxiter) stat = 2;
+        }
+    } else {
+        if (rr < tol || rr < S->delta2 * rmax || k - 1 >= S->maxiter) ru = 1;
+        else if (rr > rmax) rmax = rr;
+    }
+    if (blk == 0 && threadIdx.x == 0) {
+        S->err = rr;
+        S->iter = k - 1;
+        S->norm[w] = rr;
+        S->rmax2[w] = rmax;
+        if (first) {
+            S->bnorm = bn;
+            S->tol = tol;
+        }
+        if (stat) {
+            S->status = stat;
+            S->stop_k = k;
+        }
+        if (ru) {
+            S->need_ru = 1;
+            S->stop_k = k;
+        }
+    }
+    if (stat || ru) return false;
+    beta = first ? 0.0 : rr / prev;
+    return true;
+}
+
 template <typename T> struct EpiK1 {
     c2<T> *__restrict__ pnew;
     c2<T> *__restrict__ t;
@@ -223,63 +285,22 @@ __global__ void __launch_bounds__(kBlock) k_cg1(Latt L, const c2<T> *__restrict_
                                                 const c2<T> *__restrict__ r, c2<T> *P0,
                                                 c2<T> *P1, c2<T> *__restrict__ t, T mass2,
                                                 SolverState *S, double *Ptt, const double *Prr,
-                                                const double *Pb, int nb, int k, int bx, int by,
-                                                int nbx)
+                                                const double *Pb, int nrr, int nbb, int k, int bx,
+                                                int by, int nbx)
 {
     __shared__ double lds[kBlock / kWave + 1];
     const int sk = S->stop_k;
     if (sk != 0 && sk < k) return;
-    constexpr bool mixed = std::is_same<T, float>::value;
     const int w = k & 1, rd = (k - 1) & 1;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    const double rr = sum_prev_partials(Prr, nb, 1, 0, lds);
-    const bool first = (k == 1);
-    const double bn = first ? sum_prev_partials(Pb, nb, 1, 0, lds) : S->bnorm;
-    const double tol = first ? S->eps * bn : S->tol;
-    const double prev = first ? 0.0 : S->norm[rd];
-    double rmax = first ? rr : S->rmax2[rd];
-    const bool truer = first || S->true_at == k;
-    int stat = 0, ru = 0;
-    if (first && bn == 0.0 && !S->guess) {
-        stat = 1;  // b = 0: x = 0 exactly
-    } else if (!mixed) {
-        if (rr < tol) stat = 1;                         // src/QED2dSolver.jl:29
-        else if (k - 1 >= S->maxiter) stat = 2;
-    } else if (truer) {
-        if (rr < tol) stat = 1;
-        else {
-            rmax = rr;
-            if (k - 1 >= S->maxiter) stat = 2;
-        }
-    } else {
-        if (rr < tol || rr < S->delta2 * rmax || k - 1 >= S->maxiter) ru = 1;
-        else if (rr > rmax) rmax = rr;
-    }
-    if (p.blk == 0 && threadIdx.x == 0) {
-        S->err = rr;
-        S->iter = k - 1;
-        S->norm[w] = rr;
-        S->rmax2[w] = rmax;
-        if (first) {
-            S->bnorm = bn;
-            S->tol = tol;
-        }
-        if (stat) {
-            S->status = stat;
-            S->stop_k = k;
-        }
-        if (ru) {
-            S->need_ru = 1;
-            S->stop_k = k;
-        }
-    }
-    if (stat || ru) return;
-    const T beta = first ? T(0) : (T)(rr / prev);       // src/QED2dSolver.jl:33-35
+    double beta;
+    if (!cg_k1_scalars<std::is_same<T, float>::value>(S, Prr, Pb, nrr, nbb, k, p.blk, lds, beta))
+        return;
     const c2<T> *pold = rd ? P1 : P0;
     c2<T> *pnew = w ? P1 : P0;
     EpiK1<T> epi{pnew, t, L.V, 0.0, 0.0};
     if (p.active) {
-        SrcXpay<T> src{r, pold, beta, L.V};
+        SrcXpay<T> src{r, pold, (T)beta, L.V};
         q_march<T, R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
     const double a = block_allsum(epi.tt, lds);
@@ -320,7 +341,7 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
                                                 const c2<T> *__restrict__ t, const c2<T> *P0,
                                                 const c2<T> *P1, c2<T> *__restrict__ x,
                                                 c2<T> *__restrict__ r, T mass2, SolverState *S,
-                                                const double *Ptt, double *Prr, int nb, int k,
+                                                const double *Ptt, double *Prr, int ntt, int k,
                                                 int bx, int by, int nbx)
 {
     __shared__ double lds[kBlock / kWave + 1];
@@ -328,8 +349,8 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
     if (sk != 0 && sk <= k) return;
     const int w = k & 1;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    const double tt = sum_prev_partials(Ptt, nb, 2, 0, lds);
-    const double pp = sum_prev_partials(Ptt, nb, 2, 1, lds);
+    const double tt = sum_prev_partials(Ptt, ntt, 2, 0, lds);
+    const double pp = sum_prev_partials(Ptt, ntt, 2, 1, lds);
     const double alpha = S->norm[w] / (tt + S->mu2 * pp);  // src/QED2dSolver.jl:21-22
     const c2<T> *pv = w ? P1 : P0;
     EpiK2<T> epi{pv, x, r, (T)alpha, (T)S->mu2, L.V, 0.0};
@@ -341,67 +362,191 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
     if (threadIdx.x == 0) Prr[p.blk] = a;
 }
 
+// ---- FP32 pair kernels (two sites per thread along i1) -------------------
+struct EpiK1v {
+    float2 *__restrict__ pnew;
+    float2 *__restrict__ t;
+    long V;
+    double tt, pp;
+    __device__ __forceinline__ void pair(long s, float2 oa1, float2 oa2, float2 ob1, float2 ob2,
+                                         F2x2 c1, F2x2 c2)
+    {
+        st2(pnew, s, c1.a0, c1.a1);
+        st2(pnew, V + s, c2.a0, c2.a1);
+        st2(t, s, oa1, ob1);
+        st2(t, V + s, oa2, ob2);
+        tt += (double)(cabs2<float>(oa1) + cabs2<float>(ob1)) +
+              (double)(cabs2<float>(oa2) + cabs2<float>(ob2));
+        pp += (double)(cabs2<float>(c1.a0) + cabs2<float>(c1.a1)) +
+              (double)(cabs2<float>(c2.a0) + cabs2<float>(c2.a1));
+    }
+};
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, const float2 *__restrict__ U,
+                                                 const float2 *__restrict__ r, float2 *P0,
+                                                 float2 *P1, float2 *__restrict__ t, float mass2,
+                                                 SolverState *S, double *Ptt, const double *Prr,
+                                                 const double *Pb, int nrr, int nbb, int k, int bx,
+                                                 int by, int nbx)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const int sk = S->stop_k;
+    if (sk != 0 && sk < k) return;
+    const int w = k & 1, rd = (k - 1) & 1;
+    const MarchPos p = march_pos2(L, bx, by, nbx, R);
+    double beta;
+    if (!cg_k1_scalars<true>(S, Prr, Pb, nrr, nbb, k, p.blk, lds, beta)) return;
+    EpiK1v epi{w ? P1 : P0, t, L.V, 0.0, 0.0};
+    if (p.active) {
+        SrcXpay2 src{r, rd ? P1 : P0, (float)beta, L.V};
+        q_march2<R>(L, U, mass2, src, epi, p.i1, p.r0);
+    }
+    const double a = block_allsum(epi.tt, lds);
+    const double b = block_allsum(epi.pp, lds);
+    if (threadIdx.x == 0) {
+        Ptt[2 * p.blk] = a;
+        Ptt[2 * p.blk + 1] = b;
+    }
+}
+
+struct EpiK2v {
+    const float2 *__restrict__ p;
+    float2 *__restrict__ x;
+    float2 *__restrict__ r;
+    float alpha, mu2;
+    long V;
+    double rr;
+    __device__ __forceinline__ void comp(long i, float2 oa, float2 ob)
+    {
+        const F2x2 pv = ld2(p, i);
+        F2x2 xv = ld2(x, i), rv = ld2(r, i);
+        const float2 aa = make_float2(oa.x + mu2 * pv.a0.x, oa.y + mu2 * pv.a0.y);
+        const float2 ab = make_float2(ob.x + mu2 * pv.a1.x, ob.y + mu2 * pv.a1.y);
+        xv.a0 = make_float2(xv.a0.x + alpha * pv.a0.x, xv.a0.y + alpha * pv.a0.y);
+        xv.a1 = make_float2(xv.a1.x + alpha * pv.a1.x, xv.a1.y + alpha * pv.a1.y);
+        rv.a0 = make_float2(rv.a0.x - alpha * aa.x, rv.a0.y - alpha * aa.y);
+        rv.a1 = make_float2(rv.a1.x - alpha * ab.x, rv.a1.y - alpha * ab.y);
+        st2(x, i, xv.a0, xv.a1);
+        st2(r, i, rv.a0, rv.a1);
+        rr += (double)(cabs2<float>(rv.a0) + cabs2<float>(rv.a1));
+    }
+    __device__ __forceinline__ void pair(long s, float2 oa1, float2 oa2, float2 ob1, float2 ob2,
+                                         F2x2, F2x2)
+    {
+        comp(s, oa1, ob1);
+        comp(V + s, oa2, ob2);
+    }
+};
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_cg2v(Latt L, const float2 *__restrict__ U,
+                                                 const float2 *__restrict__ t, const float2 *P0,
+                                                 const float2 *P1, float2 *__restrict__ x,
+                                                 float2 *__restrict__ r, float mass2,
+                                                 SolverState *S, const double *Ptt, double *Prr,
+                                                 int ntt, int k, int bx, int by, int nbx)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const int sk = S->stop_k;
+    if (sk != 0 && sk <= k) return;
+    const int w = k & 1;
+    const MarchPos p = march_pos2(L, bx, by, nbx, R);
+    const double tt = sum_prev_partials(Ptt, ntt, 2, 0, lds);
+    const double pp = sum_prev_partials(Ptt, ntt, 2, 1, lds);
+    const double alpha = S->norm[w] / (tt + S->mu2 * pp);
+    EpiK2v epi{w ? P1 : P0, x, r, (float)alpha, (float)S->mu2, L.V, 0.0};
+    if (p.active) {
+        SrcPlain2 src{t, L.V};
+        q_march2<R>(L, U, mass2, src, epi, p.i1, p.r0);
+    }
+    const double a = block_allsum(epi.rr, lds);
+    if (threadIdx.x == 0) Prr[p.blk] = a;
+}
+
 template <typename T>
 static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c2<T> *p1,
-                     c2<T> *t, T am0, SolverState *S, const CGPartials &P, int k, int rows,
+                     c2<T> *t, T am0, SolverState *S, CGPartials &P, int k, int rows,
                      hipStream_t st)
 {
-    MarchGrid g = march_grid(L, rows);
-    dim3 grid(g.nblocks()), blk(kBlock);
     const T m2 = T(2) + am0;
-    const int nb = g.nblocks();
     double *ptt = P.tt[k & 1];
     const double *prr = P.rr[(k - 1) & 1];
-    switch (rows) {
-    case 1: hipLaunchKernelGGL((k_cg1<T, 1>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nb, k, g.bx, g.by, g.nbx); break;
-    case 2: hipLaunchKernelGGL((k_cg1<T, 2>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nb, k, g.bx, g.by, g.nbx); break;
-    case 4: hipLaunchKernelGGL((k_cg1<T, 4>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nb, k, g.bx, g.by, g.nbx); break;
-    case 8: hipLaunchKernelGGL((k_cg1<T, 8>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nb, k, g.bx, g.by, g.nbx); break;
-    default: hipLaunchKernelGGL((k_cg1<T, 16>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nb, k, g.bx, g.by, g.nbx); break;
+    const int nrr = P.nrr[(k - 1) & 1], nbb = P.nb;
+    if constexpr (std::is_same<T, float>::value) {
+        if (pair_ok(L)) {
+            MarchGrid g = march_grid2(L, rows);
+            dim3 grid(g.nblocks()), blk(kBlock);
+            DISPATCH_ROWS(rows, k_cg1v, grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr,
+                          P.b, nrr, nbb, k, g.bx, g.by, g.nbx);
+            P.ntt[k & 1] = g.nblocks();
+            return;
+        }
     }
+    MarchGrid g = march_grid(L, rows);
+    dim3 grid(g.nblocks()), blk(kBlock);
+    switch (rows) {
+    case 1: hipLaunchKernelGGL((k_cg1<T, 1>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    case 2: hipLaunchKernelGGL((k_cg1<T, 2>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    case 4: hipLaunchKernelGGL((k_cg1<T, 4>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    case 8: hipLaunchKernelGGL((k_cg1<T, 8>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    default: hipLaunchKernelGGL((k_cg1<T, 16>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    }
+    P.ntt[k & 1] = g.nblocks();
 }
 
 template <typename T>
 static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> *p0,
                      const c2<T> *p1, c2<T> *x, c2<T> *r, T am0, SolverState *S,
-                     const CGPartials &P, int k, int rows, hipStream_t st)
+                     CGPartials &P, int k, int rows, hipStream_t st)
 {
-    MarchGrid g = march_grid(L, rows);
-    dim3 grid(g.nblocks()), blk(kBlock);
     const T m2 = T(2) + am0;
-    const int nb = g.nblocks();
     const double *ptt = P.tt[k & 1];
     double *prr = P.rr[k & 1];
-    switch (rows) {
-    case 1: hipLaunchKernelGGL((k_cg2<T, 1>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, nb, k, g.bx, g.by, g.nbx); break;
-    case 2: hipLaunchKernelGGL((k_cg2<T, 2>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, nb, k, g.bx, g.by, g.nbx); break;
-    case 4: hipLaunchKernelGGL((k_cg2<T, 4>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, nb, k, g.bx, g.by, g.nbx); break;
-    case 8: hipLaunchKernelGGL((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, nb, k, g.bx, g.by, g.nbx); break;
-    default: hipLaunchKernelGGL((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, nb, k, g.bx, g.by, g.nbx); break;
+    const int ntt = P.ntt[k & 1];
+    if constexpr (std::is_same<T, float>::value) {
+        if (pair_ok(L)) {
+            MarchGrid g = march_grid2(L, rows);
+            dim3 grid(g.nblocks()), blk(kBlock);
+            DISPATCH_ROWS(rows, k_cg2v, grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr,
+                          ntt, k, g.bx, g.by, g.nbx);
+            P.nrr[k & 1] = g.nblocks();
+            return;
+        }
     }
+    MarchGrid g = march_grid(L, rows);
+    dim3 grid(g.nblocks()), blk(kBlock);
+    switch (rows) {
+    case 1: hipLaunchKernelGGL((k_cg2<T, 1>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    case 2: hipLaunchKernelGGL((k_cg2<T, 2>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    case 4: hipLaunchKernelGGL((k_cg2<T, 4>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    case 8: hipLaunchKernelGGL((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    default: hipLaunchKernelGGL((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    }
+    P.nrr[k & 1] = g.nblocks();
 }
 
 void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2 *p0,
-                      double2 *p1, double2 *t, double am0, SolverState *s, const CGPartials &P,
+                      double2 *p1, double2 *t, double am0, SolverState *s, CGPartials &P,
                       int k, int rows, hipStream_t st)
 {
     cg1_impl<double>(L, U, r, p0, p1, t, am0, s, P, k, rows, st);
 }
 void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
                       const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
-                      const CGPartials &P, int k, int rows, hipStream_t st)
+                      CGPartials &P, int k, int rows, hipStream_t st)
 {
     cg2_impl<double>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, st);
 }
 void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
-                      float2 *t, float am0, SolverState *s, const CGPartials &P, int k, int rows,
+                      float2 *t, float am0, SolverState *s, CGPartials &P, int k, int rows,
                       hipStream_t st)
 {
     cg1_impl<float>(L, U, r, p0, p1, t, am0, s, P, k, rows, st);
 }
 void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
                       const float2 *p1, float2 *x, float2 *r, float am0, SolverState *s,
-                      const CGPartials &P, int k, int rows, hipStream_t st)
+                      CGPartials &P, int k, int rows, hipStream_t st)
 {
     cg2_impl<float>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, st);
 }
@@ -469,8 +614,8 @@ __global__ void __launch_bounds__(kBlock) k_ms1(Latt L, const double2 *__restric
                                                 double2 *P1, double2 *__restrict__ t,
                                                 double mass2, MultiShiftState *S, double *Ptt_w,
                                                 const double *Ptt_rd, const double *Prr_rd,
-                                                int nb, int k, int nshift, int bx, int by,
-                                                int nbx)
+                                                int nrr, int ntt, int k, int nshift, int bx,
+                                                int by, int nbx)
 {
     __shared__ double lds[kBlock / kWave + 1];
     __shared__ double zj_s[kMaxShift], z1_s[kMaxShift], as_s[kMaxShift], bs_s[kMaxShift];
@@ -480,7 +625,7 @@ __global__ void __launch_bounds__(kBlock) k_ms1(Latt L, const double2 *__restric
     if (sk != 0 && sk < k) return;
     const int w = k & 1, rd = (k - 1) & 1;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    const double rr = sum_prev_partials(Prr_rd, nb, 1, 0, lds);
+    const double rr = sum_prev_partials(Prr_rd, nrr, 1, 0, lds);
     const bool first = (k == 1);
     double tol, alpha_jm1, beta_j;
     if (first) {
@@ -488,8 +633,8 @@ __global__ void __launch_bounds__(kBlock) k_ms1(Latt L, const double2 *__restric
         alpha_jm1 = 1.0;
         beta_j = 0.0;
     } else {
-        const double tt = sum_prev_partials(Ptt_rd, nb, 2, 0, lds);
-        const double pp = sum_prev_partials(Ptt_rd, nb, 2, 1, lds);
+        const double tt = sum_prev_partials(Ptt_rd, ntt, 2, 0, lds);
+        const double pp = sum_prev_partials(Ptt_rd, ntt, 2, 1, lds);
         tol = S->tol;
         alpha_jm1 = S->norm[rd] / (tt + S->sigma_b * pp);
         beta_j = rr / S->norm[rd];
@@ -603,7 +748,7 @@ __global__ void __launch_bounds__(kBlock) k_ms2(Latt L, const double2 *__restric
                                                 const double2 *__restrict__ t, const double2 *P0,
                                                 const double2 *P1, double2 *__restrict__ r,
                                                 double mass2, MultiShiftState *S,
-                                                const double *Ptt_w, double *Prr_w, int nb, int k,
+                                                const double *Ptt_w, double *Prr_w, int ntt, int k,
                                                 int bx, int by, int nbx)
 {
     __shared__ double lds[kBlock / kWave + 1];
@@ -611,8 +756,8 @@ __global__ void __launch_bounds__(kBlock) k_ms2(Latt L, const double2 *__restric
     if (sk != 0 && sk <= k) return;
     const int w = k & 1;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    const double tt = sum_prev_partials(Ptt_w, nb, 2, 0, lds);
-    const double pp = sum_prev_partials(Ptt_w, nb, 2, 1, lds);
+    const double tt = sum_prev_partials(Ptt_w, ntt, 2, 0, lds);
+    const double pp = sum_prev_partials(Ptt_w, ntt, 2, 1, lds);
     const double alpha = S->norm[w] / (tt + S->sigma_b * pp);
     EpiMS2 epi{r, w ? P1 : P0, alpha, S->sigma_b, L.V, 0.0};
     if (p.active) {
@@ -656,32 +801,35 @@ __global__ void __launch_bounds__(kBlock) k_ms_init(Latt L, const double2 *__res
 }
 
 void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0, MultiShiftState *s,
-                    const CGPartials &P, int nshift, int rows, hipStream_t st)
+                    CGPartials &P, int nshift, int rows, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
     hipLaunchKernelGGL(k_ms_init, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, r, p0, s, P.rr[0],
                        nshift, g.bx, g.by, g.nbx, rows);
+    P.nrr[0] = g.nblocks();
 }
 
 void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, double2 *p0, double2 *p1,
-                  double2 *t, double am0, MultiShiftState *s, const CGPartials &P, int k,
+                  double2 *t, double am0, MultiShiftState *s, CGPartials &P, int k,
                   int nshift, int rows, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
     dim3 grid(g.nblocks()), blk(kBlock);
     DISPATCH_ROWS(rows, k_ms1, grid, blk, 0, st, L, U, r, p0, p1, t, 2.0 + am0, s, P.tt[k & 1],
-                  P.tt[(k - 1) & 1], P.rr[(k - 1) & 1], g.nblocks(), k, nshift, g.bx, g.by,
-                  g.nbx);
+                  P.tt[(k - 1) & 1], P.rr[(k - 1) & 1], P.nrr[(k - 1) & 1], P.ntt[(k - 1) & 1], k,
+                  nshift, g.bx, g.by, g.nbx);
+    P.ntt[k & 1] = g.nblocks();
 }
 
 void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
                   const double2 *p1, double2 *r, double am0, MultiShiftState *s,
-                  const CGPartials &P, int k, int nshift, int rows, hipStream_t st)
+                  CGPartials &P, int k, int nshift, int rows, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
     dim3 grid(g.nblocks()), blk(kBlock);
     DISPATCH_ROWS(rows, k_ms2, grid, blk, 0, st, L, U, t, p0, p1, r, 2.0 + am0, s, P.tt[k & 1],
-                  P.rr[k & 1], g.nblocks(), k, g.bx, g.by, g.nbx);
+                  P.rr[k & 1], P.ntt[k & 1], k, g.bx, g.by, g.nbx);
+    P.nrr[k & 1] = g.nblocks();
 }
 
 }  // namespace qed2d

@@ -74,8 +74,10 @@ void qed2d_ctx_destroy(qed2d_ctx *ctx);
 const char *qed2d_last_error(const qed2d_ctx *ctx);
 int qed2d_ctx_set_beta(qed2d_ctx *ctx, double beta);
 int qed2d_ctx_set_sync(qed2d_ctx *ctx, int sync);
-/* Rows marched per thread by the stencil kernels (1, 2, 4, 8 or 16). */
+/* Rows marched per thread by the FP64 / the FP32 stencil kernels (1, 2, 4, 8
+ * or 16). */
 int qed2d_ctx_set_tile(qed2d_ctx *ctx, int rows);
+int qed2d_ctx_set_tile_f32(qed2d_ctx *ctx, int rows);
 int qed2d_synchronize(qed2d_ctx *ctx);
 
 /* ---- Dirac operator -------------------------------------------------------- */

@@ -45,17 +45,24 @@ def test_gamm5Dw_large(Q, O):
     assert relerr(host(so), O.gamm5Dw(U, si, 0.1)) < 1e-13
 
 
-@pytest.mark.parametrize("L1,L2", [(24, 24), (128, 64)])
-def test_gamm5Dw_f32(Q, O, L1, L2):
+@pytest.mark.parametrize("L1,L2,bc", [(24, 24, 0), (128, 64, 1), (4, 6, 0), (25, 16, 1),
+                                       (2, 8, 0), (66, 33, 0)])
+def test_gamm5Dw_f32(Q, O, L1, L2, bc):
+    """Even L1 >= 4 runs the two-sites-per-thread kernel, odd L1 (and L1 = 2)
+    the one-site kernel; every tile height."""
     import torch
-    say("gamm5Dw f32", L1, L2)
-    rng = np.random.default_rng(3)
+    say("gamm5Dw f32", L1, L2, bc)
+    rng = np.random.default_rng(3 + L1)
     U = random_links(L1, L2, rng, unitary=True)
     si = random_spinor(L1, L2, rng)
-    prm = Q.LattParm((L1, L2), 4.0)
-    so = torch.zeros((2, L2, L1), dtype=torch.complex64, device="cuda")
-    Q.gamm5Dw_f32(so, dev(U.astype(np.complex64)), dev(si.astype(np.complex64)), 0.2, prm)
-    assert relerr(host(so), O.gamm5Dw(U, si, 0.2)) < 2e-6
+    prm = Q.LattParm((L1, L2), 4.0, bc)
+    ref = O.gamm5Dw(U, si, 0.2, bc)
+    dU, dsi = dev(U.astype(np.complex64)), dev(si.astype(np.complex64))
+    for rows in (1, 2, 4, 8, 16):
+        assert Q.lib.qed2d_ctx_set_tile_f32(prm.ctx(), rows) == 0
+        so = torch.zeros((2, L2, L1), dtype=torch.complex64, device="cuda")
+        Q.gamm5Dw_f32(so, dU, dsi, 0.2, prm)
+        assert relerr(host(so), ref) < 2e-6, rows
 
 
 @pytest.mark.parametrize("k", [1, 3, 16])

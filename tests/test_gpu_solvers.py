@@ -19,7 +19,7 @@ def _true_res(O, U, x, b, am0, mu=0.0, bc=0):
     return O.norm2(b - Ax) / O.norm2(b)
 
 
-@pytest.mark.parametrize("L1,L2", [(24, 24), (64, 40)])
+@pytest.mark.parametrize("L1,L2", [(24, 24), (64, 40), (33, 20)])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_cg_v1(Q, O, L1, L2, prec):
     say("CG v1", L1, L2, "prec", prec)
@@ -68,6 +68,28 @@ def test_cg_v2_shift_and_guess(Q, O):
     Q.CG(x, dU, db, 0.5, 0.6, 2000, 1e-22, Q.gamm5Dw_sqr_musq, prm, Q.KernelParm(),
          precision=Q.PREC_MIXED)
     assert _true_res(O, U, host(x), b, 0.5, 0.6) < 1e-22
+
+
+@pytest.mark.parametrize("rows", [1, 2, 8, 16])
+def test_cg_tiles(Q, O, rows):
+    """CG (FP64 and mixed) gives the oracle's solution for every tile height."""
+    say("CG tiles", rows)
+    L1, L2 = 48, 36
+    rng = np.random.default_rng(rows)
+    U = random_links(L1, L2, rng, unitary=True)
+    b = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 4.0)
+    Q.lib.qed2d_ctx_set_tile(prm.ctx(), rows)
+    Q.lib.qed2d_ctx_set_tile_f32(prm.ctx(), rows)
+    xr, itr, st = O.cg(U, b, 0.3, 2000, 1e-20)
+    for prec in (0, 1):
+        x = Q.spinor(prm)
+        it = Q.CG(x, dev(U), dev(b), 0.3, 2000, 1e-20, Q.gamm5Dw_sqr_msq, prm, None,
+                  precision=prec)
+        assert _true_res(O, U, host(x), b, 0.3) < 1e-20
+        assert relerr(host(x), xr) < 1e-8
+        if prec == 0:
+            assert abs(it - itr) <= 1
 
 
 def test_cg_not_converged_raises(Q):
