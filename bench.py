@@ -61,7 +61,8 @@ def main():
     ap.add_argument("--therm", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tile", type=int, default=4)
+    ap.add_argument("--tile", type=int, default=0, help="FP64 tile rows (0: library default)")
+    ap.add_argument("--tile32", type=int, default=0, help="FP32 tile rows (0: library default)")
     ap.add_argument("--no-dw", action="store_true")
     args = ap.parse_args()
 
@@ -78,7 +79,10 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     L = args.L
     prm = Q.LattParm((L, L), args.beta)
-    Q.lib.qed2d_ctx_set_tile(prm.ctx(), args.tile)
+    if args.tile:
+        Q.lib.qed2d_ctx_set_tile(prm.ctx(), args.tile)
+    if args.tile32:
+        Q.lib.qed2d_ctx_set_tile_f32(prm.ctx(), args.tile32)
     seed = 1234
 
     # ---- start configuration: hot start + pure-gauge thermalisation --------
@@ -206,7 +210,10 @@ def dw_apply(Q, torch, args):
               (1024, 1, "f32"), (1024, 16, "f32"), (4096, 1, "f32")]
     for L, k, dt in shapes:
         prm = Q.LattParm((L, L), 4.0)
-        Q.lib.qed2d_ctx_set_tile(prm.ctx(), args.tile)
+        if args.tile:
+            Q.lib.qed2d_ctx_set_tile(prm.ctx(), args.tile)
+        if args.tile32:
+            Q.lib.qed2d_ctx_set_tile_f32(prm.ctx(), args.tile32)
         cdt = torch.complex128 if dt == "f64" else torch.complex64
         U = torch.exp(1j * torch.rand((2, L, L), dtype=torch.float64, device="cuda") * 6.283)
         U = U.to(cdt)

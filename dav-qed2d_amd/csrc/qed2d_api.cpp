@@ -36,7 +36,8 @@ struct qed2d_ctx {
     hipStream_t stream = nullptr;
     int sync = 1;
     int rows = 16;   // rows marched per thread, FP64 stencil kernels
-    int rows32 = 4;  // rows marched per thread, FP32 stencil kernels
+    int rows32 = 8;  // rows marched per thread, FP32 stencil kernels
+    double ru_delta = 0.01;  // mixed CG: reliable update after |r| drops by this factor
     std::string err;
     double2 *spin[S_NSLOTS] = {};
     float2 *fspin[FS_NSLOTS] = {};
@@ -330,7 +331,7 @@ int upload_state(qed2d_ctx *c, const SolveArgs &a)
     s.eps = a.eps;
     s.maxiter = a.maxiter;
     s.guess = a.guess;
-    s.delta2 = 1e-2;  // reliable update once |r| has dropped 10x since the last one
+    s.delta2 = c->ru_delta * c->ru_delta;
     HIPCHK(c, hipMemcpyAsync(c->state, c->hstate, sizeof(SolverState), hipMemcpyHostToDevice,
                              c->stream));
     return QED2D_OK;
@@ -936,6 +937,15 @@ int qed2d_ctx_set_tile(qed2d_ctx *c, int rows)
     if (rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16)
         return fail(c, QED2D_EINVAL, "tile rows must be 1, 2, 4, 8 or 16 (got %d)", rows);
     c->rows = rows;
+    return QED2D_OK;
+}
+
+int qed2d_ctx_set_reliable_delta(qed2d_ctx *c, double delta)
+{
+    if (!c) return QED2D_EINVAL;
+    if (!(delta > 0.0 && delta < 1.0))
+        return fail(c, QED2D_EINVAL, "reliable-update delta must be in (0, 1)");
+    c->ru_delta = delta;
     return QED2D_OK;
 }
 

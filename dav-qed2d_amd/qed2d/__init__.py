@@ -54,6 +54,7 @@ SIGNATURES = {
     "qed2d_ctx_set_sync": (_i, [_p, _i]),
     "qed2d_ctx_set_tile": (_i, [_p, _i]),
     "qed2d_ctx_set_tile_f32": (_i, [_p, _i]),
+    "qed2d_ctx_set_reliable_delta": (_i, [_p, _d]),
     "qed2d_synchronize": (_i, [_p]),
     "qed2d_gamm5Dw": (_i, [_p, _p, _p, _p, _d]),
     "qed2d_gamm5Dw_f32": (_i, [_p, _p, _p, _p, _f]),

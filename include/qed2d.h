@@ -78,6 +78,9 @@ int qed2d_ctx_set_sync(qed2d_ctx *ctx, int sync);
  * or 16). */
 int qed2d_ctx_set_tile(qed2d_ctx *ctx, int rows);
 int qed2d_ctx_set_tile_f32(qed2d_ctx *ctx, int rows);
+/* Mixed-precision CG: a reliable update (FP64 true residual) once the FP32
+ * iterated residual has dropped by delta since the last one (default 0.01). */
+int qed2d_ctx_set_reliable_delta(qed2d_ctx *ctx, double delta);
 int qed2d_synchronize(qed2d_ctx *ctx);
 
 /* ---- Dirac operator -------------------------------------------------------- */
