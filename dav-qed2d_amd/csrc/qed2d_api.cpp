@@ -24,7 +24,7 @@ using namespace qed2d;
 // ---------------------------------------------------------------------------
 enum SpinorSlot {
     S_R, S_P0, S_P1, S_T, S_TMP, S_TMP2, S_UCP, S_X, S_F, S_G5DX, S_AUX, S_PB, S_PBA, S_PBL,
-    S_NSLOTS
+    S_RW_X, S_RW_Z, S_RW_IN, S_RW_T1, S_RW_T2, S_NSLOTS
 };
 enum FloatSlot { FS_R, FS_P0, FS_P1, FS_T, FS_X, FS_U, FS_NSLOTS };
 enum RealSlot { R_MOM, R_FACC, R_NSLOTS };
@@ -839,6 +839,34 @@ int gen_pf_impl(qed2d_ctx *c, double *F, const double *U, const double *X, doubl
     return launched(c);
 }
 
+// so = (A / r_b) (si + sum_j rho_j (Q^2 + mu_j^2)^{-1} si)  (src/QED2dRHMC.jl:261-271)
+int r_impl(qed2d_ctx *c, double *so, const double *U, const double *si, double am0, int n,
+           const double *mu, const double *rho, double A, double r_b, int maxiter, double eps)
+{
+    TRY(multicg_impl(c, so, U, si, am0, n, mu, rho, maxiter, eps));
+    // so .= r_b^(-1) * A * so  (src/QED2dRHMC.jl:268)
+    const double f = std::pow(r_b, -1.0) * A;
+    launch_scale_add((double2 *)so, f, (const double2 *)so, 0.0, spinor_elems(c), c->stream);
+    return launched(c);
+}
+
+// Z = (Q^2 R^2 - 1) X  (src/QED2dRHMC.jl:174-186).  Z must not alias X; the
+// last Q application and the subtraction of X share one stencil pass.
+int luscher_z_impl(qed2d_ctx *c, double *Z, const double *U, const double *X, double am0,
+                   int n, const double *mu, const double *rho, double A, double r_b,
+                   int maxiter, double tol)
+{
+    double2 *t1, *t2;
+    TRY(get_spin(c, S_RW_T1, &t1));
+    TRY(get_spin(c, S_RW_T2, &t2));
+    TRY(r_impl(c, (double *)t1, U, X, am0, n, mu, rho, A, r_b, maxiter, tol));
+    TRY(r_impl(c, (double *)t2, U, (const double *)t1, am0, n, mu, rho, A, r_b, maxiter, tol));
+    q64(c, U, (const double *)t2, (double *)t1, am0);
+    launch_q_shift_f64(c->L, (const double2 *)U, t1, (const double2 *)X, (double2 *)Z, am0, -1.0,
+                       c->rows, c->stream);
+    return launched(c);
+}
+
 bool overlaps(const void *a, const void *b) { return a && b && a == b; }
 
 }  // namespace
@@ -1227,10 +1255,71 @@ int qed2d_multicg(qed2d_ctx *c, double *so, const double *U, const double *si, d
 int qed2d_R(qed2d_ctx *c, double *so, const double *U, const double *si, double am0, int n,
             const double *mu, const double *rho, double A, double r_b, int maxiter, double eps)
 {
-    TRY(qed2d_multicg(c, so, U, si, am0, n, mu, rho, maxiter, eps));
-    // so .= r_b^(-1) * A * so  (src/QED2dRHMC.jl:268)
-    const double f = std::pow(r_b, -1.0) * A;
-    launch_scale_add((double2 *)so, f, (const double2 *)so, 0.0, spinor_elems(c), c->stream);
+    CHECK_CTX(c);
+    NEED(c, so); NEED(c, U); NEED(c, si);
+    NOALIAS(c, so, si);
+    if (n < 0 || (n > 0 && (!mu || !rho))) return fail(c, QED2D_EINVAL, "bad pole arrays");
+    TRY(r_impl(c, so, U, si, am0, n, mu, rho, A, r_b, maxiter, eps));
+    return finish(c);
+}
+
+int qed2d_luscher_z(qed2d_ctx *c, double *Z, int p, const double *U, const double *X, double am0,
+                    int n, const double *mu, const double *rho, double A, double r_b,
+                    int maxiter, double tol)
+{
+    CHECK_CTX(c);
+    NEED(c, Z); NEED(c, U); NEED(c, X);
+    if (p < 0) return fail(c, QED2D_EINVAL, "power p must be >= 0");
+    if (n < 0 || (n > 0 && (!mu || !rho))) return fail(c, QED2D_EINVAL, "bad pole arrays");
+    double2 *in;
+    TRY(get_spin(c, S_RW_IN, &in));
+    // src/QED2dRHMC.jl:189-201: tmp = X; repeat p times Z = Z(tmp), tmp = Z
+    launch_copy(in, (const double2 *)X, spinor_elems(c), c->stream);
+    for (int i = 0; i < p; ++i) {
+        TRY(luscher_z_impl(c, Z, U, (const double *)in, am0, n, mu, rho, A, r_b, maxiter, tol));
+        launch_copy(in, (const double2 *)Z, spinor_elems(c), c->stream);
+    }
+    return finish(c);
+}
+
+int qed2d_reweighting_factor(qed2d_ctx *c, const double *U, double am0, int n, const double *mu,
+                             const double *rho, double A, double r_b, int N, int taylor,
+                             int maxiter, double tol, uint64_t seed, uint32_t chain,
+                             uint32_t traj, double *W)
+{
+    CHECK_CTX(c);
+    NEED(c, U); NEED(c, W);
+    if (N < 1 || taylor < 0) return fail(c, QED2D_EINVAL, "need N >= 1 and taylor >= 0");
+    if ((long)N > (1L << 30)) return fail(c, QED2D_EINVAL, "N too large");
+    if (n < 0 || (n > 0 && (!mu || !rho))) return fail(c, QED2D_EINVAL, "bad pole arrays");
+    double2 *X, *Z, *in;
+    TRY(get_spin(c, S_RW_X, &X));
+    TRY(get_spin(c, S_RW_Z, &Z));
+    TRY(get_spin(c, S_RW_IN, &in));
+    const long n2 = spinor_elems(c);
+    uint32_t k0, k1;
+    key_of(seed, chain, &k0, &k1);
+    double w = 0.0;
+    // src/QED2dRHMC.jl:215-228
+    for (int j = 0; j < N; ++j) {
+        double argexp = 0.0, tf = 0.5;
+        launch_fill_cgauss(c->L, X, k0, k1, traj, kPurposeRew, 2u * (uint32_t)j, c->stream);
+        launch_copy(in, X, n2, c->stream);
+        for (int i = 1; i <= taylor; ++i) {
+            TRY(luscher_z_impl(c, (double *)Z, U, (const double *)in, am0, n, mu, rho, A, r_b,
+                               maxiter, tol));
+            double re, im;
+            TRY(dot_host(c, (const double *)X, (const double *)Z, &re, &im));
+            argexp += tf * re;
+            // Tfactor *= (-1)*(2i+1)/2/factorial(i+1)*factorial(i), left to right
+            double fi = 1.0;
+            for (int q = 2; q <= i; ++q) fi *= q;
+            tf *= (double)(-(2 * i + 1)) / 2.0 / (fi * (i + 1)) * fi;
+            launch_copy(in, Z, n2, c->stream);
+        }
+        w += std::exp(argexp) / N;
+    }
+    *W = w;
     return finish(c);
 }
 

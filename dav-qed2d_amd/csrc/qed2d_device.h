@@ -254,7 +254,7 @@ __device__ __host__ inline void gauss_pair(uint32_t idx, uint32_t sub, uint32_t 
 }
 
 enum : uint32_t { kPurposeMom = 0x10, kPurposePF = 0x20, kPurposeMetro = 0x30, kPurposeHot = 0x40,
-                  kPurposeVec = 0x50 };
+                  kPurposeVec = 0x50, kPurposeRew = 0x60 };
 
 // XCD-aware block order: the hardware deals consecutive workgroups to the 8
 // XCDs round robin; this mapping gives every XCD a contiguous band of tiles

@@ -165,6 +165,8 @@ void launch_negate(double *a, long n, hipStream_t st);
 // RNG fills.
 void launch_fill_momenta(const Latt &L, double *mom, uint32_t k0, uint32_t k1, uint32_t traj,
                          hipStream_t st);
+void launch_fill_cgauss(const Latt &L, double2 *X, uint32_t k0, uint32_t k1, uint32_t traj,
+                        uint32_t purpose, uint32_t sub0, hipStream_t st);
 void launch_fill_pseudofermion(const Latt &L, double2 *X, uint32_t k0, uint32_t k1,
                                uint32_t traj, uint32_t flavour, hipStream_t st);
 void launch_hot_start(const Latt &L, double2 *U, uint32_t k0, uint32_t k1, hipStream_t st);

@@ -638,8 +638,8 @@ void launch_fill_momenta(const Latt &L, double *mom, uint32_t k0, uint32_t k1, u
                        k1, traj);
 }
 
-__global__ void k_fill_pf(double2 *__restrict__ X, long V, uint32_t k0, uint32_t k1,
-                          uint32_t traj, uint32_t flavour)
+__global__ void k_fill_cgauss(double2 *__restrict__ X, long V, uint32_t k0, uint32_t k1,
+                              uint32_t traj, uint32_t purpose, uint32_t sub0)
 {
 #pragma clang fp contract(off)
     const double sq2 = 1.4142135623730951;
@@ -647,16 +647,21 @@ __global__ void k_fill_pf(double2 *__restrict__ X, long V, uint32_t k0, uint32_t
          s += (long)gridDim.x * blockDim.x) {
         for (uint32_t c = 0; c < 2; ++c) {
             double g1, g2;
-            gauss_pair((uint32_t)s, c, kPurposePF + flavour, traj, k0, k1, &g1, &g2);
+            gauss_pair((uint32_t)s, sub0 + c, purpose, traj, k0, k1, &g1, &g2);
             X[c * V + s] = mk<double>(g1 / sq2, g2 / sq2);
         }
     }
 }
+void launch_fill_cgauss(const Latt &L, double2 *X, uint32_t k0, uint32_t k1, uint32_t traj,
+                        uint32_t purpose, uint32_t sub0, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_cgauss, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, X, L.V, k0, k1,
+                       traj, purpose, sub0);
+}
 void launch_fill_pseudofermion(const Latt &L, double2 *X, uint32_t k0, uint32_t k1,
                                uint32_t traj, uint32_t flavour, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fill_pf, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, X, L.V, k0, k1, traj,
-                       flavour);
+    launch_fill_cgauss(L, X, k0, k1, traj, kPurposePF + flavour, 0, st);
 }
 
 __global__ void k_hot_start(double2 *__restrict__ U, long V, uint32_t k0, uint32_t k1)

@@ -83,6 +83,9 @@ SIGNATURES = {
     "qed2d_multicg": (_i, [_p, _p, _p, _p, _d, _i, _pd, _pd, _i, _d]),
     "qed2d_R": (_i, [_p, _p, _p, _p, _d, _i, _pd, _pd, _d, _d, _i, _d]),
     "qed2d_generate_pseudofermion": (_i, [_p, _p, _p, _p, _d, _i, _pd, _pd, _i, _d]),
+    "qed2d_luscher_z": (_i, [_p, _p, _i, _p, _p, _d, _i, _pd, _pd, _d, _d, _i, _d]),
+    "qed2d_reweighting_factor": (_i, [_p, _p, _d, _i, _pd, _pd, _d, _d, _i, _i, _i, _d, _u64,
+                                      _u32, _u32, _pd]),
     "qed2d_power_method": (_i, [_p, _p, _p, _d, _i, _pd, _pd]),
     "qed2d_zolotarev": (_i, [_i, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd]),
     "qed2d_fill_momenta": (_i, [_p, _p, _u64, _u32, _u32]),
@@ -538,6 +541,37 @@ def generate_pseudofermion(F, U, X, am0, CGmaxiter, CGtol, prm, kprm, rprm):
     _check(prm, lib.qed2d_generate_pseudofermion(prm.ctx(), _sp(prm, F), _sp(prm, U),
                                                  _sp(prm, X), am0, rprm.n, _arr(rprm.mu),
                                                  _arr(rprm.nu), int(CGmaxiter), CGtol))
+
+
+def LuscherZp(Z, p, U, X_in, am0, CGmaxiter, CGtol, rprm, prm, kprm=None):
+    """LuscherZp (src/QED2dRHMC.jl:189-201): Z = (Q^2 R^2 - 1)^p X_in."""
+    _check(prm, lib.qed2d_luscher_z(prm.ctx(), _sp(prm, Z), int(p), _sp(prm, U), _sp(prm, X_in),
+                                    am0, rprm.n, _arr(rprm.mu), _arr(rprm.rho), rprm.A, rprm.r_b,
+                                    int(CGmaxiter), CGtol))
+
+
+def LuscherZ(Z, U, X_in, am0, CGmaxiter, CGtol, rprm, prm, kprm=None):
+    """LuscherZ (src/QED2dRHMC.jl:174-186): Z = (Q^2 R^2 - 1) X_in."""
+    LuscherZp(Z, 1, U, X_in, am0, CGmaxiter, CGtol, rprm, prm, kprm)
+
+
+def reweighting_factor(U, am0, CGmaxiter, CGtol, prm, kprm, rprm):
+    """reweighting_factor (src/QED2dRHMC.jl:210-251).  With lists of masses and
+    RHMCParm, the product over fermions.  The Gaussian fields come from the
+    build's stream (seed(); one trajectory index per fermion)."""
+    if isinstance(rprm, (list, tuple)):
+        w = 1.0
+        for m, r in zip(am0, rprm):
+            w *= reweighting_factor(U, m, CGmaxiter, CGtol, prm, kprm, r)
+        return w
+    W = _d()
+    _check(prm, lib.qed2d_reweighting_factor(prm.ctx(), _sp(prm, U), am0, rprm.n, _arr(rprm.mu),
+                                             _arr(rprm.rho), rprm.A, rprm.r_b,
+                                             int(rprm.reweighting_N),
+                                             int(rprm.reweighting_Taylor), int(CGmaxiter), CGtol,
+                                             _rng.seed, _rng.chain, _rng.next_traj(),
+                                             C.byref(W)))
+    return W.value
 
 
 def power_method(U, am0, prm, kprm=None, iter=1000, b0=None):

@@ -182,6 +182,22 @@ int qed2d_generate_pseudofermion(qed2d_ctx *ctx, double *F, const double *U, con
 int qed2d_power_method(qed2d_ctx *ctx, const double *U, const double *b0, double am0, int iter,
                        double *lambda_min, double *lambda_max);
 
+/* LuscherZp(Z, p, U, X_in, am0, CGmaxiter, CGtol, rprm, prm, kprm)
+ * (src/QED2dRHMC.jl:189-201; p = 1 is LuscherZ, :174-186): Z = (Q^2 R^2 - 1)^p X.
+ * Z must not alias X.  p = 0 leaves Z unchanged. */
+int qed2d_luscher_z(qed2d_ctx *ctx, double *Z, int p, const double *U, const double *X,
+                    double am0, int n, const double *mu, const double *rho, double A, double r_b,
+                    int maxiter, double tol);
+/* reweighting_factor(U, am0, CGmaxiter, CGtol, prm, kprm, rprm)
+ * (src/QED2dRHMC.jl:210-239): W_N from N Gaussian fields and a Taylor expansion of
+ * order `taylor` (RHMCParm.reweighting_N / reweighting_Taylor).  The j-th field is
+ * drawn from the (seed, chain) stream at trajectory `traj` (it replaces CUDA.randn
+ * at :218).  The product over several fermions (:242-251) is left to the caller. */
+int qed2d_reweighting_factor(qed2d_ctx *ctx, const double *U, double am0, int n,
+                             const double *mu, const double *rho, double A, double r_b, int N,
+                             int taylor, int maxiter, double tol, uint64_t seed, uint32_t chain,
+                             uint32_t traj, double *W);
+
 /* ---- Zolotarev coefficients (host only) ---------------------------------------
  * get_rhmc_params(n, r_a, r_b) (src/QED2dRHMC.jl:93-110) -> RHMCParm fields
  * (src/QED2d.jl:15-26).  rho, mu, nu hold n doubles each. */

@@ -158,10 +158,10 @@ void orc_fill_momenta(double *mom, int L1, int L2, uint64_t seed, uint32_t chain
     }
 }
 
-/* X = (randn + i randn)/sqrt(2) (src/QED2dHMC.jl:70,138): one Philox block per
- * site and spin component. */
-void orc_fill_pseudofermion(double *X, int L1, int L2, uint64_t seed, uint32_t chain,
-                            uint32_t traj, uint32_t flavour)
+/* X = (randn + i randn)/sqrt(2) with the stream of one purpose: one Philox
+ * block per site and spin component. */
+void orc_fill_cgauss(double *X, int L1, int L2, uint64_t seed, uint32_t chain, uint32_t traj,
+                     uint32_t purpose, uint32_t sub0)
 {
     long V = (long)L1 * L2;
     uint32_t key[2];
@@ -170,13 +170,20 @@ void orc_fill_pseudofermion(double *X, int L1, int L2, uint64_t seed, uint32_t c
 #pragma omp parallel for
     for (long s = 0; s < V; ++s) {
         for (uint32_t c = 0; c < 2; ++c) {
-            uint32_t ctr[4] = {(uint32_t)s, c, PURPOSE_PF + flavour, traj};
+            uint32_t ctr[4] = {(uint32_t)s, sub0 + c, purpose, traj};
             double g1, g2;
             orc_gauss_pair(ctr, key, &g1, &g2);
             X[2 * (c * V + s)] = g1 / sq2;
             X[2 * (c * V + s) + 1] = g2 / sq2;
         }
     }
+}
+
+/* Pseudofermion noise (src/QED2dHMC.jl:70,138). */
+void orc_fill_pseudofermion(double *X, int L1, int L2, uint64_t seed, uint32_t chain,
+                            uint32_t traj, uint32_t flavour)
+{
+    orc_fill_cgauss(X, L1, L2, seed, chain, traj, PURPOSE_PF + flavour, 0);
 }
 
 /* r = rand() of the Metropolis step (src/QED2dHMC.jl:42,95,177). */
@@ -1116,6 +1123,78 @@ int orc_hmc_rhmc(double *U, const double *am0, int nf, const int *npole, double 
     for (int j = 0; j < nf; ++j) free(F[j]);
     free(F); free(Ucp); free(mom); free(X); free(g5DX);
     return bad;
+}
+
+/* Z X = (Q^2 R^2 - 1) X: src/QED2dRHMC.jl:174-186. */
+int orc_luscher_z(double *Z, const double *U, const double *X, double am0, int n,
+                  const double *mu, const double *rho, double A, double r_b, int maxiter,
+                  double tol, int L1, int L2, int bc)
+{
+    long m = 4L * L1 * L2;
+    double *tmp = (double *)malloc(sizeof(double) * m);
+    int bad = orc_R(Z, U, X, am0, n, mu, rho, A, r_b, maxiter, tol, L1, L2, bc);
+    memcpy(tmp, Z, sizeof(double) * m);
+    bad |= orc_R(Z, U, tmp, am0, n, mu, rho, A, r_b, maxiter, tol, L1, L2, bc);
+    memcpy(tmp, Z, sizeof(double) * m);
+    orc_gamm5Dw(Z, U, tmp, am0, L1, L2, bc);
+    memcpy(tmp, Z, sizeof(double) * m);
+    orc_gamm5Dw(Z, U, tmp, am0, L1, L2, bc);
+    for (long i = 0; i < m; ++i) Z[i] = Z[i] - X[i];
+    free(tmp);
+    return bad;
+}
+
+/* Z^p X: src/QED2dRHMC.jl:189-201. */
+int orc_luscher_zp(double *Z, int pw, const double *U, const double *X, double am0, int n,
+                   const double *mu, const double *rho, double A, double r_b, int maxiter,
+                   double tol, int L1, int L2, int bc)
+{
+    long m = 4L * L1 * L2;
+    double *tmp = (double *)malloc(sizeof(double) * m);
+    int bad = 0;
+    memcpy(tmp, X, sizeof(double) * m);
+    for (int i = 0; i < pw; ++i) {
+        bad |= orc_luscher_z(Z, U, tmp, am0, n, mu, rho, A, r_b, maxiter, tol, L1, L2, bc);
+        memcpy(tmp, Z, sizeof(double) * m);
+    }
+    free(tmp);
+    return bad;
+}
+
+/* Reweighting factor W_N (Luescher eq. 4.1): src/QED2dRHMC.jl:210-239.  The
+ * j-th Gaussian field X (drawn with CUDA.randn at :218 in the reference) comes
+ * from the Philox stream with purpose 0x60 and spin sub-counter 2j + c. */
+double orc_reweighting_factor(const double *U, double am0, int n, const double *mu,
+                              const double *rho, double A, double r_b, int N, int taylor,
+                              int maxiter, double tol, uint64_t seed, uint32_t chain,
+                              uint32_t traj, int L1, int L2, int bc, int *status)
+{
+    long V = (long)L1 * L2, m = 4 * V;
+    double *X = (double *)malloc(sizeof(double) * m);
+    double *ZpX = (double *)malloc(sizeof(double) * m);
+    double *tmp = (double *)malloc(sizeof(double) * m);
+    double W = 0.0;
+    *status = 0;
+    for (int j = 0; j < N; ++j) {
+        double argexp = 0.0, Tfactor = 0.5;
+        orc_fill_cgauss(X, L1, L2, seed, chain, traj, 0x60u, 2u * (uint32_t)j);
+        memcpy(tmp, X, sizeof(double) * m);
+        for (int i = 1; i <= taylor; ++i) {
+            *status |= orc_luscher_z(ZpX, U, tmp, am0, n, mu, rho, A, r_b, maxiter, tol, L1, L2, bc);
+            double re, im;
+            orc_dot(X, ZpX, 2 * V, &re, &im);
+            argexp += Tfactor * re;
+            /* (-1)*(2i+1)/2/factorial(i+1)*factorial(i), evaluated left to right */
+            double fi = 1.0, fi1;
+            for (int q = 2; q <= i; ++q) fi *= q;
+            fi1 = fi * (i + 1);
+            Tfactor *= (double)(-1 * (2 * i + 1)) / 2.0 / fi1 * fi;
+            memcpy(tmp, ZpX, sizeof(double) * m);
+        }
+        W += exp(argexp) / N;
+    }
+    free(X); free(ZpX); free(tmp);
+    return W;
 }
 
 /* Lattice doubling: src/QED2dFields.jl:21-50 (reflect = conjugated copies). */

@@ -48,6 +48,8 @@ void orc_det_sincos2pi(double u, double *s, double *c);
 void orc_gauss_pair(const uint32_t ctr[4], const uint32_t key[2], double *g1, double *g2);
 double orc_uniform(const uint32_t ctr[4], const uint32_t key[2]);
 void orc_fill_momenta(double *mom, int L1, int L2, uint64_t seed, uint32_t chain, uint32_t traj);
+void orc_fill_cgauss(double *X, int L1, int L2, uint64_t seed, uint32_t chain, uint32_t traj,
+                     uint32_t purpose, uint32_t sub0);
 void orc_fill_pseudofermion(double *X, int L1, int L2, uint64_t seed, uint32_t chain,
                             uint32_t traj, uint32_t flavour);
 double orc_metropolis_uniform(uint64_t seed, uint32_t chain, uint32_t traj);
@@ -150,6 +152,18 @@ int orc_hmc_rhmc(double *U, const double *am0, int nf, const int *npole, double 
                  int maxiter, double tol, int qzero, int reversibility_test, uint64_t seed,
                  uint32_t chain, uint32_t traj, const orc_latt *p, double *hini, double *hfin,
                  int *accepted);
+
+/* ---- reweighting (src/QED2dRHMC.jl:174-251) ------------------------------ */
+int orc_luscher_z(double *Z, const double *U, const double *X, double am0, int n,
+                  const double *mu, const double *rho, double A, double r_b, int maxiter,
+                  double tol, int L1, int L2, int bc);
+int orc_luscher_zp(double *Z, int pw, const double *U, const double *X, double am0, int n,
+                   const double *mu, const double *rho, double A, double r_b, int maxiter,
+                   double tol, int L1, int L2, int bc);
+double orc_reweighting_factor(const double *U, double am0, int n, const double *mu,
+                              const double *rho, double A, double r_b, int N, int taylor,
+                              int maxiter, double tol, uint64_t seed, uint32_t chain,
+                              uint32_t traj, int L1, int L2, int bc, int *status);
 
 /* ---- lattice I/O and master-field growth -------------------------------- */
 void orc_unfold(double *Uout, const double *Uin, int L1, int L2, int reflect);

@@ -79,6 +79,11 @@ _sig("orc_ellipk", _d, [_d])
 _sig("orc_ellip_sncn", None, [_d, _d, _pd, _pd])
 _sig("orc_zolotarev", None, [_i, _d, _d, _pd, _pd, _pd, _p, _p, _p])
 _sig("orc_unfold", None, [_p, _p, _i, _i, _i])
+_sig("orc_fill_cgauss", None, [_p, _i, _i, _u64, _u32, _u32, _u32, _u32])
+_sig("orc_luscher_z", _i, [_p, _p, _p, _d, _i, _p, _p, _d, _d, _i, _d, _i, _i, _i])
+_sig("orc_luscher_zp", _i, [_p, _i, _p, _p, _d, _i, _p, _p, _d, _d, _i, _d, _i, _i, _i])
+_sig("orc_reweighting_factor", _d, [_p, _d, _i, _p, _p, _d, _d, _i, _i, _i, _d, _u64, _u32, _u32,
+                                    _i, _i, _i, C.POINTER(_i)])
 
 
 class Latt(C.Structure):
@@ -139,6 +144,12 @@ def momenta(L1, L2, seed, chain=0, traj=0):
 def pseudofermion(L1, L2, seed, chain=0, traj=0, flavour=0):
     x = _cspinor(L1, L2)
     lib.orc_fill_pseudofermion(_ptr(x), L1, L2, seed, chain, traj, flavour)
+    return x
+
+
+def cgauss(L1, L2, seed, chain, traj, purpose, sub0=0):
+    x = _cspinor(L1, L2)
+    lib.orc_fill_cgauss(_ptr(x), L1, L2, seed, chain, traj, purpose, sub0)
     return x
 
 
@@ -389,6 +400,28 @@ def hmc_rhmc(U, am0s, zs, eps, ns, maxiter, tol, p, seed, traj, chain=0, qzero=0
                           C.byref(p),
                           C.byref(hi), C.byref(hf), C.byref(acc))
     return hi.value, hf.value, acc.value, rc
+
+
+def luscher_z(U, X, am0, z, maxiter, tol, p=1, bc=0):
+    """Z^p X with Z = Q^2 R^2 - 1."""
+    L1, L2 = _shape(X)
+    Z = _cspinor(L1, L2)
+    mu = np.ascontiguousarray(z["mu"], np.float64)
+    rho = np.ascontiguousarray(z["rho"], np.float64)
+    bad = lib.orc_luscher_zp(_ptr(Z), p, _ptr(U), _ptr(np.ascontiguousarray(X)), am0, len(mu),
+                             _ptr(mu), _ptr(rho), z["A"], z["r_b"], maxiter, tol, L1, L2, bc)
+    return Z, bad
+
+
+def reweighting_factor(U, am0, z, N, taylor, maxiter, tol, seed, chain=0, traj=0, bc=0):
+    L1, L2 = _shape(U)
+    mu = np.ascontiguousarray(z["mu"], np.float64)
+    rho = np.ascontiguousarray(z["rho"], np.float64)
+    st = _i()
+    W = lib.orc_reweighting_factor(_ptr(U), am0, len(mu), _ptr(mu), _ptr(rho), z["A"], z["r_b"],
+                                   N, taylor, maxiter, tol, seed, chain, traj, L1, L2, bc,
+                                   C.byref(st))
+    return W, st.value
 
 
 def unfold(U, reflect=False):

@@ -314,3 +314,49 @@ def test_multicg_multishift_and_sequential_agree(Q, O):
     Q.set_multishift(prm, True)
     ref, bad = O.multicg(U, X, 0.3, z["mu"], z["rho"], 5000, 1e-24)
     assert relerr(host(a), ref) < 1e-10 and relerr(host(b), ref) < 1e-10
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_luscher_z_vs_oracle(Q, O, p):
+    say("LuscherZp", p)
+    L1, L2 = 24, 16
+    rng = np.random.default_rng(31 + p)
+    U = random_links(L1, L2, rng, unitary=True)
+    X = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 4.0)
+    rp = Q.get_rhmc_params(3, np.sqrt(0.02), np.sqrt(70.0))
+    z = O.zolotarev(3, np.sqrt(0.02), np.sqrt(70.0))
+    Z = Q.spinor(prm)
+    Q.LuscherZp(Z, p, dev(U), dev(X), 0.25, 10000, 1e-26, rp, prm)
+    ref, bad = O.luscher_z(U, X, 0.25, z, 10000, 1e-26, p=p)
+    assert bad == 0
+    # Z^p X = O(delta^p) X: the error is measured against X
+    assert np.max(np.abs(host(Z) - ref)) / np.max(np.abs(X)) < 1e-11
+    if p == 1:
+        Z1 = Q.spinor(prm)
+        Q.LuscherZ(Z1, dev(U), dev(X), 0.25, 10000, 1e-26, rp, prm)
+        assert np.array_equal(host(Z1), host(Z))
+
+
+def test_reweighting_factor_vs_oracle(Q, O):
+    say("reweighting_factor")
+    L1, L2 = 32, 24
+    rng = np.random.default_rng(41)
+    U = random_links(L1, L2, rng, unitary=True)
+    prm = Q.LattParm((L1, L2), 4.0)
+    rp = Q.get_rhmc_params(3, np.sqrt(0.02), np.sqrt(70.0), reweighting_N=3, reweighting_Taylor=4)
+    z = O.zolotarev(3, np.sqrt(0.02), np.sqrt(70.0))
+    dU = dev(U)
+    Q.seed(4242, 1, 7)
+    W = Q.reweighting_factor(dU, 0.25, 10000, 1e-26, prm, None, rp)
+    Wr, st = O.reweighting_factor(U, 0.25, z, 3, 4, 10000, 1e-26, 4242, 1, 7)
+    assert st == 0
+    assert abs(W - Wr) < 1e-10 * abs(Wr)
+    assert abs(W - 1.0) > 1e-8
+    # list form: product over fermions, one trajectory index each
+    Q.seed(4242, 1, 7)
+    rp2 = Q.get_rhmc_params(4, np.sqrt(0.02), np.sqrt(70.0))
+    z2 = O.zolotarev(4, np.sqrt(0.02), np.sqrt(70.0))
+    Wl = Q.reweighting_factor(dU, [0.25, 0.3], 10000, 1e-26, prm, None, [rp, rp2])
+    Wr2, _ = O.reweighting_factor(U, 0.3, z2, 1, 5, 10000, 1e-26, 4242, 1, 8)
+    assert abs(Wl - Wr * Wr2) < 1e-10 * abs(Wr * Wr2)
