@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -35,8 +36,12 @@ struct qed2d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int sync = 1;
-    int rows = 16;   // rows marched per thread, FP64 stencil kernels
-    int rows32 = 8;  // rows marched per thread, FP32 stencil kernels
+    // Rows marched per thread (the tile height) of each kernel family; the
+    // defaults come from default_tiles(), qed2d_ctx_autotune measures them.
+    int rows = 16;    // FP64 solver kernels (CG, multi-shift, residuals)
+    int rows32 = 8;   // FP32 solver kernels (mixed-precision CG)
+    int rows_q = 4;   // FP64 Q / Dw applies
+    int rows_q32 = 2; // FP32 Q apply of the public entry point
     double ru_delta = 0.01;  // mixed CG: reliable update after |r| drops by this factor
     std::string err;
     double2 *spin[S_NSLOTS] = {};
@@ -249,7 +254,7 @@ void q64(qed2d_ctx *c, const double *U, const double *si, double *so, double am0
 {
     PROF(c, QED2D_PROF_Q_F64,
          launch_q_apply_f64(c->L, (const double2 *)U, (const double2 *)si, (double2 *)so, am0,
-                            c->rows, c->stream));
+                            c->rows_q, c->stream));
 }
 
 // so = Q^2 si (+ mu^2 si), tmp = Q si
@@ -258,7 +263,7 @@ void qsq64(qed2d_ctx *c, const double *U, const double *si, double *so, double *
 {
     q64(c, U, si, tmp, am0);
     launch_q_shift_f64(c->L, (const double2 *)U, (const double2 *)tmp, (const double2 *)si,
-                       (double2 *)so, am0, mu2, c->rows, c->stream);
+                       (double2 *)so, am0, mu2, c->rows_q, c->stream);
 }
 
 int dot_host(qed2d_ctx *c, const double *a, const double *b, double *re, double *im)
@@ -451,7 +456,7 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             k = c->hstate->iter + 1;
             hipEvent_t ev = prof_begin(c, QED2D_PROF_RELIABLE);
             launch_ru_accum(L, (double2 *)x, xs, st);
-            launch_q_apply_f64(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->rows, st);
+            launch_q_apply_f64(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->rows_q, st);
             launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
                             nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, k, 2, c->rows,
                             st);
@@ -863,7 +868,111 @@ int luscher_z_impl(qed2d_ctx *c, double *Z, const double *U, const double *X, do
     TRY(r_impl(c, (double *)t2, U, (const double *)t1, am0, n, mu, rho, A, r_b, maxiter, tol));
     q64(c, U, (const double *)t2, (double *)t1, am0);
     launch_q_shift_f64(c->L, (const double2 *)U, t1, (const double2 *)X, (double2 *)Z, am0, -1.0,
-                       c->rows, c->stream);
+                       c->rows_q, c->stream);
+    return launched(c);
+}
+
+bool tile_ok(int r) { return r == 1 || r == 2 || r == 4 || r == 8 || r == 16; }
+
+// Tile heights measured best on MI355X (pipelined marches): applies are
+// bandwidth bound at any height >= 2; the FP64 CG kernels need tall tiles on
+// lattices that fit the 256 MB Infinity Cache and medium ones above.
+void default_tiles(qed2d_ctx *c)
+{
+    c->rows_q = 4;
+    c->rows_q32 = 2;
+    c->rows = (c->L.V <= (1L << 21)) ? 16 : 8;
+    c->rows32 = 8;
+}
+
+// Mean duration in microseconds of `reps` runs of fn (after one warm-up run)
+// on the context's stream; fn's nonzero statuses are ignored (the timed CG
+// runs are cut off by their iteration limit on purpose).
+int time_runs(qed2d_ctx *c, int reps, const std::function<void()> &fn, double *us)
+{
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    fn();
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i) fn();
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *us = 1e3 * ms / reps;
+    return QED2D_OK;
+}
+
+// The autotuner: times every tile height of each kernel family on this
+// context's lattice (random links and source, scratch buffers freed on
+// return) and keeps the fastest.  CG families are timed over `iters`
+// iterations of a solve that cannot converge (tolerance 0).
+int autotune_impl(qed2d_ctx *c, int iters)
+{
+    const Latt &L = c->L;
+    const long n2 = spinor_elems(c);
+    hipStream_t st = c->stream;
+    void *mem = nullptr;
+    if (hipMalloc(&mem, 3 * sizeof(double2) * n2 + 3 * sizeof(float2) * n2) != hipSuccess)
+        return fail(c, QED2D_ENOMEM, "autotune scratch allocation failed");
+    double2 *U = (double2 *)mem, *b = U + n2, *x = b + n2;
+    float2 *Uf = (float2 *)(x + n2), *bf = Uf + n2, *xf = bf + n2;
+    launch_hot_start(L, U, 0x7475u, 0x6e65u, st);
+    launch_fill_cgauss(L, b, 0x7475u, 0x6e65u, 0, kPurposeVec, 0, st);
+    launch_to_f32(U, Uf, n2, st);
+    launch_to_f32(b, bf, n2, st);
+    const int cand[5] = {1, 2, 4, 8, 16};
+    const int save_sync = c->sync;
+    c->sync = 0;
+    int rc = QED2D_OK;
+    double best, us;
+    int pick;
+    // FP64 apply
+    best = 1e30, pick = c->rows_q;
+    for (int R : cand) {
+        rc = time_runs(c, 5, [&] { launch_q_apply_f64(L, U, b, x, 0.1, R, st); }, &us);
+        if (rc) break;
+        if (us < best) best = us, pick = R;
+    }
+    if (!rc) c->rows_q = pick;
+    // FP32 apply
+    best = 1e30, pick = c->rows_q32;
+    for (int R : cand) {
+        if (rc) break;
+        rc = time_runs(c, 5, [&] { launch_q_apply_f32(L, Uf, bf, xf, 0.1f, R, st); }, &us);
+        if (us < best) best = us, pick = R;
+    }
+    if (!rc) c->rows_q32 = pick;
+    // CG kernels (one timed solve each, `iters` iterations)
+    SolveArgs a;
+    a.am0 = 0.1;
+    a.eps = 0.0;
+    a.maxiter = iters;
+    int it;
+    for (int prec = 0; prec < 2 && !rc; ++prec) {
+        int &tile = prec ? c->rows32 : c->rows;
+        const int keep = tile;
+        best = 1e30, pick = keep;
+        for (int R : cand) {
+            if (prec == 0 && R == 1) continue;
+            tile = R;
+            rc = time_runs(c, 1, [&] {
+                if (prec) (void)cg_mixed(c, (double *)x, (const double *)U, (const double *)b, a, &it);
+                else (void)cg_f64(c, (double *)x, (const double *)U, (const double *)b, a, &it);
+            }, &us);
+            if (rc) break;
+            if (us < best) best = us, pick = R;
+        }
+        tile = rc ? keep : pick;
+    }
+    c->sync = save_sync;
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(mem);
+    if (rc) return rc;
+    c->err.clear();
     return launched(c);
 }
 
@@ -886,6 +995,7 @@ int qed2d_ctx_create(qed2d_ctx **out, int L1, int L2, double beta, int bc_flags,
     c->L.L1 = L1;
     c->L.L2 = L2;
     c->L.V = (long)L1 * L2;
+    default_tiles(c);
     c->L.bc = bc_flags;
     c->beta = beta;
     c->stream = (hipStream_t)stream;
@@ -962,9 +1072,33 @@ int qed2d_ctx_set_sync(qed2d_ctx *c, int sync)
 int qed2d_ctx_set_tile(qed2d_ctx *c, int rows)
 {
     if (!c) return QED2D_EINVAL;
-    if (rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16)
+    if (!tile_ok(rows))
         return fail(c, QED2D_EINVAL, "tile rows must be 1, 2, 4, 8 or 16 (got %d)", rows);
     c->rows = rows;
+    c->rows_q = rows;
+    return QED2D_OK;
+}
+
+int qed2d_ctx_set_tiles(qed2d_ctx *c, int q64, int q32, int cg64, int cg32)
+{
+    if (!c) return QED2D_EINVAL;
+    for (int v : {q64, q32, cg64, cg32})
+        if (v != 0 && !tile_ok(v))
+            return fail(c, QED2D_EINVAL, "tile rows must be 0, 1, 2, 4, 8 or 16 (got %d)", v);
+    if (q64) c->rows_q = q64;
+    if (q32) c->rows_q32 = q32;
+    if (cg64) c->rows = cg64;
+    if (cg32) c->rows32 = cg32;
+    return QED2D_OK;
+}
+
+int qed2d_ctx_get_tiles(qed2d_ctx *c, int *tiles)
+{
+    if (!c || !tiles) return QED2D_EINVAL;
+    tiles[0] = c->rows_q;
+    tiles[1] = c->rows_q32;
+    tiles[2] = c->rows;
+    tiles[3] = c->rows32;
     return QED2D_OK;
 }
 
@@ -980,10 +1114,20 @@ int qed2d_ctx_set_reliable_delta(qed2d_ctx *c, double delta)
 int qed2d_ctx_set_tile_f32(qed2d_ctx *c, int rows)
 {
     if (!c) return QED2D_EINVAL;
-    if (rows != 1 && rows != 2 && rows != 4 && rows != 8 && rows != 16)
+    if (!tile_ok(rows))
         return fail(c, QED2D_EINVAL, "tile rows must be 1, 2, 4, 8 or 16 (got %d)", rows);
     c->rows32 = rows;
+    c->rows_q32 = rows;
     return QED2D_OK;
+}
+
+int qed2d_ctx_autotune(qed2d_ctx *c, int iters, int *tiles)
+{
+    if (!c) return QED2D_EINVAL;
+    if (iters < 4) iters = 4;
+    TRY(autotune_impl(c, iters));
+    if (tiles) qed2d_ctx_get_tiles(c, tiles);
+    return finish(c);
 }
 
 int qed2d_synchronize(qed2d_ctx *c)
@@ -1015,7 +1159,7 @@ int qed2d_gamm5Dw_f32(qed2d_ctx *c, float *so, const float *U, const float *si, 
     CHECK_CTX(c);
     NEED(c, so); NEED(c, U); NEED(c, si);
     NOALIAS(c, so, si);
-    launch_q_apply_f32(c->L, (const float2 *)U, (const float2 *)si, (float2 *)so, am0, c->rows32,
+    launch_q_apply_f32(c->L, (const float2 *)U, (const float2 *)si, (float2 *)so, am0, c->rows_q32,
                        c->stream);
     return finish(c);
 }
@@ -1110,7 +1254,7 @@ int qed2d_gamm5Dw_sqr_sqr_musq(qed2d_ctx *c, double *so, double *tmp, const doub
     q64(c, U, (const double *)t1, (double *)t2, am0);
     q64(c, U, (const double *)t2, (double *)t1, am0);
     launch_q_shift_f64(c->L, (const double2 *)U, t1, (const double2 *)si, (double2 *)so, am0,
-                       mu * mu, c->rows, c->stream);
+                       mu * mu, c->rows_q, c->stream);
     return finish(c);
 }
 

@@ -34,13 +34,16 @@ MarchGrid march_grid(const Latt &L, int rows)
 // ===========================================================================
 // Q apply: so = gamma5 Dw si  (src/QED2dDirac.jl:2-33)
 // ===========================================================================
+// The apply's output is not read back by this kernel, so its stores stream
+// past the caches (nontemporal).
 template <typename T> struct EpiStore {
     c2<T> *__restrict__ so;
     long V;
-    __device__ __forceinline__ void operator()(long s, c2<T> o1, c2<T> o2, c2<T>, c2<T>)
+    __device__ __forceinline__ NoPre pre(long) const { return {}; }
+    __device__ __forceinline__ void operator()(long s, NoPre, c2<T> o1, c2<T> o2, c2<T>, c2<T>)
     {
-        so[s] = o1;
-        so[V + s] = o2;
+        st<T, true>(so, s, o1);
+        st<T, true>(so, V + s, o2);
     }
 };
 
@@ -63,11 +66,15 @@ struct EpiShift {
     const double2 *__restrict__ x;
     double mu2;
     long V;
-    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2, double2)
+    struct Pre {
+        double2 x1, x2;
+    };
+    __device__ __forceinline__ Pre pre(long s) const { return Pre{x[s], x[V + s]}; }
+    __device__ __forceinline__ void operator()(long s, const Pre &q, double2 o1, double2 o2,
+                                               double2, double2)
     {
-        double2 x1 = x[s], x2 = x[V + s];
-        so[s] = mk<double>(o1.x + mu2 * x1.x, o1.y + mu2 * x1.y);
-        so[V + s] = mk<double>(o2.x + mu2 * x2.x, o2.y + mu2 * x2.y);
+        so[s] = mk<double>(o1.x + mu2 * q.x1.x, o1.y + mu2 * q.x1.y);
+        so[V + s] = mk<double>(o2.x + mu2 * q.x2.x, o2.y + mu2 * q.x2.y);
     }
 };
 
@@ -89,11 +96,12 @@ __global__ void __launch_bounds__(kBlock) k_q_shift(Latt L, const double2 *__res
 struct EpiStore2 {
     float2 *__restrict__ so;
     long V;
-    __device__ __forceinline__ void pair(long s, float2 oa1, float2 oa2, float2 ob1, float2 ob2,
-                                         F2x2, F2x2)
+    __device__ __forceinline__ NoPre pre(long) const { return {}; }
+    __device__ __forceinline__ void pair(long s, NoPre, float2 oa1, float2 oa2, float2 ob1,
+                                         float2 ob2, F2x2, F2x2)
     {
-        st2(so, s, oa1, ob1);
-        st2(so, V + s, oa2, ob2);
+        st2<true>(so, s, oa1, ob1);
+        st2<true>(so, V + s, oa2, ob2);
     }
 };
 

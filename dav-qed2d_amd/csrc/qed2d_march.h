@@ -4,7 +4,10 @@
 // A block holds kBlock threads arranged bx (along i1) x by (along i2).  Each
 // thread owns one i1 column and marches up R consecutive rows, keeping the
 // spinor rows i2-1, i2, i2+1 and the link U2(x-2) in registers, so each new
-// row costs one spinor load, one link load and the epilogue's stores.
+// row costs one spinor load, one link load and the epilogue's stores.  The
+// march is software pipelined one row deep: a wave keeps two rows of loads in
+// flight, which is what hides HBM latency at the one or two waves per SIMD a
+// marching launch leaves on a 1024^2 lattice.
 #pragma once
 
 #include "qed2d_internal.h"
@@ -31,14 +34,21 @@ __device__ __forceinline__ MarchPos march_pos(const Latt &L, int bx, int by, int
     return p;
 }
 
-// Spinor sources for the stencil.
+// Spinor sources for the stencil.  fetch() issues the loads of one site and
+// returns them raw; value() turns them into the two spin components.  The
+// split lets the march issue the loads of the next row before it computes the
+// current one.
 template <typename T> struct SrcPlain {
     const c2<T> *__restrict__ s;
     long V;
-    __device__ __forceinline__ void load(long i, c2<T> &a, c2<T> &b) const
+    struct Raw {
+        c2<T> a, b;
+    };
+    __device__ __forceinline__ Raw fetch(long i) const { return Raw{s[i], s[V + i]}; }
+    __device__ __forceinline__ void value(const Raw &w, c2<T> &a, c2<T> &b) const
     {
-        a = s[i];
-        b = s[V + i];
+        a = w.a;
+        b = w.b;
     }
 };
 
@@ -48,53 +58,105 @@ template <typename T> struct SrcXpay {
     const c2<T> *__restrict__ p;
     T beta;
     long V;
-    __device__ __forceinline__ void load(long i, c2<T> &a, c2<T> &b) const
+    struct Raw {
+        c2<T> r0, r1, p0, p1;
+    };
+    __device__ __forceinline__ Raw fetch(long i) const { return Raw{r[i], r[V + i], p[i], p[V + i]}; }
+    __device__ __forceinline__ void value(const Raw &w, c2<T> &a, c2<T> &b) const
     {
-        c2<T> r0 = r[i], r1 = r[V + i], p0 = p[i], p1 = p[V + i];
-        a = mk<T>(r0.x + beta * p0.x, r0.y + beta * p0.y);
-        b = mk<T>(r1.x + beta * p1.x, r1.y + beta * p1.y);
+        a = mk<T>(w.r0.x + beta * w.p0.x, w.r0.y + beta * w.p0.y);
+        b = mk<T>(w.r1.x + beta * w.p1.x, w.r1.y + beta * w.p1.y);
     }
 };
 
-// March one thread's column: for rows r0 .. r0+R-1 compute (Q s)(x) and hand
-// it to the epilogue together with s(x).
+// Epilogues: pre(s) issues the loads the epilogue needs at site s (fetched one
+// row ahead with the stencil operands); operator()(s, pre, o1, o2, c1, c2)
+// receives Q s at x (o1, o2) and s(x) itself (c1, c2).  NoPre is the empty
+// pre-load of epilogues that only store.
+struct NoPre {};
+
+// Stores; NT selects nontemporal stores (streamed past the caches, for
+// outputs that are not read back soon).
+template <typename T, bool NT> __device__ __forceinline__ void st(c2<T> *p, long i, c2<T> v)
+{
+    if constexpr (NT) {
+        __builtin_nontemporal_store(v.x, &p[i].x);
+        __builtin_nontemporal_store(v.y, &p[i].y);
+    } else {
+        p[i] = v;
+    }
+}
+
+// March one thread's column over rows r0 .. r0+R-1: compute (Q s)(x) and hand
+// it to the epilogue together with s(x).  The operands of row r+1 (the spinor
+// of row r+2, the row's i1 neighbours and links, the epilogue's pre-loads) are
+// in flight while row r is computed.
 template <typename T, int R, typename Src, typename Epi>
 __device__ __forceinline__ void q_march(const Latt &L, const c2<T> *__restrict__ U, T mass2,
                                         const Src &src, Epi &epi, int i1, int r0)
 {
+    using Raw = typename Src::Raw;
+    using Pre = decltype(epi.pre(0L));
     const int L1 = L.L1, L2 = L.L2;
     const long V = L.V;
     const int i1p = (i1 + 1 == L1) ? 0 : i1 + 1;
     const int i1m = (i1 == 0) ? L1 - 1 : i1 - 1;
     const bool ap = (L.bc & 1) != 0;
     const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
+    const int nr = (r0 + R <= L2) ? R : L2 - r0;
     c2<T> dn1, dn2, c1, cc2;
-    src.load((long)rm * L1 + i1, dn1, dn2);
-    src.load((long)r0 * L1 + i1, c1, cc2);
+    src.value(src.fetch((long)rm * L1 + i1), dn1, dn2);
+    src.value(src.fetch((long)r0 * L1 + i1), c1, cc2);
     c2<T> U2m = U[V + (long)rm * L1 + i1];
     if (ap && r0 == 0) U2m = cneg<T>(U2m);
+    // operands of row r0
+    long row = (long)r0 * L1;
+    Raw wu = src.fetch((long)((r0 + 1 == L2) ? 0 : r0 + 1) * L1 + i1);
+    Raw wp = src.fetch(row + i1p), wm = src.fetch(row + i1m);
+    c2<T> U1 = U[row + i1], U1m = U[row + i1m], U2 = U[V + row + i1];
+    Pre q = epi.pre(row + i1);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
+        if (k >= nr) break;
         const int r = r0 + k;
-        if (r >= L2) break;
-        const int rp = (r + 1 == L2) ? 0 : r + 1;
-        const long row = (long)r * L1;
-        const long s = row + i1;
+        const bool more = (k + 1 < nr);
+        Raw nwu, nwp, nwm;
+        c2<T> nU1, nU1m, nU2;
+        Pre nq;
+        if (more) {
+            const long row1 = (long)(r + 1) * L1;
+            const int r2 = (r + 2 >= L2) ? r + 2 - L2 : r + 2;
+            nwu = src.fetch((long)r2 * L1 + i1);
+            nwp = src.fetch(row1 + i1p);
+            nwm = src.fetch(row1 + i1m);
+            nU1 = U[row1 + i1];
+            nU1m = U[row1 + i1m];
+            nU2 = U[V + row1 + i1];
+            nq = epi.pre(row1 + i1);
+        }
         c2<T> up1, up2, p11, p12, m11, m12;
-        src.load((long)rp * L1 + i1, up1, up2);
-        src.load(row + i1p, p11, p12);
-        src.load(row + i1m, m11, m12);
-        const c2<T> U1 = U[s], U1m = U[row + i1m], U2 = U[V + s];
+        src.value(wu, up1, up2);
+        src.value(wp, p11, p12);
+        src.value(wm, m11, m12);
         const c2<T> U2e = (ap && r == L2 - 1) ? cneg<T>(U2) : U2;
         c2<T> o1, o2;
         q_site<T>(mass2, c1, cc2, p11, p12, m11, m12, up1, up2, dn1, dn2, U1, U1m, U2e, U2m, o1,
                   o2);
-        epi(s, o1, o2, c1, cc2);
+        epi((long)r * L1 + i1, q, o1, o2, c1, cc2);
         dn1 = c1;
         dn2 = cc2;
         c1 = up1;
         cc2 = up2;
         U2m = U2;
+        if (more) {
+            wu = nwu;
+            wp = nwp;
+            wm = nwm;
+            U1 = nU1;
+            U1m = nU1m;
+            U2 = nU2;
+            q = nq;
+        }
     }
 }
 
@@ -116,9 +178,16 @@ __device__ __forceinline__ F2x2 ld2(const float2 *__restrict__ p, long i)
     r.a1 = make_float2(v.z, v.w);
     return r;
 }
+template <bool NT = false>
 __device__ __forceinline__ void st2(float2 *__restrict__ p, long i, float2 a, float2 b)
 {
-    *reinterpret_cast<float4 *>(p + i) = make_float4(a.x, a.y, b.x, b.y);
+    if constexpr (NT) {
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        const v4f v = {a.x, a.y, b.x, b.y};
+        __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(p + i));
+    } else {
+        *reinterpret_cast<float4 *>(p + i) = make_float4(a.x, a.y, b.x, b.y);
+    }
 }
 __device__ __forceinline__ F2x2 neg2(F2x2 v)
 {
@@ -141,18 +210,27 @@ __device__ __forceinline__ MarchPos march_pos2(const Latt &L, int bx, int by, in
     return p;
 }
 
+// Pair sources: fetch2/value2 for the column pair, fetch1/value1 for one site.
 struct SrcPlain2 {
     const float2 *__restrict__ s;
     long V;
-    __device__ __forceinline__ void load2(long i, F2x2 &a, F2x2 &b) const
+    struct Raw2 {
+        F2x2 a, b;
+    };
+    struct Raw1 {
+        float2 a, b;
+    };
+    __device__ __forceinline__ Raw2 fetch2(long i) const { return Raw2{ld2(s, i), ld2(s, V + i)}; }
+    __device__ __forceinline__ Raw1 fetch1(long i) const { return Raw1{s[i], s[V + i]}; }
+    __device__ __forceinline__ void value2(const Raw2 &w, F2x2 &a, F2x2 &b) const
     {
-        a = ld2(s, i);
-        b = ld2(s, V + i);
+        a = w.a;
+        b = w.b;
     }
-    __device__ __forceinline__ void load1(long i, float2 &a, float2 &b) const
+    __device__ __forceinline__ void value1(const Raw1 &w, float2 &a, float2 &b) const
     {
-        a = s[i];
-        b = s[V + i];
+        a = w.a;
+        b = w.b;
     }
 };
 
@@ -161,33 +239,50 @@ struct SrcXpay2 {
     const float2 *__restrict__ p;
     float beta;
     long V;
+    struct Raw2 {
+        F2x2 r0, r1, p0, p1;
+    };
+    struct Raw1 {
+        float2 r0, r1, p0, p1;
+    };
     __device__ __forceinline__ float2 xp(float2 a, float2 b) const
     {
         return make_float2(a.x + beta * b.x, a.y + beta * b.y);
     }
-    __device__ __forceinline__ void load2(long i, F2x2 &a, F2x2 &b) const
+    __device__ __forceinline__ Raw2 fetch2(long i) const
     {
-        const F2x2 r0 = ld2(r, i), r1 = ld2(r, V + i), p0 = ld2(p, i), p1 = ld2(p, V + i);
-        a.a0 = xp(r0.a0, p0.a0);
-        a.a1 = xp(r0.a1, p0.a1);
-        b.a0 = xp(r1.a0, p1.a0);
-        b.a1 = xp(r1.a1, p1.a1);
+        return Raw2{ld2(r, i), ld2(r, V + i), ld2(p, i), ld2(p, V + i)};
     }
-    __device__ __forceinline__ void load1(long i, float2 &a, float2 &b) const
+    __device__ __forceinline__ Raw1 fetch1(long i) const
     {
-        a = xp(r[i], p[i]);
-        b = xp(r[V + i], p[V + i]);
+        return Raw1{r[i], r[V + i], p[i], p[V + i]};
+    }
+    __device__ __forceinline__ void value2(const Raw2 &w, F2x2 &a, F2x2 &b) const
+    {
+        a.a0 = xp(w.r0.a0, w.p0.a0);
+        a.a1 = xp(w.r0.a1, w.p0.a1);
+        b.a0 = xp(w.r1.a0, w.p1.a0);
+        b.a1 = xp(w.r1.a1, w.p1.a1);
+    }
+    __device__ __forceinline__ void value1(const Raw1 &w, float2 &a, float2 &b) const
+    {
+        a = xp(w.r0, w.p0);
+        b = xp(w.r1, w.p1);
     }
 };
 
-// March the column pair (2j, 2j+1) over rows r0 .. r0+R-1; the epilogue gets
-// both sites' outputs and inputs: epi.pair(s, oa1, oa2, ob1, ob2, c1, c2) with
-// s the index of site 2j, (oa, ob) the outputs of sites 2j, 2j+1 for spin
-// components 1, 2 and c1, c2 the inputs.
+// March the column pair (2j, 2j+1) over rows r0 .. r0+R-1, software pipelined
+// like q_march; the epilogue gets both sites' outputs and inputs:
+// epi.pair(s, pre, oa1, oa2, ob1, ob2, c1, c2) with s the index of site 2j,
+// (oa, ob) the outputs of sites 2j, 2j+1 for spin components 1, 2 and c1, c2
+// the inputs.
 template <int R, typename Src, typename Epi>
 __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict__ U, float mass2,
                                          const Src &src, Epi &epi, int j, int r0)
 {
+    using Raw2 = typename Src::Raw2;
+    using Raw1 = typename Src::Raw1;
+    using Pre = decltype(epi.pre(0L));
     const int L1 = L.L1, L2 = L.L2;
     const long V = L.V;
     const int ia = 2 * j;
@@ -195,25 +290,44 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
     const int ip = (ia + 2 == L1) ? 0 : ia + 2;
     const bool ap = (L.bc & 1) != 0;
     const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
+    const int nr = (r0 + R <= L2) ? R : L2 - r0;
     F2x2 dn1, dn2, c1, cc2;
-    src.load2((long)rm * L1 + ia, dn1, dn2);
-    src.load2((long)r0 * L1 + ia, c1, cc2);
+    src.value2(src.fetch2((long)rm * L1 + ia), dn1, dn2);
+    src.value2(src.fetch2((long)r0 * L1 + ia), c1, cc2);
     F2x2 U2m = ld2(U, V + (long)rm * L1 + ia);
     if (ap && r0 == 0) U2m = neg2(U2m);
+    long row = (long)r0 * L1;
+    Raw2 wu = src.fetch2((long)((r0 + 1 == L2) ? 0 : r0 + 1) * L1 + ia);
+    Raw1 wl = src.fetch1(row + im), wq = src.fetch1(row + ip);
+    F2x2 U1 = ld2(U, row + ia), U2 = ld2(U, V + row + ia);
+    float2 U1l = U[row + im];
+    Pre q = epi.pre(row + ia);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
+        if (k >= nr) break;
         const int r = r0 + k;
-        if (r >= L2) break;
-        const int rp = (r + 1 == L2) ? 0 : r + 1;
-        const long row = (long)r * L1;
-        const long s = row + ia;
+        const bool more = (k + 1 < nr);
+        Raw2 nwu;
+        Raw1 nwl, nwq;
+        F2x2 nU1, nU2;
+        float2 nU1l;
+        Pre nq;
+        if (more) {
+            const long row1 = (long)(r + 1) * L1;
+            const int r2 = (r + 2 >= L2) ? r + 2 - L2 : r + 2;
+            nwu = src.fetch2((long)r2 * L1 + ia);
+            nwl = src.fetch1(row1 + im);
+            nwq = src.fetch1(row1 + ip);
+            nU1 = ld2(U, row1 + ia);
+            nU2 = ld2(U, V + row1 + ia);
+            nU1l = U[row1 + im];
+            nq = epi.pre(row1 + ia);
+        }
         F2x2 up1, up2;
         float2 l1, l2, q1, q2;
-        src.load2((long)rp * L1 + ia, up1, up2);
-        src.load1(row + im, l1, l2);
-        src.load1(row + ip, q1, q2);
-        const F2x2 U1 = ld2(U, s), U2 = ld2(U, V + s);
-        const float2 U1l = U[row + im];
+        src.value2(wu, up1, up2);
+        src.value1(wl, l1, l2);
+        src.value1(wq, q1, q2);
         const F2x2 U2e = (ap && r == L2 - 1) ? neg2(U2) : U2;
         float2 oa1, oa2, ob1, ob2;
         // site 2j: x+1 = 2j+1, x-1 = the left neighbour
@@ -222,12 +336,21 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
         // site 2j+1: x+1 = the right neighbour, x-1 = 2j
         q_site<float>(mass2, c1.a1, cc2.a1, q1, q2, c1.a0, cc2.a0, up1.a1, up2.a1, dn1.a1,
                       dn2.a1, U1.a1, U1.a0, U2e.a1, U2m.a1, ob1, ob2);
-        epi.pair(s, oa1, oa2, ob1, ob2, c1, cc2);
+        epi.pair((long)r * L1 + ia, q, oa1, oa2, ob1, ob2, c1, cc2);
         dn1 = c1;
         dn2 = cc2;
         c1 = up1;
         cc2 = up2;
         U2m = U2;
+        if (more) {
+            wu = nwu;
+            wl = nwl;
+            wq = nwq;
+            U1 = nU1;
+            U2 = nU2;
+            U1l = nU1l;
+            q = nq;
+        }
     }
 }
 

@@ -73,13 +73,42 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Latt L, const double2 *__res
     }
 }
 
+// ---------------------------------------------------------------------------
+// Folding.  A consumer kernel sums all partials of its producer in every one
+// of its blocks, which costs O(blocks^2) reads.  Past kFoldMin producer blocks
+// a one-block kernel folds them first, in place, to the single value the
+// consumer would have computed: the same code (sum_prev_partials) over the
+// same partials in the same order, so the bits do not depend on the folding.
+// ---------------------------------------------------------------------------
+constexpr int kFoldMin = 1024;
+
+__global__ void __launch_bounds__(kBlock) k_fold(double *part, int n, int stride)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const double v0 = sum_prev_partials(part, n, stride, 0, lds);
+    const double v1 = (stride > 1) ? sum_prev_partials(part, n, stride, 1, lds) : 0.0;
+    if (threadIdx.x == 0) {
+        part[0] = v0;
+        if (stride > 1) part[1] = v1;
+    }
+}
+
+// Returns the partial count a consumer must use.
+static int fold(double *part, int n, int stride, hipStream_t st)
+{
+    if (n <= kFoldMin) return n;
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, st, part, n, stride);
+    return 1;
+}
+
 void launch_cg_init_f64(const Latt &L, const double2 *b, double2 *x, double2 *r, double2 *p0,
                         CGPartials &P, int rows, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
     hipLaunchKernelGGL(k_cg_init<double>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, x, r, p0,
                        (float2 *)nullptr, P.rr[0], P.b, g.bx, g.by, g.nbx, rows);
-    P.nrr[0] = P.nb = g.nblocks();
+    P.nrr[0] = fold(P.rr[0], g.nblocks(), 1, st);
+    P.nb = fold(P.b, g.nblocks(), 1, st);
 }
 
 void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *rs, float2 *xs,
@@ -88,7 +117,8 @@ void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *r
     MarchGrid g = march_grid(L, rows);
     hipLaunchKernelGGL(k_cg_init<float>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, x, rs, p0,
                        xs, P.rr[0], P.b, g.bx, g.by, g.nbx, rows);
-    P.nrr[0] = P.nb = g.nblocks();
+    P.nrr[0] = fold(P.rr[0], g.nblocks(), 1, st);
+    P.nb = fold(P.b, g.nblocks(), 1, st);
 }
 
 // ===========================================================================
@@ -105,9 +135,12 @@ struct EpiResid {
     double mu2;
     long V;
     double rr, bb;
-    __device__ __forceinline__ void one(long i, double2 o)
+    struct Pre {
+        double2 b1, b2, x1, x2;
+    };
+    __device__ __forceinline__ Pre pre(long s) const { return Pre{b[s], b[V + s], x[s], x[V + s]}; }
+    __device__ __forceinline__ void one(long i, double2 o, double2 bv, double2 xv)
     {
-        const double2 bv = b[i], xv = x[i];
         const double2 rv = mk<double>(bv.x - (o.x + mu2 * xv.x), bv.y - (o.y + mu2 * xv.y));
         if (r) r[i] = rv;
         if (rs) rs[i] = mk<float>((float)rv.x, (float)rv.y);
@@ -116,10 +149,11 @@ struct EpiResid {
         rr += rv.x * rv.x + rv.y * rv.y;
         bb += bv.x * bv.x + bv.y * bv.y;
     }
-    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2, double2)
+    __device__ __forceinline__ void operator()(long s, const Pre &q, double2 o1, double2 o2,
+                                               double2, double2)
     {
-        one(s, o1);
-        one(V + s, o2);
+        one(s, o1, q.b1, q.x1);
+        one(V + s, o2, q.b2, q.x2);
     }
 };
 
@@ -167,8 +201,8 @@ void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const do
     dim3 grid(g.nblocks()), blk(kBlock);
     DISPATCH_ROWS(rows, k_cg_resid, grid, blk, 0, st, L, U, t, e, 2.0 + am0, s, P.rr[(k - 1) & 1],
                   P.b, k, mode, g.bx, g.by, g.nbx);
-    P.nrr[(k - 1) & 1] = g.nblocks();
-    if (mode != 2) P.nb = g.nblocks();
+    P.nrr[(k - 1) & 1] = fold(P.rr[(k - 1) & 1], g.nblocks(), 1, st);
+    if (mode != 2) P.nb = fold(P.b, g.nblocks(), 1, st);
 }
 
 // x += double(xs); xs = 0
@@ -264,7 +298,9 @@ template <typename T> struct EpiK1 {
     c2<T> *__restrict__ t;
     long V;
     double tt, pp;
-    __device__ __forceinline__ void operator()(long s, c2<T> o1, c2<T> o2, c2<T> c1, c2<T> cc2)
+    __device__ __forceinline__ NoPre pre(long) const { return {}; }
+    __device__ __forceinline__ void operator()(long s, NoPre, c2<T> o1, c2<T> o2, c2<T> c1,
+                                               c2<T> cc2)
     {
         pnew[s] = c1;
         pnew[V + s] = cc2;
@@ -318,21 +354,27 @@ template <typename T> struct EpiK2 {
     T alpha, mu2;
     long V;
     double rr;
-    __device__ __forceinline__ void one(long i, c2<T> o)
+    struct Pre {
+        c2<T> p1, p2, x1, x2, r1, r2;
+    };
+    __device__ __forceinline__ Pre pre(long s) const
     {
-        const c2<T> pv = p[i];
+        return Pre{p[s], p[V + s], x[s], x[V + s], r[s], r[V + s]};
+    }
+    __device__ __forceinline__ void one(long i, c2<T> o, c2<T> pv, c2<T> xv, c2<T> rv)
+    {
         const c2<T> a = mk<T>(o.x + mu2 * pv.x, o.y + mu2 * pv.y);
-        c2<T> xv = x[i], rv = r[i];
         xv = mk<T>(xv.x + alpha * pv.x, xv.y + alpha * pv.y);   // src/QED2dSolver.jl:24
         rv = mk<T>(rv.x - alpha * a.x, rv.y - alpha * a.y);     // src/QED2dSolver.jl:25
         x[i] = xv;
         r[i] = rv;
         rr += (double)(rv.x * rv.x + rv.y * rv.y);
     }
-    __device__ __forceinline__ void operator()(long s, c2<T> o1, c2<T> o2, c2<T>, c2<T>)
+    __device__ __forceinline__ void operator()(long s, const Pre &q, c2<T> o1, c2<T> o2, c2<T>,
+                                               c2<T>)
     {
-        one(s, o1);
-        one(V + s, o2);
+        one(s, o1, q.p1, q.x1, q.r1);
+        one(V + s, o2, q.p2, q.x2, q.r2);
     }
 };
 
@@ -368,8 +410,9 @@ struct EpiK1v {
     float2 *__restrict__ t;
     long V;
     double tt, pp;
-    __device__ __forceinline__ void pair(long s, float2 oa1, float2 oa2, float2 ob1, float2 ob2,
-                                         F2x2 c1, F2x2 c2)
+    __device__ __forceinline__ NoPre pre(long) const { return {}; }
+    __device__ __forceinline__ void pair(long s, NoPre, float2 oa1, float2 oa2, float2 ob1,
+                                         float2 ob2, F2x2 c1, F2x2 c2)
     {
         st2(pnew, s, c1.a0, c1.a1);
         st2(pnew, V + s, c2.a0, c2.a1);
@@ -417,10 +460,15 @@ struct EpiK2v {
     float alpha, mu2;
     long V;
     double rr;
-    __device__ __forceinline__ void comp(long i, float2 oa, float2 ob)
+    struct Pre {
+        F2x2 p1, p2, x1, x2, r1, r2;
+    };
+    __device__ __forceinline__ Pre pre(long s) const
     {
-        const F2x2 pv = ld2(p, i);
-        F2x2 xv = ld2(x, i), rv = ld2(r, i);
+        return Pre{ld2(p, s), ld2(p, V + s), ld2(x, s), ld2(x, V + s), ld2(r, s), ld2(r, V + s)};
+    }
+    __device__ __forceinline__ void comp(long i, float2 oa, float2 ob, F2x2 pv, F2x2 xv, F2x2 rv)
+    {
         const float2 aa = make_float2(oa.x + mu2 * pv.a0.x, oa.y + mu2 * pv.a0.y);
         const float2 ab = make_float2(ob.x + mu2 * pv.a1.x, ob.y + mu2 * pv.a1.y);
         xv.a0 = make_float2(xv.a0.x + alpha * pv.a0.x, xv.a0.y + alpha * pv.a0.y);
@@ -431,11 +479,11 @@ struct EpiK2v {
         st2(r, i, rv.a0, rv.a1);
         rr += (double)(cabs2<float>(rv.a0) + cabs2<float>(rv.a1));
     }
-    __device__ __forceinline__ void pair(long s, float2 oa1, float2 oa2, float2 ob1, float2 ob2,
-                                         F2x2, F2x2)
+    __device__ __forceinline__ void pair(long s, const Pre &q, float2 oa1, float2 oa2, float2 ob1,
+                                         float2 ob2, F2x2, F2x2)
     {
-        comp(s, oa1, ob1);
-        comp(V + s, oa2, ob2);
+        comp(s, oa1, ob1, q.p1, q.x1, q.r1);
+        comp(V + s, oa2, ob2, q.p2, q.x2, q.r2);
     }
 };
 
@@ -479,7 +527,7 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
             dim3 grid(g.nblocks()), blk(kBlock);
             DISPATCH_ROWS(rows, k_cg1v, grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr,
                           P.b, nrr, nbb, k, g.bx, g.by, g.nbx);
-            P.ntt[k & 1] = g.nblocks();
+            P.ntt[k & 1] = fold(ptt, g.nblocks(), 2, st);
             return;
         }
     }
@@ -492,7 +540,7 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
     case 8: hipLaunchKernelGGL((k_cg1<T, 8>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
     default: hipLaunchKernelGGL((k_cg1<T, 16>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
     }
-    P.ntt[k & 1] = g.nblocks();
+    P.ntt[k & 1] = fold(ptt, g.nblocks(), 2, st);
 }
 
 template <typename T>
@@ -510,7 +558,7 @@ static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> 
             dim3 grid(g.nblocks()), blk(kBlock);
             DISPATCH_ROWS(rows, k_cg2v, grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr,
                           ntt, k, g.bx, g.by, g.nbx);
-            P.nrr[k & 1] = g.nblocks();
+            P.nrr[k & 1] = fold(prr, g.nblocks(), 1, st);
             return;
         }
     }
@@ -523,7 +571,7 @@ static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> 
     case 8: hipLaunchKernelGGL((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
     default: hipLaunchKernelGGL((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
     }
-    P.nrr[k & 1] = g.nblocks();
+    P.nrr[k & 1] = fold(prr, g.nblocks(), 1, st);
 }
 
 void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2 *p0,
@@ -594,8 +642,12 @@ struct EpiMS1 {
             }
         }
     }
-    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2 c1,
-                                               double2 cc2)
+    struct Pre {
+        double2 r1, r2;
+    };
+    __device__ __forceinline__ Pre pre(long s) const { return Pre{r[s], r[V + s]}; }
+    __device__ __forceinline__ void operator()(long s, const Pre &q, double2 o1, double2 o2,
+                                               double2 c1, double2 cc2)
     {
         pnew[s] = c1;
         pnew[V + s] = cc2;
@@ -603,8 +655,8 @@ struct EpiMS1 {
         t[V + s] = o2;
         tt += (o1.x * o1.x + o1.y * o1.y) + (o2.x * o2.x + o2.y * o2.y);
         pp += (c1.x * c1.x + c1.y * c1.y) + (cc2.x * cc2.x + cc2.y * cc2.y);
-        shifts(s, r[s]);
-        shifts(V + s, r[V + s]);
+        shifts(s, q.r1);
+        shifts(V + s, q.r2);
     }
 };
 
@@ -728,18 +780,21 @@ struct EpiMS2 {
     double alpha, sigma;
     long V;
     double rr;
-    __device__ __forceinline__ void one(long i, double2 o)
+    struct Pre {
+        double2 p1, p2, r1, r2;
+    };
+    __device__ __forceinline__ Pre pre(long s) const { return Pre{p[s], p[V + s], r[s], r[V + s]}; }
+    __device__ __forceinline__ void one(long i, double2 o, double2 pv, double2 rv)
     {
-        const double2 pv = p[i];
-        double2 rv = r[i];
         rv = mk<double>(rv.x - alpha * (o.x + sigma * pv.x), rv.y - alpha * (o.y + sigma * pv.y));
         r[i] = rv;
         rr += rv.x * rv.x + rv.y * rv.y;
     }
-    __device__ __forceinline__ void operator()(long s, double2 o1, double2 o2, double2, double2)
+    __device__ __forceinline__ void operator()(long s, const Pre &q, double2 o1, double2 o2,
+                                               double2, double2)
     {
-        one(s, o1);
-        one(V + s, o2);
+        one(s, o1, q.p1, q.r1);
+        one(V + s, o2, q.p2, q.r2);
     }
 };
 
@@ -806,7 +861,7 @@ void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0, Mu
     MarchGrid g = march_grid(L, rows);
     hipLaunchKernelGGL(k_ms_init, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, r, p0, s, P.rr[0],
                        nshift, g.bx, g.by, g.nbx, rows);
-    P.nrr[0] = g.nblocks();
+    P.nrr[0] = fold(P.rr[0], g.nblocks(), 1, st);
 }
 
 void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, double2 *p0, double2 *p1,
@@ -818,7 +873,7 @@ void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, double2 *p0
     DISPATCH_ROWS(rows, k_ms1, grid, blk, 0, st, L, U, r, p0, p1, t, 2.0 + am0, s, P.tt[k & 1],
                   P.tt[(k - 1) & 1], P.rr[(k - 1) & 1], P.nrr[(k - 1) & 1], P.ntt[(k - 1) & 1], k,
                   nshift, g.bx, g.by, g.nbx);
-    P.ntt[k & 1] = g.nblocks();
+    P.ntt[k & 1] = fold(P.tt[k & 1], g.nblocks(), 2, st);
 }
 
 void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
@@ -829,7 +884,7 @@ void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const doubl
     dim3 grid(g.nblocks()), blk(kBlock);
     DISPATCH_ROWS(rows, k_ms2, grid, blk, 0, st, L, U, t, p0, p1, r, 2.0 + am0, s, P.tt[k & 1],
                   P.rr[k & 1], P.ntt[k & 1], k, g.bx, g.by, g.nbx);
-    P.nrr[k & 1] = g.nblocks();
+    P.nrr[k & 1] = fold(P.rr[k & 1], g.nblocks(), 1, st);
 }
 
 }  // namespace qed2d

@@ -54,6 +54,9 @@ SIGNATURES = {
     "qed2d_ctx_set_sync": (_i, [_p, _i]),
     "qed2d_ctx_set_tile": (_i, [_p, _i]),
     "qed2d_ctx_set_tile_f32": (_i, [_p, _i]),
+    "qed2d_ctx_set_tiles": (_i, [_p, _i, _i, _i, _i]),
+    "qed2d_ctx_get_tiles": (_i, [_p, _p]),
+    "qed2d_ctx_autotune": (_i, [_p, _i, _p]),
     "qed2d_ctx_set_reliable_delta": (_i, [_p, _d]),
     "qed2d_synchronize": (_i, [_p]),
     "qed2d_gamm5Dw": (_i, [_p, _p, _p, _p, _d]),
@@ -118,6 +121,26 @@ for _name, (_res, _args) in SIGNATURES.items():
 
 PROF_CLASSES = ["cg_k1_f64", "cg_k2_f64", "cg_k1_f32", "cg_k2_f32", "q_f64", "force_mom",
                 "updt_fld", "reliable_update", "other"]
+
+
+def set_tiles(prm, q64=0, q32=0, cg64=0, cg32=0):
+    """Tile heights (rows marched per thread) of the FP64 / FP32 Q applies and
+    the FP64 / FP32 solver kernels; 0 leaves a family unchanged."""
+    _check(prm, lib.qed2d_ctx_set_tiles(prm.ctx(), int(q64), int(q32), int(cg64), int(cg32)))
+
+
+def get_tiles(prm):
+    t = (_i * 4)()
+    _check(prm, lib.qed2d_ctx_get_tiles(prm.ctx(), t))
+    return dict(zip(("q64", "q32", "cg64", "cg32"), list(t)))
+
+
+def autotune(prm, iters=40):
+    """Times every tile height of each kernel family on this lattice and keeps
+    the fastest (returns the choice)."""
+    t = (_i * 4)()
+    _check(prm, lib.qed2d_ctx_autotune(prm.ctx(), int(iters), t))
+    return dict(zip(("q64", "q32", "cg64", "cg32"), list(t)))
 
 
 def profile_enable(prm, on=True, every=1):

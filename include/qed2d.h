@@ -78,6 +78,19 @@ int qed2d_ctx_set_sync(qed2d_ctx *ctx, int sync);
  * or 16). */
 int qed2d_ctx_set_tile(qed2d_ctx *ctx, int rows);
 int qed2d_ctx_set_tile_f32(qed2d_ctx *ctx, int rows);
+/* Tile heights per kernel family (rows marched per thread: 1, 2, 4, 8 or 16;
+ * 0 leaves a family unchanged): FP64 Q/Dw applies, the FP32 Q apply, the FP64
+ * solver kernels and the FP32 solver kernels.  set_tile sets both FP64
+ * families, set_tile_f32 both FP32 ones.  No counterpart in the reference,
+ * whose KernelParm launch shape these replace (src/QED2d.jl:10-13). */
+int qed2d_ctx_set_tiles(qed2d_ctx *ctx, int q64, int q32, int cg64, int cg32);
+int qed2d_ctx_get_tiles(qed2d_ctx *ctx, int tiles[4]);
+/* The autotuner: times each tile height of the four families on this
+ * context's lattice (scratch fields; CG families over `iters` iterations of a
+ * solve that cannot converge) and keeps the fastest.  tiles (may be NULL)
+ * receives the choice.  Tile heights change the order of the solvers' global
+ * sums, so a run that must reproduce bits fixes its tiles instead. */
+int qed2d_ctx_autotune(qed2d_ctx *ctx, int iters, int tiles[4]);
 /* Mixed-precision CG: a reliable update (FP64 true residual) once the FP32
  * iterated residual has dropped by delta since the last one (default 0.01). */
 int qed2d_ctx_set_reliable_delta(qed2d_ctx *ctx, double delta);

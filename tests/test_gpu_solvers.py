@@ -360,3 +360,28 @@ def test_reweighting_factor_vs_oracle(Q, O):
     Wl = Q.reweighting_factor(dU, [0.25, 0.3], 10000, 1e-26, prm, None, [rp, rp2])
     Wr2, _ = O.reweighting_factor(U, 0.3, z2, 1, 5, 10000, 1e-26, 4242, 1, 8)
     assert abs(Wl - Wr * Wr2) < 1e-10 * abs(Wr * Wr2)
+
+
+def test_autotune_then_parity(Q, O):
+    """The autotuner picks valid tiles and leaves results within parity."""
+    say("autotune")
+    L1, L2 = 64, 48
+    rng = np.random.default_rng(8)
+    U = random_links(L1, L2, rng, unitary=True)
+    si = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 4.0)
+    t = Q.autotune(prm, 12)
+    assert set(t) == {"q64", "q32", "cg64", "cg32"}
+    assert all(v in (1, 2, 4, 8, 16) for v in t.values())
+    assert Q.get_tiles(prm) == t
+    so = Q.spinor(prm)
+    Q.gamm5Dw(so, dev(U), dev(si), 0.2, prm)
+    assert relerr(host(so), O.gamm5Dw(U, si, 0.2)) < 1e-13
+    for prec in (0, 1):
+        x = Q.spinor(prm)
+        Q.CG(x, dev(U), dev(si), 0.2, 5000, 1e-24, Q.gamm5Dw_sqr_msq, prm, None, precision=prec)
+        ref, _, st = O.cg(U, si, 0.2, 5000, 1e-24)
+        assert st == 0
+        assert relerr(host(x), ref) < 1e-9
+    Q.set_tiles(prm, 2, 0, 4, 0)
+    assert Q.get_tiles(prm)["q64"] == 2 and Q.get_tiles(prm)["cg64"] == 4
