@@ -6,9 +6,12 @@ dominant kernel and the CPU oracle on the host cores.
 Workload (one "step" = one HMC trajectory): compact U(1) gauge action + Nf=2
 Wilson fermions, beta = 4.0, m0 = 0.2, leapfrog ns = 10, eps = 0.1, CG on
 (gamma5 Dw)^2 to |r|^2 < tol |b|^2, start from a hot configuration (link
-angles uniform in [-pi, pi), seed 1234) thermalised by pure-gauge OMF4 HMC.
+angles uniform in [-pi, pi), seed 1234) thermalised by 200 pure-gauge OMF4
+trajectories.  The acceptance step checks the last MD solve in FP64 (true
+residual below tol, FP64 iterations if not) and evaluates the pseudofermion
+action as 2 Re X^dag F - |Q X|^2, in both precisions.
 
-    python bench.py [--steps K] [--warmup W] [--L 1024] [--tol 1e-16]
+    python bench.py [--steps K] [--warmup W] [--L 1024] [--tol 1e-22]
 
 Prints ONE JSON line (rank 0).  Only N = 1 GPU is supported.
 """
@@ -48,22 +51,27 @@ def log(*a):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--rocprof-summary":
+        return rocprof_summary(sys.argv[2:])
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--L", type=int, default=1024)
     ap.add_argument("--beta", type=float, default=4.0)
     ap.add_argument("--m0", type=float, default=0.2)
     ap.add_argument("--ns", type=int, default=10)
     ap.add_argument("--eps", type=float, default=0.1)
-    ap.add_argument("--tol", type=float, default=1e-16)
-    ap.add_argument("--therm", type=int, default=20)
+    ap.add_argument("--tol", type=float, default=1e-22)
+    ap.add_argument("--therm", type=int, default=200)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=0, help="FP64 tile rows (0: library default)")
     ap.add_argument("--tile32", type=int, default=0, help="FP32 tile rows (0: library default)")
     ap.add_argument("--no-dw", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tiles", type=str, default="",
+                    help="q64,q32,cg64,cg32 tile heights (skips the autotuner)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -96,6 +104,13 @@ def main():
     log(f"thermalised {args.therm} pure-gauge trajectories in {time.time() - t0:.2f}s, "
         f"plaquette {Q.Plaquette(U, prm):.6f}")
     U0 = U.clone()
+    if args.tiles:
+        Q.set_tiles(prm, *[int(v) for v in args.tiles.split(",")])
+    elif not (args.no_autotune or args.tile or args.tile32):
+        t0 = time.time()
+        tiles = Q.autotune(prm, 40)
+        log(f"autotuned tiles {tiles} in {time.time() - t0:.2f}s")
+    tiles = Q.get_tiles(prm)
 
     def run(precision, label, profile=False):
         U.copy_(U0)
@@ -104,7 +119,7 @@ def main():
         infos = []
         for _ in range(args.warmup):
             Q.HMC(U, args.m0, args.eps, args.ns, acc, 100000, args.tol, prm, None,
-                  precision=precision)
+                  precision=precision, acc_tol=args.tol)
         torch.cuda.synchronize()
         if profile:
             Q.profile_enable(prm, True, every=1)
@@ -112,7 +127,7 @@ def main():
         t = time.perf_counter()
         for _ in range(args.steps):
             infos.append(Q.HMC(U, args.m0, args.eps, args.ns, acc, 100000, args.tol, prm, None,
-                               precision=precision))
+                               precision=precision, acc_tol=args.tol))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
         prof = None
@@ -139,6 +154,7 @@ def main():
         res["accept_identical_vs_fp64"] = [a.accepted for a in inf_mx] == [b.accepted for b in inf_64]
         res["cg_iters_per_traj_fp64"] = sum(i.cg_iters for i in inf_64) / len(inf_64)
     res["cg_iters_per_traj_mixed"] = sum(i.cg_iters for i in inf_mx) / len(inf_mx)
+    res["tiles"] = tiles
     res["dH_mixed"] = [round(i.dH, 10) for i in inf_mx]
     res["acceptance"] = sum(i.accepted for i in inf_mx) / len(inf_mx)
 
@@ -155,8 +171,9 @@ def main():
     n, ms = prof_mx[dom]
     avg_s = ms / units[dom] / 1000.0
     achieved = BYTES_PER_SITE[dom] * V / avg_s / 1e9
+    traffic = pmc_traffic(dom, L)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": dom, "avg_us": round(avg_s * 1e6, 2), "launches_timed": n,
             "bytes_per_site": BYTES_PER_SITE[dom], "sites": V,
             "note": "1024^2 working set fits the 256 MB Infinity Cache: cache-assisted"}
@@ -167,6 +184,10 @@ def main():
             kern[k] = {"avg_us": round(1000 * ms / units[k], 2), "GBs": round(a, 1),
                        "launches": n}
     res["kernels_mixed"] = kern
+    stream = Q.hbm_stream(prm, 1 << 30)
+    res["hbm_stream_GBs"] = {k: round(v, 1) for k, v in stream.items()}
+    log(f"STREAM (3 x 1 GiB): " + ", ".join(f"{k} {v:.0f} GB/s" for k, v in stream.items()))
+    roof["achievable_copy_GBs"] = round(stream["copy"], 1)
     if not args.no_dw:
         res["dw_apply"] = dw_apply(Q, torch, args)
 
@@ -203,17 +224,14 @@ def main():
 
 def dw_apply(Q, torch, args):
     """gamm5Dw apply bandwidth at the shapes the HBM claim is made on
-    (1024^2 x 16 RHS, 4096^2; 1024^2 x 1 RHS is cache-assisted), FP64 and
-    FP32, timed with HIP events on the context's stream."""
+    (1024^2 x 16 RHS, 4096^2, 8192^2; 1024^2 x 1 RHS is cache-assisted), FP64
+    and FP32, timed with HIP events on the context's stream; every tile
+    height in TILES is timed and the best kept."""
     out = {}
-    shapes = [(1024, 1, "f64"), (1024, 16, "f64"), (4096, 1, "f64"),
-              (1024, 1, "f32"), (1024, 16, "f32"), (4096, 1, "f32")]
+    shapes = [(1024, 1, "f64"), (1024, 16, "f64"), (4096, 1, "f64"), (8192, 1, "f64"),
+              (1024, 1, "f32"), (1024, 16, "f32"), (4096, 1, "f32"), (8192, 1, "f32")]
     for L, k, dt in shapes:
         prm = Q.LattParm((L, L), 4.0)
-        if args.tile:
-            Q.lib.qed2d_ctx_set_tile(prm.ctx(), args.tile)
-        if args.tile32:
-            Q.lib.qed2d_ctx_set_tile_f32(prm.ctx(), args.tile32)
         cdt = torch.complex128 if dt == "f64" else torch.complex64
         U = torch.exp(1j * torch.rand((2, L, L), dtype=torch.float64, device="cuda") * 6.283)
         U = U.to(cdt)
@@ -229,28 +247,57 @@ def dw_apply(Q, torch, args):
             else:
                 Q.gamm5Dw_f32(so, U, si, 0.1, prm)
         Q.lib.qed2d_ctx_set_sync(prm.ctx(), 0)
-        for _ in range(3):
-            apply()
         reps = 50 if L <= 1024 else 20
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record()
-        for _ in range(reps):
-            apply()
-        e1.record()
-        torch.cuda.synchronize()
-        us = e0.elapsed_time(e1) * 1000.0 / reps
+        per_tile = {}
+        for rows in DW_TILES:
+            if dt == "f64":
+                Q.set_tiles(prm, q64=rows)
+            else:
+                Q.set_tiles(prm, q32=rows)
+            for _ in range(3):
+                apply()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                apply()
+            e1.record()
+            torch.cuda.synchronize()
+            per_tile[rows] = e0.elapsed_time(e1) * 1000.0 / reps
+        rows = min(per_tile, key=per_tile.get)
+        us = per_tile[rows]
         b = (16 if dt == "f64" else 8)
         bytes_site = 2 * b + 4 * b * k          # U (2 links) + k x (read + write 2 components)
         gbs = bytes_site * L * L / (us * 1e-6) / 1e9
         key = f"{L}x{L}x{k}rhs_{dt}"
         out[key] = {"us": round(us, 2), "GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3),
-                    "bytes_per_site": bytes_site}
-        log(f"dw {key}: {us:.1f} us, {gbs:.0f} GB/s")
+                    "bytes_per_site": bytes_site, "tile": rows,
+                    "us_per_tile": {str(r): round(v, 2) for r, v in per_tile.items()}}
+        log(f"dw {key}: {us:.1f} us, {gbs:.0f} GB/s (tile {rows})")
         del U, si, so
         prm.close()
         torch.cuda.empty_cache()
     return out
+
+
+DW_TILES = (2, 4, 8)
+
+
+def pmc_traffic(kernel, L):
+    """HBM bytes per launch of `kernel` on an L x L lattice, from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a copy kernel of known size), or None
+    when that summary does not cover the kernel at this size."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = d.get("kernels", {}).get(kernel)
+    if not e or e.get("lattice") != [L, L]:
+        return None
+    return e.get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(args, U0, prm):
@@ -287,6 +334,91 @@ def cpu_baseline(args, U0, prm):
                       f"update) and one link update at {L}x{L}, scaled by the leapfrog's "
                       f"{args.ns + 1} force evaluations and {args.ns} link updates per "
                       f"trajectory"}
+
+
+# ---------------------------------------------------------------------------
+# rocprofv3 summaries: python bench.py --rocprof-summary KT.db FETCH.db WRITE.db
+#   [--L 1024] [--json profiles/pmc_traffic.json]
+# KT.db from `rocprofv3 --kernel-trace --stats`, FETCH.db / WRITE.db from
+# `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs of the same bench
+# command.  Prints a markdown summary; with --json also writes the per-launch
+# HBM traffic of the kernel classes that pmc_traffic() reads.  The counters
+# are scaled by what they read for the STREAM copy kernel (1 GiB read, 1 GiB
+# written per dispatch) in the same runs.
+# ---------------------------------------------------------------------------
+KERNEL_CLASS = [("k_cg1v", "cg_k1_f32"), ("k_cg2v", "cg_k2_f32"), ("k_cg1<float", "cg_k1_f32"),
+                ("k_cg2<float", "cg_k2_f32"), ("k_cg1<double", "cg_k1_f64"),
+                ("k_cg2<double", "cg_k2_f64"), ("k_q_apply<double", "q_f64"),
+                ("k_force_mom", "force_mom"), ("k_updt_fld", "updt_fld")]
+
+
+def _short(name):
+    import re
+    name = re.sub(r"\(.*", "", name)
+    return name.replace("void ", "").replace("qed2d::", "")
+
+
+def rocprof_summary(argv):
+    import sqlite3
+    import numpy as np
+    ap = argparse.ArgumentParser(prog="bench.py --rocprof-summary")
+    ap.add_argument("kt")
+    ap.add_argument("fetch")
+    ap.add_argument("write")
+    ap.add_argument("--L", type=int, default=1024)
+    ap.add_argument("--json", default="")
+    a = ap.parse_args(argv)
+    con = sqlite3.connect(a.kt)
+    by = {}
+    for n, d in con.execute("select name, duration from kernels"):
+        by.setdefault(_short(n), []).append(d / 1000.0)
+    rows = sorted(((n, np.array(d)) for n, d in by.items()), key=lambda r: -r[1].sum())
+    total = sum(d.sum() for _, d in rows)
+    print("| kernel | calls | total us | mean us | median us (calls > 5 us) | % time |")
+    print("|---|---|---|---|---|---|")
+    for n, d in rows[:16]:
+        w = d[d > 5.0]
+        print(f"| {n} | {len(d)} | {d.sum():.0f} | {d.mean():.2f} | "
+              f"{np.median(w) if len(w) else 0.0:.2f} | {100 * d.sum() / total:.1f} |")
+
+    def counter(db, cname):
+        c = sqlite3.connect(db)
+        out = {}
+        for n, v in c.execute("select kernel_name, value from counters_collection "
+                              "where counter_name=?", (cname,)):
+            out.setdefault(_short(n), []).append(v)
+        return {n: np.array(v) for n, v in out.items()}
+    F, W = counter(a.fetch, "FETCH_SIZE"), counter(a.write, "WRITE_SIZE")
+    gib_kib = float(1 << 20)
+    if "k_stream_copy" not in F or "k_stream_copy" not in W:
+        sys.exit("the counter runs hold no k_stream_copy dispatch to calibrate against")
+    fcal = gib_kib / np.median(F["k_stream_copy"])
+    wcal = gib_kib / np.median(W["k_stream_copy"])
+    print()
+    print(f"Calibration (STREAM copy, 1 GiB read + 1 GiB written per dispatch): FETCH_SIZE x "
+          f"{fcal:.3f}, WRITE_SIZE x {wcal:.3f}")
+    print()
+    print("| kernel | dispatches | FETCH_SIZE KiB (median, dispatches > 1 MiB) | "
+          "WRITE_SIZE KiB (median) | HBM MB per launch (scaled) |")
+    print("|---|---|---|---|---|")
+    traffic = {}
+    for n in sorted(set(F) | set(W), key=lambda n: -np.sum(F.get(n, [0]))):
+        f = F.get(n, np.zeros(1))
+        w = W.get(n, np.zeros(1))
+        fb, wb = f[f > 1024], w[w > 1024]
+        fm = float(np.median(fb) if len(fb) else np.median(f))
+        wm = float(np.median(wb) if len(wb) else np.median(w))
+        mb = (fm * fcal + wm * wcal) * 1024 / 1e6
+        print(f"| {n} | {len(f)} | {fm:.0f} | {wm:.0f} | {mb:.1f} |")
+        for pat, cls in KERNEL_CLASS:
+            if n.startswith(pat) and cls not in traffic:
+                traffic[cls] = {"kernel": n, "lattice": [a.L, a.L], "fetch_kib": fm,
+                                "write_kib": wm, "fetch_scale": fcal, "write_scale": wcal,
+                                "hbm_bytes_per_launch": int(round((fm * fcal + wm * wcal) * 1024))}
+    if a.json:
+        with open(a.json, "w") as fo:
+            json.dump({"source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py",
+                       "kernels": traffic}, fo, indent=1)
 
 
 if __name__ == "__main__":

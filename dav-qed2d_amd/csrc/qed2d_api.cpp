@@ -1130,6 +1130,31 @@ int qed2d_ctx_autotune(qed2d_ctx *c, int iters, int *tiles)
     return finish(c);
 }
 
+int qed2d_hbm_stream(qed2d_ctx *c, long nbytes, double *gbs)
+{
+    if (!c || !gbs) return QED2D_EINVAL;
+    const long n = nbytes / (long)sizeof(double2);
+    if (n < kBlock) return fail(c, QED2D_EINVAL, "stream arrays too small");
+    void *mem = nullptr;
+    if (hipMalloc(&mem, 3 * n * sizeof(double2) + kStreamReadBlocks * sizeof(double)) !=
+        hipSuccess)
+        return fail(c, QED2D_ENOMEM, "hipMalloc of 3 x %ld bytes failed", nbytes);
+    double2 *a = (double2 *)mem, *b = a + n, *d = b + n;
+    double *part = (double *)(d + n);
+    HIPCHK(c, hipMemsetAsync(a, 0x3f, n * sizeof(double2), c->stream));
+    HIPCHK(c, hipMemsetAsync(b, 0x40, n * sizeof(double2), c->stream));
+    const double moved[3] = {2.0, 3.0, 1.0};  // arrays read + written per element
+    int rc = QED2D_OK;
+    for (int w = 0; w < 3 && !rc; ++w) {
+        double us;
+        rc = time_runs(c, 10, [&] { launch_stream(w, a, b, d, part, n, c->stream); }, &us);
+        gbs[w] = moved[w] * n * sizeof(double2) / (us * 1e-6) / 1e9;
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(mem);
+    return rc ? rc : finish(c);
+}
+
 int qed2d_synchronize(qed2d_ctx *c)
 {
     if (!c) return QED2D_EINVAL;
@@ -1171,8 +1196,10 @@ int qed2d_gamm5Dw_mrhs(qed2d_ctx *c, double *so, const double *U, const double *
     NEED(c, so); NEED(c, U); NEED(c, si);
     NOALIAS(c, so, si);
     if (nrhs < 1) return fail(c, QED2D_EINVAL, "nrhs must be >= 1");
+    if ((long)march_grid(c->L, c->rows_q).nblocks() * nrhs > 0x7fffffffL)
+        return fail(c, QED2D_EINVAL, "nrhs %d too large for one launch", nrhs);
     launch_q_apply_mrhs_f64(c->L, (const double2 *)U, (const double2 *)si, (double2 *)so, nrhs,
-                            am0, c->stream);
+                            am0, c->rows_q, c->stream);
     return finish(c);
 }
 
@@ -1183,8 +1210,10 @@ int qed2d_gamm5Dw_mrhs_f32(qed2d_ctx *c, float *so, const float *U, const float 
     NEED(c, so); NEED(c, U); NEED(c, si);
     NOALIAS(c, so, si);
     if (nrhs < 1) return fail(c, QED2D_EINVAL, "nrhs must be >= 1");
+    if ((long)march_grid(c->L, c->rows_q32).nblocks() * nrhs > 0x7fffffffL)
+        return fail(c, QED2D_EINVAL, "nrhs %d too large for one launch", nrhs);
     launch_q_apply_mrhs_f32(c->L, (const float2 *)U, (const float2 *)si, (float2 *)so, nrhs, am0,
-                            c->stream);
+                            c->rows_q32, c->stream);
     return finish(c);
 }
 

@@ -82,9 +82,14 @@ void launch_q_apply_f64(const Latt &L, const double2 *U, const double2 *si, doub
 void launch_q_apply_f32(const Latt &L, const float2 *U, const float2 *si, float2 *so,
                         float am0, int rows, hipStream_t st);
 void launch_q_apply_mrhs_f64(const Latt &L, const double2 *U, const double2 *si, double2 *so,
-                             int nrhs, double am0, hipStream_t st);
+                             int nrhs, double am0, int rows, hipStream_t st);
 void launch_q_apply_mrhs_f32(const Latt &L, const float2 *U, const float2 *si, float2 *so,
-                             int nrhs, float am0, hipStream_t st);
+                             int nrhs, float am0, int rows, hipStream_t st);
+// STREAM kernels: which 0 = copy c = a, 1 = triad c = a + b/2, 2 = read a
+// (kStreamReadBlocks partial sums into part).
+constexpr int kStreamReadBlocks = 1024;
+void launch_stream(int which, const double2 *a, const double2 *b, double2 *c, double *part,
+                   long n, hipStream_t st);
 void launch_gamm5(const Latt &L, const double2 *si, double2 *so, hipStream_t st);
 // so = Q si + mu^2 x  (x may alias nothing; used for Q^2 + mu^2 with si = Q x)
 void launch_q_shift_f64(const Latt &L, const double2 *U, const double2 *si, const double2 *x,

@@ -169,57 +169,121 @@ void launch_q_shift_f64(const Latt &L, const double2 *U, const double2 *si, cons
 // loaded once and reused for every RHS (32 + 64k bytes per site in FP64).
 // Layout: RHS j occupies si[j * 2V .. (j+1) * 2V).
 // ---------------------------------------------------------------------------
-template <typename T>
+// k right-hand sides sharing the links.  The logical block index t (grouped
+// by XCD) maps to rhs = t % nrhs, tile = t / nrhs, so the nrhs blocks of one
+// tile run together on one XCD and its links come from HBM once.
+template <typename T, int R>
 __global__ void __launch_bounds__(kBlock) k_q_mrhs(Latt L, const c2<T> *__restrict__ U,
                                                    const c2<T> *__restrict__ si,
                                                    c2<T> *__restrict__ so, int nrhs, T mass2,
                                                    int bx, int by, int nbx)
 {
-    MarchPos p = march_pos(L, bx, by, nbx, 1);
-    if (!p.active) return;
-    const int L1 = L.L1, L2 = L.L2;
-    const long V = L.V;
-    const int i1 = p.i1, r = p.r0;
-    const int i1p = (i1 + 1 == L1) ? 0 : i1 + 1;
-    const int i1m = (i1 == 0) ? L1 - 1 : i1 - 1;
-    const int rp = (r + 1 == L2) ? 0 : r + 1;
-    const int rm = (r == 0) ? L2 - 1 : r - 1;
-    const bool ap = (L.bc & 1) != 0;
-    const long s = (long)r * L1 + i1;
-    const c2<T> U1 = U[s], U1m = U[(long)r * L1 + i1m];
-    c2<T> U2 = U[V + s], U2m = U[V + (long)rm * L1 + i1];
-    if (ap && r == L2 - 1) U2 = cneg<T>(U2);
-    if (ap && r == 0) U2m = cneg<T>(U2m);
-    const long sp1 = (long)r * L1 + i1p, sm1 = (long)r * L1 + i1m;
-    const long sp2 = (long)rp * L1 + i1, sm2 = (long)rm * L1 + i1;
-    for (int j = 0; j < nrhs; ++j) {
-        const c2<T> *in = si + (long)j * 2 * V;
-        c2<T> *out = so + (long)j * 2 * V;
-        c2<T> o1, o2;
-        q_site<T>(mass2, in[s], in[V + s], in[sp1], in[V + sp1], in[sm1], in[V + sm1], in[sp2],
-                  in[V + sp2], in[sm2], in[V + sm2], U1, U1m, U2, U2m, o1, o2);
-        out[s] = o1;
-        out[V + s] = o2;
-    }
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = t % nrhs, tile = t / nrhs;
+    const int i1 = (tile % nbx) * bx + threadIdx.x % bx;
+    const int r0 = ((tile / nbx) * by + threadIdx.x / bx) * R;
+    if (i1 >= L.L1 || r0 >= L.L2) return;
+    SrcPlain<T> src{si + (long)j * 2 * L.V, L.V};
+    EpiStore<T> epi{so + (long)j * 2 * L.V, L.V};
+    q_march<T, R>(L, U, mass2, src, epi, i1, r0);
+}
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_q_mrhs2(Latt L, const float2 *__restrict__ U,
+                                                    const float2 *__restrict__ si,
+                                                    float2 *__restrict__ so, int nrhs,
+                                                    float mass2, int bx, int by, int nbx)
+{
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = t % nrhs, tile = t / nrhs;
+    const int jp = (tile % nbx) * bx + threadIdx.x % bx;
+    const int r0 = ((tile / nbx) * by + threadIdx.x / bx) * R;
+    if (jp >= L.L1 / 2 || r0 >= L.L2) return;
+    SrcPlain2 src{si + (long)j * 2 * L.V, L.V};
+    EpiStore2 epi{so + (long)j * 2 * L.V, L.V};
+    q_march2<R>(L, U, mass2, src, epi, jp, r0);
 }
 
 template <typename T>
 static void q_mrhs_impl(const Latt &L, const c2<T> *U, const c2<T> *si, c2<T> *so, int nrhs,
-                        T am0, hipStream_t st)
+                        T am0, int rows, hipStream_t st)
 {
-    MarchGrid g = march_grid(L, 1);
-    hipLaunchKernelGGL(k_q_mrhs<T>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, U, si, so, nrhs,
-                       T(2) + am0, g.bx, g.by, g.nbx);
+    const T m2 = T(2) + am0;
+    if constexpr (std::is_same<T, float>::value) {
+        if (pair_ok(L)) {
+            MarchGrid g = march_grid2(L, rows);
+            dim3 grid(g.nblocks() * nrhs), blk(kBlock);
+            DISPATCH_ROWS(rows, k_q_mrhs2, grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by,
+                          g.nbx);
+            return;
+        }
+    }
+    MarchGrid g = march_grid(L, rows);
+    dim3 grid(g.nblocks() * nrhs), blk(kBlock);
+    switch (rows) {
+    case 1: hipLaunchKernelGGL((k_q_mrhs<T, 1>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
+    case 2: hipLaunchKernelGGL((k_q_mrhs<T, 2>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
+    case 4: hipLaunchKernelGGL((k_q_mrhs<T, 4>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
+    case 8: hipLaunchKernelGGL((k_q_mrhs<T, 8>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
+    default: hipLaunchKernelGGL((k_q_mrhs<T, 16>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
+    }
 }
 void launch_q_apply_mrhs_f64(const Latt &L, const double2 *U, const double2 *si, double2 *so,
-                             int nrhs, double am0, hipStream_t st)
+                             int nrhs, double am0, int rows, hipStream_t st)
 {
-    q_mrhs_impl<double>(L, U, si, so, nrhs, am0, st);
+    q_mrhs_impl<double>(L, U, si, so, nrhs, am0, rows, st);
 }
 void launch_q_apply_mrhs_f32(const Latt &L, const float2 *U, const float2 *si, float2 *so,
-                             int nrhs, float am0, hipStream_t st)
+                             int nrhs, float am0, int rows, hipStream_t st)
 {
-    q_mrhs_impl<float>(L, U, si, so, nrhs, am0, st);
+    q_mrhs_impl<float>(L, U, si, so, nrhs, am0, rows, st);
+}
+
+// ===========================================================================
+// STREAM kernels (achievable HBM bandwidth of the device, 16-byte elements):
+// copy b = a, triad c = a + s b, read sum(a) into per-block partials.
+// ===========================================================================
+__global__ void __launch_bounds__(kBlock) k_stream_copy(const double2 *__restrict__ a,
+                                                        double2 *__restrict__ b, long n)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+__global__ void __launch_bounds__(kBlock) k_stream_triad(const double2 *__restrict__ a,
+                                                         const double2 *__restrict__ b,
+                                                         double2 *__restrict__ c, double s,
+                                                         long n)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double2 x = a[i], y = b[i];
+        c[i] = mk<double>(x.x + s * y.x, x.y + s * y.y);
+    }
+}
+__global__ void __launch_bounds__(kBlock) k_stream_read(const double2 *__restrict__ a,
+                                                        double *__restrict__ part, long n)
+{
+    __shared__ double lds[kBlock / kWave];
+    double acc = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        const double2 x = a[i];
+        acc += x.x + x.y;
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+void launch_stream(int which, const double2 *a, const double2 *b, double2 *c, double *part,
+                   long n, hipStream_t st)
+{
+    const long nb = (n + kBlock - 1) / kBlock;
+    if (which == 0)
+        hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(kBlock), 0, st, a, c, n);
+    else if (which == 1)
+        hipLaunchKernelGGL(k_stream_triad, dim3(nb), dim3(kBlock), 0, st, a, b, c, 0.5, n);
+    else
+        hipLaunchKernelGGL(k_stream_read, dim3(kStreamReadBlocks), dim3(kBlock), 0, st, a, part,
+                           n);
 }
 
 // ===========================================================================

@@ -57,6 +57,7 @@ SIGNATURES = {
     "qed2d_ctx_set_tiles": (_i, [_p, _i, _i, _i, _i]),
     "qed2d_ctx_get_tiles": (_i, [_p, _p]),
     "qed2d_ctx_autotune": (_i, [_p, _i, _p]),
+    "qed2d_hbm_stream": (_i, [_p, C.c_long, _p]),
     "qed2d_ctx_set_reliable_delta": (_i, [_p, _d]),
     "qed2d_synchronize": (_i, [_p]),
     "qed2d_gamm5Dw": (_i, [_p, _p, _p, _p, _d]),
@@ -141,6 +142,13 @@ def autotune(prm, iters=40):
     t = (_i * 4)()
     _check(prm, lib.qed2d_ctx_autotune(prm.ctx(), int(iters), t))
     return dict(zip(("q64", "q32", "cg64", "cg32"), list(t)))
+
+
+def hbm_stream(prm, nbytes=1 << 30):
+    """STREAM copy / triad / read bandwidth of the device in GB/s."""
+    g = (_d * 3)()
+    _check(prm, lib.qed2d_hbm_stream(prm.ctx(), int(nbytes), g))
+    return dict(zip(("copy", "triad", "read"), list(g)))
 
 
 def profile_enable(prm, on=True, every=1):

@@ -91,6 +91,11 @@ int qed2d_ctx_get_tiles(qed2d_ctx *ctx, int tiles[4]);
  * receives the choice.  Tile heights change the order of the solvers' global
  * sums, so a run that must reproduce bits fixes its tiles instead. */
 int qed2d_ctx_autotune(qed2d_ctx *ctx, int iters, int tiles[4]);
+/* Achievable HBM bandwidth of the context's device, for the roofline beside
+ * the datasheet peak: STREAM copy, triad and read over three scratch arrays of
+ * nbytes each (allocated and freed here); gbs[3] receives GB/s counting the
+ * bytes each kernel must move.  No counterpart in the reference. */
+int qed2d_hbm_stream(qed2d_ctx *ctx, long nbytes, double gbs[3]);
 /* Mixed-precision CG: a reliable update (FP64 true residual) once the FP32
  * iterated residual has dropped by delta since the last one (default 0.01). */
 int qed2d_ctx_set_reliable_delta(qed2d_ctx *ctx, double delta);

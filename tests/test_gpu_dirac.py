@@ -65,19 +65,30 @@ def test_gamm5Dw_f32(Q, O, L1, L2, bc):
         assert relerr(host(so), ref) < 2e-6, rows
 
 
-@pytest.mark.parametrize("k", [1, 3, 16])
-def test_gamm5Dw_mrhs(Q, O, k):
-    say("gamm5Dw mrhs", k)
-    L1, L2 = 64, 32
-    rng = np.random.default_rng(k)
-    U = random_links(L1, L2, rng)
+@pytest.mark.parametrize("k,L1,L2,bc", [(1, 64, 32, 0), (3, 64, 32, 1), (16, 64, 32, 0),
+                                        (5, 33, 20, 1)])
+def test_gamm5Dw_mrhs(Q, O, k, L1, L2, bc):
+    """Every tile height; FP64 and FP32 (pair kernel for even L1, one-site
+    kernel for odd L1)."""
+    say("gamm5Dw mrhs", k, L1, L2, bc)
+    rng = np.random.default_rng(k + L1)
+    U = random_links(L1, L2, rng, unitary=True)
     si = np.stack([random_spinor(L1, L2, rng) for _ in range(k)])
-    prm = Q.LattParm((L1, L2), 4.0)
-    so = dev(np.zeros_like(si))
-    Q.gamm5Dw_mrhs(so, dev(U), dev(si), 0.15, prm)
-    out = host(so)
-    for j in range(k):
-        assert relerr(out[j], O.gamm5Dw(U, si[j], 0.15)) < 1e-13
+    prm = Q.LattParm((L1, L2), 4.0, bc)
+    refs = [O.gamm5Dw(U, si[j], 0.15, bc) for j in range(k)]
+    dU, dsi = dev(U), dev(si)
+    dU32, dsi32 = dev(U.astype(np.complex64)), dev(si.astype(np.complex64))
+    for rows in (1, 2, 4, 8, 16):
+        Q.set_tiles(prm, q64=rows, q32=rows)
+        so = dev(np.zeros_like(si))
+        Q.gamm5Dw_mrhs(so, dU, dsi, 0.15, prm)
+        out = host(so)
+        so32 = dev(np.zeros_like(si, dtype=np.complex64))
+        Q.gamm5Dw_mrhs(so32, dU32, dsi32, 0.15, prm)
+        out32 = host(so32)
+        for j in range(k):
+            assert relerr(out[j], refs[j]) < 1e-13, (rows, j)
+            assert relerr(out32[j], refs[j]) < 2e-6, (rows, j)
 
 
 def test_operator_compositions(Q, O):
