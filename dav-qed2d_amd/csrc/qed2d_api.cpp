@@ -581,10 +581,6 @@ int multishift_cg(qed2d_ctx *c, double2 *const *x, const double *U, const double
 // ---------------------------------------------------------------------------
 // MD
 // ---------------------------------------------------------------------------
-const double OMF_R1 = 0.08398315262876693;
-const double OMF_R2 = 0.2539785108410595;
-const double OMF_R3 = 0.6822365335719091;
-const double OMF_R4 = -0.03230286765269967;
 
 void updt(qed2d_ctx *c, double *U, const double *mom, double eps)
 {
@@ -979,6 +975,17 @@ int autotune_impl(qed2d_ctx *c, int iters)
 bool overlaps(const void *a, const void *b) { return a && b && a == b; }
 
 }  // namespace
+
+namespace qed2d {
+int ctx_hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, double *hi, double *lo)
+{
+    hdd h;
+    TRY(hamiltonian_dd(c, mom, U, &h));
+    *hi = h.hi;
+    *lo = h.lo;
+    return QED2D_OK;
+}
+}  // namespace qed2d
 
 // ===========================================================================
 // C ABI

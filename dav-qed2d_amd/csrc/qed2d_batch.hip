@@ -1,0 +1,422 @@
+// qed2d_batch.hip -- batched ensembles: many independent lattices ("chains")
+// of different sizes, couplings and masses advanced together.  Each kernel
+// here is one launch over every chain: a block finds its chain by binary
+// search over the launch's block offsets (tile kernels) or site offsets
+// (site kernels), then runs the same device code as the single-lattice
+// kernels on that chain's slice of the arena.  A chain tiled like a
+// single-lattice launch of the same tile height therefore produces the same
+// bits as that launch, partial sums included.
+//
+// Per-chain CG: every chain keeps its own iteration index in its solver state
+// (K1 reads kk, commits k2 for K2; K2 advances kk), so chains that finished or
+// wait for a reliable update freeze while the others iterate.
+#include <type_traits>
+
+#include "qed2d_cg.h"
+#include "qed2d_fields.h"
+
+namespace qed2d {
+
+struct BWhere {
+    int c;
+    Latt L;
+    long o;      // element offset of the chain's fields (2 * off)
+    int pb;      // first partial slot of the chain
+    int nb;      // blocks of the chain in this launch
+    double am0, beta;
+    MarchPos p;  // p.blk: block index inside the chain
+};
+
+template <bool PAIR>
+__device__ __forceinline__ BWhere bwhere(const BGeo &g, int R)
+{
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    int lo = 0, hi = g.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g.blk0[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const BChain &ch = g.ch[lo];
+    const int4 sh = g.shape[lo];
+    const int local = b - g.blk0[lo];
+    BWhere w;
+    w.c = lo;
+    w.L = ch.L;
+    w.o = 2 * ch.off;
+    w.pb = ch.pb;
+    w.nb = sh.w;
+    w.am0 = ch.am0;
+    w.beta = ch.beta;
+    const int tx = threadIdx.x % sh.x, ty = threadIdx.x / sh.x;
+    w.p.i1 = (local % sh.z) * sh.x + tx;
+    w.p.r0 = ((local / sh.z) * sh.y + ty) * R;
+    w.p.active = (w.p.i1 < (PAIR ? ch.L.L1 / 2 : ch.L.L1)) && (w.p.r0 < ch.L.L2);
+    w.p.blk = local;
+    return w;
+}
+
+// Chain of arena element e (0 <= e < 2 * total sites).
+__device__ __forceinline__ int chain_of_elem(const BChain *ch, int n, long e)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (2 * ch[mid].off <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// so = Q si on every chain (on the chains waiting for a reliable update only,
+// when mask is given).
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ void __launch_bounds__(kBlock) kb_q_apply(BGeo g, const double2 *__restrict__ U,
+                                                     const double2 *__restrict__ si,
+                                                     double2 *__restrict__ so, const int *mask)
+{
+    const BWhere w = bwhere<false>(g, R);
+    if (!w.p.active) return;
+    if (mask && !mask[w.c]) return;
+    SrcPlain<double> src{si + w.o, w.L.V};
+    EpiStore<double> epi{so + w.o, w.L.V};
+    q_march<double, R>(w.L, U + w.o, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
+}
+void launch_b_q_apply(const BGeo &g, int rows, const double2 *U, const double2 *si,
+                      double2 *so, const int *mask, hipStream_t st)
+{
+    DISPATCH_ROWS(rows, kb_q_apply, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, si, so, mask);
+}
+
+// ---------------------------------------------------------------------------
+// Gauge + fermion force and momentum update, one thread per arena site.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) kb_force_mom(const BChain *ch, int n, long vtot,
+                                                       const double2 *__restrict__ U,
+                                                       const double2 *__restrict__ X,
+                                                       const double2 *__restrict__ Y,
+                                                       double *__restrict__ mom, double eps)
+{
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < vtot;
+         e += (long)gridDim.x * blockDim.x) {
+        const int c = chain_of_elem(ch, n, 2 * e);
+        const BChain &b = ch[c];
+        const long s = e - b.off;
+        const long o = 2 * b.off;
+        force_mom_site(b.L, U + o, X ? X + o : nullptr, Y ? Y + o : nullptr, nullptr, mom + o,
+                       b.beta, eps, 1.0, s);
+    }
+}
+void launch_b_force_mom(const BChain *ch, int nchain, long vtot, const double2 *U,
+                        const double2 *X, const double2 *Y, double *mom, double eps,
+                        hipStream_t st)
+{
+    long nb = (vtot + kBlock - 1) / kBlock;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(kb_force_mom, dim3((int)nb), dim3(kBlock), 0, st, ch, nchain, vtot, U, X,
+                       Y, mom, eps);
+}
+
+// ---------------------------------------------------------------------------
+// CG start from x = 0 (src/QED2dSolver.jl:10-16) on every chain.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kb_cg_init(BGeo g, int rows,
+                                                     const double2 *__restrict__ b,
+                                                     double2 *__restrict__ x,
+                                                     c2<T> *__restrict__ r,
+                                                     c2<T> *__restrict__ p0,
+                                                     float2 *__restrict__ xs, BParts P)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const BWhere w = bwhere<false>(g, rows);
+    const Latt &L = w.L;
+    double acc = 0.0;
+    if (w.p.active) {
+        for (int kk = 0; kk < rows; ++kk) {
+            const int row = w.p.r0 + kk;
+            if (row >= L.L2) break;
+            for (int c = 0; c < 2; ++c) {
+                const long i = w.o + c * L.V + (long)row * L.L1 + w.p.i1;
+                const double2 v = b[i];
+                x[i] = mk<double>(0.0, 0.0);
+                r[i] = from_d2<T>(v);
+                p0[i] = mk<T>(T(0), T(0));
+                if (xs) xs[i] = mk<float>(0.0f, 0.0f);
+                acc += v.x * v.x + v.y * v.y;
+            }
+        }
+    }
+    const double s = block_allsum(acc, lds);
+    if (threadIdx.x == 0) {
+        P.rr[0][w.pb + w.p.blk] = s;
+        P.b[w.pb + w.p.blk] = s;
+        if (w.p.blk == 0) {
+            P.nrr[0][w.c] = w.nb;
+            P.nb[w.c] = w.nb;
+        }
+    }
+}
+void launch_b_cg_init(const BGeo &g, int rows, const double2 *b, double2 *x, void *r, void *p0,
+                      float2 *xs, int mixed, BParts P, hipStream_t st)
+{
+    if (mixed)
+        hipLaunchKernelGGL(kb_cg_init<float>, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, b,
+                           x, (float2 *)r, (float2 *)p0, xs, P);
+    else
+        hipLaunchKernelGGL(kb_cg_init<double>, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, b,
+                           x, (double2 *)r, (double2 *)p0, (float2 *)nullptr, P);
+}
+
+// ---------------------------------------------------------------------------
+// Residual from x with t = Q x: mode 0 starts an FP64 solve from a guess on
+// every chain; mode 2 is the reliable update of the chains that wait for one
+// (mask[c] != 0, set by the host from the states it read: need_ru itself is
+// cleared by this kernel and cannot select its blocks).
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__restrict__ U,
+                                                      const double2 *__restrict__ t,
+                                                      const double2 *__restrict__ b,
+                                                      const double2 *__restrict__ x,
+                                                      double2 *__restrict__ r,
+                                                      float2 *__restrict__ rs,
+                                                      double2 *__restrict__ p0, SolverState *S,
+                                                      BParts P, int mode, const int *mask)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const BWhere w = bwhere<false>(g, R);
+    SolverState *Sc = S + w.c;
+    if (mode == 2 && !mask[w.c]) return;
+    const int k = (mode == 2) ? Sc->kk : 1;
+    const long o = w.o;
+    EpiResid epi{b + o, x + o, r ? r + o : nullptr, rs ? rs + o : nullptr,
+                 p0 ? p0 + o : nullptr, nullptr, Sc->mu2, w.L.V, 0.0, 0.0};
+    if (w.p.active) {
+        SrcPlain<double> src{t + o, w.L.V};
+        q_march<double, R>(w.L, U + o, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
+    }
+    const double a = block_allsum(epi.rr, lds);
+    const double bb = block_allsum(epi.bb, lds);
+    if (threadIdx.x == 0) {
+        const int rd = (k - 1) & 1;
+        P.rr[rd][w.pb + w.p.blk] = a;
+        if (mode != 2) P.b[w.pb + w.p.blk] = bb;
+        if (w.p.blk == 0) {
+            P.nrr[rd][w.c] = w.nb;
+            if (mode != 2) P.nb[w.c] = w.nb;
+            Sc->true_at = k;
+            if (mode == 2) {
+                Sc->need_ru = 0;
+                Sc->n_ru += 1;
+                Sc->stop_k = 0;
+            }
+        }
+    }
+}
+void launch_b_cg_resid(const BGeo &g, int rows, const double2 *U, const double2 *b,
+                       const double2 *t, const double2 *x, double2 *r, float2 *rs, double2 *p0,
+                       SolverState *S, BParts P, int mode, const int *mask, hipStream_t st)
+{
+    DISPATCH_ROWS(rows, kb_cg_resid, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, b, x, r, rs,
+                  p0, S, P, mode, mask);
+}
+
+// x += double(xs), xs = 0 on the chains that wait for a reliable update.
+__global__ void __launch_bounds__(kBlock) kb_ru_accum(const BChain *ch, int n, long nel,
+                                                      double2 *__restrict__ x,
+                                                      float2 *__restrict__ xs, const int *mask)
+{
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nel;
+         e += (long)gridDim.x * blockDim.x) {
+        if (!mask[chain_of_elem(ch, n, e)]) continue;
+        const float2 d = xs[e];
+        const double2 v = x[e];
+        x[e] = mk<double>(v.x + (double)d.x, v.y + (double)d.y);
+        xs[e] = mk<float>(0.0f, 0.0f);
+    }
+}
+void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,
+                       const int *mask, hipStream_t st)
+{
+    long nb = (2 * vtot + kBlock - 1) / kBlock;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(kb_ru_accum, dim3((int)nb), dim3(kBlock), 0, st, ch, nchain, 2 * vtot, x,
+                       xs, mask);
+}
+
+// ---------------------------------------------------------------------------
+// CG iteration kernels, per-chain iteration index (see the file comment).
+// ---------------------------------------------------------------------------
+template <typename T, int R>
+__global__ void __launch_bounds__(kBlock) kb_cg1(BGeo g, const c2<T> *__restrict__ U,
+                                                 const c2<T> *__restrict__ r, c2<T> *P0,
+                                                 c2<T> *P1, c2<T> *__restrict__ t,
+                                                 SolverState *S, BParts P)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const BWhere w = bwhere<false>(g, R);
+    SolverState *Sc = S + w.c;
+    if (Sc->stop_k != 0) return;
+    const int k = Sc->kk;
+    const int wp = k & 1, rd = (k - 1) & 1;
+    if (w.p.blk == 0 && threadIdx.x == 0) Sc->k2 = k;
+    double beta;
+    if (!cg_k1_scalars<std::is_same<T, float>::value>(Sc, P.rr[rd] + w.pb, P.b + w.pb,
+                                                         P.nrr[rd][w.c], P.nb[w.c], k, w.p.blk,
+                                                         lds, beta))
+        return;
+    const long o = w.o;
+    const c2<T> *pold = (rd ? P1 : P0) + o;
+    c2<T> *pnew = (wp ? P1 : P0) + o;
+    EpiK1<T> epi{pnew, t + o, w.L.V, 0.0, 0.0};
+    if (w.p.active) {
+        SrcXpay<T> src{r + o, pold, (T)beta, w.L.V};
+        q_march<T, R>(w.L, U + o, T(2) + (T)w.am0, src, epi, w.p.i1, w.p.r0);
+    }
+    const double a = block_allsum(epi.tt, lds);
+    const double b = block_allsum(epi.pp, lds);
+    if (threadIdx.x == 0) {
+        P.tt[wp][2 * (w.pb + w.p.blk)] = a;
+        P.tt[wp][2 * (w.pb + w.p.blk) + 1] = b;
+        if (w.p.blk == 0) P.ntt[wp][w.c] = w.nb;
+    }
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(kBlock) kb_cg2(BGeo g, const c2<T> *__restrict__ U,
+                                                 const c2<T> *__restrict__ t, const c2<T> *P0,
+                                                 const c2<T> *P1, c2<T> *__restrict__ x,
+                                                 c2<T> *__restrict__ r, SolverState *S, BParts P)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const BWhere w = bwhere<false>(g, R);
+    SolverState *Sc = S + w.c;
+    if (Sc->stop_k != 0) return;
+    const int k = Sc->k2;
+    const int wp = k & 1;
+    const double tt = sum_prev_partials(P.tt[wp] + 2 * w.pb, P.ntt[wp][w.c], 2, 0, lds);
+    const double pp = sum_prev_partials(P.tt[wp] + 2 * w.pb, P.ntt[wp][w.c], 2, 1, lds);
+    const double alpha = Sc->norm[wp] / (tt + Sc->mu2 * pp);
+    const long o = w.o;
+    EpiK2<T> epi{(wp ? P1 : P0) + o, x + o, r + o, (T)alpha, (T)Sc->mu2, w.L.V, 0.0};
+    if (w.p.active) {
+        SrcPlain<T> src{t + o, w.L.V};
+        q_march<T, R>(w.L, U + o, T(2) + (T)w.am0, src, epi, w.p.i1, w.p.r0);
+    }
+    const double a = block_allsum(epi.rr, lds);
+    if (threadIdx.x == 0) {
+        P.rr[wp][w.pb + w.p.blk] = a;
+        if (w.p.blk == 0) {
+            P.nrr[wp][w.c] = w.nb;
+            Sc->kk = k + 1;
+        }
+    }
+}
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, const float2 *__restrict__ U,
+                                                  const float2 *__restrict__ r, float2 *P0,
+                                                  float2 *P1, float2 *__restrict__ t,
+                                                  SolverState *S, BParts P)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const BWhere w = bwhere<true>(g, R);
+    SolverState *Sc = S + w.c;
+    if (Sc->stop_k != 0) return;
+    const int k = Sc->kk;
+    const int wp = k & 1, rd = (k - 1) & 1;
+    if (w.p.blk == 0 && threadIdx.x == 0) Sc->k2 = k;
+    double beta;
+    if (!cg_k1_scalars<true>(Sc, P.rr[rd] + w.pb, P.b + w.pb, P.nrr[rd][w.c], P.nb[w.c], k,
+                             w.p.blk, lds, beta))
+        return;
+    const long o = w.o;
+    EpiK1v epi{(wp ? P1 : P0) + o, t + o, w.L.V, 0.0, 0.0};
+    if (w.p.active) {
+        SrcXpay2 src{r + o, (rd ? P1 : P0) + o, (float)beta, w.L.V};
+        q_march2<R>(w.L, U + o, 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
+    }
+    const double a = block_allsum(epi.tt, lds);
+    const double b = block_allsum(epi.pp, lds);
+    if (threadIdx.x == 0) {
+        P.tt[wp][2 * (w.pb + w.p.blk)] = a;
+        P.tt[wp][2 * (w.pb + w.p.blk) + 1] = b;
+        if (w.p.blk == 0) P.ntt[wp][w.c] = w.nb;
+    }
+}
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, const float2 *__restrict__ U,
+                                                  const float2 *__restrict__ t, const float2 *P0,
+                                                  const float2 *P1, float2 *__restrict__ x,
+                                                  float2 *__restrict__ r, SolverState *S,
+                                                  BParts P)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const BWhere w = bwhere<true>(g, R);
+    SolverState *Sc = S + w.c;
+    if (Sc->stop_k != 0) return;
+    const int k = Sc->k2;
+    const int wp = k & 1;
+    const double tt = sum_prev_partials(P.tt[wp] + 2 * w.pb, P.ntt[wp][w.c], 2, 0, lds);
+    const double pp = sum_prev_partials(P.tt[wp] + 2 * w.pb, P.ntt[wp][w.c], 2, 1, lds);
+    const double alpha = Sc->norm[wp] / (tt + Sc->mu2 * pp);
+    const long o = w.o;
+    EpiK2v epi{(wp ? P1 : P0) + o, x + o, r + o, (float)alpha, (float)Sc->mu2, w.L.V, 0.0};
+    if (w.p.active) {
+        SrcPlain2 src{t + o, w.L.V};
+        q_march2<R>(w.L, U + o, 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
+    }
+    const double a = block_allsum(epi.rr, lds);
+    if (threadIdx.x == 0) {
+        P.rr[wp][w.pb + w.p.blk] = a;
+        if (w.p.blk == 0) {
+            P.nrr[wp][w.c] = w.nb;
+            Sc->kk = k + 1;
+        }
+    }
+}
+
+void launch_b_cg_k1_f64(const BGeo &g, int rows, const double2 *U, const double2 *r,
+                        double2 *p0, double2 *p1, double2 *t, SolverState *S, BParts P,
+                        hipStream_t st)
+{
+    dim3 grid(g.nblocks), blk(kBlock);
+    switch (rows) {
+    case 1: hipLaunchKernelGGL((kb_cg1<double, 1>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
+    case 2: hipLaunchKernelGGL((kb_cg1<double, 2>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
+    case 4: hipLaunchKernelGGL((kb_cg1<double, 4>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
+    case 8: hipLaunchKernelGGL((kb_cg1<double, 8>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
+    default: hipLaunchKernelGGL((kb_cg1<double, 16>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
+    }
+}
+void launch_b_cg_k2_f64(const BGeo &g, int rows, const double2 *U, const double2 *t,
+                        const double2 *p0, const double2 *p1, double2 *x, double2 *r,
+                        SolverState *S, BParts P, hipStream_t st)
+{
+    dim3 grid(g.nblocks), blk(kBlock);
+    switch (rows) {
+    case 1: hipLaunchKernelGGL((kb_cg2<double, 1>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
+    case 2: hipLaunchKernelGGL((kb_cg2<double, 2>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
+    case 4: hipLaunchKernelGGL((kb_cg2<double, 4>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
+    case 8: hipLaunchKernelGGL((kb_cg2<double, 8>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
+    default: hipLaunchKernelGGL((kb_cg2<double, 16>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
+    }
+}
+void launch_b_cg_k1_f32(const BGeo &g, int rows, const float2 *U, const float2 *r, float2 *p0,
+                        float2 *p1, float2 *t, SolverState *S, BParts P, hipStream_t st)
+{
+    DISPATCH_ROWS(rows, kb_cg1v, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, r, p0, p1, t, S, P);
+}
+void launch_b_cg_k2_f32(const BGeo &g, int rows, const float2 *U, const float2 *t,
+                        const float2 *p0, const float2 *p1, float2 *x, float2 *r,
+                        SolverState *S, BParts P, hipStream_t st)
+{
+    DISPATCH_ROWS(rows, kb_cg2v, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, p0, p1, x, r, S,
+                  P);
+}
+
+}  // namespace qed2d

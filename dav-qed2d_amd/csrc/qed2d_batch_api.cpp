@@ -1,0 +1,654 @@
+// qed2d_batch_api.cpp -- batched ensembles of independent lattices: the C ABI
+// (qed2d_batch_*) and its orchestration.  Each chain's trajectory is the
+// reference's HMC! for Nf=2 (src/QED2dHMC.jl:64-113) on that chain, drawn from
+// the same (seed, chain, trajectory) streams as the single-lattice entry
+// point qed2d_hmc_nf2, so a chain of a batch and a single-lattice run with the
+// same tiles produce the same bits.  What is batched: every CG iteration, Q
+// apply, force + momentum update and link update is one launch over all
+// chains; the per-chain reductions of the Hamiltonian and the Metropolis
+// decisions are per chain.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/qed2d.h"
+#include "qed2d_march.h"
+
+using namespace qed2d;
+
+namespace {
+
+struct GeoBuf {
+    int *blk0 = nullptr;
+    int4 *shape = nullptr;
+    int nblocks = 0;
+};
+
+}  // namespace
+
+struct qed2d_batch {
+    hipStream_t stream = nullptr;
+    int n = 0;
+    long vtot = 0;
+    int bc = 0;
+    std::vector<BChain> hch;
+    BChain *dch = nullptr;
+    std::vector<qed2d_ctx *> sub;  // one context per chain: its reductions
+    std::string err;
+    int rows = 16, rows32 = 8, rows_q = 4;
+    double ru_delta = 0.01;
+    std::map<int, GeoBuf> geo;  // key: 2 * rows + pair
+    // fields (arena)
+    double2 *U = nullptr, *Ucp = nullptr, *X = nullptr, *F = nullptr, *Y = nullptr;
+    double2 *r = nullptr, *p0 = nullptr, *p1 = nullptr, *t = nullptr;
+    float2 *Us = nullptr, *rs = nullptr, *p0s = nullptr, *p1s = nullptr, *ts = nullptr,
+           *xs = nullptr;
+    double *mom = nullptr;
+    // solver state and partials
+    SolverState *S = nullptr, *hS = nullptr;
+    int *mask = nullptr, *hmask = nullptr;
+    double *part = nullptr;
+    int *cnt = nullptr;
+    BParts P{};
+};
+
+namespace {
+
+int bfail(qed2d_batch *b, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    return code;
+}
+
+#define BTRY(x)                     \
+    do {                            \
+        int rc_ = (x);              \
+        if (rc_ != QED2D_OK) return rc_; \
+    } while (0)
+#define BHIP(b, x)                                                                          \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess) return bfail((b), QED2D_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+int blaunched(qed2d_batch *b)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bfail(b, QED2D_EHIP, "kernel launch: %s", hipGetErrorString(e));
+    return QED2D_OK;
+}
+
+// A sub-context's error, carried to the batch with the chain it came from.
+int subfail(qed2d_batch *b, int c, int rc)
+{
+    return bfail(b, rc, "chain %d: %s", c, qed2d_last_error(b->sub[c]));
+}
+
+Latt arena(const qed2d_batch *b)
+{
+    Latt L;
+    L.L1 = 1;
+    L.L2 = 1;
+    L.V = b->vtot;
+    L.bc = 0;
+    return L;
+}
+
+template <typename T> int balloc(qed2d_batch *b, T **p, long n)
+{
+    if (*p) return QED2D_OK;
+    void *q = nullptr;
+    if (hipMalloc(&q, sizeof(T) * n) != hipSuccess)
+        return bfail(b, QED2D_ENOMEM, "hipMalloc of %ld bytes failed", (long)(sizeof(T) * n));
+    *p = (T *)q;
+    return QED2D_OK;
+}
+
+// Launch geometry of one kernel family (rows per thread, pair or one-site
+// march) over the chains, built once and cached.
+int get_geo(qed2d_batch *b, int rows, bool pair, BGeo *out)
+{
+    const int key = 2 * rows + (pair ? 1 : 0);
+    auto it = b->geo.find(key);
+    if (it == b->geo.end()) {
+        std::vector<int> blk0(b->n + 1);
+        std::vector<int4> shape(b->n);
+        long tot = 0;
+        for (int c = 0; c < b->n; ++c) {
+            const MarchGrid g = pair ? march_grid2(b->hch[c].L, rows) : march_grid(b->hch[c].L, rows);
+            blk0[c] = (int)tot;
+            shape[c] = make_int4(g.bx, g.by, g.nbx, g.nblocks());
+            tot += g.nblocks();
+        }
+        if (tot > 0x7fffffffL) return bfail(b, QED2D_EINVAL, "batch too large for one launch");
+        blk0[b->n] = (int)tot;
+        GeoBuf gb;
+        BHIP(b, hipMalloc((void **)&gb.blk0, sizeof(int) * (b->n + 1)));
+        BHIP(b, hipMalloc((void **)&gb.shape, sizeof(int4) * b->n));
+        BHIP(b, hipMemcpy(gb.blk0, blk0.data(), sizeof(int) * (b->n + 1), hipMemcpyHostToDevice));
+        BHIP(b, hipMemcpy(gb.shape, shape.data(), sizeof(int4) * b->n, hipMemcpyHostToDevice));
+        gb.nblocks = (int)tot;
+        it = b->geo.emplace(key, gb).first;
+    }
+    out->ch = b->dch;
+    out->blk0 = it->second.blk0;
+    out->shape = it->second.shape;
+    out->n = b->n;
+    out->nblocks = it->second.nblocks;
+    return QED2D_OK;
+}
+
+int set_masses(qed2d_batch *b, const double *am0)
+{
+    for (int c = 0; c < b->n; ++c) b->hch[c].am0 = am0[c];
+    BHIP(b, hipMemcpyAsync(b->dch, b->hch.data(), sizeof(BChain) * b->n, hipMemcpyHostToDevice,
+                           b->stream));
+    return QED2D_OK;
+}
+
+int read_states(qed2d_batch *b)
+{
+    BHIP(b, hipMemcpyAsync(b->hS, b->S, sizeof(SolverState) * b->n, hipMemcpyDeviceToHost,
+                           b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return QED2D_OK;
+}
+
+struct Predict {  // the single-lattice solvers' chunk predictor, per chain
+    double err0 = 0.0;
+    int it0 = -1;
+    int next(double err, int iter, double tol, double ru_target)
+    {
+        int chunk = 8;
+        if (it0 >= 0 && iter > it0 && err > 0.0 && err0 > err) {
+            const double rate = std::log(err / err0) / (double)(iter - it0);
+            double target = tol;
+            if (ru_target > target) target = ru_target;
+            const double left = std::log(target / err) / rate;
+            if (left > 0.0 && left < 1e6) chunk = (int)std::ceil(left) + 1;
+            else chunk = 2;
+        }
+        it0 = iter;
+        err0 = err;
+        if (chunk < 2) chunk = 2;
+        if (chunk > 64) chunk = 64;
+        return chunk;
+    }
+};
+
+// (Q^2) x_c = src_c on every chain from x = 0, to |r|^2 < tol |b|^2 (the
+// single-lattice cg_f64 / cg_mixed, one launch per kernel over all chains).
+int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double tol,
+             int precision, std::vector<int> &iters)
+{
+    const long nel = 2 * b->vtot;
+    hipStream_t st = b->stream;
+    BTRY(balloc(b, &b->r, nel));
+    BTRY(balloc(b, &b->p0, nel));
+    BTRY(balloc(b, &b->p1, nel));
+    BTRY(balloc(b, &b->t, nel));
+    for (int c = 0; c < b->n; ++c) {
+        SolverState &s = b->hS[c];
+        std::memset(&s, 0, sizeof s);
+        s.eps = tol;
+        s.maxiter = maxiter;
+        s.delta2 = b->ru_delta * b->ru_delta;
+        s.kk = 1;
+        s.k2 = 1;
+    }
+    BHIP(b, hipMemcpyAsync(b->S, b->hS, sizeof(SolverState) * b->n, hipMemcpyHostToDevice, st));
+    BGeo g64, gq;
+    BTRY(get_geo(b, b->rows, false, &g64));
+    BTRY(get_geo(b, b->rows_q, false, &gq));
+    const bool mixed = (precision == QED2D_PREC_MIXED);
+    BGeo g32;
+    if (mixed) {
+        for (int c = 0; c < b->n; ++c)
+            if (!pair_ok(b->hch[c].L))
+                return bfail(b, QED2D_EINVAL, "mixed-precision batches need even L1 >= 4 "
+                             "(chain %d has L1 = %d)", c, b->hch[c].L.L1);
+        BTRY(balloc(b, &b->Us, nel));
+        BTRY(balloc(b, &b->rs, nel));
+        BTRY(balloc(b, &b->p0s, nel));
+        BTRY(balloc(b, &b->p1s, nel));
+        BTRY(balloc(b, &b->ts, nel));
+        BTRY(balloc(b, &b->xs, nel));
+        BTRY(get_geo(b, b->rows32, true, &g32));
+        launch_to_f32(b->U, b->Us, nel, st);
+        launch_b_cg_init(g64, b->rows, src, x, b->rs, b->p0s, b->xs, 1, b->P, st);
+    } else {
+        launch_b_cg_init(g64, b->rows, src, x, b->r, b->p0, nullptr, 0, b->P, st);
+    }
+    BTRY(blaunched(b));
+    std::vector<Predict> pred(b->n);
+    int chunk = 8;
+    long launched_k = 0;
+    for (;;) {
+        for (int j = 0; j < chunk; ++j) {
+            if (mixed) {
+                launch_b_cg_k1_f32(g32, b->rows32, b->Us, b->rs, b->p0s, b->p1s, b->ts, b->S,
+                                   b->P, st);
+                launch_b_cg_k2_f32(g32, b->rows32, b->Us, b->ts, b->p0s, b->p1s, b->xs, b->rs,
+                                   b->S, b->P, st);
+            } else {
+                launch_b_cg_k1_f64(g64, b->rows, b->U, b->r, b->p0, b->p1, b->t, b->S, b->P, st);
+                launch_b_cg_k2_f64(g64, b->rows, b->U, b->t, b->p0, b->p1, x, b->r, b->S, b->P,
+                                   st);
+            }
+        }
+        launched_k += chunk;
+        BTRY(blaunched(b));
+        BTRY(read_states(b));
+        bool running = false, ru = false;
+        int next = 64;
+        for (int c = 0; c < b->n; ++c) {
+            const SolverState &h = b->hS[c];
+            b->hmask[c] = h.need_ru ? 1 : 0;
+            if (h.status != 0) continue;
+            running = true;
+            if (h.need_ru) {
+                ru = true;
+                continue;
+            }
+            const double rut = mixed ? h.delta2 * h.rmax2[(h.iter + 1) & 1] : 0.0;
+            const int p = pred[c].next(h.err, h.iter, h.tol, rut);
+            if (p < next) next = p;
+        }
+        if (!running) break;
+        if (launched_k > 4L * (maxiter + 2) + 64)
+            return bfail(b, QED2D_EHIP, "batched CG: chains stopped advancing");
+        if (ru) {
+            // the chains that paused: fold the FP32 partial solution into x,
+            // recompute the true residual in FP64, resume at the same K1
+            BHIP(b, hipMemcpyAsync(b->mask, b->hmask, sizeof(int) * b->n, hipMemcpyHostToDevice,
+                                   st));
+            launch_b_ru_accum(b->dch, b->n, b->vtot, x, b->xs, b->mask, st);
+            launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, b->mask, st);
+            launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, nullptr, b->rs, nullptr, b->S,
+                              b->P, 2, b->mask, st);
+            BTRY(blaunched(b));
+            chunk = 2;
+            continue;
+        }
+        chunk = next;
+    }
+    iters.assign(b->n, 0);
+    int bad = -1;
+    for (int c = 0; c < b->n; ++c) {
+        iters[c] = b->hS[c].iter;
+        if (b->hS[c].status != 1 && bad < 0) bad = c;
+    }
+    if (bad >= 0)
+        return bfail(b, QED2D_ENOCONV, "CG on chain %d not converged after %d iterations "
+                     "(|r|^2 = %g, tol = %g)", bad, maxiter, b->hS[bad].err, b->hS[bad].tol);
+    return QED2D_OK;
+}
+
+int batch_update_momenta(qed2d_batch *b, double eps, int maxiter, double tol, int precision,
+                         std::vector<long> &iters)
+{
+    std::vector<int> it;
+    BTRY(batch_cg(b, b->X, b->F, maxiter, tol, precision, it));
+    for (int c = 0; c < b->n; ++c) iters[c] += it[c];
+    BGeo gq;
+    BTRY(get_geo(b, b->rows_q, false, &gq));
+    launch_b_q_apply(gq, b->rows_q, b->U, b->X, b->Y, nullptr, b->stream);
+    launch_b_force_mom(b->dch, b->n, b->vtot, b->U, b->X, b->Y, b->mom, eps, b->stream);
+    return blaunched(b);
+}
+
+void batch_updt(qed2d_batch *b, double eps)
+{
+    launch_updt_fld(arena(b), b->U, b->mom, eps, b->stream);
+}
+
+int batch_md(qed2d_batch *b, int integrator, double eps, int ns, int maxiter, double tol,
+             int precision, std::vector<long> &iters)
+{
+    if (integrator == QED2D_OMF4) {
+        const double r1 = OMF_R1, r2 = OMF_R2, r3 = OMF_R3, r4 = OMF_R4;
+        const double r5 = 0.5 - r1 - r3, r6 = 1.0 - 2.0 * (r2 + r4);
+        const double pc[6] = {r1, r3, r5, r5, r3, r1};
+        const double uc[5] = {r2, r4, r6, r4, r2};
+        for (int i = 0; i < ns; ++i)
+            for (int s = 0; s < 6; ++s) {
+                BTRY(batch_update_momenta(b, pc[s] * eps, maxiter, tol, precision, iters));
+                if (s < 5) batch_updt(b, eps * uc[s]);
+            }
+        return blaunched(b);
+    }
+    BTRY(batch_update_momenta(b, 0.5 * eps, maxiter, tol, precision, iters));
+    for (int i = 0; i < ns - 1; ++i) {
+        batch_updt(b, eps);
+        BTRY(batch_update_momenta(b, eps, maxiter, tol, precision, iters));
+    }
+    batch_updt(b, eps);
+    BTRY(batch_update_momenta(b, 0.5 * eps, maxiter, tol, precision, iters));
+    return blaunched(b);
+}
+
+void key_of(uint64_t seed, uint32_t chain, uint32_t *k0, uint32_t *k1)
+{
+    *k0 = (uint32_t)seed;
+    *k1 = (uint32_t)(seed >> 32) ^ chain;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qed2d_batch_create(qed2d_batch **out, int nchain, const int *L1, const int *L2,
+                       const double *beta, int bc_flags, void *stream)
+{
+    if (!out) return QED2D_EINVAL;
+    *out = nullptr;
+    if (nchain < 1 || !L1 || !L2 || !beta) return QED2D_EINVAL;
+    if (bc_flags & ~QED2D_BC_APERIODIC_T) return QED2D_EINVAL;
+    qed2d_batch *b = new qed2d_batch();
+    b->stream = (hipStream_t)stream;
+    b->n = nchain;
+    b->bc = bc_flags;
+    b->hch.resize(nchain);
+    long off = 0, pb = 0;
+    for (int c = 0; c < nchain; ++c) {
+        if (L1[c] < 2 || L2[c] < 2 || (long)L1[c] * L2[c] > (1L << 31)) {
+            delete b;
+            return QED2D_EINVAL;
+        }
+        BChain &ch = b->hch[c];
+        ch.L.L1 = L1[c];
+        ch.L.L2 = L2[c];
+        ch.L.V = (long)L1[c] * L2[c];
+        ch.L.bc = bc_flags;
+        ch.off = off;
+        ch.pb = (int)pb;
+        ch.pad = 0;
+        ch.am0 = 0.0;
+        ch.beta = beta[c];
+        off += ch.L.V;
+        pb += march_grid(ch.L, 1).nblocks();
+    }
+    b->vtot = off;
+    const long nel = 2 * off;
+    bool ok = hipMalloc((void **)&b->dch, sizeof(BChain) * nchain) == hipSuccess &&
+              hipMalloc((void **)&b->U, sizeof(double2) * nel) == hipSuccess &&
+              hipMalloc((void **)&b->S, sizeof(SolverState) * nchain) == hipSuccess &&
+              hipHostMalloc((void **)&b->hS, sizeof(SolverState) * nchain) == hipSuccess &&
+              hipMalloc((void **)&b->mask, sizeof(int) * nchain) == hipSuccess &&
+              hipHostMalloc((void **)&b->hmask, sizeof(int) * nchain) == hipSuccess &&
+              hipMalloc((void **)&b->part, sizeof(double) * 7 * pb) == hipSuccess &&
+              hipMalloc((void **)&b->cnt, sizeof(int) * 5 * nchain) == hipSuccess;
+    if (ok)
+        ok = hipMemcpy(b->dch, b->hch.data(), sizeof(BChain) * nchain, hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             hipMemset(b->cnt, 0, sizeof(int) * 5 * nchain) == hipSuccess;
+    if (!ok) {
+        qed2d_batch_destroy(b);
+        return QED2D_ENOMEM;
+    }
+    b->P.tt[0] = b->part;
+    b->P.tt[1] = b->part + 2 * pb;
+    b->P.rr[0] = b->part + 4 * pb;
+    b->P.rr[1] = b->part + 5 * pb;
+    b->P.b = b->part + 6 * pb;
+    b->P.ntt[0] = b->cnt;
+    b->P.ntt[1] = b->cnt + nchain;
+    b->P.nrr[0] = b->cnt + 2 * nchain;
+    b->P.nrr[1] = b->cnt + 3 * nchain;
+    b->P.nb = b->cnt + 4 * nchain;
+    b->sub.assign(nchain, nullptr);
+    for (int c = 0; c < nchain; ++c) {
+        int rc = qed2d_ctx_create(&b->sub[c], L1[c], L2[c], beta[c], bc_flags, stream);
+        if (rc != QED2D_OK) {
+            qed2d_batch_destroy(b);
+            return rc;
+        }
+        qed2d_ctx_set_sync(b->sub[c], 0);
+    }
+    *out = b;
+    return QED2D_OK;
+}
+
+void qed2d_batch_destroy(qed2d_batch *b)
+{
+    if (!b) return;
+    (void)hipStreamSynchronize(b->stream);
+    for (qed2d_ctx *c : b->sub)
+        if (c) qed2d_ctx_destroy(c);
+    for (auto &kv : b->geo) {
+        (void)hipFree(kv.second.blk0);
+        (void)hipFree(kv.second.shape);
+    }
+    void *dev[] = {b->dch, b->U, b->Ucp, b->X, b->F, b->Y, b->r, b->p0, b->p1, b->t, b->Us,
+                   b->rs, b->p0s, b->p1s, b->ts, b->xs, b->mom, b->S, b->mask, b->part, b->cnt};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    if (b->hS) (void)hipHostFree(b->hS);
+    if (b->hmask) (void)hipHostFree(b->hmask);
+    delete b;
+}
+
+const char *qed2d_batch_last_error(const qed2d_batch *b)
+{
+    return b ? b->err.c_str() : "null batch";
+}
+
+int qed2d_batch_size(const qed2d_batch *b, int *nchain, long *sites)
+{
+    if (!b) return QED2D_EINVAL;
+    if (nchain) *nchain = b->n;
+    if (sites) *sites = b->vtot;
+    return QED2D_OK;
+}
+
+int qed2d_batch_offset(const qed2d_batch *b, int chain, long *site_offset)
+{
+    if (!b || !site_offset || chain < 0 || chain >= b->n) return QED2D_EINVAL;
+    *site_offset = b->hch[chain].off;
+    return QED2D_OK;
+}
+
+int qed2d_batch_links(qed2d_batch *b, int chain, double **U)
+{
+    if (!b || !U) return QED2D_EINVAL;
+    if (chain < 0 || chain >= b->n) return bfail(b, QED2D_EINVAL, "no chain %d", chain);
+    *U = (double *)(b->U + 2 * b->hch[chain].off);
+    return QED2D_OK;
+}
+
+int qed2d_batch_get_links(qed2d_batch *b, int chain, double *dst)
+{
+    if (!b || !dst) return QED2D_EINVAL;
+    if (chain < 0 || chain >= b->n) return bfail(b, QED2D_EINVAL, "no chain %d", chain);
+    const BChain &ch = b->hch[chain];
+    BHIP(b, hipMemcpyAsync(dst, b->U + 2 * ch.off, sizeof(double2) * 2 * ch.L.V, hipMemcpyDefault,
+                           b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return QED2D_OK;
+}
+
+int qed2d_batch_set_links(qed2d_batch *b, int chain, const double *src)
+{
+    if (!b || !src) return QED2D_EINVAL;
+    if (chain < 0 || chain >= b->n) return bfail(b, QED2D_EINVAL, "no chain %d", chain);
+    const BChain &ch = b->hch[chain];
+    BHIP(b, hipMemcpyAsync(b->U + 2 * ch.off, src, sizeof(double2) * 2 * ch.L.V, hipMemcpyDefault,
+                           b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return QED2D_OK;
+}
+
+int qed2d_batch_set_tiles(qed2d_batch *b, int q64, int cg64, int cg32)
+{
+    if (!b) return QED2D_EINVAL;
+    auto ok = [](int r) { return r == 0 || r == 1 || r == 2 || r == 4 || r == 8 || r == 16; };
+    if (!ok(q64) || !ok(cg64) || !ok(cg32))
+        return bfail(b, QED2D_EINVAL, "tile rows must be 0, 1, 2, 4, 8 or 16");
+    if (q64) b->rows_q = q64;
+    if (cg64) b->rows = cg64;
+    if (cg32) b->rows32 = cg32;
+    return QED2D_OK;
+}
+
+int qed2d_batch_hot_start(qed2d_batch *b, const uint64_t *seed)
+{
+    if (!b || !seed) return QED2D_EINVAL;
+    for (int c = 0; c < b->n; ++c) {
+        uint32_t k0, k1;
+        key_of(seed[c], (uint32_t)c, &k0, &k1);
+        launch_hot_start(b->hch[c].L, b->U + 2 * b->hch[c].off, k0, k1, b->stream);
+    }
+    BTRY(blaunched(b));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return QED2D_OK;
+}
+
+int qed2d_batch_cg(qed2d_batch *b, double *x, const double *src, const double *am0,
+                   int maxiter, double tol, int precision, int *iters)
+{
+    if (!b) return QED2D_EINVAL;
+    if (!x || !src || !am0) return bfail(b, QED2D_EINVAL, "null pointer");
+    if (x == src) return bfail(b, QED2D_EINVAL, "x aliases src");
+    if (precision != QED2D_PREC_F64 && precision != QED2D_PREC_MIXED)
+        return bfail(b, QED2D_EINVAL, "unknown precision %d", precision);
+    BTRY(set_masses(b, am0));
+    std::vector<int> it;
+    const int rc = batch_cg(b, (double2 *)x, (const double2 *)src, maxiter, tol, precision, it);
+    if (iters)
+        for (int c = 0; c < b->n && c < (int)it.size(); ++c) iters[c] = it[c];
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return rc;
+}
+
+int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, int integrator,
+                        int maxiter, double tol, int precision, double acc_tol,
+                        const uint64_t *seed, uint32_t traj, double *hini, double *hfin,
+                        int *accepted, int *cg_iters)
+{
+    if (!b) return QED2D_EINVAL;
+    if (!am0 || !seed || !hini || !hfin || !accepted)
+        return bfail(b, QED2D_EINVAL, "null pointer");
+    if (precision != QED2D_PREC_F64 && precision != QED2D_PREC_MIXED)
+        return bfail(b, QED2D_EINVAL, "unknown precision %d", precision);
+    const long nel = 2 * b->vtot;
+    hipStream_t st = b->stream;
+    BTRY(balloc(b, &b->Ucp, nel));
+    BTRY(balloc(b, &b->X, nel));
+    BTRY(balloc(b, &b->F, nel));
+    BTRY(balloc(b, &b->Y, nel));
+    BTRY(balloc(b, &b->mom, nel));
+    BTRY(set_masses(b, am0));
+    launch_copy(b->Ucp, b->U, nel, st);
+    std::vector<double> h0h(b->n), h0l(b->n);
+    for (int c = 0; c < b->n; ++c) {
+        const BChain &ch = b->hch[c];
+        const long o = 2 * ch.off;
+        uint32_t k0, k1;
+        key_of(seed[c], (uint32_t)c, &k0, &k1);
+        launch_fill_momenta(ch.L, b->mom + o, k0, k1, traj, st);
+        launch_fill_pseudofermion(ch.L, b->X + o, k0, k1, traj, 0, st);
+    }
+    BTRY(blaunched(b));
+    // hini = H + X^dag X  (src/QED2dHMC.jl:74), per chain
+    for (int c = 0; c < b->n; ++c) {
+        const long o = 2 * b->hch[c].off;
+        int rc = ctx_hamiltonian_dd(b->sub[c], b->mom + o, (const double *)(b->U + o), &h0h[c],
+                                    &h0l[c]);
+        if (rc) return subfail(b, c, rc);
+        double xx;
+        rc = qed2d_norm2(b->sub[c], (const double *)(b->X + o), &xx);
+        if (rc) return subfail(b, c, rc);
+        dd h{h0h[c], h0l[c]};
+        h = dd_add(h, xx);
+        h0h[c] = h.hi;
+        h0l[c] = h.lo;
+    }
+    BGeo gq;
+    BTRY(get_geo(b, b->rows_q, false, &gq));
+    launch_b_q_apply(gq, b->rows_q, b->U, b->X, b->F, nullptr, st);  // F = Q X  (:76-78)
+    std::vector<long> its(b->n, 0);
+    int rc = batch_md(b, integrator, eps, ns, maxiter, tol, precision, its);
+    if (cg_iters)
+        for (int c = 0; c < b->n; ++c) cg_iters[c] = (int)its[c];
+    if (rc != QED2D_OK) {
+        std::string msg = b->err;
+        launch_copy(b->U, b->Ucp, nel, st);
+        (void)hipStreamSynchronize(st);
+        b->err = msg;
+        for (int c = 0; c < b->n; ++c) accepted[c] = 0;
+        return rc;
+    }
+    if (acc_tol > 0.0) {
+        // FP64 check of the last MD solve: re-solve from it where its true
+        // residual is above acc_tol (single-lattice acc_tol path)
+        for (int c = 0; c < b->n; ++c) {
+            const long o = 2 * b->hch[c].off;
+            int it2 = 0;
+            rc = qed2d_cg(b->sub[c], (double *)(b->X + o), (const double *)(b->U + o),
+                          (const double *)(b->F + o), am0[c], 0.0, QED2D_OP_SQR_MSQ, maxiter,
+                          acc_tol, 1, QED2D_PREC_F64, &it2);
+            if (rc) return subfail(b, c, rc);
+        }
+        launch_b_q_apply(gq, b->rows_q, b->U, b->X, b->Y, nullptr, st);
+    }
+    for (int c = 0; c < b->n; ++c) {
+        const BChain &ch = b->hch[c];
+        const long o = 2 * ch.off;
+        double hh, hl;
+        rc = ctx_hamiltonian_dd(b->sub[c], b->mom + o, (const double *)(b->U + o), &hh, &hl);
+        if (rc) return subfail(b, c, rc);
+        dd h1{hh, hl};
+        double re, im;
+        rc = qed2d_dot(b->sub[c], (const double *)(b->X + o), (const double *)(b->F + o), &re,
+                       &im);
+        if (rc) return subfail(b, c, rc);
+        if (acc_tol <= 0.0) {
+            h1 = dd_add(h1, re);  // hfin = H + Re X^dag F  (:82)
+        } else {
+            double yy;
+            rc = qed2d_norm2(b->sub[c], (const double *)(b->Y + o), &yy);
+            if (rc) return subfail(b, c, rc);
+            h1 = dd_add(h1, 2.0 * re);
+            h1 = dd_add(h1, -yy);
+        }
+        hini[c] = h0h[c] + h0l[c];
+        hfin[c] = h1.hi + h1.lo;
+        // Metropolis (src/QED2dHMC.jl:95-102)
+        const double pacc = std::exp(-(hfin[c] - hini[c]));
+        if (pacc < 1.0)
+            accepted[c] = (pacc > qed2d_metropolis_uniform(seed[c], (uint32_t)c, traj)) ? 1 : 0;
+        else
+            accepted[c] = 1;
+        if (!accepted[c])
+            launch_copy(b->U + o, b->Ucp + o, 2 * ch.L.V, st);
+    }
+    BTRY(blaunched(b));
+    BHIP(b, hipStreamSynchronize(st));
+    return QED2D_OK;
+}
+
+int qed2d_batch_observables(qed2d_batch *b, double *plaquette, double *qtop, double *action)
+{
+    if (!b) return QED2D_EINVAL;
+    for (int c = 0; c < b->n; ++c) {
+        const double *Uc = (const double *)(b->U + 2 * b->hch[c].off);
+        int rc = QED2D_OK;
+        if (plaquette && (rc = qed2d_plaquette(b->sub[c], Uc, &plaquette[c]))) return subfail(b, c, rc);
+        if (qtop && (rc = qed2d_qtop(b->sub[c], Uc, &qtop[c]))) return subfail(b, c, rc);
+        if (action && (rc = qed2d_action(b->sub[c], Uc, &action[c]))) return subfail(b, c, rc);
+    }
+    return QED2D_OK;
+}
+
+}  // extern "C"

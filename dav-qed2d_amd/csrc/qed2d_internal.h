@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/qed2d.h"
 #include "qed2d_device.h"
 
 namespace qed2d {
@@ -29,6 +30,7 @@ struct SolverState {
     int guess;         // the solve started from a caller's guess
     int true_at;       // K1(true_at) sees a true (FP64) residual norm
     int stop_k;        // nonzero: K1(stop_k) ended or paused the solve; later kernels return
+    int kk, k2;        // batched solves: the iteration K1 runs next / K2 runs (per chain)
 };
 
 // Multi-shift CG state (all shifts sigma_i = mu_i^2 of (Q^2 + sigma_i) x_i = b on
@@ -75,6 +77,68 @@ struct MarchGrid {
     int nblocks() const { return nbx * nby; }
 };
 MarchGrid march_grid(const Latt &L, int rows);
+
+// Coefficients of the fourth-order Omelyan-Mryglod-Folk integrator
+// (src/QED2dHMC.jl:241-246; Omelyan et al., CPC 151 (2003) 272).
+constexpr double OMF_R1 = 0.08398315262876693;
+constexpr double OMF_R2 = 0.2539785108410595;
+constexpr double OMF_R3 = 0.6822365335719091;
+constexpr double OMF_R4 = -0.03230286765269967;
+
+// H = sum p^2 / 2 + S_g of a context's lattice as a double-double (hi, lo).
+int ctx_hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, double *hi, double *lo);
+
+// ---- batched ensembles ----------------------------------------------------
+// A chain set: independent lattices whose fields are concatenated in one
+// arena.  Chain c's spinor, link and momentum fields are reference-layout
+// fields starting at element 2 * off (its two planes of V sites); its solver
+// state is S[c]; its partial sums occupy slots pb .. pb + (its blocks) - 1.
+struct BChain {
+    Latt L;
+    long off;
+    int pb;
+    int pad;
+    double am0, beta;
+};
+// Launch geometry of one kernel family over a chain set: chain c owns blocks
+// blk0[c] .. blk0[c+1]-1 of the launch, tiled by shape[c] = (bx, by, nbx, nb).
+struct BGeo {
+    const BChain *ch;
+    const int *blk0;
+    const int4 *shape;
+    int n;        // chains
+    int nblocks;  // blocks of the launch (blk0[n])
+};
+// Per-chain partial sums of the batched CG: values at pb + local block, and
+// the producer's block count per chain (written by its local block 0).
+struct BParts {
+    double *tt[2], *rr[2], *b;
+    int *ntt[2], *nrr[2], *nb;
+};
+// mask (may be null): per chain, nonzero = apply on that chain.
+void launch_b_q_apply(const BGeo &g, int rows, const double2 *U, const double2 *si,
+                      double2 *so, const int *mask, hipStream_t st);
+void launch_b_force_mom(const BChain *ch, int nchain, long vtot, const double2 *U,
+                        const double2 *X, const double2 *Y, double *mom, double eps,
+                        hipStream_t st);
+void launch_b_cg_init(const BGeo &g, int rows, const double2 *b, double2 *x, void *r, void *p0,
+                      float2 *xs, int mixed, BParts P, hipStream_t st);
+void launch_b_cg_resid(const BGeo &g, int rows, const double2 *U, const double2 *b,
+                       const double2 *t, const double2 *x, double2 *r, float2 *rs, double2 *p0,
+                       SolverState *S, BParts P, int mode, const int *mask, hipStream_t st);
+void launch_b_cg_k1_f64(const BGeo &g, int rows, const double2 *U, const double2 *r,
+                        double2 *p0, double2 *p1, double2 *t, SolverState *S, BParts P,
+                        hipStream_t st);
+void launch_b_cg_k2_f64(const BGeo &g, int rows, const double2 *U, const double2 *t,
+                        const double2 *p0, const double2 *p1, double2 *x, double2 *r,
+                        SolverState *S, BParts P, hipStream_t st);
+void launch_b_cg_k1_f32(const BGeo &g, int rows, const float2 *U, const float2 *r, float2 *p0,
+                        float2 *p1, float2 *t, SolverState *S, BParts P, hipStream_t st);
+void launch_b_cg_k2_f32(const BGeo &g, int rows, const float2 *U, const float2 *t,
+                        const float2 *p0, const float2 *p1, float2 *x, float2 *r,
+                        SolverState *S, BParts P, hipStream_t st);
+void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,
+                       const int *mask, hipStream_t st);
 
 // ---- launches (all asynchronous on `st`) ---------------------------------
 void launch_q_apply_f64(const Latt &L, const double2 *U, const double2 *si, double2 *so,

@@ -8,7 +8,7 @@
 // arrive so that no kernel boundary needs a host round trip.
 #include <type_traits>
 
-#include "qed2d_march.h"
+#include "qed2d_fields.h"
 
 namespace qed2d {
 
@@ -34,19 +34,6 @@ MarchGrid march_grid(const Latt &L, int rows)
 // ===========================================================================
 // Q apply: so = gamma5 Dw si  (src/QED2dDirac.jl:2-33)
 // ===========================================================================
-// The apply's output is not read back by this kernel, so its stores stream
-// past the caches (nontemporal).
-template <typename T> struct EpiStore {
-    c2<T> *__restrict__ so;
-    long V;
-    __device__ __forceinline__ NoPre pre(long) const { return {}; }
-    __device__ __forceinline__ void operator()(long s, NoPre, c2<T> o1, c2<T> o2, c2<T>, c2<T>)
-    {
-        st<T, true>(so, s, o1);
-        st<T, true>(so, V + s, o2);
-    }
-};
-
 template <typename T, int R>
 __global__ void __launch_bounds__(kBlock) k_q_apply(Latt L, const c2<T> *__restrict__ U,
                                                     const c2<T> *__restrict__ si,
@@ -92,18 +79,6 @@ __global__ void __launch_bounds__(kBlock) k_q_shift(Latt L, const double2 *__res
     q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
 }
 
-
-struct EpiStore2 {
-    float2 *__restrict__ so;
-    long V;
-    __device__ __forceinline__ NoPre pre(long) const { return {}; }
-    __device__ __forceinline__ void pair(long s, NoPre, float2 oa1, float2 oa2, float2 ob1,
-                                         float2 ob2, F2x2, F2x2)
-    {
-        st2<true>(so, s, oa1, ob1);
-        st2<true>(so, V + s, oa2, ob2);
-    }
-};
 
 // FP32 apply, two sites per thread along i1.
 template <int R>
@@ -452,20 +427,6 @@ __device__ __forceinline__ dd block_sum_dd(dd v, double *lds)
     return s;
 }
 
-// Plaquette P(x) = U1(x) U2(x+1) conj(U1(x+2) U2(x))  (src/QED2dAction.jl:10-13)
-__device__ __forceinline__ double2 plaq_at(const double2 *__restrict__ U, const Latt &L, int i1,
-                                           int i2)
-{
-    const int iu1 = (i1 + 1 == L.L1) ? 0 : i1 + 1;
-    const int iu2 = (i2 + 1 == L.L2) ? 0 : i2 + 1;
-    const long V = L.V;
-    double2 a = cmul<double>(U[(long)i2 * L.L1 + i1], U[V + (long)i2 * L.L1 + iu1]);
-    double2 b = cmul<double>(U[(long)iu2 * L.L1 + i1], U[V + (long)i2 * L.L1 + i1]);
-    return cmul<double>(a, cconj<double>(b));
-}
-
-
-
 // Sum Re P, sum arg P (src/QED2dAction.jl:2-32) and sum p^2 (src/QED2dHMC.jl:200),
 // each as a double-double, per block; optional per-site density outputs.
 __global__ void __launch_bounds__(kBlock) k_observables(Latt L, const double2 *__restrict__ U,
@@ -530,41 +491,6 @@ void launch_observables(const Latt &L, const double2 *U, const double *mom, doub
 // Forces and link updates
 // ===========================================================================
 
-// Fermion force z_mu at site x, returns Re(i z1), Re(i z2)  (src/QED2dDirac.jl:172-199)
-__device__ __forceinline__ void ferm_force(const double2 *__restrict__ U,
-                                           const double2 *__restrict__ X,
-                                           const double2 *__restrict__ Y, const Latt &L, int i1,
-                                           int i2, double &f1, double &f2)
-{
-    const long V = L.V;
-    const int iu1 = (i1 + 1 == L.L1) ? 0 : i1 + 1;
-    const int iu2 = (i2 + 1 == L.L2) ? 0 : i2 + 1;
-    const long s = (long)i2 * L.L1 + i1;
-    const long s1 = (long)i2 * L.L1 + iu1;
-    const long s2 = (long)iu2 * L.L1 + i1;
-    double2 U1 = U[s], U2 = U[V + s];
-    if ((L.bc & 1) && i2 == L.L2 - 1) U2 = cneg<double>(U2);
-    const double2 X1 = X[s], X2 = X[V + s], Y1 = Y[s], Y2 = Y[V + s];
-    const double2 Xa1 = X[s1], Xa2 = X[V + s1], Ya1 = Y[s1], Ya2 = Y[V + s1];
-    const double2 Xb1 = X[s2], Xb2 = X[V + s2], Yb1 = Y[s2], Yb2 = Y[V + s2];
-    // z1 = U1* (X1(x+1)* (Y1+Y2) - X2(x+1)* (Y1+Y2)) - U1 (X1* + X2*)(Y1-Y2)(x+1)
-    const double2 ys = cadd<double>(Y1, Y2);
-    const double2 yd = csub<double>(Ya1, Ya2);
-    double2 z1 = csub<double>(
-        cmulc<double>(U1, csub<double>(cmulc<double>(Xa1, ys), cmulc<double>(Xa2, ys))),
-        cmul<double>(U1, cadd<double>(cmulc<double>(X1, yd), cmulc<double>(X2, yd))));
-    // z2 = U2* (X1(x+2)* (Y1 - i Y2) - X2(x+2)* (i Y1 + Y2))
-    //      - U2 (X1* (Y1 + i Y2)(x+2) + X2* (i Y1 - Y2)(x+2))
-    double2 z2 = csub<double>(
-        cmulc<double>(U2, csub<double>(cmulc<double>(Xb1, csub<double>(Y1, cmuli<double>(Y2))),
-                                       cmulc<double>(Xb2, cadd<double>(cmuli<double>(Y1), Y2)))),
-        cmul<double>(U2, cadd<double>(cmulc<double>(X1, cadd<double>(Yb1, cmuli<double>(Yb2))),
-                                      cmulc<double>(X2, csub<double>(cmuli<double>(Yb1), Yb2)))));
-    // Re(i z) = -Im z
-    f1 = -z1.y;
-    f2 = -z2.y;
-}
-
 __global__ void k_tr_dQwdU(Latt L, const double2 *__restrict__ U, const double2 *__restrict__ X,
                            const double2 *__restrict__ Y, double *__restrict__ frc)
 {
@@ -606,37 +532,15 @@ void launch_krnl_force(const Latt &L, const double2 *U, double beta, double *frc
                        frc2);
 }
 
-// Gather form of the gauge force + fermion force + momentum update, one pass:
-//   F(x,1) = -v(x) + v(x-2),  F(x,2) = v(x) - v(x-1),  v = beta Im P
-// (the sums that src/QED2dAction.jl:47-50 scatters into frc1/frc2), then
-//   mom += eps ((F_gauge) + cf F_fermion + facc)     (src/QED2dHMC.jl:478,554)
+// Gather form of the gauge force + fermion force + momentum update, one pass
+// (force_mom_site).
 __global__ void k_force_mom(Latt L, const double2 *__restrict__ U, const double2 *__restrict__ X,
                             const double2 *__restrict__ Y, const double *__restrict__ facc,
                             double *__restrict__ mom, double beta, double eps, double cf)
 {
     for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < L.V;
-         s += (long)gridDim.x * blockDim.x) {
-        const int i2 = (int)(s / L.L1), i1 = (int)(s - (long)i2 * L.L1);
-        const int id1 = (i1 == 0) ? L.L1 - 1 : i1 - 1;
-        const int id2 = (i2 == 0) ? L.L2 - 1 : i2 - 1;
-        const double v0 = beta * plaq_at(U, L, i1, i2).y;
-        const double vd2 = beta * plaq_at(U, L, i1, id2).y;
-        const double vd1 = beta * plaq_at(U, L, id1, i2).y;
-        double g1 = -v0 + vd2;
-        double g2 = v0 - vd1;
-        if (X) {
-            double f1, f2;
-            ferm_force(U, X, Y, L, i1, i2, f1, f2);
-            g1 = g1 + cf * f1;
-            g2 = g2 + cf * f2;
-        }
-        if (facc) {
-            g1 = g1 + facc[s];
-            g2 = g2 + facc[L.V + s];
-        }
-        mom[s] = mom[s] + eps * g1;
-        mom[L.V + s] = mom[L.V + s] + eps * g2;
-    }
+         s += (long)gridDim.x * blockDim.x)
+        force_mom_site(L, U, X, Y, facc, mom, beta, eps, cf, s);
 }
 void launch_force_mom(const Latt &L, const double2 *U, const double2 *X, const double2 *Y,
                       const double *facc, double *mom, double beta, double eps, double cf,

@@ -354,6 +354,31 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
     }
 }
 
+// The apply's output is not read back by this kernel, so its stores stream
+// past the caches (nontemporal).
+template <typename T> struct EpiStore {
+    c2<T> *__restrict__ so;
+    long V;
+    __device__ __forceinline__ NoPre pre(long) const { return {}; }
+    __device__ __forceinline__ void operator()(long s, NoPre, c2<T> o1, c2<T> o2, c2<T>, c2<T>)
+    {
+        st<T, true>(so, s, o1);
+        st<T, true>(so, V + s, o2);
+    }
+};
+
+struct EpiStore2 {
+    float2 *__restrict__ so;
+    long V;
+    __device__ __forceinline__ NoPre pre(long) const { return {}; }
+    __device__ __forceinline__ void pair(long s, NoPre, float2 oa1, float2 oa2, float2 ob1,
+                                         float2 ob2, F2x2, F2x2)
+    {
+        st2<true>(so, s, oa1, ob1);
+        st2<true>(so, V + s, oa2, ob2);
+    }
+};
+
 // Launch shape of the pair kernels (width L1/2).
 inline MarchGrid march_grid2(const Latt &L, int rows)
 {

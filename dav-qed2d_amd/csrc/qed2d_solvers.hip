@@ -19,7 +19,7 @@
 // block reads are never overwritten during the kernel.
 #include <type_traits>
 
-#include "qed2d_march.h"
+#include "qed2d_cg.h"
 
 namespace qed2d {
 
@@ -29,11 +29,6 @@ static inline int ew_grid(long n)
     if (b > 8192) b = 8192;
     if (b < 1) b = 1;
     return (int)b;
-}
-
-template <typename T> __device__ __forceinline__ c2<T> from_d2(double2 v)
-{
-    return mk<T>((T)v.x, (T)v.y);
 }
 
 // ===========================================================================
@@ -125,38 +120,6 @@ void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *r
 // Residual of an FP64 x: r = b - (Q t + mu^2 x), t = Q x  (the guess branch
 // of src/QED2dSolver.jl:65-72, and the reliable update of the mixed solver)
 // ===========================================================================
-struct EpiResid {
-    const double2 *__restrict__ b;
-    const double2 *__restrict__ x;
-    double2 *__restrict__ r;
-    float2 *__restrict__ rs;
-    double2 *__restrict__ p0;
-    float2 *__restrict__ p0s;
-    double mu2;
-    long V;
-    double rr, bb;
-    struct Pre {
-        double2 b1, b2, x1, x2;
-    };
-    __device__ __forceinline__ Pre pre(long s) const { return Pre{b[s], b[V + s], x[s], x[V + s]}; }
-    __device__ __forceinline__ void one(long i, double2 o, double2 bv, double2 xv)
-    {
-        const double2 rv = mk<double>(bv.x - (o.x + mu2 * xv.x), bv.y - (o.y + mu2 * xv.y));
-        if (r) r[i] = rv;
-        if (rs) rs[i] = mk<float>((float)rv.x, (float)rv.y);
-        if (p0) p0[i] = mk<double>(0.0, 0.0);
-        if (p0s) p0s[i] = mk<float>(0.0f, 0.0f);
-        rr += rv.x * rv.x + rv.y * rv.y;
-        bb += bv.x * bv.x + bv.y * bv.y;
-    }
-    __device__ __forceinline__ void operator()(long s, const Pre &q, double2 o1, double2 o2,
-                                               double2, double2)
-    {
-        one(s, o1, q.b1, q.x1);
-        one(V + s, o2, q.b2, q.x2);
-    }
-};
-
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_cg_resid(Latt L, const double2 *__restrict__ U,
                                                      const double2 *__restrict__ t,
@@ -235,82 +198,6 @@ void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st)
 // ===========================================================================
 // CG iteration kernels
 // ===========================================================================
-// Convergence test and scalars at the start of K1(k), identical in every
-// block: sums the |r_{k-1}|^2 partials, decides whether the solve ends
-// (converged / iteration limit) or, for the FP32 iterations of the mixed
-// solver, pauses for a reliable update; block 0 commits the scalars of slot
-// k & 1.  Returns false when the kernel has no work; beta receives
-// |r_{k-1}|^2 / |r_{k-2}|^2 (src/QED2dSolver.jl:33-35).
-template <bool mixed>
-__device__ __forceinline__ bool cg_k1_scalars(SolverState *S, const double *Prr, const double *Pb,
-                                              int nrr, int nbb, int k, int blk, double *lds,
-                                              double &beta)
-{
-    const int w = k & 1, rd = (k - 1) & 1;
-    const double rr = sum_prev_partials(Prr, nrr, 1, 0, lds);
-    const bool first = (k == 1);
-    const double bn = first ? sum_prev_partials(Pb, nbb, 1, 0, lds) : S->bnorm;
-    const double tol = first ? S->eps * bn : S->tol;
-    const double prev = first ? 0.0 : S->norm[rd];
-    double rmax = first ? rr : S->rmax2[rd];
-    const bool truer = first || S->true_at == k;
-    int stat = 0, ru = 0;
-    if (first && bn == 0.0 && !S->guess) {
-        stat = 1;  // b = 0: x = 0 exactly
-    } else if (!mixed) {
-        if (rr < tol) stat = 1;                         // src/QED2dSolver.jl:29
-        else if (k - 1 >= S->maxiter) stat = 2;
-    } else if (truer) {
-        if (rr < tol) stat = 1;
-        else {
-            rmax = rr;
-            if (k - 1 >= S->maxiter) stat = 2;
-        }
-    } else {
-        if (rr < tol || rr < S->delta2 * rmax || k - 1 >= S->maxiter) ru = 1;
-        else if (rr > rmax) rmax = rr;
-    }
-    if (blk == 0 && threadIdx.x == 0) {
-        S->err = rr;
-        S->iter = k - 1;
-        S->norm[w] = rr;
-        S->rmax2[w] = rmax;
-        if (first) {
-            S->bnorm = bn;
-            S->tol = tol;
-        }
-        if (stat) {
-            S->status = stat;
-            S->stop_k = k;
-        }
-        if (ru) {
-            S->need_ru = 1;
-            S->stop_k = k;
-        }
-    }
-    if (stat || ru) return false;
-    beta = first ? 0.0 : rr / prev;
-    return true;
-}
-
-template <typename T> struct EpiK1 {
-    c2<T> *__restrict__ pnew;
-    c2<T> *__restrict__ t;
-    long V;
-    double tt, pp;
-    __device__ __forceinline__ NoPre pre(long) const { return {}; }
-    __device__ __forceinline__ void operator()(long s, NoPre, c2<T> o1, c2<T> o2, c2<T> c1,
-                                               c2<T> cc2)
-    {
-        pnew[s] = c1;
-        pnew[V + s] = cc2;
-        t[s] = o1;
-        t[V + s] = o2;
-        tt += (double)(o1.x * o1.x + o1.y * o1.y) + (double)(o2.x * o2.x + o2.y * o2.y);
-        pp += (double)(c1.x * c1.x + c1.y * c1.y) + (double)(cc2.x * cc2.x + cc2.y * cc2.y);
-    }
-};
-
 // K1(k).  FP64 (T = double): the reference's test |r|^2 < eps |b|^2.  FP32
 // (T = float, the iterations of the mixed solver): the iterated residual asks
 // for a reliable update once it has dropped by delta since the last one, looks
@@ -347,37 +234,6 @@ __global__ void __launch_bounds__(kBlock) k_cg1(Latt L, const c2<T> *__restrict_
     }
 }
 
-template <typename T> struct EpiK2 {
-    const c2<T> *__restrict__ p;
-    c2<T> *__restrict__ x;
-    c2<T> *__restrict__ r;
-    T alpha, mu2;
-    long V;
-    double rr;
-    struct Pre {
-        c2<T> p1, p2, x1, x2, r1, r2;
-    };
-    __device__ __forceinline__ Pre pre(long s) const
-    {
-        return Pre{p[s], p[V + s], x[s], x[V + s], r[s], r[V + s]};
-    }
-    __device__ __forceinline__ void one(long i, c2<T> o, c2<T> pv, c2<T> xv, c2<T> rv)
-    {
-        const c2<T> a = mk<T>(o.x + mu2 * pv.x, o.y + mu2 * pv.y);
-        xv = mk<T>(xv.x + alpha * pv.x, xv.y + alpha * pv.y);   // src/QED2dSolver.jl:24
-        rv = mk<T>(rv.x - alpha * a.x, rv.y - alpha * a.y);     // src/QED2dSolver.jl:25
-        x[i] = xv;
-        r[i] = rv;
-        rr += (double)(rv.x * rv.x + rv.y * rv.y);
-    }
-    __device__ __forceinline__ void operator()(long s, const Pre &q, c2<T> o1, c2<T> o2, c2<T>,
-                                               c2<T>)
-    {
-        one(s, o1, q.p1, q.x1, q.r1);
-        one(V + s, o2, q.p2, q.x2, q.r2);
-    }
-};
-
 template <typename T, int R>
 __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict__ U,
                                                 const c2<T> *__restrict__ t, const c2<T> *P0,
@@ -405,26 +261,6 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
 }
 
 // ---- FP32 pair kernels (two sites per thread along i1) -------------------
-struct EpiK1v {
-    float2 *__restrict__ pnew;
-    float2 *__restrict__ t;
-    long V;
-    double tt, pp;
-    __device__ __forceinline__ NoPre pre(long) const { return {}; }
-    __device__ __forceinline__ void pair(long s, NoPre, float2 oa1, float2 oa2, float2 ob1,
-                                         float2 ob2, F2x2 c1, F2x2 c2)
-    {
-        st2(pnew, s, c1.a0, c1.a1);
-        st2(pnew, V + s, c2.a0, c2.a1);
-        st2(t, s, oa1, ob1);
-        st2(t, V + s, oa2, ob2);
-        tt += (double)(cabs2<float>(oa1) + cabs2<float>(ob1)) +
-              (double)(cabs2<float>(oa2) + cabs2<float>(ob2));
-        pp += (double)(cabs2<float>(c1.a0) + cabs2<float>(c1.a1)) +
-              (double)(cabs2<float>(c2.a0) + cabs2<float>(c2.a1));
-    }
-};
-
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, const float2 *__restrict__ U,
                                                  const float2 *__restrict__ r, float2 *P0,
@@ -452,40 +288,6 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, const float2 *__restric
         Ptt[2 * p.blk + 1] = b;
     }
 }
-
-struct EpiK2v {
-    const float2 *__restrict__ p;
-    float2 *__restrict__ x;
-    float2 *__restrict__ r;
-    float alpha, mu2;
-    long V;
-    double rr;
-    struct Pre {
-        F2x2 p1, p2, x1, x2, r1, r2;
-    };
-    __device__ __forceinline__ Pre pre(long s) const
-    {
-        return Pre{ld2(p, s), ld2(p, V + s), ld2(x, s), ld2(x, V + s), ld2(r, s), ld2(r, V + s)};
-    }
-    __device__ __forceinline__ void comp(long i, float2 oa, float2 ob, F2x2 pv, F2x2 xv, F2x2 rv)
-    {
-        const float2 aa = make_float2(oa.x + mu2 * pv.a0.x, oa.y + mu2 * pv.a0.y);
-        const float2 ab = make_float2(ob.x + mu2 * pv.a1.x, ob.y + mu2 * pv.a1.y);
-        xv.a0 = make_float2(xv.a0.x + alpha * pv.a0.x, xv.a0.y + alpha * pv.a0.y);
-        xv.a1 = make_float2(xv.a1.x + alpha * pv.a1.x, xv.a1.y + alpha * pv.a1.y);
-        rv.a0 = make_float2(rv.a0.x - alpha * aa.x, rv.a0.y - alpha * aa.y);
-        rv.a1 = make_float2(rv.a1.x - alpha * ab.x, rv.a1.y - alpha * ab.y);
-        st2(x, i, xv.a0, xv.a1);
-        st2(r, i, rv.a0, rv.a1);
-        rr += (double)(cabs2<float>(rv.a0) + cabs2<float>(rv.a1));
-    }
-    __device__ __forceinline__ void pair(long s, const Pre &q, float2 oa1, float2 oa2, float2 ob1,
-                                         float2 ob2, F2x2, F2x2)
-    {
-        comp(s, oa1, ob1, q.p1, q.x1, q.r1);
-        comp(V + s, oa2, ob2, q.p2, q.x2, q.r2);
-    }
-};
 
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_cg2v(Latt L, const float2 *__restrict__ U,

@@ -56,6 +56,19 @@ SIGNATURES = {
     "qed2d_ctx_set_tile_f32": (_i, [_p, _i]),
     "qed2d_ctx_set_tiles": (_i, [_p, _i, _i, _i, _i]),
     "qed2d_ctx_get_tiles": (_i, [_p, _p]),
+    "qed2d_batch_create": (_i, [_p, _i, _p, _p, _p, _i, _p]),
+    "qed2d_batch_destroy": (None, [_p]),
+    "qed2d_batch_last_error": (C.c_char_p, [_p]),
+    "qed2d_batch_size": (_i, [_p, _p, _p]),
+    "qed2d_batch_offset": (_i, [_p, _i, _p]),
+    "qed2d_batch_links": (_i, [_p, _i, _p]),
+    "qed2d_batch_get_links": (_i, [_p, _i, _p]),
+    "qed2d_batch_set_links": (_i, [_p, _i, _p]),
+    "qed2d_batch_set_tiles": (_i, [_p, _i, _i, _i]),
+    "qed2d_batch_hot_start": (_i, [_p, _p]),
+    "qed2d_batch_cg": (_i, [_p, _p, _p, _p, _i, _d, _i, _p]),
+    "qed2d_batch_hmc_nf2": (_i, [_p, _p, _d, _i, _i, _i, _d, _i, _d, _p, _u32, _p, _p, _p, _p]),
+    "qed2d_batch_observables": (_i, [_p, _p, _p, _p]),
     "qed2d_ctx_autotune": (_i, [_p, _i, _p]),
     "qed2d_hbm_stream": (_i, [_p, C.c_long, _p]),
     "qed2d_ctx_set_reliable_delta": (_i, [_p, _d]),
@@ -751,6 +764,117 @@ def HMC(U, *args, integrator=None, qzero=False, reversibility_test=False,
     if acc_o.value >= 0 and acc is not None:
         acc.append(acc_o.value)
     return TrajInfo(hi.value, hf.value, acc_o.value, it.value, t)
+
+
+# ---------------------------------------------------------------------------
+# Batched ensembles (no counterpart in the reference: it runs one lattice per
+# process, main/hmc-nf2.jl)
+# ---------------------------------------------------------------------------
+class Batch:
+    """Independent chains of different lattice sizes and couplings advanced
+    together: every CG iteration, Q apply, force and link update is one launch
+    over all chains.  Chain c draws its random numbers from (seeds[c], chain c,
+    trajectory), as ``HMC`` on a single lattice with ``seed(seeds[c], chain=c)``.
+    """
+
+    def __init__(self, sizes, betas, bc=BC_PERIODIC):
+        import torch
+        n = len(sizes)
+        if len(betas) != n:
+            raise ValueError("one beta per chain")
+        self.sizes = [(int(a), int(b)) for a, b in sizes]
+        self.betas = [float(x) for x in betas]
+        L1 = (_i * n)(*[a for a, _ in self.sizes])
+        L2 = (_i * n)(*[b for _, b in self.sizes])
+        h = _p()
+        rc = lib.qed2d_batch_create(C.byref(h), n, L1, L2, _arr(self.betas), int(bc),
+                                    _p(torch.cuda.current_stream().cuda_stream))
+        if rc != QED2D_OK:
+            raise QED2dError(rc, "qed2d_batch_create failed")
+        self._b = h
+        self.n = n
+        nchain, sites = _i(), C.c_long()
+        lib.qed2d_batch_size(h, C.byref(nchain), C.byref(sites))
+        self.sites = sites.value
+        self.offsets = []
+        for c in range(n):
+            o = C.c_long()
+            lib.qed2d_batch_offset(h, c, C.byref(o))
+            self.offsets.append(o.value)
+
+    def _check(self, rc):
+        if rc != QED2D_OK:
+            msg = lib.qed2d_batch_last_error(self._b)
+            raise (CGNotConverged if rc == QED2D_ENOCONV else QED2dError)(
+                rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "_b", None) is not None and not _exiting:
+            lib.qed2d_batch_destroy(self._b)
+        self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def arena_spinor(self):
+        """A zero arena-shaped spinor (all chains' fields back to back)."""
+        import torch
+        return torch.zeros(2 * self.sites, dtype=torch.complex128, device="cuda")
+
+    def field(self, t, c):
+        """Chain c's (2, L2, L1) view of an arena-shaped field."""
+        L1, L2 = self.sizes[c]
+        o = 2 * self.offsets[c]
+        return t[o:o + 2 * L1 * L2].view(2, L2, L1)
+
+    def get_links(self, c):
+        import torch
+        L1, L2 = self.sizes[c]
+        U = torch.empty((2, L2, L1), dtype=torch.complex128, device="cuda")
+        self._check(lib.qed2d_batch_get_links(self._b, c, _ptr(U)))
+        return U
+
+    def set_links(self, c, U):
+        L1, L2 = self.sizes[c]
+        self._check(lib.qed2d_batch_set_links(self._b, c, _ptr(U, None, (2, L2, L1))))
+
+    def set_tiles(self, q64=0, cg64=0, cg32=0):
+        self._check(lib.qed2d_batch_set_tiles(self._b, int(q64), int(cg64), int(cg32)))
+
+    def hot_start(self, seeds):
+        s = (_u64 * self.n)(*[int(x) for x in seeds])
+        self._check(lib.qed2d_batch_hot_start(self._b, s))
+
+    def CG(self, x, src, am0s, maxiter, tol, precision=PREC_F64):
+        """(gamm5Dw)^2 x = src on every chain (x, src arena-shaped); returns the
+        iterations per chain."""
+        it = (_i * self.n)()
+        self._check(lib.qed2d_batch_cg(self._b, _ptr(x), _ptr(src), _arr(am0s), int(maxiter),
+                                       float(tol), int(precision), it))
+        return list(it)
+
+    def HMC(self, am0s, eps, ns, CGmaxiter, tol, seeds, traj, integrator=None,
+            precision=PREC_F64, acc_tol=0.0):
+        """One Nf=2 trajectory of every chain; returns a TrajInfo per chain."""
+        integ = integrator or Leapfrog()
+        n = self.n
+        hi, hf = (_d * n)(), (_d * n)()
+        acc, its = (_i * n)(), (_i * n)()
+        s = (_u64 * n)(*[int(x) for x in seeds])
+        self._check(lib.qed2d_batch_hmc_nf2(self._b, _arr(am0s), float(eps), int(ns), integ.code,
+                                            int(CGmaxiter), float(tol), int(precision),
+                                            float(acc_tol), s, int(traj), hi, hf, acc, its))
+        return [TrajInfo(hi[c], hf[c], acc[c], its[c], int(traj)) for c in range(n)]
+
+    def observables(self):
+        """Per-chain (plaquette, topological charge, gauge action) lists."""
+        n = self.n
+        p, q, a = (_d * n)(), (_d * n)(), (_d * n)()
+        self._check(lib.qed2d_batch_observables(self._b, p, q, a))
+        return list(p), list(q), list(a)
 
 
 # ---------------------------------------------------------------------------

@@ -216,6 +216,46 @@ int qed2d_reweighting_factor(qed2d_ctx *ctx, const double *U, double am0, int n,
                              int taylor, int maxiter, double tol, uint64_t seed, uint32_t chain,
                              uint32_t traj, double *W);
 
+/* ---- batched ensembles ------------------------------------------------------
+ * Many independent chains (lattices of different sizes and couplings) advanced
+ * together; no counterpart in the reference, which runs one lattice per
+ * process (main/hmc-nf2.jl).  The links of all chains live in one device arena
+ * owned by the batch: chain c's links are a reference-layout [L1,L2,2] field at
+ * qed2d_batch_links(b, c) (site offset qed2d_batch_offset).  Every CG
+ * iteration, Q apply, force + momentum update and link update of a call is
+ * one launch over all chains; each chain's trajectory is HMC! for Nf=2
+ * (src/QED2dHMC.jl:64-113) drawn from the streams of (seed[c], chain c, traj),
+ * the same as qed2d_hmc_nf2(..., seed[c], c, traj, ...) on that lattice. */
+typedef struct qed2d_batch qed2d_batch;
+int qed2d_batch_create(qed2d_batch **b, int nchain, const int *L1, const int *L2,
+                       const double *beta, int bc_flags, void *stream);
+void qed2d_batch_destroy(qed2d_batch *b);
+const char *qed2d_batch_last_error(const qed2d_batch *b);
+int qed2d_batch_size(const qed2d_batch *b, int *nchain, long *sites);
+int qed2d_batch_offset(const qed2d_batch *b, int chain, long *site_offset);
+int qed2d_batch_links(qed2d_batch *b, int chain, double **U);
+/* Copies of chain c's links out of / into the arena (host or device buffer,
+ * reference layout; synchronous). */
+int qed2d_batch_get_links(qed2d_batch *b, int chain, double *dst);
+int qed2d_batch_set_links(qed2d_batch *b, int chain, const double *src);
+/* Tile heights of the batch's FP64 apply, FP64 and FP32 solver kernels
+ * (0 leaves one unchanged). */
+int qed2d_batch_set_tiles(qed2d_batch *b, int q64, int cg64, int cg32);
+/* Hot start of every chain: link angles uniform in [-pi, pi) from (seed[c], c). */
+int qed2d_batch_hot_start(qed2d_batch *b, const uint64_t *seed);
+/* (Q^2) x = src on every chain from x = 0 (CG v1 semantics per chain); x and
+ * src are arena-shaped (2 x total sites complex).  iters[c] per chain. */
+int qed2d_batch_cg(qed2d_batch *b, double *x, const double *src, const double *am0,
+                   int maxiter, double tol, int precision, int *iters);
+/* One Nf=2 trajectory of every chain; am0, seed, hini, hfin, accepted, cg_iters
+ * hold one entry per chain (cg_iters may be NULL). */
+int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, int integrator,
+                        int maxiter, double tol, int precision, double acc_tol,
+                        const uint64_t *seed, uint32_t traj, double *hini, double *hfin,
+                        int *accepted, int *cg_iters);
+/* Per-chain plaquette, topological charge and gauge action (any may be NULL). */
+int qed2d_batch_observables(qed2d_batch *b, double *plaquette, double *qtop, double *action);
+
 /* ---- Zolotarev coefficients (host only) ---------------------------------------
  * get_rhmc_params(n, r_a, r_b) (src/QED2dRHMC.jl:93-110) -> RHMCParm fields
  * (src/QED2d.jl:15-26).  rho, mu, nu hold n doubles each. */

@@ -1,0 +1,117 @@
+"""Batched ensembles (qed2d_batch_*): a chain of a batch against the
+single-lattice entry points on the same lattice, seeds and tiles.  The batch
+runs the same device code per chain with the same tiling, so the comparison
+is bit-exact (CG solutions, energies, accept/reject, links), FP64 and mixed
+precision.  The ensemble of 64 chains of uneven sizes is checked for mixed vs
+FP64 agreement (accept/reject identical, |dH_mixed - dH_fp64| < 1e-9)."""
+import numpy as np
+import pytest
+
+from gpu_util import dev, host, relerr, say
+
+pytestmark = pytest.mark.gpu
+
+TILES = dict(q64=4, cg64=16, cg32=8)
+
+
+def _single(Q, L1, L2, beta, bc=0):
+    prm = Q.LattParm((L1, L2), beta, bc)
+    Q.set_tiles(prm, q64=TILES["q64"], cg64=TILES["cg64"], cg32=TILES["cg32"])
+    return prm
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_cg_bit_exact(Q, O, prec):
+    import torch
+    say("batch CG prec", prec)
+    sizes = [(16, 16), (32, 24), (64, 64), (30, 18)]
+    betas = [2.0, 4.0, 6.0, 4.0]
+    am0s = [0.1, 0.2, 0.15, 0.3]
+    B = Q.Batch(sizes, betas)
+    B.set_tiles(TILES["q64"], TILES["cg64"], TILES["cg32"])
+    B.hot_start([11, 12, 13, 14])
+    src = B.arena_spinor()
+    for c, (L1, L2) in enumerate(sizes):
+        prm = _single(Q, L1, L2, betas[c])
+        Q.fill_pseudofermion(B.field(src, c), prm, 500 + c, c, 0, 0)
+    x = B.arena_spinor()
+    its = B.CG(x, src, am0s, 10000, 1e-20, precision=prec)
+    for c, (L1, L2) in enumerate(sizes):
+        prm = _single(Q, L1, L2, betas[c])
+        U = B.get_links(c)
+        U1 = Q.spinor(prm)
+        Q.hot_start(U1, prm, 11 + c, chain=c)
+        assert torch.equal(U, U1)
+        xs = Q.spinor(prm)
+        it = Q.CG(xs, U, B.field(src, c).contiguous(), am0s[c], 10000, 1e-20,
+                  Q.gamm5Dw_sqr_msq, prm, None, precision=prec)
+        assert it == its[c], (c, it, its[c])
+        assert torch.equal(xs, B.field(x, c)), c
+
+
+@pytest.mark.parametrize("prec,integ", [(0, "leapfrog"), (1, "leapfrog"), (1, "omf4")])
+def test_batch_hmc_bit_exact(Q, O, prec, integ):
+    import torch
+    say("batch HMC prec", prec, integ)
+    sizes = [(16, 16), (32, 32), (24, 16)]
+    betas = [2.0, 4.0, 6.0]
+    am0s = [0.1, 0.2, 0.1]
+    seeds = [100, 101, 102]
+    integrator = Q.OMF4() if integ == "omf4" else Q.Leapfrog()
+    ns = 3 if integ == "omf4" else 10
+    B = Q.Batch(sizes, betas)
+    B.set_tiles(TILES["q64"], TILES["cg64"], TILES["cg32"])
+    B.hot_start(seeds)
+    singles = []
+    for c, (L1, L2) in enumerate(sizes):
+        prm = _single(Q, L1, L2, betas[c])
+        singles.append((prm, B.get_links(c).clone()))
+    for t in range(3):
+        infos = B.HMC(am0s, 0.1, ns, 100000, 1e-22, seeds, t, integrator=integrator,
+                      precision=prec)
+        for c, (prm, U) in enumerate(singles):
+            Q.seed(seeds[c], chain=c, traj=t)
+            ref = Q.HMC(U, am0s[c], 0.1, ns, [], 100000, 1e-22, prm, None,
+                        integrator=integrator, precision=prec)
+            assert infos[c].accepted == ref.accepted, (t, c)
+            assert infos[c].hini == ref.hini and infos[c].hfin == ref.hfin, (t, c)
+            assert infos[c].cg_iters == ref.cg_iters, (t, c)
+            assert torch.equal(B.get_links(c), U), (t, c)
+    p, q, a = B.observables()
+    for c, (prm, U) in enumerate(singles):
+        assert p[c] == Q.Plaquette(U, prm) and a[c] == Q.Action(U, prm)
+
+
+def _ensemble():
+    sizes = [(16, 16)] * 16 + [(32, 32)] * 16 + [(64, 64)] * 16 + [(128, 128)] * 8 + \
+            [(256, 256)] * 8
+    betas = [(2.0, 4.0, 6.0)[c % 3] for c in range(64)]
+    am0s = [(0.1, 0.2)[(c // 3) % 2] for c in range(64)]
+    seeds = [1000 + c for c in range(64)]
+    return sizes, betas, am0s, seeds
+
+
+def test_batch_ensemble_64_chains_mixed_vs_fp64(Q, O):
+    """64 chains of uneven sizes (16:16:16:8:8 of 16^2 .. 256^2), beta in
+    {2, 4, 6}, m0 in {0.1, 0.2}, each seeded separately; leapfrog ns 10 eps
+    0.1; from hot starts and two mixed-precision trajectories, one trajectory
+    in mixed precision against the same trajectory in FP64 (FP64 acceptance
+    check, CG tol 1e-24; at 1e-22 the largest difference was 1.05e-9)."""
+    import torch
+    say("64-chain ensemble")
+    sizes, betas, am0s, seeds = _ensemble()
+    B = Q.Batch(sizes, betas)
+    B.hot_start(seeds)
+    for t in range(2):
+        B.HMC(am0s, 0.1, 10, 100000, 1e-20, seeds, t, precision=1)
+    U0 = [B.get_links(c).clone() for c in range(64)]
+    res = {}
+    for prec in (1, 0):
+        for c in range(64):
+            B.set_links(c, U0[c])
+        res[prec] = B.HMC(am0s, 0.1, 10, 100000, 1e-24, seeds, 2, precision=prec,
+                          acc_tol=1e-24)
+    dd = [abs(a.dH - b.dH) for a, b in zip(res[1], res[0])]
+    say("max |dH_mixed - dH_fp64|", max(dd), "accepted", sum(i.accepted for i in res[1]))
+    assert [i.accepted for i in res[1]] == [i.accepted for i in res[0]]
+    assert max(dd) < 1e-9

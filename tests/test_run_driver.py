@@ -1,0 +1,86 @@
+"""The run drivers (qed2d.run, counterpart of main/hmc-nf2.jl, main/rhmc.jl,
+main/MC.jl): input schema of extra/*.in and the statistics / config file
+conventions on the CPU; whole runs, and a continued run reproducing an
+uninterrupted one, on the GPU."""
+import json
+import os
+
+import pytest
+
+NF2_IN = {"Run": {"user": "u", "host": "h", "name": "t", "seed": 77},
+          "Lattice": {"size": [8, 6], "beta": 4.0},
+          "HMC": {"tau": 1.0, "ns": 10, "nthm": 0, "ntraj": 4, "Qzero": False, "mass": 0.3}}
+RHMC_IN = {"Run": {"user": "u", "host": "h", "name": "t", "seed": 5},
+           "Lattice": {"size": [8, 8], "beta": 5.0},
+           "HMC": {"tau": 1.0, "ns": 10, "nthm": 0, "ntraj": 2, "Qzero": False},
+           "RHMC": {"masses": [0.3, 0.3], "n_rhmc": [4, 4], "r_a_g5Dw_sqr": [0.05, 0.05],
+                    "r_b_g5Dw_sqr": [40.0, 40.0]}}
+MC_IN = {"Run": {"user": "u", "host": "h", "name": "t", "seed": 1234, "measurements": 4},
+         "Lattice": {"size": [16, 16], "beta": 4.0},
+         "HMC": {"ns": 10, "eps": 0.1, "ntraj": 3, "nthm": 2, "Qzero": False}}
+
+
+def _write(tmp_path, name, d):
+    p = tmp_path / name
+    p.write_text(json.dumps(d))
+    return str(p)
+
+
+def test_read_options(tmp_path):
+    from qed2d import run as R
+    o = R.read_options(_write(tmp_path, "nf2.in", NF2_IN), "hmc-nf2")
+    assert o["size"] == (8, 6) and o["mass"] == 0.3 and o["eps"] == 0.1 and o["seed"] == 77
+    o = R.read_options(_write(tmp_path, "rhmc.in", RHMC_IN), "rhmc")
+    assert o["masses"] == [0.3, 0.3] and o["n_rhmc"] == [4, 4]
+    assert abs(o["r_a"][0] - 0.05 ** 0.5) < 1e-15 and abs(o["r_b"][1] - 40.0 ** 0.5) < 1e-15
+    o = R.read_options(_write(tmp_path, "mc.in", MC_IN), "mc")
+    assert o["eps"] == 0.1 and o["nthm"] == 2
+    d = json.loads(json.dumps(NF2_IN))
+    d["Run"]["seed"] = 0
+    assert R.read_options(_write(tmp_path, "z.in", d), "hmc-nf2")["seed"] > 0
+
+
+def test_config_name():
+    import qed2d as Q
+    prm = Q.LattParm((128, 64), 11.25)
+    assert Q.config_name(prm, 0.2, 300) == "config_128_64_b11.25_m0.2_n300"
+
+
+@pytest.mark.gpu
+def test_nf2_run_and_continuation(tmp_path):
+    from qed2d import run as R
+    inp = _write(tmp_path, "nf2.in", NF2_IN)
+    a = str(tmp_path / "a")
+    rows = R.run("hmc-nf2", inp, a, 0, save_every=2, tol=1e-20)
+    assert len(rows) == 4
+    lines = open(os.path.join(a, "statistics.txt")).read().splitlines()
+    assert lines[0] == "step,accepted,plaquette,top_charge,acc_rate"
+    assert [ln.split(",")[0] for ln in lines[1:]] == ["1", "2", "3", "4"]
+    assert os.path.exists(os.path.join(a, "configs", "config_8_6_b4.0_m0.3_n2"))
+    # continue from configuration 2: trajectories 3 and 4 again
+    b = str(tmp_path / "b")
+    os.makedirs(os.path.join(b, "configs"))
+    for f in ("config_8_6_b4.0_m0.3_n2", "config_8_6_b4.0_m0.3_n2.rng.json"):
+        with open(os.path.join(a, "configs", f)) as src, open(os.path.join(b, "configs", f),
+                                                             "w") as dst:
+            dst.write(src.read())
+    open(os.path.join(b, "statistics.txt"), "w").write("".join(ln + "\n" for ln in lines[:3]))
+    d = json.loads(json.dumps(NF2_IN))
+    d["HMC"]["ntraj"] = 2
+    rows_b = R.run("hmc-nf2", _write(tmp_path, "nf2b.in", d), b, 2, save_every=2, tol=1e-20)
+    # the acceptance rate restarts with the continued run (as the reference's
+    # acc vector does); step, accept, plaquette and charge must agree
+    assert [r.split(",")[:4] for r in rows_b] == [r.split(",")[:4] for r in rows[2:]]
+
+
+@pytest.mark.gpu
+def test_rhmc_and_mc_runs(tmp_path):
+    from qed2d import run as R
+    rows = R.run("rhmc", _write(tmp_path, "r.in", RHMC_IN), str(tmp_path / "r"), 0, tol=1e-20)
+    assert len(rows) == 2 and all(len(r.split(",")) == 8 for r in rows)
+    head = open(str(tmp_path / "r" / "statistics.txt")).readline()
+    assert head == R.RHMC_HEADER
+    w = float(rows[0].split(",")[4])
+    assert 0.5 < w < 2.0
+    rows = R.run("mc", _write(tmp_path, "m.in", MC_IN), str(tmp_path / "m"), 0)
+    assert len(rows) == 3
