@@ -11,7 +11,7 @@ trajectories.  The acceptance step checks the last MD solve in FP64 (true
 residual below tol, FP64 iterations if not) and evaluates the pseudofermion
 action as 2 Re X^dag F - |Q X|^2, in both precisions.
 
-    python bench.py [--steps K] [--warmup W] [--L 1024] [--tol 1e-22]
+    python bench.py [--steps K] [--warmup W] [--L 1024] [--tol 1e-24]
 
 Prints ONE JSON line (rank 0).  Only N = 1 GPU is supported.
 """
@@ -62,7 +62,7 @@ def main():
     ap.add_argument("--m0", type=float, default=0.2)
     ap.add_argument("--ns", type=int, default=10)
     ap.add_argument("--eps", type=float, default=0.1)
-    ap.add_argument("--tol", type=float, default=1e-22)
+    ap.add_argument("--tol", type=float, default=1e-24)
     ap.add_argument("--therm", type=int, default=200)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -70,6 +70,7 @@ def main():
     ap.add_argument("--tile32", type=int, default=0, help="FP32 tile rows (0: library default)")
     ap.add_argument("--no-dw", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--tiles", type=str, default="",
                     help="q64,q32,cg64,cg32 tile heights (skips the autotuner)")
     args = ap.parse_args()
@@ -190,6 +191,8 @@ def main():
     roof["achievable_copy_GBs"] = round(stream["copy"], 1)
     if not args.no_dw:
         res["dw_apply"] = dw_apply(Q, torch, args)
+    if not args.no_ensemble:
+        res["ensemble_64_chains"] = ensemble(Q, torch)
 
     # ---- CPU oracle on the host cores (bounded sample) ----------------------
     cpu = None
@@ -281,6 +284,58 @@ def dw_apply(Q, torch, args):
 
 
 DW_TILES = (2, 4, 8)
+
+
+def ensemble(Q, torch, ntraj=3):
+    """64 independent Nf=2 chains of uneven sizes (16:16:16:8:8 chains of
+    16^2, 32^2, 64^2, 128^2, 256^2; beta cycling over {2, 4, 6}, m0 over
+    {0.1, 0.2}, seed 1000 + chain), leapfrog ns 10 eps 0.1, CG tol 1e-16:
+    chain-trajectories per second of the batched entry point (mixed and FP64)
+    beside the same chains run one after another through the single-lattice
+    entry point (mixed)."""
+    sizes = [(16, 16)] * 16 + [(32, 32)] * 16 + [(64, 64)] * 16 + [(128, 128)] * 8 + \
+            [(256, 256)] * 8
+    n = len(sizes)
+    betas = [(2.0, 4.0, 6.0)[c % 3] for c in range(n)]
+    am0s = [(0.1, 0.2)[(c // 3) % 2] for c in range(n)]
+    seeds = [1000 + c for c in range(n)]
+    B = Q.Batch(sizes, betas)
+    B.hot_start(seeds)
+    for t in range(2):
+        B.HMC(am0s, 0.1, 10, 100000, 1e-16, seeds, t, precision=Q.PREC_MIXED)
+    U0 = [B.get_links(c).clone() for c in range(n)]
+    out = {"chains": n, "sites": B.sites, "trajectories_timed": ntraj}
+    for prec, key in ((Q.PREC_MIXED, "mixed"), (Q.PREC_F64, "fp64")):
+        for c in range(n):
+            B.set_links(c, U0[c])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in range(ntraj):
+            B.HMC(am0s, 0.1, 10, 100000, 1e-16, seeds, 2 + t, precision=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[f"{key}_chain_traj_per_s"] = round(n * ntraj / dt, 2)
+        log(f"ensemble {key}: {n} chains x {ntraj} traj in {dt:.3f}s")
+    # the same chains one at a time (single-lattice entry point, mixed)
+    prms = [Q.LattParm(sz, b) for sz, b in zip(sizes, betas)]
+    Us = [u.clone() for u in U0]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(ntraj):
+        for c in range(n):
+            Q.seed(seeds[c], chain=c, traj=2 + t)
+            Q.HMC(Us[c], am0s[c], 0.1, 10, [], 100000, 1e-16, prms[c], None,
+                  precision=Q.PREC_MIXED)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out["sequential_mixed_chain_traj_per_s"] = round(n * ntraj / dt, 2)
+    out["batched_vs_sequential"] = round(out["mixed_chain_traj_per_s"] /
+                                         out["sequential_mixed_chain_traj_per_s"], 2)
+    log(f"ensemble sequential: {n} chains x {ntraj} traj in {dt:.3f}s")
+    for p in prms:
+        p.close()
+    B.close()
+    return out
 
 
 def pmc_traffic(kernel, L):

@@ -43,6 +43,7 @@ struct qed2d_ctx {
     int rows_q = 4;   // FP64 Q / Dw applies
     int rows_q32 = 2; // FP32 Q apply of the public entry point
     double ru_delta = 0.01;  // mixed CG: reliable update after |r| drops by this factor
+    double last_dh = 0.0;    // dH of the last trajectory, from double-double energies
     std::string err;
     double2 *spin[S_NSLOTS] = {};
     float2 *fspin[FS_NSLOTS] = {};
@@ -246,6 +247,14 @@ struct hdd {
     }
     double value() const { return hi + lo; }
 };
+
+// a - b of two double-doubles, rounded once: dH resolved below the ulp of H.
+double dd_diff(const hdd &a, const hdd &b)
+{
+    dd x{a.hi, a.lo}, y{-b.hi, -b.lo};
+    x = dd_add(x, y);
+    return x.hi + x.lo;
+}
 
 // ---------------------------------------------------------------------------
 // elementary operators
@@ -1162,6 +1171,13 @@ int qed2d_hbm_stream(qed2d_ctx *c, long nbytes, double *gbs)
     return rc ? rc : finish(c);
 }
 
+int qed2d_ctx_last_dh(const qed2d_ctx *c, double *dH)
+{
+    if (!c || !dH) return QED2D_EINVAL;
+    *dH = c->last_dh;
+    return QED2D_OK;
+}
+
 int qed2d_synchronize(qed2d_ctx *c)
 {
     if (!c) return QED2D_EINVAL;
@@ -1644,6 +1660,7 @@ int qed2d_hmc_pure(qed2d_ctx *c, double *U, double eps, int ns, int integrator, 
     TRY(hamiltonian_dd(c, mom, U, &h1));
     *hini = h0.value();
     *hfin = h1.value();
+    c->last_dh = dd_diff(h1, h0);
     TRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
     if (!*accepted) launch_copy((double2 *)U, Ucp, n2, c->stream);
     return finish(c);
@@ -1708,6 +1725,7 @@ int qed2d_hmc_nf2(qed2d_ctx *c, double *U, double am0, double eps, int ns, int i
     }
     *hini = h0.value();
     *hfin = h1.value();
+    c->last_dh = dd_diff(h1, h0);
     TRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
     if (!*accepted) launch_copy((double2 *)U, Ucp, n2, st);
     return finish(c);
@@ -1774,6 +1792,7 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
     }
     *hini = h0.value();
     *hfin = h1.value();
+    c->last_dh = dd_diff(h1, h0);
     if (reversibility_test) {
         // src/QED2dHMC.jl:159-164
         launch_negate(mom, 2 * c->L.V, st);

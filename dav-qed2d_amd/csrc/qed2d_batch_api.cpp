@@ -43,6 +43,7 @@ struct qed2d_batch {
     std::string err;
     int rows = 16, rows32 = 8, rows_q = 4;
     double ru_delta = 0.01;
+    std::vector<double> last_dh;  // per chain, from the double-double energies
     std::map<int, GeoBuf> geo;  // key: 2 * rows + pair
     // fields (arena)
     double2 *U = nullptr, *Ucp = nullptr, *X = nullptr, *F = nullptr, *Y = nullptr;
@@ -380,6 +381,7 @@ int qed2d_batch_create(qed2d_batch **out, int nchain, const int *L1, const int *
         pb += march_grid(ch.L, 1).nblocks();
     }
     b->vtot = off;
+    b->last_dh.assign(nchain, 0.0);
     const long nel = 2 * off;
     bool ok = hipMalloc((void **)&b->dch, sizeof(BChain) * nchain) == hipSuccess &&
               hipMalloc((void **)&b->U, sizeof(double2) * nel) == hipSuccess &&
@@ -624,6 +626,8 @@ int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, i
         }
         hini[c] = h0h[c] + h0l[c];
         hfin[c] = h1.hi + h1.lo;
+        const dd dh = dd_add(h1, dd{-h0h[c], -h0l[c]});
+        b->last_dh[c] = dh.hi + dh.lo;
         // Metropolis (src/QED2dHMC.jl:95-102)
         const double pacc = std::exp(-(hfin[c] - hini[c]));
         if (pacc < 1.0)
@@ -635,6 +639,13 @@ int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, i
     }
     BTRY(blaunched(b));
     BHIP(b, hipStreamSynchronize(st));
+    return QED2D_OK;
+}
+
+int qed2d_batch_last_dh(const qed2d_batch *b, double *dH)
+{
+    if (!b || !dH) return QED2D_EINVAL;
+    for (int c = 0; c < b->n; ++c) dH[c] = b->last_dh[c];
     return QED2D_OK;
 }
 

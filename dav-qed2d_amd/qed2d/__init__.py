@@ -69,6 +69,8 @@ SIGNATURES = {
     "qed2d_batch_cg": (_i, [_p, _p, _p, _p, _i, _d, _i, _p]),
     "qed2d_batch_hmc_nf2": (_i, [_p, _p, _d, _i, _i, _i, _d, _i, _d, _p, _u32, _p, _p, _p, _p]),
     "qed2d_batch_observables": (_i, [_p, _p, _p, _p]),
+    "qed2d_batch_last_dh": (_i, [_p, _p]),
+    "qed2d_ctx_last_dh": (_i, [_p, _p]),
     "qed2d_ctx_autotune": (_i, [_p, _i, _p]),
     "qed2d_hbm_stream": (_i, [_p, C.c_long, _p]),
     "qed2d_ctx_set_reliable_delta": (_i, [_p, _d]),
@@ -703,10 +705,11 @@ class TrajInfo:
     accepted: int
     cg_iters: int = 0
     traj: int = 0
+    dh: float = None  # hfin - hini from the double-double energies (below the ulp of H)
 
     @property
     def dH(self):
-        return self.hfin - self.hini
+        return self.hfin - self.hini if self.dh is None else self.dh
 
 
 def HMC(U, *args, integrator=None, qzero=False, reversibility_test=False,
@@ -763,7 +766,9 @@ def HMC(U, *args, integrator=None, qzero=False, reversibility_test=False,
                         "after U")
     if acc_o.value >= 0 and acc is not None:
         acc.append(acc_o.value)
-    return TrajInfo(hi.value, hf.value, acc_o.value, it.value, t)
+    dh = _d()
+    lib.qed2d_ctx_last_dh(prm.ctx(), C.byref(dh))
+    return TrajInfo(hi.value, hf.value, acc_o.value, it.value, t, dh.value)
 
 
 # ---------------------------------------------------------------------------
@@ -867,7 +872,9 @@ class Batch:
         self._check(lib.qed2d_batch_hmc_nf2(self._b, _arr(am0s), float(eps), int(ns), integ.code,
                                             int(CGmaxiter), float(tol), int(precision),
                                             float(acc_tol), s, int(traj), hi, hf, acc, its))
-        return [TrajInfo(hi[c], hf[c], acc[c], its[c], int(traj)) for c in range(n)]
+        dh = (_d * n)()
+        self._check(lib.qed2d_batch_last_dh(self._b, dh))
+        return [TrajInfo(hi[c], hf[c], acc[c], its[c], int(traj), dh[c]) for c in range(n)]
 
     def observables(self):
         """Per-chain (plaquette, topological charge, gauge action) lists."""

@@ -96,6 +96,12 @@ int qed2d_ctx_autotune(qed2d_ctx *ctx, int iters, int tiles[4]);
  * nbytes each (allocated and freed here); gbs[3] receives GB/s counting the
  * bytes each kernel must move.  No counterpart in the reference. */
 int qed2d_hbm_stream(qed2d_ctx *ctx, long nbytes, double gbs[3]);
+/* dH = hfin - hini of the last trajectory of qed2d_hmc_pure / _nf2 / _rhmc on
+ * this context, formed from the double-double energies before they are
+ * rounded (hfin - hini of the returned doubles resolves only the ulp of H,
+ * 2^-30 at 1024^2).  The Metropolis test uses exp(-(hfin - hini)) as the
+ * reference (src/QED2dHMC.jl:95).  No counterpart in the reference. */
+int qed2d_ctx_last_dh(const qed2d_ctx *ctx, double *dH);
 /* Mixed-precision CG: a reliable update (FP64 true residual) once the FP32
  * iterated residual has dropped by delta since the last one (default 0.01). */
 int qed2d_ctx_set_reliable_delta(qed2d_ctx *ctx, double delta);
@@ -253,6 +259,9 @@ int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, i
                         int maxiter, double tol, int precision, double acc_tol,
                         const uint64_t *seed, uint32_t traj, double *hini, double *hfin,
                         int *accepted, int *cg_iters);
+/* Per-chain dH of the last trajectory from the double-double energies (see
+ * qed2d_ctx_last_dh). */
+int qed2d_batch_last_dh(const qed2d_batch *b, double *dH);
 /* Per-chain plaquette, topological charge and gauge action (any may be NULL). */
 int qed2d_batch_observables(qed2d_batch *b, double *plaquette, double *qtop, double *action);
 
