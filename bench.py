@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--no-dw", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--tiles", type=str, default="",
                     help="q64,q32,cg64,cg32 tile heights (skips the autotuner)")
     args = ap.parse_args()
@@ -193,6 +194,8 @@ def main():
         res["dw_apply"] = dw_apply(Q, torch, args)
     if not args.no_ensemble:
         res["ensemble_64_chains"] = ensemble(Q, torch)
+    if not args.no_configs:
+        res["configs"] = config_measurements(Q, torch, args.no_cpu)
 
     # ---- CPU oracle on the host cores (bounded sample) ----------------------
     cpu = None
@@ -335,6 +338,169 @@ def ensemble(Q, torch, ntraj=3):
     for p in prms:
         p.close()
     B.close()
+    return out
+
+
+def _oracle():
+    from oracle import qed2d_oracle as O
+    return O
+
+
+def config_measurements(Q, torch, no_cpu):
+    """The other workloads of the path, each timed on the GPU and (a bounded
+    sample) on the FP64 CPU oracle with the host's OpenMP threads."""
+    import numpy as np
+    O = None if no_cpu else _oracle()
+    out = {}
+
+    def sync_time(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    # Nf=2 24x24, beta 4, m0 0.2, hot start seed 1234, leapfrog ns 10 eps 0.1,
+    # CG tol 1e-12, 20 trajectories, FP64
+    prm = Q.LattParm((24, 24), 4.0)
+    U = Q.spinor(prm)
+    Q.hot_start(U, prm, 1234)
+    U0 = U.cpu().numpy()
+    Q.seed(1234)
+    dt, infos = sync_time(lambda: [Q.HMC(U, 0.2, 0.1, 10, [], 100000, 1e-12, prm, None)
+                                   for _ in range(20)])
+    c1 = {"gpu_traj_per_s": round(20 / dt, 2), "accepted": sum(i.accepted for i in infos)}
+    if O is not None:
+        Uh = U0.copy()
+        p = O.latt(24, 24, 4.0)
+        t0 = time.perf_counter()
+        ref = [O.hmc_nf2(Uh, 0.2, 0.1, 10, 100000, 1e-12, p, 1234, t) for t in range(20)]
+        c1["cpu_traj_per_s"] = round(20 / (time.perf_counter() - t0), 2)
+        c1["accept_identical"] = [i.accepted for i in infos] == [r[2] for r in ref]
+        c1["max_abs_ddH"] = max(abs(i.dH - (r[1] - r[0])) for i, r in zip(infos, ref))
+    out["nf2_24x24_20traj"] = c1
+    prm.close()
+
+    # 1024^2, links from 200 pure-gauge OMF4 trajectories at beta 4 (cold start,
+    # seed 1234), 16 Gaussian RHS, m0 0.1, CG to 1e-12: FP64 and mixed
+    L = 1024
+    prm = Q.LattParm((L, L), 4.0)
+    U = Q.cold_links(prm)
+    Q.seed(1234)
+    for _ in range(200):
+        Q.HMC(U, 0.1, 10, [], prm, None)
+    bs = []
+    for j in range(16):
+        b = Q.spinor(prm)
+        Q.fill_pseudofermion(b, prm, 1234, 5, 0, j)
+        bs.append(b)
+    c2 = {}
+    for prec, key in ((Q.PREC_F64, "fp64"), (Q.PREC_MIXED, "mixed")):
+        xs = [Q.spinor(prm) for _ in range(16)]
+        Q.CG(xs[0], U, bs[0], 0.1, 100000, 1e-12, Q.gamm5Dw_sqr_msq, prm, None, precision=prec)
+        dt, its = sync_time(lambda: [Q.CG(x, U, b, 0.1, 100000, 1e-12, Q.gamm5Dw_sqr_msq, prm,
+                                          None, precision=prec) for x, b in zip(xs, bs)])
+        bps = 384 if prec == Q.PREC_F64 else 192
+        c2[key] = {"seconds_16_rhs": round(dt, 4), "iterations": its,
+                   "GBs_cg_kernels": round(bps * L * L * sum(its) / dt / 1e9, 1)}
+    c2["mixed_vs_fp64"] = round(c2["fp64"]["seconds_16_rhs"] / c2["mixed"]["seconds_16_rhs"], 3)
+    if O is not None:
+        Uh, bh = U.cpu().numpy(), bs[0].cpu().numpy()
+        t0 = time.perf_counter()
+        _, it, st = O.cg(Uh, bh, 0.1, 100000, 1e-12)
+        c2["cpu_seconds_per_rhs"] = round(time.perf_counter() - t0, 3)
+        c2["cpu_iterations"] = it
+    out["cg_16rhs_1024x1024"] = c2
+    del bs, U
+    prm.close()
+
+    # RHMC 128^2, beta 4, m0 0.1, 16 poles on power-method bounds, 4
+    # pseudofermions, OMF4 ns 12 eps 0.083
+    prm = Q.LattParm((128, 128), 4.0)
+    U = Q.cold_links(prm)
+    Q.seed(77)
+    for _ in range(20):
+        Q.HMC(U, 0.1, 10, [], prm, None)
+    b0 = Q.spinor(prm)
+    Q.fill_pseudofermion(b0, prm, 77, 3, 0, 0)
+    lmin, lmax = Q.power_method(U, 0.1, prm, iter=5000, b0=b0)
+    rp = Q.get_rhmc_params(16, np.sqrt(lmin * 0.99), np.sqrt(lmax * 1.01))
+    Uc = U.clone()
+    dt, infos = sync_time(lambda: [Q.HMC(U, [0.1] * 4, 0.083, 12, [], 100000, 1e-16, prm, None,
+                                         [rp] * 4, integrator=Q.OMF4()) for _ in range(2)])
+    c3 = {"gpu_traj_per_s": round(2 / dt, 3), "dH": [round(i.dH, 8) for i in infos],
+          "delta": rp.delta}
+    if O is not None:
+        Uh = Uc.cpu().numpy()
+        z = O.zolotarev(16, np.sqrt(lmin * 0.99), np.sqrt(lmax * 1.01))
+        p = O.latt(128, 128, 4.0)
+        mom = np.zeros((2, 128, 128))
+        F = [O.pseudofermion(128, 128, 77, 0, 0, j) for j in range(4)]
+        X, g5 = np.zeros_like(F[0]), np.zeros_like(F[0])
+        mus = [np.ascontiguousarray(z["mu"]) for _ in range(4)]
+        rhos = [np.ascontiguousarray(z["rho"]) for _ in range(4)]
+        arr = lambda v: (O.C.c_void_p * 4)(*[a.ctypes.data for a in v])
+        am0 = np.full(4, 0.1)
+        npole = np.full(4, 16, dtype=np.int32)
+        t0 = time.perf_counter()
+        O.lib.orc_update_momenta_rhmc(O._ptr(mom), O._ptr(Uh), O._ptr(X), arr(F), O._ptr(g5),
+                                      O._ptr(am0), 4, O._ptr(npole), arr(mus), arr(rhos), 0.01,
+                                      100000, 1e-16, O.C.byref(p))
+        tf = time.perf_counter() - t0
+        c3["cpu_traj_per_s"] = round(1.0 / (tf * 6 * 12), 5)
+        c3["cpu_sample"] = "one RHMC force evaluation (4 pseudofermions x 16 sequential CG " \
+                           "solves), scaled by OMF4's 6 x 12 force evaluations"
+    out["rhmc_128x128_16poles_4pf"] = c3
+    prm.close()
+
+    # master field: pure gauge 8192^2, beta 11.25, OMF4 ns 20 eps 0.025
+    L = 8192
+    prm = Q.LattParm((L, L), 11.25)
+    U = Q.spinor(prm)
+    Q.hot_start(U, prm, 4321)
+    Q.seed(4321)
+    Q.HMC(U, 0.025, 20, [], prm, None)
+    dt, infos = sync_time(lambda: [Q.HMC(U, 0.025, 20, [], prm, None) for _ in range(2)])
+    c5 = {"gpu_traj_per_s": round(2 / dt, 3), "Q": round(Q.Qtop(U, prm), 6),
+          "plaquette": Q.Plaquette(U, prm)}
+    if O is not None:
+        Uh = U.cpu().numpy()
+        mom = O.momenta(L, L, 4321, 0, 9)
+        p = O.latt(L, L, 11.25)
+        t0 = time.perf_counter()
+        O.omf4_pure(mom, Uh, 0.025, 1, p)
+        c5["cpu_traj_per_s"] = round(1.0 / ((time.perf_counter() - t0) * 20), 5)
+        c5["cpu_sample"] = "one OMF4 step (6 gauge forces, 5 link updates) scaled by 20 steps"
+    out["pure_gauge_8192x8192"] = c5
+    del U
+    prm.close()
+    torch.cuda.empty_cache()
+
+    # Nf=2 4096^2, beta 11.25, m0 0.05, links after 3 pure-gauge trajectories
+    # from a cold start; one leapfrog trajectory ns 2 eps 0.05, CG tol 1e-20
+    L = 4096
+    prm = Q.LattParm((L, L), 11.25)
+    U = Q.cold_links(prm)
+    Q.seed(55, chain=2)
+    for _ in range(3):
+        Q.HMC(U, 0.025, 20, [], prm, None)
+    U0 = U.clone()
+    c6 = {}
+    for prec, key in ((Q.PREC_MIXED, "mixed"), (Q.PREC_F64, "fp64")):
+        U.copy_(U0)
+        Q.seed(55)
+        dt, info = sync_time(lambda: Q.HMC(U, 0.05, 0.05, 2, [], 100000, 1e-20, prm, None,
+                                           precision=prec, acc_tol=1e-20))
+        c6[key] = {"seconds": round(dt, 3), "cg_iters": info.cg_iters, "dH": info.dH,
+                   "accepted": info.accepted}
+    c6["mixed_vs_fp64"] = round(c6["fp64"]["seconds"] / c6["mixed"]["seconds"], 3)
+    c6["abs_dH_diff"] = abs(c6["mixed"]["dH"] - c6["fp64"]["dH"])
+    out["nf2_4096x4096_m0.05"] = c6
+    del U, U0
+    prm.close()
+    torch.cuda.empty_cache()
+    for k, v in out.items():
+        log(f"config {k}: {v}")
     return out
 
 

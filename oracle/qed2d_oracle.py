@@ -96,6 +96,8 @@ _sig("orc_hmc_nf2", _i, [_p, _d, _d, _i, _i, _i, _d, _i, _u64, _u32, _u32, C.POI
                          _pd, _pd, C.POINTER(_i), C.POINTER(_i)])
 _sig("orc_hmc_rhmc", _i, [_p, _p, _i, _p, _p, _p, _p, _d, _i, _i, _i, _d, _i, _i, _u64, _u32,
                           _u32, C.POINTER(Latt), _pd, _pd, C.POINTER(_i)])
+_sig("orc_update_momenta_rhmc", _i, [_p, _p, _p, _p, _p, _p, _i, _p, _p, _p, _d, _i, _d,
+                                     C.POINTER(Latt)])
 _sig("orc_update_momenta_nf2", _i, [_p, _p, _p, _p, _p, _d, _d, _i, _d, C.POINTER(Latt)])
 _sig("orc_omf4_pure", None, [_p, _p, _d, _i, C.POINTER(Latt)])
 _sig("orc_leapfrog_pure", None, [_p, _p, _d, _i, C.POINTER(Latt)])
