@@ -51,6 +51,21 @@ struct qed2d_ctx {
     std::vector<double2 *> flav;  // RHMC pseudofermions, one per flavour
     std::vector<double2 *> shiftbuf;  // multi-shift CG: internal x_i
     std::vector<double2 *> psbuf;     // multi-shift CG: search directions p_i
+    // multi-shift CG of several right-hand sides in one launch per kernel (the
+    // pseudofermions of an RHMC force): a chain set of nf copies of this
+    // lattice sharing the links
+    struct {
+        int cap = 0, n = 0, rows = 0;
+        BChain *dch = nullptr;
+        int *blk0 = nullptr;
+        int4 *shape = nullptr;
+        MultiShiftState *M = nullptr, *hM = nullptr;
+        double *part = nullptr;
+        int *cnt = nullptr;
+        BParts P{};
+        double2 *arena = nullptr;  // r, p0, p1, t of every right-hand side
+        std::vector<std::vector<double2 *>> x, ps;  // per right-hand side and shift
+    } mr;
     int multishift = 1;               // MultiCG / RHMC force through multi-shift CG
     MultiShiftState *ms = nullptr;    // device
     MultiShiftState *hms = nullptr;   // pinned
@@ -587,6 +602,156 @@ int multishift_cg(qed2d_ctx *c, double2 *const *x, const double *U, const double
     return fail(c, QED2D_ENOCONV, "multi-shift CG not converged after %d iterations", maxiter);
 }
 
+// Multi-shift CG of nf right-hand sides b[j] with shifts mu[j][0..npole[j]-1]
+// and masses am0[j] at once: every K1/K2 is one launch over all right-hand
+// sides (a chain set of nf copies of the lattice sharing U), each with its own
+// Krylov scalars and stopping.  The solutions land in c->mr.x[j][i].  Per
+// right-hand side the tiling and arithmetic are those of multishift_cg.
+int mr_setup(qed2d_ctx *c, int nf)
+{
+    auto &m = c->mr;
+    if (nf > m.cap) {
+        for (void *p : {(void *)m.dch, (void *)m.blk0, (void *)m.shape, (void *)m.M,
+                        (void *)m.part, (void *)m.cnt, (void *)m.arena})
+            if (p) (void)hipFree(p);
+        if (m.hM) (void)hipHostFree(m.hM);
+        m = {};
+        const long nbc = march_grid(c->L, 1).nblocks();
+        const long n2 = spinor_elems(c);
+        if (hipMalloc((void **)&m.dch, sizeof(BChain) * nf) != hipSuccess ||
+            hipMalloc((void **)&m.blk0, sizeof(int) * (nf + 1)) != hipSuccess ||
+            hipMalloc((void **)&m.shape, sizeof(int4) * nf) != hipSuccess ||
+            hipMalloc((void **)&m.M, sizeof(MultiShiftState) * nf) != hipSuccess ||
+            hipHostMalloc((void **)&m.hM, sizeof(MultiShiftState) * nf) != hipSuccess ||
+            hipMalloc((void **)&m.part, sizeof(double) * 7 * nbc * nf) != hipSuccess ||
+            hipMalloc((void **)&m.cnt, sizeof(int) * 5 * nf) != hipSuccess ||
+            hipMalloc((void **)&m.arena, sizeof(double2) * 4 * n2 * nf) != hipSuccess)
+            return fail(c, QED2D_ENOMEM, "multi-shift right-hand-side set allocation failed");
+        const long pb = nbc * nf;
+        m.P.tt[0] = m.part;
+        m.P.tt[1] = m.part + 2 * pb;
+        m.P.rr[0] = m.part + 4 * pb;
+        m.P.rr[1] = m.part + 5 * pb;
+        m.P.b = m.part + 6 * pb;
+        m.P.ntt[0] = m.cnt;
+        m.P.ntt[1] = m.cnt + nf;
+        m.P.nrr[0] = m.cnt + 2 * nf;
+        m.P.nrr[1] = m.cnt + 3 * nf;
+        m.P.nb = m.cnt + 4 * nf;
+        m.cap = nf;
+    }
+    if (m.n != nf || m.rows != c->rows) {
+        const MarchGrid g = march_grid(c->L, c->rows);
+        std::vector<int> blk0(nf + 1);
+        std::vector<int4> shape(nf);
+        for (int j = 0; j <= nf; ++j) blk0[j] = j * g.nblocks();
+        for (int j = 0; j < nf; ++j) shape[j] = make_int4(g.bx, g.by, g.nbx, g.nblocks());
+        HIPCHK(c, hipMemcpyAsync(m.blk0, blk0.data(), sizeof(int) * (nf + 1),
+                                 hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(m.shape, shape.data(), sizeof(int4) * nf, hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        m.n = nf;
+        m.rows = c->rows;
+    }
+    if ((int)m.x.size() < nf) {
+        m.x.resize(nf);
+        m.ps.resize(nf);
+    }
+    return QED2D_OK;
+}
+
+int multishift_multi(qed2d_ctx *c, const double *U, int nf, const double *const *b,
+                     const double *am0, const int *npole, const double *const *mu, int maxiter,
+                     double eps, int *iters)
+{
+    TRY(mr_setup(c, nf));
+    auto &m = c->mr;
+    const long n2 = spinor_elems(c);
+    const long nbc = march_grid(c->L, 1).nblocks();
+    std::vector<BChain> ch(nf);
+    for (int j = 0; j < nf; ++j) {
+        if (npole[j] < 1 || npole[j] > kMaxShift)
+            return fail(c, QED2D_EINVAL, "multi-shift CG takes 1..%d shifts", kMaxShift);
+        TRY(grow(c, m.x[j], npole[j]));
+        TRY(grow(c, m.ps[j], npole[j]));
+        ch[j].L = c->L;
+        ch[j].off = (long)j * c->L.V;
+        ch[j].uoff = 0;
+        ch[j].pb = (int)(j * nbc);
+        ch[j].pad = 0;
+        ch[j].am0 = am0[j];
+        ch[j].beta = c->beta;
+        MultiShiftState &h = m.hM[j];
+        std::memset(&h, 0, sizeof h);
+        double sb = mu[j][0] * mu[j][0];
+        for (int k = 1; k < npole[j]; ++k) sb = std::min(sb, mu[j][k] * mu[j][k]);
+        h.sigma_b = sb;
+        h.eps = eps;
+        h.maxiter = maxiter;
+        h.nshift = npole[j];
+        for (int k = 0; k < npole[j]; ++k) {
+            h.s[k] = mu[j][k] * mu[j][k] - sb;
+            h.x[k] = m.x[j][k];
+            h.ps[k] = m.ps[j][k];
+        }
+        h.b = (const double2 *)b[j];
+        h.kk = 1;
+        h.k2 = 1;
+    }
+    hipStream_t st = c->stream;
+    HIPCHK(c, hipMemcpyAsync(m.dch, ch.data(), sizeof(BChain) * nf, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(m.M, m.hM, sizeof(MultiShiftState) * nf, hipMemcpyHostToDevice,
+                             st));
+    BGeo g;
+    g.ch = m.dch;
+    g.blk0 = m.blk0;
+    g.shape = m.shape;
+    g.n = nf;
+    g.nblocks = nf * march_grid(c->L, c->rows).nblocks();
+    double2 *r = m.arena, *p0 = r + n2 * nf, *p1 = p0 + n2 * nf, *t = p1 + n2 * nf;
+    launch_b_ms_init(g, c->rows, r, p0, m.M, m.P, st);
+    std::vector<ChunkPredictor> pred(nf);
+    int chunk = 8;
+    long launched_k = 0;
+    for (;;) {
+        for (int jj = 0; jj < chunk; ++jj) {
+            PROF(c, QED2D_PROF_OTHER,
+                 launch_b_ms_k1(g, c->rows, (const double2 *)U, r, p0, p1, t, m.M, m.P, st));
+            PROF(c, QED2D_PROF_OTHER,
+                 launch_b_ms_k2(g, c->rows, (const double2 *)U, t, p0, p1, r, m.M, m.P, st));
+        }
+        launched_k += chunk;
+        TRY(launched(c));
+        HIPCHK(c, hipMemcpyAsync(m.hM, m.M, sizeof(MultiShiftState) * nf, hipMemcpyDeviceToHost,
+                                 st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        prof_harvest(c);
+        bool running = false;
+        int next = 64;
+        for (int j = 0; j < nf; ++j) {
+            const MultiShiftState &h = m.hM[j];
+            if (h.status != 0) continue;
+            running = true;
+            const int pch = pred[j].next(h.err, h.iter, h.tol, 0.0);
+            if (pch < next) next = pch;
+        }
+        if (!running) break;
+        if (launched_k > 4L * (maxiter + 2) + 64)
+            return fail(c, QED2D_EHIP, "multi-shift right-hand-side set stopped advancing");
+        chunk = next;
+    }
+    int it = 0;
+    for (int j = 0; j < nf; ++j) {
+        if (m.hM[j].status != 1)
+            return fail(c, QED2D_ENOCONV, "multi-shift CG of right-hand side %d not converged "
+                        "after %d iterations", j, maxiter);
+        it = std::max(it, m.hM[j].iter);
+    }
+    if (iters) *iters = it;
+    return QED2D_OK;
+}
+
 // ---------------------------------------------------------------------------
 // MD
 // ---------------------------------------------------------------------------
@@ -714,6 +879,27 @@ int update_momenta_rhmc(qed2d_ctx *c, double *mom, const double *U, double *X,
     double *facc = nullptr;
     TRY(get_real(c, R_FACC, &facc));
     bool first = true;
+    bool fused = (c->multishift == 1 && nf > 0);
+    for (int j = 0; j < nf; ++j) fused = fused && npole[j] >= 1 && npole[j] <= kMaxShift;
+    if (fused) {
+        // all poles of all flavours: one multi-shift solve over the right-hand sides
+        std::vector<const double *> bj(nf);
+        for (int j = 0; j < nf; ++j) bj[j] = (const double *)F[j];
+        int it = 0;
+        TRY(multishift_multi(c, U, nf, bj.data(), am0, npole, mu, maxiter, tol, &it));
+        for (int j = 0; j < nf; ++j)
+            for (int i = 0; i < npole[j]; ++i) {
+                const double *Xi = (const double *)c->mr.x[j][i];
+                q64(c, U, Xi, g5DX, am0[j]);
+                launch_force_acc(c->L, (const double2 *)U, (const double2 *)Xi,
+                                 (const double2 *)g5DX, facc, rho[j][i], first ? 1 : 0,
+                                 c->stream);
+                first = false;
+            }
+        launch_force_mom(c->L, (const double2 *)U, nullptr, nullptr, first ? nullptr : facc, mom,
+                         c->beta, eps, 0.0, c->stream);
+        return launched(c);
+    }
     if (c->multishift) {
         // all poles of a flavour in one multi-shift solve
         for (int j = 0; j < nf; ++j) {
@@ -1053,6 +1239,12 @@ void qed2d_ctx_destroy(qed2d_ctx *c)
     for (auto p : c->flav) if (p) (void)hipFree(p);
     for (auto p : c->shiftbuf) if (p) (void)hipFree(p);
     for (auto p : c->psbuf) if (p) (void)hipFree(p);
+    for (auto &v : c->mr.x) for (auto p : v) if (p) (void)hipFree(p);
+    for (auto &v : c->mr.ps) for (auto p : v) if (p) (void)hipFree(p);
+    for (void *p : {(void *)c->mr.dch, (void *)c->mr.blk0, (void *)c->mr.shape, (void *)c->mr.M,
+                    (void *)c->mr.part, (void *)c->mr.cnt, (void *)c->mr.arena})
+        if (p) (void)hipFree(p);
+    if (c->mr.hM) (void)hipHostFree(c->mr.hM);
     if (c->ms) (void)hipFree(c->ms);
     if (c->hms) (void)hipHostFree(c->hms);
     if (c->part) (void)hipFree(c->part);
@@ -1832,10 +2024,11 @@ int qed2d_load_gauge(qed2d_ctx *c, double *U, const char *path)
     return QED2D_OK;
 }
 
-int qed2d_ctx_set_multishift(qed2d_ctx *c, int on)
+int qed2d_ctx_set_multishift(qed2d_ctx *c, int mode)
 {
     CHECK_CTX(c);
-    c->multishift = on ? 1 : 0;
+    if (mode < 0 || mode > 2) return fail(c, QED2D_EINVAL, "multi-shift mode must be 0, 1 or 2");
+    c->multishift = mode;
     return QED2D_OK;
 }
 

@@ -21,6 +21,7 @@ struct BWhere {
     int c;
     Latt L;
     long o;      // element offset of the chain's fields (2 * off)
+    long uo;     // element offset of the chain's links (2 * uoff)
     int pb;      // first partial slot of the chain
     int nb;      // blocks of the chain in this launch
     double am0, beta;
@@ -44,6 +45,7 @@ __device__ __forceinline__ BWhere bwhere(const BGeo &g, int R)
     w.c = lo;
     w.L = ch.L;
     w.o = 2 * ch.off;
+    w.uo = 2 * ch.uoff;
     w.pb = ch.pb;
     w.nb = sh.w;
     w.am0 = ch.am0;
@@ -82,7 +84,7 @@ __global__ void __launch_bounds__(kBlock) kb_q_apply(BGeo g, const double2 *__re
     if (mask && !mask[w.c]) return;
     SrcPlain<double> src{si + w.o, w.L.V};
     EpiStore<double> epi{so + w.o, w.L.V};
-    q_march<double, R>(w.L, U + w.o, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
+    q_march<double, R>(w.L, U + w.uo, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
 }
 void launch_b_q_apply(const BGeo &g, int rows, const double2 *U, const double2 *si,
                       double2 *so, const int *mask, hipStream_t st)
@@ -105,7 +107,7 @@ __global__ void __launch_bounds__(kBlock) kb_force_mom(const BChain *ch, int n, 
         const BChain &b = ch[c];
         const long s = e - b.off;
         const long o = 2 * b.off;
-        force_mom_site(b.L, U + o, X ? X + o : nullptr, Y ? Y + o : nullptr, nullptr, mom + o,
+        force_mom_site(b.L, U + 2 * b.uoff, X ? X + o : nullptr, Y ? Y + o : nullptr, nullptr, mom + o,
                        b.beta, eps, 1.0, s);
     }
 }
@@ -196,7 +198,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
                  p0 ? p0 + o : nullptr, nullptr, Sc->mu2, w.L.V, 0.0, 0.0};
     if (w.p.active) {
         SrcPlain<double> src{t + o, w.L.V};
-        q_march<double, R>(w.L, U + o, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
+        q_march<double, R>(w.L, U + w.uo, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
     }
     const double a = block_allsum(epi.rr, lds);
     const double bb = block_allsum(epi.bb, lds);
@@ -274,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg1(BGeo g, const c2<T> *__restrict
     EpiK1<T> epi{pnew, t + o, w.L.V, 0.0, 0.0};
     if (w.p.active) {
         SrcXpay<T> src{r + o, pold, (T)beta, w.L.V};
-        q_march<T, R>(w.L, U + o, T(2) + (T)w.am0, src, epi, w.p.i1, w.p.r0);
+        q_march<T, R>(w.L, U + w.uo, T(2) + (T)w.am0, src, epi, w.p.i1, w.p.r0);
     }
     const double a = block_allsum(epi.tt, lds);
     const double b = block_allsum(epi.pp, lds);
@@ -304,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2(BGeo g, const c2<T> *__restrict
     EpiK2<T> epi{(wp ? P1 : P0) + o, x + o, r + o, (T)alpha, (T)Sc->mu2, w.L.V, 0.0};
     if (w.p.active) {
         SrcPlain<T> src{t + o, w.L.V};
-        q_march<T, R>(w.L, U + o, T(2) + (T)w.am0, src, epi, w.p.i1, w.p.r0);
+        q_march<T, R>(w.L, U + w.uo, T(2) + (T)w.am0, src, epi, w.p.i1, w.p.r0);
     }
     const double a = block_allsum(epi.rr, lds);
     if (threadIdx.x == 0) {
@@ -337,7 +339,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, const float2 *__restri
     EpiK1v epi{(wp ? P1 : P0) + o, t + o, w.L.V, 0.0, 0.0};
     if (w.p.active) {
         SrcXpay2 src{r + o, (rd ? P1 : P0) + o, (float)beta, w.L.V};
-        q_march2<R>(w.L, U + o, 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
+        q_march2<R>(w.L, U + w.uo, 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
     }
     const double a = block_allsum(epi.tt, lds);
     const double b = block_allsum(epi.pp, lds);
@@ -368,7 +370,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, const float2 *__restri
     EpiK2v epi{(wp ? P1 : P0) + o, x + o, r + o, (float)alpha, (float)Sc->mu2, w.L.V, 0.0};
     if (w.p.active) {
         SrcPlain2 src{t + o, w.L.V};
-        q_march2<R>(w.L, U + o, 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
+        q_march2<R>(w.L, U + w.uo, 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
     }
     const double a = block_allsum(epi.rr, lds);
     if (threadIdx.x == 0) {
@@ -417,6 +419,113 @@ void launch_b_cg_k2_f32(const BGeo &g, int rows, const float2 *U, const float2 *
 {
     DISPATCH_ROWS(rows, kb_cg2v, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, p0, p1, x, r, S,
                   P);
+}
+
+// ---------------------------------------------------------------------------
+// Multi-shift CG of several right-hand sides at once, one per chain (the
+// pseudofermions of an RHMC force, sharing the links: uoff = 0).  Chain c's
+// multi-shift state M[c] holds its shifts, its solution and direction
+// buffers and its right-hand side; the iteration index is per chain as in
+// the batched CG.  A chain that stopped at K1(s) has its K2(s) advance kk past
+// s, so K1(s) never runs twice: the step in which the last shifts converge
+// updates their solutions and must not repeat.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) kb_ms_init(BGeo g, int rows, double2 *__restrict__ r,
+                                                     double2 *__restrict__ p0,
+                                                     const MultiShiftState *M, BParts P)
+{
+    __shared__ double lds[kBlock / kWave + 1];
+    const BWhere w = bwhere<false>(g, rows);
+    const MultiShiftState *Mc = M + w.c;
+    const Latt &L = w.L;
+    const int nshift = Mc->nshift;
+    const double2 z = mk<double>(0.0, 0.0);
+    double acc = 0.0;
+    if (w.p.active) {
+        for (int kk = 0; kk < rows; ++kk) {
+            const int row = w.p.r0 + kk;
+            if (row >= L.L2) break;
+            for (int c = 0; c < 2; ++c) {
+                const long i = c * L.V + (long)row * L.L1 + w.p.i1;
+                const double2 v = Mc->b[i];
+                r[w.o + i] = v;
+                p0[w.o + i] = z;
+                for (int k = 0; k < nshift; ++k) {
+                    Mc->x[k][i] = z;
+                    Mc->ps[k][i] = z;
+                }
+                acc += v.x * v.x + v.y * v.y;
+            }
+        }
+    }
+    const double s = block_allsum(acc, lds);
+    if (threadIdx.x == 0) {
+        P.rr[0][w.pb + w.p.blk] = s;
+        if (w.p.blk == 0) P.nrr[0][w.c] = w.nb;
+    }
+}
+void launch_b_ms_init(const BGeo &g, int rows, double2 *r, double2 *p0,
+                      const MultiShiftState *M, BParts P, hipStream_t st)
+{
+    hipLaunchKernelGGL(kb_ms_init, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, r, p0, M, P);
+}
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) kb_ms1(BGeo g, const double2 *__restrict__ U,
+                                                 const double2 *__restrict__ r, double2 *P0,
+                                                 double2 *P1, double2 *__restrict__ t,
+                                                 MultiShiftState *M, BParts P)
+{
+    const BWhere w = bwhere<false>(g, R);
+    MultiShiftState *Mc = M + w.c;
+    const int k = Mc->kk;
+    const int sk = Mc->stop_k;
+    if (sk != 0 && sk < k) return;
+    const int wp = k & 1, rd = (k - 1) & 1;
+    if (w.p.blk == 0 && threadIdx.x == 0) {
+        Mc->k2 = k;
+        P.ntt[wp][w.c] = w.nb;
+    }
+    const long o = w.o;
+    ms1_body<R>(w.L, U + w.uo, r + o, P0 + o, P1 + o, t + o, 2.0 + w.am0, Mc,
+                P.tt[wp] + 2 * w.pb, P.tt[rd] + 2 * w.pb, P.rr[rd] + w.pb, P.nrr[rd][w.c],
+                P.ntt[rd][w.c], k, Mc->nshift, w.p);
+}
+
+template <int R>
+__global__ void __launch_bounds__(kBlock) kb_ms2(BGeo g, const double2 *__restrict__ U,
+                                                 const double2 *__restrict__ t, const double2 *P0,
+                                                 const double2 *P1, double2 *__restrict__ r,
+                                                 MultiShiftState *M, BParts P)
+{
+    const BWhere w = bwhere<false>(g, R);
+    MultiShiftState *Mc = M + w.c;
+    const int k = Mc->k2;
+    const int sk = Mc->stop_k;
+    if (sk != 0 && sk <= k) {
+        if (w.p.blk == 0 && threadIdx.x == 0) Mc->kk = k + 1;
+        return;
+    }
+    const int wp = k & 1;
+    const long o = w.o;
+    ms2_body<R>(w.L, U + w.uo, t + o, P0 + o, P1 + o, r + o, 2.0 + w.am0, Mc,
+                P.tt[wp] + 2 * w.pb, P.rr[wp] + w.pb, P.ntt[wp][w.c], k, w.p);
+    if (w.p.blk == 0 && threadIdx.x == 0) {
+        P.nrr[wp][w.c] = w.nb;
+        Mc->kk = k + 1;
+    }
+}
+
+void launch_b_ms_k1(const BGeo &g, int rows, const double2 *U, const double2 *r, double2 *p0,
+                    double2 *p1, double2 *t, MultiShiftState *M, BParts P, hipStream_t st)
+{
+    DISPATCH_ROWS(rows, kb_ms1, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, r, p0, p1, t, M, P);
+}
+void launch_b_ms_k2(const BGeo &g, int rows, const double2 *U, const double2 *t,
+                    const double2 *p0, const double2 *p1, double2 *r, MultiShiftState *M,
+                    BParts P, hipStream_t st)
+{
+    DISPATCH_ROWS(rows, kb_ms2, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, p0, p1, r, M, P);
 }
 
 }  // namespace qed2d

@@ -373,6 +373,7 @@ int qed2d_batch_create(qed2d_batch **out, int nchain, const int *L1, const int *
         ch.L.V = (long)L1[c] * L2[c];
         ch.L.bc = bc_flags;
         ch.off = off;
+        ch.uoff = off;
         ch.pb = (int)pb;
         ch.pad = 0;
         ch.am0 = 0.0;

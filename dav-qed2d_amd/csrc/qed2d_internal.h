@@ -49,6 +49,8 @@ struct MultiShiftState {
     int conv_iter[kMaxShift];
     double2 *x[kMaxShift];           // solution of shift i
     double2 *ps[kMaxShift];          // search direction of shift i
+    const double2 *b;                // batched solves: the right-hand side
+    int kk, k2;                      // batched solves: iteration K1 runs next / K2 runs
 };
 
 // Per-block partial sums of the solver kernels.  tt: (|Q p|^2, |p|^2) per block,
@@ -89,13 +91,15 @@ constexpr double OMF_R4 = -0.03230286765269967;
 int ctx_hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, double *hi, double *lo);
 
 // ---- batched ensembles ----------------------------------------------------
-// A chain set: independent lattices whose fields are concatenated in one
-// arena.  Chain c's spinor, link and momentum fields are reference-layout
-// fields starting at element 2 * off (its two planes of V sites); its solver
-// state is S[c]; its partial sums occupy slots pb .. pb + (its blocks) - 1.
+// A chain set: lattices whose fields are concatenated in one arena.  Chain
+// c's spinor and momentum fields are reference-layout fields starting at
+// element 2 * off (its two planes of V sites), its links at element 2 * uoff
+// (uoff = off for independent chains; uoff = 0 for right-hand sides that share
+// one gauge field); its solver state is S[c]; its partial sums occupy slots
+// pb .. pb + (its blocks) - 1.
 struct BChain {
     Latt L;
-    long off;
+    long off, uoff;
     int pb;
     int pad;
     double am0, beta;
@@ -139,6 +143,14 @@ void launch_b_cg_k2_f32(const BGeo &g, int rows, const float2 *U, const float2 *
                         SolverState *S, BParts P, hipStream_t st);
 void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,
                        const int *mask, hipStream_t st);
+// Multi-shift CG of one right-hand side per chain (M[c]: chain c's state).
+void launch_b_ms_init(const BGeo &g, int rows, double2 *r, double2 *p0,
+                      const MultiShiftState *M, BParts P, hipStream_t st);
+void launch_b_ms_k1(const BGeo &g, int rows, const double2 *U, const double2 *r, double2 *p0,
+                    double2 *p1, double2 *t, MultiShiftState *M, BParts P, hipStream_t st);
+void launch_b_ms_k2(const BGeo &g, int rows, const double2 *U, const double2 *t,
+                    const double2 *p0, const double2 *p1, double2 *r, MultiShiftState *M,
+                    BParts P, hipStream_t st);
 
 // ---- launches (all asynchronous on `st`) ---------------------------------
 void launch_q_apply_f64(const Latt &L, const double2 *U, const double2 *si, double2 *so,

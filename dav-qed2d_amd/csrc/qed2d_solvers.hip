@@ -417,51 +417,6 @@ void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const flo
 // once zeta_j^2 |r_j|^2 < eps |b|^2.  Bytes per site and iteration (FP64):
 // K1 160 + 128 per active shift, K2 160.
 // ===========================================================================
-struct EpiMS1 {
-    double2 *__restrict__ pnew;
-    double2 *__restrict__ t;
-    const double2 *__restrict__ r;
-    const MultiShiftState *S;
-    const double *zj, *as, *bs;
-    const int *upd;
-    int nshift;
-    long V;
-    double tt, pp;
-    __device__ __forceinline__ void shifts(long i, double2 rv)
-    {
-        for (int k = 0; k < nshift; ++k) {
-            const int u = upd[k];
-            if (u == 0) continue;
-            double2 *ps = S->ps[k];
-            double2 *x = S->x[k];
-            const double2 pv = ps[i];
-            const double a = as[k];
-            const double2 xv = x[i];
-            x[i] = mk<double>(xv.x + a * pv.x, xv.y + a * pv.y);
-            if (u == 2) {
-                const double z = zj[k], b = bs[k];
-                ps[i] = mk<double>(z * rv.x + b * pv.x, z * rv.y + b * pv.y);
-            }
-        }
-    }
-    struct Pre {
-        double2 r1, r2;
-    };
-    __device__ __forceinline__ Pre pre(long s) const { return Pre{r[s], r[V + s]}; }
-    __device__ __forceinline__ void operator()(long s, const Pre &q, double2 o1, double2 o2,
-                                               double2 c1, double2 cc2)
-    {
-        pnew[s] = c1;
-        pnew[V + s] = cc2;
-        t[s] = o1;
-        t[V + s] = o2;
-        tt += (o1.x * o1.x + o1.y * o1.y) + (o2.x * o2.x + o2.y * o2.y);
-        pp += (c1.x * c1.x + c1.y * c1.y) + (cc2.x * cc2.x + cc2.y * cc2.y);
-        shifts(s, q.r1);
-        shifts(V + s, q.r2);
-    }
-};
-
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_ms1(Latt L, const double2 *__restrict__ U,
                                                 const double2 *__restrict__ r, double2 *P0,
@@ -471,134 +426,11 @@ __global__ void __launch_bounds__(kBlock) k_ms1(Latt L, const double2 *__restric
                                                 int nrr, int ntt, int k, int nshift, int bx,
                                                 int by, int nbx)
 {
-    __shared__ double lds[kBlock / kWave + 1];
-    __shared__ double zj_s[kMaxShift], z1_s[kMaxShift], as_s[kMaxShift], bs_s[kMaxShift];
-    __shared__ int upd_s[kMaxShift];
-    __shared__ int anyleft_s;
     const int sk = S->stop_k;
     if (sk != 0 && sk < k) return;
-    const int w = k & 1, rd = (k - 1) & 1;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    const double rr = sum_prev_partials(Prr_rd, nrr, 1, 0, lds);
-    const bool first = (k == 1);
-    double tol, alpha_jm1, beta_j;
-    if (first) {
-        tol = S->eps * rr;
-        alpha_jm1 = 1.0;
-        beta_j = 0.0;
-    } else {
-        const double tt = sum_prev_partials(Ptt_rd, ntt, 2, 0, lds);
-        const double pp = sum_prev_partials(Ptt_rd, ntt, 2, 1, lds);
-        tol = S->tol;
-        alpha_jm1 = S->norm[rd] / (tt + S->sigma_b * pp);
-        beta_j = rr / S->norm[rd];
-    }
-    if (threadIdx.x < nshift) {
-        const int i = threadIdx.x;
-        double zj = 1.0, z1 = 1.0, as = 0.0, bs = 0.0;
-        int act = 1, conv = 0;
-        if (first) {
-            conv = (rr == 0.0);
-        } else {
-            act = S->active[rd][i];
-            if (act) {
-                z1 = S->zeta[rd][i];
-                const double z2 = S->zeta_prev[rd][i];
-                const double alpha_jm2 = S->alpha[rd], beta_jm1 = S->beta[rd];
-                zj = z1 * z2 * alpha_jm2 /
-                     (alpha_jm1 * beta_jm1 * (z2 - z1) + z2 * alpha_jm2 * (1.0 + S->s[i] * alpha_jm1));
-                as = alpha_jm1 * zj / z1;
-                bs = beta_j * (zj / z1) * (zj / z1);
-                conv = zj * zj * rr < tol;
-            }
-        }
-        zj_s[i] = zj;
-        z1_s[i] = z1;
-        as_s[i] = as;
-        bs_s[i] = bs;
-        upd_s[i] = act ? (conv ? 1 : 2) : 0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int any = 0;
-        for (int i = 0; i < nshift; ++i) any |= (upd_s[i] == 2);
-        anyleft_s = any;
-    }
-    __syncthreads();
-    const int anyleft = anyleft_s;
-    const int stat = anyleft ? ((k - 1 >= S->maxiter) ? 2 : 0) : 1;
-    if (p.blk == 0 && threadIdx.x == 0) {
-        S->norm[w] = rr;
-        S->alpha[w] = alpha_jm1;
-        S->beta[w] = beta_j;
-        for (int i = 0; i < nshift; ++i) {
-            S->zeta[w][i] = zj_s[i];
-            S->zeta_prev[w][i] = z1_s[i];
-            S->active[w][i] = (upd_s[i] == 2);
-            if (upd_s[i] == 1) S->conv_iter[i] = k - 1;
-        }
-        S->iter = k - 1;
-        S->err = rr;
-        if (first) {
-            S->tol = tol;
-            S->bnorm = rr;
-        }
-        if (stat) {
-            S->status = stat;
-            S->stop_k = k;
-        }
-    }
-    if (stat == 2) return;
-    if (anyleft) {
-        const double2 *pold = rd ? P1 : P0;
-        double2 *pnew = w ? P1 : P0;
-        EpiMS1 epi{pnew, t, r, S, zj_s, as_s, bs_s, upd_s, nshift, L.V, 0.0, 0.0};
-        if (p.active) {
-            SrcXpay<double> src{r, pold, beta_j, L.V};
-            q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
-        }
-        const double a = block_allsum(epi.tt, lds);
-        const double b = block_allsum(epi.pp, lds);
-        if (threadIdx.x == 0) {
-            Ptt_w[2 * p.blk] = a;
-            Ptt_w[2 * p.blk + 1] = b;
-        }
-    } else if (p.active) {
-        // every shift converged at this step: only the last x updates remain
-        EpiMS1 epi{nullptr, nullptr, r, S, zj_s, as_s, bs_s, upd_s, nshift, L.V, 0.0, 0.0};
-        for (int kk = 0; kk < R; ++kk) {
-            const int row = p.r0 + kk;
-            if (row >= L.L2) break;
-            const long s = (long)row * L.L1 + p.i1;
-            epi.shifts(s, r[s]);
-            epi.shifts(L.V + s, r[L.V + s]);
-        }
-    }
+    ms1_body<R>(L, U, r, P0, P1, t, mass2, S, Ptt_w, Ptt_rd, Prr_rd, nrr, ntt, k, nshift, p);
 }
-
-struct EpiMS2 {
-    double2 *__restrict__ r;
-    const double2 *__restrict__ p;
-    double alpha, sigma;
-    long V;
-    double rr;
-    struct Pre {
-        double2 p1, p2, r1, r2;
-    };
-    __device__ __forceinline__ Pre pre(long s) const { return Pre{p[s], p[V + s], r[s], r[V + s]}; }
-    __device__ __forceinline__ void one(long i, double2 o, double2 pv, double2 rv)
-    {
-        rv = mk<double>(rv.x - alpha * (o.x + sigma * pv.x), rv.y - alpha * (o.y + sigma * pv.y));
-        r[i] = rv;
-        rr += rv.x * rv.x + rv.y * rv.y;
-    }
-    __device__ __forceinline__ void operator()(long s, const Pre &q, double2 o1, double2 o2,
-                                               double2, double2)
-    {
-        one(s, o1, q.p1, q.r1);
-        one(V + s, o2, q.p2, q.r2);
-    }
-};
 
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_ms2(Latt L, const double2 *__restrict__ U,
@@ -608,21 +440,10 @@ __global__ void __launch_bounds__(kBlock) k_ms2(Latt L, const double2 *__restric
                                                 const double *Ptt_w, double *Prr_w, int ntt, int k,
                                                 int bx, int by, int nbx)
 {
-    __shared__ double lds[kBlock / kWave + 1];
     const int sk = S->stop_k;
     if (sk != 0 && sk <= k) return;
-    const int w = k & 1;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    const double tt = sum_prev_partials(Ptt_w, ntt, 2, 0, lds);
-    const double pp = sum_prev_partials(Ptt_w, ntt, 2, 1, lds);
-    const double alpha = S->norm[w] / (tt + S->sigma_b * pp);
-    EpiMS2 epi{r, w ? P1 : P0, alpha, S->sigma_b, L.V, 0.0};
-    if (p.active) {
-        SrcPlain<double> src{t, L.V};
-        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
-    const double a = block_allsum(epi.rr, lds);
-    if (threadIdx.x == 0) Prr_w[p.blk] = a;
+    ms2_body<R>(L, U, t, P0, P1, r, mass2, S, Ptt_w, Prr_w, ntt, k, p);
 }
 
 // r = b, p0 = 0, x_i = 0, p_i = 0, |b|^2 partials.

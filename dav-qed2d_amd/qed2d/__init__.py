@@ -565,9 +565,11 @@ def multishift_CG(xs, U, si, am0, mus, maxiter, eps, prm):
 
 
 def set_multishift(prm, on):
-    """MultiCG / R / RHMC force through multi-shift CG (default) or through the
-    reference's sequential CG solves."""
-    _check(prm, lib.qed2d_ctx_set_multishift(prm.ctx(), 1 if on else 0))
+    """MultiCG / R / RHMC force through multi-shift CG (True or 1: all
+    pseudofermions of the RHMC force in one solve; 2: one pseudofermion at a
+    time) or through the reference's sequential CG solves (False or 0)."""
+    mode = int(on) if not isinstance(on, bool) else (1 if on else 0)
+    _check(prm, lib.qed2d_ctx_set_multishift(prm.ctx(), mode))
 
 
 def MultiCG(so, U, si, am0, maxiter, eps, A, rprm, prm, kprm=None):

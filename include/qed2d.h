@@ -190,9 +190,13 @@ int qed2d_multicg(qed2d_ctx *ctx, double *so, const double *U, const double *si,
  * |r_i|^2 < eps |b|^2. */
 int qed2d_multishift_cg(qed2d_ctx *ctx, double *const *x, const double *U, const double *b,
                         double am0, int n, const double *mu, int maxiter, double eps, int *iters);
-/* 1 (default): MultiCG, R and the RHMC force use qed2d_multishift_cg; 0: the
- * reference's sequential (warm-started) CG solves. */
-int qed2d_ctx_set_multishift(qed2d_ctx *ctx, int on);
+/* How MultiCG, R and the RHMC force solve their shifted systems:
+ * 1 (default): multi-shift CG, and in the RHMC force all pseudofermions'
+ *    solves together (one launch per kernel over the right-hand sides);
+ * 2: multi-shift CG, one pseudofermion after another;
+ * 0: the reference's sequential (warm-started) CG solves.
+ * Modes 1 and 2 give the same bits. */
+int qed2d_ctx_set_multishift(qed2d_ctx *ctx, int mode);
 /* R(so, U, si, am0, maxiter, eps, A, rprm, prm, kprm) (src/QED2dRHMC.jl:261-271) */
 int qed2d_R(qed2d_ctx *ctx, double *so, const double *U, const double *si, double am0, int n,
             const double *mu, const double *rho, double A, double r_b, int maxiter, double eps);

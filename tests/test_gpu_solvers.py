@@ -385,3 +385,30 @@ def test_autotune_then_parity(Q, O):
         assert relerr(host(x), ref) < 1e-9
     Q.set_tiles(prm, 2, 0, 4, 0)
     assert Q.get_tiles(prm)["q64"] == 2 and Q.get_tiles(prm)["cg64"] == 4
+
+
+@pytest.mark.parametrize("L1,L2,nf", [(16, 16, 4), (48, 32, 3)])
+def test_rhmc_fused_pseudofermions_bit_exact(Q, O, L1, L2, nf):
+    """The RHMC force with every pseudofermion's multi-shift solve in one
+    launch per kernel (mode 1) against one pseudofermion at a time (mode 2):
+    identical bits (energies, accept/reject, links), with different masses and
+    pole counts per pseudofermion."""
+    import torch
+    say("RHMC fused pseudofermions", L1, L2, nf)
+    prm = Q.LattParm((L1, L2), 4.0)
+    U0 = Q.spinor(prm)
+    Q.hot_start(U0, prm, 21)
+    am0 = [0.1, 0.2, 0.15, 0.1][:nf]
+    rps = [Q.get_rhmc_params(n, np.sqrt(0.02), np.sqrt(70.0)) for n in (16, 8, 12, 16)[:nf]]
+    out = {}
+    for mode in (1, 2):
+        U = U0.clone()
+        Q.set_multishift(prm, mode)
+        Q.seed(33)
+        infos = [Q.HMC(U, am0, 0.1, 3, [], 100000, 1e-22, prm, None, rps, integrator=Q.OMF4())
+                 for _ in range(2)]
+        out[mode] = (infos, U)
+    Q.set_multishift(prm, 1)
+    for a, b in zip(out[1][0], out[2][0]):
+        assert a.hini == b.hini and a.hfin == b.hfin and a.accepted == b.accepted
+    assert torch.equal(out[1][1], out[2][1])
