@@ -40,7 +40,7 @@ struct qed2d_ctx {
     // defaults come from default_tiles(), qed2d_ctx_autotune measures them.
     int rows = 16;    // FP64 solver kernels (CG, multi-shift, residuals)
     int rows32 = 8;   // FP32 solver kernels (mixed-precision CG)
-    int rows_q = 4;   // FP64 Q / Dw applies
+    int rows_q = 2;   // FP64 Q / Dw applies
     int rows_q32 = 2; // FP32 Q apply of the public entry point
     double ru_delta = 0.01;  // mixed CG: reliable update after |r| drops by this factor
     double last_dh = 0.0;    // dH of the last trajectory, from double-double energies
@@ -1070,7 +1070,7 @@ bool tile_ok(int r) { return r == 1 || r == 2 || r == 4 || r == 8 || r == 16; }
 // lattices that fit the 256 MB Infinity Cache and medium ones above.
 void default_tiles(qed2d_ctx *c)
 {
-    c->rows_q = 4;
+    c->rows_q = 2;
     c->rows_q32 = 2;
     c->rows = (c->L.V <= (1L << 21)) ? 16 : 8;
     c->rows32 = 8;

@@ -87,10 +87,44 @@ template <typename T, bool NT> __device__ __forceinline__ void st(c2<T> *p, long
     }
 }
 
+// Values of the neighbouring lanes (DPP wavefront shifts): from_left(v) is
+// lane i-1's v at lane i (wave_shr:1), from_right(v) lane i+1's (wave_shl:1).
+// A wave covers 64 consecutive sites along i1, so these are the i1 -/+ 1
+// neighbours except at the wave's edges and at the periodic wrap, where the
+// march loads them instead.  Every lane executes the shift; lanes whose
+// source is out of range receive 0 and load.
+__device__ __forceinline__ float lane_shift(float v, bool from_left)
+{
+    const int i = __float_as_int(v);
+    const int r = from_left ? __builtin_amdgcn_update_dpp(0, i, 0x138, 0xF, 0xF, false)
+                            : __builtin_amdgcn_update_dpp(0, i, 0x130, 0xF, 0xF, false);
+    return __int_as_float(r);
+}
+__device__ __forceinline__ double lane_shift(double v, bool from_left)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+    const int rlo = from_left ? __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xF, 0xF, false)
+                              : __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xF, 0xF, false);
+    const int rhi = from_left ? __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xF, 0xF, false)
+                              : __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)rhi << 32) | (unsigned int)rlo);
+}
+template <typename T> __device__ __forceinline__ c2<T> from_left(c2<T> v)
+{
+    return mk<T>(lane_shift(v.x, true), lane_shift(v.y, true));
+}
+template <typename T> __device__ __forceinline__ c2<T> from_right(c2<T> v)
+{
+    return mk<T>(lane_shift(v.x, false), lane_shift(v.y, false));
+}
+
 // March one thread's column over rows r0 .. r0+R-1: compute (Q s)(x) and hand
 // it to the epilogue together with s(x).  The operands of row r+1 (the spinor
-// of row r+2, the row's i1 neighbours and links, the epilogue's pre-loads) are
-// in flight while row r is computed.
+// of row r+2, the row's links, the epilogue's pre-loads, and the i1
+// neighbours of the wave's edge lanes) are in flight while row r is computed;
+// the i1 neighbours and U(x-1,1) of the other lanes come from the adjacent
+// lanes' registers.
 template <typename T, int R, typename Src, typename Epi>
 __device__ __forceinline__ void q_march(const Latt &L, const c2<T> *__restrict__ U, T mass2,
                                         const Src &src, Epi &epi, int i1, int r0)
@@ -101,6 +135,9 @@ __device__ __forceinline__ void q_march(const Latt &L, const c2<T> *__restrict__
     const long V = L.V;
     const int i1p = (i1 + 1 == L1) ? 0 : i1 + 1;
     const int i1m = (i1 == 0) ? L1 - 1 : i1 - 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool ldR = (lane == kWave - 1) || (i1p == 0);  // i1+1 not in lane+1
+    const bool ldL = (lane == 0) || (i1 == 0);           // i1-1 not in lane-1
     const bool ap = (L.bc & 1) != 0;
     const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
     const int nr = (r0 + R <= L2) ? R : L2 - r0;
@@ -112,35 +149,52 @@ __device__ __forceinline__ void q_march(const Latt &L, const c2<T> *__restrict__
     // operands of row r0
     long row = (long)r0 * L1;
     Raw wu = src.fetch((long)((r0 + 1 == L2) ? 0 : r0 + 1) * L1 + i1);
-    Raw wp = src.fetch(row + i1p), wm = src.fetch(row + i1m);
-    c2<T> U1 = U[row + i1], U1m = U[row + i1m], U2 = U[V + row + i1];
+    Raw wp{}, wm{};
+    c2<T> U1m{};
+    if (ldR) wp = src.fetch(row + i1p);
+    if (ldL) {
+        wm = src.fetch(row + i1m);
+        U1m = U[row + i1m];
+    }
+    c2<T> U1 = U[row + i1], U2 = U[V + row + i1];
     Pre q = epi.pre(row + i1);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         if (k >= nr) break;
         const int r = r0 + k;
         const bool more = (k + 1 < nr);
-        Raw nwu, nwp, nwm;
-        c2<T> nU1, nU1m, nU2;
+        Raw nwu, nwp{}, nwm{};
+        c2<T> nU1, nU1m{}, nU2;
         Pre nq;
         if (more) {
             const long row1 = (long)(r + 1) * L1;
             const int r2 = (r + 2 >= L2) ? r + 2 - L2 : r + 2;
             nwu = src.fetch((long)r2 * L1 + i1);
-            nwp = src.fetch(row1 + i1p);
-            nwm = src.fetch(row1 + i1m);
+            if (ldR) nwp = src.fetch(row1 + i1p);
+            if (ldL) {
+                nwm = src.fetch(row1 + i1m);
+                nU1m = U[row1 + i1m];
+            }
             nU1 = U[row1 + i1];
-            nU1m = U[row1 + i1m];
             nU2 = U[V + row1 + i1];
             nq = epi.pre(row1 + i1);
         }
         c2<T> up1, up2, p11, p12, m11, m12;
         src.value(wu, up1, up2);
-        src.value(wp, p11, p12);
-        src.value(wm, m11, m12);
+        // neighbours along i1 from the adjacent lanes (every lane shifts)
+        p11 = from_right<T>(c1);
+        p12 = from_right<T>(cc2);
+        m11 = from_left<T>(c1);
+        m12 = from_left<T>(cc2);
+        c2<T> U1l = from_left<T>(U1);
+        if (ldR) src.value(wp, p11, p12);
+        if (ldL) {
+            src.value(wm, m11, m12);
+            U1l = U1m;
+        }
         const c2<T> U2e = (ap && r == L2 - 1) ? cneg<T>(U2) : U2;
         c2<T> o1, o2;
-        q_site<T>(mass2, c1, cc2, p11, p12, m11, m12, up1, up2, dn1, dn2, U1, U1m, U2e, U2m, o1,
+        q_site<T>(mass2, c1, cc2, p11, p12, m11, m12, up1, up2, dn1, dn2, U1, U1l, U2e, U2m, o1,
                   o2);
         epi((long)r * L1 + i1, q, o1, o2, c1, cc2);
         dn1 = c1;
@@ -272,10 +326,11 @@ struct SrcXpay2 {
 };
 
 // March the column pair (2j, 2j+1) over rows r0 .. r0+R-1, software pipelined
-// like q_march; the epilogue gets both sites' outputs and inputs:
-// epi.pair(s, pre, oa1, oa2, ob1, ob2, c1, c2) with s the index of site 2j,
-// (oa, ob) the outputs of sites 2j, 2j+1 for spin components 1, 2 and c1, c2
-// the inputs.
+// like q_march, with the neighbours 2j-1 and 2j+2 (and U(2j-1, 1)) from the
+// adjacent lanes away from the wave's edges; the epilogue gets both sites'
+// outputs and inputs: epi.pair(s, pre, oa1, oa2, ob1, ob2, c1, c2) with s the
+// index of site 2j, (oa, ob) the outputs of sites 2j, 2j+1 for spin
+// components 1, 2 and c1, c2 the inputs.
 template <int R, typename Src, typename Epi>
 __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict__ U, float mass2,
                                          const Src &src, Epi &epi, int j, int r0)
@@ -288,6 +343,9 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
     const int ia = 2 * j;
     const int im = (ia == 0) ? L1 - 1 : ia - 1;
     const int ip = (ia + 2 == L1) ? 0 : ia + 2;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool ldR = (lane == kWave - 1) || (ip == 0);
+    const bool ldL = (lane == 0) || (ia == 0);
     const bool ap = (L.bc & 1) != 0;
     const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
     const int nr = (r0 + R <= L2) ? R : L2 - r0;
@@ -298,9 +356,14 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
     if (ap && r0 == 0) U2m = neg2(U2m);
     long row = (long)r0 * L1;
     Raw2 wu = src.fetch2((long)((r0 + 1 == L2) ? 0 : r0 + 1) * L1 + ia);
-    Raw1 wl = src.fetch1(row + im), wq = src.fetch1(row + ip);
+    Raw1 wl{}, wq{};
+    float2 U1lm{};
+    if (ldL) {
+        wl = src.fetch1(row + im);
+        U1lm = U[row + im];
+    }
+    if (ldR) wq = src.fetch1(row + ip);
     F2x2 U1 = ld2(U, row + ia), U2 = ld2(U, V + row + ia);
-    float2 U1l = U[row + im];
     Pre q = epi.pre(row + ia);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -308,26 +371,37 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
         const int r = r0 + k;
         const bool more = (k + 1 < nr);
         Raw2 nwu;
-        Raw1 nwl, nwq;
+        Raw1 nwl{}, nwq{};
         F2x2 nU1, nU2;
-        float2 nU1l;
+        float2 nU1lm{};
         Pre nq;
         if (more) {
             const long row1 = (long)(r + 1) * L1;
             const int r2 = (r + 2 >= L2) ? r + 2 - L2 : r + 2;
             nwu = src.fetch2((long)r2 * L1 + ia);
-            nwl = src.fetch1(row1 + im);
-            nwq = src.fetch1(row1 + ip);
+            if (ldL) {
+                nwl = src.fetch1(row1 + im);
+                nU1lm = U[row1 + im];
+            }
+            if (ldR) nwq = src.fetch1(row1 + ip);
             nU1 = ld2(U, row1 + ia);
             nU2 = ld2(U, V + row1 + ia);
-            nU1l = U[row1 + im];
             nq = epi.pre(row1 + ia);
         }
         F2x2 up1, up2;
         float2 l1, l2, q1, q2;
         src.value2(wu, up1, up2);
-        src.value1(wl, l1, l2);
-        src.value1(wq, q1, q2);
+        // neighbours 2j-1 (lane-1's second site) and 2j+2 (lane+1's first site)
+        l1 = from_left<float>(c1.a1);
+        l2 = from_left<float>(cc2.a1);
+        q1 = from_right<float>(c1.a0);
+        q2 = from_right<float>(cc2.a0);
+        float2 U1l = from_left<float>(U1.a1);
+        if (ldL) {
+            src.value1(wl, l1, l2);
+            U1l = U1lm;
+        }
+        if (ldR) src.value1(wq, q1, q2);
         const F2x2 U2e = (ap && r == L2 - 1) ? neg2(U2) : U2;
         float2 oa1, oa2, ob1, ob2;
         // site 2j: x+1 = 2j+1, x-1 = the left neighbour
@@ -348,7 +422,7 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
             wq = nwq;
             U1 = nU1;
             U2 = nU2;
-            U1l = nU1l;
+            U1lm = nU1lm;
             q = nq;
         }
     }
