@@ -77,7 +77,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+        print("bench.py runs on one GPU only: this package has no multi-GPU path "
               "(use --gpus 1).", file=sys.stderr)
         sys.exit(2)
 

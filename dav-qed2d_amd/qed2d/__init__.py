@@ -624,7 +624,7 @@ def reweighting_factor(U, am0, CGmaxiter, CGtol, prm, kprm, rprm):
 
 def power_method(U, am0, prm, kprm=None, iter=1000, b0=None):
     """power_method (src/QED2dRHMC.jl:135-170).  The starting vector is drawn
-    from the build's own stream (seed(), then flavour 15) unless given."""
+    from this package's own stream (seed(), then flavour 15) unless given."""
     if b0 is None:
         b0 = spinor(prm)
         _check(prm, lib.qed2d_fill_pseudofermion(prm.ctx(), _sp(prm, b0), _rng.seed, _rng.chain,
@@ -654,7 +654,7 @@ _rng = _Stream()
 
 
 def seed(s, chain=0, traj=0):
-    """Counterpart of Random.seed!: sets the build's counter-based stream."""
+    """Counterpart of Random.seed!: sets this package's counter-based stream."""
     _rng.seed = int(s)
     _rng.chain = int(chain)
     _rng.traj = int(traj)

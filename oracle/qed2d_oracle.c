@@ -23,7 +23,7 @@ typedef double _Complex cplx;
 /* ========================================================================
  * Random numbers.  The reference draws from CURAND and Julia's Mersenne
  * Twister (src/QED2dHMC.jl:25,42,69-70,95); those streams are not
- * reproducible here, so the build defines its own counter-based streams:
+ * reproducible here, so this restatement defines counter-based streams:
  * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) + Box-Muller, with
  * log and sin/cos evaluated by fixed FMA sequences so that host and device
  * produce identical bits.

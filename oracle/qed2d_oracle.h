@@ -1,8 +1,8 @@
 /*
  * qed2d_oracle.h -- FP64 CPU restatement of the QED2d molecular-dynamics path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP library
- * (libqed2d_hip) is compared against.  It is loaded by tests/, by the smoke
+ * TEST INFRASTRUCTURE ONLY.  This library is the FP64 restatement that the HIP
+ * library (libqed2d_hip) is compared against.  It is loaded by tests/, by the smoke
  * check in __graft_entry__.py and by the cpu_baseline leg of bench.py, and by
  * nothing else.  No product code path may call it.
  *
@@ -41,7 +41,7 @@ extern "C" {
 /* Fermion boundary flags: bit 0 set = antiperiodic in direction 2. */
 #define ORC_BC_APERIODIC_2 1
 
-/* ---- random numbers (build's own counter-based streams) ---------------- */
+/* ---- random numbers (counter-based Philox streams) ---------------- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double orc_det_log(double u);
 void orc_det_sincos2pi(double u, double *s, double *c);

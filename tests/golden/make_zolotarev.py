@@ -21,7 +21,7 @@ CASES = [
     (5, math.sqrt(0.1), math.sqrt(26.0)),
     (2, math.sqrt(0.3), math.sqrt(7.5)),
     (10, math.sqrt(0.02), math.sqrt(12.0)),
-    (16, math.sqrt(0.005), math.sqrt(17.0)),  # 16 poles on a config-3-like interval
+    (16, math.sqrt(0.005), math.sqrt(17.0)),  # 16 poles on [0.005, 17], the RHMC spectrum scale
     (8, math.sqrt(1.0e-4), math.sqrt(8.0)),
 ]
 

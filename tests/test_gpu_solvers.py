@@ -216,7 +216,7 @@ def test_hmc_pure_sequence(Q, O):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_hmc_nf2_sequence(Q, O, prec):
-    """Config-1 physics at 24^2 (beta 4, m0 0.2, leapfrog ns=10 eps=0.1) with a
+    """Nf=2 at 24^2 (beta 4, m0 0.2, leapfrog ns=10 eps=0.1) with a
     tight CG tolerance, so that FP64 and mixed MD forces agree with the oracle
     to the level the energies are compared at."""
     say("HMC nf2 prec", prec)

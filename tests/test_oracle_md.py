@@ -135,7 +135,7 @@ def test_nf2_pseudofermion_action_at_start(O):
 
 
 def test_hmc_nf2_driver_accounting(O):
-    """hmc_nf2 reproduces hini = H + |X|^2 with the build's random streams and
+    """hmc_nf2 reproduces hini = H + |X|^2 with the Philox random streams and
     accepts / rejects on the Metropolis uniform of the same trajectory."""
     L = 8
     p = O.latt(L, L, 4.0)

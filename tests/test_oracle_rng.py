@@ -1,5 +1,5 @@
-"""Pins the build's random streams.  The reference's CURAND / Mersenne Twister
-streams (src/QED2dHMC.jl:25,42,69-70,95) cannot be reproduced; the build uses
+"""Pins the Philox random streams.  The reference's CURAND / Mersenne Twister
+streams (src/QED2dHMC.jl:25,42,69-70,95) cannot be reproduced; the library uses
 Philox4x32-10 (Salmon et al., SC'11), checked against the Random123
 known-answer vectors, and Box-Muller with fixed-FMA log / sincos, checked
 against libm and by distribution tests."""
