@@ -389,11 +389,10 @@ def config_measurements(Q, torch, no_cpu):
     Q.seed(1234)
     for _ in range(200):
         Q.HMC(U, 0.1, 10, [], prm, None)
-    bs = []
+    si = torch.zeros((16, 2, L, L), dtype=torch.complex128, device="cuda")
     for j in range(16):
-        b = Q.spinor(prm)
-        Q.fill_pseudofermion(b, prm, 1234, 5, 0, j)
-        bs.append(b)
+        Q.fill_pseudofermion(si[j], prm, 1234, 5, 0, j)
+    bs = [si[j] for j in range(16)]
     c2 = {}
     for prec, key in ((Q.PREC_F64, "fp64"), (Q.PREC_MIXED, "mixed")):
         xs = [Q.spinor(prm) for _ in range(16)]
@@ -401,9 +400,19 @@ def config_measurements(Q, torch, no_cpu):
         dt, its = sync_time(lambda: [Q.CG(x, U, b, 0.1, 100000, 1e-12, Q.gamm5Dw_sqr_msq, prm,
                                           None, precision=prec) for x, b in zip(xs, bs)])
         bps = 384 if prec == Q.PREC_F64 else 192
+        # the 16 solves as one set (CG_mrhs: one launch per kernel, shared links)
+        so = torch.zeros_like(si)
+        Q.CG_mrhs(so, U, si, 0.1, 100000, 1e-12, prm, None, precision=prec)
+        dts, itm = sync_time(lambda: Q.CG_mrhs(so, U, si, 0.1, 100000, 1e-12, prm, None,
+                                               precision=prec))
         c2[key] = {"seconds_16_rhs": round(dt, 4), "iterations": its,
-                   "GBs_cg_kernels": round(bps * L * L * sum(its) / dt / 1e9, 1)}
+                   "GBs_cg_kernels": round(bps * L * L * sum(its) / dt / 1e9, 1),
+                   "seconds_16_rhs_one_set": round(dts, 4),
+                   "one_set_equals_sequential": bool(itm == its and all(
+                       torch.equal(so[j], xs[j]) for j in range(16)))}
     c2["mixed_vs_fp64"] = round(c2["fp64"]["seconds_16_rhs"] / c2["mixed"]["seconds_16_rhs"], 3)
+    c2["mixed_vs_fp64_one_set"] = round(c2["fp64"]["seconds_16_rhs_one_set"] /
+                                        c2["mixed"]["seconds_16_rhs_one_set"], 3)
     if O is not None:
         Uh, bh = U.cpu().numpy(), bs[0].cpu().numpy()
         t0 = time.perf_counter()
@@ -411,7 +420,7 @@ def config_measurements(Q, torch, no_cpu):
         c2["cpu_seconds_per_rhs"] = round(time.perf_counter() - t0, 3)
         c2["cpu_iterations"] = it
     out["cg_16rhs_1024x1024"] = c2
-    del bs, U
+    del bs, si, U
     prm.close()
 
     # RHMC 128^2, beta 4, m0 0.1, 16 poles on power-method bounds, 4

@@ -66,6 +66,7 @@ struct qed2d_ctx {
         double2 *arena = nullptr;  // r, p0, p1, t of every right-hand side
         std::vector<std::vector<double2 *>> x, ps;  // per right-hand side and shift
     } mr;
+    qed2d_batch *rhs = nullptr;  // qed2d_cg_mrhs: right-hand sides sharing the links
     int multishift = 1;               // MultiCG / RHMC force through multi-shift CG
     MultiShiftState *ms = nullptr;    // device
     MultiShiftState *hms = nullptr;   // pinned
@@ -709,6 +710,7 @@ int multishift_multi(qed2d_ctx *c, const double *U, int nf, const double *const 
     g.shape = m.shape;
     g.n = nf;
     g.nblocks = nf * march_grid(c->L, c->rows).nblocks();
+    g.inter = 1;  // one lattice shape, shared links
     double2 *r = m.arena, *p0 = r + n2 * nf, *p1 = p0 + n2 * nf, *t = p1 + n2 * nf;
     launch_b_ms_init(g, c->rows, r, p0, m.M, m.P, st);
     std::vector<ChunkPredictor> pred(nf);
@@ -1239,6 +1241,7 @@ void qed2d_ctx_destroy(qed2d_ctx *c)
     for (auto p : c->flav) if (p) (void)hipFree(p);
     for (auto p : c->shiftbuf) if (p) (void)hipFree(p);
     for (auto p : c->psbuf) if (p) (void)hipFree(p);
+    if (c->rhs) qed2d_batch_destroy(c->rhs);
     for (auto &v : c->mr.x) for (auto p : v) if (p) (void)hipFree(p);
     for (auto &v : c->mr.ps) for (auto p : v) if (p) (void)hipFree(p);
     for (void *p : {(void *)c->mr.dch, (void *)c->mr.blk0, (void *)c->mr.shape, (void *)c->mr.M,
@@ -1627,6 +1630,41 @@ int qed2d_cg(qed2d_ctx *c, double *so, const double *U, const double *si, double
         return fail(c, QED2D_EINVAL, "unknown precision %d", precision);
     if (maxiter < 1) return fail(c, QED2D_EINVAL, "maxiter must be >= 1");
     return cg_any(c, so, U, si, am0, mu, op, maxiter, eps, so_as_guess, precision, iters);
+}
+
+int qed2d_cg_mrhs(qed2d_ctx *c, double *x, const double *U, const double *b, int nrhs,
+                  double am0, int maxiter, double eps, int precision, int *iters)
+{
+    CHECK_CTX(c);
+    NEED(c, x); NEED(c, U); NEED(c, b);
+    NOALIAS(c, x, b);
+    if (nrhs < 1) return fail(c, QED2D_EINVAL, "nrhs must be >= 1");
+    if (precision != QED2D_PREC_F64 && precision != QED2D_PREC_MIXED)
+        return fail(c, QED2D_EINVAL, "unknown precision %d", precision);
+    if (maxiter < 1) return fail(c, QED2D_EINVAL, "maxiter must be >= 1");
+    const long V2 = spinor_elems(c) * 2;  // doubles per field
+    if (precision == QED2D_PREC_MIXED && (c->L.L1 % 2 != 0 || c->L.L1 < 4)) {
+        // the set's FP32 kernels march site pairs: one solve at a time
+        for (int j = 0; j < nrhs; ++j)
+            TRY(cg_any(c, x + j * V2, U, b + j * V2, am0, 0.0, QED2D_OP_SQR_MSQ, maxiter, eps, 0,
+                       precision, iters ? iters + j : nullptr));
+        return finish(c);
+    }
+    int n = 0;
+    long sites = 0;
+    if (c->rhs) (void)qed2d_batch_size(c->rhs, &n, &sites);
+    if (n != nrhs) {
+        if (c->rhs) qed2d_batch_destroy(c->rhs);
+        c->rhs = nullptr;
+        const int rc = qed2d_batch_create_rhs(&c->rhs, c->L.L1, c->L.L2, nrhs, c->beta, c->L.bc,
+                                              (void *)c->stream);
+        if (rc) return fail(c, rc, "cannot create a set of %d right-hand sides", nrhs);
+    }
+    std::string err;
+    const int rc = batch_rhs_cg(c->rhs, U, x, b, am0, maxiter, eps, precision, iters, c->rows_q,
+                                c->rows, c->rows32, c->ru_delta, &err);
+    if (rc) return fail(c, rc, "%s", err.c_str());
+    return finish(c);
 }
 
 int qed2d_multicg(qed2d_ctx *c, double *so, const double *U, const double *si, double am0,

@@ -32,15 +32,21 @@ template <bool PAIR>
 __device__ __forceinline__ BWhere bwhere(const BGeo &g, int R)
 {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
-    int lo = 0, hi = g.n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (g.blk0[mid] <= b) lo = mid;
-        else hi = mid - 1;
+    int lo = 0, local;
+    if (g.inter) {
+        lo = b % g.n;
+        local = b / g.n;
+    } else {
+        int hi = g.n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (g.blk0[mid] <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        local = b - g.blk0[lo];
     }
     const BChain &ch = g.ch[lo];
     const int4 sh = g.shape[lo];
-    const int local = b - g.blk0[lo];
     BWhere w;
     w.c = lo;
     w.L = ch.L;

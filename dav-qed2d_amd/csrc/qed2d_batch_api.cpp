@@ -37,6 +37,8 @@ struct qed2d_batch {
     int n = 0;
     long vtot = 0;
     int bc = 0;
+    bool shared = false;  // right-hand sides of one lattice sharing the caller's links
+    long nU = 0;          // elements of the link field(s) U points to
     std::vector<BChain> hch;
     BChain *dch = nullptr;
     std::vector<qed2d_ctx *> sub;  // one context per chain: its reductions
@@ -147,6 +149,7 @@ int get_geo(qed2d_batch *b, int rows, bool pair, BGeo *out)
     out->shape = it->second.shape;
     out->n = b->n;
     out->nblocks = it->second.nblocks;
+    out->inter = b->shared ? 1 : 0;
     return QED2D_OK;
 }
 
@@ -219,14 +222,14 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             if (!pair_ok(b->hch[c].L))
                 return bfail(b, QED2D_EINVAL, "mixed-precision batches need even L1 >= 4 "
                              "(chain %d has L1 = %d)", c, b->hch[c].L.L1);
-        BTRY(balloc(b, &b->Us, nel));
+        BTRY(balloc(b, &b->Us, b->nU));
         BTRY(balloc(b, &b->rs, nel));
         BTRY(balloc(b, &b->p0s, nel));
         BTRY(balloc(b, &b->p1s, nel));
         BTRY(balloc(b, &b->ts, nel));
         BTRY(balloc(b, &b->xs, nel));
         BTRY(get_geo(b, b->rows32, true, &g32));
-        launch_to_f32(b->U, b->Us, nel, st);
+        launch_to_f32(b->U, b->Us, b->nU, st);
         launch_b_cg_init(g64, b->rows, src, x, b->rs, b->p0s, b->xs, 1, b->P, st);
     } else {
         launch_b_cg_init(g64, b->rows, src, x, b->r, b->p0, nullptr, 0, b->P, st);
@@ -382,6 +385,7 @@ int qed2d_batch_create(qed2d_batch **out, int nchain, const int *L1, const int *
         pb += march_grid(ch.L, 1).nblocks();
     }
     b->vtot = off;
+    b->nU = 2 * off;
     b->last_dh.assign(nchain, 0.0);
     const long nel = 2 * off;
     bool ok = hipMalloc((void **)&b->dch, sizeof(BChain) * nchain) == hipSuccess &&
@@ -433,6 +437,7 @@ void qed2d_batch_destroy(qed2d_batch *b)
         (void)hipFree(kv.second.blk0);
         (void)hipFree(kv.second.shape);
     }
+    if (b->shared) b->U = nullptr;  // the caller's links
     void *dev[] = {b->dch, b->U, b->Ucp, b->X, b->F, b->Y, b->r, b->p0, b->p1, b->t, b->Us,
                    b->rs, b->p0s, b->p1s, b->ts, b->xs, b->mom, b->S, b->mask, b->part, b->cnt};
     for (void *p : dev)
@@ -465,6 +470,7 @@ int qed2d_batch_offset(const qed2d_batch *b, int chain, long *site_offset)
 int qed2d_batch_links(qed2d_batch *b, int chain, double **U)
 {
     if (!b || !U) return QED2D_EINVAL;
+    if (b->shared) return bfail(b, QED2D_EINVAL, "a right-hand-side set has no links of its own");
     if (chain < 0 || chain >= b->n) return bfail(b, QED2D_EINVAL, "no chain %d", chain);
     *U = (double *)(b->U + 2 * b->hch[chain].off);
     return QED2D_OK;
@@ -473,6 +479,7 @@ int qed2d_batch_links(qed2d_batch *b, int chain, double **U)
 int qed2d_batch_get_links(qed2d_batch *b, int chain, double *dst)
 {
     if (!b || !dst) return QED2D_EINVAL;
+    if (b->shared) return bfail(b, QED2D_EINVAL, "a right-hand-side set has no links of its own");
     if (chain < 0 || chain >= b->n) return bfail(b, QED2D_EINVAL, "no chain %d", chain);
     const BChain &ch = b->hch[chain];
     BHIP(b, hipMemcpyAsync(dst, b->U + 2 * ch.off, sizeof(double2) * 2 * ch.L.V, hipMemcpyDefault,
@@ -484,6 +491,7 @@ int qed2d_batch_get_links(qed2d_batch *b, int chain, double *dst)
 int qed2d_batch_set_links(qed2d_batch *b, int chain, const double *src)
 {
     if (!b || !src) return QED2D_EINVAL;
+    if (b->shared) return bfail(b, QED2D_EINVAL, "a right-hand-side set has no links of its own");
     if (chain < 0 || chain >= b->n) return bfail(b, QED2D_EINVAL, "no chain %d", chain);
     const BChain &ch = b->hch[chain];
     BHIP(b, hipMemcpyAsync(b->U + 2 * ch.off, src, sizeof(double2) * 2 * ch.L.V, hipMemcpyDefault,
@@ -507,6 +515,7 @@ int qed2d_batch_set_tiles(qed2d_batch *b, int q64, int cg64, int cg32)
 int qed2d_batch_hot_start(qed2d_batch *b, const uint64_t *seed)
 {
     if (!b || !seed) return QED2D_EINVAL;
+    if (b->shared) return bfail(b, QED2D_EINVAL, "a right-hand-side set has no links of its own");
     for (int c = 0; c < b->n; ++c) {
         uint32_t k0, k1;
         key_of(seed[c], (uint32_t)c, &k0, &k1);
@@ -517,14 +526,80 @@ int qed2d_batch_hot_start(qed2d_batch *b, const uint64_t *seed)
     return QED2D_OK;
 }
 
-int qed2d_batch_cg(qed2d_batch *b, double *x, const double *src, const double *am0,
-                   int maxiter, double tol, int precision, int *iters)
+int qed2d_batch_create_rhs(qed2d_batch **out, int L1, int L2, int nrhs, double beta,
+                           int bc_flags, void *stream)
 {
-    if (!b) return QED2D_EINVAL;
+    if (!out) return QED2D_EINVAL;
+    *out = nullptr;
+    if (nrhs < 1 || L1 < 2 || L2 < 2 || (long)L1 * L2 * nrhs > (1L << 33)) return QED2D_EINVAL;
+    if (bc_flags & ~QED2D_BC_APERIODIC_T) return QED2D_EINVAL;
+    qed2d_batch *b = new qed2d_batch();
+    b->stream = (hipStream_t)stream;
+    b->n = nrhs;
+    b->bc = bc_flags;
+    b->shared = true;
+    b->hch.resize(nrhs);
+    Latt L;
+    L.L1 = L1;
+    L.L2 = L2;
+    L.V = (long)L1 * L2;
+    L.bc = bc_flags;
+    const long nbc = march_grid(L, 1).nblocks();
+    for (int c = 0; c < nrhs; ++c) {
+        BChain &ch = b->hch[c];
+        ch.L = L;
+        ch.off = c * L.V;
+        ch.uoff = 0;
+        ch.pb = (int)(c * nbc);
+        ch.pad = 0;
+        ch.am0 = 0.0;
+        ch.beta = beta;
+    }
+    b->vtot = nrhs * L.V;
+    b->nU = 2 * L.V;
+    b->last_dh.assign(nrhs, 0.0);
+    const long pb = nbc * nrhs;
+    bool ok = hipMalloc((void **)&b->dch, sizeof(BChain) * nrhs) == hipSuccess &&
+              hipMalloc((void **)&b->S, sizeof(SolverState) * nrhs) == hipSuccess &&
+              hipHostMalloc((void **)&b->hS, sizeof(SolverState) * nrhs) == hipSuccess &&
+              hipMalloc((void **)&b->mask, sizeof(int) * nrhs) == hipSuccess &&
+              hipHostMalloc((void **)&b->hmask, sizeof(int) * nrhs) == hipSuccess &&
+              hipMalloc((void **)&b->part, sizeof(double) * 7 * pb) == hipSuccess &&
+              hipMalloc((void **)&b->cnt, sizeof(int) * 5 * nrhs) == hipSuccess;
+    if (ok)
+        ok = hipMemcpy(b->dch, b->hch.data(), sizeof(BChain) * nrhs, hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             hipMemset(b->cnt, 0, sizeof(int) * 5 * nrhs) == hipSuccess;
+    if (!ok) {
+        qed2d_batch_destroy(b);
+        return QED2D_ENOMEM;
+    }
+    b->P.tt[0] = b->part;
+    b->P.tt[1] = b->part + 2 * pb;
+    b->P.rr[0] = b->part + 4 * pb;
+    b->P.rr[1] = b->part + 5 * pb;
+    b->P.b = b->part + 6 * pb;
+    b->P.ntt[0] = b->cnt;
+    b->P.ntt[1] = b->cnt + nrhs;
+    b->P.nrr[0] = b->cnt + 2 * nrhs;
+    b->P.nrr[1] = b->cnt + 3 * nrhs;
+    b->P.nb = b->cnt + 4 * nrhs;
+    *out = b;
+    return QED2D_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int cg_entry(qed2d_batch *b, double *x, const double *src, const double *am0, int maxiter,
+             double tol, int precision, int *iters)
+{
     if (!x || !src || !am0) return bfail(b, QED2D_EINVAL, "null pointer");
     if (x == src) return bfail(b, QED2D_EINVAL, "x aliases src");
     if (precision != QED2D_PREC_F64 && precision != QED2D_PREC_MIXED)
         return bfail(b, QED2D_EINVAL, "unknown precision %d", precision);
+    if (maxiter < 1) return bfail(b, QED2D_EINVAL, "maxiter must be >= 1");
     BTRY(set_masses(b, am0));
     std::vector<int> it;
     const int rc = batch_cg(b, (double2 *)x, (const double2 *)src, maxiter, tol, precision, it);
@@ -534,12 +609,59 @@ int qed2d_batch_cg(qed2d_batch *b, double *x, const double *src, const double *a
     return rc;
 }
 
+}  // namespace
+
+extern "C" {
+
+int qed2d_batch_cg_shared(qed2d_batch *b, const double *U, double *x, const double *src,
+                          const double *am0, int maxiter, double tol, int precision, int *iters)
+{
+    if (!b) return QED2D_EINVAL;
+    if (!b->shared) return bfail(b, QED2D_EINVAL, "not a right-hand-side set");
+    if (!U) return bfail(b, QED2D_EINVAL, "null links");
+    b->U = (double2 *)U;
+    const int rc = cg_entry(b, x, src, am0, maxiter, tol, precision, iters);
+    b->U = nullptr;
+    return rc;
+}
+
+int qed2d_batch_cg(qed2d_batch *b, double *x, const double *src, const double *am0,
+                   int maxiter, double tol, int precision, int *iters)
+{
+    if (!b) return QED2D_EINVAL;
+    if (b->shared) return bfail(b, QED2D_EINVAL, "a right-hand-side set: use qed2d_batch_cg_shared");
+    return cg_entry(b, x, src, am0, maxiter, tol, precision, iters);
+}
+
+}  // extern "C"
+
+namespace qed2d {
+
+int batch_rhs_cg(qed2d_batch *b, const double *U, double *x, const double *src, double am0,
+                 int maxiter, double eps, int precision, int *iters, int rows_q, int rows,
+                 int rows32, double ru_delta, std::string *err)
+{
+    b->rows_q = rows_q;
+    b->rows = rows;
+    b->rows32 = rows32;
+    b->ru_delta = ru_delta;
+    std::vector<double> m(b->n, am0);
+    const int rc = qed2d_batch_cg_shared(b, U, x, src, m.data(), maxiter, eps, precision, iters);
+    if (rc && err) *err = b->err;
+    return rc;
+}
+
+}  // namespace qed2d
+
+extern "C" {
+
 int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, int integrator,
                         int maxiter, double tol, int precision, double acc_tol,
                         const uint64_t *seed, uint32_t traj, double *hini, double *hfin,
                         int *accepted, int *cg_iters)
 {
     if (!b) return QED2D_EINVAL;
+    if (b->shared) return bfail(b, QED2D_EINVAL, "a right-hand-side set has no links of its own");
     if (!am0 || !seed || !hini || !hfin || !accepted)
         return bfail(b, QED2D_EINVAL, "null pointer");
     if (precision != QED2D_PREC_F64 && precision != QED2D_PREC_MIXED)
@@ -653,6 +775,7 @@ int qed2d_batch_last_dh(const qed2d_batch *b, double *dH)
 int qed2d_batch_observables(qed2d_batch *b, double *plaquette, double *qtop, double *action)
 {
     if (!b) return QED2D_EINVAL;
+    if (b->shared) return bfail(b, QED2D_EINVAL, "a right-hand-side set has no links of its own");
     for (int c = 0; c < b->n; ++c) {
         const double *Uc = (const double *)(b->U + 2 * b->hch[c].off);
         int rc = QED2D_OK;

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "../../include/qed2d.h"
 #include "qed2d_device.h"
 
@@ -112,6 +114,10 @@ struct BGeo {
     const int4 *shape;
     int n;        // chains
     int nblocks;  // blocks of the launch (blk0[n])
+    // nonzero for chains of one lattice shape: block b is local block b / n of
+    // chain b % n, so the chains' blocks of one tile run side by side (and
+    // right-hand sides sharing the links read them once into L2)
+    int inter;
 };
 // Per-chain partial sums of the batched CG: values at pb + local block, and
 // the producer's block count per chain (written by its local block 0).
@@ -119,6 +125,12 @@ struct BParts {
     double *tt[2], *rr[2], *b;
     int *ntt[2], *nrr[2], *nb;
 };
+// CG (Q^2 x_j = src_j from x = 0) on a right-hand-side set made by
+// qed2d_batch_create_rhs, with the given tile rows and reliable-update factor;
+// on failure *err receives the set's message.
+int batch_rhs_cg(qed2d_batch *b, const double *U, double *x, const double *src, double am0,
+                 int maxiter, double eps, int precision, int *iters, int rows_q, int rows,
+                 int rows32, double ru_delta, std::string *err);
 // mask (may be null): per chain, nonzero = apply on that chain.
 void launch_b_q_apply(const BGeo &g, int rows, const double2 *U, const double2 *si,
                       double2 *so, const int *mask, hipStream_t st);

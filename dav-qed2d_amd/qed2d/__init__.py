@@ -67,6 +67,8 @@ SIGNATURES = {
     "qed2d_batch_set_tiles": (_i, [_p, _i, _i, _i]),
     "qed2d_batch_hot_start": (_i, [_p, _p]),
     "qed2d_batch_cg": (_i, [_p, _p, _p, _p, _i, _d, _i, _p]),
+    "qed2d_batch_create_rhs": (_i, [_p, _i, _i, _i, _d, _i, _p]),
+    "qed2d_batch_cg_shared": (_i, [_p, _p, _p, _p, _p, _i, _d, _i, _p]),
     "qed2d_batch_hmc_nf2": (_i, [_p, _p, _d, _i, _i, _i, _d, _i, _d, _p, _u32, _p, _p, _p, _p]),
     "qed2d_batch_observables": (_i, [_p, _p, _p, _p]),
     "qed2d_batch_last_dh": (_i, [_p, _p]),
@@ -99,6 +101,7 @@ SIGNATURES = {
     "qed2d_dot": (_i, [_p, _p, _p, _pd, _pd]),
     "qed2d_norm2": (_i, [_p, _p, _pd]),
     "qed2d_cg": (_i, [_p, _p, _p, _p, _d, _d, _i, _i, _d, _i, _i, _pi]),
+    "qed2d_cg_mrhs": (_i, [_p, _p, _p, _p, _i, _d, _i, _d, _i, _pi]),
     "qed2d_multicg": (_i, [_p, _p, _p, _p, _d, _i, _pd, _pd, _i, _d]),
     "qed2d_R": (_i, [_p, _p, _p, _p, _d, _i, _pd, _pd, _d, _d, _i, _d]),
     "qed2d_generate_pseudofermion": (_i, [_p, _p, _p, _p, _d, _i, _pd, _pd, _i, _d]),
@@ -528,6 +531,25 @@ def CG(*args, so_as_guess=False, precision=PREC_F64):
                              _OPS[A], int(maxiter), eps, 1 if so_as_guess else 0, precision,
                              C.byref(it)))
     return it.value
+
+
+def CG_mrhs(so, U, si, am0, maxiter, eps, prm, kprm=None, precision=PREC_F64):
+    """CG (src/QED2dSolver.jl:3-45, operator ``gamm5Dw_sqr_msq``) on k
+    right-hand sides sharing the links, all of them in one launch per kernel.
+    ``si`` and ``so`` have shape ``(k, 2, L2, L1)``; ``so[j]`` is bit for bit
+    what ``CG(so[j], U, si[j], ...)`` returns with the same tiles.  Returns the
+    iteration counts; raises ``CGNotConverged`` if any right-hand side does not
+    converge."""
+    import torch
+    k = si.shape[0]
+    shp = (k, 2, prm.iL[1], prm.iL[0])
+    if tuple(si.shape) != shp or tuple(so.shape) != shp:
+        raise ValueError(f"so and si must have shape {shp}")
+    its = (_i * k)()
+    _check(prm, lib.qed2d_cg_mrhs(prm.ctx(), _ptr(so, torch.complex128, shp), _sp(prm, U),
+                                  _ptr(si, torch.complex128, shp), k, am0, int(maxiter), eps,
+                                  precision, its))
+    return list(its)
 
 
 # ---------------------------------------------------------------------------

@@ -115,6 +115,14 @@ int qed2d_gamm5Dw_f32(qed2d_ctx *ctx, float *so, const float *U, const float *si
 /* nrhs right-hand sides sharing U; RHS j at si + j*4V doubles. */
 int qed2d_gamm5Dw_mrhs(qed2d_ctx *ctx, double *so, const double *U, const double *si, int nrhs,
                        double am0);
+/* CG (v1 semantics, operator Q^2 [+ 0]) on nrhs right-hand sides sharing U, all
+ * in one launch per kernel; x and b are nrhs fields back to back ((k, 2, L2, L1)),
+ * iters (may be NULL) receives the iterations per right-hand side.  Each
+ * solution equals qed2d_cg on that right-hand side with the same tiles, bit
+ * for bit.  No counterpart in the reference, which solves one right-hand side
+ * per call (src/QED2dSolver.jl:3-45). */
+int qed2d_cg_mrhs(qed2d_ctx *ctx, double *x, const double *U, const double *b, int nrhs,
+                  double am0, int maxiter, double eps, int precision, int *iters);
 int qed2d_gamm5Dw_mrhs_f32(qed2d_ctx *ctx, float *so, const float *U, const float *si, int nrhs,
                            float am0);
 /* gamm5(so, si, am0, prm) kernel (src/QED2dDirac.jl:130-146) */
@@ -257,6 +265,14 @@ int qed2d_batch_hot_start(qed2d_batch *b, const uint64_t *seed);
  * src are arena-shaped (2 x total sites complex).  iters[c] per chain. */
 int qed2d_batch_cg(qed2d_batch *b, double *x, const double *src, const double *am0,
                    int maxiter, double tol, int precision, int *iters);
+/* A set of nrhs right-hand sides of one L1 x L2 lattice that share the caller's
+ * links (the chain-set machinery with every chain's links at the same field):
+ * qed2d_batch_cg_shared solves (Q^2) x_j = src_j for all of them, one launch
+ * per kernel; x and src hold nrhs reference-layout fields back to back. */
+int qed2d_batch_create_rhs(qed2d_batch **b, int L1, int L2, int nrhs, double beta,
+                           int bc_flags, void *stream);
+int qed2d_batch_cg_shared(qed2d_batch *b, const double *U, double *x, const double *src,
+                          const double *am0, int maxiter, double tol, int precision, int *iters);
 /* One Nf=2 trajectory of every chain; am0, seed, hini, hfin, accepted, cg_iters
  * hold one entry per chain (cg_iters may be NULL). */
 int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, int integrator,

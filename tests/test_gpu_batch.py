@@ -115,3 +115,41 @@ def test_batch_ensemble_64_chains_mixed_vs_fp64(Q, O):
     say("max |dH_mixed - dH_fp64|", max(dd), "accepted", sum(i.accepted for i in res[1]))
     assert [i.accepted for i in res[1]] == [i.accepted for i in res[0]]
     assert max(dd) < 1e-9
+
+
+@pytest.mark.parametrize("L1,L2,prec", [(64, 48, 0), (64, 48, 1), (37, 20, 0), (37, 20, 1)])
+def test_cg_mrhs_bit_exact(Q, O, L1, L2, prec):
+    """CG_mrhs (k right-hand sides sharing the links, one launch per kernel)
+    against CG on each right-hand side alone, same context and tiles:
+    identical iteration counts and solutions; a second call with fewer
+    right-hand sides rebuilds the set.  The odd L1 = 37 mixed case runs the
+    right-hand sides one at a time (the set's FP32 kernels march site pairs)."""
+    import torch
+    say("cg_mrhs", L1, L2, "prec", prec)
+    prm = Q.LattParm((L1, L2), 4.0)
+    U = Q.spinor(prm)
+    Q.hot_start(U, prm, 21)
+    for k in (5, 3):
+        si = torch.zeros((k, 2, L2, L1), dtype=torch.complex128, device="cuda")
+        for j in range(k):
+            Q.fill_pseudofermion(si[j], prm, 31, 0, k, j)
+        so = torch.zeros_like(si)
+        its = Q.CG_mrhs(so, U, si, 0.15, 10000, 1e-20, prm, None, precision=prec)
+        for j in range(k):
+            x = Q.spinor(prm)
+            it = Q.CG(x, U, si[j].contiguous(), 0.15, 10000, 1e-20, Q.gamm5Dw_sqr_msq, prm,
+                      None, precision=prec)
+            assert it == its[j], (k, j, it, its[j])
+            assert torch.equal(x, so[j]), (k, j)
+    # the tiles in effect are the context's: change them and compare again
+    Q.set_tiles(prm, q64=4, cg64=8, cg32=4)
+    si = torch.zeros((2, 2, L2, L1), dtype=torch.complex128, device="cuda")
+    for j in range(2):
+        Q.fill_pseudofermion(si[j], prm, 32, 0, 0, j)
+    so = torch.zeros_like(si)
+    its = Q.CG_mrhs(so, U, si, 0.15, 10000, 1e-20, prm, None, precision=prec)
+    for j in range(2):
+        x = Q.spinor(prm)
+        assert Q.CG(x, U, si[j].contiguous(), 0.15, 10000, 1e-20, Q.gamm5Dw_sqr_msq, prm, None,
+                    precision=prec) == its[j]
+        assert torch.equal(x, so[j]), j
