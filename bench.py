@@ -4,7 +4,7 @@ FP64 form on identical random streams, plus the roofline position of the
 dominant kernel and the CPU oracle on the host cores.
 
 Workload (one "step" = one HMC trajectory): compact U(1) gauge action + Nf=2
-Wilson fermions, beta = 4.0, m0 = 0.2, leapfrog ns = 10, eps = 0.1, CG on
+Wilson fermions, beta = 4.0, m0 = 0.2, leapfrog ns = 50, eps = 0.02, CG on
 (gamma5 Dw)^2 to |r|^2 < tol |b|^2, start from a hot configuration (link
 angles uniform in [-pi, pi), seed 1234) thermalised by 200 pure-gauge OMF4
 trajectories.  The acceptance step checks the last MD solve in FP64 (true
@@ -33,10 +33,13 @@ HBM_PEAK_GBS = 8000.0
 # access counted once; FP64 complex = 16 B, FP32 complex = 8 B, 2 spin
 # components or 2 link directions per site).
 BYTES_PER_SITE = {
-    # p = r + beta p_old (reads r, p_old), t = Q p (reads U), writes p, t
-    "cg_k1_f64": 32 * 5, "cg_k1_f32": 16 * 5,
-    # Ap = Q t + mu^2 p (reads t, U, p), x += a p, r -= a Ap (x, r read+write)
-    "cg_k2_f64": 32 * 7, "cg_k2_f32": 16 * 7,
+    # p = r + beta p_old (reads r, p_old), t = Q p (reads U), writes p, t; the
+    # FP32 site-pair K1 also applies the previous iteration's x += a p_old
+    # (x read+write)
+    "cg_k1_f64": 32 * 5, "cg_k1_f32": 16 * 7,
+    # Ap = Q t (+ mu^2 p) (reads t, U), r -= a Ap (r read+write); FP64 also
+    # x += a p (reads p, x read+write)
+    "cg_k2_f64": 32 * 7, "cg_k2_f32": 16 * 4,
     # so = Q si: si, U read, so written
     "q_f64": 96,
     # mom += eps (F_gauge + F_fermion(X, Q X)): U, X, Y read, mom read+write
@@ -60,8 +63,12 @@ def main():
     ap.add_argument("--L", type=int, default=1024)
     ap.add_argument("--beta", type=float, default=4.0)
     ap.add_argument("--m0", type=float, default=0.2)
-    ap.add_argument("--ns", type=int, default=10)
-    ap.add_argument("--eps", type=float, default=0.1)
+    # leapfrog tau = 1 in 50 steps: at 1024^2 the energy violation is O(1) and
+    # about half the trajectories are accepted, so the accept/reject sequences
+    # of the two precisions are a real comparison (at eps 0.1, ns 10, dH ~ 500
+    # and every trajectory is rejected)
+    ap.add_argument("--ns", type=int, default=50)
+    ap.add_argument("--eps", type=float, default=0.02)
     ap.add_argument("--tol", type=float, default=1e-24)
     ap.add_argument("--therm", type=int, default=200)
     ap.add_argument("--no-fp64", action="store_true")
@@ -114,7 +121,8 @@ def main():
         log(f"autotuned tiles {tiles} in {time.time() - t0:.2f}s")
     tiles = Q.get_tiles(prm)
 
-    def run(precision, label, profile=False):
+    def run(precision, label, profile=False, steps=None):
+        steps = steps or args.steps
         U.copy_(U0)
         Q.seed(seed, chain=0)
         acc = []
@@ -127,7 +135,7 @@ def main():
             Q.profile_enable(prm, True, every=1)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             infos.append(Q.HMC(U, args.m0, args.eps, args.ns, acc, 100000, args.tol, prm, None,
                                precision=precision, acc_tol=args.tol))
         torch.cuda.synchronize()
@@ -137,7 +145,7 @@ def main():
             prof = Q.profile_read(prm)
             Q.profile_enable(prm, False)
         it = sum(i.cg_iters for i in infos) / len(infos)
-        log(f"{label}: {args.steps} traj in {dt:.3f}s = {args.steps / dt:.3f} traj/s, "
+        log(f"{label}: {steps} traj in {dt:.3f}s = {steps / dt:.3f} traj/s, "
             f"{it:.0f} CG it/traj, acc {sum(i.accepted for i in infos)}/{len(infos)}")
         if prof:
             for k, (n, ms) in prof.items():
@@ -164,7 +172,8 @@ def main():
     # workload, HIP events around every launch on the context's stream.  The
     # CG kernels' time is divided by the iterations that did work (launches
     # that only found the solve finished or paused are charged to them).
-    _, inf_p, prof_mx = run(Q.PREC_MIXED, "mixed (profiled)", profile=True)
+    _, inf_p, prof_mx = run(Q.PREC_MIXED, "mixed (profiled)", profile=True,
+                            steps=min(args.steps, 10))
     V = L * L
     real_it = sum(i.cg_iters for i in inf_p)
     units = {k: (real_it if k.startswith("cg_") else n) for k, (n, ms) in prof_mx.items()}

@@ -311,6 +311,19 @@ int norm2_host(qed2d_ctx *c, const double *a, double *n2)
     return QED2D_OK;
 }
 
+// Re a^dag b (|a|^2 when b is null) as a double-double, for the fermion terms
+// of H.
+int dot_dd_host(qed2d_ctx *c, const double *a, const double *b, hdd *re)
+{
+    launch_dot_dd((const double2 *)a, (const double2 *)b, spinor_elems(c), c->red, c->part,
+                  c->stream);
+    TRY(launched(c));
+    TRY(read_red(c));
+    *re = hdd();
+    re->add(c->hred->v[0], c->hred->v[1]);
+    return QED2D_OK;
+}
+
 // Sum Re P, sum arg P, sum p^2 as double-doubles.
 int observables(qed2d_ctx *c, const double *U, const double *mom, hdd *sp, hdd *sq, hdd *sm)
 {
@@ -466,11 +479,11 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     for (;;) {
         for (int j = 0; j < chunk; ++j, ++k) {
             PROF(c, QED2D_PROF_CG_K1_F32,
-                 launch_cg_k1_f32(L, Us, rs, P[0], P[1], ts, am0f, c->state, c->P, k, c->rows32,
-                                  st));
-            PROF(c, QED2D_PROF_CG_K2_F32,
-                 launch_cg_k2_f32(L, Us, ts, P[0], P[1], xs, rs, am0f, c->state, c->P, k,
+                 launch_cg_k1_f32(L, Us, rs, P[0], P[1], ts, xs, am0f, c->state, c->P, k,
                                   c->rows32, st));
+            PROF(c, QED2D_PROF_CG_K2_F32,
+                 launch_cg_k2_f32(L, Us, ts, P[0], P[1], xs, rs, am0f, a.mu2 != 0.0, c->state,
+                                  c->P, k, c->rows32, st));
         }
         TRY(launched(c));
         TRY(read_state(c));
@@ -480,7 +493,9 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             // true residual in FP64 and run K1(k) again on it.
             k = c->hstate->iter + 1;
             hipEvent_t ev = prof_begin(c, QED2D_PROF_RELIABLE);
-            launch_ru_accum(L, (double2 *)x, xs, st);
+            // (the site-pair kernels pause with K2(k-1)'s x update pending)
+            launch_ru_accum(L, (double2 *)x, xs, pair_ok(L) ? P[0] : nullptr, P[1],
+                            c->state, st);
             launch_q_apply_f64(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->rows_q, st);
             launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
                             nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, k, 2, c->rows,
@@ -1182,6 +1197,15 @@ int ctx_hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, double 
     *lo = h.lo;
     return QED2D_OK;
 }
+
+int ctx_dot_dd(qed2d_ctx *c, const double *a, const double *b, double *hi, double *lo)
+{
+    hdd h;
+    TRY(dot_dd_host(c, a, b, &h));
+    *hi = h.hi;
+    *lo = h.lo;
+    return QED2D_OK;
+}
 }  // namespace qed2d
 
 // ===========================================================================
@@ -1643,7 +1667,7 @@ int qed2d_cg_mrhs(qed2d_ctx *c, double *x, const double *U, const double *b, int
         return fail(c, QED2D_EINVAL, "unknown precision %d", precision);
     if (maxiter < 1) return fail(c, QED2D_EINVAL, "maxiter must be >= 1");
     const long V2 = spinor_elems(c) * 2;  // doubles per field
-    if (precision == QED2D_PREC_MIXED && (c->L.L1 % 2 != 0 || c->L.L1 < 4)) {
+    if (precision == QED2D_PREC_MIXED && !pair_ok(c->L)) {
         // the set's FP32 kernels march site pairs: one solve at a time
         for (int j = 0; j < nrhs; ++j)
             TRY(cg_any(c, x + j * V2, U, b + j * V2, am0, 0.0, QED2D_OP_SQR_MSQ, maxiter, eps, 0,
@@ -1920,9 +1944,9 @@ int qed2d_hmc_nf2(qed2d_ctx *c, double *U, double am0, double eps, int ns, int i
     // hini = H + X^dag X  (src/QED2dHMC.jl:74)
     hdd h0, h1;
     TRY(hamiltonian_dd(c, mom, U, &h0));
-    double xx;
-    TRY(norm2_host(c, (const double *)X, &xx));
-    h0.add(xx);
+    hdd xx;
+    TRY(dot_dd_host(c, (const double *)X, nullptr, &xx));
+    h0.add(xx.hi, xx.lo);
     q64(c, U, (const double *)X, (double *)F, am0);  // F = Q X  (:76-78)
     int it = 0;
     int rc = md_nf2(c, integrator, mom, U, (double *)X, (const double *)F, (double *)g5DX, am0,
@@ -1939,19 +1963,19 @@ int qed2d_hmc_nf2(qed2d_ctx *c, double *U, double am0, double eps, int ns, int i
     TRY(hamiltonian_dd(c, mom, U, &h1));
     if (acc_tol <= 0.0) {
         // hfin = H + Re X^dag F with X the last MD solve  (:82)
-        double re, im;
-        TRY(dot_host(c, (const double *)X, (const double *)F, &re, &im));
-        h1.add(re);
+        hdd re;
+        TRY(dot_dd_host(c, (const double *)X, (const double *)F, &re));
+        h1.add(re.hi, re.lo);
     } else {
         int ait = 0;
         TRY(cg_any(c, (double *)X, U, (const double *)F, am0, 0.0, QED2D_OP_SQR_MSQ, maxiter,
                    acc_tol, 1, QED2D_PREC_F64, &ait));
         q64(c, U, (const double *)X, (double *)g5DX, am0);
-        double re, im, yy;
-        TRY(dot_host(c, (const double *)X, (const double *)F, &re, &im));
-        TRY(norm2_host(c, (const double *)g5DX, &yy));
-        h1.add(2.0 * re);
-        h1.add(-yy);
+        hdd re, yy;
+        TRY(dot_dd_host(c, (const double *)X, (const double *)F, &re));
+        TRY(dot_dd_host(c, (const double *)g5DX, nullptr, &yy));
+        h1.add(2.0 * re.hi, 2.0 * re.lo);
+        h1.add(-yy.hi, -yy.lo);
     }
     *hini = h0.value();
     *hfin = h1.value();
@@ -1995,9 +2019,9 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
     for (int j = 0; j < nf; ++j) {
         // src/QED2dHMC.jl:136-144
         launch_fill_pseudofermion(c->L, X, k0, k1, traj, (uint32_t)j, st);
-        double xx;
-        TRY(norm2_host(c, (const double *)X, &xx));
-        h0.add(xx);
+        hdd xx;
+        TRY(dot_dd_host(c, (const double *)X, nullptr, &xx));
+        h0.add(xx.hi, xx.lo);
         TRY(gen_pf_impl(c, (double *)F[j], U, (const double *)X, am0[j], npole[j], mu[j], nu[j],
                         maxiter, tol));
     }
@@ -2016,9 +2040,9 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
         // src/QED2dHMC.jl:150-156
         TRY(multicg_impl(c, (double *)X, U, (const double *)F[j], am0[j], npole[j], mu[j], rho[j],
                          maxiter, tol));
-        double re, im;
-        TRY(dot_host(c, (const double *)X, (const double *)F[j], &re, &im));
-        h1.add(re);
+        hdd re;
+        TRY(dot_dd_host(c, (const double *)X, (const double *)F[j], &re));
+        h1.add(re.hi, re.lo);
     }
     *hini = h0.value();
     *hfin = h1.value();

@@ -232,27 +232,36 @@ void launch_b_cg_resid(const BGeo &g, int rows, const double2 *U, const double2 
                   p0, S, P, mode, mask);
 }
 
-// x += double(xs), xs = 0 on the chains that wait for a reliable update.
+// x += double(xs), xs = 0 on the chains that wait for a reliable update, each
+// chain's pending xs += alpha_x p_{stop_k - 1} applied first (the site-pair K1
+// paused before applying it).
 __global__ void __launch_bounds__(kBlock) kb_ru_accum(const BChain *ch, int n, long nel,
                                                       double2 *__restrict__ x,
-                                                      float2 *__restrict__ xs, const int *mask)
+                                                      float2 *__restrict__ xs,
+                                                      const float2 *__restrict__ P0,
+                                                      const float2 *__restrict__ P1,
+                                                      const SolverState *S, const int *mask)
 {
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nel;
          e += (long)gridDim.x * blockDim.x) {
-        if (!mask[chain_of_elem(ch, n, e)]) continue;
-        const float2 d = xs[e];
+        const int c = chain_of_elem(ch, n, e);
+        if (!mask[c]) continue;
+        const SolverState *Sc = S + c;
+        const float2 *pp = ((Sc->stop_k - 1) & 1) ? P1 : P0;
+        const float2 d = xpa(xs[e], (float)Sc->alpha_x, pp[e]);
         const double2 v = x[e];
         x[e] = mk<double>(v.x + (double)d.x, v.y + (double)d.y);
         xs[e] = mk<float>(0.0f, 0.0f);
     }
 }
 void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,
+                       const float2 *p0, const float2 *p1, const SolverState *S,
                        const int *mask, hipStream_t st)
 {
     long nb = (2 * vtot + kBlock - 1) / kBlock;
     if (nb > 8192) nb = 8192;
     hipLaunchKernelGGL(kb_ru_accum, dim3((int)nb), dim3(kBlock), 0, st, ch, nchain, 2 * vtot, x,
-                       xs, mask);
+                       xs, p0, p1, S, mask);
 }
 
 // ---------------------------------------------------------------------------
@@ -328,7 +337,8 @@ template <int R>
 __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, const float2 *__restrict__ U,
                                                   const float2 *__restrict__ r, float2 *P0,
                                                   float2 *P1, float2 *__restrict__ t,
-                                                  SolverState *S, BParts P)
+                                                  float2 *__restrict__ x, SolverState *S,
+                                                  BParts P)
 {
     __shared__ double lds[kBlock / kWave + 1];
     const BWhere w = bwhere<true>(g, R);
@@ -338,13 +348,16 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, const float2 *__restri
     const int wp = k & 1, rd = (k - 1) & 1;
     if (w.p.blk == 0 && threadIdx.x == 0) Sc->k2 = k;
     double beta;
+    bool fresh;
     if (!cg_k1_scalars<true>(Sc, P.rr[rd] + w.pb, P.b + w.pb, P.nrr[rd][w.c], P.nb[w.c], k,
-                             w.p.blk, lds, beta))
+                             w.p.blk, lds, beta, &fresh))
         return;
     const long o = w.o;
-    EpiK1v epi{(wp ? P1 : P0) + o, t + o, w.L.V, 0.0, 0.0};
+    const float2 *pold = (rd ? P1 : P0) + o;
+    EpiK1v epi{(wp ? P1 : P0) + o, t + o, x + o, pold, (float)Sc->alpha_x, !fresh, w.L.V, 0.0,
+               0.0};
     if (w.p.active) {
-        SrcXpay2 src{r + o, (rd ? P1 : P0) + o, (float)beta, w.L.V};
+        SrcXpay2 src{r + o, pold, (float)beta, w.L.V};
         q_march2<R>(w.L, U + w.uo, 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
     }
     const double a = block_allsum(epi.tt, lds);
@@ -359,9 +372,8 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, const float2 *__restri
 template <int R>
 __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, const float2 *__restrict__ U,
                                                   const float2 *__restrict__ t, const float2 *P0,
-                                                  const float2 *P1, float2 *__restrict__ x,
-                                                  float2 *__restrict__ r, SolverState *S,
-                                                  BParts P)
+                                                  const float2 *P1, float2 *__restrict__ r,
+                                                  SolverState *S, BParts P)
 {
     __shared__ double lds[kBlock / kWave + 1];
     const BWhere w = bwhere<true>(g, R);
@@ -373,7 +385,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, const float2 *__restri
     const double pp = sum_prev_partials(P.tt[wp] + 2 * w.pb, P.ntt[wp][w.c], 2, 1, lds);
     const double alpha = Sc->norm[wp] / (tt + Sc->mu2 * pp);
     const long o = w.o;
-    EpiK2v epi{(wp ? P1 : P0) + o, x + o, r + o, (float)alpha, (float)Sc->mu2, w.L.V, 0.0};
+    EpiK2v<false> epi{(wp ? P1 : P0) + o, r + o, (float)alpha, 0.0f, w.L.V, 0.0};
     if (w.p.active) {
         SrcPlain2 src{t + o, w.L.V};
         q_march2<R>(w.L, U + w.uo, 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
@@ -384,6 +396,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, const float2 *__restri
         if (w.p.blk == 0) {
             P.nrr[wp][w.c] = w.nb;
             Sc->kk = k + 1;
+            Sc->alpha_x = alpha;  // for K1(k + 1)
         }
     }
 }
@@ -415,16 +428,18 @@ void launch_b_cg_k2_f64(const BGeo &g, int rows, const double2 *U, const double2
     }
 }
 void launch_b_cg_k1_f32(const BGeo &g, int rows, const float2 *U, const float2 *r, float2 *p0,
-                        float2 *p1, float2 *t, SolverState *S, BParts P, hipStream_t st)
+                        float2 *p1, float2 *t, float2 *x, SolverState *S, BParts P,
+                        hipStream_t st)
 {
-    DISPATCH_ROWS(rows, kb_cg1v, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, r, p0, p1, t, S, P);
+    DISPATCH_ROWS(rows, kb_cg1v, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, r, p0, p1, t, x, S,
+                  P);
 }
+// (x is updated by the next K1 or the reliable update's flush)
 void launch_b_cg_k2_f32(const BGeo &g, int rows, const float2 *U, const float2 *t,
-                        const float2 *p0, const float2 *p1, float2 *x, float2 *r,
+                        const float2 *p0, const float2 *p1, float2 *, float2 *r,
                         SolverState *S, BParts P, hipStream_t st)
 {
-    DISPATCH_ROWS(rows, kb_cg2v, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, p0, p1, x, r, S,
-                  P);
+    DISPATCH_ROWS(rows, kb_cg2v, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, p0, p1, r, S, P);
 }
 
 // ---------------------------------------------------------------------------

@@ -241,8 +241,8 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     for (;;) {
         for (int j = 0; j < chunk; ++j) {
             if (mixed) {
-                launch_b_cg_k1_f32(g32, b->rows32, b->Us, b->rs, b->p0s, b->p1s, b->ts, b->S,
-                                   b->P, st);
+                launch_b_cg_k1_f32(g32, b->rows32, b->Us, b->rs, b->p0s, b->p1s, b->ts, b->xs,
+                                   b->S, b->P, st);
                 launch_b_cg_k2_f32(g32, b->rows32, b->Us, b->ts, b->p0s, b->p1s, b->xs, b->rs,
                                    b->S, b->P, st);
             } else {
@@ -277,7 +277,8 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             // recompute the true residual in FP64, resume at the same K1
             BHIP(b, hipMemcpyAsync(b->mask, b->hmask, sizeof(int) * b->n, hipMemcpyHostToDevice,
                                    st));
-            launch_b_ru_accum(b->dch, b->n, b->vtot, x, b->xs, b->mask, st);
+            launch_b_ru_accum(b->dch, b->n, b->vtot, x, b->xs, b->p0s, b->p1s, b->S, b->mask,
+                              st);
             launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, b->mask, st);
             launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, nullptr, b->rs, nullptr, b->S,
                               b->P, 2, b->mask, st);
@@ -691,8 +692,8 @@ int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, i
         int rc = ctx_hamiltonian_dd(b->sub[c], b->mom + o, (const double *)(b->U + o), &h0h[c],
                                     &h0l[c]);
         if (rc) return subfail(b, c, rc);
-        double xx;
-        rc = qed2d_norm2(b->sub[c], (const double *)(b->X + o), &xx);
+        dd xx;
+        rc = ctx_dot_dd(b->sub[c], (const double *)(b->X + o), nullptr, &xx.hi, &xx.lo);
         if (rc) return subfail(b, c, rc);
         dd h{h0h[c], h0l[c]};
         h = dd_add(h, xx);
@@ -734,18 +735,18 @@ int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, i
         rc = ctx_hamiltonian_dd(b->sub[c], b->mom + o, (const double *)(b->U + o), &hh, &hl);
         if (rc) return subfail(b, c, rc);
         dd h1{hh, hl};
-        double re, im;
-        rc = qed2d_dot(b->sub[c], (const double *)(b->X + o), (const double *)(b->F + o), &re,
-                       &im);
+        dd re;
+        rc = ctx_dot_dd(b->sub[c], (const double *)(b->X + o), (const double *)(b->F + o), &re.hi,
+                        &re.lo);
         if (rc) return subfail(b, c, rc);
         if (acc_tol <= 0.0) {
             h1 = dd_add(h1, re);  // hfin = H + Re X^dag F  (:82)
         } else {
-            double yy;
-            rc = qed2d_norm2(b->sub[c], (const double *)(b->Y + o), &yy);
+            dd yy;
+            rc = ctx_dot_dd(b->sub[c], (const double *)(b->Y + o), nullptr, &yy.hi, &yy.lo);
             if (rc) return subfail(b, c, rc);
-            h1 = dd_add(h1, 2.0 * re);
-            h1 = dd_add(h1, -yy);
+            h1 = dd_add(h1, dd{2.0 * re.hi, 2.0 * re.lo});
+            h1 = dd_add(h1, dd{-yy.hi, -yy.lo});
         }
         hini[c] = h0h[c] + h0l[c];
         hfin[c] = h1.hi + h1.lo;

@@ -18,11 +18,13 @@ template <typename T> __device__ __forceinline__ c2<T> from_d2(double2 v)
 // (converged / iteration limit) or, for the FP32 iterations of the mixed
 // solver, pauses for a reliable update; block 0 commits the scalars of slot
 // k & 1.  Returns false when the kernel has no work; beta receives
-// |r_{k-1}|^2 / |r_{k-2}|^2 (src/QED2dSolver.jl:33-35).
+// |r_{k-1}|^2 / |r_{k-2}|^2 (src/QED2dSolver.jl:33-35); fresh (if given) is
+// set when r_{k-1} is a new start (k = 1 or right after a reliable update),
+// i.e. when no deferred x update is pending.
 template <bool mixed>
 __device__ __forceinline__ bool cg_k1_scalars(SolverState *S, const double *Prr, const double *Pb,
                                               int nrr, int nbb, int k, int blk, double *lds,
-                                              double &beta)
+                                              double &beta, bool *fresh = nullptr)
 {
     const int w = k & 1, rd = (k - 1) & 1;
     const double rr = sum_prev_partials(Prr, nrr, 1, 0, lds);
@@ -32,6 +34,7 @@ __device__ __forceinline__ bool cg_k1_scalars(SolverState *S, const double *Prr,
     const double prev = first ? 0.0 : S->norm[rd];
     double rmax = first ? rr : S->rmax2[rd];
     const bool truer = first || S->true_at == k;
+    if (fresh) *fresh = truer;
     int stat = 0, ru = 0;
     if (first && bn == 0.0 && !S->guess) {
         stat = 1;  // b = 0: x = 0 exactly
@@ -120,19 +123,48 @@ template <typename T> struct EpiK2 {
     }
 };
 
+// x + alpha p of the FP32 site-pair solver: K1(k+1) and the reliable update's
+// flush both apply the update that K2(k) deferred, with this one expression.
+__device__ __forceinline__ float2 xpa(float2 x, float a, float2 p)
+{
+    return make_float2(__builtin_fmaf(a, p.x, x.x), __builtin_fmaf(a, p.y, x.y));
+}
+
+// K1(k) of the site-pair solver: stores p_k and t = Q p_k, sums |t|^2 and
+// |p_k|^2, and unless r_{k-1} is fresh applies K2(k-1)'s deferred
+// x += alpha_{k-1} p_{k-1} (src/QED2dSolver.jl:24) at the same sites.
 struct EpiK1v {
     float2 *__restrict__ pnew;
     float2 *__restrict__ t;
+    float2 *__restrict__ x;
+    const float2 *__restrict__ pold;
+    float alpha;  // alpha_{k-1}
+    bool upd;
     long V;
     double tt, pp;
-    __device__ __forceinline__ NoPre pre(long) const { return {}; }
-    __device__ __forceinline__ void pair(long s, NoPre, float2 oa1, float2 oa2, float2 ob1,
+    struct Pre {
+        F2x2 x1, x2, p1, p2;
+    };
+    __device__ __forceinline__ Pre pre(long s) const
+    {
+        if (!upd) return Pre{};
+        return Pre{ld2(x, s), ld2(x, V + s), ld2(pold, s), ld2(pold, V + s)};
+    }
+    __device__ __forceinline__ void xupd(long i, F2x2 xv, F2x2 pv)
+    {
+        st2(x, i, xpa(xv.a0, alpha, pv.a0), xpa(xv.a1, alpha, pv.a1));
+    }
+    __device__ __forceinline__ void pair(long s, const Pre &q, float2 oa1, float2 oa2, float2 ob1,
                                          float2 ob2, F2x2 c1, F2x2 c2)
     {
         st2(pnew, s, c1.a0, c1.a1);
         st2(pnew, V + s, c2.a0, c2.a1);
         st2(t, s, oa1, ob1);
         st2(t, V + s, oa2, ob2);
+        if (upd) {
+            xupd(s, q.x1, q.p1);
+            xupd(V + s, q.x2, q.p2);
+        }
         tt += (double)(cabs2<float>(oa1) + cabs2<float>(ob1)) +
               (double)(cabs2<float>(oa2) + cabs2<float>(ob2));
         pp += (double)(cabs2<float>(c1.a0) + cabs2<float>(c1.a1)) +
@@ -140,37 +172,45 @@ struct EpiK1v {
     }
 };
 
-struct EpiK2v {
+// K2(k) of the site-pair solver: r -= alpha_k (Q t + mu2 p_k)
+// (src/QED2dSolver.jl:25); the x update is left to K1(k+1).  MU: mu2 != 0
+// (only then is p_k read here).
+template <bool MU> struct EpiK2v {
     const float2 *__restrict__ p;
-    float2 *__restrict__ x;
     float2 *__restrict__ r;
     float alpha, mu2;
     long V;
     double rr;
     struct Pre {
-        F2x2 p1, p2, x1, x2, r1, r2;
+        F2x2 p1, p2, r1, r2;
     };
     __device__ __forceinline__ Pre pre(long s) const
     {
-        return Pre{ld2(p, s), ld2(p, V + s), ld2(x, s), ld2(x, V + s), ld2(r, s), ld2(r, V + s)};
+        Pre q{};
+        if (MU) {
+            q.p1 = ld2(p, s);
+            q.p2 = ld2(p, V + s);
+        }
+        q.r1 = ld2(r, s);
+        q.r2 = ld2(r, V + s);
+        return q;
     }
-    __device__ __forceinline__ void comp(long i, float2 oa, float2 ob, F2x2 pv, F2x2 xv, F2x2 rv)
+    __device__ __forceinline__ void comp(long i, float2 oa, float2 ob, F2x2 pv, F2x2 rv)
     {
-        const float2 aa = make_float2(oa.x + mu2 * pv.a0.x, oa.y + mu2 * pv.a0.y);
-        const float2 ab = make_float2(ob.x + mu2 * pv.a1.x, ob.y + mu2 * pv.a1.y);
-        xv.a0 = make_float2(xv.a0.x + alpha * pv.a0.x, xv.a0.y + alpha * pv.a0.y);
-        xv.a1 = make_float2(xv.a1.x + alpha * pv.a1.x, xv.a1.y + alpha * pv.a1.y);
-        rv.a0 = make_float2(rv.a0.x - alpha * aa.x, rv.a0.y - alpha * aa.y);
-        rv.a1 = make_float2(rv.a1.x - alpha * ab.x, rv.a1.y - alpha * ab.y);
-        st2(x, i, xv.a0, xv.a1);
+        if (MU) {
+            oa = make_float2(oa.x + mu2 * pv.a0.x, oa.y + mu2 * pv.a0.y);
+            ob = make_float2(ob.x + mu2 * pv.a1.x, ob.y + mu2 * pv.a1.y);
+        }
+        rv.a0 = make_float2(rv.a0.x - alpha * oa.x, rv.a0.y - alpha * oa.y);
+        rv.a1 = make_float2(rv.a1.x - alpha * ob.x, rv.a1.y - alpha * ob.y);
         st2(r, i, rv.a0, rv.a1);
         rr += (double)(cabs2<float>(rv.a0) + cabs2<float>(rv.a1));
     }
     __device__ __forceinline__ void pair(long s, const Pre &q, float2 oa1, float2 oa2, float2 ob1,
                                          float2 ob2, F2x2, F2x2)
     {
-        comp(s, oa1, ob1, q.p1, q.x1, q.r1);
-        comp(V + s, oa2, ob2, q.p2, q.x2, q.r2);
+        comp(s, oa1, ob1, q.p1, q.r1);
+        comp(V + s, oa2, ob2, q.p2, q.r2);
     }
 };
 

@@ -21,6 +21,9 @@ struct Latt {
     int bc;  // bit 0: fermion fields antiperiodic across i2 = L2-1 -> 0
 };
 
+// The FP32 kernels march site pairs (i1 = 2j, 2j+1) on lattices with even L1 >= 4.
+inline bool pair_ok(const Latt &L) { return (L.L1 % 2) == 0 && L.L1 >= 4; }
+
 template <typename T> struct cvec;
 template <> struct cvec<double> { using type = double2; };
 template <> struct cvec<float> { using type = float2; };

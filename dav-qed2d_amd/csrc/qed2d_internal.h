@@ -33,6 +33,9 @@ struct SolverState {
     int true_at;       // K1(true_at) sees a true (FP64) residual norm
     int stop_k;        // nonzero: K1(stop_k) ended or paused the solve; later kernels return
     int kk, k2;        // batched solves: the iteration K1 runs next / K2 runs (per chain)
+    // FP32 site-pair iterations: K2(k) leaves x += alpha_k p_k to K1(k+1), which
+    // reads p_k anyway; alpha_x is that pending alpha_k (written by K2's block 0)
+    double alpha_x;
 };
 
 // Multi-shift CG state (all shifts sigma_i = mu_i^2 of (Q^2 + sigma_i) x_i = b on
@@ -91,6 +94,8 @@ constexpr double OMF_R4 = -0.03230286765269967;
 
 // H = sum p^2 / 2 + S_g of a context's lattice as a double-double (hi, lo).
 int ctx_hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, double *hi, double *lo);
+// Re a^dag b (|a|^2 when b is null) of a context's lattice as a double-double.
+int ctx_dot_dd(qed2d_ctx *c, const double *a, const double *b, double *hi, double *lo);
 
 // ---- batched ensembles ----------------------------------------------------
 // A chain set: lattices whose fields are concatenated in one arena.  Chain
@@ -149,11 +154,14 @@ void launch_b_cg_k2_f64(const BGeo &g, int rows, const double2 *U, const double2
                         const double2 *p0, const double2 *p1, double2 *x, double2 *r,
                         SolverState *S, BParts P, hipStream_t st);
 void launch_b_cg_k1_f32(const BGeo &g, int rows, const float2 *U, const float2 *r, float2 *p0,
-                        float2 *p1, float2 *t, SolverState *S, BParts P, hipStream_t st);
+                        float2 *p1, float2 *t, float2 *x, SolverState *S, BParts P,
+                        hipStream_t st);
 void launch_b_cg_k2_f32(const BGeo &g, int rows, const float2 *U, const float2 *t,
                         const float2 *p0, const float2 *p1, float2 *x, float2 *r,
                         SolverState *S, BParts P, hipStream_t st);
+// (each masked chain's pending FP32 x update applied first, as launch_ru_accum)
 void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,
+                       const float2 *p0, const float2 *p1, const SolverState *S,
                        const int *mask, hipStream_t st);
 // Multi-shift CG of one right-hand side per chain (M[c]: chain c's state).
 void launch_b_ms_init(const BGeo &g, int rows, double2 *r, double2 *p0,
@@ -207,13 +215,17 @@ void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const d
                       const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
                       CGPartials &P, int k, int rows, hipStream_t st);
 void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
-                      float2 *t, float am0, SolverState *s, CGPartials &P, int k, int rows,
+                      float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P, int k, int rows,
                       hipStream_t st);
+// (mu: the operator has mu2 != 0.  On even L1 >= 4 the site-pair kernels
+// leave x += alpha p to the next K1, see SolverState::alpha_x.)
 void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
-                      const float2 *p1, float2 *x, float2 *r, float am0, SolverState *s,
+                      const float2 *p1, float2 *x, float2 *r, float am0, bool mu, SolverState *s,
                       CGPartials &P, int k, int rows, hipStream_t st);
-// x += double(xs); xs = 0 (a reliable update).
-void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, hipStream_t st);
+// A reliable update's x += double(xs), xs = 0; with p0 / p1 given (the FP32 site-pair solver, whose
+// K1 was paused with an x update pending), xs + alpha_x p_{stop_k - 1} first.
+void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const float2 *p0, const float2 *p1,
+                     const SolverState *S, hipStream_t st);
 void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st);
 
 // ---- multi-shift CG --------------------------------------------------------
@@ -229,6 +241,10 @@ void launch_ms_k2(const Latt &L, const double2 *U, const double2 *t, const doubl
 
 // Reductions (deterministic; results in out->v[]).
 void launch_norm2(const double2 *a, long n, RedOut *out, double *part, hipStream_t st);
+// a^dag b (|a|^2 when b is null) as double-doubles: out->v = (re hi, re lo,
+// im hi, im lo); part holds 4 doubles per block (at most 1024 blocks).
+void launch_dot_dd(const double2 *a, const double2 *b, long n, RedOut *out, double *part,
+                   hipStream_t st);
 void launch_dot(const double2 *a, const double2 *b, long n, RedOut *out, double *part,
                 hipStream_t st);
 // out->v[0..1] = sum Re P (double-double hi, lo); v[2..3] = sum arg P; v[4..5] = sum p^2

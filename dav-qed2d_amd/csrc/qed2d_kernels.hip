@@ -480,6 +480,46 @@ __global__ void __launch_bounds__(kBlock) k_finish_dd(const double *part, int nb
     }
 }
 
+// a^dag b or |a|^2 (b == nullptr) accumulated as double-doubles: partials
+// (re hi, re lo, im hi, im lo) per block.  For the fermion terms of H, whose
+// size is ~V and whose differences between two trajectories must be resolved
+// below 1e-9.
+__global__ void __launch_bounds__(kBlock) k_dot_dd(const double2 *__restrict__ a,
+                                                   const double2 *__restrict__ b, long n,
+                                                   double *part)
+{
+    __shared__ double lds[2 * (kBlock / kWave)];
+    dd sr = {0.0, 0.0}, si = {0.0, 0.0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        const double2 x = a[i];
+        if (b) {
+            const double2 y = b[i];
+            sr = dd_add(sr, x.x * y.x);
+            sr = dd_add(sr, x.y * y.y);
+            si = dd_add(si, x.x * y.y);
+            si = dd_add(si, -(x.y * y.x));
+        } else {
+            sr = dd_add(sr, x.x * x.x);
+            sr = dd_add(sr, x.y * x.y);
+        }
+    }
+    const dd br = block_sum_dd(sr, lds);
+    const dd bi = block_sum_dd(si, lds);
+    if (threadIdx.x == 0) {
+        double *pp = part + 4 * (long)blockIdx.x;
+        pp[0] = br.hi; pp[1] = br.lo; pp[2] = bi.hi; pp[3] = bi.lo;
+    }
+}
+
+void launch_dot_dd(const double2 *a, const double2 *b, long n, RedOut *out, double *part,
+                   hipStream_t st)
+{
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(k_dot_dd, dim3(nb), dim3(kBlock), 0, st, a, b, n, part);
+    hipLaunchKernelGGL(k_finish_dd, dim3(1), dim3(kBlock), 0, st, part, nb, 2, out);
+}
+
 void launch_observables(const Latt &L, const double2 *U, const double *mom, double *plx,
                         double *qdens, RedOut *out, double *part, hipStream_t st)
 {

@@ -461,7 +461,6 @@ inline MarchGrid march_grid2(const Latt &L, int rows)
     return march_grid(h, rows);
 }
 
-inline bool pair_ok(const Latt &L) { return (L.L1 % 2) == 0 && L.L1 >= 4; }
 
 #define DISPATCH_ROWS(rows, KERNEL, ...)                                              \
     switch (rows) {                                                                   \

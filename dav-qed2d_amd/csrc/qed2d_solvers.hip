@@ -168,20 +168,32 @@ void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const do
     if (mode != 2) P.nb = fold(P.b, g.nblocks(), 1, st);
 }
 
-// x += double(xs); xs = 0
-__global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs, long n)
+// x += double(xs); xs = 0 (with the pending xs += alpha_x p_{stop_k - 1} first
+// when P0 is given)
+__global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs,
+                           const float2 *__restrict__ P0, const float2 *__restrict__ P1,
+                           const SolverState *S, long n)
 {
+    const float2 *pp = nullptr;
+    float a = 0.0f;
+    if (P0) {
+        pp = ((S->stop_k - 1) & 1) ? P1 : P0;
+        a = (float)S->alpha_x;
+    }
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
          i += (long)gridDim.x * blockDim.x) {
-        const float2 d = xs[i];
+        float2 d = xs[i];
+        if (pp) d = xpa(d, a, pp[i]);
         const double2 v = x[i];
         x[i] = mk<double>(v.x + (double)d.x, v.y + (double)d.y);
         xs[i] = mk<float>(0.0f, 0.0f);
     }
 }
-void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, hipStream_t st)
+void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const float2 *p0, const float2 *p1,
+                     const SolverState *S, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ru_accum, dim3(ew_grid(2 * L.V)), dim3(kBlock), 0, st, x, xs, 2 * L.V);
+    hipLaunchKernelGGL(k_ru_accum, dim3(ew_grid(2 * L.V)), dim3(kBlock), 0, st, x, xs, p0, p1, S,
+                       2 * L.V);
 }
 
 __global__ void k_to_f32(const double2 *__restrict__ in, float2 *__restrict__ out, long n)
@@ -264,7 +276,8 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, const float2 *__restrict__ U,
                                                  const float2 *__restrict__ r, float2 *P0,
-                                                 float2 *P1, float2 *__restrict__ t, float mass2,
+                                                 float2 *P1, float2 *__restrict__ t,
+                                                 float2 *__restrict__ x, float mass2,
                                                  SolverState *S, double *Ptt, const double *Prr,
                                                  const double *Pb, int nrr, int nbb, int k, int bx,
                                                  int by, int nbx)
@@ -275,10 +288,12 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, const float2 *__restric
     const int w = k & 1, rd = (k - 1) & 1;
     const MarchPos p = march_pos2(L, bx, by, nbx, R);
     double beta;
-    if (!cg_k1_scalars<true>(S, Prr, Pb, nrr, nbb, k, p.blk, lds, beta)) return;
-    EpiK1v epi{w ? P1 : P0, t, L.V, 0.0, 0.0};
+    bool fresh;
+    if (!cg_k1_scalars<true>(S, Prr, Pb, nrr, nbb, k, p.blk, lds, beta, &fresh)) return;
+    float2 *pold = rd ? P1 : P0;
+    EpiK1v epi{w ? P1 : P0, t, x, pold, (float)S->alpha_x, !fresh, L.V, 0.0, 0.0};
     if (p.active) {
-        SrcXpay2 src{r, rd ? P1 : P0, (float)beta, L.V};
+        SrcXpay2 src{r, pold, (float)beta, L.V};
         q_march2<R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
     const double a = block_allsum(epi.tt, lds);
@@ -289,13 +304,12 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, const float2 *__restric
     }
 }
 
-template <int R>
-__global__ void __launch_bounds__(kBlock) k_cg2v(Latt L, const float2 *__restrict__ U,
-                                                 const float2 *__restrict__ t, const float2 *P0,
-                                                 const float2 *P1, float2 *__restrict__ x,
-                                                 float2 *__restrict__ r, float mass2,
-                                                 SolverState *S, const double *Ptt, double *Prr,
-                                                 int ntt, int k, int bx, int by, int nbx)
+template <int R, bool MU>
+__device__ __forceinline__ void cg2v_body(const Latt &L, const float2 *__restrict__ U,
+                                          const float2 *__restrict__ t, const float2 *P0,
+                                          const float2 *P1, float2 *__restrict__ r, float mass2,
+                                          SolverState *S, const double *Ptt, double *Prr, int ntt,
+                                          int k, int bx, int by, int nbx)
 {
     __shared__ double lds[kBlock / kWave + 1];
     const int sk = S->stop_k;
@@ -305,18 +319,33 @@ __global__ void __launch_bounds__(kBlock) k_cg2v(Latt L, const float2 *__restric
     const double tt = sum_prev_partials(Ptt, ntt, 2, 0, lds);
     const double pp = sum_prev_partials(Ptt, ntt, 2, 1, lds);
     const double alpha = S->norm[w] / (tt + S->mu2 * pp);
-    EpiK2v epi{w ? P1 : P0, x, r, (float)alpha, (float)S->mu2, L.V, 0.0};
+    EpiK2v<MU> epi{w ? P1 : P0, r, (float)alpha, (float)S->mu2, L.V, 0.0};
     if (p.active) {
         SrcPlain2 src{t, L.V};
         q_march2<R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
     const double a = block_allsum(epi.rr, lds);
-    if (threadIdx.x == 0) Prr[p.blk] = a;
+    if (threadIdx.x == 0) {
+        Prr[p.blk] = a;
+        if (p.blk == 0) S->alpha_x = alpha;  // for K1(k + 1)
+    }
 }
+#define CG2V_KERNEL(NAME, MU)                                                              \
+    template <int R>                                                                       \
+    __global__ void __launch_bounds__(kBlock)                                              \
+        NAME(Latt L, const float2 *__restrict__ U, const float2 *__restrict__ t,           \
+             const float2 *P0, const float2 *P1, float2 *__restrict__ r, float mass2,      \
+             SolverState *S, const double *Ptt, double *Prr, int ntt, int k, int bx, int by, \
+             int nbx)                                                                      \
+    {                                                                                      \
+        cg2v_body<R, MU>(L, U, t, P0, P1, r, mass2, S, Ptt, Prr, ntt, k, bx, by, nbx);     \
+    }
+CG2V_KERNEL(k_cg2v, false)
+CG2V_KERNEL(k_cg2vm, true)  // mu2 != 0
 
 template <typename T>
 static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c2<T> *p1,
-                     c2<T> *t, T am0, SolverState *S, CGPartials &P, int k, int rows,
+                     c2<T> *t, c2<T> *x, T am0, SolverState *S, CGPartials &P, int k, int rows,
                      hipStream_t st)
 {
     const T m2 = T(2) + am0;
@@ -327,8 +356,8 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
         if (pair_ok(L)) {
             MarchGrid g = march_grid2(L, rows);
             dim3 grid(g.nblocks()), blk(kBlock);
-            DISPATCH_ROWS(rows, k_cg1v, grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr,
-                          P.b, nrr, nbb, k, g.bx, g.by, g.nbx);
+            DISPATCH_ROWS(rows, k_cg1v, grid, blk, 0, st, L, U, r, p0, p1, t, x, m2, S, ptt,
+                          prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx);
             P.ntt[k & 1] = fold(ptt, g.nblocks(), 2, st);
             return;
         }
@@ -348,7 +377,7 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
 template <typename T>
 static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> *p0,
                      const c2<T> *p1, c2<T> *x, c2<T> *r, T am0, SolverState *S,
-                     CGPartials &P, int k, int rows, hipStream_t st)
+                     CGPartials &P, int k, int rows, bool mu, hipStream_t st)
 {
     const T m2 = T(2) + am0;
     const double *ptt = P.tt[k & 1];
@@ -358,8 +387,14 @@ static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> 
         if (pair_ok(L)) {
             MarchGrid g = march_grid2(L, rows);
             dim3 grid(g.nblocks()), blk(kBlock);
-            DISPATCH_ROWS(rows, k_cg2v, grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr,
-                          ntt, k, g.bx, g.by, g.nbx);
+            // x is updated by the next K1 (or the reliable update's flush)
+            if (mu) {
+                DISPATCH_ROWS(rows, k_cg2vm, grid, blk, 0, st, L, U, t, p0, p1, r, m2, S, ptt,
+                              prr, ntt, k, g.bx, g.by, g.nbx);
+            } else {
+                DISPATCH_ROWS(rows, k_cg2v, grid, blk, 0, st, L, U, t, p0, p1, r, m2, S, ptt, prr,
+                              ntt, k, g.bx, g.by, g.nbx);
+            }
             P.nrr[k & 1] = fold(prr, g.nblocks(), 1, st);
             return;
         }
@@ -380,25 +415,25 @@ void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2
                       double2 *p1, double2 *t, double am0, SolverState *s, CGPartials &P,
                       int k, int rows, hipStream_t st)
 {
-    cg1_impl<double>(L, U, r, p0, p1, t, am0, s, P, k, rows, st);
+    cg1_impl<double>(L, U, r, p0, p1, t, nullptr, am0, s, P, k, rows, st);
 }
 void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
                       const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
                       CGPartials &P, int k, int rows, hipStream_t st)
 {
-    cg2_impl<double>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, st);
+    cg2_impl<double>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, false, st);
 }
 void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
-                      float2 *t, float am0, SolverState *s, CGPartials &P, int k, int rows,
-                      hipStream_t st)
+                      float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P, int k,
+                      int rows, hipStream_t st)
 {
-    cg1_impl<float>(L, U, r, p0, p1, t, am0, s, P, k, rows, st);
+    cg1_impl<float>(L, U, r, p0, p1, t, x, am0, s, P, k, rows, st);
 }
 void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
-                      const float2 *p1, float2 *x, float2 *r, float am0, SolverState *s,
+                      const float2 *p1, float2 *x, float2 *r, float am0, bool mu, SolverState *s,
                       CGPartials &P, int k, int rows, hipStream_t st)
 {
-    cg2_impl<float>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, st);
+    cg2_impl<float>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, mu, st);
 }
 
 // ===========================================================================
