@@ -137,7 +137,6 @@ struct EpiK1v {
     float2 *__restrict__ pnew;
     float2 *__restrict__ t;
     float2 *__restrict__ x;
-    const float2 *__restrict__ pold;
     float alpha;  // alpha_{k-1}
     bool upd;
     long V;
@@ -145,10 +144,11 @@ struct EpiK1v {
     struct Pre {
         F2x2 x1, x2, p1, p2;
     };
-    __device__ __forceinline__ Pre pre(long s) const
+    // p_{k-1} of the row comes with the source's raw loads (SrcXpay2::Raw2)
+    template <typename Raw> __device__ __forceinline__ Pre pre_raw(long s, const Raw &w) const
     {
         if (!upd) return Pre{};
-        return Pre{ld2(x, s), ld2(x, V + s), ld2(pold, s), ld2(pold, V + s)};
+        return Pre{ld2(x, s), ld2(x, V + s), w.p0, w.p1};
     }
     __device__ __forceinline__ void xupd(long i, F2x2 xv, F2x2 pv)
     {

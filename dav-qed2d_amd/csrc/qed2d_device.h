@@ -87,11 +87,33 @@ __device__ __forceinline__ void q_site(T mass2, c2<T> c1, c2<T> c2_, c2<T> p11, 
 // The order of the tree is fixed by the launch shape, so results are
 // reproducible run to run.
 // ---------------------------------------------------------------------------
+// One DPP data movement of a double (two 32-bit halves); lanes outside
+// ROWMASK receive 0.
+template <int CTRL, int ROWMASK = 0xF> __device__ __forceinline__ double dpp_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// Sum over the 64 lanes of a wave, returned to every lane.  Within each row of
+// 16 lanes: quad swaps (xor 1, xor 2) and row rotations by 4 and 8 leave the
+// row's sum in every lane; row_bcast:15 / row_bcast:31 then accumulate the
+// rows into lane 63, which is broadcast.  A fixed order of additions: every
+// wave that sums the same values gets the same bits.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    v += dpp_f64<0xB1>(v);       // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);       // quad_perm [2,3,0,1]
+    v += dpp_f64<0x124>(v);      // row_ror:4
+    v += dpp_f64<0x128>(v);      // row_ror:8
+    v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 -> rows 1, 3
+    v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 -> rows 2, 3
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, kWave - 1);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), kWave - 1);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }
 
 // Sums v over the block; result valid in thread 0.  lds must hold blockDim/64

@@ -325,6 +325,21 @@ struct SrcXpay2 {
     }
 };
 
+// The epilogue's loads for the row of site s: epi.pre_raw(s, raw) when the
+// epilogue has it (raw: the source's raw loads of that row, already in
+// registers), else epi.pre(s).
+template <typename E, typename Raw>
+__device__ __forceinline__ auto pre_of(const E &e, long s, const Raw &raw, int)
+    -> decltype(e.pre_raw(s, raw))
+{
+    return e.pre_raw(s, raw);
+}
+template <typename E, typename Raw>
+__device__ __forceinline__ auto pre_of(const E &e, long s, const Raw &, long) -> decltype(e.pre(s))
+{
+    return e.pre(s);
+}
+
 // March the column pair (2j, 2j+1) over rows r0 .. r0+R-1, software pipelined
 // like q_march, with the neighbours 2j-1 and 2j+2 (and U(2j-1, 1)) from the
 // adjacent lanes away from the wave's edges; the epilogue gets both sites'
@@ -337,7 +352,7 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
 {
     using Raw2 = typename Src::Raw2;
     using Raw1 = typename Src::Raw1;
-    using Pre = decltype(epi.pre(0L));
+    using Pre = decltype(pre_of(epi, 0L, Raw2{}, 0));
     const int L1 = L.L1, L2 = L.L2;
     const long V = L.V;
     const int ia = 2 * j;
@@ -351,7 +366,8 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
     const int nr = (r0 + R <= L2) ? R : L2 - r0;
     F2x2 dn1, dn2, c1, cc2;
     src.value2(src.fetch2((long)rm * L1 + ia), dn1, dn2);
-    src.value2(src.fetch2((long)r0 * L1 + ia), c1, cc2);
+    const Raw2 wc = src.fetch2((long)r0 * L1 + ia);
+    src.value2(wc, c1, cc2);
     F2x2 U2m = ld2(U, V + (long)rm * L1 + ia);
     if (ap && r0 == 0) U2m = neg2(U2m);
     long row = (long)r0 * L1;
@@ -364,7 +380,7 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
     }
     if (ldR) wq = src.fetch1(row + ip);
     F2x2 U1 = ld2(U, row + ia), U2 = ld2(U, V + row + ia);
-    Pre q = epi.pre(row + ia);
+    Pre q = pre_of(epi, row + ia, wc, 0);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         if (k >= nr) break;
@@ -386,7 +402,7 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
             if (ldR) nwq = src.fetch1(row1 + ip);
             nU1 = ld2(U, row1 + ia);
             nU2 = ld2(U, V + row1 + ia);
-            nq = epi.pre(row1 + ia);
+            nq = pre_of(epi, row1 + ia, wu, 0);
         }
         F2x2 up1, up2;
         float2 l1, l2, q1, q2;
@@ -492,14 +508,29 @@ __device__ __forceinline__ double block_allsum(double v, double *lds)
 }
 
 // Sum of the per-block partials part[b * stride + off], b < n, in a fixed
-// order; every block that calls this gets the same bits.  The partials were
-// written by an earlier kernel, so the kernel boundary has made them visible.
+// order: lane l adds part[l], part[l + 64], ... in turn, then the lanes are
+// summed by wave_sum.  Every wave of every block computes it by itself (no
+// LDS, no barrier) and gets the same bits.  The partials were written by an
+// earlier kernel, so the kernel boundary has made them visible.  (lds is not
+// used.)
 __device__ __forceinline__ double sum_prev_partials(const double *part, int n, int stride,
-                                                    int off, double *lds)
+                                                    int off, double *)
 {
+    const int lane = threadIdx.x & (kWave - 1);
     double s = 0.0;
-    for (int b = threadIdx.x; b < n; b += blockDim.x) s += part[(long)b * stride + off];
-    return block_allsum(s, lds);
+    int b = lane;
+    for (; b + 3 * kWave < n; b += 4 * kWave) {
+        const double v0 = part[(long)b * stride + off];
+        const double v1 = part[(long)(b + kWave) * stride + off];
+        const double v2 = part[(long)(b + 2 * kWave) * stride + off];
+        const double v3 = part[(long)(b + 3 * kWave) * stride + off];
+        s += v0;
+        s += v1;
+        s += v2;
+        s += v3;
+    }
+    for (; b < n; b += kWave) s += part[(long)b * stride + off];
+    return wave_sum(s);
 }
 
 }  // namespace qed2d

@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, const float2 *__restric
     bool fresh;
     if (!cg_k1_scalars<true>(S, Prr, Pb, nrr, nbb, k, p.blk, lds, beta, &fresh)) return;
     float2 *pold = rd ? P1 : P0;
-    EpiK1v epi{w ? P1 : P0, t, x, pold, (float)S->alpha_x, !fresh, L.V, 0.0, 0.0};
+    EpiK1v epi{w ? P1 : P0, t, x, (float)S->alpha_x, !fresh, L.V, 0.0, 0.0};
     if (p.active) {
         SrcXpay2 src{r, pold, (float)beta, L.V};
         q_march2<R>(L, U, mass2, src, epi, p.i1, p.r0);
