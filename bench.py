@@ -188,7 +188,9 @@ def main():
             "note": "1024^2 working set fits the 256 MB Infinity Cache: cache-assisted"}
     kern = {}
     for k, (n, ms) in prof_mx.items():
-        if n and k in BYTES_PER_SITE:
+        # (the FP64 CG classes of a mixed run are the acceptance checks of the
+        # last MD solve, which mostly find it converged: no per-iteration figure)
+        if n and k in BYTES_PER_SITE and not (k.startswith("cg_") and k.endswith("_f64")):
             a = BYTES_PER_SITE[k] * V / (ms / units[k] / 1000.0) / 1e9
             kern[k] = {"avg_us": round(1000 * ms / units[k], 2), "GBs": round(a, 1),
                        "launches": n}

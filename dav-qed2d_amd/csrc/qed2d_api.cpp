@@ -88,6 +88,10 @@ struct qed2d_ctx {
     long prof_n[QED2D_PROF_NCLASS] = {};
 };
 
+namespace qed2d {
+thread_local ProfArm g_prof_arm{nullptr, nullptr};
+}
+
 namespace {
 
 int fail(qed2d_ctx *c, int code, const char *fmt, ...)
@@ -220,11 +224,47 @@ void prof_harvest(qed2d_ctx *c)
     c->pending.clear();
 }
 
-#define PROF(c, cls, launch)                 \
-    do {                                     \
-        hipEvent_t ev_ = prof_begin(c, cls); \
-        launch;                              \
-        prof_end((c), (cls), ev_);           \
+// Arms the kernel-exact timing of the next launch of class cls (when this
+// launch is sampled); returns false otherwise.
+bool prof_arm(qed2d_ctx *c, int cls, hipEvent_t *a, hipEvent_t *b)
+{
+    if (!c->prof) return false;
+    if ((c->prof_seen[cls]++ % c->prof) != 0) return false;
+    *a = prof_event(c);
+    *b = prof_event(c);
+    if (!*a || !*b) {
+        if (*a) c->evpool.push_back(*a);
+        if (*b) c->evpool.push_back(*b);
+        return false;
+    }
+    g_prof_arm = ProfArm{*a, *b};
+    return true;
+}
+
+// After the armed launch: keeps the events for harvesting if a kernel took
+// them, returns them to the pool otherwise.
+void prof_fire(qed2d_ctx *c, int cls, hipEvent_t a, hipEvent_t b)
+{
+    if (g_prof_arm.a == a) {
+        g_prof_arm = ProfArm{nullptr, nullptr};
+        c->evpool.push_back(a);
+        c->evpool.push_back(b);
+        return;
+    }
+    c->pending.push_back({cls, a, b});
+}
+
+// Times one kernel of class cls by its execution (start and end events of
+// the first kernel the launch makes).
+#define PROF(c, cls, launch)                              \
+    do {                                                  \
+        hipEvent_t pa_ = nullptr, pb_ = nullptr;          \
+        if (prof_arm((c), (cls), &pa_, &pb_)) {           \
+            launch;                                       \
+            prof_fire((c), (cls), pa_, pb_);              \
+        } else {                                          \
+            launch;                                       \
+        }                                                 \
     } while (0)
 
 int read_state(qed2d_ctx *c)

@@ -123,7 +123,7 @@ void launch_b_force_mom(const BChain *ch, int nchain, long vtot, const double2 *
 {
     long nb = (vtot + kBlock - 1) / kBlock;
     if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL(kb_force_mom, dim3((int)nb), dim3(kBlock), 0, st, ch, nchain, vtot, U, X,
+    launch_k(kb_force_mom, dim3((int)nb), dim3(kBlock), 0, st, ch, nchain, vtot, U, X,
                        Y, mom, eps);
 }
 
@@ -171,10 +171,10 @@ void launch_b_cg_init(const BGeo &g, int rows, const double2 *b, double2 *x, voi
                       float2 *xs, int mixed, BParts P, hipStream_t st)
 {
     if (mixed)
-        hipLaunchKernelGGL(kb_cg_init<float>, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, b,
+        launch_k(kb_cg_init<float>, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, b,
                            x, (float2 *)r, (float2 *)p0, xs, P);
     else
-        hipLaunchKernelGGL(kb_cg_init<double>, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, b,
+        launch_k(kb_cg_init<double>, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, b,
                            x, (double2 *)r, (double2 *)p0, (float2 *)nullptr, P);
 }
 
@@ -260,7 +260,7 @@ void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, floa
 {
     long nb = (2 * vtot + kBlock - 1) / kBlock;
     if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL(kb_ru_accum, dim3((int)nb), dim3(kBlock), 0, st, ch, nchain, 2 * vtot, x,
+    launch_k(kb_ru_accum, dim3((int)nb), dim3(kBlock), 0, st, ch, nchain, 2 * vtot, x,
                        xs, p0, p1, S, mask);
 }
 
@@ -406,11 +406,11 @@ void launch_b_cg_k1_f64(const BGeo &g, int rows, const double2 *U, const double2
 {
     dim3 grid(g.nblocks), blk(kBlock);
     switch (rows) {
-    case 1: hipLaunchKernelGGL((kb_cg1<double, 1>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
-    case 2: hipLaunchKernelGGL((kb_cg1<double, 2>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
-    case 4: hipLaunchKernelGGL((kb_cg1<double, 4>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
-    case 8: hipLaunchKernelGGL((kb_cg1<double, 8>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
-    default: hipLaunchKernelGGL((kb_cg1<double, 16>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
+    case 1: launch_k((kb_cg1<double, 1>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
+    case 2: launch_k((kb_cg1<double, 2>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
+    case 4: launch_k((kb_cg1<double, 4>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
+    case 8: launch_k((kb_cg1<double, 8>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
+    default: launch_k((kb_cg1<double, 16>), grid, blk, 0, st, g, U, r, p0, p1, t, S, P); break;
     }
 }
 void launch_b_cg_k2_f64(const BGeo &g, int rows, const double2 *U, const double2 *t,
@@ -419,11 +419,11 @@ void launch_b_cg_k2_f64(const BGeo &g, int rows, const double2 *U, const double2
 {
     dim3 grid(g.nblocks), blk(kBlock);
     switch (rows) {
-    case 1: hipLaunchKernelGGL((kb_cg2<double, 1>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
-    case 2: hipLaunchKernelGGL((kb_cg2<double, 2>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
-    case 4: hipLaunchKernelGGL((kb_cg2<double, 4>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
-    case 8: hipLaunchKernelGGL((kb_cg2<double, 8>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
-    default: hipLaunchKernelGGL((kb_cg2<double, 16>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
+    case 1: launch_k((kb_cg2<double, 1>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
+    case 2: launch_k((kb_cg2<double, 2>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
+    case 4: launch_k((kb_cg2<double, 4>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
+    case 8: launch_k((kb_cg2<double, 8>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
+    default: launch_k((kb_cg2<double, 16>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
     }
 }
 void launch_b_cg_k1_f32(const BGeo &g, int rows, const float2 *U, const float2 *r, float2 *p0,
@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(kBlock) kb_ms_init(BGeo g, int rows, double2 *
 void launch_b_ms_init(const BGeo &g, int rows, double2 *r, double2 *p0,
                       const MultiShiftState *M, BParts P, hipStream_t st)
 {
-    hipLaunchKernelGGL(kb_ms_init, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, r, p0, M, P);
+    launch_k(kb_ms_init, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, r, p0, M, P);
 }
 
 template <int R>

@@ -2,6 +2,7 @@
 // translation unit (qed2d_kernels.hip) and the orchestration / C-ABI layer.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -11,6 +12,29 @@
 #include "qed2d_device.h"
 
 namespace qed2d {
+
+// Kernel-class profiling (qed2d_profile_*): when armed, the next launch made
+// through launch_k records these events at the start and at the end of the
+// kernel's execution, not around its dispatch.  Host side, per thread.
+struct ProfArm {
+    hipEvent_t a, b;
+};
+extern thread_local ProfArm g_prof_arm;
+
+// Every kernel of the library is launched through this.
+template <typename K, typename... Args>
+inline void launch_k(K kernel, const dim3 &grid, const dim3 &block, uint32_t shmem,
+                     hipStream_t st, Args... args)
+{
+    if (g_prof_arm.a) {
+        const ProfArm e = g_prof_arm;
+        g_prof_arm = ProfArm{nullptr, nullptr};
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, e.a, e.b, 0u, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, st, args...);
+    }
+}
+
 
 // Device-resident solver state.  No kernel synchronises the grid: each
 // kernel writes one partial sum per block, and the next kernel (after the

@@ -111,11 +111,11 @@ static void q_apply_impl(const Latt &L, const c2<T> *U, const c2<T> *si, c2<T> *
     const T mass2 = T(2) + am0;
     dim3 grid(g.nblocks()), blk(kBlock);
     switch (rows) {
-    case 1: hipLaunchKernelGGL((k_q_apply<T, 1>), grid, blk, 0, st, L, U, si, so, mass2, g.bx, g.by, g.nbx); break;
-    case 2: hipLaunchKernelGGL((k_q_apply<T, 2>), grid, blk, 0, st, L, U, si, so, mass2, g.bx, g.by, g.nbx); break;
-    case 4: hipLaunchKernelGGL((k_q_apply<T, 4>), grid, blk, 0, st, L, U, si, so, mass2, g.bx, g.by, g.nbx); break;
-    case 8: hipLaunchKernelGGL((k_q_apply<T, 8>), grid, blk, 0, st, L, U, si, so, mass2, g.bx, g.by, g.nbx); break;
-    default: hipLaunchKernelGGL((k_q_apply<T, 16>), grid, blk, 0, st, L, U, si, so, mass2, g.bx, g.by, g.nbx); break;
+    case 1: launch_k((k_q_apply<T, 1>), grid, blk, 0, st, L, U, si, so, mass2, g.bx, g.by, g.nbx); break;
+    case 2: launch_k((k_q_apply<T, 2>), grid, blk, 0, st, L, U, si, so, mass2, g.bx, g.by, g.nbx); break;
+    case 4: launch_k((k_q_apply<T, 4>), grid, blk, 0, st, L, U, si, so, mass2, g.bx, g.by, g.nbx); break;
+    case 8: launch_k((k_q_apply<T, 8>), grid, blk, 0, st, L, U, si, so, mass2, g.bx, g.by, g.nbx); break;
+    default: launch_k((k_q_apply<T, 16>), grid, blk, 0, st, L, U, si, so, mass2, g.bx, g.by, g.nbx); break;
     }
 }
 
@@ -196,11 +196,11 @@ static void q_mrhs_impl(const Latt &L, const c2<T> *U, const c2<T> *si, c2<T> *s
     MarchGrid g = march_grid(L, rows);
     dim3 grid(g.nblocks() * nrhs), blk(kBlock);
     switch (rows) {
-    case 1: hipLaunchKernelGGL((k_q_mrhs<T, 1>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
-    case 2: hipLaunchKernelGGL((k_q_mrhs<T, 2>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
-    case 4: hipLaunchKernelGGL((k_q_mrhs<T, 4>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
-    case 8: hipLaunchKernelGGL((k_q_mrhs<T, 8>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
-    default: hipLaunchKernelGGL((k_q_mrhs<T, 16>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
+    case 1: launch_k((k_q_mrhs<T, 1>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
+    case 2: launch_k((k_q_mrhs<T, 2>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
+    case 4: launch_k((k_q_mrhs<T, 4>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
+    case 8: launch_k((k_q_mrhs<T, 8>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
+    default: launch_k((k_q_mrhs<T, 16>), grid, blk, 0, st, L, U, si, so, nrhs, m2, g.bx, g.by, g.nbx); break;
     }
 }
 void launch_q_apply_mrhs_f64(const Latt &L, const double2 *U, const double2 *si, double2 *so,
@@ -253,11 +253,11 @@ void launch_stream(int which, const double2 *a, const double2 *b, double2 *c, do
 {
     const long nb = (n + kBlock - 1) / kBlock;
     if (which == 0)
-        hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(kBlock), 0, st, a, c, n);
+        launch_k(k_stream_copy, dim3(nb), dim3(kBlock), 0, st, a, c, n);
     else if (which == 1)
-        hipLaunchKernelGGL(k_stream_triad, dim3(nb), dim3(kBlock), 0, st, a, b, c, 0.5, n);
+        launch_k(k_stream_triad, dim3(nb), dim3(kBlock), 0, st, a, b, c, 0.5, n);
     else
-        hipLaunchKernelGGL(k_stream_read, dim3(kStreamReadBlocks), dim3(kBlock), 0, st, a, part,
+        launch_k(k_stream_read, dim3(kStreamReadBlocks), dim3(kBlock), 0, st, a, part,
                            n);
 }
 
@@ -282,7 +282,7 @@ __global__ void k_gamm5(const double2 *__restrict__ si, double2 *__restrict__ so
 }
 void launch_gamm5(const Latt &L, const double2 *si, double2 *so, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gamm5, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, si, so, L.V);
+    launch_k(k_gamm5, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, si, so, L.V);
 }
 
 __global__ void k_axpy_c(double2 *__restrict__ y, const double2 *__restrict__ x, double ar,
@@ -296,7 +296,7 @@ __global__ void k_axpy_c(double2 *__restrict__ y, const double2 *__restrict__ x,
 }
 void launch_axpy_c(double2 *y, const double2 *x, double ar, double ai, long n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_axpy_c, dim3(ew_blocks(n)), dim3(kBlock), 0, st, y, x, ar, ai, n);
+    launch_k(k_axpy_c, dim3(ew_blocks(n)), dim3(kBlock), 0, st, y, x, ar, ai, n);
 }
 
 // y = a y + b x (x may alias y)
@@ -310,7 +310,7 @@ __global__ void k_scale_add(double2 *y, double a, const double2 *x, double b, lo
 }
 void launch_scale_add(double2 *y, double a, const double2 *x, double b, long n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_scale_add, dim3(ew_blocks(n)), dim3(kBlock), 0, st, y, a, x, b, n);
+    launch_k(k_scale_add, dim3(ew_blocks(n)), dim3(kBlock), 0, st, y, a, x, b, n);
 }
 
 __global__ void k_copy(double2 *__restrict__ d, const double2 *__restrict__ s, long n)
@@ -321,7 +321,7 @@ __global__ void k_copy(double2 *__restrict__ d, const double2 *__restrict__ s, l
 }
 void launch_copy(double2 *dst, const double2 *src, long n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_copy, dim3(ew_blocks(n)), dim3(kBlock), 0, st, dst, src, n);
+    launch_k(k_copy, dim3(ew_blocks(n)), dim3(kBlock), 0, st, dst, src, n);
 }
 
 __global__ void k_negate(double *a, long n)
@@ -332,7 +332,7 @@ __global__ void k_negate(double *a, long n)
 }
 void launch_negate(double *a, long n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_negate, dim3(ew_blocks(n)), dim3(kBlock), 0, st, a, n);
+    launch_k(k_negate, dim3(ew_blocks(n)), dim3(kBlock), 0, st, a, n);
 }
 
 
@@ -393,15 +393,15 @@ __global__ void __launch_bounds__(kBlock) k_finish_sum(const double *part, int n
 void launch_norm2(const double2 *a, long n, RedOut *out, double *part, hipStream_t st)
 {
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kBlock), 0, st, a, nullptr, n, part);
-    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBlock), 0, st, part, nb, 2, out);
+    launch_k(k_dot, dim3(nb), dim3(kBlock), 0, st, a, nullptr, n, part);
+    launch_k(k_finish_sum, dim3(1), dim3(kBlock), 0, st, part, nb, 2, out);
 }
 void launch_dot(const double2 *a, const double2 *b, long n, RedOut *out, double *part,
                 hipStream_t st)
 {
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kBlock), 0, st, a, b, n, part);
-    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBlock), 0, st, part, nb, 2, out);
+    launch_k(k_dot, dim3(nb), dim3(kBlock), 0, st, a, b, n, part);
+    launch_k(k_finish_sum, dim3(1), dim3(kBlock), 0, st, part, nb, 2, out);
 }
 
 // Block-wide double-double sum; result in thread 0.
@@ -516,16 +516,16 @@ void launch_dot_dd(const double2 *a, const double2 *b, long n, RedOut *out, doub
                    hipStream_t st)
 {
     const int nb = red_blocks(n);
-    hipLaunchKernelGGL(k_dot_dd, dim3(nb), dim3(kBlock), 0, st, a, b, n, part);
-    hipLaunchKernelGGL(k_finish_dd, dim3(1), dim3(kBlock), 0, st, part, nb, 2, out);
+    launch_k(k_dot_dd, dim3(nb), dim3(kBlock), 0, st, a, b, n, part);
+    launch_k(k_finish_dd, dim3(1), dim3(kBlock), 0, st, part, nb, 2, out);
 }
 
 void launch_observables(const Latt &L, const double2 *U, const double *mom, double *plx,
                         double *qdens, RedOut *out, double *part, hipStream_t st)
 {
     const int nb = red_blocks(L.V);
-    hipLaunchKernelGGL(k_observables, dim3(nb), dim3(kBlock), 0, st, L, U, mom, plx, qdens, part);
-    hipLaunchKernelGGL(k_finish_dd, dim3(1), dim3(kBlock), 0, st, part, nb, 3, out);
+    launch_k(k_observables, dim3(nb), dim3(kBlock), 0, st, L, U, mom, plx, qdens, part);
+    launch_k(k_finish_dd, dim3(1), dim3(kBlock), 0, st, part, nb, 3, out);
 }
 // ===========================================================================
 // Forces and link updates
@@ -546,7 +546,7 @@ __global__ void k_tr_dQwdU(Latt L, const double2 *__restrict__ U, const double2 
 void launch_tr_dQwdU(const Latt &L, const double2 *U, const double2 *X, const double2 *Y,
                      double *frc, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_tr_dQwdU, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, frc);
+    launch_k(k_tr_dQwdU, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, frc);
 }
 
 // Scatter form, as the reference kernel (src/QED2dAction.jl:34-53).
@@ -568,7 +568,7 @@ __global__ void k_krnl_force(Latt L, const double2 *__restrict__ U, double beta,
 void launch_krnl_force(const Latt &L, const double2 *U, double beta, double *frc1, double *frc2,
                        hipStream_t st)
 {
-    hipLaunchKernelGGL(k_krnl_force, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, beta, frc1,
+    launch_k(k_krnl_force, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, beta, frc1,
                        frc2);
 }
 
@@ -586,7 +586,7 @@ void launch_force_mom(const Latt &L, const double2 *U, const double2 *X, const d
                       const double *facc, double *mom, double beta, double eps, double cf,
                       hipStream_t st)
 {
-    hipLaunchKernelGGL(k_force_mom, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, facc,
+    launch_k(k_force_mom, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, facc,
                        mom, beta, eps, cf);
 }
 
@@ -612,7 +612,7 @@ __global__ void k_force_acc(Latt L, const double2 *__restrict__ U, const double2
 void launch_force_acc(const Latt &L, const double2 *U, const double2 *X, const double2 *Y,
                       double *facc, double rho, int first, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_force_acc, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, facc,
+    launch_k(k_force_acc, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, facc,
                        rho, first);
 }
 
@@ -629,7 +629,7 @@ __global__ void k_updt_fld(double2 *__restrict__ U, const double *__restrict__ m
 }
 void launch_updt_fld(const Latt &L, double2 *U, const double *mom, double eps, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_updt_fld, dim3(ew_blocks(2 * L.V)), dim3(kBlock), 0, st, U, mom, eps,
+    launch_k(k_updt_fld, dim3(ew_blocks(2 * L.V)), dim3(kBlock), 0, st, U, mom, eps,
                        2 * L.V);
 }
 
@@ -650,7 +650,7 @@ __global__ void k_fill_momenta(double *__restrict__ mom, long V, uint32_t k0, ui
 void launch_fill_momenta(const Latt &L, double *mom, uint32_t k0, uint32_t k1, uint32_t traj,
                          hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fill_momenta, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, mom, L.V, k0,
+    launch_k(k_fill_momenta, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, mom, L.V, k0,
                        k1, traj);
 }
 
@@ -671,7 +671,7 @@ __global__ void k_fill_cgauss(double2 *__restrict__ X, long V, uint32_t k0, uint
 void launch_fill_cgauss(const Latt &L, double2 *X, uint32_t k0, uint32_t k1, uint32_t traj,
                         uint32_t purpose, uint32_t sub0, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fill_cgauss, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, X, L.V, k0, k1,
+    launch_k(k_fill_cgauss, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, X, L.V, k0, k1,
                        traj, purpose, sub0);
 }
 void launch_fill_pseudofermion(const Latt &L, double2 *X, uint32_t k0, uint32_t k1,
@@ -697,7 +697,7 @@ __global__ void k_hot_start(double2 *__restrict__ U, long V, uint32_t k0, uint32
 }
 void launch_hot_start(const Latt &L, double2 *U, uint32_t k0, uint32_t k1, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_hot_start, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, U, L.V, k0, k1);
+    launch_k(k_hot_start, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, U, L.V, k0, k1);
 }
 
 // Lattice doubling (src/QED2dFields.jl:21-50).
@@ -721,7 +721,7 @@ __global__ void k_unfold(Latt L, const double2 *__restrict__ Uin, double2 *__res
 }
 void launch_unfold(const Latt &L, const double2 *Uin, double2 *Uout, int reflect, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_unfold, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, Uin, Uout,
+    launch_k(k_unfold, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, Uin, Uout,
                        reflect);
 }
 

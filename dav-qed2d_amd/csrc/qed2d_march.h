@@ -480,11 +480,11 @@ inline MarchGrid march_grid2(const Latt &L, int rows)
 
 #define DISPATCH_ROWS(rows, KERNEL, ...)                                              \
     switch (rows) {                                                                   \
-    case 1: hipLaunchKernelGGL((KERNEL<1>), __VA_ARGS__); break;                      \
-    case 2: hipLaunchKernelGGL((KERNEL<2>), __VA_ARGS__); break;                      \
-    case 4: hipLaunchKernelGGL((KERNEL<4>), __VA_ARGS__); break;                      \
-    case 8: hipLaunchKernelGGL((KERNEL<8>), __VA_ARGS__); break;                      \
-    default: hipLaunchKernelGGL((KERNEL<16>), __VA_ARGS__); break;                    \
+    case 1: launch_k((KERNEL<1>), __VA_ARGS__); break;                      \
+    case 2: launch_k((KERNEL<2>), __VA_ARGS__); break;                      \
+    case 4: launch_k((KERNEL<4>), __VA_ARGS__); break;                      \
+    case 8: launch_k((KERNEL<8>), __VA_ARGS__); break;                      \
+    default: launch_k((KERNEL<16>), __VA_ARGS__); break;                    \
     }
 
 // Sum of v over the block, returned to every thread.  lds: kBlock/64 + 1 doubles.

@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(kBlock) k_fold(double *part, int n, int stride
 static int fold(double *part, int n, int stride, hipStream_t st)
 {
     if (n <= kFoldMin) return n;
-    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, st, part, n, stride);
+    launch_k(k_fold, dim3(1), dim3(kBlock), 0, st, part, n, stride);
     return 1;
 }
 
@@ -100,7 +100,7 @@ void launch_cg_init_f64(const Latt &L, const double2 *b, double2 *x, double2 *r,
                         CGPartials &P, int rows, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
-    hipLaunchKernelGGL(k_cg_init<double>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, x, r, p0,
+    launch_k(k_cg_init<double>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, x, r, p0,
                        (float2 *)nullptr, P.rr[0], P.b, g.bx, g.by, g.nbx, rows);
     P.nrr[0] = fold(P.rr[0], g.nblocks(), 1, st);
     P.nb = fold(P.b, g.nblocks(), 1, st);
@@ -110,7 +110,7 @@ void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *r
                           float2 *p0, CGPartials &P, int rows, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
-    hipLaunchKernelGGL(k_cg_init<float>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, x, rs, p0,
+    launch_k(k_cg_init<float>, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, x, rs, p0,
                        xs, P.rr[0], P.b, g.bx, g.by, g.nbx, rows);
     P.nrr[0] = fold(P.rr[0], g.nblocks(), 1, st);
     P.nb = fold(P.b, g.nblocks(), 1, st);
@@ -192,7 +192,7 @@ __global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs,
 void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const float2 *p0, const float2 *p1,
                      const SolverState *S, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ru_accum, dim3(ew_grid(2 * L.V)), dim3(kBlock), 0, st, x, xs, p0, p1, S,
+    launch_k(k_ru_accum, dim3(ew_grid(2 * L.V)), dim3(kBlock), 0, st, x, xs, p0, p1, S,
                        2 * L.V);
 }
 
@@ -204,7 +204,7 @@ __global__ void k_to_f32(const double2 *__restrict__ in, float2 *__restrict__ ou
 }
 void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_to_f32, dim3(ew_grid(n)), dim3(kBlock), 0, st, in, out, n);
+    launch_k(k_to_f32, dim3(ew_grid(n)), dim3(kBlock), 0, st, in, out, n);
 }
 
 // ===========================================================================
@@ -365,11 +365,11 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
     MarchGrid g = march_grid(L, rows);
     dim3 grid(g.nblocks()), blk(kBlock);
     switch (rows) {
-    case 1: hipLaunchKernelGGL((k_cg1<T, 1>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
-    case 2: hipLaunchKernelGGL((k_cg1<T, 2>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
-    case 4: hipLaunchKernelGGL((k_cg1<T, 4>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
-    case 8: hipLaunchKernelGGL((k_cg1<T, 8>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
-    default: hipLaunchKernelGGL((k_cg1<T, 16>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    case 1: launch_k((k_cg1<T, 1>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    case 2: launch_k((k_cg1<T, 2>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    case 4: launch_k((k_cg1<T, 4>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    case 8: launch_k((k_cg1<T, 8>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    default: launch_k((k_cg1<T, 16>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
     }
     P.ntt[k & 1] = fold(ptt, g.nblocks(), 2, st);
 }
@@ -402,11 +402,11 @@ static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> 
     MarchGrid g = march_grid(L, rows);
     dim3 grid(g.nblocks()), blk(kBlock);
     switch (rows) {
-    case 1: hipLaunchKernelGGL((k_cg2<T, 1>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
-    case 2: hipLaunchKernelGGL((k_cg2<T, 2>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
-    case 4: hipLaunchKernelGGL((k_cg2<T, 4>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
-    case 8: hipLaunchKernelGGL((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
-    default: hipLaunchKernelGGL((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    case 1: launch_k((k_cg2<T, 1>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    case 2: launch_k((k_cg2<T, 2>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    case 4: launch_k((k_cg2<T, 4>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    case 8: launch_k((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    default: launch_k((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
     }
     P.nrr[k & 1] = fold(prr, g.nblocks(), 1, st);
 }
@@ -517,7 +517,7 @@ void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0, Mu
                     CGPartials &P, int nshift, int rows, hipStream_t st)
 {
     MarchGrid g = march_grid(L, rows);
-    hipLaunchKernelGGL(k_ms_init, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, r, p0, s, P.rr[0],
+    launch_k(k_ms_init, dim3(g.nblocks()), dim3(kBlock), 0, st, L, b, r, p0, s, P.rr[0],
                        nshift, g.bx, g.by, g.nbx, rows);
     P.nrr[0] = fold(P.rr[0], g.nblocks(), 1, st);
 }
