@@ -5,6 +5,7 @@ their outputs, on this package.
     python -m qed2d.run hmc-nf2 -i extra/hmc-nf2.in --savedir runs/a/ --cont 0
     python -m qed2d.run rhmc    -i extra/rhmc.in    --savedir runs/b/ --cont 0
     python -m qed2d.run mc      -i extra/MC.in      --savedir runs/c/
+    python -m qed2d.run master-field -i extra/master_field.in --savedir runs/d/
 
 Outputs in ``savedir`` (main/hmc-nf2.jl:100-147, main/rhmc.jl:107-160):
 
@@ -24,6 +25,18 @@ Outputs in ``savedir`` (main/hmc-nf2.jl:100-147, main/rhmc.jl:107-160):
 ``Run.seed`` 0 draws a random seed, as the reference.  The reference's MC.jl
 writes BDIO files; ``mc`` writes ``statistics.txt`` instead (BDIO is not part of
 this package).
+
+``master-field`` (main/master_field.jl): pure-gauge HMC grown from a cold 16 x 16
+lattice by ``unfold_fld`` doublings (``ntraj`` trajectories at each size) up to
+the target size, then ``measurements`` rounds of (save the per-site densities,
+``ntraj`` trajectories) and a last save.  The reference's BDIO records become
+``measurements.f64``: per save, the plaquette density then the topological-charge
+density, each as L1 rows of L2 little-endian doubles (the reference writes
+``dfoo[i,:]`` for i = 1..L1); ``master_field.json`` describes the file and
+``master_field.txt`` holds, per save, the plaquette mean and error from 16 x 16
+blocks (``err_avg``, main/master_field.jl:43-57) and Q.  The reference seeds
+with 1234 whatever the input says (main/master_field.jl:69-70); this driver uses
+``Run.seed``.
 """
 import argparse
 import json
@@ -49,7 +62,9 @@ def read_options(fname, mode):
          "beta": float(s["Lattice"]["beta"]), "qzero": bool(s["HMC"].get("Qzero", False)),
          "ns": int(s["HMC"]["ns"]), "ntraj": int(s["HMC"]["ntraj"]),
          "nthm": int(s["HMC"].get("nthm", 0))}
-    if mode == "mc":
+    if mode == "master-field":
+        o["measurements"] = int(s["Run"].get("measurements", 0))
+    if mode in ("mc", "master-field"):
         o["eps"] = float(s["HMC"]["eps"])
     else:
         o["tau"] = float(s["HMC"]["tau"])
@@ -66,6 +81,78 @@ def read_options(fname, mode):
     if o["seed"] == 0:
         o["seed"] = random.SystemRandom().randrange(1, 2 ** 63)
     return o
+
+
+def err_avg(dens):
+    """Mean and error of a density over 16 x 16 blocks (main/master_field.jl:43-57):
+    the block averages' mean and their sample standard deviation / 16.  dens is
+    indexed [i2, i1] (this package's layout)."""
+    import numpy as np
+    L2, L1 = dens.shape
+    b1, b2 = L1 // 16, L2 // 16
+    d = np.asarray(dens)[:16 * b2, :16 * b1]
+    blks = d.reshape(16, b2, 16, b1).sum(axis=(1, 3)) / (b1 * b2)
+    return float(blks.mean()), float(blks.std(ddof=1) / 16.0)
+
+
+def master_field(infile, savedir, out=None, start=16):
+    """The master-field run (main/master_field.jl); returns the final links,
+    their lattice parameters and the rows of master_field.txt."""
+    import numpy as np
+    import torch
+    o = read_options(infile, "master-field")
+    out = out or sys.stdout
+    os.makedirs(savedir, exist_ok=True)
+    L1t, L2t = o["size"]
+    if L1t < 128 or L2t < 128:
+        print("warning: lattice is too small for a master field", file=out, flush=True)
+    Q.seed(o["seed"])
+    l1 = l2 = start
+    prm = Q.LattParm((l1, l2), o["beta"])
+    U = Q.cold_links(prm)
+    acc = []
+    while True:
+        print(f"## lattice {l1} x {l2}", file=out, flush=True)
+        for _ in range(o["ntraj"]):
+            Q.HMC(U, o["eps"], o["ns"], acc, prm, None, qzero=o["qzero"])
+        if 2 * l1 > L1t:
+            break
+        U2 = Q.unfold_fld(U, prm)
+        prm.close()
+        l1, l2 = 2 * l1, 2 * l2
+        prm = Q.LattParm((l1, l2), o["beta"])
+        U = U2
+    with open(os.path.join(savedir, "master_field.json"), "w") as f:
+        json.dump({"L1": l1, "L2": l2, "beta": o["beta"], "measurements": o["measurements"],
+                   "records": "per save: plaquette density, then Q density; each L1 rows "
+                              "of L2 float64 (little endian)"}, f)
+    rows = []
+    dens = torch.zeros((l2, l1), dtype=torch.float64, device=U.device)
+    fdat = open(os.path.join(savedir, "measurements.f64"), "wb")
+    with open(os.path.join(savedir, "master_field.txt"), "w") as ft:
+        ft.write("save,plaquette,plaquette_err,top_charge\n")
+
+        def save(j):
+            Q.plaquette(dens, U, prm)
+            d = dens.cpu().numpy()
+            fdat.write(np.ascontiguousarray(d.T).astype("<f8").tobytes())
+            pavg, perr = err_avg(d)
+            Q.qtop(dens, U, prm)
+            fdat.write(np.ascontiguousarray(dens.cpu().numpy().T).astype("<f8").tobytes())
+            qt = Q.Qtop(U, prm)
+            row = f"{j},{pavg!r},{perr!r},{qt!r}\n"
+            ft.write(row)
+            rows.append(row)
+            print(f"measurement {j}: plaquette {pavg:.8f} +/- {perr:.2e}, Q {qt:.6f}", file=out,
+                  flush=True)
+
+        for j in range(1, o["measurements"] + 1):
+            save(j)
+            for _ in range(o["ntraj"]):
+                Q.HMC(U, o["eps"], o["ns"], acc, prm, None, qzero=o["qzero"])
+        save(o["measurements"] + 1)
+    fdat.close()
+    return U, prm, rows
 
 
 def _mass_tag(o, mode):
@@ -162,7 +249,7 @@ def run(mode, infile, savedir, cont=0, save_every=None, maxiter=10000, tol=1e-16
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m qed2d.run")
-    ap.add_argument("mode", choices=["hmc-nf2", "rhmc", "mc"])
+    ap.add_argument("mode", choices=["hmc-nf2", "rhmc", "mc", "master-field"])
     ap.add_argument("-i", required=True, help="input file")
     ap.add_argument("--savedir", default="./", help="directory for statistics, logs, configs")
     ap.add_argument("--cont", type=int, default=0,
@@ -171,6 +258,9 @@ def main(argv=None):
     ap.add_argument("--mixed", action="store_true",
                     help="FP32 MD solves with FP64 reliable updates (hmc-nf2)")
     a = ap.parse_args(argv)
+    if a.mode == "master-field":
+        master_field(a.i, a.savedir)
+        return
     run(a.mode, a.i, a.savedir, a.cont, a.save_every,
         precision=Q.PREC_MIXED if a.mixed else Q.PREC_F64)
 

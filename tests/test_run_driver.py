@@ -84,3 +84,56 @@ def test_rhmc_and_mc_runs(tmp_path):
     assert 0.5 < w < 2.0
     rows = R.run("mc", _write(tmp_path, "m.in", MC_IN), str(tmp_path / "m"), 0)
     assert len(rows) == 3
+
+
+MF_IN = {"Run": {"user": "u", "host": "h", "name": "t", "seed": 21, "measurements": 2},
+         "Lattice": {"size": [64, 64], "beta": 11.25},
+         "HMC": {"ns": 4, "eps": 0.05, "ntraj": 1, "Qzero": False}}
+
+
+def test_err_avg_blocks():
+    """err_avg (main/master_field.jl:43-57): 16 x 16 block averages, their mean
+    and sample standard deviation / 16; a field constant on each block with
+    block (b1, b2) = b1 + 16 b2 has mean 127.5 and the spread of 0..255."""
+    import numpy as np
+    from qed2d import run as R
+    L1, L2 = 64, 48
+    d = np.zeros((L2, L1))
+    for i2 in range(L2):
+        for i1 in range(L1):
+            d[i2, i1] = (i1 // (L1 // 16)) + 16 * (i2 // (L2 // 16))
+    m, e = R.err_avg(d)
+    assert m == 127.5
+    assert abs(e - np.std(np.arange(256.0), ddof=1) / 16) < 1e-12
+    m, e = R.err_avg(np.full((32, 32), 0.75))
+    assert m == 0.75 and e == 0.0
+
+
+def test_master_field_options(tmp_path):
+    from qed2d import run as R
+    o = R.read_options(_write(tmp_path, "mf.in", MF_IN), "master-field")
+    assert o["measurements"] == 2 and o["eps"] == 0.05 and o["ns"] == 4 and o["size"] == (64, 64)
+
+
+@pytest.mark.gpu
+def test_master_field_run(tmp_path, O):
+    """Growth 16 -> 32 -> 64 by unfold_fld, two measurement rounds and a last
+    save: file sizes and layout, and the last saved densities against the
+    oracle's plaquette and charge densities of the final links."""
+    import numpy as np
+    from qed2d import run as R
+    out = str(tmp_path / "mf")
+    U, prm, rows = R.master_field(_write(tmp_path, "mf.in", MF_IN), out)
+    assert prm.iL == (64, 64) and tuple(U.shape) == (2, 64, 64)
+    meta = json.load(open(os.path.join(out, "master_field.json")))
+    assert meta["L1"] == 64 and meta["measurements"] == 2
+    raw = np.fromfile(os.path.join(out, "measurements.f64"), dtype="<f8")
+    assert raw.size == 3 * 2 * 64 * 64
+    last = raw[-2 * 64 * 64:].reshape(2, 64, 64)  # [density][i1][i2]
+    Uh = U.cpu().numpy()
+    assert np.max(np.abs(last[0].T - O.plaquette_field(Uh))) < 1e-13
+    assert np.max(np.abs(last[1].T - O.qtop_field(Uh))) < 1e-13
+    lines = open(os.path.join(out, "master_field.txt")).read().splitlines()
+    assert lines[0] == "save,plaquette,plaquette_err,top_charge" and len(lines) == 4
+    m, _ = R.err_avg(last[0].T)
+    assert abs(float(lines[-1].split(",")[1]) - m) < 1e-15
