@@ -425,13 +425,13 @@ int upload_state(qed2d_ctx *c, const SolveArgs &a)
 // update) from the convergence rate seen so far.  A solve stops at most a
 // few queued iterations early; the rest of the chunk returns at once.
 struct ChunkPredictor {
-    double err0 = 0.0;
+    double err0 = 0.0, rate = 0.0;
     int it0 = -1;
     int next(double err, int iter, double tol, double ru_target)
     {
         int chunk = 8;
         if (it0 >= 0 && iter > it0 && err > 0.0 && err0 > err) {
-            const double rate = std::log(err / err0) / (double)(iter - it0);  // < 0
+            rate = std::log(err / err0) / (double)(iter - it0);  // < 0
             double target = tol;
             if (ru_target > target) target = ru_target;
             const double left = std::log(target / err) / rate;
@@ -440,6 +440,21 @@ struct ChunkPredictor {
         }
         it0 = iter;
         err0 = err;
+        if (chunk < 2) chunk = 2;
+        if (chunk > 64) chunk = 64;
+        return chunk;
+    }
+    // The chunk after a reliable update at |r|^2 = err: 2 (K1 decides at once
+    // whether the true residual converged) when the update was triggered near
+    // the tolerance or no rate is known yet, else the iterations to the next
+    // update (|r|^2 down by delta2) or to the tolerance, whichever comes first.
+    int after_ru(double err, double tol, double delta2) const
+    {
+        if (!(rate < 0.0) || !(err > 100.0 * tol)) return 2;
+        double drop = tol / err;
+        if (delta2 > drop) drop = delta2;
+        const double left = std::log(drop) / rate;
+        int chunk = (left > 0.0 && left < 1e6) ? (int)std::ceil(left) + 1 : 2;
         if (chunk < 2) chunk = 2;
         if (chunk > 64) chunk = 64;
         return chunk;
@@ -541,7 +556,8 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
                             nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, k, 2, c->rows,
                             st);
             prof_end(c, QED2D_PROF_RELIABLE, ev);
-            chunk = 2;  // K1(k) decides at once whether the true residual converged
+            const SolverState &h = *c->hstate;
+            chunk = pred.after_ru(h.err, h.tol, h.delta2);
             continue;
         }
         const SolverState &h = *c->hstate;

@@ -170,13 +170,13 @@ int read_states(qed2d_batch *b)
 }
 
 struct Predict {  // the single-lattice solvers' chunk predictor, per chain
-    double err0 = 0.0;
+    double err0 = 0.0, rate = 0.0;
     int it0 = -1;
     int next(double err, int iter, double tol, double ru_target)
     {
         int chunk = 8;
         if (it0 >= 0 && iter > it0 && err > 0.0 && err0 > err) {
-            const double rate = std::log(err / err0) / (double)(iter - it0);
+            rate = std::log(err / err0) / (double)(iter - it0);
             double target = tol;
             if (ru_target > target) target = ru_target;
             const double left = std::log(target / err) / rate;
@@ -185,6 +185,17 @@ struct Predict {  // the single-lattice solvers' chunk predictor, per chain
         }
         it0 = iter;
         err0 = err;
+        if (chunk < 2) chunk = 2;
+        if (chunk > 64) chunk = 64;
+        return chunk;
+    }
+    int after_ru(double err, double tol, double delta2) const
+    {
+        if (!(rate < 0.0) || !(err > 100.0 * tol)) return 2;
+        double drop = tol / err;
+        if (delta2 > drop) drop = delta2;
+        const double left = std::log(drop) / rate;
+        int chunk = (left > 0.0 && left < 1e6) ? (int)std::ceil(left) + 1 : 2;
         if (chunk < 2) chunk = 2;
         if (chunk > 64) chunk = 64;
         return chunk;
@@ -263,6 +274,8 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             running = true;
             if (h.need_ru) {
                 ru = true;
+                const int p = pred[c].after_ru(h.err, h.tol, h.delta2);
+                if (p < next) next = p;
                 continue;
             }
             const double rut = mixed ? h.delta2 * h.rmax2[(h.iter + 1) & 1] : 0.0;
@@ -283,8 +296,6 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, nullptr, b->rs, nullptr, b->S,
                               b->P, 2, b->mask, st);
             BTRY(blaunched(b));
-            chunk = 2;
-            continue;
         }
         chunk = next;
     }
