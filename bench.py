@@ -35,11 +35,11 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_SITE = {
     # p = r + beta p_old (reads r, p_old), t = Q p (reads U), writes p, t; the
     # FP32 site-pair K1 also applies the previous iteration's x += a p_old
-    # (x read+write)
-    "cg_k1_f64": 32 * 5, "cg_k1_f32": 16 * 7,
+    # (x read+write) and reads its U(1) links packed, one float each (8 B/site)
+    "cg_k1_f64": 32 * 5, "cg_k1_f32": 16 * 6 + 8,
     # Ap = Q t (+ mu^2 p) (reads t, U), r -= a Ap (r read+write); FP64 also
-    # x += a p (reads p, x read+write)
-    "cg_k2_f64": 32 * 7, "cg_k2_f32": 16 * 4,
+    # x += a p (reads p, x read+write); FP32: packed links
+    "cg_k2_f64": 32 * 7, "cg_k2_f32": 16 * 3 + 8,
     # so = Q si: si, U read, so written
     "q_f64": 96,
     # mom += eps (F_gauge + F_fermion(X, Q X)): U, X, Y read, mom read+write

@@ -515,33 +515,55 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     TRY(get_fspin(c, FS_X, &xs));
     TRY(get_fspin(c, FS_U, &Us));
     TRY(get_spin(c, S_T, &t));
-    TRY(upload_state(c, a));
     const Latt &L = c->L;
     hipStream_t st = c->stream;
     const float am0f = (float)a.am0;
-    launch_to_f32((const double2 *)U, Us, 2 * L.V, st);
-    if (a.guess) {
-        q64(c, U, x, (double *)t, a.am0);
-        launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x, nullptr,
-                        rs, nullptr, P[0], a.am0, c->state, c->P, 1, 1, c->rows, st);
-        HIPCHK(c, hipMemsetAsync(xs, 0, sizeof(float2) * 2 * L.V, st));
-    } else {
-        launch_cg_init_mixed(L, (const double2 *)b, (double2 *)x, rs, xs, P[0], c->P, c->rows, st);
-    }
+    // The site-pair kernels read the links packed (one float each) when they
+    // are U(1); the first look at the state tells, and a solve whose links are
+    // not restarts with FP32 complex links (x is untouched until then).
+    bool packed = pair_ok(L);
     int k = 1;
     int chunk = 8;
     ChunkPredictor pred;
+    auto start = [&]() -> int {
+        TRY(upload_state(c, a));
+        if (packed)
+            launch_link_pack((const double2 *)U, (float *)Us, 2 * L.V, c->state, st);
+        else
+            launch_to_f32((const double2 *)U, Us, 2 * L.V, st);
+        if (a.guess) {
+            q64(c, U, x, (double *)t, a.am0);
+            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
+                            nullptr, rs, nullptr, P[0], a.am0, c->state, c->P, 1, 1, c->rows, st);
+            HIPCHK(c, hipMemsetAsync(xs, 0, sizeof(float2) * 2 * L.V, st));
+        } else {
+            launch_cg_init_mixed(L, (const double2 *)b, (double2 *)x, rs, xs, P[0], c->P, c->rows,
+                                 st);
+        }
+        k = 1;
+        chunk = 8;
+        pred = ChunkPredictor();
+        return QED2D_OK;
+    };
+    TRY(start());
+    bool first_look = true;
     for (;;) {
         for (int j = 0; j < chunk; ++j, ++k) {
             PROF(c, QED2D_PROF_CG_K1_F32,
-                 launch_cg_k1_f32(L, Us, rs, P[0], P[1], ts, xs, am0f, c->state, c->P, k,
+                 launch_cg_k1_f32(L, Us, packed, rs, P[0], P[1], ts, xs, am0f, c->state, c->P, k,
                                   c->rows32, st));
             PROF(c, QED2D_PROF_CG_K2_F32,
-                 launch_cg_k2_f32(L, Us, ts, P[0], P[1], xs, rs, am0f, a.mu2 != 0.0, c->state,
-                                  c->P, k, c->rows32, st));
+                 launch_cg_k2_f32(L, Us, packed, ts, P[0], P[1], xs, rs, am0f, a.mu2 != 0.0,
+                                  c->state, c->P, k, c->rows32, st));
         }
         TRY(launched(c));
         TRY(read_state(c));
+        if (first_look && packed && c->hstate->bad_links) {
+            packed = false;
+            TRY(start());
+            continue;
+        }
+        first_look = false;
         if (c->hstate->status != 0) break;
         if (c->hstate->need_ru) {
             // K1(k) paused: fold the FP32 partial solution into x, recompute the

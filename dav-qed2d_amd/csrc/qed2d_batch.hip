@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2(BGeo g, const c2<T> *__restrict
 }
 
 template <int R>
-__global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, const float2 *__restrict__ U,
+__global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, LinksF32 U,
                                                   const float2 *__restrict__ r, float2 *P0,
                                                   float2 *P1, float2 *__restrict__ t,
                                                   float2 *__restrict__ x, SolverState *S,
@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, const float2 *__restri
     EpiK1v epi{(wp ? P1 : P0) + o, t + o, x + o, (float)Sc->alpha_x, !fresh, w.L.V, 0.0, 0.0};
     if (w.p.active) {
         SrcXpay2 src{r + o, pold, (float)beta, w.L.V};
-        q_march2<R>(w.L, U + w.uo, 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
+        q_march2x<R>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
     }
     const double a = block_allsum(epi.tt, lds);
     const double b = block_allsum(epi.pp, lds);
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, const float2 *__restri
 }
 
 template <int R>
-__global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, const float2 *__restrict__ U,
+__global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, LinksF32 U,
                                                   const float2 *__restrict__ t, const float2 *P0,
                                                   const float2 *P1, float2 *__restrict__ r,
                                                   SolverState *S, BParts P)
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, const float2 *__restri
     EpiK2v<false> epi{(wp ? P1 : P0) + o, r + o, (float)alpha, 0.0f, w.L.V, 0.0};
     if (w.p.active) {
         SrcPlain2 src{t + o, w.L.V};
-        q_march2<R>(w.L, U + w.uo, 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
+        q_march2x<R>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
     }
     const double a = block_allsum(epi.rr, lds);
     if (threadIdx.x == 0) {
@@ -426,19 +426,22 @@ void launch_b_cg_k2_f64(const BGeo &g, int rows, const double2 *U, const double2
     default: launch_k((kb_cg2<double, 16>), grid, blk, 0, st, g, U, t, p0, p1, x, r, S, P); break;
     }
 }
-void launch_b_cg_k1_f32(const BGeo &g, int rows, const float2 *U, const float2 *r, float2 *p0,
-                        float2 *p1, float2 *t, float2 *x, SolverState *S, BParts P,
+void launch_b_cg_k1_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *r,
+                        float2 *p0, float2 *p1, float2 *t, float2 *x, SolverState *S, BParts P,
                         hipStream_t st)
 {
-    DISPATCH_ROWS(rows, kb_cg1v, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, r, p0, p1, t, x, S,
+    const LinksF32 lk{U, packed};
+    DISPATCH_ROWS(rows, kb_cg1v, dim3(g.nblocks), dim3(kBlock), 0, st, g, lk, r, p0, p1, t, x, S,
                   P);
 }
 // (x is updated by the next K1 or the reliable update's flush)
-void launch_b_cg_k2_f32(const BGeo &g, int rows, const float2 *U, const float2 *t,
+void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *t,
                         const float2 *p0, const float2 *p1, float2 *, float2 *r,
                         SolverState *S, BParts P, hipStream_t st)
 {
-    DISPATCH_ROWS(rows, kb_cg2v, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, p0, p1, r, S, P);
+    const LinksF32 lk{U, packed};
+    DISPATCH_ROWS(rows, kb_cg2v, dim3(g.nblocks), dim3(kBlock), 0, st, g, lk, t, p0, p1, r, S,
+                  P);
 }
 
 // ---------------------------------------------------------------------------

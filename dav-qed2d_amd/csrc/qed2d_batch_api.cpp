@@ -213,16 +213,6 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     BTRY(balloc(b, &b->p0, nel));
     BTRY(balloc(b, &b->p1, nel));
     BTRY(balloc(b, &b->t, nel));
-    for (int c = 0; c < b->n; ++c) {
-        SolverState &s = b->hS[c];
-        std::memset(&s, 0, sizeof s);
-        s.eps = tol;
-        s.maxiter = maxiter;
-        s.delta2 = b->ru_delta * b->ru_delta;
-        s.kk = 1;
-        s.k2 = 1;
-    }
-    BHIP(b, hipMemcpyAsync(b->S, b->hS, sizeof(SolverState) * b->n, hipMemcpyHostToDevice, st));
     BGeo g64, gq;
     BTRY(get_geo(b, b->rows, false, &g64));
     BTRY(get_geo(b, b->rows_q, false, &gq));
@@ -240,22 +230,50 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         BTRY(balloc(b, &b->ts, nel));
         BTRY(balloc(b, &b->xs, nel));
         BTRY(get_geo(b, b->rows32, true, &g32));
-        launch_to_f32(b->U, b->Us, b->nU, st);
-        launch_b_cg_init(g64, b->rows, src, x, b->rs, b->p0s, b->xs, 1, b->P, st);
-    } else {
-        launch_b_cg_init(g64, b->rows, src, x, b->r, b->p0, nullptr, 0, b->P, st);
     }
-    BTRY(blaunched(b));
-    std::vector<Predict> pred(b->n);
+    // FP32 links packed (one float each) when every link is U(1), as the
+    // single-lattice solver; the first look at the states tells, and a solve
+    // whose links are not restarts with FP32 complex links.
+    bool packed = mixed;
+    std::vector<Predict> pred;
     int chunk = 8;
     long launched_k = 0;
+    auto start = [&]() -> int {
+        for (int c = 0; c < b->n; ++c) {
+            SolverState &s = b->hS[c];
+            std::memset(&s, 0, sizeof s);
+            s.eps = tol;
+            s.maxiter = maxiter;
+            s.delta2 = b->ru_delta * b->ru_delta;
+            s.kk = 1;
+            s.k2 = 1;
+        }
+        BHIP(b, hipMemcpyAsync(b->S, b->hS, sizeof(SolverState) * b->n, hipMemcpyHostToDevice,
+                               st));
+        if (mixed) {
+            if (packed)
+                launch_link_pack(b->U, (float *)b->Us, b->nU, b->S, st);
+            else
+                launch_to_f32(b->U, b->Us, b->nU, st);
+            launch_b_cg_init(g64, b->rows, src, x, b->rs, b->p0s, b->xs, 1, b->P, st);
+        } else {
+            launch_b_cg_init(g64, b->rows, src, x, b->r, b->p0, nullptr, 0, b->P, st);
+        }
+        BTRY(blaunched(b));
+        pred.assign(b->n, Predict());
+        chunk = 8;
+        launched_k = 0;
+        return QED2D_OK;
+    };
+    BTRY(start());
+    bool first_look = true;
     for (;;) {
         for (int j = 0; j < chunk; ++j) {
             if (mixed) {
-                launch_b_cg_k1_f32(g32, b->rows32, b->Us, b->rs, b->p0s, b->p1s, b->ts, b->xs,
-                                   b->S, b->P, st);
-                launch_b_cg_k2_f32(g32, b->rows32, b->Us, b->ts, b->p0s, b->p1s, b->xs, b->rs,
-                                   b->S, b->P, st);
+                launch_b_cg_k1_f32(g32, b->rows32, b->Us, packed, b->rs, b->p0s, b->p1s, b->ts,
+                                   b->xs, b->S, b->P, st);
+                launch_b_cg_k2_f32(g32, b->rows32, b->Us, packed, b->ts, b->p0s, b->p1s, b->xs,
+                                   b->rs, b->S, b->P, st);
             } else {
                 launch_b_cg_k1_f64(g64, b->rows, b->U, b->r, b->p0, b->p1, b->t, b->S, b->P, st);
                 launch_b_cg_k2_f64(g64, b->rows, b->U, b->t, b->p0, b->p1, x, b->r, b->S, b->P,
@@ -265,6 +283,12 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         launched_k += chunk;
         BTRY(blaunched(b));
         BTRY(read_states(b));
+        if (first_look && packed && b->hS[0].bad_links) {
+            packed = false;
+            BTRY(start());
+            continue;
+        }
+        first_look = false;
         bool running = false, ru = false;
         int next = 64;
         for (int c = 0; c < b->n; ++c) {

@@ -13,6 +13,19 @@
 
 namespace qed2d {
 
+// FP32 links of the site-pair solver kernels: FP32 complex (float2 per link)
+// or packed (one float per link, k_link_pack); at(o) is the field from link o.
+struct LinksF32 {
+    const void *U;
+    int packed;
+    __host__ __device__ LinksF32 at(long o) const
+    {
+        return LinksF32{packed ? (const void *)((const float *)U + o)
+                               : (const void *)((const float2 *)U + o),
+                        packed};
+    }
+};
+
 // Kernel-class profiling (qed2d_profile_*): when armed, the next launch made
 // through launch_k records these events at the start and at the end of the
 // kernel's execution, not around its dispatch.  Host side, per thread.
@@ -60,6 +73,7 @@ struct SolverState {
     // FP32 site-pair iterations: K2(k) leaves x += alpha_k p_k to K1(k+1), which
     // reads p_k anyway; alpha_x is that pending alpha_k (written by K2's block 0)
     double alpha_x;
+    int bad_links;  // set by launch_link_pack: the packed FP32 links are not usable
 };
 
 // Multi-shift CG state (all shifts sigma_i = mu_i^2 of (Q^2 + sigma_i) x_i = b on
@@ -177,10 +191,10 @@ void launch_b_cg_k1_f64(const BGeo &g, int rows, const double2 *U, const double2
 void launch_b_cg_k2_f64(const BGeo &g, int rows, const double2 *U, const double2 *t,
                         const double2 *p0, const double2 *p1, double2 *x, double2 *r,
                         SolverState *S, BParts P, hipStream_t st);
-void launch_b_cg_k1_f32(const BGeo &g, int rows, const float2 *U, const float2 *r, float2 *p0,
-                        float2 *p1, float2 *t, float2 *x, SolverState *S, BParts P,
+void launch_b_cg_k1_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *r,
+                        float2 *p0, float2 *p1, float2 *t, float2 *x, SolverState *S, BParts P,
                         hipStream_t st);
-void launch_b_cg_k2_f32(const BGeo &g, int rows, const float2 *U, const float2 *t,
+void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *t,
                         const float2 *p0, const float2 *p1, float2 *x, float2 *r,
                         SolverState *S, BParts P, hipStream_t st);
 // (each masked chain's pending FP32 x update applied first, as launch_ru_accum)
@@ -238,14 +252,19 @@ void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2
 void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
                       const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
                       CGPartials &P, int k, int rows, hipStream_t st);
-void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
-                      float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P, int k, int rows,
-                      hipStream_t st);
+// U: FP32 complex links, or (packed, site-pair kernels only: even L1 >= 4)
+// links packed by launch_link_pack.
+void launch_cg_k1_f32(const Latt &L, const void *U, bool packed, const float2 *r, float2 *p0,
+                      float2 *p1, float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P,
+                      int k, int rows, hipStream_t st);
 // (mu: the operator has mu2 != 0.  On even L1 >= 4 the site-pair kernels
 // leave x += alpha p to the next K1, see SolverState::alpha_x.)
-void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
-                      const float2 *p1, float2 *x, float2 *r, float am0, bool mu, SolverState *s,
-                      CGPartials &P, int k, int rows, hipStream_t st);
+void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t,
+                      const float2 *p0, const float2 *p1, float2 *x, float2 *r, float am0,
+                      bool mu, SolverState *s, CGPartials &P, int k, int rows, hipStream_t st);
+// n FP64 U(1) links -> one float each (the packed format of the FP32 site-pair
+// solver); sets S->bad_links if a link is not of unit modulus to 1e-6.
+void launch_link_pack(const double2 *U, float *out, long n, SolverState *S, hipStream_t st);
 // A reliable update's x += double(xs), xs = 0; with p0 / p1 given (the FP32 site-pair solver, whose
 // K1 was paused with an x update pending), xs + alpha_x p_{stop_k - 1} first.
 void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const float2 *p0, const float2 *p1,

@@ -345,13 +345,17 @@ __device__ __forceinline__ auto pre_of(const E &e, long s, const Raw &, long) ->
 // adjacent lanes away from the wave's edges; the epilogue gets both sites'
 // outputs and inputs: epi.pair(s, pre, oa1, oa2, ob1, ob2, c1, c2) with s the
 // index of site 2j, (oa, ob) the outputs of sites 2j, 2j+1 for spin
-// components 1, 2 and c1, c2 the inputs.
-template <int R, typename Src, typename Epi>
-__device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict__ U, float mass2,
-                                         const Src &src, Epi &epi, int j, int r0)
+// components 1, 2 and c1, c2 the inputs.  The links come through the policy
+// lk (LinksF2 or LinksPacked): their raw loads travel one row ahead like the
+// spinors' and are decoded where they are used.
+template <int R, typename Lk, typename Src, typename Epi>
+__device__ __forceinline__ void q_march2l(const Latt &L, const Lk &lk, float mass2, const Src &src,
+                                          Epi &epi, int j, int r0)
 {
     using Raw2 = typename Src::Raw2;
     using Raw1 = typename Src::Raw1;
+    using LR2 = typename Lk::Raw2;
+    using LR1 = typename Lk::Raw1;
     using Pre = decltype(pre_of(epi, 0L, Raw2{}, 0));
     const int L1 = L.L1, L2 = L.L2;
     const long V = L.V;
@@ -368,18 +372,18 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
     src.value2(src.fetch2((long)rm * L1 + ia), dn1, dn2);
     const Raw2 wc = src.fetch2((long)r0 * L1 + ia);
     src.value2(wc, c1, cc2);
-    F2x2 U2m = ld2(U, V + (long)rm * L1 + ia);
+    F2x2 U2m = lk.value2(lk.fetch2(V + (long)rm * L1 + ia));
     if (ap && r0 == 0) U2m = neg2(U2m);
     long row = (long)r0 * L1;
     Raw2 wu = src.fetch2((long)((r0 + 1 == L2) ? 0 : r0 + 1) * L1 + ia);
     Raw1 wl{}, wq{};
-    float2 U1lm{};
+    LR1 rU1lm{};
     if (ldL) {
         wl = src.fetch1(row + im);
-        U1lm = U[row + im];
+        rU1lm = lk.fetch1(row + im);
     }
     if (ldR) wq = src.fetch1(row + ip);
-    F2x2 U1 = ld2(U, row + ia), U2 = ld2(U, V + row + ia);
+    LR2 rU1 = lk.fetch2(row + ia), rU2 = lk.fetch2(V + row + ia);
     Pre q = pre_of(epi, row + ia, wc, 0);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -388,8 +392,8 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
         const bool more = (k + 1 < nr);
         Raw2 nwu;
         Raw1 nwl{}, nwq{};
-        F2x2 nU1, nU2;
-        float2 nU1lm{};
+        LR2 nrU1, nrU2;
+        LR1 nrU1lm{};
         Pre nq;
         if (more) {
             const long row1 = (long)(r + 1) * L1;
@@ -397,13 +401,14 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
             nwu = src.fetch2((long)r2 * L1 + ia);
             if (ldL) {
                 nwl = src.fetch1(row1 + im);
-                nU1lm = U[row1 + im];
+                nrU1lm = lk.fetch1(row1 + im);
             }
             if (ldR) nwq = src.fetch1(row1 + ip);
-            nU1 = ld2(U, row1 + ia);
-            nU2 = ld2(U, V + row1 + ia);
+            nrU1 = lk.fetch2(row1 + ia);
+            nrU2 = lk.fetch2(V + row1 + ia);
             nq = pre_of(epi, row1 + ia, wu, 0);
         }
+        const F2x2 U1 = lk.value2(rU1), U2 = lk.value2(rU2);
         F2x2 up1, up2;
         float2 l1, l2, q1, q2;
         src.value2(wu, up1, up2);
@@ -415,7 +420,7 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
         float2 U1l = from_left<float>(U1.a1);
         if (ldL) {
             src.value1(wl, l1, l2);
-            U1l = U1lm;
+            U1l = lk.value1(rU1lm);
         }
         if (ldR) src.value1(wq, q1, q2);
         const F2x2 U2e = (ap && r == L2 - 1) ? neg2(U2) : U2;
@@ -436,13 +441,80 @@ __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict
             wu = nwu;
             wl = nwl;
             wq = nwq;
-            U1 = nU1;
-            U2 = nU2;
-            U1lm = nU1lm;
+            rU1 = nrU1;
+            rU2 = nrU2;
+            rU1lm = nrU1lm;
             q = nq;
         }
     }
 }
+
+// FP32 complex links.
+struct LinksF2 {
+    const float2 *__restrict__ U;
+    using Raw2 = F2x2;
+    using Raw1 = float2;
+    __device__ __forceinline__ F2x2 fetch2(long i) const { return ld2(U, i); }
+    __device__ __forceinline__ float2 fetch1(long i) const { return U[i]; }
+    __device__ __forceinline__ F2x2 value2(const F2x2 &r) const { return r; }
+    __device__ __forceinline__ float2 value1(const float2 &r) const { return r; }
+};
+
+// U(1) links packed into one float each (k_link_pack): the smaller of |Re|,
+// |Im| with the last mantissa bit cleared; bit 30 (always 0 for magnitudes
+// below 2) set when that is Re, the last bit set when the other component is
+// negative.  The other component is sqrt(1 - v^2), v^2 <= 1/2.
+__device__ __forceinline__ float2 link_dec(float w)
+{
+    const uint32_t b = __float_as_uint(w);
+    const float v = __uint_as_float(b & 0xBFFFFFFEu);
+    float o = __builtin_amdgcn_sqrtf(__builtin_fmaf(-v, v, 1.0f));
+    if (b & 1u) o = -o;
+    return (b & 0x40000000u) ? make_float2(v, o) : make_float2(o, v);
+}
+struct LinksPacked {
+    const float *__restrict__ U;
+    using Raw2 = float2;
+    using Raw1 = float;
+    __device__ __forceinline__ float2 fetch2(long i) const
+    {
+        return *reinterpret_cast<const float2 *>(U + i);
+    }
+    __device__ __forceinline__ float fetch1(long i) const { return U[i]; }
+    __device__ __forceinline__ F2x2 value2(const float2 &r) const
+    {
+        F2x2 u;
+        u.a0 = link_dec(r.x);
+        u.a1 = link_dec(r.y);
+        return u;
+    }
+    __device__ __forceinline__ float2 value1(float r) const { return link_dec(r); }
+};
+
+// q_march2l with the links the solver kernels were given.
+template <int R, typename Src, typename Epi>
+__device__ __forceinline__ void q_march2x(const Latt &L, const LinksF32 &u, float mass2,
+                                          const Src &src, Epi &epi, int j, int r0)
+{
+    if (u.packed) {
+        const LinksPacked lk{(const float *)u.U};
+        q_march2l<R>(L, lk, mass2, src, epi, j, r0);
+    } else {
+        const LinksF2 lk{(const float2 *)u.U};
+        q_march2l<R>(L, lk, mass2, src, epi, j, r0);
+    }
+}
+
+// q_march2l with FP32 complex links.
+template <int R, typename Src, typename Epi>
+__device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict__ U, float mass2,
+                                         const Src &src, Epi &epi, int j, int r0)
+{
+    const LinksF2 lk{U};
+    q_march2l<R>(L, lk, mass2, src, epi, j, r0);
+}
+
+
 
 // The apply's output is not read back by this kernel, so its stores stream
 // past the caches (nontemporal).

@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
 
 // ---- FP32 pair kernels (two sites per thread along i1) -------------------
 template <int R>
-__global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, const float2 *__restrict__ U,
+__global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
                                                  const float2 *__restrict__ r, float2 *P0,
                                                  float2 *P1, float2 *__restrict__ t,
                                                  float2 *__restrict__ x, float mass2,
@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, const float2 *__restric
     EpiK1v epi{w ? P1 : P0, t, x, (float)S->alpha_x, !fresh, L.V, 0.0, 0.0};
     if (p.active) {
         SrcXpay2 src{r, pold, (float)beta, L.V};
-        q_march2<R>(L, U, mass2, src, epi, p.i1, p.r0);
+        q_march2x<R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
     const double a = block_allsum(epi.tt, lds);
     const double b = block_allsum(epi.pp, lds);
@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, const float2 *__restric
 }
 
 template <int R, bool MU>
-__device__ __forceinline__ void cg2v_body(const Latt &L, const float2 *__restrict__ U,
+__device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
                                           const float2 *__restrict__ t, const float2 *P0,
                                           const float2 *P1, float2 *__restrict__ r, float mass2,
                                           SolverState *S, const double *Ptt, double *Prr, int ntt,
@@ -322,7 +322,7 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const float2 *__restric
     EpiK2v<MU> epi{w ? P1 : P0, r, (float)alpha, (float)S->mu2, L.V, 0.0};
     if (p.active) {
         SrcPlain2 src{t, L.V};
-        q_march2<R>(L, U, mass2, src, epi, p.i1, p.r0);
+        q_march2x<R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
     const double a = block_allsum(epi.rr, lds);
     if (threadIdx.x == 0) {
@@ -333,7 +333,7 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const float2 *__restric
 #define CG2V_KERNEL(NAME, MU)                                                              \
     template <int R>                                                                       \
     __global__ void __launch_bounds__(kBlock)                                              \
-        NAME(Latt L, const float2 *__restrict__ U, const float2 *__restrict__ t,           \
+        NAME(Latt L, LinksF32 U, const float2 *__restrict__ t,                            \
              const float2 *P0, const float2 *P1, float2 *__restrict__ r, float mass2,      \
              SolverState *S, const double *Ptt, double *Prr, int ntt, int k, int bx, int by, \
              int nbx)                                                                      \
@@ -346,7 +346,7 @@ CG2V_KERNEL(k_cg2vm, true)  // mu2 != 0
 template <typename T>
 static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c2<T> *p1,
                      c2<T> *t, c2<T> *x, T am0, SolverState *S, CGPartials &P, int k, int rows,
-                     hipStream_t st)
+                     hipStream_t st, int packed = 0)
 {
     const T m2 = T(2) + am0;
     double *ptt = P.tt[k & 1];
@@ -356,7 +356,8 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
         if (pair_ok(L)) {
             MarchGrid g = march_grid2(L, rows);
             dim3 grid(g.nblocks()), blk(kBlock);
-            DISPATCH_ROWS(rows, k_cg1v, grid, blk, 0, st, L, U, r, p0, p1, t, x, m2, S, ptt,
+            const LinksF32 lk{U, packed};
+            DISPATCH_ROWS(rows, k_cg1v, grid, blk, 0, st, L, lk, r, p0, p1, t, x, m2, S, ptt,
                           prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx);
             P.ntt[k & 1] = fold(ptt, g.nblocks(), 2, st);
             return;
@@ -377,7 +378,7 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
 template <typename T>
 static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> *p0,
                      const c2<T> *p1, c2<T> *x, c2<T> *r, T am0, SolverState *S,
-                     CGPartials &P, int k, int rows, bool mu, hipStream_t st)
+                     CGPartials &P, int k, int rows, bool mu, hipStream_t st, int packed = 0)
 {
     const T m2 = T(2) + am0;
     const double *ptt = P.tt[k & 1];
@@ -388,12 +389,13 @@ static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> 
             MarchGrid g = march_grid2(L, rows);
             dim3 grid(g.nblocks()), blk(kBlock);
             // x is updated by the next K1 (or the reliable update's flush)
+            const LinksF32 lk{U, packed};
             if (mu) {
-                DISPATCH_ROWS(rows, k_cg2vm, grid, blk, 0, st, L, U, t, p0, p1, r, m2, S, ptt,
+                DISPATCH_ROWS(rows, k_cg2vm, grid, blk, 0, st, L, lk, t, p0, p1, r, m2, S, ptt,
                               prr, ntt, k, g.bx, g.by, g.nbx);
             } else {
-                DISPATCH_ROWS(rows, k_cg2v, grid, blk, 0, st, L, U, t, p0, p1, r, m2, S, ptt, prr,
-                              ntt, k, g.bx, g.by, g.nbx);
+                DISPATCH_ROWS(rows, k_cg2v, grid, blk, 0, st, L, lk, t, p0, p1, r, m2, S, ptt,
+                              prr, ntt, k, g.bx, g.by, g.nbx);
             }
             P.nrr[k & 1] = fold(prr, g.nblocks(), 1, st);
             return;
@@ -423,17 +425,44 @@ void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const d
 {
     cg2_impl<double>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, false, st);
 }
-void launch_cg_k1_f32(const Latt &L, const float2 *U, const float2 *r, float2 *p0, float2 *p1,
-                      float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P, int k,
-                      int rows, hipStream_t st)
+void launch_cg_k1_f32(const Latt &L, const void *U, bool packed, const float2 *r, float2 *p0,
+                      float2 *p1, float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P,
+                      int k, int rows, hipStream_t st)
 {
-    cg1_impl<float>(L, U, r, p0, p1, t, x, am0, s, P, k, rows, st);
+    cg1_impl<float>(L, (const float2 *)U, r, p0, p1, t, x, am0, s, P, k, rows, st, packed);
 }
-void launch_cg_k2_f32(const Latt &L, const float2 *U, const float2 *t, const float2 *p0,
-                      const float2 *p1, float2 *x, float2 *r, float am0, bool mu, SolverState *s,
-                      CGPartials &P, int k, int rows, hipStream_t st)
+void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t,
+                      const float2 *p0, const float2 *p1, float2 *x, float2 *r, float am0,
+                      bool mu, SolverState *s, CGPartials &P, int k, int rows, hipStream_t st)
 {
-    cg2_impl<float>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, mu, st);
+    cg2_impl<float>(L, (const float2 *)U, t, p0, p1, x, r, am0, s, P, k, rows, mu, st, packed);
+}
+
+// U(1) links packed into one float each (the format link_dec reads): the
+// smaller-magnitude component, rounded to a float whose last mantissa bit is
+// 0 (towards the exact value), bit 30 set when it is Re, the last bit set when
+// the other component is negative.  Links further than 1e-6 from |U| = 1 set
+// S->bad_links (the format assumes unit modulus).
+__global__ void k_link_pack(const double2 *__restrict__ in, float *__restrict__ out, long n,
+                            SolverState *S)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        const double2 u = in[i];
+        const bool sw = fabs(u.y) > fabs(u.x);  // store Re
+        const double v = sw ? u.x : u.y, o = sw ? u.y : u.x;
+        const float w = (float)v;
+        uint32_t b = __float_as_uint(w);
+        if (b & 1u) b = (fabs(v) > fabs((double)w)) ? b + 1u : b - 1u;
+        if (sw) b |= 0x40000000u;
+        if (o < 0.0) b |= 1u;
+        out[i] = __uint_as_float(b);
+        if (fabs(u.x * u.x + u.y * u.y - 1.0) > 1e-6) S->bad_links = 1;
+    }
+}
+void launch_link_pack(const double2 *U, float *out, long n, SolverState *S, hipStream_t st)
+{
+    launch_k(k_link_pack, dim3(ew_grid(n)), dim3(kBlock), 0, st, U, out, n, S);
 }
 
 // ===========================================================================

@@ -412,3 +412,32 @@ def test_rhmc_fused_pseudofermions_bit_exact(Q, O, L1, L2, nf):
     for a, b in zip(out[1][0], out[2][0]):
         assert a.hini == b.hini and a.hfin == b.hfin and a.accepted == b.accepted
     assert torch.equal(out[1][1], out[2][1])
+
+
+@pytest.mark.parametrize("unitary", [True, False])
+def test_mixed_cg_packed_links_and_fallback(Q, O, unitary):
+    """The mixed solver's site-pair kernels read U(1) links packed into one
+    float each; links that are not of unit modulus (here: random complex)
+    make the first look at the state restart the solve with FP32 complex
+    links.  Either way the FP64 true residual meets the tolerance and the
+    solution matches the FP64 solve; the batched solver agrees bit for bit."""
+    import torch
+    say("mixed CG packed links, unitary", unitary)
+    L1, L2 = 48, 32
+    rng = np.random.default_rng(9 + unitary)
+    U = random_links(L1, L2, rng, unitary=unitary)
+    b = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 4.0)
+    dU, db = dev(U), dev(b)
+    x1, x0 = Q.spinor(prm), Q.spinor(prm)
+    it1 = Q.CG(x1, dU, db, 0.3, 10000, 1e-22, Q.gamm5Dw_sqr_msq, prm, None, precision=1)
+    it0 = Q.CG(x0, dU, db, 0.3, 10000, 1e-22, Q.gamm5Dw_sqr_msq, prm, None, precision=0)
+    assert _true_res(O, U, host(x1), b, 0.3) < 1e-22
+    assert relerr(host(x1), host(x0)) < 1e-9
+    say("iterations mixed", it1, "fp64", it0)
+    assert it1 < 1.5 * it0 + 20
+    # the same solve through a right-hand-side set of one
+    xs = torch.zeros((1, 2, L2, L1), dtype=torch.complex128, device="cuda")
+    its = Q.CG_mrhs(xs, dU, db.reshape(1, 2, L2, L1).contiguous(), 0.3, 10000, 1e-22, prm, None,
+                    precision=1)
+    assert its[0] == it1 and torch.equal(xs[0], x1)
