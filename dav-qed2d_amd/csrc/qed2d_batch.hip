@@ -138,7 +138,6 @@ __global__ void __launch_bounds__(kBlock) kb_cg_init(BGeo g, int rows,
                                                      c2<T> *__restrict__ p0,
                                                      float2 *__restrict__ xs, BParts P)
 {
-    __shared__ double lds[kBlock / kWave + 1];
     const BWhere w = bwhere<false>(g, rows);
     const Latt &L = w.L;
     double acc = 0.0;
@@ -157,7 +156,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg_init(BGeo g, int rows,
             }
         }
     }
-    const double s = block_allsum(acc, lds);
+    const double s = block_sum_t0(acc);
     if (threadIdx.x == 0) {
         P.rr[0][w.pb + w.p.blk] = s;
         P.b[w.pb + w.p.blk] = s;
@@ -194,7 +193,6 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
                                                       double2 *__restrict__ p0, SolverState *S,
                                                       BParts P, int mode, const int *mask)
 {
-    __shared__ double lds[kBlock / kWave + 1];
     const BWhere w = bwhere<false>(g, R);
     SolverState *Sc = S + w.c;
     if (mode == 2 && !mask[w.c]) return;
@@ -206,8 +204,8 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
         SrcPlain<double> src{t + o, w.L.V};
         q_march<double, R>(w.L, U + w.uo, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
     }
-    const double a = block_allsum(epi.rr, lds);
-    const double bb = block_allsum(epi.bb, lds);
+    double a, bb;
+    block_sum2_t0(epi.rr, epi.bb, a, bb);
     if (threadIdx.x == 0) {
         const int rd = (k - 1) & 1;
         P.rr[rd][w.pb + w.p.blk] = a;
@@ -293,8 +291,8 @@ __global__ void __launch_bounds__(kBlock) kb_cg1(BGeo g, const c2<T> *__restrict
         SrcXpay<T> src{r + o, pold, (T)beta, w.L.V};
         q_march<T, R>(w.L, U + w.uo, T(2) + (T)w.am0, src, epi, w.p.i1, w.p.r0);
     }
-    const double a = block_allsum(epi.tt, lds);
-    const double b = block_allsum(epi.pp, lds);
+    double a, b;
+    block_sum2_t0(epi.tt, epi.pp, a, b);
     if (threadIdx.x == 0) {
         P.tt[wp][2 * (w.pb + w.p.blk)] = a;
         P.tt[wp][2 * (w.pb + w.p.blk) + 1] = b;
@@ -323,7 +321,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2(BGeo g, const c2<T> *__restrict
         SrcPlain<T> src{t + o, w.L.V};
         q_march<T, R>(w.L, U + w.uo, T(2) + (T)w.am0, src, epi, w.p.i1, w.p.r0);
     }
-    const double a = block_allsum(epi.rr, lds);
+    const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) {
         P.rr[wp][w.pb + w.p.blk] = a;
         if (w.p.blk == 0) {
@@ -359,8 +357,8 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, LinksF32 U,
         SrcXpay2 src{r + o, pold, (float)beta, w.L.V};
         q_march2x<R>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
     }
-    const double a = block_allsum(epi.tt, lds);
-    const double b = block_allsum(epi.pp, lds);
+    double a, b;
+    block_sum2_t0(epi.tt, epi.pp, a, b);
     if (threadIdx.x == 0) {
         P.tt[wp][2 * (w.pb + w.p.blk)] = a;
         P.tt[wp][2 * (w.pb + w.p.blk) + 1] = b;
@@ -389,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, LinksF32 U,
         SrcPlain2 src{t + o, w.L.V};
         q_march2x<R>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
     }
-    const double a = block_allsum(epi.rr, lds);
+    const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) {
         P.rr[wp][w.pb + w.p.blk] = a;
         if (w.p.blk == 0) {
@@ -457,7 +455,6 @@ __global__ void __launch_bounds__(kBlock) kb_ms_init(BGeo g, int rows, double2 *
                                                      double2 *__restrict__ p0,
                                                      const MultiShiftState *M, BParts P)
 {
-    __shared__ double lds[kBlock / kWave + 1];
     const BWhere w = bwhere<false>(g, rows);
     const MultiShiftState *Mc = M + w.c;
     const Latt &L = w.L;
@@ -481,7 +478,7 @@ __global__ void __launch_bounds__(kBlock) kb_ms_init(BGeo g, int rows, double2 *
             }
         }
     }
-    const double s = block_allsum(acc, lds);
+    const double s = block_sum_t0(acc);
     if (threadIdx.x == 0) {
         P.rr[0][w.pb + w.p.blk] = s;
         if (w.p.blk == 0) P.nrr[0][w.c] = w.nb;

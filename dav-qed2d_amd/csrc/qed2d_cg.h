@@ -387,8 +387,8 @@ __device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restric
             SrcXpay<double> src{r, pold, beta_j, L.V};
             q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
         }
-        const double a = block_allsum(epi.tt, lds);
-        const double b = block_allsum(epi.pp, lds);
+        double a, b;
+        block_sum2_t0(epi.tt, epi.pp, a, b);
         if (threadIdx.x == 0) {
             Ptt_w[2 * p.blk] = a;
             Ptt_w[2 * p.blk + 1] = b;
@@ -447,7 +447,7 @@ __device__ __forceinline__ void ms2_body(const Latt &L, const double2 *__restric
         SrcPlain<double> src{t, L.V};
         q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
-    const double a = block_allsum(epi.rr, lds);
+    const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) Prr_w[p.blk] = a;
 }
 

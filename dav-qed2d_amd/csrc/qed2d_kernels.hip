@@ -358,7 +358,6 @@ __global__ void __launch_bounds__(kBlock) k_dot(const double2 *__restrict__ a,
                                                 const double2 *__restrict__ b, long n,
                                                 double *part)
 {
-    __shared__ double lds[kBlock / kWave + 1];
     double sr = 0.0, si = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
          i += (long)gridDim.x * blockDim.x) {
@@ -371,8 +370,8 @@ __global__ void __launch_bounds__(kBlock) k_dot(const double2 *__restrict__ a,
             sr += x.x * x.x + x.y * x.y;
         }
     }
-    double br = block_allsum(sr, lds);
-    double bi = block_allsum(si, lds);
+    double br, bi;
+    block_sum2_t0(sr, si, br, bi);
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = br;
         part[2 * blockIdx.x + 1] = bi;

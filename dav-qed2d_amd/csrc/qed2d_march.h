@@ -559,25 +559,38 @@ inline MarchGrid march_grid2(const Latt &L, int rows)
     default: launch_k((KERNEL<16>), __VA_ARGS__); break;                    \
     }
 
-// Sum of v over the block, returned to every thread.  lds: kBlock/64 + 1 doubles.
-__device__ __forceinline__ double block_allsum(double v, double *lds)
+// Sums over the block, valid in thread 0 (a producer's per-block partial):
+// one barrier, the waves' sums added in wave order (as block_allsum).  At most
+// one call (of either) per kernel: they share their LDS.
+__device__ __forceinline__ void block_sum2_t0(double a, double b, double &sa, double &sb)
 {
-    v = wave_sum(v);
+    __shared__ double red[2 * (kBlock / kWave)];
+    a = wave_sum(a);
+    b = wave_sum(b);
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    const int nw = blockDim.x / kWave;
-    if (lane == 0) lds[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < nw; ++i) s += lds[i];
-        lds[nw] = s;
+    if (lane == 0) {
+        red[2 * w] = a;
+        red[2 * w + 1] = b;
     }
     __syncthreads();
-    const double r = lds[nw];
-    __syncthreads();
-    return r;
+    sa = 0.0;
+    sb = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x / kWave;
+        for (int i = 0; i < nw; ++i) {
+            sa += red[2 * i];
+            sb += red[2 * i + 1];
+        }
+    }
 }
+__device__ __forceinline__ double block_sum_t0(double v)
+{
+    double s, z;
+    block_sum2_t0(v, 0.0, s, z);
+    return s;
+}
+
 
 // Sum of the per-block partials part[b * stride + off], b < n, in a fixed
 // order: lane l adds part[l], part[l + 64], ... in turn, then the lanes are

@@ -43,7 +43,6 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Latt L, const double2 *__res
                                                     double *Pb, int bx, int by, int nbx,
                                                     int rows)
 {
-    __shared__ double lds[kBlock / kWave + 1];
     MarchPos p = march_pos(L, bx, by, nbx, rows);
     double acc = 0.0;
     if (p.active) {
@@ -61,7 +60,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Latt L, const double2 *__res
             }
         }
     }
-    const double s = block_allsum(acc, lds);
+    const double s = block_sum_t0(acc);
     if (threadIdx.x == 0) {
         Prr[p.blk] = s;
         Pb[p.blk] = s;
@@ -127,7 +126,6 @@ __global__ void __launch_bounds__(kBlock) k_cg_resid(Latt L, const double2 *__re
                                                      double *Prr, double *Pb, int k, int mode,
                                                      int bx, int by, int nbx)
 {
-    __shared__ double lds[kBlock / kWave + 1];
     MarchPos p = march_pos(L, bx, by, nbx, R);
     EpiResid epi = e0;
     epi.mu2 = S->mu2;
@@ -137,8 +135,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_resid(Latt L, const double2 *__re
         SrcPlain<double> src{t, L.V};
         q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
-    const double a = block_allsum(epi.rr, lds);
-    const double bb = block_allsum(epi.bb, lds);
+    double a, bb;
+    block_sum2_t0(epi.rr, epi.bb, a, bb);
     if (threadIdx.x == 0) {
         Prr[p.blk] = a;
         if (mode != 2) Pb[p.blk] = bb;
@@ -238,8 +236,8 @@ __global__ void __launch_bounds__(kBlock) k_cg1(Latt L, const c2<T> *__restrict_
         SrcXpay<T> src{r, pold, (T)beta, L.V};
         q_march<T, R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
-    const double a = block_allsum(epi.tt, lds);
-    const double b = block_allsum(epi.pp, lds);
+    double a, b;
+    block_sum2_t0(epi.tt, epi.pp, a, b);
     if (threadIdx.x == 0) {
         Ptt[2 * p.blk] = a;
         Ptt[2 * p.blk + 1] = b;
@@ -268,7 +266,7 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
         SrcPlain<T> src{t, L.V};
         q_march<T, R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
-    const double a = block_allsum(epi.rr, lds);
+    const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) Prr[p.blk] = a;
 }
 
@@ -296,8 +294,8 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
         SrcXpay2 src{r, pold, (float)beta, L.V};
         q_march2x<R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
-    const double a = block_allsum(epi.tt, lds);
-    const double b = block_allsum(epi.pp, lds);
+    double a, b;
+    block_sum2_t0(epi.tt, epi.pp, a, b);
     if (threadIdx.x == 0) {
         Ptt[2 * p.blk] = a;
         Ptt[2 * p.blk + 1] = b;
@@ -324,7 +322,7 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
         SrcPlain2 src{t, L.V};
         q_march2x<R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
-    const double a = block_allsum(epi.rr, lds);
+    const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) {
         Prr[p.blk] = a;
         if (p.blk == 0) S->alpha_x = alpha;  // for K1(k + 1)
@@ -517,7 +515,6 @@ __global__ void __launch_bounds__(kBlock) k_ms_init(Latt L, const double2 *__res
                                                     const MultiShiftState *S, double *Prr,
                                                     int nshift, int bx, int by, int nbx, int rows)
 {
-    __shared__ double lds[kBlock / kWave + 1];
     MarchPos p = march_pos(L, bx, by, nbx, rows);
     double acc = 0.0;
     const double2 z = mk<double>(0.0, 0.0);
@@ -538,7 +535,7 @@ __global__ void __launch_bounds__(kBlock) k_ms_init(Latt L, const double2 *__res
             }
         }
     }
-    const double s = block_allsum(acc, lds);
+    const double s = block_sum_t0(acc);
     if (threadIdx.x == 0) Prr[p.blk] = s;
 }
 
