@@ -78,7 +78,8 @@ def main():
                     help="trajectories of the acceptance comparison (0: skip)")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--tiles", type=str, default="",
-                    help="q64,q32,cg64,cg32 tile heights (skips the autotuner)")
+                    help="q64,q32,cg64,cg32[,cg64_k2,cg32_k2] tile heights (skips the "
+                         "autotuner)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -112,7 +113,10 @@ def main():
         f"plaquette {Q.Plaquette(U, prm):.6f}")
     U0 = U.clone()
     if args.tiles:
-        Q.set_tiles(prm, *[int(v) for v in args.tiles.split(",")])
+        tv = [int(v) for v in args.tiles.split(",")]
+        Q.set_tiles(prm, *tv[:4])
+        if len(tv) == 6:
+            Q.set_tiles(prm, cg64_k2=tv[4], cg32_k2=tv[5])
     elif not (args.no_autotune or args.tile or args.tile32):
         t0 = time.time()
         tiles = Q.autotune(prm, 40)

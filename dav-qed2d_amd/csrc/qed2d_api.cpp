@@ -40,6 +40,8 @@ struct qed2d_ctx {
     // defaults come from default_tiles(), qed2d_ctx_autotune measures them.
     int rows = 16;    // FP64 solver kernels (CG, multi-shift, residuals)
     int rows32 = 8;   // FP32 solver kernels (mixed-precision CG)
+    int rows_k2 = 0;    // the second CG kernel (K2) of the FP64 CG; 0: rows
+    int rows32_k2 = 0;  // K2 of the mixed-precision CG; 0: rows32
     int rows_q = 2;   // FP64 Q / Dw applies
     int rows_q32 = 2; // FP32 Q apply of the public entry point
     double ru_delta = 0.01;  // mixed CG: reliable update after |r| drops by this factor
@@ -489,7 +491,7 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
                                   k, c->rows, st));
             PROF(c, QED2D_PROF_CG_K2_F64,
                  launch_cg_k2_f64(L, (const double2 *)U, t, P[0], P[1], (double2 *)x, r, a.am0,
-                                  c->state, c->P, k, c->rows, st));
+                                  c->state, c->P, k, c->rows_k2 ? c->rows_k2 : c->rows, st));
         }
         TRY(launched(c));
         TRY(read_state(c));
@@ -554,7 +556,8 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
                                   c->rows32, st));
             PROF(c, QED2D_PROF_CG_K2_F32,
                  launch_cg_k2_f32(L, Us, packed, ts, P[0], P[1], xs, rs, am0f, a.mu2 != 0.0,
-                                  c->state, c->P, k, c->rows32, st));
+                                  c->state, c->P, k, c->rows32_k2 ? c->rows32_k2 : c->rows32,
+                                  st));
         }
         TRY(launched(c));
         TRY(read_state(c));
@@ -1238,21 +1241,31 @@ int autotune_impl(qed2d_ctx *c, int iters)
     a.eps = 0.0;
     a.maxiter = iters;
     int it;
+    // (the two kernels of an iteration are timed together, every pair of
+    // heights: K1 and K2 may prefer different ones)
     for (int prec = 0; prec < 2 && !rc; ++prec) {
         int &tile = prec ? c->rows32 : c->rows;
-        const int keep = tile;
+        int &tile2 = prec ? c->rows32_k2 : c->rows_k2;
+        const int keep = tile, keep2 = tile2;
+        int pick2 = keep2;
         best = 1e30, pick = keep;
         for (int R : cand) {
             if (prec == 0 && R == 1) continue;
-            tile = R;
-            rc = time_runs(c, 1, [&] {
-                if (prec) (void)cg_mixed(c, (double *)x, (const double *)U, (const double *)b, a, &it);
-                else (void)cg_f64(c, (double *)x, (const double *)U, (const double *)b, a, &it);
-            }, &us);
+            for (int R2 : cand) {
+                if (prec == 0 && R2 == 1) continue;
+                tile = R;
+                tile2 = R2;
+                rc = time_runs(c, 1, [&] {
+                    if (prec) (void)cg_mixed(c, (double *)x, (const double *)U, (const double *)b, a, &it);
+                    else (void)cg_f64(c, (double *)x, (const double *)U, (const double *)b, a, &it);
+                }, &us);
+                if (rc) break;
+                if (us < best) best = us, pick = R, pick2 = R2;
+            }
             if (rc) break;
-            if (us < best) best = us, pick = R;
         }
         tile = rc ? keep : pick;
+        tile2 = rc ? keep2 : (pick2 == pick ? 0 : pick2);
     }
     c->sync = save_sync;
     (void)hipStreamSynchronize(st);
@@ -1400,8 +1413,33 @@ int qed2d_ctx_set_tiles(qed2d_ctx *c, int q64, int q32, int cg64, int cg32)
             return fail(c, QED2D_EINVAL, "tile rows must be 0, 1, 2, 4, 8 or 16 (got %d)", v);
     if (q64) c->rows_q = q64;
     if (q32) c->rows_q32 = q32;
-    if (cg64) c->rows = cg64;
-    if (cg32) c->rows32 = cg32;
+    if (cg64) {
+        c->rows = cg64;
+        c->rows_k2 = 0;
+    }
+    if (cg32) {
+        c->rows32 = cg32;
+        c->rows32_k2 = 0;
+    }
+    return QED2D_OK;
+}
+
+int qed2d_ctx_set_tiles_k2(qed2d_ctx *c, int cg64, int cg32)
+{
+    if (!c) return QED2D_EINVAL;
+    for (int v : {cg64, cg32})
+        if (v != 0 && !tile_ok(v))
+            return fail(c, QED2D_EINVAL, "tile rows must be 0, 1, 2, 4, 8 or 16 (got %d)", v);
+    c->rows_k2 = cg64;
+    c->rows32_k2 = cg32;
+    return QED2D_OK;
+}
+
+int qed2d_ctx_get_tiles_k2(qed2d_ctx *c, int *tiles)
+{
+    if (!c || !tiles) return QED2D_EINVAL;
+    tiles[0] = c->rows_k2 ? c->rows_k2 : c->rows;
+    tiles[1] = c->rows32_k2 ? c->rows32_k2 : c->rows32;
     return QED2D_OK;
 }
 

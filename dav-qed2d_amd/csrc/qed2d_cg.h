@@ -20,35 +20,57 @@ template <typename T> __device__ __forceinline__ c2<T> from_d2(double2 v)
 // k & 1.  Returns false when the kernel has no work; beta receives
 // |r_{k-1}|^2 / |r_{k-2}|^2 (src/QED2dSolver.jl:33-35); fresh (if given) is
 // set when r_{k-1} is a new start (k = 1 or right after a reliable update),
-// i.e. when no deferred x update is pending.
-template <bool mixed>
-__device__ __forceinline__ bool cg_k1_scalars(SolverState *S, const double *Prr, const double *Pb,
-                                              int nrr, int nbb, int k, int blk, double *lds,
-                                              double &beta, bool *fresh = nullptr)
+// i.e. when no deferred x update is pending.  cg_k1_decide takes
+// |r_{k-1}|^2 (rr) already summed; cg_k1_scalars sums it first.
+// The solver state K1(k) reads, loaded in one go (so a kernel can issue the
+// loads early).
+struct K1In {
+    double bnorm, eps, tol, prev, rmax, delta2;
+    int true_at, guess, maxiter;
+};
+__device__ __forceinline__ K1In k1_in(const SolverState *S, int k)
 {
-    const int w = k & 1, rd = (k - 1) & 1;
-    const double rr = sum_prev_partials(Prr, nrr, 1, 0, lds);
+    const int rd = (k - 1) & 1;
+    K1In v;
+    v.bnorm = S->bnorm;
+    v.eps = S->eps;
+    v.tol = S->tol;
+    v.prev = S->norm[rd];
+    v.rmax = S->rmax2[rd];
+    v.delta2 = S->delta2;
+    v.true_at = S->true_at;
+    v.guess = S->guess;
+    v.maxiter = S->maxiter;
+    return v;
+}
+
+template <bool mixed>
+__device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, double rr,
+                                             const double *Pb, int nbb, int k, int blk,
+                                             double *lds, double &beta, bool *fresh = nullptr)
+{
+    const int w = k & 1;
     const bool first = (k == 1);
-    const double bn = first ? sum_prev_partials(Pb, nbb, 1, 0, lds) : S->bnorm;
-    const double tol = first ? S->eps * bn : S->tol;
-    const double prev = first ? 0.0 : S->norm[rd];
-    double rmax = first ? rr : S->rmax2[rd];
-    const bool truer = first || S->true_at == k;
+    const double bn = first ? sum_prev_partials(Pb, nbb, 1, 0, lds) : in.bnorm;
+    const double tol = first ? in.eps * bn : in.tol;
+    const double prev = first ? 0.0 : in.prev;
+    double rmax = first ? rr : in.rmax;
+    const bool truer = first || in.true_at == k;
     if (fresh) *fresh = truer;
     int stat = 0, ru = 0;
-    if (first && bn == 0.0 && !S->guess) {
+    if (first && bn == 0.0 && !in.guess) {
         stat = 1;  // b = 0: x = 0 exactly
     } else if (!mixed) {
         if (rr < tol) stat = 1;                         // src/QED2dSolver.jl:29
-        else if (k - 1 >= S->maxiter) stat = 2;
+        else if (k - 1 >= in.maxiter) stat = 2;
     } else if (truer) {
         if (rr < tol) stat = 1;
         else {
             rmax = rr;
-            if (k - 1 >= S->maxiter) stat = 2;
+            if (k - 1 >= in.maxiter) stat = 2;
         }
     } else {
-        if (rr < tol || rr < S->delta2 * rmax || k - 1 >= S->maxiter) ru = 1;
+        if (rr < tol || rr < in.delta2 * rmax || k - 1 >= in.maxiter) ru = 1;
         else if (rr > rmax) rmax = rr;
     }
     if (blk == 0 && threadIdx.x == 0) {
@@ -72,6 +94,15 @@ __device__ __forceinline__ bool cg_k1_scalars(SolverState *S, const double *Prr,
     if (stat || ru) return false;
     beta = first ? 0.0 : rr / prev;
     return true;
+}
+
+template <bool mixed>
+__device__ __forceinline__ bool cg_k1_scalars(SolverState *S, const double *Prr, const double *Pb,
+                                              int nrr, int nbb, int k, int blk, double *lds,
+                                              double &beta, bool *fresh = nullptr)
+{
+    const double rr = sum_prev_partials(Prr, nrr, 1, 0, lds);
+    return cg_k1_decide<mixed>(S, k1_in(S, k), rr, Pb, nbb, k, blk, lds, beta, fresh);
 }
 
 template <typename T> struct EpiK1 {

@@ -125,9 +125,14 @@ template <typename T> __device__ __forceinline__ c2<T> from_right(c2<T> v)
 // neighbours of the wave's edge lanes) are in flight while row r is computed;
 // the i1 neighbours and U(x-1,1) of the other lanes come from the adjacent
 // lanes' registers.
-template <typename T, int R, typename Src, typename Epi>
-__device__ __forceinline__ void q_march(const Latt &L, const c2<T> *__restrict__ U, T mass2,
-                                        const Src &src, Epi &epi, int i1, int r0)
+//
+// ready() runs after the loads of the first rows are issued and before any of
+// them is used (see q_march2r); active = false skips the loads and the rows
+// but still runs ready().
+template <typename T, int R, typename Src, typename Epi, typename Ready>
+__device__ __forceinline__ bool q_march_r(const Latt &L, const c2<T> *__restrict__ U, T mass2,
+                                          Src &src, Epi &epi, int i1, int r0, bool active,
+                                          Ready &&ready)
 {
     using Raw = typename Src::Raw;
     using Pre = decltype(epi.pre(0L));
@@ -141,23 +146,31 @@ __device__ __forceinline__ void q_march(const Latt &L, const c2<T> *__restrict__
     const bool ap = (L.bc & 1) != 0;
     const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
     const int nr = (r0 + R <= L2) ? R : L2 - r0;
-    c2<T> dn1, dn2, c1, cc2;
-    src.value(src.fetch((long)rm * L1 + i1), dn1, dn2);
-    src.value(src.fetch((long)r0 * L1 + i1), c1, cc2);
-    c2<T> U2m = U[V + (long)rm * L1 + i1];
-    if (ap && r0 == 0) U2m = cneg<T>(U2m);
-    // operands of row r0
-    long row = (long)r0 * L1;
-    Raw wu = src.fetch((long)((r0 + 1 == L2) ? 0 : r0 + 1) * L1 + i1);
-    Raw wp{}, wm{};
-    c2<T> U1m{};
-    if (ldR) wp = src.fetch(row + i1p);
-    if (ldL) {
-        wm = src.fetch(row + i1m);
-        U1m = U[row + i1m];
+    // operands of rows r0 - 1, r0 and r0 + 1
+    const long row = (long)r0 * L1;
+    Raw wd{}, wc{}, wu{}, wp{}, wm{};
+    c2<T> U2m{}, U1m{}, U1{}, U2{};
+    Pre q{};
+    if (active) {
+        wd = src.fetch((long)rm * L1 + i1);
+        wc = src.fetch(row + i1);
+        U2m = U[V + (long)rm * L1 + i1];
+        wu = src.fetch((long)((r0 + 1 == L2) ? 0 : r0 + 1) * L1 + i1);
+        if (ldR) wp = src.fetch(row + i1p);
+        if (ldL) {
+            wm = src.fetch(row + i1m);
+            U1m = U[row + i1m];
+        }
+        U1 = U[row + i1];
+        U2 = U[V + row + i1];
+        q = epi.pre(row + i1);
     }
-    c2<T> U1 = U[row + i1], U2 = U[V + row + i1];
-    Pre q = epi.pre(row + i1);
+    if (!ready()) return false;
+    if (!active) return true;
+    c2<T> dn1, dn2, c1, cc2;
+    src.value(wd, dn1, dn2);
+    src.value(wc, c1, cc2);
+    if (ap && r0 == 0) U2m = cneg<T>(U2m);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         if (k >= nr) break;
@@ -212,6 +225,16 @@ __device__ __forceinline__ void q_march(const Latt &L, const c2<T> *__restrict__
             q = nq;
         }
     }
+    return true;
+}
+
+// q_march_r without a ready step.
+template <typename T, int R, typename Src, typename Epi>
+__device__ __forceinline__ void q_march(const Latt &L, const c2<T> *__restrict__ U, T mass2,
+                                        const Src &src, Epi &epi, int i1, int r0)
+{
+    Src s = src;
+    q_march_r<T, R>(L, U, mass2, s, epi, i1, r0, true, [] { return true; });
 }
 
 
@@ -348,9 +371,15 @@ __device__ __forceinline__ auto pre_of(const E &e, long s, const Raw &, long) ->
 // components 1, 2 and c1, c2 the inputs.  The links come through the policy
 // lk (LinksF2 or LinksPacked): their raw loads travel one row ahead like the
 // spinors' and are decoded where they are used.
-template <int R, typename Lk, typename Src, typename Epi>
-__device__ __forceinline__ void q_march2l(const Latt &L, const Lk &lk, float mass2, const Src &src,
-                                          Epi &epi, int j, int r0)
+//
+// ready() runs after the loads of the first rows are issued and before any of
+// them is used: a solver kernel finishes there the sums of the previous
+// kernel's partials and sets the scalars the source and the epilogue use
+// (beta, alpha), so their latency overlaps the first rows' loads.  When it
+// returns false the march stops (no output) and returns false.
+template <int R, typename Lk, typename Src, typename Epi, typename Ready>
+__device__ __forceinline__ bool q_march2r(const Latt &L, const Lk &lk, float mass2, Src &src,
+                                          Epi &epi, int j, int r0, bool active, Ready &&ready)
 {
     using Raw2 = typename Src::Raw2;
     using Raw1 = typename Src::Raw1;
@@ -368,23 +397,33 @@ __device__ __forceinline__ void q_march2l(const Latt &L, const Lk &lk, float mas
     const bool ap = (L.bc & 1) != 0;
     const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
     const int nr = (r0 + R <= L2) ? R : L2 - r0;
-    F2x2 dn1, dn2, c1, cc2;
-    src.value2(src.fetch2((long)rm * L1 + ia), dn1, dn2);
-    const Raw2 wc = src.fetch2((long)r0 * L1 + ia);
-    src.value2(wc, c1, cc2);
-    F2x2 U2m = lk.value2(lk.fetch2(V + (long)rm * L1 + ia));
-    if (ap && r0 == 0) U2m = neg2(U2m);
-    long row = (long)r0 * L1;
-    Raw2 wu = src.fetch2((long)((r0 + 1 == L2) ? 0 : r0 + 1) * L1 + ia);
+    const long row = (long)r0 * L1;
+    Raw2 wd{}, wc{}, wu{};
     Raw1 wl{}, wq{};
+    LR2 rU2m{}, rU1{}, rU2{};
     LR1 rU1lm{};
-    if (ldL) {
-        wl = src.fetch1(row + im);
-        rU1lm = lk.fetch1(row + im);
+    Pre q{};
+    if (active) {
+        wd = src.fetch2((long)rm * L1 + ia);
+        wc = src.fetch2(row + ia);
+        rU2m = lk.fetch2(V + (long)rm * L1 + ia);
+        wu = src.fetch2((long)((r0 + 1 == L2) ? 0 : r0 + 1) * L1 + ia);
+        if (ldL) {
+            wl = src.fetch1(row + im);
+            rU1lm = lk.fetch1(row + im);
+        }
+        if (ldR) wq = src.fetch1(row + ip);
+        rU1 = lk.fetch2(row + ia);
+        rU2 = lk.fetch2(V + row + ia);
+        q = pre_of(epi, row + ia, wc, 0);
     }
-    if (ldR) wq = src.fetch1(row + ip);
-    LR2 rU1 = lk.fetch2(row + ia), rU2 = lk.fetch2(V + row + ia);
-    Pre q = pre_of(epi, row + ia, wc, 0);
+    if (!ready()) return false;
+    if (!active) return true;
+    F2x2 dn1, dn2, c1, cc2;
+    src.value2(wd, dn1, dn2);
+    src.value2(wc, c1, cc2);
+    F2x2 U2m = lk.value2(rU2m);
+    if (ap && r0 == 0) U2m = neg2(U2m);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         if (k >= nr) break;
@@ -447,6 +486,16 @@ __device__ __forceinline__ void q_march2l(const Latt &L, const Lk &lk, float mas
             q = nq;
         }
     }
+    return true;
+}
+
+// q_march2r without a ready step.
+template <int R, typename Lk, typename Src, typename Epi>
+__device__ __forceinline__ void q_march2l(const Latt &L, const Lk &lk, float mass2, const Src &src,
+                                          Epi &epi, int j, int r0)
+{
+    Src s = src;
+    q_march2r<R>(L, lk, mass2, s, epi, j, r0, true, [] { return true; });
 }
 
 // FP32 complex links.
@@ -503,6 +552,19 @@ __device__ __forceinline__ void q_march2x(const Latt &L, const LinksF32 &u, floa
         const LinksF2 lk{(const float2 *)u.U};
         q_march2l<R>(L, lk, mass2, src, epi, j, r0);
     }
+}
+
+// q_march2r with the links the solver kernels were given.
+template <int R, typename Src, typename Epi, typename Ready>
+__device__ __forceinline__ bool q_march2xr(const Latt &L, const LinksF32 &u, float mass2, Src &src,
+                                           Epi &epi, int j, int r0, bool active, Ready &&ready)
+{
+    if (u.packed) {
+        const LinksPacked lk{(const float *)u.U};
+        return q_march2r<R>(L, lk, mass2, src, epi, j, r0, active, ready);
+    }
+    const LinksF2 lk{(const float2 *)u.U};
+    return q_march2r<R>(L, lk, mass2, src, epi, j, r0, active, ready);
 }
 
 // q_march2l with FP32 complex links.
@@ -617,5 +679,68 @@ __device__ __forceinline__ double sum_prev_partials(const double *part, int n, i
     for (; b < n; b += kWave) s += part[(long)b * stride + off];
     return wave_sum(s);
 }
+
+// sum_prev_partials split in two: load() issues the loads (lane l holds
+// part[l], part[l + 64], ...; lanes past n load part[0] and ignore it, so no
+// load waits on a branch), sum() adds them in the same order (an ignored
+// value adds +0.0, which leaves a sum of nonnegative partials unchanged) and
+// sums the lanes: the same bits as sum_prev_partials.  A kernel issues the
+// loads first and sums where the value is needed, with other loads in flight
+// meanwhile.  For n <= kPartMax (a consumer never sees more: larger
+// producers are folded).
+constexpr int kPartMax = 1024;
+constexpr int kPartLane = kPartMax / kWave;
+struct PartialsAhead {
+    double v[kPartLane];
+    int n;
+    __device__ __forceinline__ void load(const double *part, int n_, int stride, int off)
+    {
+        n = n_;
+        const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+        for (int i = 0; i < kPartLane; ++i) {
+            const int b = lane + i * kWave;
+            v[i] = part[(long)(b < n ? b : 0) * stride + off];
+        }
+    }
+    __device__ __forceinline__ double sum() const
+    {
+        const int lane = threadIdx.x & (kWave - 1);
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < kPartLane; ++i) s += (lane + i * kWave < n) ? v[i] : 0.0;
+        return wave_sum(s);
+    }
+};
+// Two interleaved partials per block (part[2b], part[2b + 1]), one 16-byte
+// load each.
+struct PartialsAhead2 {
+    double2 v[kPartLane];
+    int n;
+    __device__ __forceinline__ void load(const double *part, int n_)
+    {
+        n = n_;
+        const int lane = threadIdx.x & (kWave - 1);
+        const double2 *p2 = reinterpret_cast<const double2 *>(part);
+#pragma unroll
+        for (int i = 0; i < kPartLane; ++i) {
+            const int b = lane + i * kWave;
+            v[i] = p2[b < n ? b : 0];
+        }
+    }
+    __device__ __forceinline__ void sum(double &s0, double &s1) const
+    {
+        const int lane = threadIdx.x & (kWave - 1);
+        double a = 0.0, c = 0.0;
+#pragma unroll
+        for (int i = 0; i < kPartLane; ++i) {
+            const bool in = lane + i * kWave < n;
+            a += in ? v[i].x : 0.0;
+            c += in ? v[i].y : 0.0;
+        }
+        s0 = wave_sum(a);
+        s1 = wave_sum(c);
+    }
+};
 
 }  // namespace qed2d

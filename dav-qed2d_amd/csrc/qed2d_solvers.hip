@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Latt L, const double2 *__res
 // consumer would have computed: the same code (sum_prev_partials) over the
 // same partials in the same order, so the bits do not depend on the folding.
 // ---------------------------------------------------------------------------
-constexpr int kFoldMin = 1024;
+constexpr int kFoldMin = kPartMax;  // consumers load at most this many partials
 
 __global__ void __launch_bounds__(kBlock) k_fold(double *part, int n, int stride)
 {
@@ -226,16 +226,24 @@ __global__ void __launch_bounds__(kBlock) k_cg1(Latt L, const c2<T> *__restrict_
     if (sk != 0 && sk < k) return;
     const int w = k & 1, rd = (k - 1) & 1;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    double beta;
-    if (!cg_k1_scalars<std::is_same<T, float>::value>(S, Prr, Pb, nrr, nbb, k, p.blk, lds, beta))
-        return;
+    // the partials' loads go out first, the first rows' loads next; beta is
+    // formed once both are in flight
+    PartialsAhead prr;
+    prr.load(Prr, nrr, 1, 0);
+    const K1In in = k1_in(S, k);
     const c2<T> *pold = rd ? P1 : P0;
     c2<T> *pnew = w ? P1 : P0;
     EpiK1<T> epi{pnew, t, L.V, 0.0, 0.0};
-    if (p.active) {
-        SrcXpay<T> src{r, pold, (T)beta, L.V};
-        q_march<T, R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
+    SrcXpay<T> src{r, pold, T(0), L.V};
+    auto ready = [&]() -> bool {
+        double beta;
+        if (!cg_k1_decide<std::is_same<T, float>::value>(S, in, prr.sum(), Pb, nbb, k, p.blk,
+                                                          lds, beta))
+            return false;
+        src.beta = (T)beta;
+        return true;
+    };
+    if (!q_march_r<T, R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) return;
     double a, b;
     block_sum2_t0(epi.tt, epi.pp, a, b);
     if (threadIdx.x == 0) {
@@ -252,20 +260,25 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
                                                 const double *Ptt, double *Prr, int ntt, int k,
                                                 int bx, int by, int nbx)
 {
-    __shared__ double lds[kBlock / kWave + 1];
     const int sk = S->stop_k;
     if (sk != 0 && sk <= k) return;
     const int w = k & 1;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    const double tt = sum_prev_partials(Ptt, ntt, 2, 0, lds);
-    const double pp = sum_prev_partials(Ptt, ntt, 2, 1, lds);
-    const double alpha = S->norm[w] / (tt + S->mu2 * pp);  // src/QED2dSolver.jl:21-22
+    // alpha (src/QED2dSolver.jl:21-22) is formed once the partials' loads and
+    // the first rows' loads are in flight
+    PartialsAhead2 ptt;
+    ptt.load(Ptt, ntt);
+    const double nw = S->norm[w], mu2 = S->mu2;
     const c2<T> *pv = w ? P1 : P0;
-    EpiK2<T> epi{pv, x, r, (T)alpha, (T)S->mu2, L.V, 0.0};
-    if (p.active) {
-        SrcPlain<T> src{t, L.V};
-        q_march<T, R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
+    EpiK2<T> epi{pv, x, r, T(0), (T)mu2, L.V, 0.0};
+    auto ready = [&]() -> bool {
+        double tt, pp;
+        ptt.sum(tt, pp);
+        epi.alpha = (T)(nw / (tt + mu2 * pp));
+        return true;
+    };
+    SrcPlain<T> src{t, L.V};
+    q_march_r<T, R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready);
     const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) Prr[p.blk] = a;
 }
@@ -285,15 +298,22 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
     if (sk != 0 && sk < k) return;
     const int w = k & 1, rd = (k - 1) & 1;
     const MarchPos p = march_pos2(L, bx, by, nbx, R);
-    double beta;
-    bool fresh;
-    if (!cg_k1_scalars<true>(S, Prr, Pb, nrr, nbb, k, p.blk, lds, beta, &fresh)) return;
+    // the partials' loads go out first, the first rows' loads next; beta is
+    // formed once both are in flight
+    PartialsAhead prr;
+    prr.load(Prr, nrr, 1, 0);
+    const K1In in = k1_in(S, k);
+    const bool fresh = (k == 1) || (in.true_at == k);
     float2 *pold = rd ? P1 : P0;
     EpiK1v epi{w ? P1 : P0, t, x, (float)S->alpha_x, !fresh, L.V, 0.0, 0.0};
-    if (p.active) {
-        SrcXpay2 src{r, pold, (float)beta, L.V};
-        q_march2x<R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
+    SrcXpay2 src{r, pold, 0.0f, L.V};
+    auto ready = [&]() -> bool {
+        double beta;
+        if (!cg_k1_decide<true>(S, in, prr.sum(), Pb, nbb, k, p.blk, lds, beta)) return false;
+        src.beta = (float)beta;
+        return true;
+    };
+    if (!q_march2xr<R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) return;
     double a, b;
     block_sum2_t0(epi.tt, epi.pp, a, b);
     if (threadIdx.x == 0) {
@@ -309,19 +329,26 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
                                           SolverState *S, const double *Ptt, double *Prr, int ntt,
                                           int k, int bx, int by, int nbx)
 {
-    __shared__ double lds[kBlock / kWave + 1];
     const int sk = S->stop_k;
     if (sk != 0 && sk <= k) return;
     const int w = k & 1;
     const MarchPos p = march_pos2(L, bx, by, nbx, R);
-    const double tt = sum_prev_partials(Ptt, ntt, 2, 0, lds);
-    const double pp = sum_prev_partials(Ptt, ntt, 2, 1, lds);
-    const double alpha = S->norm[w] / (tt + S->mu2 * pp);
-    EpiK2v<MU> epi{w ? P1 : P0, r, (float)alpha, (float)S->mu2, L.V, 0.0};
-    if (p.active) {
-        SrcPlain2 src{t, L.V};
-        q_march2x<R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
+    // alpha is formed once the partials' loads and the first rows' loads are
+    // in flight (src/QED2dSolver.jl:21-22)
+    PartialsAhead2 ptt;
+    ptt.load(Ptt, ntt);
+    const double nw = S->norm[w], mu2 = S->mu2;
+    EpiK2v<MU> epi{w ? P1 : P0, r, 0.0f, (float)mu2, L.V, 0.0};
+    double alpha = 0.0;
+    auto ready = [&]() -> bool {
+        double tt, pp;
+        ptt.sum(tt, pp);
+        alpha = nw / (tt + mu2 * pp);
+        epi.alpha = (float)alpha;
+        return true;
+    };
+    SrcPlain2 src{t, L.V};
+    q_march2xr<R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready);
     const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) {
         Prr[p.blk] = a;

@@ -56,6 +56,8 @@ SIGNATURES = {
     "qed2d_ctx_set_tile_f32": (_i, [_p, _i]),
     "qed2d_ctx_set_tiles": (_i, [_p, _i, _i, _i, _i]),
     "qed2d_ctx_get_tiles": (_i, [_p, _p]),
+    "qed2d_ctx_set_tiles_k2": (_i, [_p, _i, _i]),
+    "qed2d_ctx_get_tiles_k2": (_i, [_p, _p]),
     "qed2d_batch_create": (_i, [_p, _i, _p, _p, _p, _i, _p]),
     "qed2d_batch_destroy": (None, [_p]),
     "qed2d_batch_last_error": (C.c_char_p, [_p]),
@@ -142,24 +144,35 @@ PROF_CLASSES = ["cg_k1_f64", "cg_k2_f64", "cg_k1_f32", "cg_k2_f32", "q_f64", "fo
                 "updt_fld", "reliable_update", "other"]
 
 
-def set_tiles(prm, q64=0, q32=0, cg64=0, cg32=0):
+def set_tiles(prm, q64=0, q32=0, cg64=0, cg32=0, cg64_k2=None, cg32_k2=None):
     """Tile heights (rows marched per thread) of the FP64 / FP32 Q applies and
-    the FP64 / FP32 solver kernels; 0 leaves a family unchanged."""
+    the FP64 / FP32 solver kernels; 0 leaves a family unchanged.  A nonzero
+    cg64 / cg32 also sets the second CG kernel's height, unless cg64_k2 /
+    cg32_k2 give it separately."""
     _check(prm, lib.qed2d_ctx_set_tiles(prm.ctx(), int(q64), int(q32), int(cg64), int(cg32)))
+    if cg64_k2 is not None or cg32_k2 is not None:
+        cur = get_tiles(prm)
+        a = cur["cg64"] if cg64_k2 is None else cg64_k2
+        b = cur["cg32"] if cg32_k2 is None else cg32_k2
+        _check(prm, lib.qed2d_ctx_set_tiles_k2(prm.ctx(), int(a), int(b)))
 
 
 def get_tiles(prm):
+    """The tile heights in use: the four families, and the second CG kernel's
+    (cg64_k2, cg32_k2)."""
     t = (_i * 4)()
     _check(prm, lib.qed2d_ctx_get_tiles(prm.ctx(), t))
-    return dict(zip(("q64", "q32", "cg64", "cg32"), list(t)))
+    t2 = (_i * 2)()
+    _check(prm, lib.qed2d_ctx_get_tiles_k2(prm.ctx(), t2))
+    return dict(zip(("q64", "q32", "cg64", "cg32", "cg64_k2", "cg32_k2"), list(t) + list(t2)))
 
 
 def autotune(prm, iters=40):
     """Times every tile height of each kernel family on this lattice and keeps
-    the fastest (returns the choice)."""
+    the fastest (returns the choice, as get_tiles)."""
     t = (_i * 4)()
     _check(prm, lib.qed2d_ctx_autotune(prm.ctx(), int(iters), t))
-    return dict(zip(("q64", "q32", "cg64", "cg32"), list(t)))
+    return get_tiles(prm)
 
 
 def hbm_stream(prm, nbytes=1 << 30):

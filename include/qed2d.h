@@ -85,9 +85,17 @@ int qed2d_ctx_set_tile_f32(qed2d_ctx *ctx, int rows);
  * whose KernelParm launch shape these replace (src/QED2d.jl:10-13). */
 int qed2d_ctx_set_tiles(qed2d_ctx *ctx, int q64, int q32, int cg64, int cg32);
 int qed2d_ctx_get_tiles(qed2d_ctx *ctx, int tiles[4]);
+/* Tile heights of the second kernel of a CG iteration (Q t and the r update)
+ * of the FP64 and the FP32 solver, where they should differ from the first
+ * kernel's (0: the family's height, the default; set_tiles with a nonzero
+ * cg64 / cg32 resets them to 0).  get_tiles_k2 returns the heights in use.
+ * No counterpart in the reference (KernelParm, src/QED2d.jl:10-13). */
+int qed2d_ctx_set_tiles_k2(qed2d_ctx *ctx, int cg64, int cg32);
+int qed2d_ctx_get_tiles_k2(qed2d_ctx *ctx, int tiles[2]);
 /* The autotuner: times each tile height of the four families on this
  * context's lattice (scratch fields; CG families over `iters` iterations of a
- * solve that cannot converge) and keeps the fastest.  tiles (may be NULL)
+ * solve that cannot converge, every pair of heights of the iteration's two
+ * kernels) and keeps the fastest.  tiles (may be NULL)
  * receives the choice.  Tile heights change the order of the solvers' global
  * sums, so a run that must reproduce bits fixes its tiles instead. */
 int qed2d_ctx_autotune(qed2d_ctx *ctx, int iters, int tiles[4]);

@@ -92,6 +92,35 @@ def test_cg_tiles(Q, O, rows):
             assert abs(it - itr) <= 1
 
 
+@pytest.mark.parametrize("k1,k2", [(2, 16), (16, 2), (4, 8)])
+def test_cg_split_tiles(Q, O, k1, k2):
+    """The two kernels of a CG iteration at different tile heights (each
+    producer's partial-sum count travels with it): the oracle's solution,
+    both precisions, odd L1 (one-site FP32 march) and even."""
+    for L1, L2 in ((48, 36), (33, 20)):
+        say("CG split tiles", k1, k2, L1, L2)
+        rng = np.random.default_rng(100 * k1 + k2 + L1)
+        U = random_links(L1, L2, rng, unitary=True)
+        b = random_spinor(L1, L2, rng)
+        prm = Q.LattParm((L1, L2), 4.0)
+        Q.set_tiles(prm, cg64=k1, cg32=k1, cg64_k2=k2, cg32_k2=k2)
+        t = Q.get_tiles(prm)
+        assert (t["cg64"], t["cg32"], t["cg64_k2"], t["cg32_k2"]) == (k1, k1, k2, k2)
+        xr, itr, st = O.cg(U, b, 0.3, 2000, 1e-20)
+        for prec in (0, 1):
+            x = Q.spinor(prm)
+            it = Q.CG(x, dev(U), dev(b), 0.3, 2000, 1e-20, Q.gamm5Dw_sqr_msq, prm, None,
+                      precision=prec)
+            assert _true_res(O, U, host(x), b, 0.3) < 1e-20
+            assert relerr(host(x), xr) < 1e-8
+            if prec == 0:
+                assert abs(it - itr) <= 1
+        # set_tiles with a solver height resets the second kernel's to follow it
+        Q.set_tiles(prm, cg64=8, cg32=8)
+        t = Q.get_tiles(prm)
+        assert (t["cg64_k2"], t["cg32_k2"]) == (8, 8)
+
+
 def test_cg_not_converged_raises(Q):
     say("CG maxiter")
     L1, L2 = 24, 24
@@ -371,7 +400,7 @@ def test_autotune_then_parity(Q, O):
     si = random_spinor(L1, L2, rng)
     prm = Q.LattParm((L1, L2), 4.0)
     t = Q.autotune(prm, 12)
-    assert set(t) == {"q64", "q32", "cg64", "cg32"}
+    assert set(t) == {"q64", "q32", "cg64", "cg32", "cg64_k2", "cg32_k2"}
     assert all(v in (1, 2, 4, 8, 16) for v in t.values())
     assert Q.get_tiles(prm) == t
     so = Q.spinor(prm)
