@@ -74,7 +74,7 @@ def main():
     ap.add_argument("--no-dw", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--accept-traj", type=int, default=50,
+    ap.add_argument("--accept-traj", type=int, default=100,
                     help="trajectories of the acceptance comparison (0: skip)")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--tiles", type=str, default="",
