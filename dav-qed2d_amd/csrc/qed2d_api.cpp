@@ -685,6 +685,7 @@ int multishift_cg(qed2d_ctx *c, double2 *const *x, const double *U, const double
             PROF(c, QED2D_PROF_OTHER,
                  launch_ms_k1(L, (const double2 *)U, r, P[0], P[1], t, am0, c->ms, c->P, k, n,
                               c->rows, st));
+            PROF(c, QED2D_PROF_OTHER, launch_ms_shifts(c->ms, r, spinor_elems(c), k, n, st));
             PROF(c, QED2D_PROF_OTHER,
                  launch_ms_k2(L, (const double2 *)U, t, P[0], P[1], r, am0, c->ms, c->P, k, n,
                               c->rows, st));
@@ -767,9 +768,11 @@ int multishift_multi(qed2d_ctx *c, const double *U, int nf, const double *const 
     const long n2 = spinor_elems(c);
     const long nbc = march_grid(c->L, 1).nblocks();
     std::vector<BChain> ch(nf);
+    int maxshift = 0;
     for (int j = 0; j < nf; ++j) {
         if (npole[j] < 1 || npole[j] > kMaxShift)
             return fail(c, QED2D_EINVAL, "multi-shift CG takes 1..%d shifts", kMaxShift);
+        maxshift = std::max(maxshift, npole[j]);
         TRY(grow(c, m.x[j], npole[j]));
         TRY(grow(c, m.ps[j], npole[j]));
         ch[j].L = c->L;
@@ -816,6 +819,7 @@ int multishift_multi(qed2d_ctx *c, const double *U, int nf, const double *const 
         for (int jj = 0; jj < chunk; ++jj) {
             PROF(c, QED2D_PROF_OTHER,
                  launch_b_ms_k1(g, c->rows, (const double2 *)U, r, p0, p1, t, m.M, m.P, st));
+            PROF(c, QED2D_PROF_OTHER, launch_b_ms_shifts(m.M, r, n2, nf, maxshift, st));
             PROF(c, QED2D_PROF_OTHER,
                  launch_b_ms_k2(g, c->rows, (const double2 *)U, t, p0, p1, r, m.M, m.P, st));
         }
@@ -1163,15 +1167,32 @@ int luscher_z_impl(qed2d_ctx *c, double *Z, const double *U, const double *X, do
 
 bool tile_ok(int r) { return r == 1 || r == 2 || r == 4 || r == 8 || r == 16; }
 
+// The tallest of {big, big/2, ..., 1} that leaves a launch of at least 256
+// blocks of `sites` sites per row of a tile (one block per CU).
+int fill_tile(long V, int big, int sites)
+{
+    int R = big;
+    while (R > 1 && V / ((long)sites * R) < 256) R >>= 1;
+    return R;
+}
+
 // Tile heights measured best on MI355X (pipelined marches): applies are
 // bandwidth bound at any height >= 2; the FP64 CG kernels need tall tiles on
-// lattices that fit the 256 MB Infinity Cache and medium ones above.
+// lattices that fit the 256 MB Infinity Cache and medium ones above.  Below
+// 1024^2 a tall tile leaves most CUs idle (128^2 at 16 rows is 4 blocks), so
+// the solver tiles shrink until a launch has a block per CU (128^2 RHMC
+// trajectory: 0.83 s at 16 rows, 0.43 s at 1).
 void default_tiles(qed2d_ctx *c)
 {
+    const long V = c->L.V;
     c->rows_q = 2;
     c->rows_q32 = 2;
-    c->rows = (c->L.V <= (1L << 21)) ? 16 : 8;
+    c->rows = (V <= (1L << 21)) ? 16 : 8;
     c->rows32 = 8;
+    if (V < (1L << 20)) {
+        c->rows = fill_tile(V, 16, kBlock);
+        c->rows32 = fill_tile(V, 8, 2 * kBlock);
+    }
 }
 
 // Mean duration in microseconds of `reps` runs of fn (after one warm-up run)

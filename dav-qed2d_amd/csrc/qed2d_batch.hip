@@ -536,6 +536,31 @@ __global__ void __launch_bounds__(kBlock) kb_ms2(BGeo g, const double2 *__restri
     }
 }
 
+// The shift updates of every chain's current step, after kb_ms1 (chains of n2
+// spinor elements each, bpc blocks per chain; y over the shifts).  kk is still
+// the step kb_ms1 ran: a chain that stopped before it updates nothing.
+__global__ void __launch_bounds__(kBlock) kb_ms_shifts(const MultiShiftState *M,
+                                                       const double2 *__restrict__ r, long n2,
+                                                       int bpc)
+{
+    const int c = blockIdx.x / bpc;
+    const MultiShiftState *Mc = M + c;
+    const int k = blockIdx.y;
+    if (k >= Mc->nshift) return;
+    const int sk = Mc->stop_k;
+    if (sk != 0 && sk < Mc->kk) return;
+    const long i = (long)(blockIdx.x % bpc) * kBlock + threadIdx.x;
+    if (i >= n2) return;
+    ms_shift_update(Mc, r + (long)c * n2, i, k);
+}
+void launch_b_ms_shifts(const MultiShiftState *M, const double2 *r, long n2, int nchain,
+                        int maxshift, hipStream_t st)
+{
+    const int bpc = (int)((n2 + kBlock - 1) / kBlock);
+    launch_k(kb_ms_shifts, dim3((unsigned)(bpc * nchain), maxshift), dim3(kBlock), 0, st, M, r,
+             n2, bpc);
+}
+
 void launch_b_ms_k1(const BGeo &g, int rows, const double2 *U, const double2 *r, double2 *p0,
                     double2 *p1, double2 *t, MultiShiftState *M, BParts P, hipStream_t st)
 {

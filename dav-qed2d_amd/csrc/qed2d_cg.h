@@ -285,36 +285,11 @@ struct EpiResid {
 struct EpiMS1 {
     double2 *__restrict__ pnew;
     double2 *__restrict__ t;
-    const double2 *__restrict__ r;
-    const MultiShiftState *S;
-    const double *zj, *as, *bs;
-    const int *upd;
-    int nshift;
     long V;
     double tt, pp;
-    __device__ __forceinline__ void shifts(long i, double2 rv)
-    {
-        for (int k = 0; k < nshift; ++k) {
-            const int u = upd[k];
-            if (u == 0) continue;
-            double2 *ps = S->ps[k];
-            double2 *x = S->x[k];
-            const double2 pv = ps[i];
-            const double a = as[k];
-            const double2 xv = x[i];
-            x[i] = mk<double>(xv.x + a * pv.x, xv.y + a * pv.y);
-            if (u == 2) {
-                const double z = zj[k], b = bs[k];
-                ps[i] = mk<double>(z * rv.x + b * pv.x, z * rv.y + b * pv.y);
-            }
-        }
-    }
-    struct Pre {
-        double2 r1, r2;
-    };
-    __device__ __forceinline__ Pre pre(long s) const { return Pre{r[s], r[V + s]}; }
-    __device__ __forceinline__ void operator()(long s, const Pre &q, double2 o1, double2 o2,
-                                               double2 c1, double2 cc2)
+    __device__ __forceinline__ NoPre pre(long) const { return {}; }
+    __device__ __forceinline__ void operator()(long s, NoPre, double2 o1, double2 o2, double2 c1,
+                                               double2 cc2)
     {
         pnew[s] = c1;
         pnew[V + s] = cc2;
@@ -322,10 +297,30 @@ struct EpiMS1 {
         t[V + s] = o2;
         tt += (o1.x * o1.x + o1.y * o1.y) + (o2.x * o2.x + o2.y * o2.y);
         pp += (c1.x * c1.x + c1.y * c1.y) + (cc2.x * cc2.x + cc2.y * cc2.y);
-        shifts(s, q.r1);
-        shifts(V + s, q.r2);
     }
 };
+
+// The shift updates of one Krylov step at spinor element i (after K1 of that
+// step has committed the coefficients): x_i += alpha^i p^i and, for shifts
+// still iterating, p^i = zeta^i r + beta^i p^i.  Shift k only; elementwise and
+// in place, so any number of elements and shifts run in parallel.
+__device__ __forceinline__ void ms_shift_update(const MultiShiftState *S,
+                                                const double2 *__restrict__ r, long i, int k)
+{
+    const int u = S->sh_u[k];
+    if (u == 0) return;
+    double2 *__restrict__ ps = S->ps[k];
+    double2 *__restrict__ x = S->x[k];
+    const double a = S->sh_a[k];
+    const double2 pv = ps[i];
+    const double2 xv = x[i];
+    x[i] = mk<double>(xv.x + a * pv.x, xv.y + a * pv.y);
+    if (u == 2) {
+        const double z = S->sh_z[k], b = S->sh_b[k];
+        const double2 rv = r[i];
+        ps[i] = mk<double>(z * rv.x + b * pv.x, z * rv.y + b * pv.y);
+    }
+}
 
 template <int R>
 __device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restrict__ U,
@@ -397,6 +392,11 @@ __device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restric
             S->zeta_prev[w][i] = z1_s[i];
             S->active[w][i] = (upd_s[i] == 2);
             if (upd_s[i] == 1) S->conv_iter[i] = k - 1;
+            // for the shift-update kernel that follows (none at the iteration limit)
+            S->sh_u[i] = (stat == 2) ? 0 : upd_s[i];
+            S->sh_a[i] = as_s[i];
+            S->sh_b[i] = bs_s[i];
+            S->sh_z[i] = zj_s[i];
         }
         S->iter = k - 1;
         S->err = rr;
@@ -409,31 +409,19 @@ __device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restric
             S->stop_k = k;
         }
     }
-    if (stat == 2) return;
-    if (anyleft) {
-        const double2 *pold = rd ? P1 : P0;
-        double2 *pnew = w ? P1 : P0;
-        EpiMS1 epi{pnew, t, r, S, zj_s, as_s, bs_s, upd_s, nshift, L.V, 0.0, 0.0};
-        if (p.active) {
-            SrcXpay<double> src{r, pold, beta_j, L.V};
-            q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
-        }
-        double a, b;
-        block_sum2_t0(epi.tt, epi.pp, a, b);
-        if (threadIdx.x == 0) {
-            Ptt_w[2 * p.blk] = a;
-            Ptt_w[2 * p.blk + 1] = b;
-        }
-    } else if (p.active) {
-        // every shift converged at this step: only the last x updates remain
-        EpiMS1 epi{nullptr, nullptr, r, S, zj_s, as_s, bs_s, upd_s, nshift, L.V, 0.0, 0.0};
-        for (int kk = 0; kk < R; ++kk) {
-            const int row = p.r0 + kk;
-            if (row >= L.L2) break;
-            const long s = (long)row * L.L1 + p.i1;
-            epi.shifts(s, r[s]);
-            epi.shifts(L.V + s, r[L.V + s]);
-        }
+    if (stat == 2 || !anyleft) return;  // (the last x updates are the shift kernel's)
+    const double2 *pold = rd ? P1 : P0;
+    double2 *pnew = w ? P1 : P0;
+    EpiMS1 epi{pnew, t, L.V, 0.0, 0.0};
+    if (p.active) {
+        SrcXpay<double> src{r, pold, beta_j, L.V};
+        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
+    }
+    double a, b;
+    block_sum2_t0(epi.tt, epi.pp, a, b);
+    if (threadIdx.x == 0) {
+        Ptt_w[2 * p.blk] = a;
+        Ptt_w[2 * p.blk + 1] = b;
     }
 }
 

@@ -94,6 +94,11 @@ struct MultiShiftState {
     double2 *ps[kMaxShift];          // search direction of shift i
     const double2 *b;                // batched solves: the right-hand side
     int kk, k2;                      // batched solves: iteration K1 runs next / K2 runs
+    // the shift updates of the step K1 last ran (written by its block 0, applied
+    // by the shift-update kernel that follows it): x_i += sh_a[i] p_i and, where
+    // sh_u[i] == 2, p_i = sh_z[i] r + sh_b[i] p_i; sh_u[i] == 0 skips shift i
+    double sh_a[kMaxShift], sh_b[kMaxShift], sh_z[kMaxShift];
+    int sh_u[kMaxShift];
 };
 
 // Per-block partial sums of the solver kernels.  tt: (|Q p|^2, |p|^2) per block,
@@ -204,6 +209,8 @@ void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, floa
 // Multi-shift CG of one right-hand side per chain (M[c]: chain c's state).
 void launch_b_ms_init(const BGeo &g, int rows, double2 *r, double2 *p0,
                       const MultiShiftState *M, BParts P, hipStream_t st);
+void launch_b_ms_shifts(const MultiShiftState *M, const double2 *r, long n2, int nchain,
+                        int maxshift, hipStream_t st);
 void launch_b_ms_k1(const BGeo &g, int rows, const double2 *U, const double2 *r, double2 *p0,
                     double2 *p1, double2 *t, MultiShiftState *M, BParts P, hipStream_t st);
 void launch_b_ms_k2(const BGeo &g, int rows, const double2 *U, const double2 *t,
@@ -275,6 +282,8 @@ void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st);
 // r = b, p0 = 0, x_i = 0, p_i = 0, |b|^2 partials into rr[0].
 void launch_ms_init(const Latt &L, const double2 *b, double2 *r, double2 *p0, MultiShiftState *s,
                     CGPartials &P, int nshift, int rows, hipStream_t st);
+void launch_ms_shifts(const MultiShiftState *s, const double2 *r, long n, int k, int nshift,
+                      hipStream_t st);
 void launch_ms_k1(const Latt &L, const double2 *U, const double2 *r, double2 *p0, double2 *p1,
                   double2 *t, double am0, MultiShiftState *s, CGPartials &P, int k,
                   int nshift, int rows, hipStream_t st);

@@ -535,6 +535,25 @@ __global__ void __launch_bounds__(kBlock) k_ms2(Latt L, const double2 *__restric
     ms2_body<R>(L, U, t, P0, P1, r, mass2, S, Ptt_w, Prr_w, ntt, k, p);
 }
 
+// The shift updates of step k, after K1(k) committed their coefficients: grid
+// x over the n spinor elements, y over the shifts.  A step that K1 skipped (the
+// solve stopped earlier) updates nothing.
+__global__ void __launch_bounds__(kBlock) k_ms_shifts(const MultiShiftState *S,
+                                                      const double2 *__restrict__ r, long n, int k)
+{
+    const int sk = S->stop_k;
+    if (sk != 0 && sk < k) return;
+    const long i = blockIdx.x * (long)kBlock + threadIdx.x;
+    if (i >= n) return;
+    ms_shift_update(S, r, i, blockIdx.y);
+}
+void launch_ms_shifts(const MultiShiftState *s, const double2 *r, long n, int k, int nshift,
+                      hipStream_t st)
+{
+    launch_k(k_ms_shifts, dim3((unsigned)((n + kBlock - 1) / kBlock), nshift), dim3(kBlock), 0,
+             st, s, r, n, k);
+}
+
 // r = b, p0 = 0, x_i = 0, p_i = 0, |b|^2 partials.
 __global__ void __launch_bounds__(kBlock) k_ms_init(Latt L, const double2 *__restrict__ b,
                                                     double2 *__restrict__ r,
