@@ -304,7 +304,7 @@ def dw_apply(Q, torch, args):
 DW_TILES = (2, 4, 8)
 
 
-def ensemble(Q, torch, ntraj=3):
+def ensemble(Q, torch, ntraj=10, nseq=3):
     """64 independent Nf=2 chains of uneven sizes (16:16:16:8:8 chains of
     16^2, 32^2, 64^2, 128^2, 256^2; beta cycling over {2, 4, 6}, m0 over
     {0.1, 0.2}, seed 1000 + chain), leapfrog ns 10 eps 0.1, CG tol 1e-16:
@@ -322,7 +322,11 @@ def ensemble(Q, torch, ntraj=3):
     for t in range(2):
         B.HMC(am0s, 0.1, 10, 100000, 1e-16, seeds, t, precision=Q.PREC_MIXED)
     U0 = [B.get_links(c).clone() for c in range(n)]
-    out = {"chains": n, "sites": B.sites, "trajectories_timed": ntraj}
+    t0 = time.perf_counter()
+    tiles = B.autotune(60)
+    log(f"ensemble: autotuned tiles {tiles} in {time.perf_counter() - t0:.2f}s")
+    out = {"chains": n, "sites": B.sites, "trajectories_timed": ntraj,
+           "sequential_trajectories_timed": nseq, "tiles": tiles}
     for prec, key in ((Q.PREC_MIXED, "mixed"), (Q.PREC_F64, "fp64")):
         for c in range(n):
             B.set_links(c, U0[c])
@@ -339,17 +343,17 @@ def ensemble(Q, torch, ntraj=3):
     Us = [u.clone() for u in U0]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for t in range(ntraj):
+    for t in range(nseq):
         for c in range(n):
             Q.seed(seeds[c], chain=c, traj=2 + t)
             Q.HMC(Us[c], am0s[c], 0.1, 10, [], 100000, 1e-16, prms[c], None,
                   precision=Q.PREC_MIXED)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    out["sequential_mixed_chain_traj_per_s"] = round(n * ntraj / dt, 2)
+    out["sequential_mixed_chain_traj_per_s"] = round(n * nseq / dt, 2)
     out["batched_vs_sequential"] = round(out["mixed_chain_traj_per_s"] /
                                          out["sequential_mixed_chain_traj_per_s"], 2)
-    log(f"ensemble sequential: {n} chains x {ntraj} traj in {dt:.3f}s")
+    log(f"ensemble sequential: {n} chains x {nseq} traj in {dt:.3f}s")
     for p in prms:
         p.close()
     B.close()

@@ -1823,7 +1823,7 @@ int qed2d_cg_mrhs(qed2d_ctx *c, double *x, const double *U, const double *b, int
     }
     std::string err;
     const int rc = batch_rhs_cg(c->rhs, U, x, b, am0, maxiter, eps, precision, iters, c->rows_q,
-                                c->rows, c->rows32, c->ru_delta, &err);
+                                c->rows, c->rows32, c->rows_k2, c->rows32_k2, c->ru_delta, &err);
     if (rc) return fail(c, rc, "%s", err.c_str());
     return finish(c);
 }

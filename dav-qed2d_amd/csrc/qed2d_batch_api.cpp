@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -44,6 +45,7 @@ struct qed2d_batch {
     std::vector<qed2d_ctx *> sub;  // one context per chain: its reductions
     std::string err;
     int rows = 16, rows32 = 8, rows_q = 4;
+    int rows_k2 = 0, rows32_k2 = 0;  // the second CG kernel's heights; 0: rows / rows32
     double ru_delta = 0.01;
     std::vector<double> last_dh;  // per chain, from the double-double energies
     std::map<int, GeoBuf> geo;  // key: 2 * rows + pair
@@ -161,6 +163,40 @@ int set_masses(qed2d_batch *b, const double *am0)
     return QED2D_OK;
 }
 
+// Philox key of a chain's random streams: (seed, chain).
+void key_of(uint64_t seed, uint32_t chain, uint32_t *k0, uint32_t *k1)
+{
+    *k0 = (uint32_t)seed;
+    *k1 = (uint32_t)(seed >> 32) ^ chain;
+}
+
+// Blocks of one launch over all chains at `rows` rows per thread (pair: the
+// FP32 site-pair kernels, chains with even L1 >= 4).
+long batch_blocks(const qed2d_batch *b, int rows, bool pair)
+{
+    long n = 0;
+    for (const BChain &ch : b->hch)
+        n += (pair && pair_ok(ch.L)) ? march_grid2(ch.L, rows).nblocks()
+                                     : march_grid(ch.L, rows).nblocks();
+    return n;
+}
+
+// Default tile heights: the tallest (at most 16 rows FP64, 8 FP32) that leaves
+// a solver launch of at least 1024 blocks over all chains (an ensemble of
+// small lattices has few blocks per chain: the 64-chain 16^2 ... 256^2
+// ensemble runs 597 chain-trajectories/s at 16 / 8 rows, 693 at 2 / 2).
+void default_batch_tiles(qed2d_batch *b)
+{
+    int r = 16;
+    while (r > 1 && batch_blocks(b, r, false) < 1024) r >>= 1;
+    b->rows = r;
+    int r32 = 8;
+    while (r32 > 1 && batch_blocks(b, r32, true) < 1024) r32 >>= 1;
+    b->rows32 = r32;
+    b->rows_q = (batch_blocks(b, 4, false) < 1024) ? 2 : 4;
+    b->rows_k2 = b->rows32_k2 = 0;
+}
+
 int read_states(qed2d_batch *b)
 {
     BHIP(b, hipMemcpyAsync(b->hS, b->S, sizeof(SolverState) * b->n, hipMemcpyDeviceToHost,
@@ -213,11 +249,14 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     BTRY(balloc(b, &b->p0, nel));
     BTRY(balloc(b, &b->p1, nel));
     BTRY(balloc(b, &b->t, nel));
-    BGeo g64, gq;
+    BGeo g64, g64b, gq;
+    const int rows64b = b->rows_k2 ? b->rows_k2 : b->rows;
+    const int rows32b = b->rows32_k2 ? b->rows32_k2 : b->rows32;
     BTRY(get_geo(b, b->rows, false, &g64));
+    BTRY(get_geo(b, rows64b, false, &g64b));
     BTRY(get_geo(b, b->rows_q, false, &gq));
     const bool mixed = (precision == QED2D_PREC_MIXED);
-    BGeo g32;
+    BGeo g32, g32b;
     if (mixed) {
         for (int c = 0; c < b->n; ++c)
             if (!pair_ok(b->hch[c].L))
@@ -230,6 +269,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         BTRY(balloc(b, &b->ts, nel));
         BTRY(balloc(b, &b->xs, nel));
         BTRY(get_geo(b, b->rows32, true, &g32));
+        BTRY(get_geo(b, rows32b, true, &g32b));
     }
     // FP32 links packed (one float each) when every link is U(1), as the
     // single-lattice solver; the first look at the states tells, and a solve
@@ -272,11 +312,11 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             if (mixed) {
                 launch_b_cg_k1_f32(g32, b->rows32, b->Us, packed, b->rs, b->p0s, b->p1s, b->ts,
                                    b->xs, b->S, b->P, st);
-                launch_b_cg_k2_f32(g32, b->rows32, b->Us, packed, b->ts, b->p0s, b->p1s, b->xs,
+                launch_b_cg_k2_f32(g32b, rows32b, b->Us, packed, b->ts, b->p0s, b->p1s, b->xs,
                                    b->rs, b->S, b->P, st);
             } else {
                 launch_b_cg_k1_f64(g64, b->rows, b->U, b->r, b->p0, b->p1, b->t, b->S, b->P, st);
-                launch_b_cg_k2_f64(g64, b->rows, b->U, b->t, b->p0, b->p1, x, b->r, b->S, b->P,
+                launch_b_cg_k2_f64(g64b, rows64b, b->U, b->t, b->p0, b->p1, x, b->r, b->S, b->P,
                                    st);
             }
         }
@@ -335,6 +375,92 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     return QED2D_OK;
 }
 
+// The batch's autotuner: times each tile height of the Q apply and every
+// (K1, K2) pair of heights of the FP64 and the FP32 CG on the batch's own
+// lattices and links (a Gaussian source per chain, solves of `iters`
+// iterations that cannot converge: tolerance 0) and keeps the fastest.  The
+// batch's pseudofermion fields (X, F) serve as scratch.
+int batch_autotune(qed2d_batch *b, int iters)
+{
+    const long nel = 2 * b->vtot;
+    hipStream_t st = b->stream;
+    BTRY(balloc(b, &b->X, nel));
+    BTRY(balloc(b, &b->F, nel));
+    for (int c = 0; c < b->n; ++c) {
+        const BChain &ch = b->hch[c];
+        uint32_t k0, k1;
+        key_of(0x74756e65ull + (uint64_t)c, (uint32_t)c, &k0, &k1);
+        launch_fill_pseudofermion(ch.L, b->F + 2 * ch.off, k0, k1, 0, 0, st);
+    }
+    std::vector<double> am0(b->n, 0.1);
+    BTRY(set_masses(b, am0.data()));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BHIP(b, hipEventCreate(&e0));
+    BHIP(b, hipEventCreate(&e1));
+    int rc = QED2D_OK;
+    // one warm-up run, then one timed (statuses of the cut-off solves ignored)
+    auto timed = [&](const std::function<int()> &fn, double *us) -> int {
+        int r = fn();
+        if (r != QED2D_OK && r != QED2D_ENOCONV) return r;
+        if (hipEventRecord(e0, st) != hipSuccess) return QED2D_EHIP;
+        r = fn();
+        if (r != QED2D_OK && r != QED2D_ENOCONV) return r;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return QED2D_EHIP;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return QED2D_EHIP;
+        *us = 1e3 * ms;
+        return QED2D_OK;
+    };
+    const int cand[5] = {1, 2, 4, 8, 16};
+    double best = 1e30, us = 0.0;
+    int pick = b->rows_q;
+    for (int R : cand) {
+        if (rc) break;
+        BGeo gq;
+        rc = get_geo(b, R, false, &gq);
+        if (rc) break;
+        rc = timed([&] {
+            launch_b_q_apply(gq, R, b->U, b->F, b->X, nullptr, st);
+            return blaunched(b);
+        }, &us);
+        if (!rc && us < best) best = us, pick = R;
+    }
+    if (!rc) b->rows_q = pick;
+    bool pairs = true;
+    for (const BChain &ch : b->hch) pairs = pairs && pair_ok(ch.L);
+    std::vector<int> it;
+    for (int prec = 0; prec < 2 && !rc; ++prec) {
+        if (prec == 1 && !pairs) break;  // the FP32 batch needs even L1 >= 4 everywhere
+        int &t1 = prec ? b->rows32 : b->rows;
+        int &t2 = prec ? b->rows32_k2 : b->rows_k2;
+        const int keep1 = t1, keep2 = t2;
+        int p1 = keep1, p2 = keep2;
+        best = 1e30;
+        for (int R1 : cand) {
+            for (int R2 : cand) {
+                t1 = R1;
+                t2 = R2;
+                rc = timed([&] {
+                    return batch_cg(b, b->X, b->F, iters, 0.0,
+                                    prec ? QED2D_PREC_MIXED : QED2D_PREC_F64, it);
+                }, &us);
+                if (rc) break;
+                if (us < best) best = us, p1 = R1, p2 = R2;
+            }
+            if (rc) break;
+        }
+        t1 = rc ? keep1 : p1;
+        t2 = rc ? keep2 : (p2 == p1 ? 0 : p2);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return bfail(b, rc, "batch autotune failed: %s", b->err.c_str());
+    b->err.clear();
+    BHIP(b, hipStreamSynchronize(st));
+    return QED2D_OK;
+}
+
 int batch_update_momenta(qed2d_batch *b, double eps, int maxiter, double tol, int precision,
                          std::vector<long> &iters)
 {
@@ -378,11 +504,6 @@ int batch_md(qed2d_batch *b, int integrator, double eps, int ns, int maxiter, do
     return blaunched(b);
 }
 
-void key_of(uint64_t seed, uint32_t chain, uint32_t *k0, uint32_t *k1)
-{
-    *k0 = (uint32_t)seed;
-    *k1 = (uint32_t)(seed >> 32) ^ chain;
-}
 
 }  // namespace
 
@@ -422,6 +543,7 @@ int qed2d_batch_create(qed2d_batch **out, int nchain, const int *L1, const int *
     }
     b->vtot = off;
     b->nU = 2 * off;
+    default_batch_tiles(b);
     b->last_dh.assign(nchain, 0.0);
     const long nel = 2 * off;
     bool ok = hipMalloc((void **)&b->dch, sizeof(BChain) * nchain) == hipSuccess &&
@@ -543,8 +665,46 @@ int qed2d_batch_set_tiles(qed2d_batch *b, int q64, int cg64, int cg32)
     if (!ok(q64) || !ok(cg64) || !ok(cg32))
         return bfail(b, QED2D_EINVAL, "tile rows must be 0, 1, 2, 4, 8 or 16");
     if (q64) b->rows_q = q64;
-    if (cg64) b->rows = cg64;
-    if (cg32) b->rows32 = cg32;
+    if (cg64) {
+        b->rows = cg64;
+        b->rows_k2 = 0;
+    }
+    if (cg32) {
+        b->rows32 = cg32;
+        b->rows32_k2 = 0;
+    }
+    return QED2D_OK;
+}
+
+int qed2d_batch_set_tiles_k2(qed2d_batch *b, int cg64, int cg32)
+{
+    if (!b) return QED2D_EINVAL;
+    auto ok = [](int r) { return r == 0 || r == 1 || r == 2 || r == 4 || r == 8 || r == 16; };
+    if (!ok(cg64) || !ok(cg32))
+        return bfail(b, QED2D_EINVAL, "tile rows must be 0, 1, 2, 4, 8 or 16");
+    b->rows_k2 = cg64;
+    b->rows32_k2 = cg32;
+    return QED2D_OK;
+}
+
+int qed2d_batch_get_tiles(qed2d_batch *b, int *tiles)
+{
+    if (!b || !tiles) return QED2D_EINVAL;
+    tiles[0] = b->rows_q;
+    tiles[1] = b->rows;
+    tiles[2] = b->rows32;
+    tiles[3] = b->rows_k2 ? b->rows_k2 : b->rows;
+    tiles[4] = b->rows32_k2 ? b->rows32_k2 : b->rows32;
+    return QED2D_OK;
+}
+
+int qed2d_batch_autotune(qed2d_batch *b, int iters, int *tiles)
+{
+    if (!b) return QED2D_EINVAL;
+    if (b->shared) return bfail(b, QED2D_EINVAL, "a right-hand-side set tunes through its context");
+    if (iters < 4) iters = 4;
+    BTRY(batch_autotune(b, iters));
+    if (tiles) qed2d_batch_get_tiles(b, tiles);
     return QED2D_OK;
 }
 
@@ -675,11 +835,13 @@ namespace qed2d {
 
 int batch_rhs_cg(qed2d_batch *b, const double *U, double *x, const double *src, double am0,
                  int maxiter, double eps, int precision, int *iters, int rows_q, int rows,
-                 int rows32, double ru_delta, std::string *err)
+                 int rows32, int rows_k2, int rows32_k2, double ru_delta, std::string *err)
 {
     b->rows_q = rows_q;
     b->rows = rows;
     b->rows32 = rows32;
+    b->rows_k2 = rows_k2;
+    b->rows32_k2 = rows32_k2;
     b->ru_delta = ru_delta;
     std::vector<double> m(b->n, am0);
     const int rc = qed2d_batch_cg_shared(b, U, x, src, m.data(), maxiter, eps, precision, iters);

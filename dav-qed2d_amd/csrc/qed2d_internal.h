@@ -178,7 +178,7 @@ struct BParts {
 // on failure *err receives the set's message.
 int batch_rhs_cg(qed2d_batch *b, const double *U, double *x, const double *src, double am0,
                  int maxiter, double eps, int precision, int *iters, int rows_q, int rows,
-                 int rows32, double ru_delta, std::string *err);
+                 int rows32, int rows_k2, int rows32_k2, double ru_delta, std::string *err);
 // mask (may be null): per chain, nonzero = apply on that chain.
 void launch_b_q_apply(const BGeo &g, int rows, const double2 *U, const double2 *si,
                       double2 *so, const int *mask, hipStream_t st);

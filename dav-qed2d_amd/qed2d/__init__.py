@@ -67,6 +67,9 @@ SIGNATURES = {
     "qed2d_batch_get_links": (_i, [_p, _i, _p]),
     "qed2d_batch_set_links": (_i, [_p, _i, _p]),
     "qed2d_batch_set_tiles": (_i, [_p, _i, _i, _i]),
+    "qed2d_batch_set_tiles_k2": (_i, [_p, _i, _i]),
+    "qed2d_batch_get_tiles": (_i, [_p, _p]),
+    "qed2d_batch_autotune": (_i, [_p, _i, _p]),
     "qed2d_batch_hot_start": (_i, [_p, _p]),
     "qed2d_batch_cg": (_i, [_p, _p, _p, _p, _i, _d, _i, _p]),
     "qed2d_batch_create_rhs": (_i, [_p, _i, _i, _i, _d, _i, _p]),
@@ -883,8 +886,27 @@ class Batch:
         L1, L2 = self.sizes[c]
         self._check(lib.qed2d_batch_set_links(self._b, c, _ptr(U, None, (2, L2, L1))))
 
-    def set_tiles(self, q64=0, cg64=0, cg32=0):
+    def set_tiles(self, q64=0, cg64=0, cg32=0, cg64_k2=None, cg32_k2=None):
+        """Tile heights of the FP64 apply and the FP64 / FP32 solver kernels (0
+        leaves one unchanged); cg64_k2 / cg32_k2 give the second CG kernel's."""
         self._check(lib.qed2d_batch_set_tiles(self._b, int(q64), int(cg64), int(cg32)))
+        if cg64_k2 is not None or cg32_k2 is not None:
+            cur = self.get_tiles()
+            a = cur["cg64"] if cg64_k2 is None else cg64_k2
+            b = cur["cg32"] if cg32_k2 is None else cg32_k2
+            self._check(lib.qed2d_batch_set_tiles_k2(self._b, int(a), int(b)))
+
+    def get_tiles(self):
+        t = (_i * 5)()
+        self._check(lib.qed2d_batch_get_tiles(self._b, t))
+        return dict(zip(("q64", "cg64", "cg32", "cg64_k2", "cg32_k2"), list(t)))
+
+    def autotune(self, iters=20):
+        """Times every tile height on the batch's own lattices and links and
+        keeps the fastest (returns the choice, as get_tiles)."""
+        t = (_i * 5)()
+        self._check(lib.qed2d_batch_autotune(self._b, int(iters), t))
+        return self.get_tiles()
 
     def hot_start(self, seeds):
         s = (_u64 * self.n)(*[int(x) for x in seeds])

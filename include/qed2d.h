@@ -265,8 +265,19 @@ int qed2d_batch_links(qed2d_batch *b, int chain, double **U);
 int qed2d_batch_get_links(qed2d_batch *b, int chain, double *dst);
 int qed2d_batch_set_links(qed2d_batch *b, int chain, const double *src);
 /* Tile heights of the batch's FP64 apply, FP64 and FP32 solver kernels
- * (0 leaves one unchanged). */
+ * (0 leaves one unchanged; a nonzero cg64 / cg32 also resets the second CG
+ * kernel's height to follow it).  The defaults are the tallest heights (16 / 8
+ * rows) that leave a solver launch of at least 1024 blocks over all chains.
+ * set_tiles_k2: the second CG kernel's heights (0: the family's).  get_tiles:
+ * q64, cg64, cg32, cg64 of K2, cg32 of K2 in use.  autotune: times every apply
+ * height and every (K1, K2) pair of the FP64 and FP32 CG on the batch's own
+ * lattices and links over `iters` iterations (the pseudofermion fields serve
+ * as scratch) and keeps the fastest.  No counterpart in the reference
+ * (KernelParm, src/QED2d.jl:10-13). */
 int qed2d_batch_set_tiles(qed2d_batch *b, int q64, int cg64, int cg32);
+int qed2d_batch_set_tiles_k2(qed2d_batch *b, int cg64, int cg32);
+int qed2d_batch_get_tiles(qed2d_batch *b, int tiles[5]);
+int qed2d_batch_autotune(qed2d_batch *b, int iters, int tiles[5]);
 /* Hot start of every chain: link angles uniform in [-pi, pi) from (seed[c], c). */
 int qed2d_batch_hot_start(qed2d_batch *b, const uint64_t *seed);
 /* (Q^2) x = src on every chain from x = 0 (CG v1 semantics per chain); x and

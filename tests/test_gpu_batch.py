@@ -141,8 +141,9 @@ def test_cg_mrhs_bit_exact(Q, O, L1, L2, prec):
                       None, precision=prec)
             assert it == its[j], (k, j, it, its[j])
             assert torch.equal(x, so[j]), (k, j)
-    # the tiles in effect are the context's: change them and compare again
-    Q.set_tiles(prm, q64=4, cg64=8, cg32=4)
+    # the tiles in effect are the context's (the second CG kernel's included):
+    # change them and compare again
+    Q.set_tiles(prm, q64=4, cg64=8, cg32=4, cg64_k2=2, cg32_k2=8)
     si = torch.zeros((2, 2, L2, L1), dtype=torch.complex128, device="cuda")
     for j in range(2):
         Q.fill_pseudofermion(si[j], prm, 32, 0, 0, j)
@@ -153,3 +154,41 @@ def test_cg_mrhs_bit_exact(Q, O, L1, L2, prec):
         assert Q.CG(x, U, si[j].contiguous(), 0.15, 10000, 1e-20, Q.gamm5Dw_sqr_msq, prm, None,
                     precision=prec) == its[j]
         assert torch.equal(x, so[j]), j
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_split_tiles_and_autotune_bit_exact(Q, O, prec):
+    """A batch whose CG kernels K1 and K2 run different tile heights (set, then
+    chosen by the batch's autotuner) reproduces single-lattice CG with the same
+    heights bit for bit; the default heights leave at least 1024 blocks."""
+    import torch
+    say("batch split tiles / autotune prec", prec)
+    sizes = [(16, 16), (32, 24), (64, 64), (30, 18)]
+    betas = [2.0, 4.0, 6.0, 4.0]
+    am0s = [0.1, 0.2, 0.15, 0.3]
+    B = Q.Batch(sizes, betas)
+    d = B.get_tiles()
+    assert all(v in (1, 2, 4, 8, 16) for v in d.values())
+    B.hot_start([41, 42, 43, 44])
+    src = B.arena_spinor()
+    for c, (L1, L2) in enumerate(sizes):
+        prm = Q.LattParm((L1, L2), betas[c])
+        Q.fill_pseudofermion(B.field(src, c), prm, 600 + c, c, 0, 0)
+    for mode in ("split", "autotune"):
+        if mode == "split":
+            B.set_tiles(2, 4, 2, cg64_k2=16, cg32_k2=8)
+        else:
+            B.autotune(8)
+        t = B.get_tiles()
+        say("batch tiles", mode, t)
+        x = B.arena_spinor()
+        its = B.CG(x, src, am0s, 10000, 1e-20, precision=prec)
+        for c, (L1, L2) in enumerate(sizes):
+            prm = Q.LattParm((L1, L2), betas[c])
+            Q.set_tiles(prm, q64=t["q64"], cg64=t["cg64"], cg32=t["cg32"],
+                        cg64_k2=t["cg64_k2"], cg32_k2=t["cg32_k2"])
+            xs = Q.spinor(prm)
+            it = Q.CG(xs, B.get_links(c), B.field(src, c).contiguous(), am0s[c], 10000, 1e-20,
+                      Q.gamm5Dw_sqr_msq, prm, None, precision=prec)
+            assert it == its[c], (mode, c, it, its[c])
+            assert torch.equal(xs, B.field(x, c)), (mode, c)
