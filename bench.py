@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="FP64 tile rows (0: library default)")
     ap.add_argument("--tile32", type=int, default=0, help="FP32 tile rows (0: library default)")
     ap.add_argument("--no-dw", action="store_true")
+    ap.add_argument("--dw-only", action="store_true",
+                    help="only the STREAM and gamm5Dw apply measurements (one JSON line)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--accept-traj", type=int, default=100,
@@ -93,6 +95,13 @@ def main():
     import qed2d as Q
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.dw_only:
+        sprm = Q.LattParm((64, 64), 4.0)
+        stream = Q.hbm_stream(sprm, 1 << 30)
+        sprm.close()
+        print(json.dumps({"hbm_stream_GBs": {k: round(v, 1) for k, v in stream.items()},
+                          "dw_apply": dw_apply(Q, torch, args)}), flush=True)
+        return
     L = args.L
     prm = Q.LattParm((L, L), args.beta)
     if args.tile:
@@ -651,16 +660,21 @@ def rocprof_summary(argv):
     ap.add_argument("write")
     ap.add_argument("--L", type=int, default=1024)
     ap.add_argument("--json", default="")
+    ap.add_argument("--by-grid", action="store_true",
+                    help="one row per (kernel, grid size): shapes that share a kernel")
     a = ap.parse_args(argv)
     con = sqlite3.connect(a.kt)
     by = {}
-    for n, d in con.execute("select name, duration from kernels"):
-        by.setdefault(_short(n), []).append(d / 1000.0)
+
+    def key(n, g):
+        return f"{_short(n)} [grid {g}]" if a.by_grid else _short(n)
+    for n, d, g in con.execute("select name, duration, grid_x from kernels"):
+        by.setdefault(key(n, g), []).append(d / 1000.0)
     rows = sorted(((n, np.array(d)) for n, d in by.items()), key=lambda r: -r[1].sum())
     total = sum(d.sum() for _, d in rows)
     print("| kernel | calls | total us | mean us | median us (calls > 5 us) | % time |")
     print("|---|---|---|---|---|---|")
-    for n, d in rows[:16]:
+    for n, d in rows[:40 if a.by_grid else 16]:
         w = d[d > 5.0]
         print(f"| {n} | {len(d)} | {d.sum():.0f} | {d.mean():.2f} | "
               f"{np.median(w) if len(w) else 0.0:.2f} | {100 * d.sum() / total:.1f} |")
@@ -668,16 +682,17 @@ def rocprof_summary(argv):
     def counter(db, cname):
         c = sqlite3.connect(db)
         out = {}
-        for n, v in c.execute("select kernel_name, value from counters_collection "
-                              "where counter_name=?", (cname,)):
-            out.setdefault(_short(n), []).append(v)
+        for n, v, g in c.execute("select kernel_name, value, grid_size_x from "
+                                 "counters_collection where counter_name=?", (cname,)):
+            out.setdefault(key(n, g), []).append(v)
         return {n: np.array(v) for n, v in out.items()}
     F, W = counter(a.fetch, "FETCH_SIZE"), counter(a.write, "WRITE_SIZE")
     gib_kib = float(1 << 20)
-    if "k_stream_copy" not in F or "k_stream_copy" not in W:
+    cp = [n for n in F if n.startswith("k_stream_copy")]
+    if not cp or cp[0] not in W:
         sys.exit("the counter runs hold no k_stream_copy dispatch to calibrate against")
-    fcal = gib_kib / np.median(F["k_stream_copy"])
-    wcal = gib_kib / np.median(W["k_stream_copy"])
+    fcal = gib_kib / np.median(F[cp[0]])
+    wcal = gib_kib / np.median(W[cp[0]])
     print()
     print(f"Calibration (STREAM copy, 1 GiB read + 1 GiB written per dispatch): FETCH_SIZE x "
           f"{fcal:.3f}, WRITE_SIZE x {wcal:.3f}")

@@ -249,6 +249,25 @@ static double reduce_terms(term_fn f, const void *ctx, long n)
     return s + c;
 }
 
+/* reduce_terms with its Neumaier sum and compensation added in long double
+ * (x86-64: 64-bit mantissa), for energies whose differences must be resolved
+ * below the ulp of a double. */
+static long double reduce_terms_ld(term_fn f, const void *ctx, long n)
+{
+    double part[NCHUNK], cpart[NCHUNK];
+#pragma omp parallel for
+    for (int ch = 0; ch < NCHUNK; ++ch) {
+        long lo = n * ch / NCHUNK, hi = n * (ch + 1) / NCHUNK;
+        double s = 0.0, c = 0.0;
+        for (long i = lo; i < hi; ++i) neumaier_add(&s, &c, f(ctx, i));
+        part[ch] = s;
+        cpart[ch] = c;
+    }
+    long double s = 0.0L;
+    for (int ch = 0; ch < NCHUNK; ++ch) s += (long double)part[ch] + (long double)cpart[ch];
+    return s;
+}
+
 typedef struct { const double *a, *b; } pair_ctx;
 
 static double term_abs2(const void *ctx, long i)
@@ -1033,6 +1052,31 @@ int orc_hmc_pure(double *U, double eps, int ns, int integrator, int qzero, uint6
     return 0;
 }
 
+/* H + Re X^dag Y (src/QED2dHMC.jl:74,82) in extended precision: the
+ * compensated sums of orc_hamiltonian and orc_dot combined in long double, so
+ * that the difference of two such energies of order 1e6 keeps ~1e-12. */
+static long double hamiltonian_f_ld(const double *mom, const double *U, double beta, int L1,
+                                    int L2, const double *X, const double *Y)
+{
+    long V = (long)L1 * L2;
+    pair_ctx m = {mom, mom};
+    long double kin = reduce_terms_ld(term_sq, &m, 2 * V) / 2.0L;
+    double *plx = (double *)malloc(sizeof(double) * V);
+    orc_plaquette_field(plx, U, L1, L2);
+    pair_ctx pc = {plx, plx};
+    long double sp = reduce_terms_ld(term_val, &pc, V);
+    free(plx);
+    pair_ctx f = {X, Y};
+    long double xf = reduce_terms_ld(term_dot_re, &f, 2 * V);
+    return kin + (long double)beta * ((long double)V - sp) + xf;
+}
+
+static double last_dh;
+
+/* dH of the last orc_hmc_nf2 trajectory from the extended-precision energies
+ * (hfin - hini of the returned doubles loses the bits below the ulp of H). */
+double orc_last_dh(void) { return last_dh; }
+
 /* Nf=2 HMC step: src/QED2dHMC.jl:64-113.  hfin uses the last MD solve (:82). */
 int orc_hmc_nf2(double *U, double am0, double eps, int ns, int integrator, int maxiter,
                 double tol, int qzero, uint64_t seed, uint32_t chain, uint32_t traj,
@@ -1050,6 +1094,8 @@ int orc_hmc_nf2(double *U, double am0, double eps, int ns, int integrator, int m
     double xr, xi, fr, fi;
     orc_dot(X, X, 2 * V, &xr, &xi);
     *hini = orc_hamiltonian(mom, U, p->beta, p->L1, p->L2) + xr;
+    const long double hini_ld = hamiltonian_f_ld(mom, U, p->beta, p->L1, p->L2, X, X);
+    last_dh = 0.0;
     orc_gamm5Dw(F, U, X, am0, p->L1, p->L2, p->bc);
     int it;
     if (integrator == 1) it = orc_omf4_nf2(mom, U, X, F, g5DX, am0, eps, ns, maxiter, tol, p);
@@ -1059,6 +1105,7 @@ int orc_hmc_nf2(double *U, double am0, double eps, int ns, int integrator, int m
     if (it < 0) { rc = 1; memcpy(U, Ucp, sizeof(double) * 4 * V); *accepted = 0; goto out; }
     orc_dot(X, F, 2 * V, &fr, &fi);
     *hfin = orc_hamiltonian(mom, U, p->beta, p->L1, p->L2) + fr;
+    last_dh = (double)(hamiltonian_f_ld(mom, U, p->beta, p->L1, p->L2, X, F) - hini_ld);
     *accepted = metropolis(*hini, *hfin, qzero, U, p, seed, chain, traj);
     if (!*accepted) memcpy(U, Ucp, sizeof(double) * 4 * V);
 out:

@@ -59,6 +59,34 @@ def _thermalised_1024(Q, beta, n):
     return prm, U
 
 
+def test_nf2_1024_dh_vs_oracle(Q, O):
+    """The headline trajectory: Nf=2 at 1024x1024, beta 4, m0 0.2, leapfrog
+    ns 10 eps 0.1, CG tol 1e-24, links after 200 pure-gauge trajectories.
+    Mixed precision (FP32 force solves, FP64 reliable updates) and FP64 on the
+    GPU against the FP64 oracle on the same links and random streams, hfin
+    from the last MD solve as the reference: the same accept/reject and
+    |dH - dH_oracle| < 1e-9 (both dH from energies kept below the ulp of H,
+    which is ~5e-10 at H ~ 3e6)."""
+    import torch
+    say("nf2 1024^2 dH vs oracle")
+    prm, U = _thermalised_1024(Q, 4.0, 200)
+    Uh = host(U).copy()
+    p = O.latt(1024, 1024, 4.0)
+    hi, hf, a, it, rc = O.hmc_nf2(Uh, 0.2, 0.1, 10, 100000, 1e-24, p, 1234, 0)
+    assert rc == 0
+    dh_o = O.last_dh()
+    say("oracle dH", dh_o, "iterations", it)
+    U0 = U.clone()
+    for prec in (1, 0):
+        U.copy_(U0)
+        Q.seed(1234)
+        info = Q.HMC(U, 0.2, 0.1, 10, [], 100000, 1e-24, prm, None, precision=prec)
+        torch.cuda.synchronize()
+        say("prec", prec, "dH", info.dH, "diff", info.dH - dh_o, "it", info.cg_iters)
+        assert info.accepted == a
+        assert abs(info.dH - dh_o) < 1e-9
+
+
 def test_cg_16rhs_1024_fp64_vs_mixed(Q, O):
     """Links from 200 pure-gauge OMF4 trajectories at beta 4 from a cold start,
     16 complex Gaussian right-hand sides, m0 0.1, CG to |r|^2 < 1e-12 |b|^2

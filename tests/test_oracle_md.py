@@ -153,6 +153,23 @@ def test_hmc_nf2_driver_accounting(O):
         assert np.array_equal(U, U0)
 
 
+@pytest.mark.parametrize("L,integ", [(8, 0), (24, 0), (24, 1), (64, 0)])
+def test_hmc_nf2_extended_precision_dh(O, L, integ):
+    """last_dh (energies in extended precision) equals hfin - hini where the
+    doubles resolve it: H is ~1e2 .. 1e4 here, so hfin - hini carries at most
+    ~1e-12 of rounding, while a dropped or mis-signed term of H (kinetic,
+    action, X^dag X / Re X^dag F) would differ by O(1)."""
+    p = O.latt(L, L, 4.0)
+    U = O.hot_start(L, L, 99 + L)
+    for t in range(3):
+        hi, hf, acc, it, rc = O.hmc_nf2(U, 0.3, 0.1, 4, 10000, 1e-24, p, 4321, t,
+                                        integrator=integ)
+        assert rc == 0
+        dh = O.last_dh()
+        assert abs(dh - (hf - hi)) < 4e-16 * abs(hi) + 1e-13, (t, dh, hf - hi)
+        assert dh != 0.0
+
+
 def test_rhmc_reversibility(O):
     """test/rhmc_hmc_reversibility.jl: 10x10, am0 = 10, n = 5 poles on the
     power-method bounds, 100 steps of eps = 0.01, tol 1e-25: |dS_gauge| < 1e-10."""
