@@ -581,10 +581,99 @@ __global__ void k_force_mom(Latt L, const double2 *__restrict__ U, const double2
          s += (long)gridDim.x * blockDim.x)
         force_mom_site(L, U, X, Y, facc, mom, beta, eps, cf, s);
 }
+// P = U1(x) U2(x+1) conj(U1(x+2) U2(x)) from the four links, the expression
+// of plaq_at.
+__device__ __forceinline__ double2 plaq4(double2 u1, double2 u2r, double2 u1n, double2 u2)
+{
+    const double2 a = cmul<double>(u1, u2r);
+    const double2 b = cmul<double>(u1n, u2);
+    return cmul<double>(a, cconj<double>(b));
+}
+
+// The gauge force alone (the gather form of src/QED2dAction.jl:34-53) and the
+// momentum update, mom += eps (F_gauge + facc), marching rows as the stencil
+// does: a thread owns column i1 and rows r0 .. r0+R-1, carries v = beta Im P
+// of the row below (P(x-2)), and takes U2(x+1) and v(x-1) from the adjacent
+// lanes, loading or computing them at the wave's edges and the periodic wrap.
+// Each link is loaded once, plus one halo row per tile; per site
+// g1 = -v(x) + v(x-2), g2 = v(x) - v(x-1) as force_mom_site.
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_gauge_mom(Latt L, const double2 *__restrict__ U,
+                                                      const double *__restrict__ facc,
+                                                      double *__restrict__ mom, double beta,
+                                                      double eps, int bx, int by, int nbx)
+{
+    const MarchPos p = march_pos(L, bx, by, nbx, R);
+    if (!p.active) return;
+    const int L1 = L.L1, L2 = L.L2;
+    const long V = L.V;
+    const int i1 = p.i1, r0 = p.r0;
+    const int i1p = (i1 + 1 == L1) ? 0 : i1 + 1;
+    const int i1m = (i1 == 0) ? L1 - 1 : i1 - 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool ldR = (lane == kWave - 1) || (i1p == 0);  // U2(x+1) not in lane+1
+    const bool ldL = (lane == 0) || (i1 == 0);           // v(x-1) not in lane-1
+    const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
+    const int nr = (r0 + R <= L2) ? R : L2 - r0;
+    const long sm = (long)rm * L1, s0 = (long)r0 * L1;
+    // the row below the tile: its plaquette is P(x-2) of row r0
+    double2 u1 = U[sm + i1], u2 = U[V + sm + i1];
+    double2 u1n = U[s0 + i1];
+    double2 u2r = from_right<double>(u2);
+    if (ldR) u2r = U[V + sm + i1p];
+    double vprev = beta * plaq4(u1, u2r, u1n, u2).y;
+    double2 l1{};  // U1(x-1) of the current row (edge lanes)
+    if (ldL) l1 = U[s0 + i1m];
+    u1 = u1n;
+    u2 = U[V + s0 + i1];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (k >= nr) break;
+        const int r = r0 + k;
+        const int rn = (r + 1 == L2) ? 0 : r + 1;
+        const long row = (long)r * L1, s = row + i1, sn = (long)rn * L1;
+        u1n = U[sn + i1];
+        double2 u2n{};
+        if (k + 1 < nr) u2n = U[V + sn + i1];
+        double m1 = mom[s], m2 = mom[V + s];
+        u2r = from_right<double>(u2);
+        if (ldR) u2r = U[V + row + i1p];
+        const double v = beta * plaq4(u1, u2r, u1n, u2).y;
+        double vl = lane_shift(v, true);
+        if (ldL) {
+            const double2 l2 = U[V + row + i1m];
+            const double2 l1n = U[sn + i1m];
+            vl = beta * plaq4(l1, u2, l1n, l2).y;  // P(x-1)
+            l1 = l1n;
+        }
+        double g1 = -v + vprev;
+        double g2 = v - vl;
+        if (facc) {
+            g1 = g1 + facc[s];
+            g2 = g2 + facc[V + s];
+        }
+        mom[s] = m1 + eps * g1;
+        mom[V + s] = m2 + eps * g2;
+        vprev = v;
+        u1 = u1n;
+        u2 = u2n;
+    }
+}
+
 void launch_force_mom(const Latt &L, const double2 *U, const double2 *X, const double2 *Y,
                       const double *facc, double *mom, double beta, double eps, double cf,
                       hipStream_t st)
 {
+    if (!X) {
+        // pure gauge (+ an accumulated force): the marching kernel, 8 rows per
+        // thread where the lattice leaves 1024 blocks, fewer below
+        int rows = 8;
+        while (rows > 1 && march_grid(L, rows).nblocks() < 1024) rows >>= 1;
+        const MarchGrid g = march_grid(L, rows);
+        DISPATCH_ROWS(rows, k_gauge_mom, dim3(g.nblocks()), dim3(kBlock), 0, st, L, U, facc,
+                      mom, beta, eps, g.bx, g.by, g.nbx);
+        return;
+    }
     launch_k(k_force_mom, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, facc,
                        mom, beta, eps, cf);
 }

@@ -142,6 +142,25 @@ def test_forces(Q, O, L1, L2, bc):
     assert relerr(host(dm) - mom0, ref - mom0) < 1e-12
 
 
+@pytest.mark.parametrize("L1,L2", [(8, 8), (24, 24), (33, 20), (64, 7), (100, 37),
+                                   (256, 128), (1024, 512)])
+def test_gauge_force_mom(Q, O, L1, L2):
+    """The pure-gauge force + momentum update (the marching kernel, mom +=
+    eps (frc1 + frc2)): widths below, at and above a wave, odd widths, row
+    counts that the tiles do not divide, against the oracle's krnl_force."""
+    say("gauge force + mom", L1, L2)
+    rng = np.random.default_rng(7 * L1 + L2)
+    U = random_links(L1, L2, rng, unitary=(L1 % 2 == 0))
+    prm = Q.LattParm((L1, L2), 2.75)
+    r1, r2 = O.krnl_force(U, 2.75)
+    mom0 = rng.standard_normal((2, L2, L1))
+    dm = dev(mom0)
+    Q.force_mom_update(dm, dev(U), None, None, 0.07, prm)
+    ref = mom0 + 0.07 * (r1 + r2)
+    assert relerr(host(dm) - mom0, ref - mom0) < 1e-13
+    assert relerr(host(dm), ref) < 1e-15
+
+
 def test_updt_fld_and_observables(Q, O):
     say("updt_fld / observables")
     L1, L2 = 96, 80
