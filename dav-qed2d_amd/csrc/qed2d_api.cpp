@@ -25,10 +25,10 @@ using namespace qed2d;
 // ---------------------------------------------------------------------------
 enum SpinorSlot {
     S_R, S_P0, S_P1, S_T, S_TMP, S_TMP2, S_UCP, S_X, S_F, S_G5DX, S_AUX, S_PB, S_PBA, S_PBL,
-    S_RW_X, S_RW_Z, S_RW_IN, S_RW_T1, S_RW_T2, S_NSLOTS
+    S_RW_X, S_RW_Z, S_RW_IN, S_RW_T1, S_RW_T2, S_UPP, S_NSLOTS
 };
 enum FloatSlot { FS_R, FS_P0, FS_P1, FS_T, FS_X, FS_U, FS_NSLOTS };
-enum RealSlot { R_MOM, R_FACC, R_NSLOTS };
+enum RealSlot { R_MOM, R_FACC, R_MPP, R_NSLOTS };
 
 struct qed2d_ctx {
     Latt L;
@@ -917,32 +917,71 @@ void gauge_mom(qed2d_ctx *c, double *mom, const double *U, double coef)
                      c->stream);
 }
 
-// OMF4! pure gauge (src/QED2dHMC.jl:239-305)
+// The pure-gauge integrators run each link update U(eu) together with the
+// momentum update P(ep) that follows it (launch_updt_gauge_mom), out of place:
+// (U, mom) and a scratch pair take turns, and the result is copied back when
+// it ends in the scratch pair.
+struct PurePair {
+    double2 *U[2];
+    double *m[2];
+    int cur = 0;
+};
+
+int pure_pair(qed2d_ctx *c, double *mom, double *U, PurePair *pp)
+{
+    pp->U[0] = (double2 *)U;
+    pp->m[0] = mom;
+    TRY(get_spin(c, S_UPP, &pp->U[1]));
+    TRY(get_real(c, R_MPP, &pp->m[1]));
+    pp->cur = 0;
+    return QED2D_OK;
+}
+
+void updt_gauge(qed2d_ctx *c, PurePair &pp, double eu, double ep)
+{
+    const int a = pp.cur, b = a ^ 1;
+    PROF(c, QED2D_PROF_UPDT_FLD,
+         launch_updt_gauge_mom(c->L, pp.U[a], pp.m[a], pp.U[b], pp.m[b], eu, c->beta, ep,
+                               c->stream));
+    pp.cur = b;
+}
+
+void pure_finish(qed2d_ctx *c, PurePair &pp)
+{
+    if (pp.cur == 0) return;
+    launch_copy(pp.U[0], pp.U[1], 2 * c->L.V, c->stream);
+    launch_copy((double2 *)pp.m[0], (const double2 *)pp.m[1], c->L.V, c->stream);
+    pp.cur = 0;
+}
+
+// OMF4! pure gauge (src/QED2dHMC.jl:239-305): per step P(r1) [U(r2) P(r3)]
+// [U(r4) P(r5)] [U(r6) P(r5)] [U(r4) P(r3)] [U(r2) P(r1)], each bracket one pass.
 int omf4_pure(qed2d_ctx *c, double *mom, double *U, double eps, int ns)
 {
     const double r1 = OMF_R1, r2 = OMF_R2, r3 = OMF_R3, r4 = OMF_R4;
     const double r5 = 0.5 - r1 - r3, r6 = 1.0 - 2.0 * (r2 + r4);
     const double pc[6] = {r1, r3, r5, r5, r3, r1};
     const double uc[5] = {r2, r4, r6, r4, r2};
+    PurePair pp;
+    TRY(pure_pair(c, mom, U, &pp));
     for (int i = 0; i < ns; ++i) {
-        for (int s = 0; s < 6; ++s) {
-            gauge_mom(c, mom, U, pc[s] * eps);
-            if (s < 5) updt(c, U, mom, eps * uc[s]);
-        }
+        gauge_mom(c, pp.m[pp.cur], (const double *)pp.U[pp.cur], pc[0] * eps);
+        for (int s = 0; s < 5; ++s) updt_gauge(c, pp, eps * uc[s], pc[s + 1] * eps);
     }
+    pure_finish(c, pp);
     return launched(c);
 }
 
-// Leapfrog for the pure gauge theory (stage order of src/QED2dHMC.jl:426-449).
+// Leapfrog for the pure gauge theory (stage order of src/QED2dHMC.jl:426-449):
+// P(eps/2) [U(eps) P(eps)]^(ns-1) [U(eps) P(eps/2)].
 int leapfrog_pure(qed2d_ctx *c, double *mom, double *U, double eps, int ns)
 {
-    gauge_mom(c, mom, U, 0.5 * eps);
-    for (int i = 0; i < ns - 1; ++i) {
-        updt(c, U, mom, eps);
-        gauge_mom(c, mom, U, eps);
-    }
-    updt(c, U, mom, eps);
-    gauge_mom(c, mom, U, 0.5 * eps);
+    PurePair pp;
+    TRY(pure_pair(c, mom, U, &pp));
+    gauge_mom(c, pp.m[pp.cur], (const double *)pp.U[pp.cur], 0.5 * eps);
+    for (int i = 0; i < ns - 1; ++i) updt_gauge(c, pp, eps, eps);
+    updt_gauge(c, pp, eps, 0.5 * eps);
+    pure_finish(c, pp);
     return launched(c);
 }
 

@@ -660,6 +660,99 @@ __global__ void __launch_bounds__(kBlock) k_gauge_mom(Latt L, const double2 *__r
     }
 }
 
+// exp(i eps m) u, the link update of k_updt_fld.
+__device__ __forceinline__ double2 updt_link(double2 u, double m, double eps)
+{
+    double sn, cs;
+    sincos(eps * m, &sn, &cs);
+    return cmul<double>(mk<double>(cs, sn), u);
+}
+
+// The link update U(eu) and then the pure-gauge momentum update P(ep) of an
+// integrator stage pair, in one pass: U' = exp(i eu p) U, p' = p + ep F(U').
+// Out of place (Ui, mi -> Uo, mo), since the tiles around a tile read the old
+// links and momenta of its edge rows.  The march of k_gauge_mom, where every
+// link a thread uses is formed from the old link and momentum as it is read;
+// each thread stores the updated links and momenta of its own sites.
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_updt_gauge_mom(Latt L, const double2 *__restrict__ Ui,
+                                                           const double *__restrict__ mi,
+                                                           double2 *__restrict__ Uo,
+                                                           double *__restrict__ mo, double eu,
+                                                           double beta, double ep, int bx, int by,
+                                                           int nbx)
+{
+    const MarchPos p = march_pos(L, bx, by, nbx, R);
+    if (!p.active) return;
+    const int L1 = L.L1, L2 = L.L2;
+    const long V = L.V;
+    const int i1 = p.i1, r0 = p.r0;
+    const int i1p = (i1 + 1 == L1) ? 0 : i1 + 1;
+    const int i1m = (i1 == 0) ? L1 - 1 : i1 - 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool ldR = (lane == kWave - 1) || (i1p == 0);
+    const bool ldL = (lane == 0) || (i1 == 0);
+    const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
+    const int nr = (r0 + R <= L2) ? R : L2 - r0;
+    const long sm = (long)rm * L1, s0 = (long)r0 * L1;
+    auto upd = [&](long i) { return updt_link(Ui[i], mi[i], eu); };
+    double2 u1 = upd(sm + i1), u2 = upd(V + sm + i1);
+    double2 u1n = upd(s0 + i1);
+    double2 u2r = from_right<double>(u2);
+    if (ldR) u2r = upd(V + sm + i1p);
+    double vprev = beta * plaq4(u1, u2r, u1n, u2).y;
+    double2 l1{};
+    if (ldL) l1 = upd(s0 + i1m);
+    u1 = u1n;
+    u2 = upd(V + s0 + i1);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (k >= nr) break;
+        const int r = r0 + k;
+        const int rn = (r + 1 == L2) ? 0 : r + 1;
+        const long row = (long)r * L1, s = row + i1, sn = (long)rn * L1;
+        u1n = upd(sn + i1);
+        double2 u2n{};
+        if (k + 1 < nr) u2n = upd(V + sn + i1);
+        const double m1 = mi[s], m2 = mi[V + s];
+        u2r = from_right<double>(u2);
+        if (ldR) u2r = upd(V + row + i1p);
+        const double v = beta * plaq4(u1, u2r, u1n, u2).y;
+        double vl = lane_shift(v, true);
+        if (ldL) {
+            const double2 l2 = upd(V + row + i1m);
+            const double2 l1n = upd(sn + i1m);
+            vl = beta * plaq4(l1, u2, l1n, l2).y;
+            l1 = l1n;
+        }
+        Uo[s] = u1;
+        Uo[V + s] = u2;
+        const double g1 = -v + vprev;
+        const double g2 = v - vl;
+        mo[s] = m1 + ep * g1;
+        mo[V + s] = m2 + ep * g2;
+        vprev = v;
+        u1 = u1n;
+        u2 = u2n;
+    }
+}
+
+static int gauge_rows(const Latt &L)
+{
+    int rows = 8;
+    while (rows > 1 && march_grid(L, rows).nblocks() < 1024) rows >>= 1;
+    return rows;
+}
+
+void launch_updt_gauge_mom(const Latt &L, const double2 *Ui, const double *mi, double2 *Uo,
+                           double *mo, double eu, double beta, double ep, hipStream_t st)
+{
+    const int rows = gauge_rows(L);
+    const MarchGrid g = march_grid(L, rows);
+    DISPATCH_ROWS(rows, k_updt_gauge_mom, dim3(g.nblocks()), dim3(kBlock), 0, st, L, Ui, mi, Uo,
+                  mo, eu, beta, ep, g.bx, g.by, g.nbx);
+}
+
 void launch_force_mom(const Latt &L, const double2 *U, const double2 *X, const double2 *Y,
                       const double *facc, double *mom, double beta, double eps, double cf,
                       hipStream_t st)
@@ -667,8 +760,7 @@ void launch_force_mom(const Latt &L, const double2 *U, const double2 *X, const d
     if (!X) {
         // pure gauge (+ an accumulated force): the marching kernel, 8 rows per
         // thread where the lattice leaves 1024 blocks, fewer below
-        int rows = 8;
-        while (rows > 1 && march_grid(L, rows).nblocks() < 1024) rows >>= 1;
+        const int rows = gauge_rows(L);
         const MarchGrid g = march_grid(L, rows);
         DISPATCH_ROWS(rows, k_gauge_mom, dim3(g.nblocks()), dim3(kBlock), 0, st, L, U, facc,
                       mom, beta, eps, g.bx, g.by, g.nbx);
@@ -710,9 +802,7 @@ __global__ void k_updt_fld(double2 *__restrict__ U, const double *__restrict__ m
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
          i += (long)gridDim.x * blockDim.x) {
-        double sn, cs;
-        sincos(eps * mom[i], &sn, &cs);
-        U[i] = cmul<double>(mk<double>(cs, sn), U[i]);
+        U[i] = updt_link(U[i], mom[i], eps);
     }
 }
 void launch_updt_fld(const Latt &L, double2 *U, const double *mom, double eps, hipStream_t st)

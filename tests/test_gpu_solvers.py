@@ -203,6 +203,36 @@ def test_omf4_pure_gauge(Q, O):
     assert relerr(host(dm), mom) < 1e-11
 
 
+@pytest.mark.parametrize("L1,L2", [(33, 20), (64, 7), (128, 96)])
+def test_pure_gauge_integrators_shapes(Q, O, L1, L2):
+    """The pure-gauge integrators (each link update fused with the momentum
+    update after it, out of place between two buffer pairs): OMF4 for 1, 2 and 3
+    steps (an odd number of fused passes ends in the scratch pair and is copied
+    back) and pure-gauge HMC with leapfrog and OMF4, against the oracle, on
+    shapes with wave edges, odd widths and rows the tiles do not divide."""
+    say("pure gauge integrators", L1, L2)
+    prm = Q.LattParm((L1, L2), 3.0)
+    p = O.latt(L1, L2, 3.0)
+    for ns in (1, 2, 3):
+        U = O.hot_start(L1, L2, 70 + ns)
+        mom = O.momenta(L1, L2, 70 + ns, 0, 0)
+        dU, dm = dev(U), dev(mom)
+        Q.OMF4_pure(dm, dU, 0.07, ns, prm)
+        O.omf4_pure(mom, U, 0.07, ns, p)
+        assert relerr(host(dU), U) < 1e-12, ns
+        assert relerr(host(dm), mom) < 1e-11, ns
+    for code, integ in ((0, Q.Leapfrog()), (1, Q.OMF4())):
+        U = O.hot_start(L1, L2, 90 + code)
+        dU = dev(U)
+        Q.seed(4242)
+        for t in range(3):
+            info = Q.HMC(dU, 0.05, 5, [], prm, None, integrator=integ)
+            hi, hf, a = O.hmc_pure(U, 0.05, 5, p, 4242, t, integrator=code)
+            assert abs(info.hini - hi) < 1e-9 and abs(info.hfin - hf) < 1e-9, (code, t)
+            assert info.accepted == a, (code, t)
+        assert relerr(host(dU), U) < 1e-10, code
+
+
 @pytest.mark.parametrize("integ", [0, 1])
 def test_md_nf2(Q, O, integ):
     say("MD nf2 integrator", integ)
