@@ -817,9 +817,10 @@ int multishift_multi(qed2d_ctx *c, const double *U, int nf, const double *const 
     long launched_k = 0;
     for (;;) {
         for (int jj = 0; jj < chunk; ++jj) {
+            // (the shift updates run as extra blocks of the same launch)
             PROF(c, QED2D_PROF_OTHER,
-                 launch_b_ms_k1(g, c->rows, (const double2 *)U, r, p0, p1, t, m.M, m.P, st));
-            PROF(c, QED2D_PROF_OTHER, launch_b_ms_shifts(m.M, r, n2, nf, maxshift, st));
+                 launch_b_ms_k1(g, c->rows, (const double2 *)U, r, p0, p1, t, m.M, m.P, n2,
+                                maxshift, st));
             PROF(c, QED2D_PROF_OTHER,
                  launch_b_ms_k2(g, c->rows, (const double2 *)U, t, p0, p1, r, m.M, m.P, st));
         }

@@ -31,7 +31,7 @@ struct BWhere {
 template <bool PAIR>
 __device__ __forceinline__ BWhere bwhere(const BGeo &g, int R)
 {
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int b = xcd_remap(blockIdx.x, g.nblocks);  // (launches may add blocks past these)
     int lo = 0, local;
     if (g.inter) {
         lo = b % g.n;
@@ -490,12 +490,32 @@ void launch_b_ms_init(const BGeo &g, int rows, double2 *r, double2 *p0,
     launch_k(kb_ms_init, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, r, p0, M, P);
 }
 
+// Blocks past g.nblocks are shift-update blocks: chain c, shift s, elements
+// eb * kShiftEPT * kBlock .. of the chain's spinor (maxshift * bpc of them per
+// chain).
 template <int R>
 __global__ void __launch_bounds__(kBlock) kb_ms1(BGeo g, const double2 *__restrict__ U,
                                                  const double2 *__restrict__ r, double2 *P0,
                                                  double2 *P1, double2 *__restrict__ t,
-                                                 MultiShiftState *M, BParts P)
+                                                 MultiShiftState *M, BParts P, int maxshift,
+                                                 int bpc)
 {
+    if ((int)blockIdx.x >= g.nblocks) {
+        const int idx = blockIdx.x - g.nblocks;
+        const int c = idx / (maxshift * bpc), rem = idx - c * maxshift * bpc;
+        const int sft = rem / bpc, eb = rem - sft * bpc;
+        MultiShiftState *Mc = M + c;
+        if (sft >= Mc->nshift) return;
+        const int k = Mc->kk;
+        const int sk = Mc->stop_k;
+        if (sk != 0 && sk < k) return;
+        const int rd = (k - 1) & 1;
+        const BChain &ch = g.ch[c];
+        ms1_shift_block(Mc, r + 2 * ch.off, P.tt[rd] + 2 * ch.pb, P.rr[rd] + ch.pb,
+                        P.nrr[rd][c], P.ntt[rd][c], k, Mc->nshift, sft,
+                        (long)eb * kShiftEPT * kBlock, 2 * ch.L.V);
+        return;
+    }
     const BWhere w = bwhere<false>(g, R);
     MultiShiftState *Mc = M + w.c;
     const int k = Mc->kk;
@@ -536,35 +556,14 @@ __global__ void __launch_bounds__(kBlock) kb_ms2(BGeo g, const double2 *__restri
     }
 }
 
-// The shift updates of every chain's current step, after kb_ms1 (chains of n2
-// spinor elements each, bpc blocks per chain; y over the shifts).  kk is still
-// the step kb_ms1 ran: a chain that stopped before it updates nothing.
-__global__ void __launch_bounds__(kBlock) kb_ms_shifts(const MultiShiftState *M,
-                                                       const double2 *__restrict__ r, long n2,
-                                                       int bpc)
-{
-    const int c = blockIdx.x / bpc;
-    const MultiShiftState *Mc = M + c;
-    const int k = blockIdx.y;
-    if (k >= Mc->nshift) return;
-    const int sk = Mc->stop_k;
-    if (sk != 0 && sk < Mc->kk) return;
-    const long i = (long)(blockIdx.x % bpc) * kBlock + threadIdx.x;
-    if (i >= n2) return;
-    ms_shift_update(Mc, r + (long)c * n2, i, k);
-}
-void launch_b_ms_shifts(const MultiShiftState *M, const double2 *r, long n2, int nchain,
-                        int maxshift, hipStream_t st)
-{
-    const int bpc = (int)((n2 + kBlock - 1) / kBlock);
-    launch_k(kb_ms_shifts, dim3((unsigned)(bpc * nchain), maxshift), dim3(kBlock), 0, st, M, r,
-             n2, bpc);
-}
-
 void launch_b_ms_k1(const BGeo &g, int rows, const double2 *U, const double2 *r, double2 *p0,
-                    double2 *p1, double2 *t, MultiShiftState *M, BParts P, hipStream_t st)
+                    double2 *p1, double2 *t, MultiShiftState *M, BParts P, long n2max,
+                    int maxshift, hipStream_t st)
 {
-    DISPATCH_ROWS(rows, kb_ms1, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, r, p0, p1, t, M, P);
+    const int bpc = (int)((n2max + (long)kShiftEPT * kBlock - 1) / ((long)kShiftEPT * kBlock));
+    const long nb = g.nblocks + (long)g.n * maxshift * bpc;
+    DISPATCH_ROWS(rows, kb_ms1, dim3((unsigned)nb), dim3(kBlock), 0, st, g, U, r, p0, p1, t, M,
+                  P, maxshift, bpc);
 }
 void launch_b_ms_k2(const BGeo &g, int rows, const double2 *U, const double2 *t,
                     const double2 *p0, const double2 *p1, double2 *r, MultiShiftState *M,

@@ -322,29 +322,34 @@ __device__ __forceinline__ void ms_shift_update(const MultiShiftState *S,
     }
 }
 
-template <int R>
-__device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restrict__ U,
-                                         const double2 *__restrict__ r, double2 *P0, double2 *P1,
-                                         double2 *__restrict__ t, double mass2,
-                                         MultiShiftState *S, double *Ptt_w, const double *Ptt_rd,
-                                         const double *Prr_rd, int nrr, int ntt, int k,
-                                         int nshift, const MarchPos &p)
+// Block-shared scalars of multi-shift K1(k).
+struct MS1Shared {
+    double lds[kBlock / kWave + 1];
+    double zj[kMaxShift], z1[kMaxShift], as[kMaxShift], bs[kMaxShift];
+    int upd[kMaxShift];
+    int anyleft;
+};
+
+// The scalars of K1(k) (Krylov index j = k - 1), the same in every block:
+// the base system's alpha_{j-1} and beta_j from the partials of the previous
+// kernels, and per shift zeta_j, alpha^i, beta^i and its update (0: stopped
+// earlier, 1: converges at this step, only x updates, 2: x and p update).
+// Returns the status: 0 running, 1 every shift converged, 2 iteration limit.
+__device__ __forceinline__ int ms1_scalars(const MultiShiftState *S, const double *Ptt_rd,
+                                           const double *Prr_rd, int nrr, int ntt, int k,
+                                           int nshift, MS1Shared &sh, double &rr, double &tol,
+                                           double &alpha_jm1, double &beta_j)
 {
-    __shared__ double lds[kBlock / kWave + 1];
-    __shared__ double zj_s[kMaxShift], z1_s[kMaxShift], as_s[kMaxShift], bs_s[kMaxShift];
-    __shared__ int upd_s[kMaxShift];
-    __shared__ int anyleft_s;
-    const int w = k & 1, rd = (k - 1) & 1;
-    const double rr = sum_prev_partials(Prr_rd, nrr, 1, 0, lds);
+    const int rd = (k - 1) & 1;
+    rr = sum_prev_partials(Prr_rd, nrr, 1, 0, sh.lds);
     const bool first = (k == 1);
-    double tol, alpha_jm1, beta_j;
     if (first) {
         tol = S->eps * rr;
         alpha_jm1 = 1.0;
         beta_j = 0.0;
     } else {
-        const double tt = sum_prev_partials(Ptt_rd, ntt, 2, 0, lds);
-        const double pp = sum_prev_partials(Ptt_rd, ntt, 2, 1, lds);
+        const double tt = sum_prev_partials(Ptt_rd, ntt, 2, 0, sh.lds);
+        const double pp = sum_prev_partials(Ptt_rd, ntt, 2, 1, sh.lds);
         tol = S->tol;
         alpha_jm1 = S->norm[rd] / (tt + S->sigma_b * pp);
         beta_j = rr / S->norm[rd];
@@ -368,35 +373,50 @@ __device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restric
                 conv = zj * zj * rr < tol;
             }
         }
-        zj_s[i] = zj;
-        z1_s[i] = z1;
-        as_s[i] = as;
-        bs_s[i] = bs;
-        upd_s[i] = act ? (conv ? 1 : 2) : 0;
+        sh.zj[i] = zj;
+        sh.z1[i] = z1;
+        sh.as[i] = as;
+        sh.bs[i] = bs;
+        sh.upd[i] = act ? (conv ? 1 : 2) : 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int any = 0;
-        for (int i = 0; i < nshift; ++i) any |= (upd_s[i] == 2);
-        anyleft_s = any;
+        for (int i = 0; i < nshift; ++i) any |= (sh.upd[i] == 2);
+        sh.anyleft = any;
     }
     __syncthreads();
-    const int anyleft = anyleft_s;
-    const int stat = anyleft ? ((k - 1 >= S->maxiter) ? 2 : 0) : 1;
+    return sh.anyleft ? ((k - 1 >= S->maxiter) ? 2 : 0) : 1;
+}
+
+template <int R>
+__device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restrict__ U,
+                                         const double2 *__restrict__ r, double2 *P0, double2 *P1,
+                                         double2 *__restrict__ t, double mass2,
+                                         MultiShiftState *S, double *Ptt_w, const double *Ptt_rd,
+                                         const double *Prr_rd, int nrr, int ntt, int k,
+                                         int nshift, const MarchPos &p)
+{
+    __shared__ MS1Shared sh;
+    const int w = k & 1, rd = (k - 1) & 1;
+    const bool first = (k == 1);
+    double rr, tol, alpha_jm1, beta_j;
+    const int stat = ms1_scalars(S, Ptt_rd, Prr_rd, nrr, ntt, k, nshift, sh, rr, tol, alpha_jm1,
+                                 beta_j);
     if (p.blk == 0 && threadIdx.x == 0) {
         S->norm[w] = rr;
         S->alpha[w] = alpha_jm1;
         S->beta[w] = beta_j;
         for (int i = 0; i < nshift; ++i) {
-            S->zeta[w][i] = zj_s[i];
-            S->zeta_prev[w][i] = z1_s[i];
-            S->active[w][i] = (upd_s[i] == 2);
-            if (upd_s[i] == 1) S->conv_iter[i] = k - 1;
+            S->zeta[w][i] = sh.zj[i];
+            S->zeta_prev[w][i] = sh.z1[i];
+            S->active[w][i] = (sh.upd[i] == 2);
+            if (sh.upd[i] == 1) S->conv_iter[i] = k - 1;
             // for the shift-update kernel that follows (none at the iteration limit)
-            S->sh_u[i] = (stat == 2) ? 0 : upd_s[i];
-            S->sh_a[i] = as_s[i];
-            S->sh_b[i] = bs_s[i];
-            S->sh_z[i] = zj_s[i];
+            S->sh_u[i] = (stat == 2) ? 0 : sh.upd[i];
+            S->sh_a[i] = sh.as[i];
+            S->sh_b[i] = sh.bs[i];
+            S->sh_z[i] = sh.zj[i];
         }
         S->iter = k - 1;
         S->err = rr;
@@ -409,7 +429,7 @@ __device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restric
             S->stop_k = k;
         }
     }
-    if (stat == 2 || !anyleft) return;  // (the last x updates are the shift kernel's)
+    if (stat == 2 || !sh.anyleft) return;  // (the last x updates are the shift updates')
     const double2 *pold = rd ? P1 : P0;
     double2 *pnew = w ? P1 : P0;
     EpiMS1 epi{pnew, t, L.V, 0.0, 0.0};
@@ -422,6 +442,45 @@ __device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restric
     if (threadIdx.x == 0) {
         Ptt_w[2 * p.blk] = a;
         Ptt_w[2 * p.blk + 1] = b;
+    }
+}
+
+// Spinor elements per thread of a shift-update block.
+constexpr int kShiftEPT = 8;
+
+// The shift updates of K1(k) run by extra blocks of the same launch (the
+// batched K1): each derives the step's coefficients with ms1_scalars, as the
+// stencil blocks do, and updates shift `shift` at spinor elements
+// e0 .. e0 + kShiftEPT * blockDim - 1 (of n2): x_i += alpha^i p_i, and
+// p_i = zeta r + beta^i p_i while the shift iterates (the expressions of
+// ms_shift_update).  K1 writes none of r, x_i, p_i, so these blocks may run
+// beside its stencil blocks.
+__device__ __forceinline__ void ms1_shift_block(const MultiShiftState *S,
+                                                const double2 *__restrict__ r,
+                                                const double *Ptt_rd, const double *Prr_rd,
+                                                int nrr, int ntt, int k, int nshift, int shift,
+                                                long e0, long n2)
+{
+    __shared__ MS1Shared sh;
+    double rr, tol, alpha_jm1, beta_j;
+    const int stat = ms1_scalars(S, Ptt_rd, Prr_rd, nrr, ntt, k, nshift, sh, rr, tol, alpha_jm1,
+                                 beta_j);
+    const int u = sh.upd[shift];
+    if (stat == 2 || u == 0) return;
+    double2 *__restrict__ ps = S->ps[shift];
+    double2 *__restrict__ x = S->x[shift];
+    const double a = sh.as[shift], z = sh.zj[shift], b = sh.bs[shift];
+#pragma unroll 4
+    for (int j = 0; j < kShiftEPT; ++j) {
+        const long i = e0 + (long)j * kBlock + threadIdx.x;
+        if (i >= n2) break;
+        const double2 pv = ps[i];
+        const double2 xv = x[i];
+        x[i] = mk<double>(xv.x + a * pv.x, xv.y + a * pv.y);
+        if (u == 2) {
+            const double2 rv = r[i];
+            ps[i] = mk<double>(z * rv.x + b * pv.x, z * rv.y + b * pv.y);
+        }
     }
 }
 

@@ -209,10 +209,9 @@ void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, floa
 // Multi-shift CG of one right-hand side per chain (M[c]: chain c's state).
 void launch_b_ms_init(const BGeo &g, int rows, double2 *r, double2 *p0,
                       const MultiShiftState *M, BParts P, hipStream_t st);
-void launch_b_ms_shifts(const MultiShiftState *M, const double2 *r, long n2, int nchain,
-                        int maxshift, hipStream_t st);
 void launch_b_ms_k1(const BGeo &g, int rows, const double2 *U, const double2 *r, double2 *p0,
-                    double2 *p1, double2 *t, MultiShiftState *M, BParts P, hipStream_t st);
+                    double2 *p1, double2 *t, MultiShiftState *M, BParts P, long n2max,
+                    int maxshift, hipStream_t st);
 void launch_b_ms_k2(const BGeo &g, int rows, const double2 *U, const double2 *t,
                     const double2 *p0, const double2 *p1, double2 *r, MultiShiftState *M,
                     BParts P, hipStream_t st);
