@@ -660,38 +660,78 @@ __global__ void __launch_bounds__(kBlock) k_gauge_mom(Latt L, const double2 *__r
     }
 }
 
-// exp(i eps m) u, the link update of k_updt_fld.
+// sin x and cos x.  For |x| <= pi/4, the range every step eps p of an
+// integrator falls in, the Taylor series to x^19 and x^20 (Horner, FMA): the
+// first omitted terms are below 1e-21 relative there, so the result is the
+// rounding of the series sums (a few ulp at most); sincos() otherwise.
+__device__ __forceinline__ void link_sincos(double x, double *s, double *c)
+{
+    if (!(fabs(x) <= 0.78539816339744830962)) {
+        sincos(x, s, c);
+        return;
+    }
+    const double z = x * x;
+    // sin x = x + x z (-1/3! + z/5! - ... - z^8/19!)
+    double ps = -1.0 / 121645100408832000.0;
+    ps = fma(ps, z, 1.0 / 355687428096000.0);
+    ps = fma(ps, z, -1.0 / 1307674368000.0);
+    ps = fma(ps, z, 1.0 / 6227020800.0);
+    ps = fma(ps, z, -1.0 / 39916800.0);
+    ps = fma(ps, z, 1.0 / 362880.0);
+    ps = fma(ps, z, -1.0 / 5040.0);
+    ps = fma(ps, z, 1.0 / 120.0);
+    ps = fma(ps, z, -1.0 / 6.0);
+    *s = fma(x * z, ps, x);
+    // cos x = 1 + z (-1/2! + z/4! - ... + z^9/20!)
+    double pc = 1.0 / 2432902008176640000.0;
+    pc = fma(pc, z, -1.0 / 6402373705728000.0);
+    pc = fma(pc, z, 1.0 / 20922789888000.0);
+    pc = fma(pc, z, -1.0 / 87178291200.0);
+    pc = fma(pc, z, 1.0 / 479001600.0);
+    pc = fma(pc, z, -1.0 / 3628800.0);
+    pc = fma(pc, z, 1.0 / 40320.0);
+    pc = fma(pc, z, -1.0 / 720.0);
+    pc = fma(pc, z, 1.0 / 24.0);
+    pc = fma(pc, z, -0.5);
+    *c = fma(z, pc, 1.0);
+}
+
+// exp(i eps m) u, the link update (src/QED2dAction.jl:55-64).
 __device__ __forceinline__ double2 updt_link(double2 u, double m, double eps)
 {
     double sn, cs;
-    sincos(eps * m, &sn, &cs);
+    link_sincos(eps * m, &sn, &cs);
     return cmul<double>(mk<double>(cs, sn), u);
 }
 
 // The link update U(eu) and then the pure-gauge momentum update P(ep) of an
 // integrator stage pair, in one pass: U' = exp(i eu p) U, p' = p + ep F(U').
 // Out of place (Ui, mi -> Uo, mo), since the tiles around a tile read the old
-// links and momenta of its edge rows.  The march of k_gauge_mom, where every
-// link a thread uses is formed from the old link and momentum as it is read;
-// each thread stores the updated links and momenta of its own sites.
+// links and momenta of its edge rows.  A march as k_gauge_mom's, except that
+// the waves overlap: a wave covers 64 consecutive columns (modulo L1) and
+// stores the 62 inside ones, so the updated U2(x+1) and the plaquette of
+// x - 1 always come from an adjacent lane and no lane forms extra updated
+// links (each an FP64 sincos) while the rest of its wave waits.  Each lane
+// updates its own column's links of rows r0 - 1 .. r0 + R.
+constexpr int kGaugeOut = kWave - 2;  // columns a wave stores
+
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_updt_gauge_mom(Latt L, const double2 *__restrict__ Ui,
                                                            const double *__restrict__ mi,
                                                            double2 *__restrict__ Uo,
                                                            double *__restrict__ mo, double eu,
-                                                           double beta, double ep, int bx, int by,
-                                                           int nbx)
+                                                           double beta, double ep, int nwx,
+                                                           int nwaves)
 {
-    const MarchPos p = march_pos(L, bx, by, nbx, R);
-    if (!p.active) return;
+    const int g = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;  // wave
+    if (g >= nwaves) return;
+    const int lane = threadIdx.x & (kWave - 1);
     const int L1 = L.L1, L2 = L.L2;
     const long V = L.V;
-    const int i1 = p.i1, r0 = p.r0;
-    const int i1p = (i1 + 1 == L1) ? 0 : i1 + 1;
-    const int i1m = (i1 == 0) ? L1 - 1 : i1 - 1;
-    const int lane = threadIdx.x & (kWave - 1);
-    const bool ldR = (lane == kWave - 1) || (i1p == 0);
-    const bool ldL = (lane == 0) || (i1 == 0);
+    const int wx = g % nwx, r0 = (g / nwx) * R;
+    const int cu = wx * kGaugeOut + lane - 1;  // column before wrapping
+    const int i1 = ((cu % L1) + L1) % L1;
+    const bool out = (lane >= 1) && (lane <= kGaugeOut) && (cu < L1);
     const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
     const int nr = (r0 + R <= L2) ? R : L2 - r0;
     const long sm = (long)rm * L1, s0 = (long)r0 * L1;
@@ -699,10 +739,7 @@ __global__ void __launch_bounds__(kBlock) k_updt_gauge_mom(Latt L, const double2
     double2 u1 = upd(sm + i1), u2 = upd(V + sm + i1);
     double2 u1n = upd(s0 + i1);
     double2 u2r = from_right<double>(u2);
-    if (ldR) u2r = upd(V + sm + i1p);
     double vprev = beta * plaq4(u1, u2r, u1n, u2).y;
-    double2 l1{};
-    if (ldL) l1 = upd(s0 + i1m);
     u1 = u1n;
     u2 = upd(V + s0 + i1);
 #pragma unroll
@@ -710,27 +747,22 @@ __global__ void __launch_bounds__(kBlock) k_updt_gauge_mom(Latt L, const double2
         if (k >= nr) break;
         const int r = r0 + k;
         const int rn = (r + 1 == L2) ? 0 : r + 1;
-        const long row = (long)r * L1, s = row + i1, sn = (long)rn * L1;
+        const long s = (long)r * L1 + i1, sn = (long)rn * L1;
         u1n = upd(sn + i1);
         double2 u2n{};
         if (k + 1 < nr) u2n = upd(V + sn + i1);
         const double m1 = mi[s], m2 = mi[V + s];
         u2r = from_right<double>(u2);
-        if (ldR) u2r = upd(V + row + i1p);
         const double v = beta * plaq4(u1, u2r, u1n, u2).y;
-        double vl = lane_shift(v, true);
-        if (ldL) {
-            const double2 l2 = upd(V + row + i1m);
-            const double2 l1n = upd(sn + i1m);
-            vl = beta * plaq4(l1, u2, l1n, l2).y;
-            l1 = l1n;
+        const double vl = lane_shift(v, true);
+        if (out) {
+            Uo[s] = u1;
+            Uo[V + s] = u2;
+            const double g1 = -v + vprev;
+            const double g2 = v - vl;
+            mo[s] = m1 + ep * g1;
+            mo[V + s] = m2 + ep * g2;
         }
-        Uo[s] = u1;
-        Uo[V + s] = u2;
-        const double g1 = -v + vprev;
-        const double g2 = v - vl;
-        mo[s] = m1 + ep * g1;
-        mo[V + s] = m2 + ep * g2;
         vprev = v;
         u1 = u1n;
         u2 = u2n;
@@ -747,10 +779,18 @@ static int gauge_rows(const Latt &L)
 void launch_updt_gauge_mom(const Latt &L, const double2 *Ui, const double *mi, double2 *Uo,
                            double *mo, double eu, double beta, double ep, hipStream_t st)
 {
-    const int rows = gauge_rows(L);
-    const MarchGrid g = march_grid(L, rows);
-    DISPATCH_ROWS(rows, k_updt_gauge_mom, dim3(g.nblocks()), dim3(kBlock), 0, st, L, Ui, mi, Uo,
-                  mo, eu, beta, ep, g.bx, g.by, g.nbx);
+    // (16 rows where the lattice leaves 1024 blocks: each tile recomputes the
+    // updated links of its halo rows)
+    int rows = 16;
+    while (rows > 1 && (long)((L.L1 + kGaugeOut - 1) / kGaugeOut) * ((L.L2 + rows - 1) / rows) <
+                           4L * 1024)
+        rows >>= 1;
+    const int nwx = (L.L1 + kGaugeOut - 1) / kGaugeOut;
+    const int nwaves = nwx * ((L.L2 + rows - 1) / rows);
+    const int wpb = kBlock / kWave;
+    const int nb = (nwaves + wpb - 1) / wpb;
+    DISPATCH_ROWS(rows, k_updt_gauge_mom, dim3(nb), dim3(kBlock), 0, st, L, Ui, mi, Uo, mo, eu,
+                  beta, ep, nwx, nwaves);
 }
 
 void launch_force_mom(const Latt &L, const double2 *U, const double2 *X, const double2 *Y,

@@ -172,6 +172,10 @@ def test_updt_fld_and_observables(Q, O):
     Q.updt_fld(dU, dm, 0.1, prm)
     U2 = O.updt_fld(U, mom, 0.1)
     assert relerr(host(dU), U2) < 1e-15
+    # steps beyond pi/4 (the general sincos branch) mixed with small ones
+    dU3 = dev(U)
+    Q.updt_fld(dU3, dm, 2.5, prm)
+    assert relerr(host(dU3), O.updt_fld(U, mom, 2.5)) < 1e-15
     plx = dev(np.zeros((L2, L1)))
     Q.plaquette(plx, dU, prm)
     assert relerr(host(plx), O.plaquette_field(U2)) < 1e-14
