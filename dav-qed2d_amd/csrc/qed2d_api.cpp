@@ -67,6 +67,12 @@ struct qed2d_ctx {
         BParts P{};
         double2 *arena = nullptr;  // r, p0, p1, t of every right-hand side
         std::vector<std::vector<double2 *>> x, ps;  // per right-hand side and shift
+        // the force of every pole solution: Y = Q x (yarena, ycap fields) and a
+        // table of x, Y, 2 + m0 and rho per field (device and pinned copies)
+        double2 *yarena = nullptr;
+        int ycap = 0;
+        void *tab = nullptr, *htab = nullptr;
+        int tabcap = 0;
     } mr;
     qed2d_batch *rhs = nullptr;  // qed2d_cg_mrhs: right-hand sides sharing the links
     int multishift = 1;               // MultiCG / RHMC force through multi-shift CG
@@ -1013,6 +1019,32 @@ void key_of(uint64_t seed, uint32_t chain, uint32_t *k0, uint32_t *k1)
 }
 
 // RHMC force (src/QED2dHMC.jl:516-557)
+// Room for the force of n pole solutions: n fields Y and the table.
+int force_tab(qed2d_ctx *c, int n)
+{
+    auto &m = c->mr;
+    const long n2 = spinor_elems(c);
+    if (n > m.ycap) {
+        if (m.yarena) (void)hipFree(m.yarena);
+        m.yarena = nullptr;
+        m.ycap = 0;
+        if (hipMalloc((void **)&m.yarena, sizeof(double2) * n2 * n) != hipSuccess)
+            return fail(c, QED2D_ENOMEM, "hipMalloc of %d force fields failed", n);
+        m.ycap = n;
+    }
+    if (n > m.tabcap) {
+        if (m.tab) (void)hipFree(m.tab);
+        if (m.htab) (void)hipHostFree(m.htab);
+        m.tab = m.htab = nullptr;
+        m.tabcap = 0;
+        if (hipMalloc(&m.tab, 4 * sizeof(double) * n) != hipSuccess ||
+            hipHostMalloc(&m.htab, 4 * sizeof(double) * n) != hipSuccess)
+            return fail(c, QED2D_ENOMEM, "force table allocation failed");
+        m.tabcap = n;
+    }
+    return QED2D_OK;
+}
+
 int update_momenta_rhmc(qed2d_ctx *c, double *mom, const double *U, double *X,
                         const std::vector<double2 *> &F, double *g5DX, const double *am0, int nf,
                         const int *npole, const double *const *mu, const double *const *rho,
@@ -1029,16 +1061,34 @@ int update_momenta_rhmc(qed2d_ctx *c, double *mom, const double *U, double *X,
         for (int j = 0; j < nf; ++j) bj[j] = (const double *)F[j];
         int it = 0;
         TRY(multishift_multi(c, U, nf, bj.data(), am0, npole, mu, maxiter, tol, &it));
+        // every pole solution's Q x in one launch, and the sum of their forces
+        // (in the order of the loop over flavours and poles) in another
+        int n = 0;
+        for (int j = 0; j < nf; ++j) n += npole[j];
+        TRY(force_tab(c, n));
+        auto &m = c->mr;
+        const long n2 = spinor_elems(c);
+        const double2 **hx = (const double2 **)m.htab;
+        double2 **hy = (double2 **)(hx + n);
+        double *hm = (double *)(hy + n), *hr = hm + n;
+        int k = 0;
         for (int j = 0; j < nf; ++j)
-            for (int i = 0; i < npole[j]; ++i) {
-                const double *Xi = (const double *)c->mr.x[j][i];
-                q64(c, U, Xi, g5DX, am0[j]);
-                launch_force_acc(c->L, (const double2 *)U, (const double2 *)Xi,
-                                 (const double2 *)g5DX, facc, rho[j][i], first ? 1 : 0,
-                                 c->stream);
-                first = false;
+            for (int i = 0; i < npole[j]; ++i, ++k) {
+                hx[k] = m.x[j][i];
+                hy[k] = m.yarena + (long)k * n2;
+                hm[k] = 2.0 + am0[j];
+                hr[k] = rho[j][i];
             }
-        launch_force_mom(c->L, (const double2 *)U, nullptr, nullptr, first ? nullptr : facc, mom,
+        HIPCHK(c, hipMemcpyAsync(m.tab, m.htab, 4 * sizeof(double) * n, hipMemcpyHostToDevice,
+                                 c->stream));
+        const double2 *const *dx = (const double2 *const *)m.tab;
+        double2 *const *dy = (double2 *const *)(dx + n);
+        const double *dm = (const double *)(dy + n), *dr = dm + n;
+        PROF(c, QED2D_PROF_Q_F64,
+             launch_q_apply_tab(c->L, (const double2 *)U, dx, dy, dm, n, c->rows_q, c->stream));
+        launch_force_acc_tab(c->L, (const double2 *)U, dx, (const double2 *const *)dy, dr, n,
+                             facc, c->stream);
+        launch_force_mom(c->L, (const double2 *)U, nullptr, nullptr, n ? facc : nullptr, mom,
                          c->beta, eps, 0.0, c->stream);
         return launched(c);
     }
@@ -1421,9 +1471,11 @@ void qed2d_ctx_destroy(qed2d_ctx *c)
     for (auto &v : c->mr.x) for (auto p : v) if (p) (void)hipFree(p);
     for (auto &v : c->mr.ps) for (auto p : v) if (p) (void)hipFree(p);
     for (void *p : {(void *)c->mr.dch, (void *)c->mr.blk0, (void *)c->mr.shape, (void *)c->mr.M,
-                    (void *)c->mr.part, (void *)c->mr.cnt, (void *)c->mr.arena})
+                    (void *)c->mr.part, (void *)c->mr.cnt, (void *)c->mr.arena,
+                    (void *)c->mr.yarena, c->mr.tab})
         if (p) (void)hipFree(p);
     if (c->mr.hM) (void)hipHostFree(c->mr.hM);
+    if (c->mr.htab) (void)hipHostFree(c->mr.htab);
     if (c->ms) (void)hipFree(c->ms);
     if (c->hms) (void)hipHostFree(c->hms);
     if (c->part) (void)hipFree(c->part);

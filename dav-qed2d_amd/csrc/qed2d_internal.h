@@ -308,6 +308,11 @@ void launch_tr_dQwdU(const Latt &L, const double2 *U, const double2 *X, const do
 void launch_krnl_force(const Latt &L, const double2 *U, double beta, double *frc1, double *frc2,
                        hipStream_t st);
 void launch_updt_fld(const Latt &L, double2 *U, const double *mom, double eps, hipStream_t st);
+void launch_q_apply_tab(const Latt &L, const double2 *U, const double2 *const *X,
+                        double2 *const *Y, const double *mass2, int n, int rows, hipStream_t st);
+void launch_force_acc_tab(const Latt &L, const double2 *U, const double2 *const *X,
+                          const double2 *const *Y, const double *rho, int n, double *facc,
+                          hipStream_t st);
 void launch_updt_gauge_mom(const Latt &L, const double2 *Ui, const double *mi, double2 *Uo,
                            double *mo, double eu, double beta, double ep, hipStream_t st);
 // mom += eps * (Fgauge + cf * Ffermion(X, Y) + facc)   (facc may be null; X/Y null: gauge only)

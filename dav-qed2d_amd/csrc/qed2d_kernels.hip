@@ -810,6 +810,64 @@ void launch_force_mom(const Latt &L, const double2 *U, const double2 *X, const d
                        mom, beta, eps, cf);
 }
 
+// Q X_n for a table of fields (the pole solutions of every pseudofermion of an
+// RHMC force): grid y = n, the plain apply's march and stores, mass2[n] the
+// field's 2 + m0.
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_q_apply_tab(Latt L, const double2 *__restrict__ U,
+                                                        const double2 *const *X,
+                                                        double2 *const *Y, const double *mass2,
+                                                        int bx, int by, int nbx)
+{
+    const int n = blockIdx.y;
+    MarchPos p = march_pos(L, bx, by, nbx, R);
+    if (!p.active) return;
+    SrcPlain<double> src{X[n], L.V};
+    EpiStore<double> epi{Y[n], L.V};
+    q_march<double, R>(L, U, mass2[n], src, epi, p.i1, p.r0);
+}
+void launch_q_apply_tab(const Latt &L, const double2 *U, const double2 *const *X,
+                        double2 *const *Y, const double *mass2, int n, int rows, hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    DISPATCH_ROWS(rows, k_q_apply_tab, dim3(g.nblocks(), n), dim3(kBlock), 0, st, L, U, X, Y,
+                  mass2, g.bx, g.by, g.nbx);
+}
+
+// facc = sum_n rho[n] F(X_n, Y_n) over a table of fields, per site in the order
+// n = 0, 1, ... with the expressions of k_force_acc (the same bits as its
+// launches one field after another).
+__global__ void k_force_acc_tab(Latt L, const double2 *__restrict__ U, const double2 *const *X,
+                                const double2 *const *Y, const double *rho, int n,
+                                double *__restrict__ facc)
+{
+    for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < L.V;
+         s += (long)gridDim.x * blockDim.x) {
+        const int i2 = (int)(s / L.L1), i1 = (int)(s - (long)i2 * L.L1);
+        double a1 = 0.0, a2 = 0.0;
+        for (int k = 0; k < n; ++k) {
+            double f1, f2;
+            ferm_force(U, X[k], Y[k], L, i1, i2, f1, f2);
+            if (k == 0) {
+                a1 = rho[k] * f1;
+                a2 = rho[k] * f2;
+            } else {
+                a1 = a1 + rho[k] * f1;
+                a2 = a2 + rho[k] * f2;
+            }
+        }
+        facc[s] = a1;
+        facc[L.V + s] = a2;
+    }
+}
+void launch_force_acc_tab(const Latt &L, const double2 *U, const double2 *const *X,
+                          const double2 *const *Y, const double *rho, int n, double *facc,
+                          hipStream_t st)
+{
+    launch_k(k_force_acc_tab, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, rho, n,
+             facc);
+}
+
 // facc (+)= rho * F_fermion  (src/QED2dHMC.jl:541-543)
 __global__ void k_force_acc(Latt L, const double2 *__restrict__ U, const double2 *__restrict__ X,
                             const double2 *__restrict__ Y, double *__restrict__ facc, double rho,
