@@ -74,7 +74,9 @@ struct qed2d_ctx {
         void *tab = nullptr, *htab = nullptr;
         int tabcap = 0;
     } mr;
-    qed2d_batch *rhs = nullptr;  // qed2d_cg_mrhs: right-hand sides sharing the links
+    // right-hand-side sets (copies of this lattice sharing the links), indexed by
+    // their count: qed2d_cg_mrhs and the RHMC heatbath's shifted solves
+    std::vector<qed2d_batch *> rhsets;
     int multishift = 1;               // MultiCG / RHMC force through multi-shift CG
     MultiShiftState *ms = nullptr;    // device
     MultiShiftState *hms = nullptr;   // pinned
@@ -1227,6 +1229,72 @@ int gen_pf_impl(qed2d_ctx *c, double *F, const double *U, const double *X, doubl
     return launched(c);
 }
 
+// The set of n right-hand sides on this lattice (created on first use, kept
+// until the context is destroyed).
+int rhs_set(qed2d_ctx *c, int n, qed2d_batch **out)
+{
+    if ((int)c->rhsets.size() <= n) c->rhsets.resize(n + 1, nullptr);
+    if (!c->rhsets[n]) {
+        const int rc = qed2d_batch_create_rhs(&c->rhsets[n], c->L.L1, c->L.L2, n, c->beta,
+                                              c->L.bc, (void *)c->stream);
+        if (rc) {
+            c->rhsets[n] = nullptr;
+            return fail(c, rc, "cannot create a set of %d right-hand sides", n);
+        }
+    }
+    *out = c->rhsets[n];
+    return QED2D_OK;
+}
+
+// gen_pf_impl for nf flavours at once, F[j] holding X_j on entry: the same
+// steps per flavour and pole, with pole i's shifted solves of every flavour
+// that has an i-th pole run as one FP64 solve of a right-hand-side set (a mass
+// and a shift per right-hand side), which gives each flavour the bits of its
+// own solve.
+int gen_pf_multi(qed2d_ctx *c, const std::vector<double2 *> &F, const double *U,
+                 const double *am0, int nf, const int *npole, const double *const *mu,
+                 const double *const *nu, int maxiter, double tol)
+{
+    const long n2 = spinor_elems(c);
+    hipStream_t st = c->stream;
+    TRY(force_tab(c, 2 * nf));
+    double2 *A = c->mr.yarena, *Y = A + (long)nf * n2;  // right-hand sides, solutions
+    int pmax = 0;
+    for (int j = 0; j < nf; ++j) pmax = std::max(pmax, npole[j]);
+    std::vector<double> m(nf), s(nf);
+    for (int i = 0; i < pmax; ++i) {
+        int k = 0;
+        for (int j = 0; j < nf; ++j) {
+            if (npole[j] <= i) continue;
+            double2 *a = A + (long)k * n2;
+            launch_copy(a, F[j], n2, st);
+            q64(c, U, (const double *)a, (double *)F[j], am0[j]);
+            launch_axpy_c(F[j], a, 0.0, mu[j][i], n2, st);
+            launch_copy(a, F[j], n2, st);
+            m[k] = am0[j];
+            s[k] = nu[j][i] * nu[j][i];
+            ++k;
+        }
+        qed2d_batch *set = nullptr;
+        TRY(rhs_set(c, k, &set));
+        std::string err;
+        const int rc = batch_rhs_cg_shifted(set, U, (double *)Y, (const double *)A, m.data(),
+                                            s.data(), maxiter, tol, QED2D_PREC_F64, nullptr,
+                                            c->rows_q, c->rows, c->rows32, c->rows_k2,
+                                            c->rows32_k2, c->ru_delta, &err);
+        if (rc) return fail(c, rc, "%s", err.c_str());
+        k = 0;
+        for (int j = 0; j < nf; ++j) {
+            if (npole[j] <= i) continue;
+            const double2 *y = Y + (long)k * n2;
+            q64(c, U, (const double *)y, (double *)F[j], am0[j]);
+            launch_axpy_c(F[j], y, 0.0, -nu[j][i], n2, st);
+            ++k;
+        }
+    }
+    return launched(c);
+}
+
 // so = (A / r_b) (si + sum_j rho_j (Q^2 + mu_j^2)^{-1} si)  (src/QED2dRHMC.jl:261-271)
 int r_impl(qed2d_ctx *c, double *so, const double *U, const double *si, double am0, int n,
            const double *mu, const double *rho, double A, double r_b, int maxiter, double eps)
@@ -1467,7 +1535,7 @@ void qed2d_ctx_destroy(qed2d_ctx *c)
     for (auto p : c->flav) if (p) (void)hipFree(p);
     for (auto p : c->shiftbuf) if (p) (void)hipFree(p);
     for (auto p : c->psbuf) if (p) (void)hipFree(p);
-    if (c->rhs) qed2d_batch_destroy(c->rhs);
+    for (auto p : c->rhsets) if (p) qed2d_batch_destroy(p);
     for (auto &v : c->mr.x) for (auto p : v) if (p) (void)hipFree(p);
     for (auto &v : c->mr.ps) for (auto p : v) if (p) (void)hipFree(p);
     for (void *p : {(void *)c->mr.dch, (void *)c->mr.blk0, (void *)c->mr.shape, (void *)c->mr.M,
@@ -1903,18 +1971,10 @@ int qed2d_cg_mrhs(qed2d_ctx *c, double *x, const double *U, const double *b, int
                        precision, iters ? iters + j : nullptr));
         return finish(c);
     }
-    int n = 0;
-    long sites = 0;
-    if (c->rhs) (void)qed2d_batch_size(c->rhs, &n, &sites);
-    if (n != nrhs) {
-        if (c->rhs) qed2d_batch_destroy(c->rhs);
-        c->rhs = nullptr;
-        const int rc = qed2d_batch_create_rhs(&c->rhs, c->L.L1, c->L.L2, nrhs, c->beta, c->L.bc,
-                                              (void *)c->stream);
-        if (rc) return fail(c, rc, "cannot create a set of %d right-hand sides", nrhs);
-    }
+    qed2d_batch *set = nullptr;
+    TRY(rhs_set(c, nrhs, &set));
     std::string err;
-    const int rc = batch_rhs_cg(c->rhs, U, x, b, am0, maxiter, eps, precision, iters, c->rows_q,
+    const int rc = batch_rhs_cg(set, U, x, b, am0, maxiter, eps, precision, iters, c->rows_q,
                                 c->rows, c->rows32, c->rows_k2, c->rows32_k2, c->ru_delta, &err);
     if (rc) return fail(c, rc, "%s", err.c_str());
     return finish(c);
@@ -2245,15 +2305,22 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
     hdd h0, h1;
     TRY(hamiltonian_dd(c, mom, U, &h0));
     std::vector<double2 *> F(c->flav.begin(), c->flav.begin() + nf);
+    // multi-shift mode 1: every flavour's heatbath and acceptance solves together
+    bool together = (c->multishift == 1 && nf > 1);
+    for (int j = 0; j < nf; ++j) together = together && npole[j] >= 1 && npole[j] <= kMaxShift;
     for (int j = 0; j < nf; ++j) {
         // src/QED2dHMC.jl:136-144
         launch_fill_pseudofermion(c->L, X, k0, k1, traj, (uint32_t)j, st);
         hdd xx;
         TRY(dot_dd_host(c, (const double *)X, nullptr, &xx));
         h0.add(xx.hi, xx.lo);
-        TRY(gen_pf_impl(c, (double *)F[j], U, (const double *)X, am0[j], npole[j], mu[j], nu[j],
-                        maxiter, tol));
+        if (together)
+            launch_copy(F[j], X, n2, st);
+        else
+            TRY(gen_pf_impl(c, (double *)F[j], U, (const double *)X, am0[j], npole[j], mu[j],
+                            nu[j], maxiter, tol));
     }
+    if (together) TRY(gen_pf_multi(c, F, U, am0, nf, npole, mu, nu, maxiter, tol));
     int rc = md_rhmc(c, integrator, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu,
                      rho, eps, ns, maxiter, tol);
     if (rc != QED2D_OK) {
@@ -2265,10 +2332,23 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
         return rc;
     }
     TRY(hamiltonian_dd(c, mom, U, &h1));
+    if (together) {
+        std::vector<const double *> bj(nf);
+        for (int j = 0; j < nf; ++j) bj[j] = (const double *)F[j];
+        int it = 0;
+        TRY(multishift_multi(c, U, nf, bj.data(), am0, npole, mu, maxiter, tol, &it));
+    }
     for (int j = 0; j < nf; ++j) {
         // src/QED2dHMC.jl:150-156
-        TRY(multicg_impl(c, (double *)X, U, (const double *)F[j], am0[j], npole[j], mu[j], rho[j],
-                         maxiter, tol));
+        if (together) {
+            // multicg_impl's sum over the poles, from the joint solve
+            launch_copy(X, F[j], n2, st);
+            for (int i = 0; i < npole[j]; ++i)
+                launch_scale_add(X, 1.0, c->mr.x[j][i], rho[j][i], n2, st);
+        } else {
+            TRY(multicg_impl(c, (double *)X, U, (const double *)F[j], am0[j], npole[j], mu[j],
+                             rho[j], maxiter, tol));
+        }
         hdd re;
         TRY(dot_dd_host(c, (const double *)X, (const double *)F[j], &re));
         h1.add(re.hi, re.lo);

@@ -46,6 +46,7 @@ struct qed2d_batch {
     std::string err;
     int rows = 16, rows32 = 8, rows_q = 4;
     int rows_k2 = 0, rows32_k2 = 0;  // the second CG kernel's heights; 0: rows / rows32
+    std::vector<double> mu2;  // per chain: the CG operator's shift (Q^2 + mu2); empty: 0
     double ru_delta = 0.01;
     std::vector<double> last_dh;  // per chain, from the double-double energies
     std::map<int, GeoBuf> geo;  // key: 2 * rows + pair
@@ -256,6 +257,9 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     BTRY(get_geo(b, rows64b, false, &g64b));
     BTRY(get_geo(b, b->rows_q, false, &gq));
     const bool mixed = (precision == QED2D_PREC_MIXED);
+    for (double m2 : b->mu2)
+        if (mixed && m2 != 0.0)
+            return bfail(b, QED2D_EINVAL, "shifted batched solves are FP64 only");
     BGeo g32, g32b;
     if (mixed) {
         for (int c = 0; c < b->n; ++c)
@@ -285,6 +289,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             s.eps = tol;
             s.maxiter = maxiter;
             s.delta2 = b->ru_delta * b->ru_delta;
+            s.mu2 = b->mu2.empty() ? 0.0 : b->mu2[c];
             s.kk = 1;
             s.k2 = 1;
         }
@@ -837,14 +842,25 @@ int batch_rhs_cg(qed2d_batch *b, const double *U, double *x, const double *src, 
                  int maxiter, double eps, int precision, int *iters, int rows_q, int rows,
                  int rows32, int rows_k2, int rows32_k2, double ru_delta, std::string *err)
 {
+    std::vector<double> m(b->n, am0);
+    return batch_rhs_cg_shifted(b, U, x, src, m.data(), nullptr, maxiter, eps, precision, iters,
+                                rows_q, rows, rows32, rows_k2, rows32_k2, ru_delta, err);
+}
+
+int batch_rhs_cg_shifted(qed2d_batch *b, const double *U, double *x, const double *src,
+                         const double *am0, const double *mu2, int maxiter, double eps,
+                         int precision, int *iters, int rows_q, int rows, int rows32,
+                         int rows_k2, int rows32_k2, double ru_delta, std::string *err)
+{
     b->rows_q = rows_q;
     b->rows = rows;
     b->rows32 = rows32;
     b->rows_k2 = rows_k2;
     b->rows32_k2 = rows32_k2;
     b->ru_delta = ru_delta;
-    std::vector<double> m(b->n, am0);
-    const int rc = qed2d_batch_cg_shared(b, U, x, src, m.data(), maxiter, eps, precision, iters);
+    if (mu2) b->mu2.assign(mu2, mu2 + b->n);
+    const int rc = qed2d_batch_cg_shared(b, U, x, src, am0, maxiter, eps, precision, iters);
+    b->mu2.clear();
     if (rc && err) *err = b->err;
     return rc;
 }

@@ -179,6 +179,11 @@ struct BParts {
 int batch_rhs_cg(qed2d_batch *b, const double *U, double *x, const double *src, double am0,
                  int maxiter, double eps, int precision, int *iters, int rows_q, int rows,
                  int rows32, int rows_k2, int rows32_k2, double ru_delta, std::string *err);
+// as batch_rhs_cg, with a mass and a shift (Q^2 + mu2[c], null: 0) per right-hand side
+int batch_rhs_cg_shifted(qed2d_batch *b, const double *U, double *x, const double *src,
+                         const double *am0, const double *mu2, int maxiter, double eps,
+                         int precision, int *iters, int rows_q, int rows, int rows32,
+                         int rows_k2, int rows32_k2, double ru_delta, std::string *err);
 // mask (may be null): per chain, nonzero = apply on that chain.
 void launch_b_q_apply(const BGeo &g, int rows, const double2 *U, const double2 *si,
                       double2 *so, const int *mask, hipStream_t st);

@@ -207,8 +207,10 @@ int qed2d_multicg(qed2d_ctx *ctx, double *so, const double *U, const double *si,
 int qed2d_multishift_cg(qed2d_ctx *ctx, double *const *x, const double *U, const double *b,
                         double am0, int n, const double *mu, int maxiter, double eps, int *iters);
 /* How MultiCG, R and the RHMC force solve their shifted systems:
- * 1 (default): multi-shift CG, and in the RHMC force all pseudofermions'
- *    solves together (one launch per kernel over the right-hand sides);
+ * 1 (default): multi-shift CG, and in an RHMC trajectory all pseudofermions'
+ *    solves together (one launch per kernel over the right-hand sides): the
+ *    force's and the acceptance's multi-shift solves, and the heatbath's
+ *    shifted CG solves pole by pole;
  * 2: multi-shift CG, one pseudofermion after another;
  * 0: the reference's sequential (warm-started) CG solves.
  * Modes 1 and 2 give the same bits. */

@@ -448,10 +448,12 @@ def test_autotune_then_parity(Q, O):
 
 @pytest.mark.parametrize("L1,L2,nf", [(16, 16, 4), (48, 32, 3)])
 def test_rhmc_fused_pseudofermions_bit_exact(Q, O, L1, L2, nf):
-    """The RHMC force with every pseudofermion's multi-shift solve in one
-    launch per kernel (mode 1) against one pseudofermion at a time (mode 2):
-    identical bits (energies, accept/reject, links), with different masses and
-    pole counts per pseudofermion."""
+    """An RHMC trajectory with every pseudofermion's solves together (mode 1:
+    the force's and the acceptance's multi-shift solves over all
+    pseudofermions, the heatbath's shifted CG pole by pole over the
+    pseudofermions that have that pole) against one pseudofermion at a time
+    (mode 2): identical bits (energies, accept/reject, links), with different
+    masses and pole counts per pseudofermion."""
     import torch
     say("RHMC fused pseudofermions", L1, L2, nf)
     prm = Q.LattParm((L1, L2), 4.0)
