@@ -44,7 +44,7 @@ struct qed2d_ctx {
     int rows32_k2 = 0;  // K2 of the mixed-precision CG; 0: rows32
     int rows_q = 2;   // FP64 Q / Dw applies
     int rows_q32 = 2; // FP32 Q apply of the public entry point
-    double ru_delta = 0.01;  // mixed CG: reliable update after |r| drops by this factor
+    double ru_delta = kRuDelta;  // mixed CG: reliable update after |r| drops by this factor
     double last_dh = 0.0;    // dH of the last trajectory, from double-double energies
     std::string err;
     double2 *spin[S_NSLOTS] = {};

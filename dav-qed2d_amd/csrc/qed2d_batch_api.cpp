@@ -47,7 +47,7 @@ struct qed2d_batch {
     int rows = 16, rows32 = 8, rows_q = 4;
     int rows_k2 = 0, rows32_k2 = 0;  // the second CG kernel's heights; 0: rows / rows32
     std::vector<double> mu2;  // per chain: the CG operator's shift (Q^2 + mu2); empty: 0
-    double ru_delta = 0.01;
+    double ru_delta = kRuDelta;
     std::vector<double> last_dh;  // per chain, from the double-double energies
     std::map<int, GeoBuf> geo;  // key: 2 * rows + pair
     // fields (arena)

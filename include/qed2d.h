@@ -111,7 +111,7 @@ int qed2d_hbm_stream(qed2d_ctx *ctx, long nbytes, double gbs[3]);
  * reference (src/QED2dHMC.jl:95).  No counterpart in the reference. */
 int qed2d_ctx_last_dh(const qed2d_ctx *ctx, double *dH);
 /* Mixed-precision CG: a reliable update (FP64 true residual) once the FP32
- * iterated residual has dropped by delta since the last one (default 0.01). */
+ * iterated residual has dropped by delta since the last one (default 1e-3). */
 int qed2d_ctx_set_reliable_delta(qed2d_ctx *ctx, double delta);
 int qed2d_synchronize(qed2d_ctx *ctx);
 

@@ -174,8 +174,10 @@ def test_master_field_8192_densities(Q, O):
 def test_nf2_4096_mixed_vs_fp64(Q, O):
     """Nf=2 at 4096x4096, beta 11.25, m0 0.05 (links after 3 pure-gauge
     trajectories from a cold start): one leapfrog trajectory (ns 2, eps 0.05)
-    with FP32 force solves and reliable updates against the FP64 form; same
-    accept/reject, |dH_mixed - dH_fp64| < 1e-8."""
+    with FP32 force solves and reliable updates against the FP64 form at the
+    same CG tolerance 1e-20: same accept/reject, and |dH_mixed - dH_fp64| below
+    a tenth of what the tolerance itself moves dH (FP64 at 1e-20 against FP64
+    at 1e-22, about 1e-6 here) and below 1e-7."""
     import torch
     say("nf2 4096^2 mixed vs fp64")
     L = 4096
@@ -186,15 +188,17 @@ def test_nf2_4096_mixed_vs_fp64(Q, O):
         Q.HMC(U, 0.025, 20, [], prm, None)
     U0 = U.clone()
     res = {}
-    for prec in (1, 0):
+    for key, prec, tol in ((1, 1, 1e-20), (0, 0, 1e-20), ("tight", 0, 1e-22)):
         U.copy_(U0)
         Q.seed(55)
-        res[prec] = Q.HMC(U, 0.05, 0.05, 2, [], 100000, 1e-20, prm, None, precision=prec,
-                          acc_tol=1e-20)
+        res[key] = Q.HMC(U, 0.05, 0.05, 2, [], 100000, tol, prm, None, precision=prec,
+                         acc_tol=tol)
         torch.cuda.synchronize()
-    say("dH mixed", res[1].dH, "fp64", res[0].dH, "it", res[1].cg_iters, res[0].cg_iters)
+    moved = abs(res[0].dH - res["tight"].dH)
+    say("dH mixed", res[1].dH, "fp64", res[0].dH, "it", res[1].cg_iters, res[0].cg_iters,
+        "fp64 1e-20 vs 1e-22", moved)
     assert res[1].accepted == res[0].accepted
-    assert abs(res[1].dH - res[0].dH) < 1e-8
+    assert abs(res[1].dH - res[0].dH) < min(0.1 * moved, 1e-7)
 
 
 def test_rhmc_128_sixteen_poles_four_pseudofermions(Q, O):
