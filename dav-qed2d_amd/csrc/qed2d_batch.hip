@@ -352,7 +352,8 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, LinksF32 U,
         return;
     const long o = w.o;
     const float2 *pold = (rd ? P1 : P0) + o;
-    EpiK1v epi{(wp ? P1 : P0) + o, t + o, x + o, (float)Sc->alpha_x, !fresh, w.L.V, 0.0, 0.0};
+    EpiK1v epi{(wp ? P1 : P0) + o, t + o, x + o, (float)Sc->alpha_x, !fresh, !fresh, w.L.V,
+               0.0, 0.0};
     if (w.p.active) {
         SrcXpay2 src{r + o, pold, (float)beta, w.L.V};
         q_march2x<R>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);

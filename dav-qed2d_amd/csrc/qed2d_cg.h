@@ -44,6 +44,19 @@ __device__ __forceinline__ K1In k1_in(const SolverState *S, int k)
     return v;
 }
 
+// The same through vector loads: a kernel that issues them before its first
+// rows' loads keeps them out of the scalar-load count, which the rows' address
+// arithmetic (kernel arguments) waits on.
+// The solver state seen through a per-lane address (lane offset 0, opaque to
+// the compiler): its loads and values stay in vector registers, so nothing
+// waits for them before their first use.
+__device__ __forceinline__ const SolverState *state_v(const SolverState *S)
+{
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    return (const SolverState *)((const char *)S + z);
+}
+
 template <bool mixed>
 __device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, double rr,
                                              const double *Pb, int nbb, int k, int blk,
@@ -169,7 +182,8 @@ struct EpiK1v {
     float2 *__restrict__ t;
     float2 *__restrict__ x;
     float alpha;  // alpha_{k-1}
-    bool upd;
+    bool upd;     // apply the pending update (may be decided after the loads)
+    bool ld;      // load x with the rows (set wherever upd may end up set)
     long V;
     double tt, pp;
     struct Pre {
@@ -178,7 +192,7 @@ struct EpiK1v {
     // p_{k-1} of the row comes with the source's raw loads (SrcXpay2::Raw2)
     template <typename Raw> __device__ __forceinline__ Pre pre_raw(long s, const Raw &w) const
     {
-        if (!upd) return Pre{};
+        if (!ld) return Pre{};
         return Pre{ld2(x, s), ld2(x, V + s), w.p0, w.p1};
     }
     __device__ __forceinline__ void xupd(long i, F2x2 xv, F2x2 pv)

@@ -567,6 +567,21 @@ __device__ __forceinline__ bool q_march2xr(const Latt &L, const LinksF32 &u, flo
     return q_march2r<R>(L, lk, mass2, src, epi, j, r0, active, ready);
 }
 
+// q_march2xr with the link format fixed at compile time (PK: packed), so that
+// no branch on it sits between a kernel's first loads and its march.
+template <int R, bool PK, typename Src, typename Epi, typename Ready>
+__device__ __forceinline__ bool q_march2t(const Latt &L, const LinksF32 &u, float mass2, Src &src,
+                                          Epi &epi, int j, int r0, bool active, Ready &&ready)
+{
+    if constexpr (PK) {
+        const LinksPacked lk{(const float *)u.U};
+        return q_march2r<R>(L, lk, mass2, src, epi, j, r0, active, ready);
+    } else {
+        const LinksF2 lk{(const float2 *)u.U};
+        return q_march2r<R>(L, lk, mass2, src, epi, j, r0, active, ready);
+    }
+}
+
 // q_march2l with FP32 complex links.
 template <int R, typename Src, typename Epi>
 __device__ __forceinline__ void q_march2(const Latt &L, const float2 *__restrict__ U, float mass2,
@@ -620,6 +635,24 @@ inline MarchGrid march_grid2(const Latt &L, int rows)
     case 8: launch_k((KERNEL<8>), __VA_ARGS__); break;                      \
     default: launch_k((KERNEL<16>), __VA_ARGS__); break;                    \
     }
+
+// DISPATCH_ROWS for kernels templated on <rows, packed links>.
+#define DISPATCH_ROWS_PK_(rows, KERNEL, PK, ...)                                     \
+    switch (rows) {                                                                   \
+    case 1: launch_k((KERNEL<1, PK>), __VA_ARGS__); break;                            \
+    case 2: launch_k((KERNEL<2, PK>), __VA_ARGS__); break;                            \
+    case 4: launch_k((KERNEL<4, PK>), __VA_ARGS__); break;                            \
+    case 8: launch_k((KERNEL<8, PK>), __VA_ARGS__); break;                            \
+    default: launch_k((KERNEL<16, PK>), __VA_ARGS__); break;                          \
+    }
+#define DISPATCH_ROWS_PK(rows, packed, KERNEL, ...)                                   \
+    do {                                                                              \
+        if (packed) {                                                                 \
+            DISPATCH_ROWS_PK_(rows, KERNEL, true, __VA_ARGS__)                        \
+        } else {                                                                      \
+            DISPATCH_ROWS_PK_(rows, KERNEL, false, __VA_ARGS__)                       \
+        }                                                                             \
+    } while (0)
 
 // Sums over the block, valid in thread 0 (a producer's per-block partial):
 // one barrier, the waves' sums added in wave order (as block_allsum).  At most

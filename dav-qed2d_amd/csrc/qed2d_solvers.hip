@@ -222,20 +222,22 @@ __global__ void __launch_bounds__(kBlock) k_cg1(Latt L, const c2<T> *__restrict_
                                                 int by, int nbx)
 {
     __shared__ double lds[kBlock / kWave + 1];
-    const int sk = S->stop_k;
-    if (sk != 0 && sk < k) return;
+    // The solver state's loads go out first, the partials' next, the first
+    // rows' last; nothing waits for them before the rows' loads are in flight.
+    // A solve that stopped before K1(k) returns before any store.
+    const SolverState *Sv = state_v(S);
+    const int sk = Sv->stop_k;
+    const K1In in = k1_in(Sv, k);
     const int w = k & 1, rd = (k - 1) & 1;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    // the partials' loads go out first, the first rows' loads next; beta is
-    // formed once both are in flight
     PartialsAhead prr;
     prr.load(Prr, nrr, 1, 0);
-    const K1In in = k1_in(S, k);
     const c2<T> *pold = rd ? P1 : P0;
     c2<T> *pnew = w ? P1 : P0;
     EpiK1<T> epi{pnew, t, L.V, 0.0, 0.0};
     SrcXpay<T> src{r, pold, T(0), L.V};
     auto ready = [&]() -> bool {
+        if (sk != 0 && sk < k) return false;
         double beta;
         if (!cg_k1_decide<std::is_same<T, float>::value>(S, in, prr.sum(), Pb, nbb, k, p.blk,
                                                           lds, beta))
@@ -260,31 +262,35 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
                                                 const double *Ptt, double *Prr, int ntt, int k,
                                                 int bx, int by, int nbx)
 {
-    const int sk = S->stop_k;
-    if (sk != 0 && sk <= k) return;
+    // The solver state's loads go out first, the partials' next, the first
+    // rows' last; nothing waits for them before the rows' loads are in flight.
+    // A solve that stopped before K2(k) returns before any store, and alpha
+    // (src/QED2dSolver.jl:21-22) is formed once all are in.
     const int w = k & 1;
+    const SolverState *Sv = state_v(S);
+    const int sk = Sv->stop_k;
+    const double nw = Sv->norm[w], mu2 = Sv->mu2;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    // alpha (src/QED2dSolver.jl:21-22) is formed once the partials' loads and
-    // the first rows' loads are in flight
     PartialsAhead2 ptt;
     ptt.load(Ptt, ntt);
-    const double nw = S->norm[w], mu2 = S->mu2;
     const c2<T> *pv = w ? P1 : P0;
-    EpiK2<T> epi{pv, x, r, T(0), (T)mu2, L.V, 0.0};
+    EpiK2<T> epi{pv, x, r, T(0), T(0), L.V, 0.0};
     auto ready = [&]() -> bool {
+        if (sk != 0 && sk <= k) return false;
         double tt, pp;
         ptt.sum(tt, pp);
         epi.alpha = (T)(nw / (tt + mu2 * pp));
+        epi.mu2 = (T)mu2;
         return true;
     };
     SrcPlain<T> src{t, L.V};
-    q_march_r<T, R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready);
+    if (!q_march_r<T, R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) return;
     const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) Prr[p.blk] = a;
 }
 
 // ---- FP32 pair kernels (two sites per thread along i1) -------------------
-template <int R>
+template <int R, bool PK>
 __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
                                                  const float2 *__restrict__ r, float2 *P0,
                                                  float2 *P1, float2 *__restrict__ t,
@@ -294,26 +300,33 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
                                                  int by, int nbx)
 {
     __shared__ double lds[kBlock / kWave + 1];
-    const int sk = S->stop_k;
-    if (sk != 0 && sk < k) return;
+    // The solver state's loads go out first, the partials' next, the first
+    // rows' last (x with them whenever an update may be pending); nothing waits
+    // for them before the rows' loads are in flight.  A solve that stopped
+    // before K1(k) returns before any store.
+    const SolverState *Sv = state_v(S);
+    const int sk = Sv->stop_k;
+    const K1In in = k1_in(Sv, k);
+    const float ax = (float)Sv->alpha_x;
     const int w = k & 1, rd = (k - 1) & 1;
     const MarchPos p = march_pos2(L, bx, by, nbx, R);
-    // the partials' loads go out first, the first rows' loads next; beta is
-    // formed once both are in flight
     PartialsAhead prr;
     prr.load(Prr, nrr, 1, 0);
-    const K1In in = k1_in(S, k);
-    const bool fresh = (k == 1) || (in.true_at == k);
     float2 *pold = rd ? P1 : P0;
-    EpiK1v epi{w ? P1 : P0, t, x, (float)S->alpha_x, !fresh, L.V, 0.0, 0.0};
+    EpiK1v epi{w ? P1 : P0, t, x, 0.0f, false, k != 1, L.V, 0.0, 0.0};
     SrcXpay2 src{r, pold, 0.0f, L.V};
     auto ready = [&]() -> bool {
+        if (sk != 0 && sk < k) return false;
         double beta;
-        if (!cg_k1_decide<true>(S, in, prr.sum(), Pb, nbb, k, p.blk, lds, beta)) return false;
+        bool fresh;
+        if (!cg_k1_decide<true>(S, in, prr.sum(), Pb, nbb, k, p.blk, lds, beta, &fresh))
+            return false;
         src.beta = (float)beta;
+        epi.alpha = ax;
+        epi.upd = !fresh;
         return true;
     };
-    if (!q_march2xr<R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) return;
+    if (!q_march2t<R, PK>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) return;
     double a, b;
     block_sum2_t0(epi.tt, epi.pp, a, b);
     if (threadIdx.x == 0) {
@@ -322,33 +335,36 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
     }
 }
 
-template <int R, bool MU>
+template <int R, bool MU, bool PK>
 __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
                                           const float2 *__restrict__ t, const float2 *P0,
                                           const float2 *P1, float2 *__restrict__ r, float mass2,
                                           SolverState *S, const double *Ptt, double *Prr, int ntt,
                                           int k, int bx, int by, int nbx)
 {
-    const int sk = S->stop_k;
-    if (sk != 0 && sk <= k) return;
+    // The solver state's loads go out first, the partials' next, the first
+    // rows' last; nothing waits for them before the rows' loads are in flight.
+    // A solve that stopped before K2(k) returns before any store.
     const int w = k & 1;
+    const SolverState *Sv = state_v(S);
+    const int sk = Sv->stop_k;
+    const double nw = Sv->norm[w], mu2 = Sv->mu2;
     const MarchPos p = march_pos2(L, bx, by, nbx, R);
-    // alpha is formed once the partials' loads and the first rows' loads are
-    // in flight (src/QED2dSolver.jl:21-22)
     PartialsAhead2 ptt;
     ptt.load(Ptt, ntt);
-    const double nw = S->norm[w], mu2 = S->mu2;
-    EpiK2v<MU> epi{w ? P1 : P0, r, 0.0f, (float)mu2, L.V, 0.0};
+    EpiK2v<MU> epi{w ? P1 : P0, r, 0.0f, 0.0f, L.V, 0.0};
     double alpha = 0.0;
     auto ready = [&]() -> bool {
+        if (sk != 0 && sk <= k) return false;
         double tt, pp;
         ptt.sum(tt, pp);
         alpha = nw / (tt + mu2 * pp);
         epi.alpha = (float)alpha;
+        epi.mu2 = (float)mu2;
         return true;
     };
-    SrcPlain2 src{t, L.V};
-    q_march2xr<R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready);
+    SrcPlain2 src{t, L.V};  // alpha: src/QED2dSolver.jl:21-22
+    if (!q_march2t<R, PK>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) return;
     const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) {
         Prr[p.blk] = a;
@@ -356,14 +372,14 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
     }
 }
 #define CG2V_KERNEL(NAME, MU)                                                              \
-    template <int R>                                                                       \
+    template <int R, bool PK>                                                              \
     __global__ void __launch_bounds__(kBlock)                                              \
         NAME(Latt L, LinksF32 U, const float2 *__restrict__ t,                            \
              const float2 *P0, const float2 *P1, float2 *__restrict__ r, float mass2,      \
              SolverState *S, const double *Ptt, double *Prr, int ntt, int k, int bx, int by, \
              int nbx)                                                                      \
     {                                                                                      \
-        cg2v_body<R, MU>(L, U, t, P0, P1, r, mass2, S, Ptt, Prr, ntt, k, bx, by, nbx);     \
+        cg2v_body<R, MU, PK>(L, U, t, P0, P1, r, mass2, S, Ptt, Prr, ntt, k, bx, by, nbx); \
     }
 CG2V_KERNEL(k_cg2v, false)
 CG2V_KERNEL(k_cg2vm, true)  // mu2 != 0
@@ -382,8 +398,8 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
             MarchGrid g = march_grid2(L, rows);
             dim3 grid(g.nblocks()), blk(kBlock);
             const LinksF32 lk{U, packed};
-            DISPATCH_ROWS(rows, k_cg1v, grid, blk, 0, st, L, lk, r, p0, p1, t, x, m2, S, ptt,
-                          prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx);
+            DISPATCH_ROWS_PK(rows, packed, k_cg1v, grid, blk, 0, st, L, lk, r, p0, p1, t, x, m2,
+                             S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx);
             P.ntt[k & 1] = fold(ptt, g.nblocks(), 2, st);
             return;
         }
@@ -416,11 +432,11 @@ static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> 
             // x is updated by the next K1 (or the reliable update's flush)
             const LinksF32 lk{U, packed};
             if (mu) {
-                DISPATCH_ROWS(rows, k_cg2vm, grid, blk, 0, st, L, lk, t, p0, p1, r, m2, S, ptt,
-                              prr, ntt, k, g.bx, g.by, g.nbx);
+                DISPATCH_ROWS_PK(rows, packed, k_cg2vm, grid, blk, 0, st, L, lk, t, p0, p1, r, m2,
+                                 S, ptt, prr, ntt, k, g.bx, g.by, g.nbx);
             } else {
-                DISPATCH_ROWS(rows, k_cg2v, grid, blk, 0, st, L, lk, t, p0, p1, r, m2, S, ptt,
-                              prr, ntt, k, g.bx, g.by, g.nbx);
+                DISPATCH_ROWS_PK(rows, packed, k_cg2v, grid, blk, 0, st, L, lk, t, p0, p1, r, m2,
+                                 S, ptt, prr, ntt, k, g.bx, g.by, g.nbx);
             }
             P.nrr[k & 1] = fold(prr, g.nblocks(), 1, st);
             return;
