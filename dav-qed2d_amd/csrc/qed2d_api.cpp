@@ -414,6 +414,7 @@ int hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, hdd *H)
 struct SolveArgs {
     double am0 = 0.0, mu2 = 0.0, eps = 0.0;
     int maxiter = 0, guess = 0;
+    int chunk0 = 8;  // iterations queued before the first look at the state
 };
 
 int upload_state(qed2d_ctx *c, const SolveArgs &a)
@@ -490,7 +491,7 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
         launch_cg_init_f64(L, (const double2 *)b, (double2 *)x, r, P[0], c->P, c->rows, st);
     }
     int k = 1;
-    int chunk = 8;
+    int chunk = a.chunk0;
     ChunkPredictor pred;
     for (;;) {
         for (int j = 0; j < chunk && k <= a.maxiter + 1; ++j, ++k) {
@@ -533,7 +534,7 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     // not restarts with FP32 complex links (x is untouched until then).
     bool packed = pair_ok(L);
     int k = 1;
-    int chunk = 8;
+    int chunk = a.chunk0;
     ChunkPredictor pred;
     auto start = [&]() -> int {
         TRY(upload_state(c, a));
@@ -551,7 +552,7 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
                                  st);
         }
         k = 1;
-        chunk = 8;
+        chunk = a.chunk0;
         pred = ChunkPredictor();
         return QED2D_OK;
     };
@@ -1414,11 +1415,16 @@ int autotune_impl(qed2d_ctx *c, int iters)
         if (us < best) best = us, pick = R;
     }
     if (!rc) c->rows_q32 = pick;
-    // CG kernels (one timed solve each, `iters` iterations)
+    // CG kernels: solves of `iters` iterations queued in one go (one look at
+    // the state, at the end; no reliable update before the last iteration),
+    // after one untimed solve the mean of three
     SolveArgs a;
     a.am0 = 0.1;
     a.eps = 0.0;
     a.maxiter = iters;
+    a.chunk0 = iters + 1;
+    const double keep_delta = c->ru_delta;
+    c->ru_delta = 1e-30;
     int it;
     // (the two kernels of an iteration are timed together, every pair of
     // heights: K1 and K2 may prefer different ones)
@@ -1434,7 +1440,7 @@ int autotune_impl(qed2d_ctx *c, int iters)
                 if (prec == 0 && R2 == 1) continue;
                 tile = R;
                 tile2 = R2;
-                rc = time_runs(c, 1, [&] {
+                rc = time_runs(c, 3, [&] {
                     if (prec) (void)cg_mixed(c, (double *)x, (const double *)U, (const double *)b, a, &it);
                     else (void)cg_f64(c, (double *)x, (const double *)U, (const double *)b, a, &it);
                 }, &us);
@@ -1446,6 +1452,7 @@ int autotune_impl(qed2d_ctx *c, int iters)
         tile = rc ? keep : pick;
         tile2 = rc ? keep2 : (pick2 == pick ? 0 : pick2);
     }
+    c->ru_delta = keep_delta;
     c->sync = save_sync;
     (void)hipStreamSynchronize(st);
     (void)hipFree(mem);
