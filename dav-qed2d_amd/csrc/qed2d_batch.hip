@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2(BGeo g, const c2<T> *__restrict
     }
 }
 
-template <int R>
+template <int R, bool PK>
 __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, LinksF32 U,
                                                   const float2 *__restrict__ r, float2 *P0,
                                                   float2 *P1, float2 *__restrict__ t,
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, LinksF32 U,
                0.0, 0.0};
     if (w.p.active) {
         SrcXpay2 src{r + o, pold, (float)beta, w.L.V};
-        q_march2x<R>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
+        q_march2xt<R, PK>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
     }
     double a, b;
     block_sum2_t0(epi.tt, epi.pp, a, b);
@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, LinksF32 U,
     }
 }
 
-template <int R>
+template <int R, bool PK>
 __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, LinksF32 U,
                                                   const float2 *__restrict__ t, const float2 *P0,
                                                   const float2 *P1, float2 *__restrict__ r,
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, LinksF32 U,
     EpiK2v<false> epi{(wp ? P1 : P0) + o, r + o, (float)alpha, 0.0f, w.L.V, 0.0};
     if (w.p.active) {
         SrcPlain2 src{t + o, w.L.V};
-        q_march2x<R>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
+        q_march2xt<R, PK>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
     }
     const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) {
@@ -430,8 +430,8 @@ void launch_b_cg_k1_f32(const BGeo &g, int rows, const void *U, bool packed, con
                         hipStream_t st)
 {
     const LinksF32 lk{U, packed};
-    DISPATCH_ROWS(rows, kb_cg1v, dim3(g.nblocks), dim3(kBlock), 0, st, g, lk, r, p0, p1, t, x, S,
-                  P);
+    DISPATCH_ROWS_PK(rows, packed, kb_cg1v, dim3(g.nblocks), dim3(kBlock), 0, st, g, lk, r, p0,
+                     p1, t, x, S, P);
 }
 // (x is updated by the next K1 or the reliable update's flush)
 void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *t,
@@ -439,8 +439,8 @@ void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, con
                         SolverState *S, BParts P, hipStream_t st)
 {
     const LinksF32 lk{U, packed};
-    DISPATCH_ROWS(rows, kb_cg2v, dim3(g.nblocks), dim3(kBlock), 0, st, g, lk, t, p0, p1, r, S,
-                  P);
+    DISPATCH_ROWS_PK(rows, packed, kb_cg2v, dim3(g.nblocks), dim3(kBlock), 0, st, g, lk, t, p0,
+                     p1, r, S, P);
 }
 
 // ---------------------------------------------------------------------------

@@ -540,12 +540,13 @@ struct LinksPacked {
     __device__ __forceinline__ float2 value1(float r) const { return link_dec(r); }
 };
 
-// q_march2l with the links the solver kernels were given.
-template <int R, typename Src, typename Epi>
-__device__ __forceinline__ void q_march2x(const Latt &L, const LinksF32 &u, float mass2,
-                                          const Src &src, Epi &epi, int j, int r0)
+// q_march2l over FP32 links whose format is fixed at compile time (PK: packed
+// U(1) links, else complex).
+template <int R, bool PK, typename Src, typename Epi>
+__device__ __forceinline__ void q_march2xt(const Latt &L, const LinksF32 &u, float mass2,
+                                           const Src &src, Epi &epi, int j, int r0)
 {
-    if (u.packed) {
+    if constexpr (PK) {
         const LinksPacked lk{(const float *)u.U};
         q_march2l<R>(L, lk, mass2, src, epi, j, r0);
     } else {
@@ -554,21 +555,8 @@ __device__ __forceinline__ void q_march2x(const Latt &L, const LinksF32 &u, floa
     }
 }
 
-// q_march2r with the links the solver kernels were given.
-template <int R, typename Src, typename Epi, typename Ready>
-__device__ __forceinline__ bool q_march2xr(const Latt &L, const LinksF32 &u, float mass2, Src &src,
-                                           Epi &epi, int j, int r0, bool active, Ready &&ready)
-{
-    if (u.packed) {
-        const LinksPacked lk{(const float *)u.U};
-        return q_march2r<R>(L, lk, mass2, src, epi, j, r0, active, ready);
-    }
-    const LinksF2 lk{(const float2 *)u.U};
-    return q_march2r<R>(L, lk, mass2, src, epi, j, r0, active, ready);
-}
-
-// q_march2xr with the link format fixed at compile time (PK: packed), so that
-// no branch on it sits between a kernel's first loads and its march.
+// q_march2r over FP32 links whose format is fixed at compile time, so that no
+// branch on it sits between a kernel's first loads and its march.
 template <int R, bool PK, typename Src, typename Epi, typename Ready>
 __device__ __forceinline__ bool q_march2t(const Latt &L, const LinksF32 &u, float mass2, Src &src,
                                           Epi &epi, int j, int r0, bool active, Ready &&ready)
