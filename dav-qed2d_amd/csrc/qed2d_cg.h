@@ -531,14 +531,27 @@ __device__ __forceinline__ void ms2_body(const Latt &L, const double2 *__restric
 {
     __shared__ double lds[kBlock / kWave + 1];
     const int w = k & 1;
-    const double tt = sum_prev_partials(Ptt_w, ntt, 2, 0, lds);
-    const double pp = sum_prev_partials(Ptt_w, ntt, 2, 1, lds);
-    const double alpha = S->norm[w] / (tt + S->sigma_b * pp);
-    EpiMS2 epi{r, w ? P1 : P0, alpha, S->sigma_b, L.V, 0.0};
-    if (p.active) {
-        SrcPlain<double> src{t, L.V};
-        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
-    }
+    const double nw = S->norm[w], sb = S->sigma_b;
+    // The partials' loads go out first (when they fit the per-lane buffer), the
+    // first rows' loads next; alpha is formed once both are in flight, with
+    // the same sums in the same order either way.
+    const bool ahead = ntt <= kPartMax;
+    PartialsAhead2 ptt;
+    if (ahead) ptt.load(Ptt_w, ntt);
+    EpiMS2 epi{r, w ? P1 : P0, 0.0, sb, L.V, 0.0};
+    auto ready = [&]() -> bool {
+        double tt, pp;
+        if (ahead) {
+            ptt.sum(tt, pp);
+        } else {
+            tt = sum_prev_partials(Ptt_w, ntt, 2, 0, lds);
+            pp = sum_prev_partials(Ptt_w, ntt, 2, 1, lds);
+        }
+        epi.alpha = nw / (tt + sb * pp);
+        return true;
+    };
+    SrcPlain<double> src{t, L.V};
+    q_march_r<double, R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready);
     const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) Prr_w[p.blk] = a;
 }
