@@ -376,18 +376,30 @@ __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, LinksF32 U,
     __shared__ double lds[kBlock / kWave + 1];
     const BWhere w = bwhere<true>(g, R);
     SolverState *Sc = S + w.c;
-    if (Sc->stop_k != 0) return;
-    const int k = Sc->k2;
-    const int wp = k & 1;
-    const double tt = sum_prev_partials(P.tt[wp] + 2 * w.pb, P.ntt[wp][w.c], 2, 0, lds);
-    const double pp = sum_prev_partials(P.tt[wp] + 2 * w.pb, P.ntt[wp][w.c], 2, 1, lds);
-    const double alpha = Sc->norm[wp] / (tt + Sc->mu2 * pp);
+    // The chain's state goes out first through a per-lane address, the first
+    // rows' loads next (they do not depend on the iteration); the partials of
+    // K1 (slot by iteration) and alpha follow once the state is in.  A chain
+    // that stopped returns before any store.
+    const SolverState *Sv = state_v(Sc);
+    const int skv = Sv->stop_k, kv = Sv->k2;
     const long o = w.o;
-    EpiK2v<false> epi{(wp ? P1 : P0) + o, r + o, (float)alpha, 0.0f, w.L.V, 0.0};
-    if (w.p.active) {
-        SrcPlain2 src{t + o, w.L.V};
-        q_march2xt<R, PK>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0);
-    }
+    EpiK2v<false> epi{nullptr, r + o, 0.0f, 0.0f, w.L.V, 0.0};  // (p: read only with a shift)
+    int k = 0, wp = 0;
+    double alpha = 0.0;
+    auto ready = [&]() -> bool {
+        if (__builtin_amdgcn_readfirstlane(skv) != 0) return false;
+        k = __builtin_amdgcn_readfirstlane(kv);
+        wp = k & 1;
+        const double tt = sum_prev_partials(P.tt[wp] + 2 * w.pb, P.ntt[wp][w.c], 2, 0, lds);
+        const double pp = sum_prev_partials(P.tt[wp] + 2 * w.pb, P.ntt[wp][w.c], 2, 1, lds);
+        alpha = Sc->norm[wp] / (tt + Sc->mu2 * pp);
+        epi.alpha = (float)alpha;
+        return true;
+    };
+    SrcPlain2 src{t + o, w.L.V};
+    if (!q_march2t<R, PK>(w.L, U.at(w.uo), 2.0f + (float)w.am0, src, epi, w.p.i1, w.p.r0,
+                          w.p.active, ready))
+        return;
     const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) {
         P.rr[wp][w.pb + w.p.blk] = a;
