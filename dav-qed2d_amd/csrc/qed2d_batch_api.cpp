@@ -242,7 +242,7 @@ struct Predict {  // the single-lattice solvers' chunk predictor, per chain
 // (Q^2) x_c = src_c on every chain from x = 0, to |r|^2 < tol |b|^2 (the
 // single-lattice cg_f64 / cg_mixed, one launch per kernel over all chains).
 int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double tol,
-             int precision, std::vector<int> &iters)
+             int precision, std::vector<int> &iters, int chunk0 = 8)
 {
     const long nel = 2 * b->vtot;
     hipStream_t st = b->stream;
@@ -280,7 +280,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     // whose links are not restarts with FP32 complex links.
     bool packed = mixed;
     std::vector<Predict> pred;
-    int chunk = 8;
+    int chunk = chunk0;
     long launched_k = 0;
     auto start = [&]() -> int {
         for (int c = 0; c < b->n; ++c) {
@@ -306,7 +306,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         }
         BTRY(blaunched(b));
         pred.assign(b->n, Predict());
-        chunk = 8;
+        chunk = chunk0;
         launched_k = 0;
         return QED2D_OK;
     };
@@ -403,18 +403,21 @@ int batch_autotune(qed2d_batch *b, int iters)
     BHIP(b, hipEventCreate(&e0));
     BHIP(b, hipEventCreate(&e1));
     int rc = QED2D_OK;
-    // one warm-up run, then one timed (statuses of the cut-off solves ignored)
+    // one warm-up run, then the mean of three (statuses of the cut-off solves
+    // ignored)
     auto timed = [&](const std::function<int()> &fn, double *us) -> int {
         int r = fn();
         if (r != QED2D_OK && r != QED2D_ENOCONV) return r;
         if (hipEventRecord(e0, st) != hipSuccess) return QED2D_EHIP;
-        r = fn();
-        if (r != QED2D_OK && r != QED2D_ENOCONV) return r;
+        for (int rep = 0; rep < 3; ++rep) {
+            r = fn();
+            if (r != QED2D_OK && r != QED2D_ENOCONV) return r;
+        }
         if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
             return QED2D_EHIP;
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return QED2D_EHIP;
-        *us = 1e3 * ms;
+        *us = 1e3 * ms / 3;
         return QED2D_OK;
     };
     const int cand[5] = {1, 2, 4, 8, 16};
@@ -435,6 +438,10 @@ int batch_autotune(qed2d_batch *b, int iters)
     bool pairs = true;
     for (const BChain &ch : b->hch) pairs = pairs && pair_ok(ch.L);
     std::vector<int> it;
+    // (the solves are queued in one chunk, one look at the states at the end,
+    // and make no reliable update before their last iteration)
+    const double keep_delta = b->ru_delta;
+    b->ru_delta = 1e-30;
     for (int prec = 0; prec < 2 && !rc; ++prec) {
         if (prec == 1 && !pairs) break;  // the FP32 batch needs even L1 >= 4 everywhere
         int &t1 = prec ? b->rows32 : b->rows;
@@ -448,7 +455,7 @@ int batch_autotune(qed2d_batch *b, int iters)
                 t2 = R2;
                 rc = timed([&] {
                     return batch_cg(b, b->X, b->F, iters, 0.0,
-                                    prec ? QED2D_PREC_MIXED : QED2D_PREC_F64, it);
+                                    prec ? QED2D_PREC_MIXED : QED2D_PREC_F64, it, iters + 1);
                 }, &us);
                 if (rc) break;
                 if (us < best) best = us, p1 = R1, p2 = R2;
@@ -458,6 +465,7 @@ int batch_autotune(qed2d_batch *b, int iters)
         t1 = rc ? keep1 : p1;
         t2 = rc ? keep2 : (p2 == p1 ? 0 : p2);
     }
+    b->ru_delta = keep_delta;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return bfail(b, rc, "batch autotune failed: %s", b->err.c_str());
