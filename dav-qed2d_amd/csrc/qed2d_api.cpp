@@ -2188,6 +2188,24 @@ int qed2d_md_nf2(qed2d_ctx *c, int integrator, double *mom, double *U, double *X
     return finish(c);
 }
 
+// A trajectory that fails after U was saved to Ucp restores U and reports no
+// acceptance; the first error's message is kept.
+static int restore_fail(qed2d_ctx *c, double *U, const double2 *Ucp, int rc, int *accepted)
+{
+    std::string msg = c->err;
+    launch_copy((double2 *)U, Ucp, spinor_elems(c), c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    c->err = msg;
+    *accepted = 0;
+    return rc;
+}
+
+#define RTRY(call)                                                            \
+    do {                                                                      \
+        int rc_ = (call);                                                     \
+        if (rc_ != QED2D_OK) return restore_fail(c, U, Ucp, rc_, accepted);   \
+    } while (0)
+
 int qed2d_hmc_pure(qed2d_ctx *c, double *U, double eps, int ns, int integrator, int qzero,
                    uint64_t seed, uint32_t chain, uint32_t traj, double *hini, double *hfin,
                    int *accepted)
@@ -2204,14 +2222,14 @@ int qed2d_hmc_pure(qed2d_ctx *c, double *U, double eps, int ns, int integrator, 
     key_of(seed, chain, &k0, &k1);
     launch_fill_momenta(c->L, mom, k0, k1, traj, c->stream);
     hdd h0, h1;
-    TRY(hamiltonian_dd(c, mom, U, &h0));
-    if (integrator == QED2D_OMF4) TRY(omf4_pure(c, mom, U, eps, ns));
-    else TRY(leapfrog_pure(c, mom, U, eps, ns));
-    TRY(hamiltonian_dd(c, mom, U, &h1));
+    RTRY(hamiltonian_dd(c, mom, U, &h0));
+    if (integrator == QED2D_OMF4) RTRY(omf4_pure(c, mom, U, eps, ns));
+    else RTRY(leapfrog_pure(c, mom, U, eps, ns));
+    RTRY(hamiltonian_dd(c, mom, U, &h1));
     *hini = h0.value();
     *hfin = h1.value();
     c->last_dh = dd_diff(h1, h0);
-    TRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
+    RTRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
     if (!*accepted) launch_copy((double2 *)U, Ucp, n2, c->stream);
     return finish(c);
 }
@@ -2239,44 +2257,37 @@ int qed2d_hmc_nf2(qed2d_ctx *c, double *U, double am0, double eps, int ns, int i
     launch_fill_pseudofermion(c->L, X, k0, k1, traj, 0, st);
     // hini = H + X^dag X  (src/QED2dHMC.jl:74)
     hdd h0, h1;
-    TRY(hamiltonian_dd(c, mom, U, &h0));
+    RTRY(hamiltonian_dd(c, mom, U, &h0));
     hdd xx;
-    TRY(dot_dd_host(c, (const double *)X, nullptr, &xx));
+    RTRY(dot_dd_host(c, (const double *)X, nullptr, &xx));
     h0.add(xx.hi, xx.lo);
     q64(c, U, (const double *)X, (double *)F, am0);  // F = Q X  (:76-78)
     int it = 0;
     int rc = md_nf2(c, integrator, mom, U, (double *)X, (const double *)F, (double *)g5DX, am0,
                     eps, ns, maxiter, tol, precision, &it);
     if (cg_iters) *cg_iters = it;
-    if (rc != QED2D_OK) {
-        std::string msg = c->err;
-        launch_copy((double2 *)U, Ucp, n2, st);
-        (void)hipStreamSynchronize(st);
-        c->err = msg;
-        *accepted = 0;
-        return rc;
-    }
-    TRY(hamiltonian_dd(c, mom, U, &h1));
+    RTRY(rc);
+    RTRY(hamiltonian_dd(c, mom, U, &h1));
     if (acc_tol <= 0.0) {
         // hfin = H + Re X^dag F with X the last MD solve  (:82)
         hdd re;
-        TRY(dot_dd_host(c, (const double *)X, (const double *)F, &re));
+        RTRY(dot_dd_host(c, (const double *)X, (const double *)F, &re));
         h1.add(re.hi, re.lo);
     } else {
         int ait = 0;
-        TRY(cg_any(c, (double *)X, U, (const double *)F, am0, 0.0, QED2D_OP_SQR_MSQ, maxiter,
+        RTRY(cg_any(c, (double *)X, U, (const double *)F, am0, 0.0, QED2D_OP_SQR_MSQ, maxiter,
                    acc_tol, 1, QED2D_PREC_F64, &ait));
         q64(c, U, (const double *)X, (double *)g5DX, am0);
         hdd re, yy;
-        TRY(dot_dd_host(c, (const double *)X, (const double *)F, &re));
-        TRY(dot_dd_host(c, (const double *)g5DX, nullptr, &yy));
+        RTRY(dot_dd_host(c, (const double *)X, (const double *)F, &re));
+        RTRY(dot_dd_host(c, (const double *)g5DX, nullptr, &yy));
         h1.add(2.0 * re.hi, 2.0 * re.lo);
         h1.add(-yy.hi, -yy.lo);
     }
     *hini = h0.value();
     *hfin = h1.value();
     c->last_dh = dd_diff(h1, h0);
-    TRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
+    RTRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
     if (!*accepted) launch_copy((double2 *)U, Ucp, n2, st);
     return finish(c);
 }
@@ -2310,7 +2321,7 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
     key_of(seed, chain, &k0, &k1);
     launch_fill_momenta(c->L, mom, k0, k1, traj, st);
     hdd h0, h1;
-    TRY(hamiltonian_dd(c, mom, U, &h0));
+    RTRY(hamiltonian_dd(c, mom, U, &h0));
     std::vector<double2 *> F(c->flav.begin(), c->flav.begin() + nf);
     // multi-shift mode 1: every flavour's heatbath and acceptance solves together
     bool together = (c->multishift == 1 && nf > 1);
@@ -2319,31 +2330,23 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
         // src/QED2dHMC.jl:136-144
         launch_fill_pseudofermion(c->L, X, k0, k1, traj, (uint32_t)j, st);
         hdd xx;
-        TRY(dot_dd_host(c, (const double *)X, nullptr, &xx));
+        RTRY(dot_dd_host(c, (const double *)X, nullptr, &xx));
         h0.add(xx.hi, xx.lo);
         if (together)
             launch_copy(F[j], X, n2, st);
         else
-            TRY(gen_pf_impl(c, (double *)F[j], U, (const double *)X, am0[j], npole[j], mu[j],
+            RTRY(gen_pf_impl(c, (double *)F[j], U, (const double *)X, am0[j], npole[j], mu[j],
                             nu[j], maxiter, tol));
     }
-    if (together) TRY(gen_pf_multi(c, F, U, am0, nf, npole, mu, nu, maxiter, tol));
-    int rc = md_rhmc(c, integrator, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu,
-                     rho, eps, ns, maxiter, tol);
-    if (rc != QED2D_OK) {
-        std::string msg = c->err;
-        launch_copy((double2 *)U, Ucp, n2, st);
-        (void)hipStreamSynchronize(st);
-        c->err = msg;
-        *accepted = 0;
-        return rc;
-    }
-    TRY(hamiltonian_dd(c, mom, U, &h1));
+    if (together) RTRY(gen_pf_multi(c, F, U, am0, nf, npole, mu, nu, maxiter, tol));
+    RTRY(md_rhmc(c, integrator, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu,
+                 rho, eps, ns, maxiter, tol));
+    RTRY(hamiltonian_dd(c, mom, U, &h1));
     if (together) {
         std::vector<const double *> bj(nf);
         for (int j = 0; j < nf; ++j) bj[j] = (const double *)F[j];
         int it = 0;
-        TRY(multishift_multi(c, U, nf, bj.data(), am0, npole, mu, maxiter, tol, &it));
+        RTRY(multishift_multi(c, U, nf, bj.data(), am0, npole, mu, maxiter, tol, &it));
     }
     for (int j = 0; j < nf; ++j) {
         // src/QED2dHMC.jl:150-156
@@ -2353,11 +2356,11 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
             for (int i = 0; i < npole[j]; ++i)
                 launch_scale_add(X, 1.0, c->mr.x[j][i], rho[j][i], n2, st);
         } else {
-            TRY(multicg_impl(c, (double *)X, U, (const double *)F[j], am0[j], npole[j], mu[j],
+            RTRY(multicg_impl(c, (double *)X, U, (const double *)F[j], am0[j], npole[j], mu[j],
                              rho[j], maxiter, tol));
         }
         hdd re;
-        TRY(dot_dd_host(c, (const double *)X, (const double *)F[j], &re));
+        RTRY(dot_dd_host(c, (const double *)X, (const double *)F[j], &re));
         h1.add(re.hi, re.lo);
     }
     *hini = h0.value();
@@ -2366,12 +2369,12 @@ int qed2d_hmc_rhmc(qed2d_ctx *c, double *U, const double *am0, int nf, const int
     if (reversibility_test) {
         // src/QED2dHMC.jl:159-164
         launch_negate(mom, 2 * c->L.V, st);
-        TRY(md_rhmc(c, integrator, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu,
+        RTRY(md_rhmc(c, integrator, mom, U, (double *)X, F, (double *)g5DX, am0, nf, npole, mu,
                     rho, eps, ns, maxiter, tol));
         *accepted = -1;
         return finish(c);
     }
-    TRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
+    RTRY(metropolis(c, *hini, *hfin, qzero, U, seed, chain, traj, accepted));
     if (!*accepted) launch_copy((double2 *)U, Ucp, n2, st);
     return finish(c);
 }

@@ -47,13 +47,18 @@ extern "C" int qed2d_read_links_text(double *U_host, long nlinks, const char *pa
     if (!U_host || nlinks < 1 || !path) return QED2D_EINVAL;
     FILE *f = std::fopen(path, "rb");
     if (!f) return QED2D_EINVAL;
-    std::fseek(f, 0, SEEK_END);
-    long size = std::ftell(f);
-    std::fseek(f, 0, SEEK_SET);
-    std::vector<char> text((size_t)size + 1);
-    size_t got = std::fread(text.data(), 1, (size_t)size, f);
+    // Read in chunks until EOF: no seek, so pipes and FIFOs work too.
+    std::vector<char> text;
+    size_t got = 0;
+    for (;;) {
+        if (text.size() - got < (1u << 16)) text.resize(text.size() + (1u << 20));
+        const size_t n = std::fread(text.data() + got, 1, text.size() - got, f);
+        got += n;
+        if (n == 0) break;
+    }
+    const int bad = std::ferror(f);
     std::fclose(f);
-    text[got] = '\0';
+    if (bad) return QED2D_EINVAL;
     const char *p = text.data(), *end = text.data() + got;
     long k = 0;
     while (p < end) {

@@ -482,8 +482,11 @@ void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t
 // U(1) links packed into one float each (the format link_dec reads): the
 // smaller-magnitude component, rounded to a float whose last mantissa bit is
 // 0 (towards the exact value), bit 30 set when it is Re, the last bit set when
-// the other component is negative.  Links further than 1e-6 from |U| = 1 set
-// S->bad_links (the format assumes unit modulus).
+// the other component is negative.  The format assumes unit modulus: the
+// larger component comes back as sqrt(1 - v^2), off by about 0.7 ||U|^2 - 1|.
+// Links with ||U|^2 - 1| > 2^-23 (FP32 epsilon) set S->bad_links, so a packed
+// link is never further from its FP64 value than FP32 rounding of its two
+// components (~1e-7); other links make the solver use FP32 complex links.
 __global__ void k_link_pack(const double2 *__restrict__ in, float *__restrict__ out, long n,
                             SolverState *S)
 {
@@ -498,7 +501,7 @@ __global__ void k_link_pack(const double2 *__restrict__ in, float *__restrict__ 
         if (sw) b |= 0x40000000u;
         if (o < 0.0) b |= 1u;
         out[i] = __uint_as_float(b);
-        if (fabs(u.x * u.x + u.y * u.y - 1.0) > 1e-6) S->bad_links = 1;
+        if (fabs(u.x * u.x + u.y * u.y - 1.0) > 0x1p-23) S->bad_links = 1;
     }
 }
 void launch_link_pack(const double2 *U, float *out, long n, SolverState *S, hipStream_t st)

@@ -49,3 +49,30 @@ def test_short_or_long_file_rejected(tmp_path):
         qed2d.read_links_text(f, 4, 3)
     with pytest.raises(qed2d.QED2dError):
         qed2d.read_links_text(f, 2, 3)
+
+
+def test_read_from_fifo(tmp_path):
+    """A path that cannot be seeked (a FIFO) reads like a regular file."""
+    import os
+    import threading
+
+    import qed2d
+    rng = np.random.default_rng(3)
+    U = random_links(6, 4, rng)
+    f = tmp_path / "cfg"
+    qed2d.write_links_text(U, f)
+    data = f.read_bytes()
+    fifo = tmp_path / "fifo"
+    os.mkfifo(fifo)
+
+    def writer():
+        with open(fifo, "wb") as w:
+            w.write(data)
+
+    th = threading.Thread(target=writer, daemon=True)
+    th.start()
+    try:
+        V = qed2d.read_links_text(fifo, 6, 4)
+    finally:
+        th.join(timeout=10)
+    assert np.array_equal(U.view(np.float64), V.view(np.float64))
