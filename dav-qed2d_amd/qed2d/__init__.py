@@ -78,6 +78,7 @@ SIGNATURES = {
     "qed2d_batch_observables": (_i, [_p, _p, _p, _p]),
     "qed2d_batch_last_dh": (_i, [_p, _p]),
     "qed2d_ctx_last_dh": (_i, [_p, _p]),
+    "qed2d_hamiltonian_noalloc": (_i, [_i, _i, _d, _p, _p, _p, _p]),
     "qed2d_ctx_autotune": (_i, [_p, _i, _p]),
     "qed2d_hbm_stream": (_i, [_p, C.c_long, _p]),
     "qed2d_ctx_set_reliable_delta": (_i, [_p, _d]),

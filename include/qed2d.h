@@ -177,6 +177,19 @@ int qed2d_plaquette(qed2d_ctx *ctx, const double *U, double *P);
 int qed2d_qtop(qed2d_ctx *ctx, const double *U, double *Q);
 /* Hamiltonian(mom, U, prm, kprm) (src/QED2dHMC.jl:198-203) */
 int qed2d_hamiltonian(qed2d_ctx *ctx, const double *mom, const double *U, double *H);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): Hamiltonian,
+ * Action, Plaquette and Qtop of one L1 x L2 field (as qed2d_hamiltonian,
+ * qed2d_action, qed2d_plaquette and qed2d_qtop, same bits) with no context and
+ * no allocation.  mom may be NULL (then H = S_g).  out is a DEVICE buffer of 4
+ * doubles: H, S_g, <P>, Q.  Asynchronous on `stream` (a hipStream_t, NULL =
+ * default stream).  The reduction's scratch is a fixed 48 KiB array per device
+ * (enough for any lattice); every call on a device shares it, so calls that
+ * overlap on one GPU race.  The first call on a device resolves the array's
+ * address; make it before a graph capture starts.  Lattices beyond 2^31 sites
+ * or below 2 x 2 return QED2D_EINVAL. */
+int qed2d_hamiltonian_noalloc(int L1, int L2, double beta, const double *mom, const double *U,
+                              double *out, void *stream);
 /* CUDA.dot(a, b) and CUDA.mapreduce(abs2, +, a) on spinors (src/QED2dSolver.jl:13,21) */
 int qed2d_dot(qed2d_ctx *ctx, const double *a, const double *b, double *re, double *im);
 int qed2d_norm2(qed2d_ctx *ctx, const double *a, double *n2);
