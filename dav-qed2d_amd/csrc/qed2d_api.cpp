@@ -29,6 +29,7 @@ enum SpinorSlot {
 };
 enum FloatSlot { FS_R, FS_P0, FS_P1, FS_T, FS_X, FS_U, FS_NSLOTS };
 enum RealSlot { R_MOM, R_FACC, R_MPP, R_NSLOTS };
+constexpr int kRing = 4;  // state copies of the chunks of a pipelined solve
 
 struct qed2d_ctx {
     Latt L;
@@ -45,6 +46,7 @@ struct qed2d_ctx {
     int rows_q = 2;   // FP64 Q / Dw applies
     int rows_q32 = 2; // FP32 Q apply of the public entry point
     double ru_delta = kRuDelta;  // mixed CG: reliable update after |r| drops by this factor
+    int ru_mode = 1;  // mixed CG: 1 reliable updates decided on the device, 0 by the host
     double last_dh = 0.0;    // dH of the last trajectory, from double-double energies
     std::string err;
     double2 *spin[S_NSLOTS] = {};
@@ -87,6 +89,10 @@ struct qed2d_ctx {
     SolverState *state = nullptr;
     RedOut *red = nullptr;
     SolverState *hstate = nullptr;  // pinned
+    // pipelined solves: pinned copies of the state at the end of queued chunks
+    SolverState *hring = nullptr;
+    hipEvent_t ring_ev[kRing] = {};
+    int ring_next = 0;
     RedOut *hred = nullptr;         // pinned
     // kernel-class timing (qed2d_profile_*)
     int prof = 0;  // 0 off, N: time every N-th launch of each class
@@ -417,7 +423,9 @@ struct SolveArgs {
     int chunk0 = 8;  // iterations queued before the first look at the state
 };
 
-int upload_state(qed2d_ctx *c, const SolveArgs &a)
+// soft: the mixed solver's on-device reliable updates (SolverState::soft); a
+// drop 1e-4 below the trigger (|r| by 1e-2 more) pauses the solve instead.
+int upload_state(qed2d_ctx *c, const SolveArgs &a, bool soft)
 {
     SolverState &s = *c->hstate;
     std::memset(&s, 0, sizeof s);
@@ -426,6 +434,8 @@ int upload_state(qed2d_ctx *c, const SolveArgs &a)
     s.maxiter = a.maxiter;
     s.guess = a.guess;
     s.delta2 = c->ru_delta * c->ru_delta;
+    s.soft = soft ? 1 : 0;
+    s.delta2h = s.delta2 * 1e-4;
     HIPCHK(c, hipMemcpyAsync(c->state, c->hstate, sizeof(SolverState), hipMemcpyHostToDevice,
                              c->stream));
     return QED2D_OK;
@@ -472,6 +482,93 @@ struct ChunkPredictor {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Pipelined solves.  A queued chunk of CG iterations ends with an asynchronous
+// copy of the solver state into a pinned slot of its own and an event; the
+// host queues the next chunk before it waits for the previous one, so the GPU
+// keeps iterating while the host looks at the state.  Chunks are sized from
+// the observed convergence rate to end where the solve is predicted to end; a
+// launch past the end of a solve returns at once.
+// ---------------------------------------------------------------------------
+struct ChunkQueue {
+    int slot[kRing];
+    bool last[kRing];
+    int k0[kRing];   // first iteration of the chunk
+    bool ru[kRing];  // the chunk starts with an on-device reliable update
+    int head = 0, n = 0;
+    int back() const { return (head + n - 1) % kRing; }
+    bool back_last() const { return n > 0 && last[back()]; }
+};
+
+int cq_push(qed2d_ctx *c, ChunkQueue &q, bool last, int k0 = 0, bool ru = false)
+{
+    const int s = c->ring_next;
+    c->ring_next = (s + 1) % kRing;
+    HIPCHK(c, hipMemcpyAsync(c->hring + s, c->state, sizeof(SolverState), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipEventRecord(c->ring_ev[s], c->stream));
+    const int i = (q.head + q.n) % kRing;
+    q.slot[i] = s;
+    q.last[i] = last;
+    q.k0[i] = k0;
+    q.ru[i] = ru;
+    q.n += 1;
+    return QED2D_OK;
+}
+
+// Waits for the oldest queued chunk; its state lands in *c->hstate.
+int cq_pop(qed2d_ctx *c, ChunkQueue &q)
+{
+    if (q.n < 1) return fail(c, QED2D_EHIP, "internal: no queued CG chunk to wait for");
+    const int s = q.slot[q.head];
+    q.head = (q.head + 1) % kRing;
+    q.n -= 1;
+    HIPCHK(c, hipEventSynchronize(c->ring_ev[s]));
+    *c->hstate = c->hring[s];
+    return QED2D_OK;
+}
+
+// What the observed states say about the rest of a solve.
+struct SolvePlan {
+    double rate = 0.0;  // d ln|r|^2 per iteration (< 0); 0: not known yet
+    double err = 0.0;
+    int iter = 0;
+    int nobs = 0;  // observations with a rate between two of them
+    void observe(const SolverState &h)
+    {
+        const double e = h.err;
+        const int it = h.iter;
+        if (!(e > 0.0) || it < 1 || it <= iter) return;
+        if (iter >= 1 && err > e) {
+            rate = std::log(e / err) / (double)(it - iter);
+            nobs += 1;
+        } else if (iter < 1 && h.r0norm > e) {
+            rate = std::log(e / h.r0norm) / (double)it;
+        }
+        iter = it;
+        err = e;
+    }
+    // The K1 predicted to see |r|^2 < target (0: not known).
+    int kconv(double target) const
+    {
+        if (!(rate < 0.0) || iter < 1) return 0;
+        if (err < target) return iter + 1;
+        const double left = std::log(target / err) / rate;
+        if (!(left < 1e6)) return 0;
+        return iter + 1 + (int)std::ceil(left);
+    }
+    // Iterations over which |r|^2 falls by the factor d (0: not known).
+    int span(double d) const
+    {
+        if (!(rate < 0.0)) return 0;
+        const double n = std::log(d) / rate;
+        return (n > 0.0 && n < 1e6) ? (int)std::ceil(n) : 0;
+    }
+};
+
+constexpr int kChunkMax = 64;    // iterations per queued chunk at most
+constexpr int kChunkBlind = 16;  // chunk length while the convergence rate is unknown
+
 int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const SolveArgs &a,
            int *iters)
 {
@@ -480,7 +577,7 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
     TRY(get_spin(c, S_P0, &P[0]));
     TRY(get_spin(c, S_P1, &P[1]));
     TRY(get_spin(c, S_T, &t));
-    TRY(upload_state(c, a));
+    TRY(upload_state(c, a, false));
     const Latt &L = c->L;
     hipStream_t st = c->stream;
     if (a.guess) {
@@ -490,11 +587,12 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
     } else {
         launch_cg_init_f64(L, (const double2 *)b, (double2 *)x, r, P[0], c->P, c->rows, st);
     }
+    const int kmax = a.maxiter + 1;  // K1(maxiter + 1) ends a solve that has not converged
     int k = 1;
-    int chunk = a.chunk0;
-    ChunkPredictor pred;
-    for (;;) {
-        for (int j = 0; j < chunk && k <= a.maxiter + 1; ++j, ++k) {
+    ChunkQueue q;
+    SolvePlan plan;
+    auto queue = [&](int k1, bool last) -> int {
+        for (; k <= k1 && k <= kmax; ++k) {
             PROF(c, QED2D_PROF_CG_K1_F64,
                  launch_cg_k1_f64(L, (const double2 *)U, r, P[0], P[1], t, a.am0, c->state, c->P,
                                   k, c->rows, st));
@@ -503,17 +601,47 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
                                   c->state, c->P, k, c->rows_k2 ? c->rows_k2 : c->rows, st));
         }
         TRY(launched(c));
-        TRY(read_state(c));
-        if (c->hstate->status != 0) break;
-        if (k > a.maxiter + 1) break;  // unreachable: K1(maxiter + 1) ends the solve
-        chunk = pred.next(c->hstate->err, c->hstate->iter, c->hstate->tol, 0.0);
+        return cq_push(c, q, last || k > kmax);
+    };
+    TRY(queue(a.chunk0, false));
+    // (a caller's guess may already satisfy the tolerance: no second chunk
+    // before the first look)
+    if (!a.guess) TRY(queue(k + kChunkBlind - 1, false));
+    for (;;) {
+        TRY(cq_pop(c, q));
+        const SolverState &h = *c->hstate;
+        if (h.status != 0) break;
+        plan.observe(h);
+        // keep two chunks queued unless the newer one is predicted to end the solve
+        while (k <= kmax && (q.n == 0 || (q.n < 2 && !q.back_last()))) {
+            const int kc = plan.kconv(h.tol);
+            int k1 = k + kChunkBlind - 1;
+            bool last = false;
+            if (kc > 0) {
+                k1 = std::max(kc + 1, k + 1);  // the K1 that sees it converged, and one more
+                last = k1 <= k + kChunkMax - 1;
+                if (!last) k1 = k + kChunkMax - 1;
+            }
+            TRY(queue(k1, last));
+        }
     }
+    q.n = 0;  // (launches still queued return at once)
     *iters = c->hstate->iter;
     if (c->hstate->status == 1) return QED2D_OK;
     return fail(c, QED2D_ENOCONV, "CG not converged after %d iterations (|r|^2 = %g, tol = %g)",
                 a.maxiter, c->hstate->err, c->hstate->tol);
 }
 
+// Mixed precision: FP32 iterations, FP64 reliable updates (the true residual
+// b - (Q^2 + mu^2) x in FP64 replaces the iterated one).  Where the FP32 K2's
+// partial-sum grid can take the residual kernel's (soft mode), a K1 whose
+// iterated |r| has dropped by delta only requests the update and the solve
+// goes on: the update's kernels are queued at the start of the chunks the
+// host plans there and run on the device iff a request is pending, so the
+// host does not wait for them.  A residual that looks converged, one far past
+// the trigger and the iteration limit pause the solve for an update from the
+// host, after which K1 tests the true residual (at the iteration the FP64
+// solver would stop at).
 int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const SolveArgs &a,
              int *iters)
 {
@@ -529,15 +657,27 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     const Latt &L = c->L;
     hipStream_t st = c->stream;
     const float am0f = (float)a.am0;
+    const int rows32_k2 = c->rows32_k2 ? c->rows32_k2 : c->rows32;
+    // soft mode: the on-device update's residual partials take the count K2
+    // leaves in the slot: folded to one (K2 grids past kPartsUnfolded), or
+    // padded, which needs a residual grid no larger than K2's
+    const int nk2 = cg2_f32_blocks(L, rows32_k2);
+    int rows_r = nk2 > kPartsUnfolded ? c->rows : 0;
+    for (int R : {16, 8, 4, 2, 1})
+        if (!rows_r && resid_blocks(L, R) <= nk2) rows_r = R;
+    const bool soft = c->ru_mode && rows_r > 0;
     // The site-pair kernels read the links packed (one float each) when they
     // are U(1); the first look at the state tells, and a solve whose links are
     // not restarts with FP32 complex links (x is untouched until then).
     bool packed = pair_ok(L);
+    const float2 *pend0 = pair_ok(L) ? P[0] : nullptr;  // K2 leaves x += alpha p to K1
+    const int kmax = a.maxiter + 1;
     int k = 1;
-    int chunk = a.chunk0;
-    ChunkPredictor pred;
+    int kb_last = 1;  // the last update known to have run, or planned (1: the start)
+    ChunkQueue q;
+    SolvePlan plan;
     auto start = [&]() -> int {
-        TRY(upload_state(c, a));
+        TRY(upload_state(c, a, soft));
         if (packed)
             launch_link_pack((const double2 *)U, (float *)Us, 2 * L.V, c->state, st);
         else
@@ -552,51 +692,138 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
                                  st);
         }
         k = 1;
-        chunk = a.chunk0;
-        pred = ChunkPredictor();
+        kb_last = 1;
+        q.n = 0;
+        plan = SolvePlan();
         return QED2D_OK;
     };
-    TRY(start());
-    bool first_look = true;
-    for (;;) {
-        for (int j = 0; j < chunk; ++j, ++k) {
+    // iterations k .. k1, after the on-device update when ru (soft mode)
+    auto queue = [&](int k1, bool ru, bool last) -> int {
+        const int k0 = k;
+        if (ru) {
+            hipEvent_t ev = prof_begin(c, QED2D_PROF_RELIABLE);
+            launch_ru_accum(L, (double2 *)x, xs, pend0, P[1], c->state, st, k);
+            launch_q_apply_ru(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->state,
+                              c->rows_q, st);
+            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
+                            nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, k, 3, rows_r,
+                            st);
+            prof_end(c, QED2D_PROF_RELIABLE, ev);
+        }
+        for (; k <= k1 && k <= kmax; ++k) {
             PROF(c, QED2D_PROF_CG_K1_F32,
                  launch_cg_k1_f32(L, Us, packed, rs, P[0], P[1], ts, xs, am0f, c->state, c->P, k,
                                   c->rows32, st));
             PROF(c, QED2D_PROF_CG_K2_F32,
                  launch_cg_k2_f32(L, Us, packed, ts, P[0], P[1], xs, rs, am0f, a.mu2 != 0.0,
-                                  c->state, c->P, k, c->rows32_k2 ? c->rows32_k2 : c->rows32,
-                                  st));
+                                  c->state, c->P, k, rows32_k2, st));
         }
         TRY(launched(c));
-        TRY(read_state(c));
+        return cq_push(c, q, last || k > kmax, k0, ru);
+    };
+    // The next chunk from k: its end, whether it starts with an update.  In soft
+    // mode an update is planned at the chunk start after the K1 predicted to
+    // request it, from the newest state the host has seen: right away when a
+    // request is pending; else where the residual is predicted to have fallen
+    // by delta^2 below its largest value since the last update, or, when the
+    // chunk in flight starts with an update that will run, span(delta^2)
+    // iterations after that.  Chunks last at most half a span, so an update
+    // that comes later than planned waits at most that long.  The residual
+    // that looks converged pauses the solve (the host's update then lets K1
+    // test the true residual at the iteration the FP64 solver would), so the
+    // chunk predicted to reach it ends there and is the last queued.
+    auto plan_next = [&](const SolverState &h, int *k1, bool *ru, bool *last) {
+        *ru = false;
+        *last = false;
+        // (the rate from the start alone overstates the asymptotic one)
+        const int nd = (soft && plan.nobs > 0) ? plan.span(h.delta2) : 0;
+        if (nd == 0) {
+            *k1 = k + kChunkBlind - 1;
+            // (no soft mode: up to the predicted pause, for the update or the end)
+            const int kp = soft ? 0 : plan.kconv(std::max(h.tol, h.delta2 *
+                                                              h.rmax2[(h.iter + 1) & 1]));
+            if (kp > 0) {
+                *k1 = std::max(kp + 1, k + 1);
+                if (*k1 > k + kChunkMax - 1) *k1 = k + kChunkMax - 1;
+            }
+            return;
+        }
+        // an update planned at kb_last that the state shows did not run
+        if (h.iter + 1 >= kb_last && h.true_at < kb_last) kb_last = h.true_at;
+        if (h.true_at > kb_last) kb_last = h.true_at;
+        // the K1 that requests the next update, seen from the state
+        int jt = h.ru_req ? h.iter + 1 : plan.kconv(h.delta2 * h.rmax2[(h.iter + 1) & 1]);
+        if (q.n > 0 && q.ru[q.back()] && q.k0[q.back()] > kb_last && jt > 0 &&
+            jt < q.k0[q.back()])
+            kb_last = q.k0[q.back()];  // the update in flight runs
+        if (kb_last > h.iter + 1 || jt <= 0) jt = kb_last + nd;
+        int due = jt + 2;
+        if (k >= due && k > 1) {
+            *ru = true;
+            kb_last = k;
+            due = k + nd + 2;
+        }
+        // the chunk that ends the solve: K1(kc) pauses it for the host
+        const int kc = plan.kconv(h.tol);
+        if (kc > 0 && kc < due) {
+            *k1 = std::max(kc, k);
+            *last = *k1 <= k + kChunkMax - 1;
+            if (!*last) *k1 = k + kChunkMax - 1;
+            return;
+        }
+        *k1 = std::min(due - 1, k + std::max(3, nd / 2));
+        // (a chunk short of four iterations would leave the host less time than
+        // it takes to look and queue the next one; an update then waits a few
+        // iterations)
+        if (*k1 < k + 3) *k1 = k + 3;
+        if (*k1 > k + kChunkMax - 1) *k1 = k + kChunkMax - 1;
+    };
+    TRY(start());
+    TRY(queue(a.chunk0, false, false));
+    bool first_look = true;
+    // (a caller's guess may already satisfy the tolerance, and its x is updated
+    // by the first reliable update: no second chunk before the first look)
+    if (!a.guess) TRY(queue(k + kChunkBlind - 1, false, false));
+    for (;;) {
+        TRY(cq_pop(c, q));
         if (first_look && packed && c->hstate->bad_links) {
             packed = false;
             TRY(start());
+            TRY(queue(a.chunk0, false, false));
             continue;
         }
         first_look = false;
-        if (c->hstate->status != 0) break;
-        if (c->hstate->need_ru) {
+        const SolverState &h = *c->hstate;
+        if (h.status != 0) break;
+        if (h.need_ru) {
             // K1(k) paused: fold the FP32 partial solution into x, recompute the
-            // true residual in FP64 and run K1(k) again on it.
-            k = c->hstate->iter + 1;
+            // true residual in FP64 and run K1(k) again on it.  (A chunk queued
+            // after the pause returns at once.)
+            q.n = 0;
+            k = h.stop_k;
             hipEvent_t ev = prof_begin(c, QED2D_PROF_RELIABLE);
             // (the site-pair kernels pause with K2(k-1)'s x update pending)
-            launch_ru_accum(L, (double2 *)x, xs, pair_ok(L) ? P[0] : nullptr, P[1],
-                            c->state, st);
+            launch_ru_accum(L, (double2 *)x, xs, pend0, P[1], c->state, st);
             launch_q_apply_f64(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->rows_q, st);
             launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
                             nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, k, 2, c->rows,
                             st);
             prof_end(c, QED2D_PROF_RELIABLE, ev);
-            const SolverState &h = *c->hstate;
-            chunk = pred.after_ru(h.err, h.tol, h.delta2);
+            kb_last = k;
+            plan.observe(h);
+            // K1(k) decides at once whether the true residual converged
+            TRY(queue(k, false, false));
             continue;
         }
-        const SolverState &h = *c->hstate;
-        chunk = pred.next(h.err, h.iter, h.tol, h.delta2 * h.rmax2[(h.iter + 1) & 1]);
+        plan.observe(h);
+        while (k <= kmax && (q.n == 0 || (q.n < 2 && !q.back_last()))) {
+            int k1;
+            bool ru, last;
+            plan_next(h, &k1, &ru, &last);
+            TRY(queue(k1, ru, last));
+        }
     }
+    q.n = 0;  // (launches still queued return at once)
     *iters = c->hstate->iter;
     if (c->hstate->status == 1) return QED2D_OK;
     return fail(c, QED2D_ENOCONV,
@@ -1517,7 +1744,11 @@ int qed2d_ctx_create(qed2d_ctx **out, int L1, int L2, double beta, int bc_flags,
               hipMalloc((void **)&c->red, sizeof(RedOut)) == hipSuccess &&
               hipHostMalloc((void **)&c->hstate, sizeof(SolverState), hipHostMallocDefault) ==
                   hipSuccess &&
+              hipHostMalloc((void **)&c->hring, kRing * sizeof(SolverState),
+                            hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void **)&c->hred, sizeof(RedOut), hipHostMallocDefault) == hipSuccess;
+    for (int i = 0; ok && i < kRing; ++i)
+        ok = hipEventCreateWithFlags(&c->ring_ev[i], hipEventDisableTiming) == hipSuccess;
     if (!ok) { qed2d_ctx_destroy(c); return QED2D_ENOMEM; }
     c->P.tt[0] = c->cgpart;
     c->P.tt[1] = c->cgpart + 2 * nbmax;
@@ -1558,6 +1789,8 @@ void qed2d_ctx_destroy(qed2d_ctx *c)
     if (c->state) (void)hipFree(c->state);
     if (c->red) (void)hipFree(c->red);
     if (c->hstate) (void)hipHostFree(c->hstate);
+    if (c->hring) (void)hipHostFree(c->hring);
+    for (auto e : c->ring_ev) if (e) (void)hipEventDestroy(e);
     if (c->hred) (void)hipHostFree(c->hred);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.a);
@@ -1647,6 +1880,15 @@ int qed2d_ctx_set_reliable_delta(qed2d_ctx *c, double delta)
     if (!(delta > 0.0 && delta < 1.0))
         return fail(c, QED2D_EINVAL, "reliable-update delta must be in (0, 1)");
     c->ru_delta = delta;
+    return QED2D_OK;
+}
+
+int qed2d_ctx_set_reliable_mode(qed2d_ctx *c, int on_device)
+{
+    if (!c) return QED2D_EINVAL;
+    if (on_device != 0 && on_device != 1)
+        return fail(c, QED2D_EINVAL, "reliable-update mode must be 0 or 1 (got %d)", on_device);
+    c->ru_mode = on_device;
     return QED2D_OK;
 }
 

@@ -25,8 +25,8 @@ template <typename T> __device__ __forceinline__ c2<T> from_d2(double2 v)
 // The solver state K1(k) reads, loaded in one go (so a kernel can issue the
 // loads early).
 struct K1In {
-    double bnorm, eps, tol, prev, rmax, delta2;
-    int true_at, guess, maxiter;
+    double bnorm, eps, tol, prev, rmax, delta2, delta2h;
+    int true_at, guess, maxiter, soft;
 };
 __device__ __forceinline__ K1In k1_in(const SolverState *S, int k)
 {
@@ -38,9 +38,11 @@ __device__ __forceinline__ K1In k1_in(const SolverState *S, int k)
     v.prev = S->norm[rd];
     v.rmax = S->rmax2[rd];
     v.delta2 = S->delta2;
+    v.delta2h = S->delta2h;
     v.true_at = S->true_at;
     v.guess = S->guess;
     v.maxiter = S->maxiter;
+    v.soft = S->soft;
     return v;
 }
 
@@ -70,7 +72,7 @@ __device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, dou
     double rmax = first ? rr : in.rmax;
     const bool truer = first || in.true_at == k;
     if (fresh) *fresh = truer;
-    int stat = 0, ru = 0;
+    int stat = 0, ru = 0, req = 0;
     if (first && bn == 0.0 && !in.guess) {
         stat = 1;  // b = 0: x = 0 exactly
     } else if (!mixed) {
@@ -83,8 +85,13 @@ __device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, dou
             if (k - 1 >= in.maxiter) stat = 2;
         }
     } else {
-        if (rr < tol || rr < in.delta2 * rmax || k - 1 >= in.maxiter) ru = 1;
-        else if (rr > rmax) rmax = rr;
+        if (rr < tol || k - 1 >= in.maxiter) ru = 1;
+        else if (rr < in.delta2 * rmax) {
+            // soft: keep iterating until a chunk start's reliable update, unless
+            // the residual has fallen far past the trigger
+            if (in.soft && !(rr < in.delta2h * rmax)) req = 1;
+            else ru = 1;
+        } else if (rr > rmax) rmax = rr;
     }
     if (blk == 0 && threadIdx.x == 0) {
         S->err = rr;
@@ -94,7 +101,10 @@ __device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, dou
         if (first) {
             S->bnorm = bn;
             S->tol = tol;
+            S->r0norm = rr;
         }
+        if (truer) S->ru_req = 0;
+        if (req) S->ru_req = 1;
         if (stat) {
             S->status = stat;
             S->stop_k = k;

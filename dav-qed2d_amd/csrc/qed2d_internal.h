@@ -81,6 +81,15 @@ struct SolverState {
     // reads p_k anyway; alpha_x is that pending alpha_k (written by K2's block 0)
     double alpha_x;
     int bad_links;  // set by launch_link_pack: the packed FP32 links are not usable
+    // On-device reliable updates (mixed precision, soft != 0, set by the host):
+    // a K1 whose iterated |r| has dropped by delta keeps iterating and sets
+    // ru_req; the reliable-update kernels queued at the next chunk boundary run
+    // iff ru_req && !stop_k, and the K1 after them (a fresh true residual)
+    // clears ru_req.  Only a drop below delta2h (or a residual that looks
+    // converged, or the iteration limit) pauses the solve for the host.
+    int soft, ru_req;
+    double delta2h;
+    double r0norm;  // |r_0|^2, committed by K1(1) (host information)
 };
 
 // Multi-shift CG state (all shifts sigma_i = mu_i^2 of (Q^2 + sigma_i) x_i = b on
@@ -257,7 +266,9 @@ void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *r
 // Residual from x with t = Q x already computed: r = b - (Q t + mu^2 x).
 // mode 0: FP64 start from a guess (writes r, zeroes p0, |b|^2 partials);
 // mode 1: mixed start from a guess (writes rs, zeroes p0s, |b|^2 partials);
-// mode 2: mixed reliable update before K1(k) (writes rs).
+// mode 2: mixed reliable update before K1(k) (writes rs);
+// mode 3: the on-device reliable update before K1(k) (writes rs while a soft
+// request is pending; keeps the slot's block count and pads up to it).
 // |r|^2 partials go to rr[(k - 1) & 1].
 void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const double2 *t,
                      const double2 *x, double2 *r, float2 *rs, double2 *p0, float2 *p0s,
@@ -285,8 +296,18 @@ void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t
 void launch_link_pack(const double2 *U, float *out, long n, SolverState *S, hipStream_t st);
 // A reliable update's x += double(xs), xs = 0; with p0 / p1 given (the FP32 site-pair solver, whose
 // K1 was paused with an x update pending), xs + alpha_x p_{stop_k - 1} first.
+// kb > 0: the on-device update queued before K1(kb), run only while a soft
+// request is pending (pending update from p_{kb - 1}).
 void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const float2 *p0, const float2 *p1,
-                     const SolverState *S, hipStream_t st);
+                     const SolverState *S, hipStream_t st, int kb = 0);
+// so = Q si while a soft reliable-update request is pending (SolverState::soft).
+void launch_q_apply_ru(const Latt &L, const double2 *U, const double2 *si, double2 *so,
+                       double am0, const SolverState *S, int rows, hipStream_t st);
+// Partials a consumer sums without a fold first (kPartMax of the device code).
+constexpr int kPartsUnfolded = 1024;
+// Block counts of the residual kernel and of the FP32 solver's K2 at a tile height.
+int resid_blocks(const Latt &L, int rows);
+int cg2_f32_blocks(const Latt &L, int rows);
 void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st);
 
 // ---- multi-shift CG --------------------------------------------------------

@@ -75,10 +75,17 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Latt L, const double2 *__res
 // same partials in the same order, so the bits do not depend on the folding.
 // ---------------------------------------------------------------------------
 constexpr int kFoldMin = kPartMax;  // consumers load at most this many partials
+static_assert(kPartsUnfolded == kPartMax, "host and device partial limits differ");
 
-__global__ void __launch_bounds__(kBlock) k_fold(double *part, int n, int stride)
+__device__ __forceinline__ bool soft_ru_pending(const SolverState *S);
+
+// pred: fold only while a soft reliable-update request is pending (the
+// on-device update's residual partials; null: always)
+__global__ void __launch_bounds__(kBlock) k_fold(double *part, int n, int stride,
+                                                 const SolverState *pred)
 {
     __shared__ double lds[kBlock / kWave + 1];
+    if (pred && !soft_ru_pending(pred)) return;
     const double v0 = sum_prev_partials(part, n, stride, 0, lds);
     const double v1 = (stride > 1) ? sum_prev_partials(part, n, stride, 1, lds) : 0.0;
     if (threadIdx.x == 0) {
@@ -91,7 +98,7 @@ __global__ void __launch_bounds__(kBlock) k_fold(double *part, int n, int stride
 static int fold(double *part, int n, int stride, hipStream_t st)
 {
     if (n <= kFoldMin) return n;
-    launch_k(k_fold, dim3(1), dim3(kBlock), 0, st, part, n, stride);
+    launch_k(k_fold, dim3(1), dim3(kBlock), 0, st, part, n, stride, (const SolverState *)nullptr);
     return 1;
 }
 
@@ -119,13 +126,24 @@ void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *r
 // Residual of an FP64 x: r = b - (Q t + mu^2 x), t = Q x  (the guess branch
 // of src/QED2dSolver.jl:65-72, and the reliable update of the mixed solver)
 // ===========================================================================
+// Mode 3 is the on-device reliable update queued at a chunk boundary: it runs
+// only while a soft request is pending on a running solve (ru_req && !stop_k,
+// which none of the update's kernels modify), and pads its |r|^2 partials with
+// zeros up to npad, the block count of the K2 whose slot it overwrites (adding
+// +0.0 leaves the consumer's sum bit for bit as over the real partials).
+__device__ __forceinline__ bool soft_ru_pending(const SolverState *S)
+{
+    return S->ru_req != 0 && S->stop_k == 0;
+}
+
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_cg_resid(Latt L, const double2 *__restrict__ U,
                                                      const double2 *__restrict__ t,
                                                      EpiResid e0, double mass2, SolverState *S,
                                                      double *Prr, double *Pb, int k, int mode,
-                                                     int bx, int by, int nbx)
+                                                     int npad, int bx, int by, int nbx)
 {
+    if (mode == 3 && !soft_ru_pending(S)) return;
     MarchPos p = march_pos(L, bx, by, nbx, R);
     EpiResid epi = e0;
     epi.mu2 = S->mu2;
@@ -139,13 +157,14 @@ __global__ void __launch_bounds__(kBlock) k_cg_resid(Latt L, const double2 *__re
     block_sum2_t0(epi.rr, epi.bb, a, bb);
     if (threadIdx.x == 0) {
         Prr[p.blk] = a;
-        if (mode != 2) Pb[p.blk] = bb;
+        if (mode < 2) Pb[p.blk] = bb;
     }
     if (p.blk == 0 && threadIdx.x == 0) {
+        for (int i = (int)gridDim.x; i < npad; ++i) Prr[i] = 0.0;
         S->true_at = k;  // K1(k) will see an FP64 true residual
+        if (mode >= 2) S->n_ru += 1;
         if (mode == 2) {
             S->need_ru = 0;
-            S->n_ru += 1;
             S->stop_k = 0;  // resume: K1(k) runs again
         }
     }
@@ -160,22 +179,65 @@ void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const do
     EpiResid e{b, x, r, rs, mode == 0 ? p0 : nullptr, mode == 1 ? p0s : nullptr, 0.0, L.V, 0.0,
                0.0};
     dim3 grid(g.nblocks()), blk(kBlock);
+    // mode 3 keeps the slot's count, that of the K2 before it: a folded count
+    // (1) gets the residual's partials folded the same way, an unfolded one
+    // (which the caller checked is at least this grid's) zero padding up to it
+    const int nslot = P.nrr[(k - 1) & 1];
+    const bool fold3 = (mode == 3 && nslot == 1 && g.nblocks() > 1);
+    const int npad = (mode == 3 && !fold3) ? nslot : 0;
     DISPATCH_ROWS(rows, k_cg_resid, grid, blk, 0, st, L, U, t, e, 2.0 + am0, s, P.rr[(k - 1) & 1],
-                  P.b, k, mode, g.bx, g.by, g.nbx);
+                  P.b, k, mode, npad, g.bx, g.by, g.nbx);
+    if (mode == 3) {
+        if (fold3)
+            launch_k(k_fold, dim3(1), dim3(kBlock), 0, st, P.rr[(k - 1) & 1], g.nblocks(), 1,
+                     (const SolverState *)s);
+        return;
+    }
     P.nrr[(k - 1) & 1] = fold(P.rr[(k - 1) & 1], g.nblocks(), 1, st);
     if (mode != 2) P.nb = fold(P.b, g.nblocks(), 1, st);
 }
 
-// x += double(xs); xs = 0 (with the pending xs += alpha_x p_{stop_k - 1} first
-// when P0 is given)
+// Predicated FP64 apply of the on-device reliable update: so = Q si while a
+// soft request is pending (see soft_ru_pending).
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_q_apply_ru(Latt L, const double2 *__restrict__ U,
+                                                       const double2 *__restrict__ si,
+                                                       double2 *__restrict__ so, double mass2,
+                                                       const SolverState *S, int bx, int by,
+                                                       int nbx)
+{
+    if (!soft_ru_pending(S)) return;
+    MarchPos p = march_pos(L, bx, by, nbx, R);
+    if (!p.active) return;
+    SrcPlain<double> src{si, L.V};
+    EpiStore<double> epi{so, L.V};
+    q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
+}
+void launch_q_apply_ru(const Latt &L, const double2 *U, const double2 *si, double2 *so,
+                       double am0, const SolverState *S, int rows, hipStream_t st)
+{
+    MarchGrid g = march_grid(L, rows);
+    DISPATCH_ROWS(rows, k_q_apply_ru, dim3(g.nblocks()), dim3(kBlock), 0, st, L, U, si, so,
+                  2.0 + am0, S, g.bx, g.by, g.nbx);
+}
+int resid_blocks(const Latt &L, int rows) { return march_grid(L, rows).nblocks(); }
+int cg2_f32_blocks(const Latt &L, int rows)
+{
+    return pair_ok(L) ? march_grid2(L, rows).nblocks() : march_grid(L, rows).nblocks();
+}
+
+// x += double(xs); xs = 0 (with the pending xs += alpha_x p_{k - 1} first when
+// P0 is given, k = stop_k; kb > 0: the on-device update before K1(kb), k = kb,
+// run only while a soft request is pending)
 __global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs,
                            const float2 *__restrict__ P0, const float2 *__restrict__ P1,
-                           const SolverState *S, long n)
+                           const SolverState *S, long n, int kb)
 {
+    if (kb > 0 && !soft_ru_pending(S)) return;
     const float2 *pp = nullptr;
     float a = 0.0f;
     if (P0) {
-        pp = ((S->stop_k - 1) & 1) ? P1 : P0;
+        pp = (((kb > 0 ? kb : S->stop_k) - 1) & 1) ? P1 : P0;
         a = (float)S->alpha_x;
     }
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
@@ -188,10 +250,10 @@ __global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs,
     }
 }
 void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const float2 *p0, const float2 *p1,
-                     const SolverState *S, hipStream_t st)
+                     const SolverState *S, hipStream_t st, int kb)
 {
     launch_k(k_ru_accum, dim3(ew_grid(2 * L.V)), dim3(kBlock), 0, st, x, xs, p0, p1, S,
-                       2 * L.V);
+                       2 * L.V, kb);
 }
 
 __global__ void k_to_f32(const double2 *__restrict__ in, float2 *__restrict__ out, long n)

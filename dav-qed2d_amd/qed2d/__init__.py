@@ -82,6 +82,7 @@ SIGNATURES = {
     "qed2d_ctx_autotune": (_i, [_p, _i, _p]),
     "qed2d_hbm_stream": (_i, [_p, C.c_long, _p]),
     "qed2d_ctx_set_reliable_delta": (_i, [_p, _d]),
+    "qed2d_ctx_set_reliable_mode": (_i, [_p, _i]),
     "qed2d_synchronize": (_i, [_p]),
     "qed2d_gamm5Dw": (_i, [_p, _p, _p, _p, _d]),
     "qed2d_gamm5Dw_f32": (_i, [_p, _p, _p, _p, _f]),
@@ -601,6 +602,14 @@ def multishift_CG(xs, U, si, am0, mus, maxiter, eps, prm):
     _check(prm, lib.qed2d_multishift_cg(prm.ctx(), ptrs, _sp(prm, U), _sp(prm, si), am0, n,
                                         _arr(mus), int(maxiter), eps, C.byref(it)))
     return it.value
+
+
+def set_reliable_mode(prm, on_device):
+    """Mixed-precision CG: reliable updates decided on the device (True, the
+    default; the host never waits for one) or by the host after a pause
+    (False: the batched solvers' schedule, which CG_mrhs and Batch reproduce
+    bit for bit)."""
+    _check(prm, lib.qed2d_ctx_set_reliable_mode(prm.ctx(), 1 if on_device else 0))
 
 
 def set_multishift(prm, on):

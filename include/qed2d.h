@@ -113,6 +113,17 @@ int qed2d_ctx_last_dh(const qed2d_ctx *ctx, double *dH);
 /* Mixed-precision CG: a reliable update (FP64 true residual) once the FP32
  * iterated residual has dropped by delta since the last one (default 1e-3). */
 int qed2d_ctx_set_reliable_delta(qed2d_ctx *ctx, double delta);
+/* Where the mixed-precision CG's reliable updates are decided.  1 (default):
+ * on the device -- an FP32 iteration whose residual has dropped by delta only
+ * requests an update, and the update's kernels, queued at the start of every
+ * chunk of iterations the host plans, run iff one is pending, so the host
+ * never waits for one; the host queues each chunk before it looks at the
+ * previous one.  0: the solve pauses at the request and the host queues the
+ * update after reading the state (the schedule of the batched solvers, so
+ * qed2d_cg_mrhs and qed2d_batch_* reproduce qed2d_cg bit for bit in this mode
+ * only).  Both stop on the same FP64 true residual.  No counterpart in the
+ * reference, whose CG is FP64 only (src/QED2dSolver.jl:3-105). */
+int qed2d_ctx_set_reliable_mode(qed2d_ctx *ctx, int on_device);
 int qed2d_synchronize(qed2d_ctx *ctx);
 
 /* ---- Dirac operator -------------------------------------------------------- */
@@ -127,8 +138,9 @@ int qed2d_gamm5Dw_mrhs(qed2d_ctx *ctx, double *so, const double *U, const double
  * in one launch per kernel; x and b are nrhs fields back to back ((k, 2, L2, L1)),
  * iters (may be NULL) receives the iterations per right-hand side.  Each
  * solution equals qed2d_cg on that right-hand side with the same tiles, bit
- * for bit.  No counterpart in the reference, which solves one right-hand side
- * per call (src/QED2dSolver.jl:3-45). */
+ * for bit (mixed precision: with qed2d_ctx_set_reliable_mode(ctx, 0)).  No
+ * counterpart in the reference, which solves one right-hand side per call
+ * (src/QED2dSolver.jl:3-45). */
 int qed2d_cg_mrhs(qed2d_ctx *ctx, double *x, const double *U, const double *b, int nrhs,
                   double am0, int maxiter, double eps, int precision, int *iters);
 int qed2d_gamm5Dw_mrhs_f32(qed2d_ctx *ctx, float *so, const float *U, const float *si, int nrhs,

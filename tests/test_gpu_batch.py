@@ -2,7 +2,8 @@
 single-lattice entry points on the same lattice, seeds and tiles.  The batch
 runs the same device code per chain with the same tiling, so the comparison
 is bit-exact (CG solutions, energies, accept/reject, links), FP64 and mixed
-precision.  The ensemble of 64 chains of uneven sizes is checked for mixed vs
+precision (the single lattice with host-decided reliable updates, the batch's
+schedule).  The ensemble of 64 chains of uneven sizes is checked for mixed vs
 FP64 agreement (accept/reject identical, |dH_mixed - dH_fp64| < 1e-9)."""
 import numpy as np
 import pytest
@@ -17,6 +18,7 @@ TILES = dict(q64=4, cg64=16, cg32=8)
 def _single(Q, L1, L2, beta, bc=0):
     prm = Q.LattParm((L1, L2), beta, bc)
     Q.set_tiles(prm, q64=TILES["q64"], cg64=TILES["cg64"], cg32=TILES["cg32"])
+    Q.set_reliable_mode(prm, False)
     return prm
 
 
@@ -127,6 +129,7 @@ def test_cg_mrhs_bit_exact(Q, O, L1, L2, prec):
     import torch
     say("cg_mrhs", L1, L2, "prec", prec)
     prm = Q.LattParm((L1, L2), 4.0)
+    Q.set_reliable_mode(prm, False)
     U = Q.spinor(prm)
     Q.hot_start(U, prm, 21)
     for k in (5, 3):
@@ -185,6 +188,7 @@ def test_batch_split_tiles_and_autotune_bit_exact(Q, O, prec):
         its = B.CG(x, src, am0s, 10000, 1e-20, precision=prec)
         for c, (L1, L2) in enumerate(sizes):
             prm = Q.LattParm((L1, L2), betas[c])
+            Q.set_reliable_mode(prm, False)
             Q.set_tiles(prm, q64=t["q64"], cg64=t["cg64"], cg32=t["cg32"],
                         cg64_k2=t["cg64_k2"], cg32_k2=t["cg32_k2"])
             xs = Q.spinor(prm)

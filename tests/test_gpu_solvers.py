@@ -121,6 +121,46 @@ def test_cg_split_tiles(Q, O, k1, k2):
         assert (t["cg64_k2"], t["cg32_k2"]) == (8, 8)
 
 
+@pytest.mark.parametrize("L1,L2,k2,mu,guess", [(64, 48, 0, 0.0, False), (33, 20, 0, 0.0, False),
+                                               (64, 48, 2, 0.7, True),
+                                               (2048, 512, 1, 0.0, False)])
+def test_cg_mixed_reliable_modes(Q, O, L1, L2, k2, mu, guess):
+    """Mixed-precision CG with reliable updates decided on the device (the
+    default: requested by an FP32 iteration, run by kernels queued at chunk
+    starts) and by the host after a pause: both reach the FP64 true residual,
+    agree with the FP64 solve at the level the tolerance allows, and take
+    about as many iterations.  Covers the site-pair kernels, the one-site FP32
+    march (odd L1), a shifted operator with a guess, and a K2 grid past 1024
+    blocks whose partial sums are folded (2048 x 512 at one row per thread)."""
+    say("mixed CG reliable modes", L1, L2, k2, mu, guess)
+    rng = np.random.default_rng(L1 + L2 + k2)
+    U = random_links(L1, L2, rng, unitary=True)
+    b = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 4.0)
+    if k2:
+        Q.set_tiles(prm, cg32=k2, cg32_k2=k2)
+    dU, db = dev(U), dev(b)
+    op = Q.gamm5Dw_sqr_musq if mu else Q.gamm5Dw_sqr_msq
+    x0 = Q.spinor(prm)
+    it0 = Q.CG(x0, dU, db, 0.3, mu, 20000, 1e-24, op, prm, None)
+    res = {}
+    for mode in (True, False):
+        Q.set_reliable_mode(prm, mode)
+        x = Q.spinor(prm)
+        if guess:  # a guess half way: x0 / 2
+            x.copy_(0.5 * x0)
+        it = Q.CG(x, dU, db, 0.3, mu, 20000, 1e-24, op, prm, None, so_as_guess=guess,
+                  precision=Q.PREC_MIXED)
+        res[mode] = (it, x)
+        if L1 * L2 <= 64 * 64:
+            assert _true_res(O, U, host(x), b, 0.3, mu) < 1e-24
+        assert relerr(host(x), host(x0)) < 1e-9, mode
+    Q.set_reliable_mode(prm, True)
+    say("iterations fp64", it0, "on device", res[True][0], "host", res[False][0])
+    assert abs(res[True][0] - res[False][0]) <= 4
+    assert res[True][0] <= 1.3 * it0 + 10
+
+
 def test_cg_not_converged_raises(Q):
     say("CG maxiter")
     L1, L2 = 24, 24
@@ -497,8 +537,14 @@ def test_mixed_cg_packed_links_and_fallback(Q, O, unitary):
     assert relerr(host(x1), host(x0)) < 1e-9
     say("iterations mixed", it1, "fp64", it0)
     assert it1 < 1.5 * it0 + 20
-    # the same solve through a right-hand-side set of one
+    # the same solve through a right-hand-side set of one, against the
+    # single-lattice solve on the set's schedule (host-decided updates)
+    Q.set_reliable_mode(prm, False)
+    x2 = Q.spinor(prm)
+    it2 = Q.CG(x2, dU, db, 0.3, 10000, 1e-22, Q.gamm5Dw_sqr_msq, prm, None, precision=1)
+    Q.set_reliable_mode(prm, True)
+    assert relerr(host(x2), host(x0)) < 1e-9
     xs = torch.zeros((1, 2, L2, L1), dtype=torch.complex128, device="cuda")
     its = Q.CG_mrhs(xs, dU, db.reshape(1, 2, L2, L1).contiguous(), 0.3, 10000, 1e-22, prm, None,
                     precision=1)
-    assert its[0] == it1 and torch.equal(xs[0], x1)
+    assert its[0] == it2 and torch.equal(xs[0], x2)
