@@ -107,6 +107,8 @@ struct qed2d_ctx {
 namespace qed2d {
 thread_local ProfArm g_prof_arm{nullptr, nullptr};
 }
+// why the last qed2d_ctx_create of this thread failed (qed2d_last_error(NULL))
+static thread_local std::string g_create_err = "null context";
 
 namespace {
 
@@ -1731,7 +1733,12 @@ int qed2d_ctx_create(qed2d_ctx **out, int L1, int L2, double beta, int bc_flags,
     c->L.bc = bc_flags;
     c->beta = beta;
     c->stream = (hipStream_t)stream;
-    if (hipGetDevice(&c->device) != hipSuccess) { delete c; return QED2D_EHIP; }
+    hipError_t he = hipGetDevice(&c->device);
+    if (he != hipSuccess) {
+        g_create_err = std::string("hipGetDevice: ") + hipGetErrorString(he);
+        delete c;
+        return QED2D_EHIP;
+    }
     // partial sums: the largest marching grid (rows = 1) needs 2 doubles per
     // block, the observables 6 per block of at most 1024.
     // generic reductions: at most 1024 blocks x 6 doubles; solver kernels: per
@@ -1749,16 +1756,26 @@ int qed2d_ctx_create(qed2d_ctx **out, int L1, int L2, double beta, int bc_flags,
               hipHostMalloc((void **)&c->hred, sizeof(RedOut), hipHostMallocDefault) == hipSuccess;
     for (int i = 0; ok && i < kRing; ++i)
         ok = hipEventCreateWithFlags(&c->ring_ev[i], hipEventDisableTiming) == hipSuccess;
-    if (!ok) { qed2d_ctx_destroy(c); return QED2D_ENOMEM; }
+    if (!ok) {
+        he = hipGetLastError();
+        g_create_err = std::string("allocation of the context's buffers failed: ") +
+                       hipGetErrorString(he);
+        qed2d_ctx_destroy(c);
+        return QED2D_ENOMEM;
+    }
     c->P.tt[0] = c->cgpart;
     c->P.tt[1] = c->cgpart + 2 * nbmax;
     c->P.rr[0] = c->cgpart + 4 * nbmax;
     c->P.rr[1] = c->cgpart + 5 * nbmax;
     c->P.b = c->cgpart + 6 * nbmax;
-    ok = hipMemsetAsync(c->state, 0, sizeof(SolverState), c->stream) == hipSuccess &&
-         hipMemsetAsync(c->red, 0, sizeof(RedOut), c->stream) == hipSuccess &&
-         hipStreamSynchronize(c->stream) == hipSuccess;
-    if (!ok) { qed2d_ctx_destroy(c); return QED2D_EHIP; }
+    he = hipMemsetAsync(c->state, 0, sizeof(SolverState), c->stream);
+    if (he == hipSuccess) he = hipMemsetAsync(c->red, 0, sizeof(RedOut), c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    if (he != hipSuccess) {
+        g_create_err = std::string("clearing the solver state failed: ") + hipGetErrorString(he);
+        qed2d_ctx_destroy(c);
+        return QED2D_EHIP;
+    }
     *out = c;
     return QED2D_OK;
 }
@@ -1800,7 +1817,10 @@ void qed2d_ctx_destroy(qed2d_ctx *c)
     delete c;
 }
 
-const char *qed2d_last_error(const qed2d_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *qed2d_last_error(const qed2d_ctx *c)
+{
+    return c ? c->err.c_str() : g_create_err.c_str();
+}
 
 int qed2d_ctx_set_beta(qed2d_ctx *c, double beta)
 {

@@ -35,6 +35,11 @@ if not os.path.exists(LIBPATH):
         f"libqed2d_hip.so not found at {LIBPATH}; build it with "
         "`make -C dav-qed2d_amd/csrc` (or __graft_entry__.build())")
 
+# torch first: the library then binds the HIP runtime torch has loaded, and
+# both see the same devices and streams (loaded the other way round, two HIP
+# runtimes end up in the process and the library's finds no device).
+import torch  # noqa: E402,F401
+
 lib = C.CDLL(LIBPATH)
 
 QED2D_OK, QED2D_EINVAL, QED2D_EHIP, QED2D_ENOCONV, QED2D_ENOMEM = 0, -1, -2, -3, -4
@@ -255,7 +260,8 @@ class LattParm:
             rc = lib.qed2d_ctx_create(C.byref(h), self.iL[0], self.iL[1], self.beta, self.bc,
                                       _p(stream))
             if rc != QED2D_OK:
-                raise QED2dError(rc, "qed2d_ctx_create failed")
+                raise QED2dError(rc, "qed2d_ctx_create failed: " +
+                                 lib.qed2d_last_error(None).decode())
             self._ctx = h
             self._ctx_beta = self.beta
             _live.add(self)

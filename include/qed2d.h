@@ -25,7 +25,8 @@
  * pure kernel launches asynchronous on the context's stream.
  *
  * Errors: every function returns QED2D_OK (0) or a negative status;
- * qed2d_last_error(ctx) gives the message.  The reference throws
+ * qed2d_last_error(ctx) gives the message (qed2d_last_error(NULL): why the
+ * last qed2d_ctx_create of the calling thread failed).  The reference throws
  * (src/QED2dSolver.jl:40-43,99-102; src/QED2dRHMC.jl:12-14).
  *
  * Aliasing: an output may not alias an input of the same call

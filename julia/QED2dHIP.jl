@@ -309,16 +309,16 @@ function generate_pseudofermion!(F, U, X, am0, CGmaxiter, CGtol, prm, kprm, rprm
 end
 
 # power_method(U, am0, prm, kprm; iter) (src/QED2dRHMC.jl:135-170).  The start
-# vector (CUDA.randn at :137) comes from the pseudofermion stream of the
-# current trajectory index; the results are returned as ComplexF64 like the
-# reference's dot-product ratios.
+# vector (CUDA.randn at :137) is the pseudofermion (flavour 15) of the next
+# trajectory index, as the Python mirror draws it; the results are returned as
+# ComplexF64 like the reference's dot-product ratios.
 function power_method(U, am0, prm, kprm; iter::Int64=1000)
     c = ctx(prm)
     b0 = similar(U)
-    r = RNG[]
+    r = next_traj!()
     chk(c, ccall((:qed2d_fill_pseudofermion, lib), Cint,
                  (Ptr{Cvoid}, Ptr{Cvoid}, UInt64, UInt32, UInt32, UInt32),
-                 c, dptr(b0), r.seed, r.chain, r.traj, UInt32(1000)))
+                 c, dptr(b0), r.seed, r.chain, r.traj, UInt32(15)))
     lmin, lmax = Ref{Cdouble}(0.0), Ref{Cdouble}(0.0)
     chk(c, ccall((:qed2d_power_method, lib), Cint,
                  (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Ptr{Cdouble}, Ptr{Cdouble}),
@@ -360,12 +360,12 @@ LuscherZ(Z, U, X_in, am0, CGmaxiter, CGtol, rprm::RHMCParm, prm::LattParm, kprm:
     LuscherZp(Z, 1, U, X_in, am0, CGmaxiter, CGtol, rprm, prm, kprm)
 
 # reweighting_factor(U, am0, CGmaxiter, CGtol, prm, kprm, rprm) (src/QED2dRHMC.jl:210-239);
-# the Gaussian fields (CUDA.randn at :218) come from the current trajectory's stream
+# the Gaussian fields (CUDA.randn at :218) come from the next trajectory index's stream
 function reweighting_factor(U, am0, CGmaxiter, CGtol, prm::LattParm, kprm::KernelParm,
                             rprm::RHMCParm)
     c = ctx(prm)
     W = Ref{Cdouble}(0.0)
-    r = RNG[]
+    r = next_traj!()
     chk(c, ccall((:qed2d_reweighting_factor, lib), Cint,
                  (Ptr{Cvoid}, Ptr{Cvoid}, Cdouble, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Cdouble,
                   Cdouble, Cint, Cint, Cint, Cdouble, UInt64, UInt32, UInt32, Ptr{Cdouble}),

@@ -548,3 +548,41 @@ def test_mixed_cg_packed_links_and_fallback(Q, O, unitary):
     its = Q.CG_mrhs(xs, dU, db.reshape(1, 2, L2, L1).contiguous(), 0.3, 10000, 1e-22, prm, None,
                     precision=1)
     assert its[0] == it2 and torch.equal(xs[0], x2)
+
+
+@pytest.mark.parametrize("kind", ["pure", "nf2"])
+def test_qzero_veto(Q, O, kind):
+    """HMC!'s Qzero option (src/QED2dHMC.jl:31-39,86-93): a trajectory ending
+    with |Q| > 0.1 is rejected whatever its dH and the links are restored; from
+    a cold start (Q = 0 throughout) the option changes nothing.  24x24, beta 4,
+    eps 0.02, ns 10 (OMF4 pure gauge, leapfrog Nf=2 at m0 0.3), against the
+    oracle on the same streams."""
+    L, seed = 24, 77
+    p = O.latt(L, L, 4.0)
+    prm = Q.LattParm((L, L), 4.0)
+    res = {}
+    for start in ("hot", "cold"):
+        U0 = O.hot_start(L, L, seed) if start == "hot" else np.ones((2, L, L), complex)
+        for qz in (False, True):
+            say("qzero", kind, start, qz)
+            U, dU = U0.copy(), dev(U0)
+            Q.seed(seed)
+            if kind == "pure":
+                info = Q.HMC(dU, 0.02, 10, [], prm, None, qzero=qz)
+                a = O.hmc_pure(U, 0.02, 10, p, seed, 0, qzero=int(qz))[2]
+            else:
+                info = Q.HMC(dU, 0.3, 0.02, 10, [], 10000, 1e-20, prm, None, qzero=qz)
+                _, _, a, _, rc = O.hmc_nf2(U, 0.3, 0.02, 10, 10000, 1e-20, p, seed, 0,
+                                           qzero=int(qz))
+                assert rc == 0
+            assert info.accepted == a, (start, qz)
+            assert relerr(host(dU), U) < 1e-9
+            res[(start, qz)] = (a, O.qtop(U))
+    q0 = O.qtop(O.hot_start(L, L, seed))
+    assert abs(q0) > 0.1
+    # the hot start is accepted without the option and vetoed with it
+    assert res[("hot", False)][0] == 1 and abs(res[("hot", False)][1]) > 0.1
+    assert res[("hot", True)][0] == 0 and abs(res[("hot", True)][1] - q0) < 1e-9
+    # Q = 0 from a cold start: the same decision either way
+    assert res[("cold", True)][0] == res[("cold", False)][0]
+    assert abs(res[("cold", True)][1]) < 1e-9

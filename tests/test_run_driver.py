@@ -137,3 +137,86 @@ def test_master_field_run(tmp_path, O):
     assert lines[0] == "save,plaquette,plaquette_err,top_charge" and len(lines) == 4
     m, _ = R.err_avg(last[0].T)
     assert abs(float(lines[-1].split(",")[1]) - m) < 1e-15
+
+
+def _rows(path):
+    return [ln.split(",") for ln in open(path).read().splitlines()[1:]]
+
+
+@pytest.mark.gpu
+def test_driver_rows_match_oracle(tmp_path, O):
+    """Each statistics row (main/hmc-nf2.jl:129-140, main/rhmc.jl:141-155,
+    main/MC.jl:120-135) against the oracle run on the same inputs and
+    streams: trajectory k of a run draws from (seed, chain 0, trajectory k);
+    the rhmc row's reweighting factor takes one trajectory index per fermion
+    and the power method's start vector the next (flavour 15).  Accept/reject
+    equal, plaquette and Q to 1e-10, the reweighting factor and lambda_max to
+    1e-8 relative (lambda_min: see below)."""
+    import numpy as np
+    from dense import q_dense
+    from qed2d import run as R
+    # hmc-nf2: cold start, leapfrog, CG tol 1e-20
+    rows = R.run("hmc-nf2", _write(tmp_path, "nf2.in", NF2_IN), str(tmp_path / "a"), 0,
+                 tol=1e-20)
+    o = R.read_options(_write(tmp_path, "nf2b.in", NF2_IN), "hmc-nf2")
+    L1, L2 = o["size"]
+    U = np.ones((2, L2, L1), complex)
+    p = O.latt(L1, L2, o["beta"])
+    nacc = 0
+    for t, row in enumerate(_rows(str(tmp_path / "a" / "statistics.txt"))):
+        _, _, a, _, rc = O.hmc_nf2(U, o["mass"], o["eps"], o["ns"], 10000, 1e-20, p, o["seed"], t)
+        assert rc == 0
+        nacc += a
+        assert int(row[0]) == t + 1 and int(row[1]) == a, (t, row)
+        assert abs(float(row[2]) - O.plaquette(U)) < 1e-10
+        assert abs(float(row[3]) - O.qtop(U)) < 1e-10
+        assert abs(float(row[4]) - nacc / (t + 1)) < 1e-15
+    assert len(rows) == o["ntraj"]
+    # mc: pure gauge OMF4 after nthm thermalisation trajectories
+    R.run("mc", _write(tmp_path, "m.in", MC_IN), str(tmp_path / "m"), 0)
+    o = R.read_options(_write(tmp_path, "mb.in", MC_IN), "mc")
+    L1, L2 = o["size"]
+    U = np.ones((2, L2, L1), complex)
+    p = O.latt(L1, L2, o["beta"])
+    for t in range(o["nthm"]):
+        O.hmc_pure(U, o["eps"], o["ns"], p, o["seed"], t)
+    for j, row in enumerate(_rows(str(tmp_path / "m" / "statistics.txt"))):
+        a = O.hmc_pure(U, o["eps"], o["ns"], p, o["seed"], o["nthm"] + j)[2]
+        assert int(row[1]) == a, (j, row)
+        assert abs(float(row[2]) - O.plaquette(U)) < 1e-10
+        assert abs(float(row[3]) - O.qtop(U)) < 1e-10
+    # rhmc: two fermions, leapfrog, multi-shift solves on the GPU against the
+    # oracle's sequential ones
+    R.run("rhmc", _write(tmp_path, "r.in", RHMC_IN), str(tmp_path / "r"), 0, tol=1e-20)
+    o = R.read_options(_write(tmp_path, "rb.in", RHMC_IN), "rhmc")
+    L1, L2 = o["size"]
+    U = np.ones((2, L2, L1), complex)
+    p = O.latt(L1, L2, o["beta"])
+    zs = [O.zolotarev(n, a, b) for n, a, b in zip(o["n_rhmc"], o["r_a"], o["r_b"])]
+    import qed2d as Q
+    rps = Q.get_rhmc_params(o["n_rhmc"], o["r_a"], o["r_b"])  # (reweighting N, Taylor order)
+    t = 0
+    for row in _rows(str(tmp_path / "r" / "statistics.txt")):
+        _, _, a, rc = O.hmc_rhmc(U, o["masses"], zs, o["eps"], o["ns"], 10000, 1e-20, p,
+                                 o["seed"], t)[:4]
+        t += 1
+        assert rc == 0 and int(row[1]) == a, row
+        assert abs(float(row[2]) - O.plaquette(U)) < 1e-10
+        assert abs(float(row[3]) - O.qtop(U)) < 1e-10
+        w = 1.0
+        for m, z, rp in zip(o["masses"], zs, rps):
+            w *= O.reweighting_factor(U, m, z, rp.reweighting_N, rp.reweighting_Taylor, 10000,
+                                      1e-20, o["seed"], 0, t)[0]
+            t += 1
+        assert abs(float(row[4]) - w) < 1e-8 * abs(w)
+        b0 = O.pseudofermion(L1, L2, o["seed"], 0, t, 15)
+        t += 1
+        lmin, lmax = O.power_method(U, b0, o["masses"][0], 5000)
+        assert abs(float(row[6]) - lmax) < 1e-8 * abs(lmax)
+        # lambda_min: its loop grows the low modes from the rounding noise the
+        # lambda_max loop leaves (device-specific), and 5000 steps leave it a
+        # Rayleigh quotient above the lowest eigenvalue: GPU and oracle to 1 %,
+        # neither below the dense spectrum's minimum
+        ev = np.linalg.eigvalsh(q_dense(U, o["masses"][0]) @ q_dense(U, o["masses"][0]))
+        assert abs(float(row[5]) - lmin) < 1e-2 * abs(lmin)
+        assert float(row[5]) > ev[0] * (1 - 1e-9) and lmin > ev[0] * (1 - 1e-9)
