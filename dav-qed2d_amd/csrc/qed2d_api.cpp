@@ -425,9 +425,7 @@ struct SolveArgs {
     int chunk0 = 8;  // iterations queued before the first look at the state
 };
 
-// soft: the mixed solver's on-device reliable updates (SolverState::soft); a
-// drop 1e-4 below the trigger (|r| by 1e-2 more) pauses the solve instead.
-int upload_state(qed2d_ctx *c, const SolveArgs &a, bool soft)
+int upload_state(qed2d_ctx *c, const SolveArgs &a)
 {
     SolverState &s = *c->hstate;
     std::memset(&s, 0, sizeof s);
@@ -436,8 +434,6 @@ int upload_state(qed2d_ctx *c, const SolveArgs &a, bool soft)
     s.maxiter = a.maxiter;
     s.guess = a.guess;
     s.delta2 = c->ru_delta * c->ru_delta;
-    s.soft = soft ? 1 : 0;
-    s.delta2h = s.delta2 * 1e-4;
     HIPCHK(c, hipMemcpyAsync(c->state, c->hstate, sizeof(SolverState), hipMemcpyHostToDevice,
                              c->stream));
     return QED2D_OK;
@@ -579,7 +575,7 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
     TRY(get_spin(c, S_P0, &P[0]));
     TRY(get_spin(c, S_P1, &P[1]));
     TRY(get_spin(c, S_T, &t));
-    TRY(upload_state(c, a, false));
+    TRY(upload_state(c, a));
     const Latt &L = c->L;
     hipStream_t st = c->stream;
     if (a.guess) {
@@ -679,7 +675,7 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     ChunkQueue q;
     SolvePlan plan;
     auto start = [&]() -> int {
-        TRY(upload_state(c, a, soft));
+        TRY(upload_state(c, a));
         if (packed)
             launch_link_pack((const double2 *)U, (float *)Us, 2 * L.V, c->state, st);
         else
@@ -715,7 +711,7 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
         for (; k <= k1 && k <= kmax; ++k) {
             PROF(c, QED2D_PROF_CG_K1_F32,
                  launch_cg_k1_f32(L, Us, packed, rs, P[0], P[1], ts, xs, am0f, c->state, c->P, k,
-                                  c->rows32, st));
+                                  c->rows32, st, soft));
             PROF(c, QED2D_PROF_CG_K2_F32,
                  launch_cg_k2_f32(L, Us, packed, ts, P[0], P[1], xs, rs, am0f, a.mu2 != 0.0,
                                   c->state, c->P, k, rows32_k2, st));

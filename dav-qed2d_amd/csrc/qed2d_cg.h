@@ -25,8 +25,8 @@ template <typename T> __device__ __forceinline__ c2<T> from_d2(double2 v)
 // The solver state K1(k) reads, loaded in one go (so a kernel can issue the
 // loads early).
 struct K1In {
-    double bnorm, eps, tol, prev, rmax, delta2, delta2h;
-    int true_at, guess, maxiter, soft;
+    double bnorm, eps, tol, prev, rmax, delta2;
+    int true_at, guess, maxiter;
 };
 __device__ __forceinline__ K1In k1_in(const SolverState *S, int k)
 {
@@ -38,11 +38,9 @@ __device__ __forceinline__ K1In k1_in(const SolverState *S, int k)
     v.prev = S->norm[rd];
     v.rmax = S->rmax2[rd];
     v.delta2 = S->delta2;
-    v.delta2h = S->delta2h;
     v.true_at = S->true_at;
     v.guess = S->guess;
     v.maxiter = S->maxiter;
-    v.soft = S->soft;
     return v;
 }
 
@@ -59,10 +57,13 @@ __device__ __forceinline__ const SolverState *state_v(const SolverState *S)
     return (const SolverState *)((const char *)S + z);
 }
 
+// soft: the mixed solver's on-device reliable updates (SolverState::ru_req; a
+// drop 1e-4 below the trigger pauses the solve instead).
 template <bool mixed>
 __device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, double rr,
                                              const double *Pb, int nbb, int k, int blk,
-                                             double *lds, double &beta, bool *fresh = nullptr)
+                                             double *lds, double &beta, bool *fresh = nullptr,
+                                             bool soft = false)
 {
     const int w = k & 1;
     const bool first = (k == 1);
@@ -89,7 +90,7 @@ __device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, dou
         else if (rr < in.delta2 * rmax) {
             // soft: keep iterating until a chunk start's reliable update, unless
             // the residual has fallen far past the trigger
-            if (in.soft && !(rr < in.delta2h * rmax)) req = 1;
+            if (soft && !(rr < 1e-4 * in.delta2 * rmax)) req = 1;
             else ru = 1;
         } else if (rr > rmax) rmax = rr;
     }
@@ -103,8 +104,7 @@ __device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, dou
             S->tol = tol;
             S->r0norm = rr;
         }
-        if (truer) S->ru_req = 0;
-        if (req) S->ru_req = 1;
+        if (soft) S->ru_req = req;  // (a fresh true residual clears it)
         if (stat) {
             S->status = stat;
             S->stop_k = k;

@@ -81,14 +81,13 @@ struct SolverState {
     // reads p_k anyway; alpha_x is that pending alpha_k (written by K2's block 0)
     double alpha_x;
     int bad_links;  // set by launch_link_pack: the packed FP32 links are not usable
-    // On-device reliable updates (mixed precision, soft != 0, set by the host):
+    // On-device reliable updates (the mixed solver's K1 launched in soft mode):
     // a K1 whose iterated |r| has dropped by delta keeps iterating and sets
-    // ru_req; the reliable-update kernels queued at the next chunk boundary run
-    // iff ru_req && !stop_k, and the K1 after them (a fresh true residual)
-    // clears ru_req.  Only a drop below delta2h (or a residual that looks
-    // converged, or the iteration limit) pauses the solve for the host.
-    int soft, ru_req;
-    double delta2h;
+    // ru_req; the reliable-update kernels queued at a chunk start run iff
+    // ru_req && !stop_k, and the K1 after them (a fresh true residual) clears
+    // ru_req.  A drop 1e-4 below the trigger, a residual that looks converged
+    // and the iteration limit pause the solve for the host instead.
+    int ru_req;
     double r0norm;  // |r_0|^2, committed by K1(1) (host information)
 };
 
@@ -283,9 +282,10 @@ void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const d
                       CGPartials &P, int k, int rows, hipStream_t st);
 // U: FP32 complex links, or (packed, site-pair kernels only: even L1 >= 4)
 // links packed by launch_link_pack.
+// (soft: on-device reliable updates, see SolverState::ru_req)
 void launch_cg_k1_f32(const Latt &L, const void *U, bool packed, const float2 *r, float2 *p0,
                       float2 *p1, float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P,
-                      int k, int rows, hipStream_t st);
+                      int k, int rows, hipStream_t st, bool soft = false);
 // (mu: the operator has mu2 != 0.  On even L1 >= 4 the site-pair kernels
 // leave x += alpha p to the next K1, see SolverState::alpha_x.)
 void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t,

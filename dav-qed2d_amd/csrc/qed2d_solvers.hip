@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(kBlock) k_cg1(Latt L, const c2<T> *__restrict_
                                                 c2<T> *P1, c2<T> *__restrict__ t, T mass2,
                                                 SolverState *S, double *Ptt, const double *Prr,
                                                 const double *Pb, int nrr, int nbb, int k, int bx,
-                                                int by, int nbx)
+                                                int by, int nbx, int soft)
 {
     __shared__ double lds[kBlock / kWave + 1];
     // The solver state's loads go out first, the partials' next, the first
@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(kBlock) k_cg1(Latt L, const c2<T> *__restrict_
         if (sk != 0 && sk < k) return false;
         double beta;
         if (!cg_k1_decide<std::is_same<T, float>::value>(S, in, prr.sum(), Pb, nbb, k, p.blk,
-                                                          lds, beta))
+                                                          lds, beta, nullptr, soft != 0))
             return false;
         src.beta = (T)beta;
         return true;
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
                                                  float2 *__restrict__ x, float mass2,
                                                  SolverState *S, double *Ptt, const double *Prr,
                                                  const double *Pb, int nrr, int nbb, int k, int bx,
-                                                 int by, int nbx)
+                                                 int by, int nbx, int soft)
 {
     __shared__ double lds[kBlock / kWave + 1];
     // The solver state's loads go out first, the partials' next, the first
@@ -381,7 +381,8 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
         if (sk != 0 && sk < k) return false;
         double beta;
         bool fresh;
-        if (!cg_k1_decide<true>(S, in, prr.sum(), Pb, nbb, k, p.blk, lds, beta, &fresh))
+        if (!cg_k1_decide<true>(S, in, prr.sum(), Pb, nbb, k, p.blk, lds, beta, &fresh,
+                                soft != 0))
             return false;
         src.beta = (float)beta;
         epi.alpha = ax;
@@ -449,7 +450,7 @@ CG2V_KERNEL(k_cg2vm, true)  // mu2 != 0
 template <typename T>
 static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c2<T> *p1,
                      c2<T> *t, c2<T> *x, T am0, SolverState *S, CGPartials &P, int k, int rows,
-                     hipStream_t st, int packed = 0)
+                     hipStream_t st, int packed = 0, int soft = 0)
 {
     const T m2 = T(2) + am0;
     double *ptt = P.tt[k & 1];
@@ -461,7 +462,7 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
             dim3 grid(g.nblocks()), blk(kBlock);
             const LinksF32 lk{U, packed};
             DISPATCH_ROWS_PK(rows, packed, k_cg1v, grid, blk, 0, st, L, lk, r, p0, p1, t, x, m2,
-                             S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx);
+                             S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft);
             P.ntt[k & 1] = fold(ptt, g.nblocks(), 2, st);
             return;
         }
@@ -469,11 +470,11 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
     MarchGrid g = march_grid(L, rows);
     dim3 grid(g.nblocks()), blk(kBlock);
     switch (rows) {
-    case 1: launch_k((k_cg1<T, 1>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
-    case 2: launch_k((k_cg1<T, 2>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
-    case 4: launch_k((k_cg1<T, 4>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
-    case 8: launch_k((k_cg1<T, 8>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
-    default: launch_k((k_cg1<T, 16>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx); break;
+    case 1: launch_k((k_cg1<T, 1>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft); break;
+    case 2: launch_k((k_cg1<T, 2>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft); break;
+    case 4: launch_k((k_cg1<T, 4>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft); break;
+    case 8: launch_k((k_cg1<T, 8>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft); break;
+    default: launch_k((k_cg1<T, 16>), grid, blk, 0, st, L, U, r, p0, p1, t, m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft); break;
     }
     P.ntt[k & 1] = fold(ptt, g.nblocks(), 2, st);
 }
@@ -530,9 +531,10 @@ void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const d
 }
 void launch_cg_k1_f32(const Latt &L, const void *U, bool packed, const float2 *r, float2 *p0,
                       float2 *p1, float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P,
-                      int k, int rows, hipStream_t st)
+                      int k, int rows, hipStream_t st, bool soft)
 {
-    cg1_impl<float>(L, (const float2 *)U, r, p0, p1, t, x, am0, s, P, k, rows, st, packed);
+    cg1_impl<float>(L, (const float2 *)U, r, p0, p1, t, x, am0, s, P, k, rows, st, packed,
+                    soft ? 1 : 0);
 }
 void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t,
                       const float2 *p0, const float2 *p1, float2 *x, float2 *r, float am0,

@@ -10,11 +10,18 @@
  * on the thread count.
  */
 #include "qed2d_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include <complex.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Loops shorter than this run on one thread: a parallel region costs more
+ * than it saves on a small lattice (24x24 ran 13x slower on 16 threads). */
+#define OMP_MIN 8192
 
 typedef double _Complex cplx;
 
@@ -151,7 +158,7 @@ void orc_fill_momenta(double *mom, int L1, int L2, uint64_t seed, uint32_t chain
     long V = (long)L1 * L2;
     uint32_t key[2];
     make_key(seed, chain, key);
-#pragma omp parallel for
+#pragma omp parallel for if (V >= OMP_MIN)
     for (long s = 0; s < V; ++s) {
         uint32_t ctr[4] = {(uint32_t)s, 0u, PURPOSE_MOM, traj};
         orc_gauss_pair(ctr, key, &mom[s], &mom[V + s]);
@@ -167,7 +174,7 @@ void orc_fill_cgauss(double *X, int L1, int L2, uint64_t seed, uint32_t chain, u
     uint32_t key[2];
     make_key(seed, chain, key);
     const double sq2 = sqrt(2.0);
-#pragma omp parallel for
+#pragma omp parallel for if (V >= OMP_MIN)
     for (long s = 0; s < V; ++s) {
         for (uint32_t c = 0; c < 2; ++c) {
             uint32_t ctr[4] = {(uint32_t)s, sub0 + c, purpose, traj};
@@ -201,7 +208,7 @@ void orc_hot_start(double *U, int L1, int L2, uint64_t seed, uint32_t chain)
     long V = (long)L1 * L2;
     uint32_t key[2];
     make_key(seed, chain, key);
-#pragma omp parallel for
+#pragma omp parallel for if (V >= OMP_MIN)
     for (long s = 0; s < V; ++s) {
         for (uint32_t d = 0; d < 2; ++d) {
             uint32_t ctr[4] = {(uint32_t)s, d, PURPOSE_HOT, 0u};
@@ -233,7 +240,7 @@ typedef double (*term_fn)(const void *ctx, long i);
 static double reduce_terms(term_fn f, const void *ctx, long n)
 {
     double part[NCHUNK], cpart[NCHUNK];
-#pragma omp parallel for
+#pragma omp parallel for if (n >= OMP_MIN)
     for (int ch = 0; ch < NCHUNK; ++ch) {
         long lo = n * ch / NCHUNK, hi = n * (ch + 1) / NCHUNK;
         double s = 0.0, c = 0.0;
@@ -255,7 +262,7 @@ static double reduce_terms(term_fn f, const void *ctx, long n)
 static long double reduce_terms_ld(term_fn f, const void *ctx, long n)
 {
     double part[NCHUNK], cpart[NCHUNK];
-#pragma omp parallel for
+#pragma omp parallel for if (n >= OMP_MIN)
     for (int ch = 0; ch < NCHUNK; ++ch) {
         long lo = n * ch / NCHUNK, hi = n * (ch + 1) / NCHUNK;
         double s = 0.0, c = 0.0;
@@ -329,7 +336,7 @@ void orc_gamm5Dw(double *so_, const double *U_, const double *si_, double am0,
 {
     cplx *so = (cplx *)so_;
     const cplx *U = (const cplx *)U_, *si = (const cplx *)si_;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if ((long)L1 * L2 >= OMP_MIN)
     for (int i2 = 0; i2 < L2; ++i2) {
         for (int i1 = 0; i1 < L1; ++i1) {
             int iu1 = (i1 + 1) % L1, iu2 = (i2 + 1) % L2;
@@ -450,7 +457,7 @@ void orc_tr_dQwdU(double *frc, const double *U_, const double *X_, const double 
     const cplx *U = (const cplx *)U_, *X = (const cplx *)X_, *Y = (const cplx *)Y_;
     long V = (long)L1 * L2;
     const cplx Im1 = CMPLX(0.0, 1.0);
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (V >= OMP_MIN)
     for (int i2 = 0; i2 < L2; ++i2) {
         for (int i1 = 0; i1 < L1; ++i1) {
             int iu1 = (i1 + 1) % L1, iu2 = (i2 + 1) % L2;
@@ -485,7 +492,7 @@ void orc_krnl_force(double *frc1, double *frc2, const double *U_, double beta, i
 {
     const cplx *U = (const cplx *)U_;
     long V = (long)L1 * L2;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if (V >= OMP_MIN)
     for (int i2 = 0; i2 < L2; ++i2) {
         for (int i1 = 0; i1 < L1; ++i1) {
             int iu1 = (i1 + 1) % L1, iu2 = (i2 + 1) % L2;
@@ -503,7 +510,7 @@ void orc_updt_fld(double *U_, const double *mom, double eps, int L1, int L2)
 {
     cplx *U = (cplx *)U_;
     long n = 2L * L1 * L2;
-#pragma omp parallel for
+#pragma omp parallel for if (n >= OMP_MIN)
     for (long i = 0; i < n; ++i) U[i] = CMPLX(cos(eps * mom[i]), sin(eps * mom[i])) * U[i];
 }
 
@@ -511,7 +518,7 @@ void orc_updt_fld(double *U_, const double *mom, double eps, int L1, int L2)
 void orc_plaquette_field(double *plx, const double *U_, int L1, int L2)
 {
     const cplx *U = (const cplx *)U_;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if ((long)L1 * L2 >= OMP_MIN)
     for (int i2 = 0; i2 < L2; ++i2)
         for (int i1 = 0; i1 < L1; ++i1) plx[(long)i2 * L1 + i1] = creal(plaq(U, i1, i2, L1, L2));
 }
@@ -520,7 +527,7 @@ void orc_plaquette_field(double *plx, const double *U_, int L1, int L2)
 void orc_qtop_field(double *plx, const double *U_, int L1, int L2)
 {
     const cplx *U = (const cplx *)U_;
-#pragma omp parallel for collapse(2)
+#pragma omp parallel for collapse(2) if ((long)L1 * L2 >= OMP_MIN)
     for (int i2 = 0; i2 < L2; ++i2)
         for (int i1 = 0; i1 < L1; ++i1) plx[(long)i2 * L1 + i1] = carg(plaq(U, i1, i2, L1, L2));
 }
@@ -611,7 +618,7 @@ int orc_cg(double *so_, const double *U, const double *si_, double am0, double m
         apply_op(op, (double *)Ap, (double *)tmp, U, (double *)p, am0, mu, L1, L2, bc);
         cplx prod = dotc(p, Ap, n);
         cplx alpha = norm / prod;
-#pragma omp parallel for
+#pragma omp parallel for if (n >= OMP_MIN)
         for (long i = 0; i < n; ++i) {
             so[i] = so[i] + alpha * p[i];
             r[i] = r[i] - alpha * Ap[i];
@@ -619,7 +626,7 @@ int orc_cg(double *so_, const double *U, const double *si_, double am0, double m
         err = orc_norm2((double *)r, n);
         if (err < tol) { iterations = it; break; }
         double beta = err / norm;
-#pragma omp parallel for
+#pragma omp parallel for if (n >= OMP_MIN)
         for (long i = 0; i < n; ++i) p[i] = r[i] + beta * p[i];
         norm = err;
     }
@@ -852,7 +859,7 @@ int orc_update_momenta_nf2(double *mom, const double *U, double *X, const double
     orc_gamm5Dw(g5DX, U, X, am0, p->L1, p->L2, p->bc);
     orc_tr_dQwdU(frc, U, X, g5DX, p->L1, p->L2, p->bc);
     orc_krnl_force(frc1, frc2, U, p->beta, p->L1, p->L2);
-#pragma omp parallel for
+#pragma omp parallel for if (V >= OMP_MIN)
     for (long i = 0; i < 2 * V; ++i) mom[i] = mom[i] + eps * (frc1[i] + frc2[i] + frc[i]);
     free(frc1); free(frc2); free(frc);
     return st ? -1 : it;
@@ -899,7 +906,7 @@ static void pure_momenta(double *mom, const double *U, double *frc1, double *frc
 {
     long V = (long)p->L1 * p->L2;
     orc_krnl_force(frc1, frc2, U, p->beta, p->L1, p->L2);
-#pragma omp parallel for
+#pragma omp parallel for if (V >= OMP_MIN)
     for (long i = 0; i < 2 * V; ++i) mom[i] = mom[i] + coef * (frc1[i] + frc2[i]);
 }
 
@@ -1260,4 +1267,13 @@ void orc_unfold(double *Uout_, const double *Uin_, int L1, int L2, int reflect)
                 Uout[IDX(i1, i2 + L2, d, M1, M2)] = uc;
                 Uout[IDX(i1 + L1, i2 + L2, d, M1, M2)] = u;
             }
+}
+
+int orc_omp_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }

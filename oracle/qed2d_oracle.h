@@ -149,6 +149,9 @@ int orc_hmc_nf2(double *U, double am0, double eps, int ns, int integrator, int m
                 const orc_latt *p, double *hini, double *hfin, int *accepted, int *cg_iters);
 /* dH of the last orc_hmc_nf2 trajectory from extended-precision energies. */
 double orc_last_dh(void);
+/* OpenMP threads a parallel region of the oracle uses (the CPU baseline's
+ * thread count). */
+int orc_omp_threads(void);
 int orc_hmc_rhmc(double *U, const double *am0, int nf, const int *npole, double *const *mu,
                  double *const *nu, double *const *rho, double eps, int ns, int integrator,
                  int maxiter, double tol, int qzero, int reversibility_test, uint64_t seed,

@@ -69,6 +69,7 @@ _sig("orc_plaquette", _d, [_p, _i, _i])
 _sig("orc_qtop", _d, [_p, _i, _i])
 _sig("orc_hamiltonian", _d, [_p, _p, _d, _i, _i])
 _sig("orc_last_dh", _d, [])
+_sig("orc_omp_threads", _i, [])
 _sig("orc_dot", None, [_p, _p, C.c_long, _pd, _pd])
 _sig("orc_norm2", _d, [_p, C.c_long])
 _sig("orc_cg", _i, [_p, _p, _p, _d, _d, _i, _i, _d, _i, _i, _i, _i, C.POINTER(_i)])
@@ -379,6 +380,11 @@ def hmc_pure(U, eps, ns, p, seed, traj, chain=0, integrator=1, qzero=0):
     lib.orc_hmc_pure(_ptr(U), eps, ns, integrator, qzero, seed, chain, traj, C.byref(p),
                      C.byref(hi), C.byref(hf), C.byref(acc))
     return hi.value, hf.value, acc.value
+
+
+def omp_threads():
+    """OpenMP threads of the oracle's parallel regions."""
+    return lib.orc_omp_threads()
 
 
 def last_dh():
