@@ -19,23 +19,15 @@ __device__ __forceinline__ double2 plaq_at(const double2 *__restrict__ U, const 
     return cmul<double>(a, cconj<double>(b));
 }
 
-// Fermion force z_mu at site x, returns Re(i z1), Re(i z2)  (src/QED2dDirac.jl:172-199)
-__device__ __forceinline__ void ferm_force(const double2 *__restrict__ U,
-                                           const double2 *__restrict__ X,
-                                           const double2 *__restrict__ Y, const Latt &L, int i1,
-                                           int i2, double &f1, double &f2)
+// Fermion force z_mu at site x from the values it needs: the links U1, U2 of
+// x (U2 with the fermion boundary sign already applied), X and Y at x, at
+// x + 1 (a) and at x + 2 (b).  Returns Re(i z1), Re(i z2)
+// (src/QED2dDirac.jl:172-199).
+__device__ __forceinline__ void ferm_force_v(double2 U1, double2 U2, double2 X1, double2 X2,
+                                             double2 Y1, double2 Y2, double2 Xa1, double2 Xa2,
+                                             double2 Ya1, double2 Ya2, double2 Xb1, double2 Xb2,
+                                             double2 Yb1, double2 Yb2, double &f1, double &f2)
 {
-    const long V = L.V;
-    const int iu1 = (i1 + 1 == L.L1) ? 0 : i1 + 1;
-    const int iu2 = (i2 + 1 == L.L2) ? 0 : i2 + 1;
-    const long s = (long)i2 * L.L1 + i1;
-    const long s1 = (long)i2 * L.L1 + iu1;
-    const long s2 = (long)iu2 * L.L1 + i1;
-    double2 U1 = U[s], U2 = U[V + s];
-    if ((L.bc & 1) && i2 == L.L2 - 1) U2 = cneg<double>(U2);
-    const double2 X1 = X[s], X2 = X[V + s], Y1 = Y[s], Y2 = Y[V + s];
-    const double2 Xa1 = X[s1], Xa2 = X[V + s1], Ya1 = Y[s1], Ya2 = Y[V + s1];
-    const double2 Xb1 = X[s2], Xb2 = X[V + s2], Yb1 = Y[s2], Yb2 = Y[V + s2];
     // z1 = U1* (X1(x+1)* (Y1+Y2) - X2(x+1)* (Y1+Y2)) - U1 (X1* + X2*)(Y1-Y2)(x+1)
     const double2 ys = cadd<double>(Y1, Y2);
     const double2 yd = csub<double>(Ya1, Ya2);
@@ -52,6 +44,24 @@ __device__ __forceinline__ void ferm_force(const double2 *__restrict__ U,
     // Re(i z) = -Im z
     f1 = -z1.y;
     f2 = -z2.y;
+}
+
+// The same at site (i1, i2), loading what it needs.
+__device__ __forceinline__ void ferm_force(const double2 *__restrict__ U,
+                                           const double2 *__restrict__ X,
+                                           const double2 *__restrict__ Y, const Latt &L, int i1,
+                                           int i2, double &f1, double &f2)
+{
+    const long V = L.V;
+    const int iu1 = (i1 + 1 == L.L1) ? 0 : i1 + 1;
+    const int iu2 = (i2 + 1 == L.L2) ? 0 : i2 + 1;
+    const long s = (long)i2 * L.L1 + i1;
+    const long s1 = (long)i2 * L.L1 + iu1;
+    const long s2 = (long)iu2 * L.L1 + i1;
+    double2 U2 = U[V + s];
+    if ((L.bc & 1) && i2 == L.L2 - 1) U2 = cneg<double>(U2);
+    ferm_force_v(U[s], U2, X[s], X[V + s], Y[s], Y[V + s], X[s1], X[V + s1], Y[s1], Y[V + s1],
+                 X[s2], X[V + s2], Y[s2], Y[V + s2], f1, f2);
 }
 
 // Gather form of the gauge force + fermion force + momentum update, one pass:

@@ -571,16 +571,6 @@ void launch_krnl_force(const Latt &L, const double2 *U, double beta, double *frc
                        frc2);
 }
 
-// Gather form of the gauge force + fermion force + momentum update, one pass
-// (force_mom_site).
-__global__ void k_force_mom(Latt L, const double2 *__restrict__ U, const double2 *__restrict__ X,
-                            const double2 *__restrict__ Y, const double *__restrict__ facc,
-                            double *__restrict__ mom, double beta, double eps, double cf)
-{
-    for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < L.V;
-         s += (long)gridDim.x * blockDim.x)
-        force_mom_site(L, U, X, Y, facc, mom, beta, eps, cf, s);
-}
 // P = U1(x) U2(x+1) conj(U1(x+2) U2(x)) from the four links, the expression
 // of plaq_at.
 __device__ __forceinline__ double2 plaq4(double2 u1, double2 u2r, double2 u1n, double2 u2)
@@ -657,6 +647,100 @@ __global__ void __launch_bounds__(kBlock) k_gauge_mom(Latt L, const double2 *__r
         vprev = v;
         u1 = u1n;
         u2 = u2n;
+    }
+}
+
+// The Nf=2 force evaluation's gauge + fermion force and momentum update
+// (force_mom_site), marching rows as k_gauge_mom, with the fermion fields of the
+// thread's column in registers too: X and Y of row r + 1 (x + 2) are loaded
+// one row ahead and become the next row's own, those of x + 1 come from the
+// adjacent lane (loaded at the wave's right edge and the periodic wrap).  Each
+// link and each X, Y element is loaded once, plus one halo row per tile: the
+// algorithmic 128 B/site.  Same expressions, same bits as force_mom_site.
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_force_mom(Latt L, const double2 *__restrict__ U,
+                                                        const double2 *__restrict__ X,
+                                                        const double2 *__restrict__ Y,
+                                                        const double *__restrict__ facc,
+                                                        double *__restrict__ mom, double beta,
+                                                        double eps, double cf, int bx, int by,
+                                                        int nbx)
+{
+    const MarchPos p = march_pos(L, bx, by, nbx, R);
+    if (!p.active) return;
+    const int L1 = L.L1, L2 = L.L2;
+    const long V = L.V;
+    const int i1 = p.i1, r0 = p.r0;
+    const int i1p = (i1 + 1 == L1) ? 0 : i1 + 1;
+    const int i1m = (i1 == 0) ? L1 - 1 : i1 - 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool ldR = (lane == kWave - 1) || (i1p == 0);  // x + 1 not in lane+1
+    const bool ldL = (lane == 0) || (i1 == 0);           // v(x-1) not in lane-1
+    const bool apbc = (L.bc & 1) != 0;
+    const int rm = (r0 == 0) ? L2 - 1 : r0 - 1;
+    const int nr = (r0 + R <= L2) ? R : L2 - r0;
+    const long sm = (long)rm * L1, s0 = (long)r0 * L1;
+    // the row below the tile: its plaquette is P(x-2) of row r0
+    double2 u1 = U[sm + i1], u2 = U[V + sm + i1];
+    double2 u1n = U[s0 + i1];
+    double2 u2r = from_right<double>(u2);
+    if (ldR) u2r = U[V + sm + i1p];
+    double vprev = beta * plaq4(u1, u2r, u1n, u2).y;
+    double2 l1{};  // U1(x-1) of the current row (edge lanes)
+    if (ldL) l1 = U[s0 + i1m];
+    u1 = u1n;
+    u2 = U[V + s0 + i1];
+    double2 x1 = X[s0 + i1], x2 = X[V + s0 + i1], y1 = Y[s0 + i1], y2 = Y[V + s0 + i1];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (k >= nr) break;
+        const int r = r0 + k;
+        const int rn = (r + 1 == L2) ? 0 : r + 1;
+        const long row = (long)r * L1, s = row + i1, sn = (long)rn * L1;
+        u1n = U[sn + i1];
+        double2 u2n{};
+        if (k + 1 < nr) u2n = U[V + sn + i1];
+        const double2 xb1 = X[sn + i1], xb2 = X[V + sn + i1];
+        const double2 yb1 = Y[sn + i1], yb2 = Y[V + sn + i1];
+        const double m1 = mom[s], m2 = mom[V + s];
+        u2r = from_right<double>(u2);
+        if (ldR) u2r = U[V + row + i1p];
+        double2 xa1 = from_right<double>(x1), xa2 = from_right<double>(x2);
+        double2 ya1 = from_right<double>(y1), ya2 = from_right<double>(y2);
+        if (ldR) {
+            xa1 = X[row + i1p];
+            xa2 = X[V + row + i1p];
+            ya1 = Y[row + i1p];
+            ya2 = Y[V + row + i1p];
+        }
+        const double v = beta * plaq4(u1, u2r, u1n, u2).y;
+        double vl = lane_shift(v, true);
+        if (ldL) {
+            const double2 l2 = U[V + row + i1m];
+            const double2 l1n = U[sn + i1m];
+            vl = beta * plaq4(l1, u2, l1n, l2).y;  // P(x-1)
+            l1 = l1n;
+        }
+        double f1, f2;
+        ferm_force_v(u1, (apbc && r == L2 - 1) ? cneg<double>(u2) : u2, x1, x2, y1, y2, xa1, xa2,
+                     ya1, ya2, xb1, xb2, yb1, yb2, f1, f2);
+        double g1 = -v + vprev;
+        double g2 = v - vl;
+        g1 = g1 + cf * f1;
+        g2 = g2 + cf * f2;
+        if (facc) {
+            g1 = g1 + facc[s];
+            g2 = g2 + facc[V + s];
+        }
+        mom[s] = m1 + eps * g1;
+        mom[V + s] = m2 + eps * g2;
+        vprev = v;
+        u1 = u1n;
+        u2 = u2n;
+        x1 = xb1;
+        x2 = xb2;
+        y1 = yb1;
+        y2 = yb2;
     }
 }
 
@@ -806,8 +890,11 @@ void launch_force_mom(const Latt &L, const double2 *U, const double2 *X, const d
                       mom, beta, eps, g.bx, g.by, g.nbx);
         return;
     }
-    launch_k(k_force_mom, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, facc,
-                       mom, beta, eps, cf);
+    // with the fermion force: the marching kernel at the same heights
+    const int rows = gauge_rows(L);
+    const MarchGrid g = march_grid(L, rows);
+    DISPATCH_ROWS(rows, k_force_mom, dim3(g.nblocks()), dim3(kBlock), 0, st, L, U, X, Y, facc,
+                  mom, beta, eps, cf, g.bx, g.by, g.nbx);
 }
 
 // Q X_n for a table of fields (the pole solutions of every pseudofermion of an
@@ -834,64 +921,134 @@ void launch_q_apply_tab(const Latt &L, const double2 *U, const double2 *const *X
                   mass2, g.bx, g.by, g.nbx);
 }
 
+// The fermion force of rows r0 .. r0+nr-1 of column i1, marching as
+// k_force_mom: the links and the X, Y of row r + 1 (x + 2) are loaded one row
+// ahead, X, Y of x + 1 come from the adjacent lane (loaded at the wave's right
+// edge and at the periodic wrap).  emit(k, s, f1, f2) receives row r0 + k.
+template <int R, class Emit>
+__device__ __forceinline__ void ferm_march(const Latt &L, const double2 *__restrict__ U,
+                                           const double2 *__restrict__ X,
+                                           const double2 *__restrict__ Y, int i1, int r0,
+                                           int nr, int i1p, bool ldR, Emit emit)
+{
+    const int L1 = L.L1, L2 = L.L2;
+    const long V = L.V;
+    const bool apbc = (L.bc & 1) != 0;
+    const long s0 = (long)r0 * L1;
+    double2 x1 = X[s0 + i1], x2 = X[V + s0 + i1], y1 = Y[s0 + i1], y2 = Y[V + s0 + i1];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (k >= nr) break;
+        const int r = r0 + k;
+        const int rn = (r + 1 == L2) ? 0 : r + 1;
+        const long row = (long)r * L1, s = row + i1, sn = (long)rn * L1 + i1;
+        const double2 u1 = U[s], u2 = U[V + s];
+        const double2 xb1 = X[sn], xb2 = X[V + sn], yb1 = Y[sn], yb2 = Y[V + sn];
+        double2 xa1 = from_right<double>(x1), xa2 = from_right<double>(x2);
+        double2 ya1 = from_right<double>(y1), ya2 = from_right<double>(y2);
+        if (ldR) {
+            xa1 = X[row + i1p];
+            xa2 = X[V + row + i1p];
+            ya1 = Y[row + i1p];
+            ya2 = Y[V + row + i1p];
+        }
+        double f1, f2;
+        ferm_force_v(u1, (apbc && r == L2 - 1) ? cneg<double>(u2) : u2, x1, x2, y1, y2, xa1, xa2,
+                     ya1, ya2, xb1, xb2, yb1, yb2, f1, f2);
+        emit(k, s, f1, f2);
+        x1 = xb1;
+        x2 = xb2;
+        y1 = yb1;
+        y2 = yb2;
+    }
+}
+
 // facc = sum_n rho[n] F(X_n, Y_n) over a table of fields, per site in the order
 // n = 0, 1, ... with the expressions of k_force_acc (the same bits as its
-// launches one field after another).
-__global__ void k_force_acc_tab(Latt L, const double2 *__restrict__ U, const double2 *const *X,
-                                const double2 *const *Y, const double *rho, int n,
-                                double *__restrict__ facc)
+// launches one field after another).  Marches rows as ferm_march, the field
+// loop outside the rows, the partial sums of the thread's R sites in registers.
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_force_acc_tab(Latt L, const double2 *__restrict__ U,
+                                                          const double2 *const *X,
+                                                          const double2 *const *Y,
+                                                          const double *rho, int n,
+                                                          double *__restrict__ facc, int bx,
+                                                          int by, int nbx)
 {
-    for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < L.V;
-         s += (long)gridDim.x * blockDim.x) {
-        const int i2 = (int)(s / L.L1), i1 = (int)(s - (long)i2 * L.L1);
-        double a1 = 0.0, a2 = 0.0;
-        for (int k = 0; k < n; ++k) {
-            double f1, f2;
-            ferm_force(U, X[k], Y[k], L, i1, i2, f1, f2);
-            if (k == 0) {
-                a1 = rho[k] * f1;
-                a2 = rho[k] * f2;
-            } else {
-                a1 = a1 + rho[k] * f1;
-                a2 = a2 + rho[k] * f2;
-            }
-        }
-        facc[s] = a1;
-        facc[L.V + s] = a2;
+    const MarchPos p = march_pos(L, bx, by, nbx, R);
+    if (!p.active) return;
+    // a wave lies in one row of the block (bx is a multiple of the wave size),
+    // so r0 is uniform: row offsets go to scalar registers
+    const int i1 = p.i1, r0 = __builtin_amdgcn_readfirstlane(p.r0);
+    const int i1p = (i1 + 1 == L.L1) ? 0 : i1 + 1;
+    const bool ldR = ((threadIdx.x & (kWave - 1)) == kWave - 1) || (i1p == 0);
+    const int nr = (r0 + R <= L.L2) ? R : L.L2 - r0;
+    // 0 + rho f rounds as rho f, so the first field needs no branch of its own
+    double a1[R], a2[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) a1[j] = a2[j] = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < n; ++k) {
+        const double rk = rho[k];
+        ferm_march<R>(L, U, X[k], Y[k], i1, r0, nr, i1p, ldR,
+                      [&](int j, long, double f1, double f2) {
+                          a1[j] = a1[j] + rk * f1;
+                          a2[j] = a2[j] + rk * f2;
+                      });
+    }
+    const long V = L.V;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        if (j >= nr) break;
+        const long s = (long)(r0 + j) * L.L1 + i1;
+        facc[s] = a1[j];
+        facc[V + s] = a2[j];
     }
 }
 void launch_force_acc_tab(const Latt &L, const double2 *U, const double2 *const *X,
                           const double2 *const *Y, const double *rho, int n, double *facc,
                           hipStream_t st)
 {
-    launch_k(k_force_acc_tab, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, rho, n,
-             facc);
+    // at most 2 rows: the links and partial sums of every row stay in registers
+    // across the fields
+    const int rows = gauge_rows(L) < 2 ? gauge_rows(L) : 2;
+    const MarchGrid g = march_grid(L, rows);
+    DISPATCH_ROWS(rows, k_force_acc_tab, dim3(g.nblocks()), dim3(kBlock), 0, st, L, U, X, Y, rho,
+                  n, facc, g.bx, g.by, g.nbx);
 }
 
-// facc (+)= rho * F_fermion  (src/QED2dHMC.jl:541-543)
-__global__ void k_force_acc(Latt L, const double2 *__restrict__ U, const double2 *__restrict__ X,
-                            const double2 *__restrict__ Y, double *__restrict__ facc, double rho,
-                            int first)
+// facc (+)= rho * F_fermion  (src/QED2dHMC.jl:541-543), marching as ferm_march
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_force_acc(Latt L, const double2 *__restrict__ U,
+                                                      const double2 *__restrict__ X,
+                                                      const double2 *__restrict__ Y,
+                                                      double *__restrict__ facc, double rho,
+                                                      int first, int bx, int by, int nbx)
 {
-    for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < L.V;
-         s += (long)gridDim.x * blockDim.x) {
-        const int i2 = (int)(s / L.L1), i1 = (int)(s - (long)i2 * L.L1);
-        double f1, f2;
-        ferm_force(U, X, Y, L, i1, i2, f1, f2);
+    const MarchPos p = march_pos(L, bx, by, nbx, R);
+    if (!p.active) return;
+    const int i1 = p.i1, r0 = p.r0;
+    const int i1p = (i1 + 1 == L.L1) ? 0 : i1 + 1;
+    const bool ldR = ((threadIdx.x & (kWave - 1)) == kWave - 1) || (i1p == 0);
+    const int nr = (r0 + R <= L.L2) ? R : L.L2 - r0;
+    const long V = L.V;
+    ferm_march<R>(L, U, X, Y, i1, r0, nr, i1p, ldR, [&](int, long s, double f1, double f2) {
         if (first) {
             facc[s] = rho * f1;
-            facc[L.V + s] = rho * f2;
+            facc[V + s] = rho * f2;
         } else {
             facc[s] = facc[s] + rho * f1;
-            facc[L.V + s] = facc[L.V + s] + rho * f2;
+            facc[V + s] = facc[V + s] + rho * f2;
         }
-    }
+    });
 }
 void launch_force_acc(const Latt &L, const double2 *U, const double2 *X, const double2 *Y,
                       double *facc, double rho, int first, hipStream_t st)
 {
-    launch_k(k_force_acc, dim3(ew_blocks(L.V)), dim3(kBlock), 0, st, L, U, X, Y, facc,
-                       rho, first);
+    const int rows = gauge_rows(L);
+    const MarchGrid g = march_grid(L, rows);
+    DISPATCH_ROWS(rows, k_force_acc, dim3(g.nblocks()), dim3(kBlock), 0, st, L, U, X, Y, facc,
+                  rho, first, g.bx, g.by, g.nbx);
 }
 
 // U <- (cos(eps p), sin(eps p)) U  (src/QED2dAction.jl:55-64)
