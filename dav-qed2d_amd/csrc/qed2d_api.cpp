@@ -89,10 +89,12 @@ struct qed2d_ctx {
     SolverState *state = nullptr;
     RedOut *red = nullptr;
     SolverState *hstate = nullptr;  // pinned
-    // pipelined solves: pinned copies of the state at the end of queued chunks
+    // pipelined solves: copies of the state at the end of queued chunks in
+    // host-coherent pinned memory, each with the sequence number written after it
     SolverState *hring = nullptr;
-    hipEvent_t ring_ev[kRing] = {};
+    int *hseq = nullptr;
     int ring_next = 0;
+    int seq_no = 0;
     RedOut *hred = nullptr;         // pinned
     // kernel-class timing (qed2d_profile_*)
     int prof = 0;  // 0 off, N: time every N-th launch of each class
@@ -481,10 +483,11 @@ struct ChunkPredictor {
 };
 
 // ---------------------------------------------------------------------------
-// Pipelined solves.  A queued chunk of CG iterations ends with an asynchronous
-// copy of the solver state into a pinned slot of its own and an event; the
-// host queues the next chunk before it waits for the previous one, so the GPU
-// keeps iterating while the host looks at the state.  Chunks are sized from
+// Pipelined solves.  A queued chunk of CG iterations ends with a one-thread
+// kernel that copies the solver state into a host-coherent pinned slot of its
+// own and then writes the chunk's sequence number beside it; the host queues
+// the next chunk before it waits (polling that number) for the previous one,
+// so the GPU keeps iterating while the host looks at the state.  Chunks are sized from
 // the observed convergence rate to end where the solve is predicted to end; a
 // launch past the end of a solve returns at once.
 // ---------------------------------------------------------------------------
@@ -493,6 +496,7 @@ struct ChunkQueue {
     bool last[kRing];
     int k0[kRing];   // first iteration of the chunk
     bool ru[kRing];  // the chunk starts with an on-device reliable update
+    int want[kRing];  // the sequence number of its state copy
     int head = 0, n = 0;
     int back() const { return (head + n - 1) % kRing; }
     bool back_last() const { return n > 0 && last[back()]; }
@@ -502,10 +506,11 @@ int cq_push(qed2d_ctx *c, ChunkQueue &q, bool last, int k0 = 0, bool ru = false)
 {
     const int s = c->ring_next;
     c->ring_next = (s + 1) % kRing;
-    HIPCHK(c, hipMemcpyAsync(c->hring + s, c->state, sizeof(SolverState), hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipEventRecord(c->ring_ev[s], c->stream));
+    c->seq_no = (c->seq_no == 0x7fffffff) ? 1 : c->seq_no + 1;
+    launch_state_snap(c->state, c->hring + s, c->hseq + s, c->seq_no, c->stream);
+    TRY(launched(c));
     const int i = (q.head + q.n) % kRing;
+    q.want[i] = c->seq_no;
     q.slot[i] = s;
     q.last[i] = last;
     q.k0[i] = k0;
@@ -518,11 +523,23 @@ int cq_push(qed2d_ctx *c, ChunkQueue &q, bool last, int k0 = 0, bool ru = false)
 int cq_pop(qed2d_ctx *c, ChunkQueue &q)
 {
     if (q.n < 1) return fail(c, QED2D_EHIP, "internal: no queued CG chunk to wait for");
-    const int s = q.slot[q.head];
+    const int s = q.slot[q.head], want = q.want[q.head];
     q.head = (q.head + 1) % kRing;
     q.n -= 1;
-    HIPCHK(c, hipEventSynchronize(c->ring_ev[s]));
-    *c->hstate = c->hring[s];
+    // poll the sequence number; now and then ask the stream whether a launch
+    // failed, or whether it drained without the copy arriving
+    for (long spin = 1;; ++spin) {
+        if (__atomic_load_n(c->hseq + s, __ATOMIC_ACQUIRE) == want) break;
+        if ((spin & 255) == 0) {
+            const hipError_t e = hipStreamQuery(c->stream);
+            if (e == hipSuccess && __atomic_load_n(c->hseq + s, __ATOMIC_ACQUIRE) != want)
+                return fail(c, QED2D_EHIP, "internal: solver state copy %d did not arrive", want);
+            if (e != hipSuccess && e != hipErrorNotReady)
+                return fail(c, QED2D_EHIP, "CG chunk: %s", hipGetErrorString(e));
+        }
+        __builtin_ia32_pause();
+    }
+    std::memcpy((void *)c->hstate, (const void *)(c->hring + s), sizeof(SolverState));
     return QED2D_OK;
 }
 
@@ -1748,10 +1765,11 @@ int qed2d_ctx_create(qed2d_ctx **out, int L1, int L2, double beta, int bc_flags,
               hipHostMalloc((void **)&c->hstate, sizeof(SolverState), hipHostMallocDefault) ==
                   hipSuccess &&
               hipHostMalloc((void **)&c->hring, kRing * sizeof(SolverState),
-                            hipHostMallocDefault) == hipSuccess &&
+                            hipHostMallocCoherent) == hipSuccess &&
+              hipHostMalloc((void **)&c->hseq, kRing * sizeof(int), hipHostMallocCoherent) ==
+                  hipSuccess &&
               hipHostMalloc((void **)&c->hred, sizeof(RedOut), hipHostMallocDefault) == hipSuccess;
-    for (int i = 0; ok && i < kRing; ++i)
-        ok = hipEventCreateWithFlags(&c->ring_ev[i], hipEventDisableTiming) == hipSuccess;
+    if (ok) std::memset(c->hseq, 0, kRing * sizeof(int));
     if (!ok) {
         he = hipGetLastError();
         g_create_err = std::string("allocation of the context's buffers failed: ") +
@@ -1803,7 +1821,7 @@ void qed2d_ctx_destroy(qed2d_ctx *c)
     if (c->red) (void)hipFree(c->red);
     if (c->hstate) (void)hipHostFree(c->hstate);
     if (c->hring) (void)hipHostFree(c->hring);
-    for (auto e : c->ring_ev) if (e) (void)hipEventDestroy(e);
+    if (c->hseq) (void)hipHostFree(c->hseq);
     if (c->hred) (void)hipHostFree(c->hred);
     for (auto &p : c->pending) {
         (void)hipEventDestroy(p.a);

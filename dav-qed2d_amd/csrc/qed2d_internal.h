@@ -294,6 +294,8 @@ void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t
 // n FP64 U(1) links -> one float each (the packed format of the FP32 site-pair
 // solver); sets S->bad_links if a link is not of unit modulus to 2^-23 (FP32 epsilon).
 void launch_link_pack(const double2 *U, float *out, long n, SolverState *S, hipStream_t st);
+// *out = *S in pinned host memory, then v into *seq (release, system scope)
+void launch_state_snap(const SolverState *S, SolverState *out, int *seq, int v, hipStream_t st);
 // A reliable update's x += double(xs), xs = 0; with p0 / p1 given (the FP32 site-pair solver, whose
 // K1 was paused with an x update pending), xs + alpha_x p_{stop_k - 1} first.
 // kb > 0: the on-device update queued before K1(kb), run only while a soft

@@ -573,6 +573,22 @@ void launch_link_pack(const double2 *U, float *out, long n, SolverState *S, hipS
     launch_k(k_link_pack, dim3(ew_grid(n)), dim3(kBlock), 0, st, U, out, n, S);
 }
 
+// A copy of the solver state into pinned host memory for the host's chunk
+// planning, then the sequence number v into *seq with a release at system
+// scope: the host that reads v there finds the copy complete.  (No blit and no
+// event between the chunks: the next chunk's kernels follow at once.)
+__global__ void k_state_snap(const SolverState *__restrict__ S, SolverState *out, int *seq, int v)
+{
+    if (threadIdx.x == 0) {
+        *out = *S;
+        __hip_atomic_store(seq, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+void launch_state_snap(const SolverState *S, SolverState *out, int *seq, int v, hipStream_t st)
+{
+    launch_k(k_state_snap, dim3(1), dim3(kWave), 0, st, S, out, seq, v);
+}
+
 // ===========================================================================
 // Multi-shift CG (Jegerlehner, hep-lat/9612014).  The systems
 // (Q^2 + sigma_i) x_i = b share the Krylov space of the base system
