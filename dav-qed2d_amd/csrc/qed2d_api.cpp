@@ -69,6 +69,12 @@ struct qed2d_ctx {
         BParts P{};
         double2 *arena = nullptr;  // r, p0, p1, t of every right-hand side
         std::vector<std::vector<double2 *>> x, ps;  // per right-hand side and shift
+        // the mixed-precision force solves: FP32 multi-shift x_i and p_i per
+        // right-hand side and shift; the FP64 solutions and right-hand sides of
+        // every pole, pole after pole (pcap poles)
+        std::vector<std::vector<float2 *>> x32, ps32;
+        double2 *px = nullptr, *psrc = nullptr;
+        int pcap = 0;
         // the force of every pole solution: Y = Q x (yarena, ycap fields) and a
         // table of x, Y, 2 + m0 and rho per field (device and pinned copies)
         double2 *yarena = nullptr;
@@ -80,6 +86,7 @@ struct qed2d_ctx {
     // their count: qed2d_cg_mrhs and the RHMC heatbath's shifted solves
     std::vector<qed2d_batch *> rhsets;
     int multishift = 1;               // MultiCG / RHMC force through multi-shift CG
+    int rhmc_prec = QED2D_PREC_F64;   // the RHMC force's solves (qed2d_ctx_set_rhmc_precision)
     MultiShiftState *ms = nullptr;    // device
     MultiShiftState *hms = nullptr;   // pinned
     double *part = nullptr;  // partial sums of the generic reductions
@@ -875,6 +882,17 @@ int grow(qed2d_ctx *c, std::vector<double2 *> &v, int n)
     return QED2D_OK;
 }
 
+int grow32(qed2d_ctx *c, std::vector<float2 *> &v, int n)
+{
+    while ((int)v.size() < n) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(float2) * spinor_elems(c)) != hipSuccess)
+            return fail(c, QED2D_ENOMEM, "hipMalloc of an FP32 multi-shift buffer failed");
+        v.push_back((float2 *)p);
+    }
+    return QED2D_OK;
+}
+
 // nx internal solution buffers, np search-direction buffers, the state.
 int get_shiftbufs(qed2d_ctx *c, int nx, int np)
 {
@@ -964,7 +982,17 @@ int mr_setup(qed2d_ctx *c, int nf)
                         (void *)m.part, (void *)m.cnt, (void *)m.arena})
             if (p) (void)hipFree(p);
         if (m.hM) (void)hipHostFree(m.hM);
-        m = {};
+        // (only what is reallocated here: the solution, force and pole buffers
+        // of the set stay, whoever allocated them)
+        m.dch = nullptr;
+        m.blk0 = nullptr;
+        m.shape = nullptr;
+        m.M = m.hM = nullptr;
+        m.part = nullptr;
+        m.cnt = nullptr;
+        m.arena = nullptr;
+        m.P = BParts{};
+        m.cap = m.n = m.rows = 0;
         const long nbc = march_grid(c->L, 1).nblocks();
         const long n2 = spinor_elems(c);
         if (hipMalloc((void **)&m.dch, sizeof(BChain) * nf) != hipSuccess ||
@@ -1007,12 +1035,19 @@ int mr_setup(qed2d_ctx *c, int nf)
         m.x.resize(nf);
         m.ps.resize(nf);
     }
+    if ((int)m.x32.size() < nf) {
+        m.x32.resize(nf);
+        m.ps32.resize(nf);
+    }
     return QED2D_OK;
 }
 
+// f32: the whole solve in FP32 (FP32 links, spinors, x_i and p_i; Krylov
+// scalars and sums in FP64), the solutions in c->mr.x32[j][i]; a right-hand
+// side that reaches the iteration limit is not an error (its x_i are guesses).
 int multishift_multi(qed2d_ctx *c, const double *U, int nf, const double *const *b,
                      const double *am0, const int *npole, const double *const *mu, int maxiter,
-                     double eps, int *iters)
+                     double eps, int *iters, bool f32 = false)
 {
     TRY(mr_setup(c, nf));
     auto &m = c->mr;
@@ -1024,8 +1059,13 @@ int multishift_multi(qed2d_ctx *c, const double *U, int nf, const double *const 
         if (npole[j] < 1 || npole[j] > kMaxShift)
             return fail(c, QED2D_EINVAL, "multi-shift CG takes 1..%d shifts", kMaxShift);
         maxshift = std::max(maxshift, npole[j]);
-        TRY(grow(c, m.x[j], npole[j]));
-        TRY(grow(c, m.ps[j], npole[j]));
+        if (f32) {
+            TRY(grow32(c, m.x32[j], npole[j]));
+            TRY(grow32(c, m.ps32[j], npole[j]));
+        } else {
+            TRY(grow(c, m.x[j], npole[j]));
+            TRY(grow(c, m.ps[j], npole[j]));
+        }
         ch[j].L = c->L;
         ch[j].off = (long)j * c->L.V;
         ch[j].uoff = 0;
@@ -1043,8 +1083,8 @@ int multishift_multi(qed2d_ctx *c, const double *U, int nf, const double *const 
         h.nshift = npole[j];
         for (int k = 0; k < npole[j]; ++k) {
             h.s[k] = mu[j][k] * mu[j][k] - sb;
-            h.x[k] = m.x[j][k];
-            h.ps[k] = m.ps[j][k];
+            h.x[k] = f32 ? (void *)m.x32[j][k] : (void *)m.x[j][k];
+            h.ps[k] = f32 ? (void *)m.ps32[j][k] : (void *)m.ps[j][k];
         }
         h.b = (const double2 *)b[j];
         h.kk = 1;
@@ -1062,13 +1102,32 @@ int multishift_multi(qed2d_ctx *c, const double *U, int nf, const double *const 
     g.nblocks = nf * march_grid(c->L, c->rows).nblocks();
     g.inter = 1;  // one lattice shape, shared links
     double2 *r = m.arena, *p0 = r + n2 * nf, *p1 = p0 + n2 * nf, *t = p1 + n2 * nf;
-    launch_b_ms_init(g, c->rows, r, p0, m.M, m.P, st);
+    // (FP32: the same arena holds the FP32 r, p0, p1, t; the FP32 links go to
+    // the mixed CG's link buffer)
+    float2 *r32 = (float2 *)m.arena, *p032 = r32 + n2 * nf, *p132 = p032 + n2 * nf,
+           *t32 = p132 + n2 * nf;
+    float2 *U32 = nullptr;
+    if (f32) {
+        TRY(get_fspin(c, FS_U, &U32));
+        launch_to_f32((const double2 *)U, U32, 2 * c->L.V, st);
+        launch_b_ms_init_f32(g, c->rows, r32, p032, m.M, m.P, st);
+    } else {
+        launch_b_ms_init(g, c->rows, r, p0, m.M, m.P, st);
+    }
     std::vector<ChunkPredictor> pred(nf);
     int chunk = 8;
     long launched_k = 0;
     for (;;) {
         for (int jj = 0; jj < chunk; ++jj) {
             // (the shift updates run as extra blocks of the same launch)
+            if (f32) {
+                PROF(c, QED2D_PROF_OTHER,
+                     launch_b_ms_k1_f32(g, c->rows, U32, r32, p032, p132, t32, m.M, m.P, n2,
+                                        maxshift, st));
+                PROF(c, QED2D_PROF_OTHER,
+                     launch_b_ms_k2_f32(g, c->rows, U32, t32, p032, p132, r32, m.M, m.P, st));
+                continue;
+            }
             PROF(c, QED2D_PROF_OTHER,
                  launch_b_ms_k1(g, c->rows, (const double2 *)U, r, p0, p1, t, m.M, m.P, n2,
                                 maxshift, st));
@@ -1097,7 +1156,7 @@ int multishift_multi(qed2d_ctx *c, const double *U, int nf, const double *const 
     }
     int it = 0;
     for (int j = 0; j < nf; ++j) {
-        if (m.hM[j].status != 1)
+        if (m.hM[j].status != 1 && !f32)
             return fail(c, QED2D_ENOCONV, "multi-shift CG of right-hand side %d not converged "
                         "after %d iterations", j, maxiter);
         it = std::max(it, m.hM[j].iter);
@@ -1290,6 +1349,68 @@ int force_tab(qed2d_ctx *c, int n)
     return QED2D_OK;
 }
 
+int rhs_set(qed2d_ctx *c, int n, qed2d_batch **out);
+
+// |r|^2 / |b|^2 of the FP32 multi-shift phase of the mixed-precision RHMC
+// force: |r| / |b| about 1e-5, where the FP32 iterated residuals of the
+// shifted systems still follow the true ones.
+constexpr double kRhmcF32Eps = 1e-10;
+
+// The mixed-precision RHMC force solves (qed2d_ctx_set_rhmc_precision): every
+// pole of every pseudofermion in one FP32 multi-shift solve to kRhmcF32Eps (or
+// eps, if looser), its solutions widened to FP64 as the guesses of one batched
+// mixed-precision CG of all the poles' systems (Q^2 + mu^2) x = phi (FP32
+// iterations, FP64 reliable updates, each system's FP64 stopping rule
+// |r|^2 < eps |phi|^2).  Pole k of the loop over flavours and poles lands in
+// c->mr.px + k * n2.
+int rhmc_solve_mixed(qed2d_ctx *c, const double *U, int nf, const double *const *b,
+                     const double *am0, const int *npole, const double *const *mu, int maxiter,
+                     double eps, int *iters)
+{
+    auto &m = c->mr;
+    const long n2 = spinor_elems(c);
+    hipStream_t st = c->stream;
+    int n = 0;
+    for (int j = 0; j < nf; ++j) n += npole[j];
+    int it32 = 0;
+    TRY(multishift_multi(c, U, nf, b, am0, npole, mu, maxiter, std::max(eps, kRhmcF32Eps), &it32,
+                         true));
+    if (n > m.pcap) {
+        if (m.px) (void)hipFree(m.px);
+        if (m.psrc) (void)hipFree(m.psrc);
+        m.px = m.psrc = nullptr;
+        m.pcap = 0;
+        if (hipMalloc((void **)&m.px, sizeof(double2) * n2 * n) != hipSuccess ||
+            hipMalloc((void **)&m.psrc, sizeof(double2) * n2 * n) != hipSuccess)
+            return fail(c, QED2D_ENOMEM, "hipMalloc of %d pole solutions failed", n);
+        m.pcap = n;
+    }
+    std::vector<double> mass(n), shift(n);
+    int k = 0;
+    for (int j = 0; j < nf; ++j)
+        for (int i = 0; i < npole[j]; ++i, ++k) {
+            launch_to_f64(m.x32[j][i], m.px + (long)k * n2, n2, st);
+            launch_copy(m.psrc + (long)k * n2, (const double2 *)b[j], n2, st);
+            mass[k] = am0[j];
+            shift[k] = mu[j][i] * mu[j][i];
+        }
+    TRY(launched(c));
+    qed2d_batch *set = nullptr;
+    TRY(rhs_set(c, n, &set));
+    std::vector<int> its(n);
+    std::string err;
+    const int rc = batch_rhs_cg_shifted(set, U, (double *)m.px, (const double *)m.psrc,
+                                        mass.data(), shift.data(), maxiter, eps,
+                                        QED2D_PREC_MIXED, its.data(), c->rows_q, c->rows,
+                                        c->rows32, c->rows_k2, c->rows32_k2, c->ru_delta, &err,
+                                        true);
+    if (rc) return fail(c, rc, "%s", err.c_str());
+    int itmax = 0;
+    for (int v : its) itmax = std::max(itmax, v);
+    if (iters) *iters = it32 + itmax;
+    return QED2D_OK;
+}
+
 int update_momenta_rhmc(qed2d_ctx *c, double *mom, const double *U, double *X,
                         const std::vector<double2 *> &F, double *g5DX, const double *am0, int nf,
                         const int *npole, const double *const *mu, const double *const *rho,
@@ -1305,7 +1426,11 @@ int update_momenta_rhmc(qed2d_ctx *c, double *mom, const double *U, double *X,
         std::vector<const double *> bj(nf);
         for (int j = 0; j < nf; ++j) bj[j] = (const double *)F[j];
         int it = 0;
-        TRY(multishift_multi(c, U, nf, bj.data(), am0, npole, mu, maxiter, tol, &it));
+        const bool mixed = (c->rhmc_prec == QED2D_PREC_MIXED);
+        if (mixed)
+            TRY(rhmc_solve_mixed(c, U, nf, bj.data(), am0, npole, mu, maxiter, tol, &it));
+        else
+            TRY(multishift_multi(c, U, nf, bj.data(), am0, npole, mu, maxiter, tol, &it));
         // every pole solution's Q x in one launch, and the sum of their forces
         // (in the order of the loop over flavours and poles) in another
         int n = 0;
@@ -1319,7 +1444,7 @@ int update_momenta_rhmc(qed2d_ctx *c, double *mom, const double *U, double *X,
         int k = 0;
         for (int j = 0; j < nf; ++j)
             for (int i = 0; i < npole[j]; ++i, ++k) {
-                hx[k] = m.x[j][i];
+                hx[k] = mixed ? m.px + (long)k * n2 : m.x[j][i];
                 hy[k] = m.yarena + (long)k * n2;
                 hm[k] = 2.0 + am0[j];
                 hr[k] = rho[j][i];
@@ -1807,6 +1932,10 @@ void qed2d_ctx_destroy(qed2d_ctx *c)
     for (auto p : c->rhsets) if (p) qed2d_batch_destroy(p);
     for (auto &v : c->mr.x) for (auto p : v) if (p) (void)hipFree(p);
     for (auto &v : c->mr.ps) for (auto p : v) if (p) (void)hipFree(p);
+    for (auto &v : c->mr.x32) for (auto p : v) if (p) (void)hipFree(p);
+    for (auto &v : c->mr.ps32) for (auto p : v) if (p) (void)hipFree(p);
+    if (c->mr.px) (void)hipFree(c->mr.px);
+    if (c->mr.psrc) (void)hipFree(c->mr.psrc);
     for (void *p : {(void *)c->mr.dch, (void *)c->mr.blk0, (void *)c->mr.shape, (void *)c->mr.M,
                     (void *)c->mr.part, (void *)c->mr.cnt, (void *)c->mr.arena,
                     (void *)c->mr.yarena, c->mr.tab})
@@ -1923,6 +2052,19 @@ int qed2d_ctx_set_reliable_mode(qed2d_ctx *c, int on_device)
     if (on_device != 0 && on_device != 1)
         return fail(c, QED2D_EINVAL, "reliable-update mode must be 0 or 1 (got %d)", on_device);
     c->ru_mode = on_device;
+    return QED2D_OK;
+}
+
+int qed2d_ctx_set_rhmc_precision(qed2d_ctx *c, int precision)
+{
+    if (!c) return QED2D_EINVAL;
+    if (precision != QED2D_PREC_F64 && precision != QED2D_PREC_MIXED)
+        return fail(c, QED2D_EINVAL, "RHMC force precision must be QED2D_PREC_F64 or "
+                    "QED2D_PREC_MIXED (got %d)", precision);
+    if (precision == QED2D_PREC_MIXED && !pair_ok(c->L))
+        return fail(c, QED2D_EINVAL, "the mixed-precision RHMC force needs an even L1 >= 4 "
+                    "(L1 = %d)", c->L.L1);
+    c->rhmc_prec = precision;
     return QED2D_OK;
 }
 

@@ -178,8 +178,8 @@ void launch_b_cg_init(const BGeo &g, int rows, const double2 *b, double2 *x, voi
 }
 
 // ---------------------------------------------------------------------------
-// Residual from x with t = Q x: mode 0 starts an FP64 solve from a guess on
-// every chain; mode 2 is the reliable update of the chains that wait for one
+// Residual from x with t = Q x: mode 0 starts a solve from a guess on every
+// chain (FP64: r and p0 = 0; mixed: rs and p0s = 0); mode 2 is the reliable update of the chains that wait for one
 // (mask[c] != 0, set by the host from the states it read: need_ru itself is
 // cleared by this kernel and cannot select its blocks).
 // ---------------------------------------------------------------------------
@@ -190,7 +190,8 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
                                                       const double2 *__restrict__ x,
                                                       double2 *__restrict__ r,
                                                       float2 *__restrict__ rs,
-                                                      double2 *__restrict__ p0, SolverState *S,
+                                                      double2 *__restrict__ p0,
+                                                      float2 *__restrict__ p0s, SolverState *S,
                                                       BParts P, int mode, const int *mask)
 {
     const BWhere w = bwhere<false>(g, R);
@@ -199,7 +200,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
     const int k = (mode == 2) ? Sc->kk : 1;
     const long o = w.o;
     EpiResid epi{b + o, x + o, r ? r + o : nullptr, rs ? rs + o : nullptr,
-                 p0 ? p0 + o : nullptr, nullptr, Sc->mu2, w.L.V, 0.0, 0.0};
+                 p0 ? p0 + o : nullptr, p0s ? p0s + o : nullptr, Sc->mu2, w.L.V, 0.0, 0.0};
     if (w.p.active) {
         SrcPlain<double> src{t + o, w.L.V};
         q_march<double, R>(w.L, U + w.uo, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
@@ -224,10 +225,11 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
 }
 void launch_b_cg_resid(const BGeo &g, int rows, const double2 *U, const double2 *b,
                        const double2 *t, const double2 *x, double2 *r, float2 *rs, double2 *p0,
-                       SolverState *S, BParts P, int mode, const int *mask, hipStream_t st)
+                       float2 *p0s, SolverState *S, BParts P, int mode, const int *mask,
+                       hipStream_t st)
 {
     DISPATCH_ROWS(rows, kb_cg_resid, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, b, x, r, rs,
-                  p0, S, P, mode, mask);
+                  p0, p0s, S, P, mode, mask);
 }
 
 // x += double(xs), xs = 0 on the chains that wait for a reliable update, each
@@ -367,11 +369,11 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, LinksF32 U,
     }
 }
 
-template <int R, bool PK>
-__global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, LinksF32 U,
-                                                  const float2 *__restrict__ t, const float2 *P0,
-                                                  const float2 *P1, float2 *__restrict__ r,
-                                                  SolverState *S, BParts P)
+template <int R, bool PK, bool MU>
+__device__ __forceinline__ void b_cg2v_body(const BGeo &g, const LinksF32 &U,
+                                            const float2 *__restrict__ t, const float2 *P0,
+                                            const float2 *P1, float2 *__restrict__ r,
+                                            SolverState *S, const BParts &P)
 {
     __shared__ double lds[kBlock / kWave + 1];
     const BWhere w = bwhere<true>(g, R);
@@ -379,11 +381,13 @@ __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, LinksF32 U,
     // The chain's state goes out first through a per-lane address, the first
     // rows' loads next (they do not depend on the iteration); the partials of
     // K1 (slot by iteration) and alpha follow once the state is in.  A chain
-    // that stopped returns before any store.
+    // that stopped returns before any store.  (With a shift, p_k is read with
+    // the rows: its slot waits for the state.)
     const SolverState *Sv = state_v(Sc);
     const int skv = Sv->stop_k, kv = Sv->k2;
     const long o = w.o;
-    EpiK2v<false> epi{nullptr, r + o, 0.0f, 0.0f, w.L.V, 0.0};  // (p: read only with a shift)
+    EpiK2v<MU> epi{nullptr, r + o, 0.0f, 0.0f, w.L.V, 0.0};
+    if (MU) epi.p = ((__builtin_amdgcn_readfirstlane(kv) & 1) ? P1 : P0) + o;
     int k = 0, wp = 0;
     double alpha = 0.0;
     auto ready = [&]() -> bool {
@@ -394,6 +398,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, LinksF32 U,
         const double pp = sum_prev_partials(P.tt[wp] + 2 * w.pb, P.ntt[wp][w.c], 2, 1, lds);
         alpha = Sc->norm[wp] / (tt + Sc->mu2 * pp);
         epi.alpha = (float)alpha;
+        epi.mu2 = (float)Sc->mu2;
         return true;
     };
     SrcPlain2 src{t + o, w.L.V};
@@ -409,6 +414,23 @@ __global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, LinksF32 U,
             Sc->alpha_x = alpha;  // for K1(k + 1)
         }
     }
+}
+template <int R, bool PK>
+__global__ void __launch_bounds__(kBlock) kb_cg2v(BGeo g, LinksF32 U,
+                                                  const float2 *__restrict__ t, const float2 *P0,
+                                                  const float2 *P1, float2 *__restrict__ r,
+                                                  SolverState *S, BParts P)
+{
+    b_cg2v_body<R, PK, false>(g, U, t, P0, P1, r, S, P);
+}
+// (chains with a shift mu2: r -= alpha (Q t + mu2 p_k))
+template <int R, bool PK>
+__global__ void __launch_bounds__(kBlock) kb_cg2vm(BGeo g, LinksF32 U,
+                                                   const float2 *__restrict__ t, const float2 *P0,
+                                                   const float2 *P1, float2 *__restrict__ r,
+                                                   SolverState *S, BParts P)
+{
+    b_cg2v_body<R, PK, true>(g, U, t, P0, P1, r, S, P);
 }
 
 void launch_b_cg_k1_f64(const BGeo &g, int rows, const double2 *U, const double2 *r,
@@ -448,11 +470,16 @@ void launch_b_cg_k1_f32(const BGeo &g, int rows, const void *U, bool packed, con
 // (x is updated by the next K1 or the reliable update's flush)
 void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *t,
                         const float2 *p0, const float2 *p1, float2 *, float2 *r,
-                        SolverState *S, BParts P, hipStream_t st)
+                        SolverState *S, BParts P, bool mu, hipStream_t st)
 {
     const LinksF32 lk{U, packed};
-    DISPATCH_ROWS_PK(rows, packed, kb_cg2v, dim3(g.nblocks), dim3(kBlock), 0, st, g, lk, t, p0,
-                     p1, r, S, P);
+    if (mu) {
+        DISPATCH_ROWS_PK(rows, packed, kb_cg2vm, dim3(g.nblocks), dim3(kBlock), 0, st, g, lk, t,
+                         p0, p1, r, S, P);
+    } else {
+        DISPATCH_ROWS_PK(rows, packed, kb_cg2v, dim3(g.nblocks), dim3(kBlock), 0, st, g, lk, t,
+                         p0, p1, r, S, P);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -464,15 +491,16 @@ void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, con
 // s, so K1(s) never runs twice: the step in which the last shifts converge
 // updates their solutions and must not repeat.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) kb_ms_init(BGeo g, int rows, double2 *__restrict__ r,
-                                                     double2 *__restrict__ p0,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kb_ms_init(BGeo g, int rows, c2<T> *__restrict__ r,
+                                                     c2<T> *__restrict__ p0,
                                                      const MultiShiftState *M, BParts P)
 {
     const BWhere w = bwhere<false>(g, rows);
     const MultiShiftState *Mc = M + w.c;
     const Latt &L = w.L;
     const int nshift = Mc->nshift;
-    const double2 z = mk<double>(0.0, 0.0);
+    const c2<T> z = mk<T>(T(0), T(0));
     double acc = 0.0;
     if (w.p.active) {
         for (int kk = 0; kk < rows; ++kk) {
@@ -481,11 +509,11 @@ __global__ void __launch_bounds__(kBlock) kb_ms_init(BGeo g, int rows, double2 *
             for (int c = 0; c < 2; ++c) {
                 const long i = c * L.V + (long)row * L.L1 + w.p.i1;
                 const double2 v = Mc->b[i];
-                r[w.o + i] = v;
+                r[w.o + i] = from_d2<T>(v);
                 p0[w.o + i] = z;
                 for (int k = 0; k < nshift; ++k) {
-                    Mc->x[k][i] = z;
-                    Mc->ps[k][i] = z;
+                    ms_vec<T>(Mc->x[k])[i] = z;
+                    ms_vec<T>(Mc->ps[k])[i] = z;
                 }
                 acc += v.x * v.x + v.y * v.y;
             }
@@ -500,16 +528,21 @@ __global__ void __launch_bounds__(kBlock) kb_ms_init(BGeo g, int rows, double2 *
 void launch_b_ms_init(const BGeo &g, int rows, double2 *r, double2 *p0,
                       const MultiShiftState *M, BParts P, hipStream_t st)
 {
-    launch_k(kb_ms_init, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, r, p0, M, P);
+    launch_k(kb_ms_init<double>, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, r, p0, M, P);
+}
+void launch_b_ms_init_f32(const BGeo &g, int rows, float2 *r, float2 *p0,
+                          const MultiShiftState *M, BParts P, hipStream_t st)
+{
+    launch_k(kb_ms_init<float>, dim3(g.nblocks), dim3(kBlock), 0, st, g, rows, r, p0, M, P);
 }
 
 // Blocks past g.nblocks are shift-update blocks: chain c, shift s, elements
 // eb * kShiftEPT * kBlock .. of the chain's spinor (maxshift * bpc of them per
 // chain).
-template <int R>
-__global__ void __launch_bounds__(kBlock) kb_ms1(BGeo g, const double2 *__restrict__ U,
-                                                 const double2 *__restrict__ r, double2 *P0,
-                                                 double2 *P1, double2 *__restrict__ t,
+template <typename T, int R>
+__global__ void __launch_bounds__(kBlock) kb_ms1(BGeo g, const c2<T> *__restrict__ U,
+                                                 const c2<T> *__restrict__ r, c2<T> *P0,
+                                                 c2<T> *P1, c2<T> *__restrict__ t,
                                                  MultiShiftState *M, BParts P, int maxshift,
                                                  int bpc)
 {
@@ -524,9 +557,9 @@ __global__ void __launch_bounds__(kBlock) kb_ms1(BGeo g, const double2 *__restri
         if (sk != 0 && sk < k) return;
         const int rd = (k - 1) & 1;
         const BChain &ch = g.ch[c];
-        ms1_shift_block(Mc, r + 2 * ch.off, P.tt[rd] + 2 * ch.pb, P.rr[rd] + ch.pb,
-                        P.nrr[rd][c], P.ntt[rd][c], k, Mc->nshift, sft,
-                        (long)eb * kShiftEPT * kBlock, 2 * ch.L.V);
+        ms1_shift_block<T>(Mc, r + 2 * ch.off, P.tt[rd] + 2 * ch.pb, P.rr[rd] + ch.pb,
+                           P.nrr[rd][c], P.ntt[rd][c], k, Mc->nshift, sft,
+                           (long)eb * kShiftEPT * kBlock, 2 * ch.L.V);
         return;
     }
     const BWhere w = bwhere<false>(g, R);
@@ -540,15 +573,15 @@ __global__ void __launch_bounds__(kBlock) kb_ms1(BGeo g, const double2 *__restri
         P.ntt[wp][w.c] = w.nb;
     }
     const long o = w.o;
-    ms1_body<R>(w.L, U + w.uo, r + o, P0 + o, P1 + o, t + o, 2.0 + w.am0, Mc,
-                P.tt[wp] + 2 * w.pb, P.tt[rd] + 2 * w.pb, P.rr[rd] + w.pb, P.nrr[rd][w.c],
-                P.ntt[rd][w.c], k, Mc->nshift, w.p);
+    ms1_body<T, R>(w.L, U + w.uo, r + o, P0 + o, P1 + o, t + o, T(2) + (T)w.am0, Mc,
+                   P.tt[wp] + 2 * w.pb, P.tt[rd] + 2 * w.pb, P.rr[rd] + w.pb, P.nrr[rd][w.c],
+                   P.ntt[rd][w.c], k, Mc->nshift, w.p);
 }
 
-template <int R>
-__global__ void __launch_bounds__(kBlock) kb_ms2(BGeo g, const double2 *__restrict__ U,
-                                                 const double2 *__restrict__ t, const double2 *P0,
-                                                 const double2 *P1, double2 *__restrict__ r,
+template <typename T, int R>
+__global__ void __launch_bounds__(kBlock) kb_ms2(BGeo g, const c2<T> *__restrict__ U,
+                                                 const c2<T> *__restrict__ t, const c2<T> *P0,
+                                                 const c2<T> *P1, c2<T> *__restrict__ r,
                                                  MultiShiftState *M, BParts P)
 {
     const BWhere w = bwhere<false>(g, R);
@@ -561,28 +594,58 @@ __global__ void __launch_bounds__(kBlock) kb_ms2(BGeo g, const double2 *__restri
     }
     const int wp = k & 1;
     const long o = w.o;
-    ms2_body<R>(w.L, U + w.uo, t + o, P0 + o, P1 + o, r + o, 2.0 + w.am0, Mc,
-                P.tt[wp] + 2 * w.pb, P.rr[wp] + w.pb, P.ntt[wp][w.c], k, w.p);
+    ms2_body<T, R>(w.L, U + w.uo, t + o, P0 + o, P1 + o, r + o, T(2) + (T)w.am0, Mc,
+                   P.tt[wp] + 2 * w.pb, P.rr[wp] + w.pb, P.ntt[wp][w.c], k, w.p);
     if (w.p.blk == 0 && threadIdx.x == 0) {
         P.nrr[wp][w.c] = w.nb;
         Mc->kk = k + 1;
     }
 }
+// DISPATCH_ROWS for the kernels templated on <spinor type, rows>.
+#define DISPATCH_ROWS_T(rows, KERNEL, T, ...)                                         \
+    switch (rows) {                                                                   \
+    case 1: launch_k((KERNEL<T, 1>), __VA_ARGS__); break;                             \
+    case 2: launch_k((KERNEL<T, 2>), __VA_ARGS__); break;                             \
+    case 4: launch_k((KERNEL<T, 4>), __VA_ARGS__); break;                             \
+    case 8: launch_k((KERNEL<T, 8>), __VA_ARGS__); break;                             \
+    default: launch_k((KERNEL<T, 16>), __VA_ARGS__); break;                           \
+    }
 
-void launch_b_ms_k1(const BGeo &g, int rows, const double2 *U, const double2 *r, double2 *p0,
-                    double2 *p1, double2 *t, MultiShiftState *M, BParts P, long n2max,
+template <typename T>
+static void b_ms_k1(const BGeo &g, int rows, const c2<T> *U, const c2<T> *r, c2<T> *p0,
+                    c2<T> *p1, c2<T> *t, MultiShiftState *M, BParts P, long n2max,
                     int maxshift, hipStream_t st)
 {
     const int bpc = (int)((n2max + (long)kShiftEPT * kBlock - 1) / ((long)kShiftEPT * kBlock));
     const long nb = g.nblocks + (long)g.n * maxshift * bpc;
-    DISPATCH_ROWS(rows, kb_ms1, dim3((unsigned)nb), dim3(kBlock), 0, st, g, U, r, p0, p1, t, M,
-                  P, maxshift, bpc);
+    DISPATCH_ROWS_T(rows, kb_ms1, T, dim3((unsigned)nb), dim3(kBlock), 0, st, g, U, r, p0, p1, t,
+                    M, P, maxshift, bpc);
+}
+void launch_b_ms_k1(const BGeo &g, int rows, const double2 *U, const double2 *r, double2 *p0,
+                    double2 *p1, double2 *t, MultiShiftState *M, BParts P, long n2max,
+                    int maxshift, hipStream_t st)
+{
+    b_ms_k1<double>(g, rows, U, r, p0, p1, t, M, P, n2max, maxshift, st);
+}
+void launch_b_ms_k1_f32(const BGeo &g, int rows, const float2 *U, const float2 *r, float2 *p0,
+                        float2 *p1, float2 *t, MultiShiftState *M, BParts P, long n2max,
+                        int maxshift, hipStream_t st)
+{
+    b_ms_k1<float>(g, rows, U, r, p0, p1, t, M, P, n2max, maxshift, st);
 }
 void launch_b_ms_k2(const BGeo &g, int rows, const double2 *U, const double2 *t,
                     const double2 *p0, const double2 *p1, double2 *r, MultiShiftState *M,
                     BParts P, hipStream_t st)
 {
-    DISPATCH_ROWS(rows, kb_ms2, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, p0, p1, r, M, P);
+    DISPATCH_ROWS_T(rows, kb_ms2, double, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, p0, p1,
+                    r, M, P);
+}
+void launch_b_ms_k2_f32(const BGeo &g, int rows, const float2 *U, const float2 *t,
+                        const float2 *p0, const float2 *p1, float2 *r, MultiShiftState *M,
+                        BParts P, hipStream_t st)
+{
+    DISPATCH_ROWS_T(rows, kb_ms2, float, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, p0, p1,
+                    r, M, P);
 }
 
 }  // namespace qed2d

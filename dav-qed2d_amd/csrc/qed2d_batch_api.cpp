@@ -47,6 +47,7 @@ struct qed2d_batch {
     int rows = 16, rows32 = 8, rows_q = 4;
     int rows_k2 = 0, rows32_k2 = 0;  // the second CG kernel's heights; 0: rows / rows32
     std::vector<double> mu2;  // per chain: the CG operator's shift (Q^2 + mu2); empty: 0
+    bool guess = false;       // the next solve starts from the x it is given
     double ru_delta = kRuDelta;
     std::vector<double> last_dh;  // per chain, from the double-double energies
     std::map<int, GeoBuf> geo;  // key: 2 * rows + pair
@@ -257,9 +258,8 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     BTRY(get_geo(b, rows64b, false, &g64b));
     BTRY(get_geo(b, b->rows_q, false, &gq));
     const bool mixed = (precision == QED2D_PREC_MIXED);
-    for (double m2 : b->mu2)
-        if (mixed && m2 != 0.0)
-            return bfail(b, QED2D_EINVAL, "shifted batched solves are FP64 only");
+    bool shifted = false;  // (the FP32 K2 reads p_k only for chains with a shift)
+    for (double m2 : b->mu2) shifted = shifted || m2 != 0.0;
     BGeo g32, g32b;
     if (mixed) {
         for (int c = 0; c < b->n; ++c)
@@ -300,6 +300,16 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
                 launch_link_pack(b->U, (float *)b->Us, b->nU, b->S, st);
             else
                 launch_to_f32(b->U, b->Us, b->nU, st);
+        }
+        if (b->guess) {
+            // from the guess in x: the true residual in FP64 (mixed: into rs,
+            // the FP32 partial solution xs starting at 0)
+            launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, nullptr, st);
+            launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, mixed ? nullptr : b->r,
+                              mixed ? b->rs : nullptr, mixed ? nullptr : b->p0,
+                              mixed ? b->p0s : nullptr, b->S, b->P, 0, nullptr, st);
+            if (mixed) BHIP(b, hipMemsetAsync(b->xs, 0, sizeof(float2) * nel, st));
+        } else if (mixed) {
             launch_b_cg_init(g64, b->rows, src, x, b->rs, b->p0s, b->xs, 1, b->P, st);
         } else {
             launch_b_cg_init(g64, b->rows, src, x, b->r, b->p0, nullptr, 0, b->P, st);
@@ -318,7 +328,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
                 launch_b_cg_k1_f32(g32, b->rows32, b->Us, packed, b->rs, b->p0s, b->p1s, b->ts,
                                    b->xs, b->S, b->P, st);
                 launch_b_cg_k2_f32(g32b, rows32b, b->Us, packed, b->ts, b->p0s, b->p1s, b->xs,
-                                   b->rs, b->S, b->P, st);
+                                   b->rs, b->S, b->P, shifted, st);
             } else {
                 launch_b_cg_k1_f64(g64, b->rows, b->U, b->r, b->p0, b->p1, b->t, b->S, b->P, st);
                 launch_b_cg_k2_f64(g64b, rows64b, b->U, b->t, b->p0, b->p1, x, b->r, b->S, b->P,
@@ -362,8 +372,8 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             launch_b_ru_accum(b->dch, b->n, b->vtot, x, b->xs, b->p0s, b->p1s, b->S, b->mask,
                               st);
             launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, b->mask, st);
-            launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, nullptr, b->rs, nullptr, b->S,
-                              b->P, 2, b->mask, st);
+            launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, nullptr, b->rs, nullptr, nullptr,
+                              b->S, b->P, 2, b->mask, st);
             BTRY(blaunched(b));
         }
         chunk = next;
@@ -858,7 +868,8 @@ int batch_rhs_cg(qed2d_batch *b, const double *U, double *x, const double *src, 
 int batch_rhs_cg_shifted(qed2d_batch *b, const double *U, double *x, const double *src,
                          const double *am0, const double *mu2, int maxiter, double eps,
                          int precision, int *iters, int rows_q, int rows, int rows32,
-                         int rows_k2, int rows32_k2, double ru_delta, std::string *err)
+                         int rows_k2, int rows32_k2, double ru_delta, std::string *err,
+                         bool guess)
 {
     b->rows_q = rows_q;
     b->rows = rows;
@@ -867,7 +878,9 @@ int batch_rhs_cg_shifted(qed2d_batch *b, const double *U, double *x, const doubl
     b->rows32_k2 = rows32_k2;
     b->ru_delta = ru_delta;
     if (mu2) b->mu2.assign(mu2, mu2 + b->n);
+    b->guess = guess;
     const int rc = qed2d_batch_cg_shared(b, U, x, src, am0, maxiter, eps, precision, iters);
+    b->guess = false;
     b->mu2.clear();
     if (rc && err) *err = b->err;
     return rc;

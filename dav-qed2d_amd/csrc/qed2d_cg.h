@@ -306,21 +306,28 @@ struct EpiResid {
 // and the batched kernels call them with the lattice, pointers and position
 // of the calling block).
 // ---------------------------------------------------------------------------
-struct EpiMS1 {
-    double2 *__restrict__ pnew;
-    double2 *__restrict__ t;
+template <typename T> __device__ __forceinline__ c2<T> *ms_vec(void *p)
+{
+    return static_cast<c2<T> *>(p);
+}
+
+// K1's epilogue: stores p_k and t = Q p_k, sums |t|^2 and |p_k|^2 (FP32
+// solves: each site's sum in FP32, accumulated in FP64).
+template <typename T> struct EpiMS1 {
+    c2<T> *__restrict__ pnew;
+    c2<T> *__restrict__ t;
     long V;
     double tt, pp;
     __device__ __forceinline__ NoPre pre(long) const { return {}; }
-    __device__ __forceinline__ void operator()(long s, NoPre, double2 o1, double2 o2, double2 c1,
-                                               double2 cc2)
+    __device__ __forceinline__ void operator()(long s, NoPre, c2<T> o1, c2<T> o2, c2<T> c1,
+                                               c2<T> cc2)
     {
         pnew[s] = c1;
         pnew[V + s] = cc2;
         t[s] = o1;
         t[V + s] = o2;
-        tt += (o1.x * o1.x + o1.y * o1.y) + (o2.x * o2.x + o2.y * o2.y);
-        pp += (c1.x * c1.x + c1.y * c1.y) + (cc2.x * cc2.x + cc2.y * cc2.y);
+        tt += (double)((o1.x * o1.x + o1.y * o1.y) + (o2.x * o2.x + o2.y * o2.y));
+        pp += (double)((c1.x * c1.x + c1.y * c1.y) + (cc2.x * cc2.x + cc2.y * cc2.y));
     }
 };
 
@@ -333,8 +340,8 @@ __device__ __forceinline__ void ms_shift_update(const MultiShiftState *S,
 {
     const int u = S->sh_u[k];
     if (u == 0) return;
-    double2 *__restrict__ ps = S->ps[k];
-    double2 *__restrict__ x = S->x[k];
+    double2 *__restrict__ ps = ms_vec<double>(S->ps[k]);
+    double2 *__restrict__ x = ms_vec<double>(S->x[k]);
     const double a = S->sh_a[k];
     const double2 pv = ps[i];
     const double2 xv = x[i];
@@ -345,7 +352,6 @@ __device__ __forceinline__ void ms_shift_update(const MultiShiftState *S,
         ps[i] = mk<double>(z * rv.x + b * pv.x, z * rv.y + b * pv.y);
     }
 }
-
 // Block-shared scalars of multi-shift K1(k).
 struct MS1Shared {
     double lds[kBlock / kWave + 1];
@@ -413,11 +419,11 @@ __device__ __forceinline__ int ms1_scalars(const MultiShiftState *S, const doubl
     return sh.anyleft ? ((k - 1 >= S->maxiter) ? 2 : 0) : 1;
 }
 
-template <int R>
-__device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restrict__ U,
-                                         const double2 *__restrict__ r, double2 *P0, double2 *P1,
-                                         double2 *__restrict__ t, double mass2,
-                                         MultiShiftState *S, double *Ptt_w, const double *Ptt_rd,
+template <typename T, int R>
+__device__ __forceinline__ void ms1_body(const Latt &L, const c2<T> *__restrict__ U,
+                                         const c2<T> *__restrict__ r, c2<T> *P0, c2<T> *P1,
+                                         c2<T> *__restrict__ t, T mass2, MultiShiftState *S,
+                                         double *Ptt_w, const double *Ptt_rd,
                                          const double *Prr_rd, int nrr, int ntt, int k,
                                          int nshift, const MarchPos &p)
 {
@@ -454,12 +460,12 @@ __device__ __forceinline__ void ms1_body(const Latt &L, const double2 *__restric
         }
     }
     if (stat == 2 || !sh.anyleft) return;  // (the last x updates are the shift updates')
-    const double2 *pold = rd ? P1 : P0;
-    double2 *pnew = w ? P1 : P0;
-    EpiMS1 epi{pnew, t, L.V, 0.0, 0.0};
+    const c2<T> *pold = rd ? P1 : P0;
+    c2<T> *pnew = w ? P1 : P0;
+    EpiMS1<T> epi{pnew, t, L.V, 0.0, 0.0};
     if (p.active) {
-        SrcXpay<double> src{r, pold, beta_j, L.V};
-        q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
+        SrcXpay<T> src{r, pold, (T)beta_j, L.V};
+        q_march<T, R>(L, U, mass2, src, epi, p.i1, p.r0);
     }
     double a, b;
     block_sum2_t0(epi.tt, epi.pp, a, b);
@@ -477,10 +483,11 @@ constexpr int kShiftEPT = 8;
 // stencil blocks do, and updates shift `shift` at spinor elements
 // e0 .. e0 + kShiftEPT * blockDim - 1 (of n2): x_i += alpha^i p_i, and
 // p_i = zeta r + beta^i p_i while the shift iterates (the expressions of
-// ms_shift_update).  K1 writes none of r, x_i, p_i, so these blocks may run
-// beside its stencil blocks.
+// ms_shift_update, with the coefficients rounded to T).  K1 writes none of
+// r, x_i, p_i, so these blocks may run beside its stencil blocks.
+template <typename T>
 __device__ __forceinline__ void ms1_shift_block(const MultiShiftState *S,
-                                                const double2 *__restrict__ r,
+                                                const c2<T> *__restrict__ r,
                                                 const double *Ptt_rd, const double *Prr_rd,
                                                 int nrr, int ntt, int k, int nshift, int shift,
                                                 long e0, long n2)
@@ -491,51 +498,53 @@ __device__ __forceinline__ void ms1_shift_block(const MultiShiftState *S,
                                  beta_j);
     const int u = sh.upd[shift];
     if (stat == 2 || u == 0) return;
-    double2 *__restrict__ ps = S->ps[shift];
-    double2 *__restrict__ x = S->x[shift];
-    const double a = sh.as[shift], z = sh.zj[shift], b = sh.bs[shift];
+    c2<T> *__restrict__ ps = ms_vec<T>(S->ps[shift]);
+    c2<T> *__restrict__ x = ms_vec<T>(S->x[shift]);
+    const T a = (T)sh.as[shift], z = (T)sh.zj[shift], b = (T)sh.bs[shift];
 #pragma unroll 4
     for (int j = 0; j < kShiftEPT; ++j) {
         const long i = e0 + (long)j * kBlock + threadIdx.x;
         if (i >= n2) break;
-        const double2 pv = ps[i];
-        const double2 xv = x[i];
-        x[i] = mk<double>(xv.x + a * pv.x, xv.y + a * pv.y);
+        const c2<T> pv = ps[i];
+        const c2<T> xv = x[i];
+        x[i] = mk<T>(xv.x + a * pv.x, xv.y + a * pv.y);
         if (u == 2) {
-            const double2 rv = r[i];
-            ps[i] = mk<double>(z * rv.x + b * pv.x, z * rv.y + b * pv.y);
+            const c2<T> rv = r[i];
+            ps[i] = mk<T>(z * rv.x + b * pv.x, z * rv.y + b * pv.y);
         }
     }
 }
 
-struct EpiMS2 {
-    double2 *__restrict__ r;
-    const double2 *__restrict__ p;
-    double alpha, sigma;
+// K2's epilogue: r -= alpha (Q t + sigma_b p), |r|^2 (FP32 solves: each
+// element's square in FP32, accumulated in FP64).
+template <typename T> struct EpiMS2 {
+    c2<T> *__restrict__ r;
+    const c2<T> *__restrict__ p;
+    T alpha, sigma;
     long V;
     double rr;
     struct Pre {
-        double2 p1, p2, r1, r2;
+        c2<T> p1, p2, r1, r2;
     };
     __device__ __forceinline__ Pre pre(long s) const { return Pre{p[s], p[V + s], r[s], r[V + s]}; }
-    __device__ __forceinline__ void one(long i, double2 o, double2 pv, double2 rv)
+    __device__ __forceinline__ void one(long i, c2<T> o, c2<T> pv, c2<T> rv)
     {
-        rv = mk<double>(rv.x - alpha * (o.x + sigma * pv.x), rv.y - alpha * (o.y + sigma * pv.y));
+        rv = mk<T>(rv.x - alpha * (o.x + sigma * pv.x), rv.y - alpha * (o.y + sigma * pv.y));
         r[i] = rv;
-        rr += rv.x * rv.x + rv.y * rv.y;
+        rr += (double)(rv.x * rv.x + rv.y * rv.y);
     }
-    __device__ __forceinline__ void operator()(long s, const Pre &q, double2 o1, double2 o2,
-                                               double2, double2)
+    __device__ __forceinline__ void operator()(long s, const Pre &q, c2<T> o1, c2<T> o2, c2<T>,
+                                               c2<T>)
     {
         one(s, o1, q.p1, q.r1);
         one(V + s, o2, q.p2, q.r2);
     }
 };
 
-template <int R>
-__device__ __forceinline__ void ms2_body(const Latt &L, const double2 *__restrict__ U,
-                                         const double2 *__restrict__ t, const double2 *P0,
-                                         const double2 *P1, double2 *__restrict__ r, double mass2,
+template <typename T, int R>
+__device__ __forceinline__ void ms2_body(const Latt &L, const c2<T> *__restrict__ U,
+                                         const c2<T> *__restrict__ t, const c2<T> *P0,
+                                         const c2<T> *P1, c2<T> *__restrict__ r, T mass2,
                                          MultiShiftState *S, const double *Ptt_w, double *Prr_w,
                                          int ntt, int k, const MarchPos &p)
 {
@@ -548,7 +557,7 @@ __device__ __forceinline__ void ms2_body(const Latt &L, const double2 *__restric
     const bool ahead = ntt <= kPartMax;
     PartialsAhead2 ptt;
     if (ahead) ptt.load(Ptt_w, ntt);
-    EpiMS2 epi{r, w ? P1 : P0, 0.0, sb, L.V, 0.0};
+    EpiMS2<T> epi{r, w ? P1 : P0, T(0), (T)sb, L.V, 0.0};
     auto ready = [&]() -> bool {
         double tt, pp;
         if (ahead) {
@@ -557,11 +566,11 @@ __device__ __forceinline__ void ms2_body(const Latt &L, const double2 *__restric
             tt = sum_prev_partials(Ptt_w, ntt, 2, 0, lds);
             pp = sum_prev_partials(Ptt_w, ntt, 2, 1, lds);
         }
-        epi.alpha = nw / (tt + sb * pp);
+        epi.alpha = (T)(nw / (tt + sb * pp));
         return true;
     };
-    SrcPlain<double> src{t, L.V};
-    q_march_r<double, R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready);
+    SrcPlain<T> src{t, L.V};
+    q_march_r<T, R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready);
     const double a = block_sum_t0(epi.rr);
     if (threadIdx.x == 0) Prr_w[p.blk] = a;
 }

@@ -105,8 +105,10 @@ struct MultiShiftState {
     double s[kMaxShift];             // sigma_i - sigma_b
     int iter, maxiter, status, nshift, stop_k, pad;
     int conv_iter[kMaxShift];
-    double2 *x[kMaxShift];           // solution of shift i
-    double2 *ps[kMaxShift];          // search direction of shift i
+    // solution and search direction of shift i: double2 arrays, float2 in the
+    // FP32 solves (ms_vec<T> gives the typed pointer)
+    void *x[kMaxShift];
+    void *ps[kMaxShift];
     const double2 *b;                // batched solves: the right-hand side
     int kk, k2;                      // batched solves: iteration K1 runs next / K2 runs
     // the shift updates of the step K1 last ran (written by its block 0, applied
@@ -195,10 +197,12 @@ int batch_rhs_cg(qed2d_batch *b, const double *U, double *x, const double *src, 
                  int maxiter, double eps, int precision, int *iters, int rows_q, int rows,
                  int rows32, int rows_k2, int rows32_k2, double ru_delta, std::string *err);
 // as batch_rhs_cg, with a mass and a shift (Q^2 + mu2[c], null: 0) per right-hand side
+// (guess: each solve starts from the x it is given, else from 0)
 int batch_rhs_cg_shifted(qed2d_batch *b, const double *U, double *x, const double *src,
                          const double *am0, const double *mu2, int maxiter, double eps,
                          int precision, int *iters, int rows_q, int rows, int rows32,
-                         int rows_k2, int rows32_k2, double ru_delta, std::string *err);
+                         int rows_k2, int rows32_k2, double ru_delta, std::string *err,
+                         bool guess = false);
 // mask (may be null): per chain, nonzero = apply on that chain.
 void launch_b_q_apply(const BGeo &g, int rows, const double2 *U, const double2 *si,
                       double2 *so, const int *mask, hipStream_t st);
@@ -209,7 +213,8 @@ void launch_b_cg_init(const BGeo &g, int rows, const double2 *b, double2 *x, voi
                       float2 *xs, int mixed, BParts P, hipStream_t st);
 void launch_b_cg_resid(const BGeo &g, int rows, const double2 *U, const double2 *b,
                        const double2 *t, const double2 *x, double2 *r, float2 *rs, double2 *p0,
-                       SolverState *S, BParts P, int mode, const int *mask, hipStream_t st);
+                       float2 *p0s, SolverState *S, BParts P, int mode, const int *mask,
+                       hipStream_t st);
 void launch_b_cg_k1_f64(const BGeo &g, int rows, const double2 *U, const double2 *r,
                         double2 *p0, double2 *p1, double2 *t, SolverState *S, BParts P,
                         hipStream_t st);
@@ -221,7 +226,7 @@ void launch_b_cg_k1_f32(const BGeo &g, int rows, const void *U, bool packed, con
                         hipStream_t st);
 void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *t,
                         const float2 *p0, const float2 *p1, float2 *x, float2 *r,
-                        SolverState *S, BParts P, hipStream_t st);
+                        SolverState *S, BParts P, bool mu, hipStream_t st);
 // (each masked chain's pending FP32 x update applied first, as launch_ru_accum)
 void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,
                        const float2 *p0, const float2 *p1, const SolverState *S,
@@ -235,6 +240,15 @@ void launch_b_ms_k1(const BGeo &g, int rows, const double2 *U, const double2 *r,
 void launch_b_ms_k2(const BGeo &g, int rows, const double2 *U, const double2 *t,
                     const double2 *p0, const double2 *p1, double2 *r, MultiShiftState *M,
                     BParts P, hipStream_t st);
+// the same in FP32 (spinors, links, x_i and p_i; scalars and sums in FP64)
+void launch_b_ms_init_f32(const BGeo &g, int rows, float2 *r, float2 *p0,
+                          const MultiShiftState *M, BParts P, hipStream_t st);
+void launch_b_ms_k1_f32(const BGeo &g, int rows, const float2 *U, const float2 *r, float2 *p0,
+                        float2 *p1, float2 *t, MultiShiftState *M, BParts P, long n2max,
+                        int maxshift, hipStream_t st);
+void launch_b_ms_k2_f32(const BGeo &g, int rows, const float2 *U, const float2 *t,
+                        const float2 *p0, const float2 *p1, float2 *r, MultiShiftState *M,
+                        BParts P, hipStream_t st);
 
 // ---- launches (all asynchronous on `st`) ---------------------------------
 void launch_q_apply_f64(const Latt &L, const double2 *U, const double2 *si, double2 *so,
@@ -311,6 +325,7 @@ constexpr int kPartsUnfolded = 1024;
 int resid_blocks(const Latt &L, int rows);
 int cg2_f32_blocks(const Latt &L, int rows);
 void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st);
+void launch_to_f64(const float2 *in, double2 *out, long n, hipStream_t st);
 
 // ---- multi-shift CG --------------------------------------------------------
 // r = b, p0 = 0, x_i = 0, p_i = 0, |b|^2 partials into rr[0].

@@ -267,6 +267,18 @@ void launch_to_f32(const double2 *in, float2 *out, long n, hipStream_t st)
     launch_k(k_to_f32, dim3(ew_grid(n)), dim3(kBlock), 0, st, in, out, n);
 }
 
+// out = in widened to FP64 (exact)
+__global__ void k_to_f64(const float2 *__restrict__ in, double2 *__restrict__ out, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        out[i] = mk<double>((double)in[i].x, (double)in[i].y);
+}
+void launch_to_f64(const float2 *in, double2 *out, long n, hipStream_t st)
+{
+    launch_k(k_to_f64, dim3(ew_grid(n)), dim3(kBlock), 0, st, in, out, n);
+}
+
 // ===========================================================================
 // CG iteration kernels
 // ===========================================================================
@@ -617,7 +629,8 @@ __global__ void __launch_bounds__(kBlock) k_ms1(Latt L, const double2 *__restric
     const int sk = S->stop_k;
     if (sk != 0 && sk < k) return;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    ms1_body<R>(L, U, r, P0, P1, t, mass2, S, Ptt_w, Ptt_rd, Prr_rd, nrr, ntt, k, nshift, p);
+    ms1_body<double, R>(L, U, r, P0, P1, t, mass2, S, Ptt_w, Ptt_rd, Prr_rd, nrr, ntt, k, nshift,
+                        p);
 }
 
 template <int R>
@@ -631,7 +644,7 @@ __global__ void __launch_bounds__(kBlock) k_ms2(Latt L, const double2 *__restric
     const int sk = S->stop_k;
     if (sk != 0 && sk <= k) return;
     const MarchPos p = march_pos(L, bx, by, nbx, R);
-    ms2_body<R>(L, U, t, P0, P1, r, mass2, S, Ptt_w, Prr_w, ntt, k, p);
+    ms2_body<double, R>(L, U, t, P0, P1, r, mass2, S, Ptt_w, Prr_w, ntt, k, p);
 }
 
 // The shift updates of step k, after K1(k) committed their coefficients: grid
@@ -673,8 +686,8 @@ __global__ void __launch_bounds__(kBlock) k_ms_init(Latt L, const double2 *__res
                 r[i] = v;
                 p0[i] = z;
                 for (int k = 0; k < nshift; ++k) {
-                    S->x[k][i] = z;
-                    S->ps[k][i] = z;
+                    ms_vec<double>(S->x[k])[i] = z;
+                    ms_vec<double>(S->ps[k])[i] = z;
                 }
                 acc += v.x * v.x + v.y * v.y;
             }

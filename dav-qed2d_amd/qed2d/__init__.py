@@ -88,6 +88,7 @@ SIGNATURES = {
     "qed2d_hbm_stream": (_i, [_p, C.c_long, _p]),
     "qed2d_ctx_set_reliable_delta": (_i, [_p, _d]),
     "qed2d_ctx_set_reliable_mode": (_i, [_p, _i]),
+    "qed2d_ctx_set_rhmc_precision": (_i, [_p, _i]),
     "qed2d_synchronize": (_i, [_p]),
     "qed2d_gamm5Dw": (_i, [_p, _p, _p, _p, _d]),
     "qed2d_gamm5Dw_f32": (_i, [_p, _p, _p, _p, _f]),
@@ -618,6 +619,15 @@ def set_reliable_mode(prm, on_device):
     _check(prm, lib.qed2d_ctx_set_reliable_mode(prm.ctx(), 1 if on_device else 0))
 
 
+def set_rhmc_precision(prm, precision):
+    """Precision of the RHMC force's solves (PREC_F64, the default, or
+    PREC_MIXED: FP32 multi-shift, then each pole from that guess by a batched
+    mixed-precision CG to the FP64 stopping rule); see
+    qed2d_ctx_set_rhmc_precision.  No counterpart in the reference."""
+    _check(prm, lib.qed2d_ctx_set_rhmc_precision(prm.ctx(), int(precision)))
+    prm._rhmc_prec = int(precision)
+
+
 def set_multishift(prm, on):
     """MultiCG / R / RHMC force through multi-shift CG (True or 1: all
     pseudofermions of the RHMC force in one solve; 2: one pseudofermion at a
@@ -769,7 +779,7 @@ class TrajInfo:
 
 
 def HMC(U, *args, integrator=None, qzero=False, reversibility_test=False,
-        precision=PREC_F64, acc_tol=0.0, traj=None):
+        precision=None, acc_tol=0.0, traj=None):
     """HMC! (src/QED2dHMC.jl).  Three methods, as the reference:
 
     * pure gauge ``HMC(U, eps, ns, acc, prm, kprm)``                  (:19-60)
@@ -780,6 +790,8 @@ def HMC(U, *args, integrator=None, qzero=False, reversibility_test=False,
     ``acc`` is appended with 1/0 (accept/reject) as the reference's
     ``push!(acc, ...)``.  Random numbers come from the stream set by
     ``seed()``; each call consumes one trajectory index.  Returns TrajInfo.
+    ``precision`` selects the MD force's solves: Nf=2 (default PREC_F64) and,
+    for this call, RHMC (default: the context's, ``set_rhmc_precision``).
     """
     t = _rng.next_traj() if traj is None else int(traj)
     hi, hf, acc_o, it = _d(), _d(), _i(), _i()
@@ -792,8 +804,9 @@ def HMC(U, *args, integrator=None, qzero=False, reversibility_test=False,
     elif len(args) == 8:
         am0, eps, ns, acc, CGmaxiter, tol, prm, kprm = args
         integ = integrator or Leapfrog()
+        prec = PREC_F64 if precision is None else precision
         _check(prm, lib.qed2d_hmc_nf2(prm.ctx(), _sp(prm, U), am0, eps, int(ns), integ.code,
-                                      int(CGmaxiter), tol, 1 if qzero else 0, precision, acc_tol,
+                                      int(CGmaxiter), tol, 1 if qzero else 0, prec, acc_tol,
                                       _rng.seed, _rng.chain, t, C.byref(hi), C.byref(hf),
                                       C.byref(acc_o), C.byref(it)))
     elif len(args) == 9:
@@ -812,11 +825,18 @@ def HMC(U, *args, integrator=None, qzero=False, reversibility_test=False,
             nus[j] = C.cast(b, _p)
             rhos[j] = C.cast(c, _p)
         npole = (_i * nf)(*[r.n for r in rprms])
-        _check(prm, lib.qed2d_hmc_rhmc(prm.ctx(), _sp(prm, U), _arr(am0s), nf, npole, mus, nus,
-                                       rhos, eps, int(ns), integ.code, int(CGmaxiter), tol,
-                                       1 if qzero else 0, 1 if reversibility_test else 0,
-                                       _rng.seed, _rng.chain, t, C.byref(hi), C.byref(hf),
-                                       C.byref(acc_o)))
+        keep_prec = getattr(prm, "_rhmc_prec", PREC_F64)
+        if precision is not None:
+            _check(prm, lib.qed2d_ctx_set_rhmc_precision(prm.ctx(), int(precision)))
+        try:
+            _check(prm, lib.qed2d_hmc_rhmc(prm.ctx(), _sp(prm, U), _arr(am0s), nf, npole, mus,
+                                           nus, rhos, eps, int(ns), integ.code, int(CGmaxiter),
+                                           tol, 1 if qzero else 0, 1 if reversibility_test else 0,
+                                           _rng.seed, _rng.chain, t, C.byref(hi), C.byref(hf),
+                                           C.byref(acc_o)))
+        finally:
+            if precision is not None:
+                lib.qed2d_ctx_set_rhmc_precision(prm.ctx(), int(keep_prec))
     else:
         raise TypeError("HMC takes 5 (pure gauge), 8 (Nf=2) or 9 (RHMC) positional arguments "
                         "after U")

@@ -125,6 +125,16 @@ int qed2d_ctx_set_reliable_delta(qed2d_ctx *ctx, double delta);
  * only).  Both stop on the same FP64 true residual.  No counterpart in the
  * reference, whose CG is FP64 only (src/QED2dSolver.jl:3-105). */
 int qed2d_ctx_set_reliable_mode(qed2d_ctx *ctx, int on_device);
+/* Precision of the RHMC force's solves with the multi-shift force of all
+ * pseudofermions (qed2d_ctx_set_multishift 1, the default).  QED2D_PREC_F64
+ * (default): one FP64 multi-shift solve, as MultiCG (src/QED2dRHMC.jl:281-290).
+ * QED2D_PREC_MIXED: one FP32 multi-shift solve of every pole to |r|^2 < 1e-10
+ * |phi|^2 (or the solve's tolerance, if looser), then every pole's system
+ * (Q^2 + mu^2) x = phi solved from that guess by one batched mixed-precision
+ * CG (FP32 iterations, FP64 reliable updates) to the FP64 stopping rule; needs
+ * an even L1 >= 4.  The heatbath and the acceptance step stay FP64.  No
+ * counterpart in the reference, whose solvers are FP64 only. */
+int qed2d_ctx_set_rhmc_precision(qed2d_ctx *ctx, int precision);
 int qed2d_synchronize(qed2d_ctx *ctx);
 
 /* ---- Dirac operator -------------------------------------------------------- */

@@ -356,8 +356,12 @@ def test_hmc_rhmc_reversibility(Q, O):
     assert abs(Q.Action(dU, prm) - S0) < 1e-10
 
 
-def test_hmc_rhmc_vs_oracle(Q, O):
-    say("RHMC vs oracle")
+@pytest.mark.parametrize("precision", [0, 1])
+def test_hmc_rhmc_vs_oracle(Q, O, precision):
+    """Two RHMC trajectories against the oracle; precision 1: the force's
+    solves in mixed precision (FP32 multi-shift, then every pole by the
+    batched mixed-precision CG from that guess), at the FP64 tolerances."""
+    say("RHMC vs oracle, precision", precision)
     L1, L2 = 12, 12
     prm = Q.LattParm((L1, L2), 4.0)
     U = O.hot_start(L1, L2, 99)
@@ -367,7 +371,8 @@ def test_hmc_rhmc_vs_oracle(Q, O):
     rp = Q.get_rhmc_params(6, np.sqrt(0.05), np.sqrt(20.0))
     Q.seed(99)
     for t in range(2):
-        info = Q.HMC(dU, [0.5, 0.7], 0.1, 5, [], 10000, 1e-24, prm, None, [rp, rp])
+        info = Q.HMC(dU, [0.5, 0.7], 0.1, 5, [], 10000, 1e-24, prm, None, [rp, rp],
+                     precision=precision)
         hi, hf, a, rc = O.hmc_rhmc(U, [0.5, 0.7], [z, z], 0.1, 5, 10000, 1e-24, p, 99, t)
         assert rc == 0
         assert abs(info.hini - hi) < 1e-9

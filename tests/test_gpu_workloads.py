@@ -242,16 +242,25 @@ def test_rhmc_128_sixteen_poles_four_pseudofermions(Q, O):
         out[ms] = Q.HMC(U, [0.1] * 4, 0.083, 12, [], 100000, 1e-22, prm, None, [rp] * 4,
                         integrator=Q.OMF4())
     Q.set_multishift(prm, 1)
-    say("dH multishift", out[1].dH, "sequential", out[0].dH)
+    # the force's solves in mixed precision (FP32 multi-shift, then every pole
+    # by the batched mixed-precision CG) to the same FP64 tolerance
+    U.copy_(U0)
+    Q.seed(77)
+    out["mixed"] = Q.HMC(U, [0.1] * 4, 0.083, 12, [], 100000, 1e-22, prm, None, [rp] * 4,
+                         integrator=Q.OMF4(), precision=Q.PREC_MIXED)
+    say("dH multishift", out[1].dH, "sequential", out[0].dH, "mixed", out["mixed"].dH)
     assert abs(out[1].dH) < 1.0
-    assert out[1].accepted == out[0].accepted
+    assert out[1].accepted == out[0].accepted == out["mixed"].accepted
     assert abs(out[1].dH - out[0].dH) < 1e-8
+    assert abs(out["mixed"].dH - out[1].dH) < 1e-8
 
 
-def test_rhmc_4pf_16poles_omf4_vs_oracle(Q, O):
+@pytest.mark.parametrize("precision", [0, 1])
+def test_rhmc_4pf_16poles_omf4_vs_oracle(Q, O, precision):
     """The RHMC configuration's structure (16 poles, 4 pseudofermions, OMF4)
-    against the oracle on a 16x16 lattice, one trajectory of 3 steps."""
-    say("RHMC 16 poles 4 pf OMF4 vs oracle")
+    against the oracle on a 16x16 lattice, one trajectory of 3 steps; precision
+    1: the force's solves in mixed precision, at the FP64 tolerances."""
+    say("RHMC 16 poles 4 pf OMF4 vs oracle, precision", precision)
     L = 16
     prm = Q.LattParm((L, L), 4.0)
     U = O.hot_start(L, L, 5)
@@ -261,7 +270,7 @@ def test_rhmc_4pf_16poles_omf4_vs_oracle(Q, O):
     z = O.zolotarev(16, np.sqrt(0.02), np.sqrt(70.0))
     Q.seed(5)
     info = Q.HMC(dU, [0.1] * 4, 0.083, 3, [], 100000, 1e-24, prm, None, [rp] * 4,
-                 integrator=Q.OMF4())
+                 integrator=Q.OMF4(), precision=precision)
     hi, hf, a, rc = O.hmc_rhmc(U, [0.1] * 4, [z] * 4, 0.083, 3, 100000, 1e-24, p, 5, 0,
                                integrator=1)
     assert rc == 0
