@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--accept-traj", type=int, default=100,
                     help="trajectories of the acceptance comparison (0: skip)")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--rhmc-traj", type=int, default=20,
+                    help="trajectories of the RHMC 128^2 Markov chain, per force precision")
     ap.add_argument("--tiles", type=str, default="",
                     help="q64,q32,cg64,cg32[,cg64_k2,cg32_k2] tile heights (skips the "
                          "autotuner)")
@@ -231,7 +233,7 @@ def main():
     if not args.no_dw:
         res["dw_apply"] = dw_apply(Q, torch, args)
     if not args.no_configs:
-        res["configs"] = config_measurements(Q, torch, pool)
+        res["configs"] = config_measurements(Q, torch, pool, args)
 
     # ---- CPU oracle on the host cores: whole trajectories, never scaled -----
     cpu = None
@@ -394,7 +396,7 @@ def ensemble(Q, torch, ntraj=10, nseq=3):
     return out
 
 
-def config_measurements(Q, torch, pool):
+def config_measurements(Q, torch, pool, args):
     """The other workloads of the path, each timed on the GPU; their inputs go
     to the CPU oracle jobs (cpu_baselines), which time the same work."""
     import numpy as np
@@ -485,11 +487,34 @@ def config_measurements(Q, torch, pool):
     rp = Q.get_rhmc_params(16, ra, rb)
     Uc = U.clone()
     traj0 = Q.rng_state()[2]
-    dt, infos = sync_time(lambda: [Q.HMC(U, [0.1] * 4, 0.083, 12, [], 100000, 1e-16, prm, None,
-                                         [rp] * 4, integrator=Q.OMF4()) for _ in range(2)])
-    out["rhmc_128x128_16poles_4pf"] = {
-        "gpu_traj_per_s": round(2 / dt, 3), "dH": [round(i.dH, 8) for i in infos],
-        "delta": rp.delta, "_gpu": {"acc": [infos[0].accepted], "dH": [infos[0].dH]}}
+    # a Markov chain of args.rhmc_traj trajectories from the same start and
+    # random streams with the force's solves in FP64 and in mixed precision
+    # (FP32 multi-shift, then every pole by the batched mixed-precision CG to
+    # the same FP64 tolerance; heatbath and acceptance FP64 in both)
+    nt = args.rhmc_traj
+    c3 = {"trajectories": nt, "cg_tol": 1e-16, "delta": rp.delta}
+    runs = {}
+    for prec, key in ((Q.PREC_F64, "fp64"), (Q.PREC_MIXED, "mixed")):
+        U.copy_(Uc)
+        Q.seed(77)
+        dt, infos = sync_time(lambda: [
+            Q.HMC(U, [0.1] * 4, 0.083, 12, [], 100000, 1e-16, prm, None, [rp] * 4,
+                  integrator=Q.OMF4(), precision=prec, traj=traj0 + i) for i in range(nt)])
+        runs[key] = infos
+        dh = np.array([i.dH for i in infos])
+        c3[key] = {"traj_per_s": round(nt / dt, 3),
+                   "acceptance": round(float(np.mean([i.accepted for i in infos])), 4),
+                   "exp_minus_dH_mean": round(float(np.mean(np.exp(-dh))), 4),
+                   "exp_minus_dH_stderr": round(float(np.std(np.exp(-dh)) / np.sqrt(nt)), 4),
+                   "accepted": [i.accepted for i in infos],
+                   "dH": [round(float(x), 8) for x in dh]}
+    c3["gpu_traj_per_s"] = c3["mixed"]["traj_per_s"]  # (the config's FP32 force solves)
+    c3["mixed_vs_fp64"] = round(c3["mixed"]["traj_per_s"] / c3["fp64"]["traj_per_s"], 3)
+    c3["accept_identical_vs_fp64"] = c3["mixed"]["accepted"] == c3["fp64"]["accepted"]
+    c3["max_abs_dH_diff_vs_fp64"] = max(abs(a.dH - b.dH) for a, b in
+                                         zip(runs["mixed"], runs["fp64"]))
+    c3["_gpu"] = {"acc": [runs["fp64"][0].accepted], "dH": [runs["fp64"][0].dH]}
+    out["rhmc_128x128_16poles_4pf"] = c3
     if pool is not None:
         pool.submit("rhmc_128", U=Uc.cpu().numpy(), r_a=float(ra), r_b=float(rb), traj=traj0)
     prm.close()
@@ -513,7 +538,8 @@ def config_measurements(Q, torch, pool):
     torch.cuda.empty_cache()
 
     # Nf=2 4096^2, beta 11.25, m0 0.05, links after 3 pure-gauge trajectories
-    # from a cold start; one leapfrog trajectory ns 2 eps 0.05, CG tol 1e-20
+    # from a cold start; one OMF4 trajectory ns 2 eps 0.05, CG tol 1e-20 (a
+    # leapfrog trajectory of the same length violates the energy by ~4e4)
     L = 4096
     prm = Q.LattParm((L, L), 11.25)
     U = Q.cold_links(prm)
@@ -526,7 +552,7 @@ def config_measurements(Q, torch, pool):
         U.copy_(U0)
         Q.seed(55)
         dt, info = sync_time(lambda: Q.HMC(U, 0.05, 0.05, 2, [], 100000, 1e-20, prm, None,
-                                           precision=prec, acc_tol=1e-20))
+                                           integrator=Q.OMF4(), precision=prec, acc_tol=1e-20))
         c6[key] = {"seconds": round(dt, 3), "cg_iters": info.cg_iters, "dH": info.dH,
                    "accepted": info.accepted}
     c6["mixed_vs_fp64"] = round(c6["fp64"]["seconds"] / c6["mixed"]["seconds"], 3)

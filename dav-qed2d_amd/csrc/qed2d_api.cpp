@@ -2513,13 +2513,15 @@ int qed2d_power_method(qed2d_ctx *c, const double *U, const double *b0, double a
     hipStream_t st = c->stream;
     const double shift = 2.0;
     launch_copy(b, (const double2 *)b0, n2, st);
-    double re, im;
-    // src/QED2dRHMC.jl:144-149
+    double im;
+    // src/QED2dRHMC.jl:144-149; the normalisation reads dot(ba, ba) on the
+    // device (the iterations queue without a host round trip)
     for (int i = 0; i < iter; ++i) {
         qsq64(c, U, (const double *)b, (double *)ba, (double *)tmp, am0, 0.0);
         launch_scale_add(ba, 1.0, b, shift, n2, st);
-        TRY(dot_host(c, (const double *)ba, (const double *)ba, &re, &im));
-        launch_scale_add(b, 0.0, ba, 1.0 / re, n2, st);
+        launch_dot(ba, ba, n2, c->red, c->part, st);
+        launch_scale_add_inv(b, 0.0, ba, 0.0, c->red, 1, n2, st);  // b = ba / dot(ba, ba)
+        if ((i & 255) == 255) TRY(launched(c));
     }
     qsq64(c, U, (const double *)b, (double *)ba, (double *)tmp, am0, 0.0);
     launch_scale_add(ba, 1.0, b, shift, n2, st);
@@ -2532,8 +2534,9 @@ int qed2d_power_method(qed2d_ctx *c, const double *U, const double *b0, double a
         launch_copy(bl, b, n2, st);
         qsq64(c, U, (const double *)bl, (double *)b, (double *)tmp, am0, 0.0);
         launch_scale_add(b, 1.0, bl, -lmax, n2, st);
-        TRY(dot_host(c, (const double *)b, (const double *)b, &re, &im));
-        launch_scale_add(b, 1.0 / re, b, 0.0, n2, st);
+        launch_dot(b, b, n2, c->red, c->part, st);
+        launch_scale_add_inv(b, 0.0, b, 0.0, c->red, 0, n2, st);  // b = b / dot(b, b)
+        if ((i & 255) == 255) TRY(launched(c));
     }
     qsq64(c, U, (const double *)b, (double *)ba, (double *)tmp, am0, 0.0);
     launch_scale_add(ba, 1.0, b, -lmax, n2, st);

@@ -377,6 +377,9 @@ void launch_force_acc(const Latt &L, const double2 *U, const double2 *X, const d
 void launch_axpy_c(double2 *y, const double2 *x, double ar, double ai, long n, hipStream_t st);
 void launch_scale_add(double2 *y, double a, const double2 *x, double b, long n, hipStream_t st);
 void launch_copy(double2 *dst, const double2 *src, long n, hipStream_t st);
+// y = a y + b x, the coefficient `which` (0: a, 1: b) being 1 / red->v[0]
+void launch_scale_add_inv(double2 *y, double a, const double2 *x, double b, const RedOut *red,
+                          int which, long n, hipStream_t st);
 void launch_negate(double *a, long n, hipStream_t st);
 
 // RNG fills.

@@ -313,6 +313,29 @@ void launch_scale_add(double2 *y, double a, const double2 *x, double b, long n, 
     launch_k(k_scale_add, dim3(ew_blocks(n)), dim3(kBlock), 0, st, y, a, x, b, n);
 }
 
+// y = a y + b x with one coefficient (which = 0: a, 1: b) replaced by
+// 1 / red->v[0], the result of a reduction still on the device: the power
+// method's normalisation without a host read.  The same expression as
+// k_scale_add (and 1 / v[0] rounds as on the host).
+__global__ void k_scale_add_inv(double2 *y, double a, const double2 *x, double b,
+                                const RedOut *red, int which, long n)
+{
+    const double s = 1.0 / red->v[0];
+    if (which == 0) a = s;
+    else b = s;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        double2 yv = y[i], xv = x[i];
+        y[i] = mk<double>(a * yv.x + b * xv.x, a * yv.y + b * xv.y);
+    }
+}
+void launch_scale_add_inv(double2 *y, double a, const double2 *x, double b, const RedOut *red,
+                          int which, long n, hipStream_t st)
+{
+    launch_k(k_scale_add_inv, dim3(ew_blocks(n)), dim3(kBlock), 0, st, y, a, x, b, red, which,
+             n);
+}
+
 __global__ void k_copy(double2 *__restrict__ d, const double2 *__restrict__ s, long n)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
