@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--accept-traj", type=int, default=100,
                     help="trajectories of the acceptance comparison (0: skip)")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--leg", choices=["ensemble", "rhmc", "tolerance"], default=None,
+                    help="run one side measurement alone and print its JSON (for profiling)")
     ap.add_argument("--rhmc-traj", type=int, default=20,
                     help="trajectories of the RHMC 128^2 Markov chain, per force precision")
     ap.add_argument("--tiles", type=str, default="",
@@ -110,6 +112,18 @@ def main():
         sprm.close()
         print(json.dumps({"hbm_stream_GBs": {k: round(v, 1) for k, v in stream.items()},
                           "dw_apply": dw_apply(Q, torch, args)}), flush=True)
+        return
+    if args.leg == "ensemble":
+        print(json.dumps(ensemble(Q, torch)), flush=True)
+        return
+    if args.leg == "tolerance":
+        print(json.dumps(tolerance_scan(Q, torch)), flush=True)
+        return
+    if args.leg == "rhmc":
+        r = rhmc_config(Q, torch, args, None)
+        r.pop("_gpu", None)
+        log("rhmc: " + json.dumps(r))
+        print(json.dumps(r), flush=True)
         return
     L = args.L
     prm = Q.LattParm((L, L), args.beta)
@@ -153,6 +167,10 @@ def main():
         torch.cuda.synchronize()
         if profile:
             Q.profile_enable(prm, True, every=1)
+        # (a one-element fill before and after the timed trajectories marks
+        # their window in a kernel trace: bench.py --rocprof-summary --windows)
+        mark = torch.empty(1, dtype=torch.int16, device="cuda")
+        mark.fill_(0)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(steps):
@@ -160,6 +178,7 @@ def main():
                                precision=precision, acc_tol=args.tol))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
+        mark.fill_(1)
         prof = None
         if profile:
             prof = Q.profile_read(prm)
@@ -363,15 +382,18 @@ def ensemble(Q, torch, ntraj=10, nseq=3):
     log(f"ensemble: autotuned tiles {tiles} in {time.perf_counter() - t0:.2f}s")
     out = {"chains": n, "sites": B.sites, "trajectories_timed": ntraj,
            "sequential_trajectories_timed": nseq, "tiles": tiles}
+    mark = torch.empty(1, dtype=torch.int16, device="cuda")  # (trace windows, as run())
     for prec, key in ((Q.PREC_MIXED, "mixed"), (Q.PREC_F64, "fp64")):
         for c in range(n):
             B.set_links(c, U0[c])
+        mark.fill_(0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for t in range(ntraj):
             B.HMC(am0s, 0.1, 10, 100000, 1e-16, seeds, 2 + t, precision=prec)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
+        mark.fill_(1)
         out[f"{key}_chain_traj_per_s"] = round(n * ntraj / dt, 2)
         log(f"ensemble {key}: {n} chains x {ntraj} traj in {dt:.3f}s")
     # the same chains one at a time (single-lattice entry point, mixed)
@@ -394,6 +416,143 @@ def ensemble(Q, torch, ntraj=10, nseq=3):
         p.close()
     B.close()
     return out
+
+
+def tolerance_scan(Q, torch):
+    """|dH_mixed - dH_fp64| against the CG tolerance (squared relative
+    residual) on the same start and random streams: the 1024^2 accepting chain
+    (leapfrog ns 50 eps 0.02, 3 trajectories), the 64-chain ensemble (one
+    trajectory, the largest difference over the chains) and Nf=2 4096^2 (OMF4
+    ns 2 eps 0.05, one trajectory).  Shows the tolerance at which the
+    difference falls below 1e-9."""
+    out = {}
+    L = 1024
+    prm = Q.LattParm((L, L), 4.0)
+    U = Q.spinor(prm)
+    Q.hot_start(U, prm, 1234)
+    Q.seed(1234, chain=1)
+    for _ in range(200):
+        Q.HMC(U, 0.1, 10, [], prm, None)
+    U0 = U.clone()
+    r = {}
+    for tol in (1e-16, 1e-18, 1e-20, 1e-22, 1e-24):
+        dh = {}
+        for prec in (Q.PREC_MIXED, Q.PREC_F64):
+            U.copy_(U0)
+            Q.seed(1234, chain=5)
+            dh[prec] = [Q.HMC(U, 0.2, 0.02, 50, [], 100000, tol, prm, None, precision=prec,
+                              acc_tol=tol).dH for _ in range(3)]
+        r[f"{tol:.0e}"] = max(abs(a - b) for a, b in zip(dh[Q.PREC_MIXED], dh[Q.PREC_F64]))
+        log(f"tolerance 1024^2 tol {tol:.0e}: max |ddH| {r[f'{tol:.0e}']:.2e}")
+    out["nf2_1024_leapfrog_ns50_eps0.02"] = r
+    del U, U0
+    prm.close()
+    # the ensemble
+    sizes = [(16, 16)] * 16 + [(32, 32)] * 16 + [(64, 64)] * 16 + [(128, 128)] * 8 + \
+            [(256, 256)] * 8
+    n = len(sizes)
+    betas = [(2.0, 4.0, 6.0)[c % 3] for c in range(n)]
+    am0s = [(0.1, 0.2)[(c // 3) % 2] for c in range(n)]
+    seeds = [1000 + c for c in range(n)]
+    B = Q.Batch(sizes, betas)
+    B.hot_start(seeds)
+    for t in range(2):
+        B.HMC(am0s, 0.1, 10, 100000, 1e-16, seeds, t, precision=Q.PREC_MIXED)
+    E0 = [B.get_links(c).clone() for c in range(n)]
+    r = {}
+    for tol in (1e-16, 1e-20, 1e-24):
+        dh = {}
+        for prec in (Q.PREC_MIXED, Q.PREC_F64):
+            for c in range(n):
+                B.set_links(c, E0[c])
+            dh[prec] = [i.dH for i in B.HMC(am0s, 0.1, 10, 100000, tol, seeds, 2,
+                                            precision=prec, acc_tol=tol)]
+        r[f"{tol:.0e}"] = max(abs(a - b) for a, b in zip(dh[Q.PREC_MIXED], dh[Q.PREC_F64]))
+        log(f"tolerance ensemble tol {tol:.0e}: max |ddH| {r[f'{tol:.0e}']:.2e}")
+    out["ensemble_64_chains"] = r
+    B.close()
+    torch.cuda.empty_cache()
+    # Nf=2 4096^2
+    L = 4096
+    prm = Q.LattParm((L, L), 11.25)
+    U = Q.cold_links(prm)
+    Q.seed(55, chain=2)
+    for _ in range(3):
+        Q.HMC(U, 0.025, 20, [], prm, None)
+    U0 = U.clone()
+    r = {}
+    for tol in (1e-20, 1e-22, 1e-24):
+        dh = {}
+        for prec in (Q.PREC_MIXED, Q.PREC_F64):
+            U.copy_(U0)
+            Q.seed(55)
+            dh[prec] = Q.HMC(U, 0.05, 0.05, 2, [], 100000, tol, prm, None, integrator=Q.OMF4(),
+                             precision=prec, acc_tol=tol).dH
+        r[f"{tol:.0e}"] = abs(dh[Q.PREC_MIXED] - dh[Q.PREC_F64])
+        log(f"tolerance 4096^2 tol {tol:.0e}: |ddH| {r[f'{tol:.0e}']:.2e}")
+    out["nf2_4096_omf4_ns2_eps0.05"] = r
+    del U, U0
+    prm.close()
+    return out
+
+
+def rhmc_config(Q, torch, args, pool):
+    """RHMC 128^2, beta 4, m0 0.1, 16 poles on power-method bounds, 4
+    pseudofermions, OMF4 ns 12 eps 0.083: a Markov chain in each force
+    precision (the CPU job, when a pool is given, times its first trajectory
+    on the oracle)."""
+    import numpy as np
+
+    def sync_time(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    prm = Q.LattParm((128, 128), 4.0)
+    U = Q.cold_links(prm)
+    Q.seed(77)
+    for _ in range(20):
+        Q.HMC(U, 0.1, 10, [], prm, None)
+    b0 = Q.spinor(prm)
+    Q.fill_pseudofermion(b0, prm, 77, 3, 0, 0)
+    lmin, lmax = Q.power_method(U, 0.1, prm, iter=5000, b0=b0)
+    ra, rb = np.sqrt(lmin * 0.99), np.sqrt(lmax * 1.01)
+    rp = Q.get_rhmc_params(16, ra, rb)
+    Uc = U.clone()
+    traj0 = Q.rng_state()[2]
+    # a Markov chain of args.rhmc_traj trajectories from the same start and
+    # random streams with the force's solves in FP64 and in mixed precision
+    # (FP32 multi-shift, then every pole by the batched mixed-precision CG to
+    # the same FP64 tolerance; heatbath and acceptance FP64 in both)
+    nt = args.rhmc_traj
+    c3 = {"trajectories": nt, "cg_tol": 1e-16, "delta": rp.delta}
+    runs = {}
+    for prec, key in ((Q.PREC_F64, "fp64"), (Q.PREC_MIXED, "mixed")):
+        U.copy_(Uc)
+        Q.seed(77)
+        dt, infos = sync_time(lambda: [
+            Q.HMC(U, [0.1] * 4, 0.083, 12, [], 100000, 1e-16, prm, None, [rp] * 4,
+                  integrator=Q.OMF4(), precision=prec, traj=traj0 + i) for i in range(nt)])
+        runs[key] = infos
+        dh = np.array([i.dH for i in infos])
+        c3[key] = {"traj_per_s": round(nt / dt, 3),
+                   "acceptance": round(float(np.mean([i.accepted for i in infos])), 4),
+                   "exp_minus_dH_mean": round(float(np.mean(np.exp(-dh))), 4),
+                   "exp_minus_dH_stderr": round(float(np.std(np.exp(-dh)) / np.sqrt(nt)), 4),
+                   "accepted": [i.accepted for i in infos],
+                   "dH": [round(float(x), 8) for x in dh]}
+    c3["gpu_traj_per_s"] = c3["mixed"]["traj_per_s"]  # (the config's FP32 force solves)
+    c3["mixed_vs_fp64"] = round(c3["mixed"]["traj_per_s"] / c3["fp64"]["traj_per_s"], 3)
+    c3["accept_identical_vs_fp64"] = c3["mixed"]["accepted"] == c3["fp64"]["accepted"]
+    c3["max_abs_dH_diff_vs_fp64"] = max(abs(a.dH - b.dH) for a, b in
+                                         zip(runs["mixed"], runs["fp64"]))
+    c3["_gpu"] = {"acc": [runs["fp64"][0].accepted], "dH": [runs["fp64"][0].dH]}
+    if pool is not None:
+        pool.submit("rhmc_128", U=Uc.cpu().numpy(), r_a=float(ra), r_b=float(rb), traj=traj0)
+    prm.close()
+    return c3
 
 
 def config_measurements(Q, torch, pool, args):
@@ -473,51 +632,7 @@ def config_measurements(Q, torch, pool, args):
     del bs, si, U
     prm.close()
 
-    # RHMC 128^2, beta 4, m0 0.1, 16 poles on power-method bounds, 4
-    # pseudofermions, OMF4 ns 12 eps 0.083
-    prm = Q.LattParm((128, 128), 4.0)
-    U = Q.cold_links(prm)
-    Q.seed(77)
-    for _ in range(20):
-        Q.HMC(U, 0.1, 10, [], prm, None)
-    b0 = Q.spinor(prm)
-    Q.fill_pseudofermion(b0, prm, 77, 3, 0, 0)
-    lmin, lmax = Q.power_method(U, 0.1, prm, iter=5000, b0=b0)
-    ra, rb = np.sqrt(lmin * 0.99), np.sqrt(lmax * 1.01)
-    rp = Q.get_rhmc_params(16, ra, rb)
-    Uc = U.clone()
-    traj0 = Q.rng_state()[2]
-    # a Markov chain of args.rhmc_traj trajectories from the same start and
-    # random streams with the force's solves in FP64 and in mixed precision
-    # (FP32 multi-shift, then every pole by the batched mixed-precision CG to
-    # the same FP64 tolerance; heatbath and acceptance FP64 in both)
-    nt = args.rhmc_traj
-    c3 = {"trajectories": nt, "cg_tol": 1e-16, "delta": rp.delta}
-    runs = {}
-    for prec, key in ((Q.PREC_F64, "fp64"), (Q.PREC_MIXED, "mixed")):
-        U.copy_(Uc)
-        Q.seed(77)
-        dt, infos = sync_time(lambda: [
-            Q.HMC(U, [0.1] * 4, 0.083, 12, [], 100000, 1e-16, prm, None, [rp] * 4,
-                  integrator=Q.OMF4(), precision=prec, traj=traj0 + i) for i in range(nt)])
-        runs[key] = infos
-        dh = np.array([i.dH for i in infos])
-        c3[key] = {"traj_per_s": round(nt / dt, 3),
-                   "acceptance": round(float(np.mean([i.accepted for i in infos])), 4),
-                   "exp_minus_dH_mean": round(float(np.mean(np.exp(-dh))), 4),
-                   "exp_minus_dH_stderr": round(float(np.std(np.exp(-dh)) / np.sqrt(nt)), 4),
-                   "accepted": [i.accepted for i in infos],
-                   "dH": [round(float(x), 8) for x in dh]}
-    c3["gpu_traj_per_s"] = c3["mixed"]["traj_per_s"]  # (the config's FP32 force solves)
-    c3["mixed_vs_fp64"] = round(c3["mixed"]["traj_per_s"] / c3["fp64"]["traj_per_s"], 3)
-    c3["accept_identical_vs_fp64"] = c3["mixed"]["accepted"] == c3["fp64"]["accepted"]
-    c3["max_abs_dH_diff_vs_fp64"] = max(abs(a.dH - b.dH) for a, b in
-                                         zip(runs["mixed"], runs["fp64"]))
-    c3["_gpu"] = {"acc": [runs["fp64"][0].accepted], "dH": [runs["fp64"][0].dH]}
-    out["rhmc_128x128_16poles_4pf"] = c3
-    if pool is not None:
-        pool.submit("rhmc_128", U=Uc.cpu().numpy(), r_a=float(ra), r_b=float(rb), traj=traj0)
-    prm.close()
+    out["rhmc_128x128_16poles_4pf"] = rhmc_config(Q, torch, args, pool)
 
     # master field: pure gauge 8192^2, beta 11.25, OMF4 ns 20 eps 0.025
     L = 8192
@@ -865,12 +980,15 @@ def rocprof_summary(argv):
     import numpy as np
     ap = argparse.ArgumentParser(prog="bench.py --rocprof-summary")
     ap.add_argument("kt")
-    ap.add_argument("fetch")
-    ap.add_argument("write")
+    ap.add_argument("fetch", nargs="?", default="")
+    ap.add_argument("write", nargs="?", default="")
     ap.add_argument("--L", type=int, default=1024)
     ap.add_argument("--json", default="")
     ap.add_argument("--by-grid", action="store_true",
                     help="one row per (kernel, grid size): shapes that share a kernel")
+    ap.add_argument("--windows", type=int, default=0,
+                    help="trajectories per timed run: GPU busy and idle time per trajectory "
+                         "inside the windows the timed runs mark with one-element fills")
     a = ap.parse_args(argv)
     con = sqlite3.connect(a.kt)
     by = {}
@@ -887,6 +1005,28 @@ def rocprof_summary(argv):
         w = d[d > 5.0]
         print(f"| {n} | {len(d)} | {d.sum():.0f} | {d.mean():.2f} | "
               f"{np.median(w) if len(w) else 0.0:.2f} | {100 * d.sum() / total:.1f} |")
+
+    if a.windows:
+        import re
+        ks = list(con.execute("select name, start, end from kernels order by start"))
+        # (int16 fills: nothing else in the run fills a 16-bit tensor)
+        marks = [i for i, (n, _, _) in enumerate(ks) if re.search(r"FillFunctor<short>", n)]
+        print()
+        print(f"Timed windows (between the fills that mark them; {a.windows} trajectories each):")
+        print()
+        print("| window | kernels | span ms | busy ms | idle ms | idle ms per trajectory |")
+        print("|---|---|---|---|---|---|")
+        for w, (i0, i1) in enumerate(zip(marks[0::2], marks[1::2])):
+            inner = ks[i0 + 1:i1]
+            if not inner:
+                continue
+            span = (ks[i1][1] - ks[i0][2]) / 1e6
+            busy = sum(e - s0 for _, s0, e in inner) / 1e6
+            print(f"| {w} | {len(inner)} | {span:.2f} | {busy:.2f} | {span - busy:.2f} | "
+                  f"{(span - busy) / a.windows:.3f} |")
+
+    if not (a.fetch and a.write):
+        return
 
     def counter(db, cname):
         c = sqlite3.connect(db)

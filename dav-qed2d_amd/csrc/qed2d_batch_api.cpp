@@ -227,6 +227,14 @@ struct Predict {  // the single-lattice solvers' chunk predictor, per chain
         if (chunk > 64) chunk = 64;
         return chunk;
     }
+    // iterations left to |r|^2 < tol at the observed rate (a large value while
+    // the rate is unknown)
+    double remaining(double err, double tol) const
+    {
+        if (!(rate < 0.0) || !(err > 0.0)) return 1e9;
+        const double left = std::log(tol / err) / rate;
+        return left > 0.0 ? left : 0.0;
+    }
     int after_ru(double err, double tol, double delta2) const
     {
         if (!(rate < 0.0) || !(err > 100.0 * tol)) return 2;
@@ -280,6 +288,8 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     // whose links are not restarts with FP32 complex links.
     bool packed = mixed;
     std::vector<Predict> pred;
+    std::vector<int> ev(b->n);        // per chain: iterations to its next event
+    std::vector<double> rem(b->n);    // per chain: iterations its solve has left
     int chunk = chunk0;
     long launched_k = 0;
     auto start = [&]() -> int {
@@ -344,24 +354,39 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             continue;
         }
         first_look = false;
+        // Each running chain predicts the iteration of its next event (its next
+        // reliable update or its convergence) and how many iterations its solve
+        // has left.  A chunk ends at the first event of a chain whose remaining
+        // count is within kSlack of the longest one's; any other chain's event
+        // ends it no earlier than kSlack iterations on (a chain that finishes or
+        // pauses mid-chunk idles, which delays the batch only if it is near the
+        // longest).
+        constexpr int kSlack = 16;
         bool running = false, ru = false;
-        int next = 64;
+        double rlong = 0.0;
         for (int c = 0; c < b->n; ++c) {
             const SolverState &h = b->hS[c];
             b->hmask[c] = h.need_ru ? 1 : 0;
+            ev[c] = 0;
             if (h.status != 0) continue;
             running = true;
             if (h.need_ru) {
                 ru = true;
-                const int p = pred[c].after_ru(h.err, h.tol, h.delta2);
-                if (p < next) next = p;
-                continue;
+                ev[c] = pred[c].after_ru(h.err, h.tol, h.delta2);
+            } else {
+                const double rut = mixed ? h.delta2 * h.rmax2[(h.iter + 1) & 1] : 0.0;
+                ev[c] = pred[c].next(h.err, h.iter, h.tol, rut);
             }
-            const double rut = mixed ? h.delta2 * h.rmax2[(h.iter + 1) & 1] : 0.0;
-            const int p = pred[c].next(h.err, h.iter, h.tol, rut);
-            if (p < next) next = p;
+            rem[c] = pred[c].remaining(h.err, h.tol);
+            rlong = std::max(rlong, rem[c]);
         }
         if (!running) break;
+        int next = 64;
+        for (int c = 0; c < b->n; ++c) {
+            if (ev[c] == 0) continue;
+            const int p = (rem[c] + kSlack < rlong) ? std::max(ev[c], kSlack) : ev[c];
+            if (p < next) next = p;
+        }
         if (launched_k > 4L * (maxiter + 2) + 64)
             return bfail(b, QED2D_EHIP, "batched CG: chains stopped advancing");
         if (ru) {
