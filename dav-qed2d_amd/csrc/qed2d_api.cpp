@@ -406,10 +406,8 @@ int observables(qed2d_ctx *c, const double *U, const double *mom, hdd *sp, hdd *
 
 // H = sum p^2 / 2 + beta (V - sum Re P)   (src/QED2dHMC.jl:198-215), returned
 // as a double-double so that differences of two H stay exact to ~1e-16 |H|.
-int hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, hdd *H)
+hdd hamiltonian_from(const qed2d_ctx *c, const hdd &sp, const hdd &sm)
 {
-    hdd sp, sm;
-    TRY(observables(c, U, mom, &sp, nullptr, &sm));
     // beta (V - sum ReP): form V - sumReP in double-double first
     hdd g;
     g.add((double)c->L.V);
@@ -421,7 +419,13 @@ int hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, hdd *H)
     hdd h;
     h.add(0.5 * sm.hi, 0.5 * sm.lo);
     h.add(ph, pe + c->beta * gl);
-    *H = h;
+    return h;
+}
+int hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, hdd *H)
+{
+    hdd sp, sm;
+    TRY(observables(c, U, mom, &sp, nullptr, &sm));
+    *H = hamiltonian_from(c, sp, sm);
     return QED2D_OK;
 }
 
@@ -1849,6 +1853,42 @@ int ctx_dot_dd(qed2d_ctx *c, const double *a, const double *b, double *hi, doubl
     *hi = h.hi;
     *lo = h.lo;
     return QED2D_OK;
+}
+
+// Queued forms of the two: the reduction's result lands in *dst (pinned host
+// memory) when the stream gets there; after a synchronisation the *_from forms
+// give the values the synchronous calls return.
+int ctx_queue_hamiltonian(qed2d_ctx *c, const double *mom, const double *U, RedOut *dst)
+{
+    launch_observables(c->L, (const double2 *)U, mom, nullptr, nullptr, c->red, c->part,
+                       c->stream);
+    TRY(launched(c));
+    HIPCHK(c, hipMemcpyAsync(dst, c->red, sizeof(RedOut), hipMemcpyDeviceToHost, c->stream));
+    return QED2D_OK;
+}
+int ctx_queue_dot_dd(qed2d_ctx *c, const double *a, const double *b, RedOut *dst)
+{
+    launch_dot_dd((const double2 *)a, (const double2 *)b, spinor_elems(c), c->red, c->part,
+                  c->stream);
+    TRY(launched(c));
+    HIPCHK(c, hipMemcpyAsync(dst, c->red, sizeof(RedOut), hipMemcpyDeviceToHost, c->stream));
+    return QED2D_OK;
+}
+void ctx_hamiltonian_from(const qed2d_ctx *c, const RedOut &r, double *hi, double *lo)
+{
+    hdd sp, sm;
+    sp.add(r.v[0], r.v[1]);
+    sm.add(r.v[4], r.v[5]);
+    const hdd h = hamiltonian_from(c, sp, sm);
+    *hi = h.hi;
+    *lo = h.lo;
+}
+void ctx_dot_dd_from(const RedOut &r, double *hi, double *lo)
+{
+    hdd h;
+    h.add(r.v[0], r.v[1]);
+    *hi = h.hi;
+    *lo = h.lo;
 }
 }  // namespace qed2d
 

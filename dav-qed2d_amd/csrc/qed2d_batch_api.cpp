@@ -60,6 +60,12 @@ struct qed2d_batch {
     // solver state and partials
     SolverState *S = nullptr, *hS = nullptr;
     int *mask = nullptr, *hmask = nullptr;
+    RedOut *hred = nullptr;  // pinned: the queued energy reductions of a trajectory
+    // pipelined solves: two pinned slots of chain states (each with an event) and
+    // two of reliable-update masks
+    SolverState *hSring = nullptr;
+    int *hmring = nullptr;
+    hipEvent_t sev[2] = {};
     double *part = nullptr;
     int *cnt = nullptr;
     BParts P{};
@@ -248,6 +254,17 @@ struct Predict {  // the single-lattice solvers' chunk predictor, per chain
     }
 };
 
+int ring_setup(qed2d_batch *b)
+{
+    if (b->hSring) return QED2D_OK;
+    if (hipHostMalloc((void **)&b->hSring, 2 * sizeof(SolverState) * b->n) != hipSuccess ||
+        hipHostMalloc((void **)&b->hmring, 2 * sizeof(int) * b->n) != hipSuccess ||
+        hipEventCreateWithFlags(&b->sev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->sev[1], hipEventDisableTiming) != hipSuccess)
+        return bfail(b, QED2D_ENOMEM, "pinned state slots of the batched solver");
+    return QED2D_OK;
+}
+
 // (Q^2) x_c = src_c on every chain from x = 0, to |r|^2 < tol |b|^2 (the
 // single-lattice cg_f64 / cg_mixed, one launch per kernel over all chains).
 int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double tol,
@@ -330,10 +347,19 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         launched_k = 0;
         return QED2D_OK;
     };
-    BTRY(start());
-    bool first_look = true;
-    for (;;) {
-        for (int j = 0; j < chunk; ++j) {
+    // Pipelined: behind the chunk whose states the host waits for, the next one
+    // is already queued (not behind the first, whose look may restart the solve
+    // with other links).  Each chunk's states land in a pinned slot of two, with
+    // an event.  A chain seen paused for a reliable update gets one when the host
+    // has not queued one for it since (SolverState::n_ru counts those performed):
+    // a chain stays paused through the chunk queued meanwhile, so the update
+    // finds it as the single-lattice solver's does, and the bits do not change.
+    BTRY(ring_setup(b));
+    std::vector<int> ru_sent(b->n, 0);
+    int slot = 0, mslot = 0, nq = 0, qhead = 0;
+    int qslot[2];
+    auto queue_chunk = [&](int len) -> int {
+        for (int j = 0; j < len; ++j) {
             if (mixed) {
                 launch_b_cg_k1_f32(g32, b->rows32, b->Us, packed, b->rs, b->p0s, b->p1s, b->ts,
                                    b->xs, b->S, b->P, st);
@@ -345,12 +371,32 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
                                    st);
             }
         }
-        launched_k += chunk;
+        launched_k += len;
         BTRY(blaunched(b));
-        BTRY(read_states(b));
+        BHIP(b, hipMemcpyAsync(b->hSring + (long)slot * b->n, b->S, sizeof(SolverState) * b->n,
+                               hipMemcpyDeviceToHost, st));
+        BHIP(b, hipEventRecord(b->sev[slot], st));
+        qslot[(qhead + nq) % 2] = slot;
+        nq += 1;
+        slot ^= 1;
+        return QED2D_OK;
+    };
+    BTRY(start());
+    BTRY(queue_chunk(chunk));
+    bool first_look = true;
+    for (;;) {
+        if (!first_look && nq == 1) BTRY(queue_chunk(chunk));
+        const int s0 = qslot[qhead];
+        qhead = (qhead + 1) % 2;
+        nq -= 1;
+        BHIP(b, hipEventSynchronize(b->sev[s0]));
+        std::memcpy((void *)b->hS, (const void *)(b->hSring + (long)s0 * b->n),
+                    sizeof(SolverState) * b->n);
         if (first_look && packed && b->hS[0].bad_links) {
             packed = false;
             BTRY(start());
+            ru_sent.assign(b->n, 0);
+            BTRY(queue_chunk(chunk));
             continue;
         }
         first_look = false;
@@ -362,16 +408,21 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         // pauses mid-chunk idles, which delays the batch only if it is near the
         // longest).
         constexpr int kSlack = 16;
+        int *hm = b->hmring + (long)mslot * b->n;
         bool running = false, ru = false;
         double rlong = 0.0;
         for (int c = 0; c < b->n; ++c) {
             const SolverState &h = b->hS[c];
-            b->hmask[c] = h.need_ru ? 1 : 0;
+            hm[c] = 0;
             ev[c] = 0;
             if (h.status != 0) continue;
             running = true;
             if (h.need_ru) {
-                ru = true;
+                if (h.n_ru == ru_sent[c]) {
+                    hm[c] = 1;
+                    ru_sent[c] += 1;
+                    ru = true;
+                }
                 ev[c] = pred[c].after_ru(h.err, h.tol, h.delta2);
             } else {
                 const double rut = mixed ? h.delta2 * h.rmax2[(h.iter + 1) & 1] : 0.0;
@@ -380,7 +431,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             rem[c] = pred[c].remaining(h.err, h.tol);
             rlong = std::max(rlong, rem[c]);
         }
-        if (!running) break;
+        if (!running) break;  // (a chunk still queued finds every chain stopped)
         int next = 64;
         for (int c = 0; c < b->n; ++c) {
             if (ev[c] == 0) continue;
@@ -392,8 +443,8 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         if (ru) {
             // the chains that paused: fold the FP32 partial solution into x,
             // recompute the true residual in FP64, resume at the same K1
-            BHIP(b, hipMemcpyAsync(b->mask, b->hmask, sizeof(int) * b->n, hipMemcpyHostToDevice,
-                                   st));
+            BHIP(b, hipMemcpyAsync(b->mask, hm, sizeof(int) * b->n, hipMemcpyHostToDevice, st));
+            mslot ^= 1;
             launch_b_ru_accum(b->dch, b->n, b->vtot, x, b->xs, b->p0s, b->p1s, b->S, b->mask,
                               st);
             launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, b->mask, st);
@@ -402,6 +453,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             BTRY(blaunched(b));
         }
         chunk = next;
+        BTRY(queue_chunk(chunk));
     }
     iters.assign(b->n, 0);
     int bad = -1;
@@ -650,6 +702,10 @@ void qed2d_batch_destroy(qed2d_batch *b)
         if (p) (void)hipFree(p);
     if (b->hS) (void)hipHostFree(b->hS);
     if (b->hmask) (void)hipHostFree(b->hmask);
+    if (b->hSring) (void)hipHostFree(b->hSring);
+    if (b->hred) (void)hipHostFree(b->hred);
+    if (b->hmring) (void)hipHostFree(b->hmring);
+    for (hipEvent_t e : b->sev) if (e) (void)hipEventDestroy(e);
     delete b;
 }
 
@@ -945,19 +1001,21 @@ int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, i
         launch_fill_pseudofermion(ch.L, b->X + o, k0, k1, traj, 0, st);
     }
     BTRY(blaunched(b));
-    // hini = H + X^dag X  (src/QED2dHMC.jl:74), per chain
+    // hini = H + X^dag X  (src/QED2dHMC.jl:74), per chain: the reductions are
+    // queued into pinned slots (5 per chain: H and X^dag X before the MD; H,
+    // Re X^dag F and |Y|^2 after it) and read after the trajectory's last launch
+    if (!b->hred) {
+        if (hipHostMalloc((void **)&b->hred, 5 * sizeof(RedOut) * b->n) != hipSuccess)
+            return bfail(b, QED2D_ENOMEM, "pinned reduction slots of the batch");
+    }
+    RedOut *hr = b->hred;
     for (int c = 0; c < b->n; ++c) {
         const long o = 2 * b->hch[c].off;
-        int rc = ctx_hamiltonian_dd(b->sub[c], b->mom + o, (const double *)(b->U + o), &h0h[c],
-                                    &h0l[c]);
+        int rc = ctx_queue_hamiltonian(b->sub[c], b->mom + o, (const double *)(b->U + o),
+                                       &hr[5 * c]);
         if (rc) return subfail(b, c, rc);
-        dd xx;
-        rc = ctx_dot_dd(b->sub[c], (const double *)(b->X + o), nullptr, &xx.hi, &xx.lo);
+        rc = ctx_queue_dot_dd(b->sub[c], (const double *)(b->X + o), nullptr, &hr[5 * c + 1]);
         if (rc) return subfail(b, c, rc);
-        dd h{h0h[c], h0l[c]};
-        h = dd_add(h, xx);
-        h0h[c] = h.hi;
-        h0l[c] = h.lo;
     }
     BGeo gq;
     BTRY(get_geo(b, b->rows_q, false, &gq));
@@ -988,22 +1046,36 @@ int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, i
         launch_b_q_apply(gq, b->rows_q, b->U, b->X, b->Y, nullptr, st);
     }
     for (int c = 0; c < b->n; ++c) {
+        const long o = 2 * b->hch[c].off;
+        rc = ctx_queue_hamiltonian(b->sub[c], b->mom + o, (const double *)(b->U + o),
+                                   &hr[5 * c + 2]);
+        if (rc) return subfail(b, c, rc);
+        rc = ctx_queue_dot_dd(b->sub[c], (const double *)(b->X + o), (const double *)(b->F + o),
+                              &hr[5 * c + 3]);
+        if (rc) return subfail(b, c, rc);
+        if (acc_tol > 0.0) {
+            rc = ctx_queue_dot_dd(b->sub[c], (const double *)(b->Y + o), nullptr, &hr[5 * c + 4]);
+            if (rc) return subfail(b, c, rc);
+        }
+    }
+    BHIP(b, hipStreamSynchronize(st));
+    for (int c = 0; c < b->n; ++c) {
         const BChain &ch = b->hch[c];
         const long o = 2 * ch.off;
-        double hh, hl;
-        rc = ctx_hamiltonian_dd(b->sub[c], b->mom + o, (const double *)(b->U + o), &hh, &hl);
-        if (rc) return subfail(b, c, rc);
-        dd h1{hh, hl};
-        dd re;
-        rc = ctx_dot_dd(b->sub[c], (const double *)(b->X + o), (const double *)(b->F + o), &re.hi,
-                        &re.lo);
-        if (rc) return subfail(b, c, rc);
+        dd h0, xx;
+        ctx_hamiltonian_from(b->sub[c], hr[5 * c], &h0.hi, &h0.lo);
+        ctx_dot_dd_from(hr[5 * c + 1], &xx.hi, &xx.lo);
+        h0 = dd_add(h0, xx);
+        h0h[c] = h0.hi;
+        h0l[c] = h0.lo;
+        dd h1, re;
+        ctx_hamiltonian_from(b->sub[c], hr[5 * c + 2], &h1.hi, &h1.lo);
+        ctx_dot_dd_from(hr[5 * c + 3], &re.hi, &re.lo);
         if (acc_tol <= 0.0) {
             h1 = dd_add(h1, re);  // hfin = H + Re X^dag F  (:82)
         } else {
             dd yy;
-            rc = ctx_dot_dd(b->sub[c], (const double *)(b->Y + o), nullptr, &yy.hi, &yy.lo);
-            if (rc) return subfail(b, c, rc);
+            ctx_dot_dd_from(hr[5 * c + 4], &yy.hi, &yy.lo);
             h1 = dd_add(h1, dd{2.0 * re.hi, 2.0 * re.lo});
             h1 = dd_add(h1, dd{-yy.hi, -yy.lo});
         }

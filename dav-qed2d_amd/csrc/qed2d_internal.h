@@ -156,6 +156,12 @@ constexpr double OMF_R4 = -0.03230286765269967;
 int ctx_hamiltonian_dd(qed2d_ctx *c, const double *mom, const double *U, double *hi, double *lo);
 // Re a^dag b (|a|^2 when b is null) of a context's lattice as a double-double.
 int ctx_dot_dd(qed2d_ctx *c, const double *a, const double *b, double *hi, double *lo);
+// queued forms: the reduction lands in *dst (pinned) when the stream gets there;
+// after a synchronisation the *_from forms give what the calls above return
+int ctx_queue_hamiltonian(qed2d_ctx *c, const double *mom, const double *U, RedOut *dst);
+int ctx_queue_dot_dd(qed2d_ctx *c, const double *a, const double *b, RedOut *dst);
+void ctx_hamiltonian_from(const qed2d_ctx *c, const RedOut &r, double *hi, double *lo);
+void ctx_dot_dd_from(const RedOut &r, double *hi, double *lo);
 
 // ---- batched ensembles ----------------------------------------------------
 // A chain set: lattices whose fields are concatenated in one arena.  Chain
