@@ -83,6 +83,9 @@ def main():
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--leg", choices=["ensemble", "rhmc", "tolerance"], default=None,
                     help="run one side measurement alone and print its JSON (for profiling)")
+    ap.add_argument("--ens-seq", type=int, default=3,
+                    help="trajectories of the ensemble's one-chain-at-a-time comparison "
+                         "(0 skips it)")
     ap.add_argument("--rhmc-traj", type=int, default=20,
                     help="trajectories of the RHMC 128^2 Markov chain, per force precision")
     ap.add_argument("--tiles", type=str, default="",
@@ -114,7 +117,7 @@ def main():
                           "dw_apply": dw_apply(Q, torch, args)}), flush=True)
         return
     if args.leg == "ensemble":
-        print(json.dumps(ensemble(Q, torch)), flush=True)
+        print(json.dumps(ensemble(Q, torch, nseq=args.ens_seq)), flush=True)
         return
     if args.leg == "tolerance":
         print(json.dumps(tolerance_scan(Q, torch)), flush=True)
@@ -243,7 +246,7 @@ def main():
     if args.accept_traj > 0:
         res["acceptance_comparison"] = acceptance_comparison(Q, torch, args, prm, U, U0, seed)
     if not args.no_ensemble:
-        res["ensemble_64_chains"] = ensemble(Q, torch)
+        res["ensemble_64_chains"] = ensemble(Q, torch, nseq=args.ens_seq)
     # (the CPU workers start here: nothing timed before runs beside them; the
     # apply bandwidths below are HIP-event times of the kernels themselves)
     if pool is not None:
@@ -396,6 +399,9 @@ def ensemble(Q, torch, ntraj=10, nseq=3):
         mark.fill_(1)
         out[f"{key}_chain_traj_per_s"] = round(n * ntraj / dt, 2)
         log(f"ensemble {key}: {n} chains x {ntraj} traj in {dt:.3f}s")
+    if nseq <= 0:
+        B.close()
+        return out
     # the same chains one at a time (single-lattice entry point, mixed)
     prms = [Q.LattParm(sz, b) for sz, b in zip(sizes, betas)]
     Us = [u.clone() for u in U0]
