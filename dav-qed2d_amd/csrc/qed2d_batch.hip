@@ -78,24 +78,25 @@ __device__ __forceinline__ int chain_of_elem(const BChain *ch, int n, long e)
 
 // ---------------------------------------------------------------------------
 // so = Q si on every chain (on the chains waiting for a reliable update only,
-// when mask is given).
+// when their solver states ru are given).
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ void __launch_bounds__(kBlock) kb_q_apply(BGeo g, const double2 *__restrict__ U,
                                                      const double2 *__restrict__ si,
-                                                     double2 *__restrict__ so, const int *mask)
+                                                     double2 *__restrict__ so,
+                                                     const SolverState *ru)
 {
     const BWhere w = bwhere<false>(g, R);
     if (!w.p.active) return;
-    if (mask && !mask[w.c]) return;
+    if (ru && !ru[w.c].need_ru) return;
     SrcPlain<double> src{si + w.o, w.L.V};
     EpiStore<double> epi{so + w.o, w.L.V};
     q_march<double, R>(w.L, U + w.uo, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
 }
 void launch_b_q_apply(const BGeo &g, int rows, const double2 *U, const double2 *si,
-                      double2 *so, const int *mask, hipStream_t st)
+                      double2 *so, const SolverState *ru, hipStream_t st)
 {
-    DISPATCH_ROWS(rows, kb_q_apply, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, si, so, mask);
+    DISPATCH_ROWS(rows, kb_q_apply, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, si, so, ru);
 }
 
 // ---------------------------------------------------------------------------
@@ -179,9 +180,11 @@ void launch_b_cg_init(const BGeo &g, int rows, const double2 *b, double2 *x, voi
 
 // ---------------------------------------------------------------------------
 // Residual from x with t = Q x: mode 0 starts a solve from a guess on every
-// chain (FP64: r and p0 = 0; mixed: rs and p0s = 0); mode 2 is the reliable update of the chains that wait for one
-// (mask[c] != 0, set by the host from the states it read: need_ru itself is
-// cleared by this kernel and cannot select its blocks).
+// chain (FP64: r and p0 = 0; mixed: rs and p0s = 0); mode 2 is the reliable
+// update of the chains that wait for one (need_ru, set by the K1 that paused).
+// Mode 2 resumes the chain (stop_k = 0) but leaves need_ru to the K1 it
+// resumes: a block of this kernel that starts after block 0 has finished must
+// still see the request.
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__restrict__ U,
@@ -192,11 +195,11 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
                                                       float2 *__restrict__ rs,
                                                       double2 *__restrict__ p0,
                                                       float2 *__restrict__ p0s, SolverState *S,
-                                                      BParts P, int mode, const int *mask)
+                                                      BParts P, int mode)
 {
     const BWhere w = bwhere<false>(g, R);
     SolverState *Sc = S + w.c;
-    if (mode == 2 && !mask[w.c]) return;
+    if (mode == 2 && !Sc->need_ru) return;
     const int k = (mode == 2) ? Sc->kk : 1;
     const long o = w.o;
     EpiResid epi{b + o, x + o, r ? r + o : nullptr, rs ? rs + o : nullptr,
@@ -216,7 +219,6 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
             if (mode != 2) P.nb[w.c] = w.nb;
             Sc->true_at = k;
             if (mode == 2) {
-                Sc->need_ru = 0;
                 Sc->n_ru += 1;
                 Sc->stop_k = 0;
             }
@@ -225,28 +227,33 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
 }
 void launch_b_cg_resid(const BGeo &g, int rows, const double2 *U, const double2 *b,
                        const double2 *t, const double2 *x, double2 *r, float2 *rs, double2 *p0,
-                       float2 *p0s, SolverState *S, BParts P, int mode, const int *mask,
-                       hipStream_t st)
+                       float2 *p0s, SolverState *S, BParts P, int mode, hipStream_t st)
 {
     DISPATCH_ROWS(rows, kb_cg_resid, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, b, x, r, rs,
-                  p0, p0s, S, P, mode, mask);
+                  p0, p0s, S, P, mode);
 }
 
-// x += double(xs), xs = 0 on the chains that wait for a reliable update, each
-// chain's pending xs += alpha_x p_{stop_k - 1} applied first (the site-pair K1
-// paused before applying it).
+// x += double(xs), xs = 0 on the chains that wait for a reliable update
+// (need_ru), each chain's pending xs += alpha_x p_{stop_k - 1} applied first
+// (the site-pair K1 paused before applying it).  A block's 256 elements that
+// lie in one chain not waiting for an update are skipped as a whole, so the
+// launch costs little when no chain waits.
 __global__ void __launch_bounds__(kBlock) kb_ru_accum(const BChain *ch, int n, long nel,
                                                       double2 *__restrict__ x,
                                                       float2 *__restrict__ xs,
                                                       const float2 *__restrict__ P0,
                                                       const float2 *__restrict__ P1,
-                                                      const SolverState *S, const int *mask)
+                                                      const SolverState *S)
 {
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nel;
-         e += (long)gridDim.x * blockDim.x) {
-        const int c = chain_of_elem(ch, n, e);
-        if (!mask[c]) continue;
+    for (long e0 = blockIdx.x * (long)kBlock; e0 < nel; e0 += (long)gridDim.x * kBlock) {
+        const int c0 = chain_of_elem(ch, n, e0);
+        const int c1 = chain_of_elem(ch, n, (e0 + kBlock <= nel ? e0 + kBlock : nel) - 1);
+        if (c0 == c1 && !S[c0].need_ru) continue;
+        const long e = e0 + threadIdx.x;
+        if (e >= nel) continue;
+        const int c = (c0 == c1) ? c0 : chain_of_elem(ch, n, e);
         const SolverState *Sc = S + c;
+        if (!Sc->need_ru) continue;
         const float2 *pp = ((Sc->stop_k - 1) & 1) ? P1 : P0;
         const float2 d = xpa(xs[e], (float)Sc->alpha_x, pp[e]);
         const double2 v = x[e];
@@ -256,12 +263,12 @@ __global__ void __launch_bounds__(kBlock) kb_ru_accum(const BChain *ch, int n, l
 }
 void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,
                        const float2 *p0, const float2 *p1, const SolverState *S,
-                       const int *mask, hipStream_t st)
+                       hipStream_t st)
 {
     long nb = (2 * vtot + kBlock - 1) / kBlock;
     if (nb > 8192) nb = 8192;
     launch_k(kb_ru_accum, dim3((int)nb), dim3(kBlock), 0, st, ch, nchain, 2 * vtot, x,
-                       xs, p0, p1, S, mask);
+                       xs, p0, p1, S);
 }
 
 // ---------------------------------------------------------------------------

@@ -59,12 +59,9 @@ struct qed2d_batch {
     double *mom = nullptr;
     // solver state and partials
     SolverState *S = nullptr, *hS = nullptr;
-    int *mask = nullptr, *hmask = nullptr;
     RedOut *hred = nullptr;  // pinned: the queued energy reductions of a trajectory
-    // pipelined solves: two pinned slots of chain states (each with an event) and
-    // two of reliable-update masks
+    // pipelined solves: two pinned slots of chain states, each with an event
     SolverState *hSring = nullptr;
-    int *hmring = nullptr;
     hipEvent_t sev[2] = {};
     double *part = nullptr;
     int *cnt = nullptr;
@@ -258,7 +255,6 @@ int ring_setup(qed2d_batch *b)
 {
     if (b->hSring) return QED2D_OK;
     if (hipHostMalloc((void **)&b->hSring, 2 * sizeof(SolverState) * b->n) != hipSuccess ||
-        hipHostMalloc((void **)&b->hmring, 2 * sizeof(int) * b->n) != hipSuccess ||
         hipEventCreateWithFlags(&b->sev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->sev[1], hipEventDisableTiming) != hipSuccess)
         return bfail(b, QED2D_ENOMEM, "pinned state slots of the batched solver");
@@ -309,6 +305,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     std::vector<double> rem(b->n);    // per chain: iterations its solve has left
     int chunk = chunk0;
     long launched_k = 0;
+    bool slot_ru = false;  // mixed: the next chunk starts with a reliable-update slot
     auto start = [&]() -> int {
         for (int c = 0; c < b->n; ++c) {
             SolverState &s = b->hS[c];
@@ -334,7 +331,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, nullptr, st);
             launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, mixed ? nullptr : b->r,
                               mixed ? b->rs : nullptr, mixed ? nullptr : b->p0,
-                              mixed ? b->p0s : nullptr, b->S, b->P, 0, nullptr, st);
+                              mixed ? b->p0s : nullptr, b->S, b->P, 0, st);
             if (mixed) BHIP(b, hipMemsetAsync(b->xs, 0, sizeof(float2) * nel, st));
         } else if (mixed) {
             launch_b_cg_init(g64, b->rows, src, x, b->rs, b->p0s, b->xs, 1, b->P, st);
@@ -345,20 +342,31 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         pred.assign(b->n, Predict());
         chunk = chunk0;
         launched_k = 0;
+        slot_ru = false;
         return QED2D_OK;
     };
     // Pipelined: behind the chunk whose states the host waits for, the next one
     // is already queued (not behind the first, whose look may restart the solve
     // with other links).  Each chunk's states land in a pinned slot of two, with
-    // an event.  A chain seen paused for a reliable update gets one when the host
-    // has not queued one for it since (SolverState::n_ru counts those performed):
-    // a chain stays paused through the chunk queued meanwhile, so the update
-    // finds it as the single-lattice solver's does, and the bits do not change.
+    // an event.  In mixed precision every chunk but a solve's first starts with
+    // a reliable-update slot: its kernels act on the chains whose K1 paused for
+    // an update (need_ru) and return at once on the others, so a chain that
+    // paused in one chunk resumes at the start of the next without waiting for
+    // the host to see it, and takes the same steps as a single-lattice solve
+    // whose host performs the update at the pause.
     BTRY(ring_setup(b));
-    std::vector<int> ru_sent(b->n, 0);
-    int slot = 0, mslot = 0, nq = 0, qhead = 0;
+    int slot = 0, nq = 0, qhead = 0;
     int qslot[2];
     auto queue_chunk = [&](int len) -> int {
+        if (mixed && slot_ru) {
+            // the paused chains: fold the FP32 partial solution into x,
+            // recompute the true residual in FP64, resume at the same K1
+            launch_b_ru_accum(b->dch, b->n, b->vtot, x, b->xs, b->p0s, b->p1s, b->S, st);
+            launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, b->S, st);
+            launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, nullptr, b->rs, nullptr, nullptr,
+                              b->S, b->P, 2, st);
+        }
+        slot_ru = true;
         for (int j = 0; j < len; ++j) {
             if (mixed) {
                 launch_b_cg_k1_f32(g32, b->rows32, b->Us, packed, b->rs, b->p0s, b->p1s, b->ts,
@@ -395,7 +403,6 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         if (first_look && packed && b->hS[0].bad_links) {
             packed = false;
             BTRY(start());
-            ru_sent.assign(b->n, 0);
             BTRY(queue_chunk(chunk));
             continue;
         }
@@ -408,21 +415,14 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         // pauses mid-chunk idles, which delays the batch only if it is near the
         // longest).
         constexpr int kSlack = 16;
-        int *hm = b->hmring + (long)mslot * b->n;
-        bool running = false, ru = false;
+        bool running = false;
         double rlong = 0.0;
         for (int c = 0; c < b->n; ++c) {
             const SolverState &h = b->hS[c];
-            hm[c] = 0;
             ev[c] = 0;
             if (h.status != 0) continue;
             running = true;
             if (h.need_ru) {
-                if (h.n_ru == ru_sent[c]) {
-                    hm[c] = 1;
-                    ru_sent[c] += 1;
-                    ru = true;
-                }
                 ev[c] = pred[c].after_ru(h.err, h.tol, h.delta2);
             } else {
                 const double rut = mixed ? h.delta2 * h.rmax2[(h.iter + 1) & 1] : 0.0;
@@ -440,18 +440,6 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         }
         if (launched_k > 4L * (maxiter + 2) + 64)
             return bfail(b, QED2D_EHIP, "batched CG: chains stopped advancing");
-        if (ru) {
-            // the chains that paused: fold the FP32 partial solution into x,
-            // recompute the true residual in FP64, resume at the same K1
-            BHIP(b, hipMemcpyAsync(b->mask, hm, sizeof(int) * b->n, hipMemcpyHostToDevice, st));
-            mslot ^= 1;
-            launch_b_ru_accum(b->dch, b->n, b->vtot, x, b->xs, b->p0s, b->p1s, b->S, b->mask,
-                              st);
-            launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, b->mask, st);
-            launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, nullptr, b->rs, nullptr, nullptr,
-                              b->S, b->P, 2, b->mask, st);
-            BTRY(blaunched(b));
-        }
         chunk = next;
         BTRY(queue_chunk(chunk));
     }
@@ -650,8 +638,6 @@ int qed2d_batch_create(qed2d_batch **out, int nchain, const int *L1, const int *
               hipMalloc((void **)&b->U, sizeof(double2) * nel) == hipSuccess &&
               hipMalloc((void **)&b->S, sizeof(SolverState) * nchain) == hipSuccess &&
               hipHostMalloc((void **)&b->hS, sizeof(SolverState) * nchain) == hipSuccess &&
-              hipMalloc((void **)&b->mask, sizeof(int) * nchain) == hipSuccess &&
-              hipHostMalloc((void **)&b->hmask, sizeof(int) * nchain) == hipSuccess &&
               hipMalloc((void **)&b->part, sizeof(double) * 7 * pb) == hipSuccess &&
               hipMalloc((void **)&b->cnt, sizeof(int) * 5 * nchain) == hipSuccess;
     if (ok)
@@ -697,14 +683,12 @@ void qed2d_batch_destroy(qed2d_batch *b)
     }
     if (b->shared) b->U = nullptr;  // the caller's links
     void *dev[] = {b->dch, b->U, b->Ucp, b->X, b->F, b->Y, b->r, b->p0, b->p1, b->t, b->Us,
-                   b->rs, b->p0s, b->p1s, b->ts, b->xs, b->mom, b->S, b->mask, b->part, b->cnt};
+                   b->rs, b->p0s, b->p1s, b->ts, b->xs, b->mom, b->S, b->part, b->cnt};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (b->hS) (void)hipHostFree(b->hS);
-    if (b->hmask) (void)hipHostFree(b->hmask);
     if (b->hSring) (void)hipHostFree(b->hSring);
     if (b->hred) (void)hipHostFree(b->hred);
-    if (b->hmring) (void)hipHostFree(b->hmring);
     for (hipEvent_t e : b->sev) if (e) (void)hipEventDestroy(e);
     delete b;
 }
@@ -862,8 +846,6 @@ int qed2d_batch_create_rhs(qed2d_batch **out, int L1, int L2, int nrhs, double b
     bool ok = hipMalloc((void **)&b->dch, sizeof(BChain) * nrhs) == hipSuccess &&
               hipMalloc((void **)&b->S, sizeof(SolverState) * nrhs) == hipSuccess &&
               hipHostMalloc((void **)&b->hS, sizeof(SolverState) * nrhs) == hipSuccess &&
-              hipMalloc((void **)&b->mask, sizeof(int) * nrhs) == hipSuccess &&
-              hipHostMalloc((void **)&b->hmask, sizeof(int) * nrhs) == hipSuccess &&
               hipMalloc((void **)&b->part, sizeof(double) * 7 * pb) == hipSuccess &&
               hipMalloc((void **)&b->cnt, sizeof(int) * 5 * nrhs) == hipSuccess;
     if (ok)

@@ -109,10 +109,10 @@ __device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, dou
             S->status = stat;
             S->stop_k = k;
         }
-        if (ru) {
-            S->need_ru = 1;
-            S->stop_k = k;
-        }
+        // (a batched reliable update leaves need_ru set for the K1 it resumes
+        // to clear here)
+        S->need_ru = ru;
+        if (ru) S->stop_k = k;
     }
     if (stat || ru) return false;
     beta = first ? 0.0 : rr / prev;

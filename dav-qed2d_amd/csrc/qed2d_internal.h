@@ -209,9 +209,10 @@ int batch_rhs_cg_shifted(qed2d_batch *b, const double *U, double *x, const doubl
                          int precision, int *iters, int rows_q, int rows, int rows32,
                          int rows_k2, int rows32_k2, double ru_delta, std::string *err,
                          bool guess = false);
-// mask (may be null): per chain, nonzero = apply on that chain.
+// ru (may be null): the chains' solver states; apply only on the chains that
+// wait for a reliable update (need_ru).
 void launch_b_q_apply(const BGeo &g, int rows, const double2 *U, const double2 *si,
-                      double2 *so, const int *mask, hipStream_t st);
+                      double2 *so, const SolverState *ru, hipStream_t st);
 void launch_b_force_mom(const BChain *ch, int nchain, long vtot, const double2 *U,
                         const double2 *X, const double2 *Y, double *mom, double eps,
                         hipStream_t st);
@@ -219,8 +220,7 @@ void launch_b_cg_init(const BGeo &g, int rows, const double2 *b, double2 *x, voi
                       float2 *xs, int mixed, BParts P, hipStream_t st);
 void launch_b_cg_resid(const BGeo &g, int rows, const double2 *U, const double2 *b,
                        const double2 *t, const double2 *x, double2 *r, float2 *rs, double2 *p0,
-                       float2 *p0s, SolverState *S, BParts P, int mode, const int *mask,
-                       hipStream_t st);
+                       float2 *p0s, SolverState *S, BParts P, int mode, hipStream_t st);
 void launch_b_cg_k1_f64(const BGeo &g, int rows, const double2 *U, const double2 *r,
                         double2 *p0, double2 *p1, double2 *t, SolverState *S, BParts P,
                         hipStream_t st);
@@ -233,10 +233,11 @@ void launch_b_cg_k1_f32(const BGeo &g, int rows, const void *U, bool packed, con
 void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *t,
                         const float2 *p0, const float2 *p1, float2 *x, float2 *r,
                         SolverState *S, BParts P, bool mu, hipStream_t st);
-// (each masked chain's pending FP32 x update applied first, as launch_ru_accum)
+// (on the chains that wait for a reliable update; each one's pending FP32 x
+// update applied first, as launch_ru_accum)
 void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,
                        const float2 *p0, const float2 *p1, const SolverState *S,
-                       const int *mask, hipStream_t st);
+                       hipStream_t st);
 // Multi-shift CG of one right-hand side per chain (M[c]: chain c's state).
 void launch_b_ms_init(const BGeo &g, int rows, double2 *r, double2 *p0,
                       const MultiShiftState *M, BParts P, hipStream_t st);
