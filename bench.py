@@ -386,19 +386,35 @@ def ensemble(Q, torch, ntraj=10, nseq=3):
     out = {"chains": n, "sites": B.sites, "trajectories_timed": ntraj,
            "sequential_trajectories_timed": nseq, "tiles": tiles}
     mark = torch.empty(1, dtype=torch.int16, device="cuda")  # (trace windows, as run())
-    for prec, key in ((Q.PREC_MIXED, "mixed"), (Q.PREC_F64, "fp64")):
+    dhs = {}
+    # the same trajectories with each CG chunk queued as one HIP graph launch
+    # (windows 2 and 3 of a trace): same kernels, so the same bits
+    for prec, key, graphs in ((Q.PREC_MIXED, "mixed", False), (Q.PREC_F64, "fp64", False),
+                              (Q.PREC_MIXED, "graph_mixed", True),
+                              (Q.PREC_F64, "graph_fp64", True)):
+        B.set_graphs(graphs)
         for c in range(n):
             B.set_links(c, U0[c])
+        if graphs:  # (capture every chunk shape once, outside the timed region)
+            B.HMC(am0s, 0.1, 10, 100000, 1e-16, seeds, 2, precision=prec)
+            for c in range(n):
+                B.set_links(c, U0[c])
         mark.fill_(0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        dh = []
         for t in range(ntraj):
-            B.HMC(am0s, 0.1, 10, 100000, 1e-16, seeds, 2 + t, precision=prec)
+            dh += [i.dH for i in B.HMC(am0s, 0.1, 10, 100000, 1e-16, seeds, 2 + t,
+                                       precision=prec)]
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         mark.fill_(1)
+        dhs[key] = dh
         out[f"{key}_chain_traj_per_s"] = round(n * ntraj / dt, 2)
         log(f"ensemble {key}: {n} chains x {ntraj} traj in {dt:.3f}s")
+    B.set_graphs(False)
+    out["graph_same_dH"] = (dhs["graph_mixed"] == dhs["mixed"] and
+                            dhs["graph_fp64"] == dhs["fp64"])
     if nseq <= 0:
         B.close()
         return out

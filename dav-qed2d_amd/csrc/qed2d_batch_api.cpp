@@ -16,6 +16,7 @@
 #include <functional>
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/qed2d.h"
@@ -51,6 +52,13 @@ struct qed2d_batch {
     double ru_delta = kRuDelta;
     std::vector<double> last_dh;  // per chain, from the double-double energies
     std::map<int, GeoBuf> geo;  // key: 2 * rows + pair
+    // CG chunks as HIP graphs (qed2d_batch_set_graphs): one executable graph per
+    // chunk shape, captured on a private stream and launched on `stream`
+    bool graphs = false;
+    hipStream_t cap = nullptr;
+    using GraphKey = std::tuple<int, int, int, int, int, const void *, const void *, int, int,
+                                int, int, int>;
+    std::map<GraphKey, hipGraphExec_t> gexec;
     // fields (arena)
     double2 *U = nullptr, *Ucp = nullptr, *X = nullptr, *F = nullptr, *Y = nullptr;
     double2 *r = nullptr, *p0 = nullptr, *p1 = nullptr, *t = nullptr;
@@ -357,27 +365,58 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     BTRY(ring_setup(b));
     int slot = 0, nq = 0, qhead = 0;
     int qslot[2];
-    auto queue_chunk = [&](int len) -> int {
-        if (mixed && slot_ru) {
+    // One chunk's kernels on stream s.  (The partial-sum counts that the
+    // single-lattice launch functions track on the host live on the device
+    // here, so a chunk's launches do not depend on the iteration: the same
+    // sequence serves every chunk of that length and can be replayed as a graph.)
+    auto issue = [&](hipStream_t s, int len, bool ru_slot) {
+        if (ru_slot) {
             // the paused chains: fold the FP32 partial solution into x,
             // recompute the true residual in FP64, resume at the same K1
-            launch_b_ru_accum(b->dch, b->n, b->vtot, x, b->xs, b->p0s, b->p1s, b->S, st);
-            launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, b->S, st);
+            launch_b_ru_accum(b->dch, b->n, b->vtot, x, b->xs, b->p0s, b->p1s, b->S, s);
+            launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, b->S, s);
             launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, nullptr, b->rs, nullptr, nullptr,
-                              b->S, b->P, 2, st);
+                              b->S, b->P, 2, s);
         }
-        slot_ru = true;
         for (int j = 0; j < len; ++j) {
             if (mixed) {
                 launch_b_cg_k1_f32(g32, b->rows32, b->Us, packed, b->rs, b->p0s, b->p1s, b->ts,
-                                   b->xs, b->S, b->P, st);
+                                   b->xs, b->S, b->P, s);
                 launch_b_cg_k2_f32(g32b, rows32b, b->Us, packed, b->ts, b->p0s, b->p1s, b->xs,
-                                   b->rs, b->S, b->P, shifted, st);
+                                   b->rs, b->S, b->P, shifted, s);
             } else {
-                launch_b_cg_k1_f64(g64, b->rows, b->U, b->r, b->p0, b->p1, b->t, b->S, b->P, st);
+                launch_b_cg_k1_f64(g64, b->rows, b->U, b->r, b->p0, b->p1, b->t, b->S, b->P, s);
                 launch_b_cg_k2_f64(g64b, rows64b, b->U, b->t, b->p0, b->p1, x, b->r, b->S, b->P,
-                                   st);
+                                   s);
             }
+        }
+    };
+    auto queue_chunk = [&](int len) -> int {
+        const bool ru_slot = mixed && slot_ru;
+        slot_ru = true;
+        if (b->graphs) {
+            const qed2d_batch::GraphKey key{len, ru_slot, mixed, packed, shifted, (const void *)x,
+                                            (const void *)src, b->rows, rows64b, b->rows32,
+                                            rows32b, b->rows_q};
+            auto it = b->gexec.find(key);
+            if (it == b->gexec.end()) {
+                if (!b->cap)
+                    BHIP(b, hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+                BHIP(b, hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+                issue(b->cap, len, ru_slot);
+                hipGraph_t g = nullptr;
+                const hipError_t e1 = hipStreamEndCapture(b->cap, &g);
+                hipGraphExec_t ex = nullptr;
+                const hipError_t e2 =
+                    e1 == hipSuccess ? hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) : e1;
+                if (g) (void)hipGraphDestroy(g);
+                if (e2 != hipSuccess)
+                    return bfail(b, QED2D_EHIP, "capturing a CG chunk: %s", hipGetErrorString(e2));
+                it = b->gexec.emplace(key, ex).first;
+            }
+            BHIP(b, hipGraphLaunch(it->second, st));
+        } else {
+            issue(st, len, ru_slot);
         }
         launched_k += len;
         BTRY(blaunched(b));
@@ -690,6 +729,8 @@ void qed2d_batch_destroy(qed2d_batch *b)
     if (b->hSring) (void)hipHostFree(b->hSring);
     if (b->hred) (void)hipHostFree(b->hred);
     for (hipEvent_t e : b->sev) if (e) (void)hipEventDestroy(e);
+    for (auto &kv : b->gexec) (void)hipGraphExecDestroy(kv.second);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
     delete b;
 }
 
@@ -772,6 +813,13 @@ int qed2d_batch_set_tiles_k2(qed2d_batch *b, int cg64, int cg32)
         return bfail(b, QED2D_EINVAL, "tile rows must be 0, 1, 2, 4, 8 or 16");
     b->rows_k2 = cg64;
     b->rows32_k2 = cg32;
+    return QED2D_OK;
+}
+
+int qed2d_batch_set_graphs(qed2d_batch *b, int on)
+{
+    if (!b) return QED2D_EINVAL;
+    b->graphs = on != 0;
     return QED2D_OK;
 }
 

@@ -75,6 +75,7 @@ SIGNATURES = {
     "qed2d_batch_set_tiles_k2": (_i, [_p, _i, _i]),
     "qed2d_batch_get_tiles": (_i, [_p, _p]),
     "qed2d_batch_autotune": (_i, [_p, _i, _p]),
+    "qed2d_batch_set_graphs": (_i, [_p, _i]),
     "qed2d_batch_hot_start": (_i, [_p, _p]),
     "qed2d_batch_cg": (_i, [_p, _p, _p, _p, _i, _d, _i, _p]),
     "qed2d_batch_create_rhs": (_i, [_p, _i, _i, _i, _d, _i, _p]),
@@ -899,6 +900,10 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def set_graphs(self, on=True):
+        """Queue each CG chunk as one HIP graph launch (same kernels, same bits)."""
+        self._check(lib.qed2d_batch_set_graphs(self._b, 1 if on else 0))
 
     def arena_spinor(self):
         """A zero arena-shaped spinor (all chains' fields back to back)."""

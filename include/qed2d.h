@@ -316,6 +316,11 @@ int qed2d_batch_set_tiles(qed2d_batch *b, int q64, int cg64, int cg32);
 int qed2d_batch_set_tiles_k2(qed2d_batch *b, int cg64, int cg32);
 int qed2d_batch_get_tiles(qed2d_batch *b, int tiles[5]);
 int qed2d_batch_autotune(qed2d_batch *b, int iters, int tiles[5]);
+/* on != 0: the batch's CG solves queue each chunk of iterations (its reliable-
+ * update slot and K1/K2 launches) as one HIP graph launch, captured on first
+ * use per chunk shape and kept until qed2d_batch_destroy; the same kernels and
+ * bits as the direct launches (default off).  No counterpart in the reference. */
+int qed2d_batch_set_graphs(qed2d_batch *b, int on);
 /* Hot start of every chain: link angles uniform in [-pi, pi) from (seed[c], c). */
 int qed2d_batch_hot_start(qed2d_batch *b, const uint64_t *seed);
 /* (Q^2) x = src on every chain from x = 0 (CG v1 semantics per chain); x and

@@ -84,6 +84,36 @@ def test_batch_hmc_bit_exact(Q, O, prec, integ):
         assert p[c] == Q.Plaquette(U, prm) and a[c] == Q.Action(U, prm)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_hmc_graphs_bit_exact(Q, O, prec):
+    """CG chunks queued as HIP graph launches (set_graphs) against direct
+    launches: the same kernels replayed, so the same bits, over three
+    trajectories (every chunk shape's graph replayed many times, with the
+    trajectories' other work queued directly between the replays)."""
+    import torch
+    say("batch HMC graphs prec", prec)
+    sizes = [(16, 16), (32, 32), (64, 48)]
+    betas = [2.0, 4.0, 6.0]
+    am0s = [0.1, 0.2, 0.1]
+    seeds = [200, 201, 202]
+    Bs = []
+    for graphs in (False, True):
+        B = Q.Batch(sizes, betas)
+        B.set_tiles(TILES["q64"], TILES["cg64"], TILES["cg32"])
+        B.hot_start(seeds)
+        B.set_graphs(graphs)
+        Bs.append(B)
+    for t in range(3):
+        a = Bs[0].HMC(am0s, 0.1, 10, 100000, 1e-22, seeds, t, precision=prec)
+        b = Bs[1].HMC(am0s, 0.1, 10, 100000, 1e-22, seeds, t, precision=prec)
+        for c in range(len(sizes)):
+            assert (a[c].hini, a[c].hfin, a[c].accepted, a[c].cg_iters) == \
+                (b[c].hini, b[c].hfin, b[c].accepted, b[c].cg_iters), (t, c)
+            assert torch.equal(Bs[0].get_links(c), Bs[1].get_links(c)), (t, c)
+    for B in Bs:
+        B.close()
+
+
 def _ensemble():
     sizes = [(16, 16)] * 16 + [(32, 32)] * 16 + [(64, 64)] * 16 + [(128, 128)] * 8 + \
             [(256, 256)] * 8
