@@ -66,7 +66,7 @@ def test_gamm5Dw_f32(Q, O, L1, L2, bc):
 
 
 @pytest.mark.parametrize("k,L1,L2,bc", [(1, 64, 32, 0), (3, 64, 32, 1), (16, 64, 32, 0),
-                                        (5, 33, 20, 1)])
+                                        (5, 33, 20, 1), (7, 130, 18, 0)])
 def test_gamm5Dw_mrhs(Q, O, k, L1, L2, bc):
     """Every tile height; FP64 and FP32 (pair kernel for even L1, one-site
     kernel for odd L1)."""
