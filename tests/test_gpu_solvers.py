@@ -207,22 +207,23 @@ def test_multicg_R_pseudofermion(Q, O):
     assert abs(Q.dot(so, F, prm).real - O.norm2(X)) < 1e-10 * O.norm2(X)
 
 
-def test_power_method(Q, O):
+@pytest.mark.parametrize("L,am0,iters", [(8, 0.5, 2000), (32, 0.5, 2000), (24, 0.2, 5000)])
+def test_power_method(Q, O, L, am0, iters):
     """power_method (src/QED2dRHMC.jl:135-170).  After the lambda_max loop the
     low modes survive only as rounding noise, so the lambda_min loop amplifies
     device-specific rounding: GPU and oracle are each checked against the dense
-    spectrum on a case where the iteration converges, and against each other
-    for lambda_max."""
+    spectrum on cases where the iteration converges (the oracle to 4e-15 of
+    lambda_max at 32^2 and 24^2), and against each other for lambda_max."""
     from dense import q_dense
-    say("power method")
-    L1, L2, am0 = 8, 8, 0.5
+    say("power method", L, am0, iters)
+    L1, L2 = L, L
     rng = np.random.default_rng(21)
     U = random_links(L1, L2, rng)
     b0 = random_spinor(L1, L2, rng)
     ev = np.linalg.eigvalsh(q_dense(U, am0) @ q_dense(U, am0))
     prm = Q.LattParm((L1, L2), 6.05)
-    lmin, lmax = Q.power_method(dev(U), am0, prm, iter=2000, b0=dev(b0))
-    rmin, rmax = O.power_method(U, b0, am0, 2000)
+    lmin, lmax = Q.power_method(dev(U), am0, prm, iter=iters, b0=dev(b0))
+    rmin, rmax = O.power_method(U, b0, am0, iters)
     assert abs(lmax - rmax) < 1e-12 * rmax
     assert abs(lmax - ev[-1]) < 1e-10 * ev[-1]
     assert abs(lmin - ev[0]) < 1e-8 * ev[-1]

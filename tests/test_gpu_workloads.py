@@ -277,3 +277,28 @@ def test_rhmc_4pf_16poles_omf4_vs_oracle(Q, O, precision):
     assert abs(info.hini - hi) < 1e-9 * abs(hi)
     assert abs(info.dH - (hf - hi)) < 1e-8
     assert info.accepted == a
+
+
+def test_rhmc_32_16poles_4pf_vs_oracle(Q, O):
+    """RHMC with 16 poles and 4 pseudofermions against the oracle on a 32x32
+    lattice (one OMF4 trajectory of 3 steps, CG tol 1e-24), the force's solves
+    in FP64 and in mixed precision, both against one oracle trajectory."""
+    say("RHMC 32^2 16 poles 4 pf OMF4 vs oracle")
+    L = 32
+    U = O.hot_start(L, L, 5)
+    p = O.latt(L, L, 4.0)
+    z = O.zolotarev(16, np.sqrt(0.02), np.sqrt(70.0))
+    hi, hf, a, rc = O.hmc_rhmc(U.copy(), [0.1] * 4, [z] * 4, 0.083, 3, 100000, 1e-24, p, 5,
+                               0, integrator=1)  # (the oracle evolves its links in place)
+    assert rc == 0
+    rp = Q.get_rhmc_params(16, np.sqrt(0.02), np.sqrt(70.0))
+    prm = Q.LattParm((L, L), 4.0)
+    for precision in (0, 1):
+        dU = dev(U)
+        Q.seed(5)
+        info = Q.HMC(dU, [0.1] * 4, 0.083, 3, [], 100000, 1e-24, prm, None, [rp] * 4,
+                     integrator=Q.OMF4(), precision=precision)
+        say("precision", precision, "dH", info.dH, "oracle", hf - hi)
+        assert abs(info.hini - hi) < 1e-9 * abs(hi)
+        assert abs(info.dH - (hf - hi)) < 1e-8
+        assert info.accepted == a
