@@ -129,6 +129,43 @@ void launch_b_force_mom(const BChain *ch, int nchain, long vtot, const double2 *
 }
 
 // ---------------------------------------------------------------------------
+// Momenta and the Nf=2 pseudofermion of every chain in one launch: each chain's
+// k_fill_momenta and k_fill_cgauss (flavour 0) with the chain's own key
+// (src/QED2dHMC.jl:69-70).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) kb_fill_mom_pf(const BChain *ch, int n, long vtot,
+                                                         const uint2 *keys, uint32_t traj,
+                                                         double *__restrict__ mom,
+                                                         double2 *__restrict__ X)
+{
+#pragma clang fp contract(off)
+    const double sq2 = 1.4142135623730951;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < vtot;
+         e += (long)gridDim.x * blockDim.x) {
+        const int c = chain_of_elem(ch, n, 2 * e);
+        const BChain &b = ch[c];
+        const long s = e - b.off, V = b.L.V, o = 2 * b.off;
+        const uint2 k = keys[c];
+        double g1, g2;
+        gauss_pair((uint32_t)s, 0u, kPurposeMom, traj, k.x, k.y, &g1, &g2);
+        mom[o + s] = g1;
+        mom[o + V + s] = g2;
+        for (uint32_t cc = 0; cc < 2; ++cc) {
+            gauss_pair((uint32_t)s, cc, kPurposePF, traj, k.x, k.y, &g1, &g2);
+            X[o + cc * V + s] = mk<double>(g1 / sq2, g2 / sq2);
+        }
+    }
+}
+void launch_b_fill_mom_pf(const BChain *ch, int nchain, long vtot, const uint2 *keys,
+                          uint32_t traj, double *mom, double2 *X, hipStream_t st)
+{
+    long nb = (vtot + kBlock - 1) / kBlock;
+    if (nb > 8192) nb = 8192;
+    launch_k(kb_fill_mom_pf, dim3((int)nb), dim3(kBlock), 0, st, ch, nchain, vtot, keys, traj,
+             mom, X);
+}
+
+// ---------------------------------------------------------------------------
 // CG start from x = 0 (src/QED2dSolver.jl:10-16) on every chain.
 // ---------------------------------------------------------------------------
 template <typename T>

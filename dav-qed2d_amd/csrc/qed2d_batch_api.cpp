@@ -67,7 +67,15 @@ struct qed2d_batch {
     double *mom = nullptr;
     // solver state and partials
     SolverState *S = nullptr, *hS = nullptr;
-    RedOut *hred = nullptr;  // pinned: the queued energy reductions of a trajectory
+    RedOut *hred = nullptr;  // pinned: the energy reductions of a trajectory (5 x n)
+    // per-chain reductions in one launch: producer blocks of chain c are
+    // rblk[k][c] .. rblk[k][c+1]-1 (k = 0: observables over V, 1: dot products
+    // over 2V); partials, the finished sums (5 x n) and the chains' random keys
+    int *rblk[2] = {nullptr, nullptr};
+    int nrblk[2] = {0, 0};
+    double *rpart = nullptr;
+    RedOut *dred = nullptr;
+    uint2 *dkeys = nullptr, *hkeys = nullptr;
     // pipelined solves: two pinned slots of chain states, each with an event
     SolverState *hSring = nullptr;
     hipEvent_t sev[2] = {};
@@ -266,6 +274,41 @@ int ring_setup(qed2d_batch *b)
         hipEventCreateWithFlags(&b->sev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->sev[1], hipEventDisableTiming) != hipSuccess)
         return bfail(b, QED2D_ENOMEM, "pinned state slots of the batched solver");
+    return QED2D_OK;
+}
+
+int red_setup(qed2d_batch *b)
+{
+    if (b->dred) return QED2D_OK;
+    std::vector<int> t[2];
+    for (int k = 0; k < 2; ++k) {
+        t[k].assign(b->n + 1, 0);
+        for (int c = 0; c < b->n; ++c)
+            t[k][c + 1] = t[k][c] + chain_red_blocks((k + 1) * b->hch[c].L.V);
+        b->nrblk[k] = t[k][b->n];
+    }
+    const long np = std::max(6L * b->nrblk[0], 4L * b->nrblk[1]);
+    if (hipMalloc((void **)&b->rblk[0], sizeof(int) * (b->n + 1)) != hipSuccess ||
+        hipMalloc((void **)&b->rblk[1], sizeof(int) * (b->n + 1)) != hipSuccess ||
+        hipMalloc((void **)&b->rpart, sizeof(double) * np) != hipSuccess ||
+        hipMalloc((void **)&b->dkeys, sizeof(uint2) * b->n) != hipSuccess ||
+        hipHostMalloc((void **)&b->hkeys, sizeof(uint2) * b->n) != hipSuccess ||
+        hipHostMalloc((void **)&b->hred, 5 * sizeof(RedOut) * b->n) != hipSuccess ||
+        hipMalloc((void **)&b->dred, sizeof(RedOut) * 5 * b->n) != hipSuccess) {
+        // (release what was allocated: a later call starts over)
+        for (void *p : {(void *)b->rblk[0], (void *)b->rblk[1], (void *)b->rpart,
+                        (void *)b->dkeys, (void *)b->dred})
+            if (p) (void)hipFree(p);
+        if (b->hkeys) (void)hipHostFree(b->hkeys);
+        if (b->hred) (void)hipHostFree(b->hred);
+        b->rblk[0] = b->rblk[1] = nullptr;
+        b->rpart = nullptr;
+        b->dkeys = b->hkeys = nullptr;
+        b->dred = b->hred = nullptr;
+        return bfail(b, QED2D_ENOMEM, "reduction buffers of the batch");
+    }
+    BHIP(b, hipMemcpy(b->rblk[0], t[0].data(), sizeof(int) * (b->n + 1), hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpy(b->rblk[1], t[1].data(), sizeof(int) * (b->n + 1), hipMemcpyHostToDevice));
     return QED2D_OK;
 }
 
@@ -722,12 +765,14 @@ void qed2d_batch_destroy(qed2d_batch *b)
     }
     if (b->shared) b->U = nullptr;  // the caller's links
     void *dev[] = {b->dch, b->U, b->Ucp, b->X, b->F, b->Y, b->r, b->p0, b->p1, b->t, b->Us,
-                   b->rs, b->p0s, b->p1s, b->ts, b->xs, b->mom, b->S, b->part, b->cnt};
+                   b->rs, b->p0s, b->p1s, b->ts, b->xs, b->mom, b->S, b->part, b->cnt, b->rblk[0], b->rblk[1], b->rpart,
+                   b->dred, b->dkeys};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (b->hS) (void)hipHostFree(b->hS);
     if (b->hSring) (void)hipHostFree(b->hSring);
     if (b->hred) (void)hipHostFree(b->hred);
+    if (b->hkeys) (void)hipHostFree(b->hkeys);
     for (hipEvent_t e : b->sev) if (e) (void)hipEventDestroy(e);
     for (auto &kv : b->gexec) (void)hipGraphExecDestroy(kv.second);
     if (b->cap) (void)hipStreamDestroy(b->cap);
@@ -1020,33 +1065,30 @@ int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, i
     BTRY(balloc(b, &b->Y, nel));
     BTRY(balloc(b, &b->mom, nel));
     BTRY(set_masses(b, am0));
+    BTRY(red_setup(b));
     launch_copy(b->Ucp, b->U, nel, st);
     std::vector<double> h0h(b->n), h0l(b->n);
+    // momenta and pseudofermions of every chain (its own key) in one launch; the
+    // pinned keys are free again: the previous call ended with a synchronisation
     for (int c = 0; c < b->n; ++c) {
-        const BChain &ch = b->hch[c];
-        const long o = 2 * ch.off;
         uint32_t k0, k1;
         key_of(seed[c], (uint32_t)c, &k0, &k1);
-        launch_fill_momenta(ch.L, b->mom + o, k0, k1, traj, st);
-        launch_fill_pseudofermion(ch.L, b->X + o, k0, k1, traj, 0, st);
+        b->hkeys[c] = make_uint2(k0, k1);
     }
+    BHIP(b, hipMemcpyAsync(b->dkeys, b->hkeys, sizeof(uint2) * b->n, hipMemcpyHostToDevice, st));
+    launch_b_fill_mom_pf(b->dch, b->n, b->vtot, b->dkeys, traj, b->mom, b->X, st);
     BTRY(blaunched(b));
-    // hini = H + X^dag X  (src/QED2dHMC.jl:74), per chain: the reductions are
-    // queued into pinned slots (5 per chain: H and X^dag X before the MD; H,
-    // Re X^dag F and |Y|^2 after it) and read after the trajectory's last launch
-    if (!b->hred) {
-        if (hipHostMalloc((void **)&b->hred, 5 * sizeof(RedOut) * b->n) != hipSuccess)
-            return bfail(b, QED2D_ENOMEM, "pinned reduction slots of the batch");
-    }
+    // hini = H + X^dag X  (src/QED2dHMC.jl:74), per chain, each reduction one
+    // launch over all chains into dred[k * n + c] (k = 0, 1: H and X^dag X before
+    // the MD; 2, 3, 4: H, Re X^dag F and |Y|^2 after it), read after the
+    // trajectory's last launch
     RedOut *hr = b->hred;
-    for (int c = 0; c < b->n; ++c) {
-        const long o = 2 * b->hch[c].off;
-        int rc = ctx_queue_hamiltonian(b->sub[c], b->mom + o, (const double *)(b->U + o),
-                                       &hr[5 * c]);
-        if (rc) return subfail(b, c, rc);
-        rc = ctx_queue_dot_dd(b->sub[c], (const double *)(b->X + o), nullptr, &hr[5 * c + 1]);
-        if (rc) return subfail(b, c, rc);
-    }
+    const int n = b->n;
+    launch_b_observables(b->dch, b->rblk[0], n, b->nrblk[0], b->U, b->mom, b->rpart, b->dred,
+                         st);
+    launch_b_dot_dd(b->dch, b->rblk[1], n, b->nrblk[1], b->X, nullptr, b->rpart, b->dred + n,
+                    st);
+    BTRY(blaunched(b));
     BGeo gq;
     BTRY(get_geo(b, b->rows_q, false, &gq));
     launch_b_q_apply(gq, b->rows_q, b->U, b->X, b->F, nullptr, st);  // F = Q X  (:76-78)
@@ -1075,37 +1117,34 @@ int qed2d_batch_hmc_nf2(qed2d_batch *b, const double *am0, double eps, int ns, i
         }
         launch_b_q_apply(gq, b->rows_q, b->U, b->X, b->Y, nullptr, st);
     }
-    for (int c = 0; c < b->n; ++c) {
-        const long o = 2 * b->hch[c].off;
-        rc = ctx_queue_hamiltonian(b->sub[c], b->mom + o, (const double *)(b->U + o),
-                                   &hr[5 * c + 2]);
-        if (rc) return subfail(b, c, rc);
-        rc = ctx_queue_dot_dd(b->sub[c], (const double *)(b->X + o), (const double *)(b->F + o),
-                              &hr[5 * c + 3]);
-        if (rc) return subfail(b, c, rc);
-        if (acc_tol > 0.0) {
-            rc = ctx_queue_dot_dd(b->sub[c], (const double *)(b->Y + o), nullptr, &hr[5 * c + 4]);
-            if (rc) return subfail(b, c, rc);
-        }
-    }
+    launch_b_observables(b->dch, b->rblk[0], n, b->nrblk[0], b->U, b->mom, b->rpart,
+                         b->dred + 2 * n, st);
+    launch_b_dot_dd(b->dch, b->rblk[1], n, b->nrblk[1], b->X, b->F, b->rpart, b->dred + 3 * n,
+                    st);
+    if (acc_tol > 0.0)
+        launch_b_dot_dd(b->dch, b->rblk[1], n, b->nrblk[1], b->Y, nullptr, b->rpart,
+                        b->dred + 4 * n, st);
+    BTRY(blaunched(b));
+    BHIP(b, hipMemcpyAsync(hr, b->dred, sizeof(RedOut) * (acc_tol > 0.0 ? 5 : 4) * n,
+                           hipMemcpyDeviceToHost, st));
     BHIP(b, hipStreamSynchronize(st));
     for (int c = 0; c < b->n; ++c) {
         const BChain &ch = b->hch[c];
         const long o = 2 * ch.off;
         dd h0, xx;
-        ctx_hamiltonian_from(b->sub[c], hr[5 * c], &h0.hi, &h0.lo);
-        ctx_dot_dd_from(hr[5 * c + 1], &xx.hi, &xx.lo);
+        ctx_hamiltonian_from(b->sub[c], hr[c], &h0.hi, &h0.lo);
+        ctx_dot_dd_from(hr[n + c], &xx.hi, &xx.lo);
         h0 = dd_add(h0, xx);
         h0h[c] = h0.hi;
         h0l[c] = h0.lo;
         dd h1, re;
-        ctx_hamiltonian_from(b->sub[c], hr[5 * c + 2], &h1.hi, &h1.lo);
-        ctx_dot_dd_from(hr[5 * c + 3], &re.hi, &re.lo);
+        ctx_hamiltonian_from(b->sub[c], hr[2 * n + c], &h1.hi, &h1.lo);
+        ctx_dot_dd_from(hr[3 * n + c], &re.hi, &re.lo);
         if (acc_tol <= 0.0) {
             h1 = dd_add(h1, re);  // hfin = H + Re X^dag F  (:82)
         } else {
             dd yy;
-            ctx_dot_dd_from(hr[5 * c + 4], &yy.hi, &yy.lo);
+            ctx_dot_dd_from(hr[4 * n + c], &yy.hi, &yy.lo);
             h1 = dd_add(h1, dd{2.0 * re.hi, 2.0 * re.lo});
             h1 = dd_add(h1, dd{-yy.hi, -yy.lo});
         }

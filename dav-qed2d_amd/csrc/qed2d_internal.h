@@ -233,6 +233,20 @@ void launch_b_cg_k1_f32(const BGeo &g, int rows, const void *U, bool packed, con
 void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *t,
                         const float2 *p0, const float2 *p1, float2 *x, float2 *r,
                         SolverState *S, BParts P, bool mu, hipStream_t st);
+// Each chain's momenta and Nf=2 pseudofermion from its key keys[c] (one launch).
+void launch_b_fill_mom_pf(const BChain *ch, int nchain, long vtot, const uint2 *keys,
+                          uint32_t traj, double *mom, double2 *X, hipStream_t st);
+// Per-chain double-double reductions in one launch (+ one finishing launch, one
+// block per chain), bit-equal to each chain's single-lattice reduction: chain c
+// owns producer blocks blk0[c] .. blk0[c+1]-1, blk0[c+1] - blk0[c] =
+// chain_red_blocks(V_c) for the observables (H: out[c] as launch_observables)
+// and chain_red_blocks(2 V_c) for dot products (out[c] as launch_dot_dd).
+int chain_red_blocks(long n);
+void launch_b_observables(const BChain *ch, const int *blk0, int n, int nblocks,
+                          const double2 *U, const double *mom, double *part, RedOut *out,
+                          hipStream_t st);
+void launch_b_dot_dd(const BChain *ch, const int *blk0, int n, int nblocks, const double2 *a,
+                     const double2 *b, double *part, RedOut *out, hipStream_t st);
 // (on the chains that wait for a reliable update; each one's pending FP32 x
 // update applied first, as launch_ru_accum)
 void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,

@@ -457,17 +457,18 @@ __device__ __forceinline__ dd block_sum_dd(dd v, double *lds)
 }
 
 // Sum Re P, sum arg P (src/QED2dAction.jl:2-32) and sum p^2 (src/QED2dHMC.jl:200),
-// each as a double-double, per block; optional per-site density outputs.
-__global__ void __launch_bounds__(kBlock) k_observables(Latt L, const double2 *__restrict__ U,
-                                                        const double *__restrict__ mom,
-                                                        double *__restrict__ plx,
-                                                        double *__restrict__ qdens,
-                                                        double *part)
+// each as a double-double, for block blk of nblk over the lattice (grid-stride
+// over its sites); optional per-site density outputs.  Thread 0 writes the
+// block's partials to pp[0..5].
+__device__ __forceinline__ void observables_blk(const Latt &L, const double2 *__restrict__ U,
+                                                const double *__restrict__ mom,
+                                                double *__restrict__ plx,
+                                                double *__restrict__ qdens, double *pp, int blk,
+                                                int nblk, double *lds)
 {
-    __shared__ double lds[2 * (kBlock / kWave)];
     dd sp = {0.0, 0.0}, sq = {0.0, 0.0}, sm = {0.0, 0.0};
-    for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < L.V;
-         s += (long)gridDim.x * blockDim.x) {
+    for (long s = blk * (long)blockDim.x + threadIdx.x; s < L.V;
+         s += (long)nblk * blockDim.x) {
         const int i2 = (int)(s / L.L1), i1 = (int)(s - (long)i2 * L.L1);
         double2 P = plaq_at(U, L, i1, i2);
         double q = atan2(P.y, P.x);
@@ -485,16 +486,25 @@ __global__ void __launch_bounds__(kBlock) k_observables(Latt L, const double2 *_
     dd bq = block_sum_dd(sq, lds);
     dd bm = block_sum_dd(sm, lds);
     if (threadIdx.x == 0) {
-        double *pp = part + 6 * (long)blockIdx.x;
         pp[0] = bp.hi; pp[1] = bp.lo; pp[2] = bq.hi; pp[3] = bq.lo; pp[4] = bm.hi; pp[5] = bm.lo;
     }
 }
-
-// Double-double finish of nq (hi, lo) partial pairs: out->v[2k], v[2k+1].
-__global__ void __launch_bounds__(kBlock) k_finish_dd(const double *part, int nblocks, int nq,
-                                                      RedOut *out)
+__global__ void __launch_bounds__(kBlock) k_observables(Latt L, const double2 *__restrict__ U,
+                                                        const double *__restrict__ mom,
+                                                        double *__restrict__ plx,
+                                                        double *__restrict__ qdens,
+                                                        double *part)
 {
     __shared__ double lds[2 * (kBlock / kWave)];
+    observables_blk(L, U, mom, plx, qdens, part + 6 * (long)blockIdx.x, blockIdx.x, gridDim.x,
+                    lds);
+}
+
+// Double-double finish of nq (hi, lo) partial pairs of nblocks blocks:
+// out->v[2k], v[2k+1].
+__device__ __forceinline__ void finish_dd_blk(const double *part, int nblocks, int nq,
+                                              RedOut *out, double *lds)
+{
     for (int k = 0; k < nq; ++k) {
         dd acc = {0.0, 0.0};
         for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
@@ -508,19 +518,23 @@ __global__ void __launch_bounds__(kBlock) k_finish_dd(const double *part, int nb
         }
     }
 }
-
-// a^dag b or |a|^2 (b == nullptr) accumulated as double-doubles: partials
-// (re hi, re lo, im hi, im lo) per block.  For the fermion terms of H, whose
-// size is ~V and whose differences between two trajectories must be resolved
-// below 1e-9.
-__global__ void __launch_bounds__(kBlock) k_dot_dd(const double2 *__restrict__ a,
-                                                   const double2 *__restrict__ b, long n,
-                                                   double *part)
+__global__ void __launch_bounds__(kBlock) k_finish_dd(const double *part, int nblocks, int nq,
+                                                      RedOut *out)
 {
     __shared__ double lds[2 * (kBlock / kWave)];
+    finish_dd_blk(part, nblocks, nq, out, lds);
+}
+
+// a^dag b or |a|^2 (b == nullptr) over n elements accumulated as double-doubles,
+// block blk of nblk: partials (re hi, re lo, im hi, im lo) to pp[0..3].  For the
+// fermion terms of H, whose size is ~V and whose differences between two
+// trajectories must be resolved below 1e-9.
+__device__ __forceinline__ void dot_dd_blk(const double2 *__restrict__ a,
+                                           const double2 *__restrict__ b, long n, double *pp,
+                                           int blk, int nblk, double *lds)
+{
     dd sr = {0.0, 0.0}, si = {0.0, 0.0};
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-         i += (long)gridDim.x * blockDim.x) {
+    for (long i = blk * (long)blockDim.x + threadIdx.x; i < n; i += (long)nblk * blockDim.x) {
         const double2 x = a[i];
         if (b) {
             const double2 y = b[i];
@@ -536,9 +550,77 @@ __global__ void __launch_bounds__(kBlock) k_dot_dd(const double2 *__restrict__ a
     const dd br = block_sum_dd(sr, lds);
     const dd bi = block_sum_dd(si, lds);
     if (threadIdx.x == 0) {
-        double *pp = part + 4 * (long)blockIdx.x;
         pp[0] = br.hi; pp[1] = br.lo; pp[2] = bi.hi; pp[3] = bi.lo;
     }
+}
+__global__ void __launch_bounds__(kBlock) k_dot_dd(const double2 *__restrict__ a,
+                                                   const double2 *__restrict__ b, long n,
+                                                   double *part)
+{
+    __shared__ double lds[2 * (kBlock / kWave)];
+    dot_dd_blk(a, b, n, part + 4 * (long)blockIdx.x, blockIdx.x, gridDim.x, lds);
+}
+
+// ---------------------------------------------------------------------------
+// The same reductions over the chains of a chain set, one launch for all:
+// chain c owns blocks blk0[c] .. blk0[c+1]-1, as many as its single-lattice
+// reduction launches (red_blocks), each doing what that launch's block of the
+// same index does, so every chain's sums carry the single-lattice bits.  The
+// finishing kernel runs one block per chain.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int chain_of_block(const int *blk0, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk0[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+__global__ void __launch_bounds__(kBlock) kb_observables(const BChain *ch, const int *blk0, int n,
+                                                         const double2 *__restrict__ U,
+                                                         const double *__restrict__ mom,
+                                                         double *part)
+{
+    __shared__ double lds[2 * (kBlock / kWave)];
+    const int c = chain_of_block(blk0, n, blockIdx.x);
+    const BChain &b = ch[c];
+    observables_blk(b.L, U + 2 * b.uoff, mom ? mom + 2 * b.off : nullptr, nullptr, nullptr,
+                    part + 6 * (long)blockIdx.x, blockIdx.x - blk0[c], blk0[c + 1] - blk0[c],
+                    lds);
+}
+__global__ void __launch_bounds__(kBlock) kb_dot_dd(const BChain *ch, const int *blk0, int n,
+                                                    const double2 *__restrict__ a,
+                                                    const double2 *__restrict__ b,
+                                                    double *part)
+{
+    __shared__ double lds[2 * (kBlock / kWave)];
+    const int c = chain_of_block(blk0, n, blockIdx.x);
+    const long o = 2 * ch[c].off;
+    dot_dd_blk(a + o, b ? b + o : nullptr, 2 * ch[c].L.V, part + 4 * (long)blockIdx.x,
+               blockIdx.x - blk0[c], blk0[c + 1] - blk0[c], lds);
+}
+__global__ void __launch_bounds__(kBlock) kb_finish_dd(const int *blk0, const double *part,
+                                                       int nq, RedOut *out)
+{
+    __shared__ double lds[2 * (kBlock / kWave)];
+    const int c = blockIdx.x;
+    finish_dd_blk(part + 2L * nq * blk0[c], blk0[c + 1] - blk0[c], nq, out + c, lds);
+}
+int chain_red_blocks(long n) { return red_blocks(n); }
+void launch_b_observables(const BChain *ch, const int *blk0, int n, int nblocks,
+                          const double2 *U, const double *mom, double *part, RedOut *out,
+                          hipStream_t st)
+{
+    launch_k(kb_observables, dim3(nblocks), dim3(kBlock), 0, st, ch, blk0, n, U, mom, part);
+    launch_k(kb_finish_dd, dim3(n), dim3(kBlock), 0, st, blk0, (const double *)part, 3, out);
+}
+void launch_b_dot_dd(const BChain *ch, const int *blk0, int n, int nblocks, const double2 *a,
+                     const double2 *b, double *part, RedOut *out, hipStream_t st)
+{
+    launch_k(kb_dot_dd, dim3(nblocks), dim3(kBlock), 0, st, ch, blk0, n, a, b, part);
+    launch_k(kb_finish_dd, dim3(n), dim3(kBlock), 0, st, blk0, (const double *)part, 2, out);
 }
 
 void launch_dot_dd(const double2 *a, const double2 *b, long n, RedOut *out, double *part,
