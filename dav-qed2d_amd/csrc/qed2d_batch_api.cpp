@@ -356,7 +356,6 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     std::vector<double> rem(b->n);    // per chain: iterations its solve has left
     int chunk = chunk0;
     long launched_k = 0;
-    bool slot_ru = false;  // mixed: the next chunk starts with a reliable-update slot
     auto start = [&]() -> int {
         for (int c = 0; c < b->n; ++c) {
             SolverState &s = b->hS[c];
@@ -393,21 +392,24 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         pred.assign(b->n, Predict());
         chunk = chunk0;
         launched_k = 0;
-        slot_ru = false;
         return QED2D_OK;
     };
     // Pipelined: behind the chunk whose states the host waits for, the next one
     // is already queued (not behind the first, whose look may restart the solve
     // with other links).  Each chunk's states land in a pinned slot of two, with
-    // an event.  In mixed precision every chunk but a solve's first starts with
-    // a reliable-update slot: its kernels act on the chains whose K1 paused for
-    // an update (need_ru) and return at once on the others, so a chain that
-    // paused in one chunk resumes at the start of the next without waiting for
-    // the host to see it, and takes the same steps as a single-lattice solve
-    // whose host performs the update at the pause.
+    // an event.  In mixed precision a chunk may start with a reliable-update
+    // slot: its kernels act on the chains whose K1 paused for an update
+    // (need_ru) and return at once on the others, so a chain that paused in the
+    // chunk before resumes without waiting for the host to see it, and takes
+    // the same steps as a single-lattice solve whose host performs the update
+    // at the pause.  A chunk gets a slot when the states the host last read
+    // show a chain paused, or predict a chain to pause in the chunk in flight
+    // (a slot costs about one iteration over all chains even when idle); a
+    // chain that pauses unannounced is seen by the next look and served by
+    // the chunk queued then.
     BTRY(ring_setup(b));
     int slot = 0, nq = 0, qhead = 0;
-    int qslot[2];
+    int qslot[2], qlen[2];
     // One chunk's kernels on stream s.  (The partial-sum counts that the
     // single-lattice launch functions track on the host live on the device
     // here, so a chunk's launches do not depend on the iteration: the same
@@ -434,9 +436,8 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             }
         }
     };
-    auto queue_chunk = [&](int len) -> int {
-        const bool ru_slot = mixed && slot_ru;
-        slot_ru = true;
+    auto queue_chunk = [&](int len, bool ru_slot) -> int {
+        ru_slot = ru_slot && mixed;
         if (b->graphs) {
             const qed2d_batch::GraphKey key{len, ru_slot, mixed, packed, shifted, (const void *)x,
                                             (const void *)src, b->rows, rows64b, b->rows32,
@@ -467,15 +468,16 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
                                hipMemcpyDeviceToHost, st));
         BHIP(b, hipEventRecord(b->sev[slot], st));
         qslot[(qhead + nq) % 2] = slot;
+        qlen[(qhead + nq) % 2] = len;
         nq += 1;
         slot ^= 1;
         return QED2D_OK;
     };
     BTRY(start());
-    BTRY(queue_chunk(chunk));
+    BTRY(queue_chunk(chunk, false));
     bool first_look = true;
     for (;;) {
-        if (!first_look && nq == 1) BTRY(queue_chunk(chunk));
+        if (!first_look && nq == 1) BTRY(queue_chunk(chunk, true));
         const int s0 = qslot[qhead];
         qhead = (qhead + 1) % 2;
         nq -= 1;
@@ -485,7 +487,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         if (first_look && packed && b->hS[0].bad_links) {
             packed = false;
             BTRY(start());
-            BTRY(queue_chunk(chunk));
+            BTRY(queue_chunk(chunk, false));
             continue;
         }
         first_look = false;
@@ -497,8 +499,9 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         // pauses mid-chunk idles, which delays the batch only if it is near the
         // longest).
         constexpr int kSlack = 16;
-        bool running = false;
+        bool running = false, ru_slot = false;
         double rlong = 0.0;
+        const int inflight = nq > 0 ? qlen[qhead] : 0;  // iterations of the chunk in flight
         for (int c = 0; c < b->n; ++c) {
             const SolverState &h = b->hS[c];
             ev[c] = 0;
@@ -506,9 +509,12 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             running = true;
             if (h.need_ru) {
                 ev[c] = pred[c].after_ru(h.err, h.tol, h.delta2);
+                ru_slot = true;
             } else {
                 const double rut = mixed ? h.delta2 * h.rmax2[(h.iter + 1) & 1] : 0.0;
                 ev[c] = pred[c].next(h.err, h.iter, h.tol, rut);
+                // (its pause, at the K1 ev[c] - 1 iterations on, falls in the chunk in flight)
+                if (ev[c] <= inflight) ru_slot = true;
             }
             rem[c] = pred[c].remaining(h.err, h.tol);
             rlong = std::max(rlong, rem[c]);
@@ -523,7 +529,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         if (launched_k > 4L * (maxiter + 2) + 64)
             return bfail(b, QED2D_EHIP, "batched CG: chains stopped advancing");
         chunk = next;
-        BTRY(queue_chunk(chunk));
+        BTRY(queue_chunk(chunk, ru_slot));
     }
     iters.assign(b->n, 0);
     int bad = -1;
