@@ -1038,6 +1038,7 @@ def rocprof_summary(argv):
         print()
         print("| window | kernels | span ms | busy ms | idle ms | idle ms per trajectory |")
         print("|---|---|---|---|---|---|")
+        per = []
         for w, (i0, i1) in enumerate(zip(marks[0::2], marks[1::2])):
             inner = ks[i0 + 1:i1]
             if not inner:
@@ -1046,6 +1047,20 @@ def rocprof_summary(argv):
             busy = sum(e - s0 for _, s0, e in inner) / 1e6
             print(f"| {w} | {len(inner)} | {span:.2f} | {busy:.2f} | {span - busy:.2f} | "
                   f"{(span - busy) / a.windows:.3f} |")
+            by_k = {}
+            for n, s0, e in inner:
+                t = by_k.setdefault(_short(n), [0, 0.0])
+                t[0] += 1
+                t[1] += (e - s0) / 1e3
+            per.append((w, by_k))
+        for w, by_k in per:
+            print()
+            print(f"Window {w}, per trajectory: launches and GPU time by kernel")
+            print()
+            print("| kernel | launches | us | mean us |")
+            print("|---|---|---|---|")
+            for n, (cnt, us) in sorted(by_k.items(), key=lambda kv: -kv[1][1])[:14]:
+                print(f"| {n} | {cnt / a.windows:.1f} | {us / a.windows:.0f} | {us / cnt:.2f} |")
 
     if not (a.fetch and a.write):
         return
