@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(kBlock) k_q_apply(Latt L, const c2<T> *__restr
                                                     c2<T> *__restrict__ so, T mass2, int bx,
                                                     int by, int nbx)
 {
-    MarchPos p = march_pos(L, bx, by, nbx, R);
+    MarchPos p = march_pos(L, bx, by, nbx, R, apply_bands(L));
     if (!p.active) return;
     SrcPlain<T> src{si, L.V};
     EpiStore<T> epi{so, L.V};
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(kBlock) k_q_apply2(Latt L, const float2 *__res
                                                      float2 *__restrict__ so, float mass2, int bx,
                                                      int by, int nbx)
 {
-    MarchPos p = march_pos2(L, bx, by, nbx, R);
+    MarchPos p = march_pos2(L, bx, by, nbx, R, apply_bands(L));
     if (!p.active) return;
     SrcPlain2 src{si, L.V};
     EpiStore2 epi{so, L.V};

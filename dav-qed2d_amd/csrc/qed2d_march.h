@@ -20,11 +20,22 @@ struct MarchPos {
     int blk;  // logical block index (partial-sum slot)
 };
 
-__device__ __forceinline__ MarchPos march_pos(const Latt &L, int bx, int by, int nbx, int rows)
+// Block order of the plain Q applies.  XCD bands (xcd_remap) keep the
+// vertical neighbours of a tile in one L2, which pays while the apply's
+// operands are largely served from the caches; from 4096^2 sites on, launch
+// order runs faster (same box, best tile of 2, 4, 8, TB/s, bands -> launch
+// order: 4096^2 FP64 5.89 -> 6.00, FP32 5.70 -> 5.98; 6144^2 5.76 -> 5.97,
+// 5.56 -> 5.68; 8192^2 5.80 -> 5.83, 5.79 -> 5.78; at 2048^2 bands win, 7.19
+// and 6.87 against 6.81 and 6.02).  The solver kernels keep the bands at every
+// size (launch order at 4096^2: mixed Nf=2 trajectory 1.5 % slower).
+__device__ __forceinline__ bool apply_bands(const Latt &L) { return L.V < (1L << 24); }
+
+__device__ __forceinline__ MarchPos march_pos(const Latt &L, int bx, int by, int nbx, int rows,
+                                              bool bands = true)
 {
     MarchPos p;
     const int nb = gridDim.x;
-    const int b = xcd_remap(blockIdx.x, nb);
+    const int b = bands ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
     const int bxi = b % nbx, byi = b / nbx;
     const int tx = threadIdx.x % bx, ty = threadIdx.x / bx;
     p.i1 = bxi * bx + tx;
@@ -273,11 +284,12 @@ __device__ __forceinline__ F2x2 neg2(F2x2 v)
     return v;
 }
 
-__device__ __forceinline__ MarchPos march_pos2(const Latt &L, int bx, int by, int nbx, int rows)
+__device__ __forceinline__ MarchPos march_pos2(const Latt &L, int bx, int by, int nbx, int rows,
+                                               bool bands = true)
 {
     MarchPos p;
     const int nb = gridDim.x;
-    const int b = xcd_remap(blockIdx.x, nb);
+    const int b = bands ? xcd_remap(blockIdx.x, nb) : (int)blockIdx.x;
     const int bxi = b % nbx, byi = b / nbx;
     const int tx = threadIdx.x % bx, ty = threadIdx.x / bx;
     p.i1 = bxi * bx + tx;  // pair index j

@@ -45,6 +45,31 @@ def test_gamm5Dw_large(Q, O):
     assert relerr(host(so), O.gamm5Dw(U, si, 0.1)) < 1e-13
 
 
+@pytest.mark.parametrize("L1,L2,bc", [(4096, 4096, 1), (4100, 4096, 0)])
+def test_gamm5Dw_launch_order(Q, O, L1, L2, bc):
+    """From 4096^2 sites on the plain applies run their blocks in launch order
+    instead of XCD bands: FP64 and FP32 at two tile heights, against the
+    oracle, on a square lattice and on one whose width is not a multiple of
+    the block width."""
+    say("gamm5Dw launch order", L1, L2, bc)
+    rng = np.random.default_rng(L1 + bc)
+    U = random_links(L1, L2, rng, unitary=True)
+    si = random_spinor(L1, L2, rng)
+    prm = Q.LattParm((L1, L2), 4.0, bc)
+    ref = O.gamm5Dw(U, si, 0.1, bc)
+    dU, dsi = dev(U), dev(si)
+    dU32, dsi32 = dev(U.astype(np.complex64)), dev(si.astype(np.complex64))
+    for rows in (2, 8):
+        Q.set_tiles(prm, q64=rows, q32=rows)
+        so = Q.spinor(prm)
+        Q.gamm5Dw(so, dU, dsi, 0.1, prm)
+        assert relerr(host(so), ref) < 1e-13, rows
+        so32 = dev(np.zeros_like(si, dtype=np.complex64))
+        Q.gamm5Dw_f32(so32, dU32, dsi32, 0.1, prm)
+        assert relerr(host(so32), ref) < 2e-6, rows
+    prm.close()
+
+
 @pytest.mark.parametrize("L1,L2,bc", [(24, 24, 0), (128, 64, 1), (4, 6, 0), (25, 16, 1),
                                        (2, 8, 0), (66, 33, 0)])
 def test_gamm5Dw_f32(Q, O, L1, L2, bc):
