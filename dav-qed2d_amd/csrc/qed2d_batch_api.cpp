@@ -605,6 +605,9 @@ int batch_autotune(qed2d_batch *b, int iters)
     // and make no reliable update before their last iteration)
     const double keep_delta = b->ru_delta;
     b->ru_delta = 1e-30;
+    // (direct launches: a graph per tile pair and chunk would only fill the cache)
+    const bool keep_graphs = b->graphs;
+    b->graphs = false;
     for (int prec = 0; prec < 2 && !rc; ++prec) {
         if (prec == 1 && !pairs) break;  // the FP32 batch needs even L1 >= 4 everywhere
         int &t1 = prec ? b->rows32 : b->rows;
@@ -629,6 +632,7 @@ int batch_autotune(qed2d_batch *b, int iters)
         t2 = rc ? keep2 : (p2 == p1 ? 0 : p2);
     }
     b->ru_delta = keep_delta;
+    b->graphs = keep_graphs;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return bfail(b, rc, "batch autotune failed: %s", b->err.c_str());
