@@ -114,7 +114,7 @@ def main():
         stream = Q.hbm_stream(sprm, 1 << 30)
         sprm.close()
         print(json.dumps({"hbm_stream_GBs": {k: round(v, 1) for k, v in stream.items()},
-                          "dw_apply": dw_apply(Q, torch, args)}), flush=True)
+                          "dw_apply": dw_apply(Q, torch, args, stream)}), flush=True)
         return
     if args.leg == "ensemble":
         print(json.dumps(ensemble(Q, torch, nseq=args.ens_seq)), flush=True)
@@ -253,7 +253,7 @@ def main():
         pool.submit("nf2_1024", U=U0.cpu().numpy(), m0=args.m0, eps=args.eps, ns=args.ns,
                     tol=args.tol, beta=args.beta, seed=seed)
     if not args.no_dw:
-        res["dw_apply"] = dw_apply(Q, torch, args)
+        res["dw_apply"] = dw_apply(Q, torch, args, stream)
     if not args.no_configs:
         res["configs"] = config_measurements(Q, torch, pool, args)
 
@@ -301,7 +301,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def dw_apply(Q, torch, args):
+def dw_apply(Q, torch, args, stream=None):
     """gamm5Dw apply bandwidth at the shapes the HBM claim is made on
     (1024^2 x 16 RHS, 4096^2, 8192^2; 1024^2 x 1 RHS is cache-assisted), FP64
     and FP32, timed with HIP events on the context's stream; every tile
@@ -352,6 +352,11 @@ def dw_apply(Q, torch, args):
         out[key] = {"us": round(us, 2), "GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 3),
                     "bytes_per_site": bytes_site, "tile": rows,
                     "us_per_tile": {str(r): round(v, 2) for r, v in per_tile.items()}}
+        if stream:
+            # against the measured copy of the same access pattern: one array
+            # (single RHS) or 16 arrays at once (16 RHS)
+            ref = stream["copy16"] if k > 1 else stream["copy"]
+            out[key]["frac_of_measured_copy"] = round(gbs / ref, 3)
         log(f"dw {key}: {us:.1f} us, {gbs:.0f} GB/s (tile {rows})")
         del U, si, so
         prm.close()

@@ -255,10 +255,26 @@ __global__ void __launch_bounds__(kBlock) k_stream_read(const double2 *__restric
     acc = block_sum(acc, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
+// k arrays of m elements each (back to back) copied into k others: block t
+// copies 256 elements of array t % k, so consecutive blocks stream from k
+// places at once.
+__global__ void __launch_bounds__(kBlock) k_stream_copy_k(const double2 *__restrict__ a,
+                                                          double2 *__restrict__ b, long m, int k)
+{
+    const long t = blockIdx.x;
+    const long e = (t / k) * kBlock + threadIdx.x;
+    if (e < m) b[(t % k) * m + e] = a[(t % k) * m + e];
+}
 void launch_stream(int which, const double2 *a, const double2 *b, double2 *c, double *part,
                    long n, hipStream_t st)
 {
     const long nb = (n + kBlock - 1) / kBlock;
+    if (which == 3) {
+        const long m = n / 16;  // (n: a multiple of 16 x 256)
+        launch_k(k_stream_copy_k, dim3(16 * ((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                 a, c, m, 16);
+        return;
+    }
     if (which == 0)
         launch_k(k_stream_copy, dim3(nb), dim3(kBlock), 0, st, a, c, n);
     else if (which == 1)

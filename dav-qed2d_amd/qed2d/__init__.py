@@ -188,10 +188,11 @@ def autotune(prm, iters=40):
 
 
 def hbm_stream(prm, nbytes=1 << 30):
-    """STREAM copy / triad / read bandwidth of the device in GB/s."""
-    g = (_d * 3)()
+    """STREAM copy / triad / read bandwidth of the device in GB/s, and the copy
+    of the same bytes as 16 arrays into 16 others (copy16)."""
+    g = (_d * 4)()
     _check(prm, lib.qed2d_hbm_stream(prm.ctx(), int(nbytes), g))
-    return dict(zip(("copy", "triad", "read"), list(g)))
+    return dict(zip(("copy", "triad", "read", "copy16"), list(g)))
 
 
 def profile_enable(prm, on=True, every=1):

@@ -102,9 +102,12 @@ int qed2d_ctx_get_tiles_k2(qed2d_ctx *ctx, int tiles[2]);
 int qed2d_ctx_autotune(qed2d_ctx *ctx, int iters, int tiles[4]);
 /* Achievable HBM bandwidth of the context's device, for the roofline beside
  * the datasheet peak: STREAM copy, triad and read over three scratch arrays of
- * nbytes each (allocated and freed here); gbs[3] receives GB/s counting the
- * bytes each kernel must move.  No counterpart in the reference. */
-int qed2d_hbm_stream(qed2d_ctx *ctx, long nbytes, double gbs[3]);
+ * nbytes each (allocated and freed here), and the copy of the same bytes as 16
+ * arrays into 16 others, each block copying 256 elements of one array and
+ * consecutive blocks taking consecutive arrays (the concurrent streams of a
+ * 16-right-hand-side apply); gbs[4] receives GB/s counting the bytes each
+ * kernel must move.  No counterpart in the reference. */
+int qed2d_hbm_stream(qed2d_ctx *ctx, long nbytes, double gbs[4]);
 /* dH = hfin - hini of the last trajectory of qed2d_hmc_pure / _nf2 / _rhmc on
  * this context, formed from the double-double energies before they are
  * rounded (hfin - hini of the returned doubles resolves only the ulp of H,
