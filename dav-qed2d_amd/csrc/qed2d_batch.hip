@@ -232,7 +232,8 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
                                                       float2 *__restrict__ rs,
                                                       double2 *__restrict__ p0,
                                                       float2 *__restrict__ p0s, SolverState *S,
-                                                      BParts P, int mode)
+                                                      BParts P, int mode, double2 *xout,
+                                                      float2 *xs0)
 {
     const BWhere w = bwhere<false>(g, R);
     SolverState *Sc = S + w.c;
@@ -240,7 +241,8 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
     const int k = (mode == 2) ? Sc->kk : 1;
     const long o = w.o;
     EpiResid epi{b + o, x + o, r ? r + o : nullptr, rs ? rs + o : nullptr,
-                 p0 ? p0 + o : nullptr, p0s ? p0s + o : nullptr, Sc->mu2, w.L.V, 0.0, 0.0};
+                 p0 ? p0 + o : nullptr, p0s ? p0s + o : nullptr, Sc->mu2, w.L.V, 0.0, 0.0,
+                 xout ? xout + o : nullptr, xs0 ? xs0 + o : nullptr};
     if (w.p.active) {
         SrcPlain<double> src{t + o, w.L.V};
         q_march<double, R>(w.L, U + w.uo, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
@@ -264,48 +266,82 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
 }
 void launch_b_cg_resid(const BGeo &g, int rows, const double2 *U, const double2 *b,
                        const double2 *t, const double2 *x, double2 *r, float2 *rs, double2 *p0,
-                       float2 *p0s, SolverState *S, BParts P, int mode, hipStream_t st)
+                       float2 *p0s, SolverState *S, BParts P, int mode, hipStream_t st,
+                       double2 *xout, float2 *xs0)
 {
     DISPATCH_ROWS(rows, kb_cg_resid, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, t, b, x, r, rs,
-                  p0, p0s, S, P, mode);
+                  p0, p0s, S, P, mode, xout, xs0);
 }
 
-// x += double(xs), xs = 0 on the chains that wait for a reliable update
-// (need_ru), each chain's pending xs += alpha_x p_{stop_k - 1} applied first
-// (the site-pair K1 paused before applying it).  A block's 256 elements that
-// lie in one chain not waiting for an update are skipped as a whole, so the
-// launch costs little when no chain waits.
-__global__ void __launch_bounds__(kBlock) kb_ru_accum(const BChain *ch, int n, long nel,
-                                                      double2 *__restrict__ x,
-                                                      float2 *__restrict__ xs,
+// The first kernel of a batched reliable update, on the chains that wait for
+// one (need_ru): t = Q x' and xn = x' with x' = x + double(xs + alpha_x p),
+// the new FP64 solution, formed where it is read (with the pending update of
+// the site-pair K1 that paused, in the expressions of the single-lattice
+// k_ru_accum, so the bits are the same).  kb_cg_resid in mode 2 then takes x'
+// from xn, stores it into x and clears xs: two launches where an accumulation
+// pass, a Q apply and the residual took three (ensemble 925 -> 948
+// chain-trajectories/s).
+struct SrcXNew {
+    const double2 *__restrict__ x;
+    const float2 *__restrict__ xs;
+    const float2 *__restrict__ p;
+    float alpha;
+    long V;
+    struct Raw {
+        double2 x0, x1;
+        float2 s0, s1, p0, p1;
+    };
+    __device__ __forceinline__ Raw fetch(long i) const
+    {
+        return Raw{x[i], x[V + i], xs[i], xs[V + i], p[i], p[V + i]};
+    }
+    __device__ __forceinline__ void value(const Raw &w, double2 &a, double2 &b) const
+    {
+        const float2 d0 = xpa(w.s0, alpha, w.p0), d1 = xpa(w.s1, alpha, w.p1);
+        a = mk<double>(w.x0.x + (double)d0.x, w.x0.y + (double)d0.y);
+        b = mk<double>(w.x1.x + (double)d1.x, w.x1.y + (double)d1.y);
+    }
+};
+struct EpiTX {
+    double2 *__restrict__ t;
+    double2 *__restrict__ xn;
+    long V;
+    __device__ __forceinline__ NoPre pre(long) const { return {}; }
+    __device__ __forceinline__ void operator()(long s, NoPre, double2 o1, double2 o2, double2 c1,
+                                               double2 c2v)
+    {
+        t[s] = o1;
+        t[V + s] = o2;
+        xn[s] = c1;
+        xn[V + s] = c2v;
+    }
+};
+template <int R>
+__global__ void __launch_bounds__(kBlock) kb_ru_apply(BGeo g, const double2 *__restrict__ U,
+                                                      const double2 *__restrict__ x,
+                                                      const float2 *__restrict__ xs,
                                                       const float2 *__restrict__ P0,
                                                       const float2 *__restrict__ P1,
+                                                      double2 *__restrict__ t,
+                                                      double2 *__restrict__ xn,
                                                       const SolverState *S)
 {
-    for (long e0 = blockIdx.x * (long)kBlock; e0 < nel; e0 += (long)gridDim.x * kBlock) {
-        const int c0 = chain_of_elem(ch, n, e0);
-        const int c1 = chain_of_elem(ch, n, (e0 + kBlock <= nel ? e0 + kBlock : nel) - 1);
-        if (c0 == c1 && !S[c0].need_ru) continue;
-        const long e = e0 + threadIdx.x;
-        if (e >= nel) continue;
-        const int c = (c0 == c1) ? c0 : chain_of_elem(ch, n, e);
-        const SolverState *Sc = S + c;
-        if (!Sc->need_ru) continue;
-        const float2 *pp = ((Sc->stop_k - 1) & 1) ? P1 : P0;
-        const float2 d = xpa(xs[e], (float)Sc->alpha_x, pp[e]);
-        const double2 v = x[e];
-        x[e] = mk<double>(v.x + (double)d.x, v.y + (double)d.y);
-        xs[e] = mk<float>(0.0f, 0.0f);
-    }
+    const BWhere w = bwhere<false>(g, R);
+    if (!w.p.active) return;
+    const SolverState *Sc = S + w.c;
+    if (!Sc->need_ru) return;
+    const long o = w.o;
+    const float2 *pp = ((Sc->stop_k - 1) & 1) ? P1 : P0;
+    SrcXNew src{x + o, xs + o, pp + o, (float)Sc->alpha_x, w.L.V};
+    EpiTX epi{t + o, xn + o, w.L.V};
+    q_march<double, R>(w.L, U + w.uo, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
 }
-void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,
-                       const float2 *p0, const float2 *p1, const SolverState *S,
-                       hipStream_t st)
+void launch_b_ru_apply(const BGeo &g, int rows, const double2 *U, const double2 *x,
+                       const float2 *xs, const float2 *p0, const float2 *p1, double2 *t,
+                       double2 *xn, const SolverState *S, hipStream_t st)
 {
-    long nb = (2 * vtot + kBlock - 1) / kBlock;
-    if (nb > 8192) nb = 8192;
-    launch_k(kb_ru_accum, dim3((int)nb), dim3(kBlock), 0, st, ch, nchain, 2 * vtot, x,
-                       xs, p0, p1, S);
+    DISPATCH_ROWS(rows, kb_ru_apply, dim3(g.nblocks), dim3(kBlock), 0, st, g, U, x, xs, p0, p1,
+                  t, xn, S);
 }
 
 // ---------------------------------------------------------------------------

@@ -64,6 +64,7 @@ struct qed2d_batch {
     double2 *r = nullptr, *p0 = nullptr, *p1 = nullptr, *t = nullptr;
     float2 *Us = nullptr, *rs = nullptr, *p0s = nullptr, *p1s = nullptr, *ts = nullptr,
            *xs = nullptr;
+    double2 *xn = nullptr;  // a reliable update's new FP64 solution (mixed precision)
     double *mom = nullptr;
     // solver state and partials
     SolverState *S = nullptr, *hS = nullptr;
@@ -344,6 +345,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         BTRY(balloc(b, &b->p1s, nel));
         BTRY(balloc(b, &b->ts, nel));
         BTRY(balloc(b, &b->xs, nel));
+        BTRY(balloc(b, &b->xn, nel));
         BTRY(get_geo(b, b->rows32, true, &g32));
         BTRY(get_geo(b, rows32b, true, &g32b));
     }
@@ -418,10 +420,10 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         if (ru_slot) {
             // the paused chains: fold the FP32 partial solution into x,
             // recompute the true residual in FP64, resume at the same K1
-            launch_b_ru_accum(b->dch, b->n, b->vtot, x, b->xs, b->p0s, b->p1s, b->S, s);
-            launch_b_q_apply(gq, b->rows_q, b->U, x, b->t, b->S, s);
-            launch_b_cg_resid(g64, b->rows, b->U, src, b->t, x, nullptr, b->rs, nullptr, nullptr,
-                              b->S, b->P, 2, s);
+            launch_b_ru_apply(gq, b->rows_q, b->U, x, b->xs, b->p0s, b->p1s, b->t, b->xn, b->S,
+                              s);
+            launch_b_cg_resid(g64, b->rows, b->U, src, b->t, b->xn, nullptr, b->rs, nullptr,
+                              nullptr, b->S, b->P, 2, s, x, b->xs);
         }
         for (int j = 0; j < len; ++j) {
             if (mixed) {
@@ -776,7 +778,7 @@ void qed2d_batch_destroy(qed2d_batch *b)
     if (b->shared) b->U = nullptr;  // the caller's links
     void *dev[] = {b->dch, b->U, b->Ucp, b->X, b->F, b->Y, b->r, b->p0, b->p1, b->t, b->Us,
                    b->rs, b->p0s, b->p1s, b->ts, b->xs, b->mom, b->S, b->part, b->cnt, b->rblk[0], b->rblk[1], b->rpart,
-                   b->dred, b->dkeys};
+                   b->dred, b->dkeys, b->xn};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (b->hS) (void)hipHostFree(b->hS);

@@ -279,6 +279,10 @@ struct EpiResid {
     double mu2;
     long V;
     double rr, bb;
+    // (batched reliable update: x read from the updated copy is stored back
+    // into xout, and the FP32 partial solution xs0 cleared; null otherwise)
+    double2 *__restrict__ xout = nullptr;
+    float2 *__restrict__ xs0 = nullptr;
     struct Pre {
         double2 b1, b2, x1, x2;
     };
@@ -290,6 +294,8 @@ struct EpiResid {
         if (rs) rs[i] = mk<float>((float)rv.x, (float)rv.y);
         if (p0) p0[i] = mk<double>(0.0, 0.0);
         if (p0s) p0s[i] = mk<float>(0.0f, 0.0f);
+        if (xout) xout[i] = xv;
+        if (xs0) xs0[i] = mk<float>(0.0f, 0.0f);
         rr += rv.x * rv.x + rv.y * rv.y;
         bb += bv.x * bv.x + bv.y * bv.y;
     }

@@ -220,7 +220,13 @@ void launch_b_cg_init(const BGeo &g, int rows, const double2 *b, double2 *x, voi
                       float2 *xs, int mixed, BParts P, hipStream_t st);
 void launch_b_cg_resid(const BGeo &g, int rows, const double2 *U, const double2 *b,
                        const double2 *t, const double2 *x, double2 *r, float2 *rs, double2 *p0,
-                       float2 *p0s, SolverState *S, BParts P, int mode, hipStream_t st);
+                       float2 *p0s, SolverState *S, BParts P, int mode, hipStream_t st,
+                       double2 *xout = nullptr, float2 *xs0 = nullptr);
+// The first kernel of a batched reliable update (chains with need_ru):
+// t = Q x', xn = x', x' = x + double(xs + alpha_x p_{stop_k - 1}).
+void launch_b_ru_apply(const BGeo &g, int rows, const double2 *U, const double2 *x,
+                       const float2 *xs, const float2 *p0, const float2 *p1, double2 *t,
+                       double2 *xn, const SolverState *S, hipStream_t st);
 void launch_b_cg_k1_f64(const BGeo &g, int rows, const double2 *U, const double2 *r,
                         double2 *p0, double2 *p1, double2 *t, SolverState *S, BParts P,
                         hipStream_t st);
@@ -247,11 +253,6 @@ void launch_b_observables(const BChain *ch, const int *blk0, int n, int nblocks,
                           hipStream_t st);
 void launch_b_dot_dd(const BChain *ch, const int *blk0, int n, int nblocks, const double2 *a,
                      const double2 *b, double *part, RedOut *out, hipStream_t st);
-// (on the chains that wait for a reliable update; each one's pending FP32 x
-// update applied first, as launch_ru_accum)
-void launch_b_ru_accum(const BChain *ch, int nchain, long vtot, double2 *x, float2 *xs,
-                       const float2 *p0, const float2 *p1, const SolverState *S,
-                       hipStream_t st);
 // Multi-shift CG of one right-hand side per chain (M[c]: chain c's state).
 void launch_b_ms_init(const BGeo &g, int rows, double2 *r, double2 *p0,
                       const MultiShiftState *M, BParts P, hipStream_t st);
