@@ -633,9 +633,9 @@ def config_measurements(Q, torch, pool, args):
         dt, its = sync_time(lambda: [Q.CG(x, U, b, 0.1, 100000, 1e-12, Q.gamm5Dw_sqr_msq, prm,
                                           None, precision=prec) for x, b in zip(xs, bs)])
         bps = 384 if prec == Q.PREC_F64 else 192
-        # the 16 solves as one set (CG_mrhs: one launch per kernel, shared links),
-        # against one-at-a-time solves on the set's schedule (mixed precision:
-        # reliable updates decided by the host)
+        # the 16 solves as one set (CG_mrhs: one launch per kernel, shared links);
+        # checked bit for bit against one-at-a-time solves, in mixed precision
+        # both with reliable updates decided by the host (the same schedule)
         so = torch.zeros_like(si)
         Q.CG_mrhs(so, U, si, 0.1, 100000, 1e-12, prm, None, precision=prec)
         dts, itm = sync_time(lambda: Q.CG_mrhs(so, U, si, 0.1, 100000, 1e-12, prm, None,
@@ -646,6 +646,8 @@ def config_measurements(Q, torch, pool, args):
             ref = [Q.spinor(prm) for _ in range(16)]
             itr = [Q.CG(x, U, b, 0.1, 100000, 1e-12, Q.gamm5Dw_sqr_msq, prm, None,
                         precision=prec) for x, b in zip(ref, bs)]
+            so = torch.zeros_like(si)
+            itm = Q.CG_mrhs(so, U, si, 0.1, 100000, 1e-12, prm, None, precision=prec)
             Q.set_reliable_mode(prm, True)
         c2[key] = {"seconds_16_rhs": round(dt, 4), "iterations": its,
                    "GBs_cg_kernels": round(bps * L * L * sum(its) / dt / 1e9, 1),

@@ -1617,6 +1617,9 @@ int rhs_set(qed2d_ctx *c, int n, qed2d_batch **out)
             return fail(c, rc, "cannot create a set of %d right-hand sides", n);
         }
     }
+    // (the set's mixed-precision solves follow the context's reliable-update
+    // mode, so qed2d_cg_mrhs reproduces qed2d_cg in either)
+    (void)qed2d_batch_set_reliable_mode(c->rhsets[n], c->ru_mode);
     *out = c->rhsets[n];
     return QED2D_OK;
 }

@@ -64,6 +64,16 @@ __device__ __forceinline__ BWhere bwhere(const BGeo &g, int R)
     return w;
 }
 
+// A chain takes a batched reliable update when its K1 paused for one
+// (need_ru) or requested one and went on iterating (ru_req); neither is
+// changed by the update's kernels (the K1 after it clears both), so every
+// block of them decides alike.  Either way kk is the K1 that runs next, and
+// K2(kk - 1) left the pending x update alpha_x p_{kk-1}.
+__device__ __forceinline__ bool ru_wanted(const SolverState *S)
+{
+    return S->need_ru != 0 || S->ru_req != 0;
+}
+
 // Chain of arena element e (0 <= e < 2 * total sites).
 __device__ __forceinline__ int chain_of_elem(const BChain *ch, int n, long e)
 {
@@ -218,10 +228,10 @@ void launch_b_cg_init(const BGeo &g, int rows, const double2 *b, double2 *x, voi
 // ---------------------------------------------------------------------------
 // Residual from x with t = Q x: mode 0 starts a solve from a guess on every
 // chain (FP64: r and p0 = 0; mixed: rs and p0s = 0); mode 2 is the reliable
-// update of the chains that wait for one (need_ru, set by the K1 that paused).
-// Mode 2 resumes the chain (stop_k = 0) but leaves need_ru to the K1 it
-// resumes: a block of this kernel that starts after block 0 has finished must
-// still see the request.
+// update of the chains that want one (ru_wanted: paused, or requested on the
+// way).  Mode 2 resumes a paused chain (stop_k = 0) but leaves need_ru and
+// ru_req to the K1 that runs next: a block of this kernel that starts after
+// block 0 has finished must still see the request.
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__restrict__ U,
@@ -237,7 +247,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg_resid(BGeo g, const double2 *__r
 {
     const BWhere w = bwhere<false>(g, R);
     SolverState *Sc = S + w.c;
-    if (mode == 2 && !Sc->need_ru) return;
+    if (mode == 2 && !ru_wanted(Sc)) return;
     const int k = (mode == 2) ? Sc->kk : 1;
     const long o = w.o;
     EpiResid epi{b + o, x + o, r ? r + o : nullptr, rs ? rs + o : nullptr,
@@ -329,9 +339,9 @@ __global__ void __launch_bounds__(kBlock) kb_ru_apply(BGeo g, const double2 *__r
     const BWhere w = bwhere<false>(g, R);
     if (!w.p.active) return;
     const SolverState *Sc = S + w.c;
-    if (!Sc->need_ru) return;
+    if (!ru_wanted(Sc)) return;
     const long o = w.o;
-    const float2 *pp = ((Sc->stop_k - 1) & 1) ? P1 : P0;
+    const float2 *pp = ((Sc->kk - 1) & 1) ? P1 : P0;
     SrcXNew src{x + o, xs + o, pp + o, (float)Sc->alpha_x, w.L.V};
     EpiTX epi{t + o, xn + o, w.L.V};
     q_march<double, R>(w.L, U + w.uo, 2.0 + w.am0, src, epi, w.p.i1, w.p.r0);
@@ -418,7 +428,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, LinksF32 U,
                                                   const float2 *__restrict__ r, float2 *P0,
                                                   float2 *P1, float2 *__restrict__ t,
                                                   float2 *__restrict__ x, SolverState *S,
-                                                  BParts P)
+                                                  BParts P, int soft)
 {
     __shared__ double lds[kBlock / kWave + 1];
     const BWhere w = bwhere<true>(g, R);
@@ -430,7 +440,7 @@ __global__ void __launch_bounds__(kBlock) kb_cg1v(BGeo g, LinksF32 U,
     double beta;
     bool fresh;
     if (!cg_k1_scalars<true>(Sc, P.rr[rd] + w.pb, P.b + w.pb, P.nrr[rd][w.c], P.nb[w.c], k,
-                             w.p.blk, lds, beta, &fresh))
+                             w.p.blk, lds, beta, &fresh, soft != 0))
         return;
     const long o = w.o;
     const float2 *pold = (rd ? P1 : P0) + o;
@@ -541,11 +551,11 @@ void launch_b_cg_k2_f64(const BGeo &g, int rows, const double2 *U, const double2
 }
 void launch_b_cg_k1_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *r,
                         float2 *p0, float2 *p1, float2 *t, float2 *x, SolverState *S, BParts P,
-                        hipStream_t st)
+                        hipStream_t st, bool soft)
 {
     const LinksF32 lk{U, packed};
     DISPATCH_ROWS_PK(rows, packed, kb_cg1v, dim3(g.nblocks), dim3(kBlock), 0, st, g, lk, r, p0,
-                     p1, t, x, S, P);
+                     p1, t, x, S, P, soft ? 1 : 0);
 }
 // (x is updated by the next K1 or the reliable update's flush)
 void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *t,

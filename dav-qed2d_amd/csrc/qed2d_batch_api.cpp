@@ -50,6 +50,11 @@ struct qed2d_batch {
     std::vector<double> mu2;  // per chain: the CG operator's shift (Q^2 + mu2); empty: 0
     bool guess = false;       // the next solve starts from the x it is given
     double ru_delta = kRuDelta;
+    // mixed precision: a K1 that sees the residual drop by ru_delta requests a
+    // reliable update and the chain iterates on until the next slot (soft), or
+    // pauses until the slot (the single-lattice host-decided schedule, bit for
+    // bit; qed2d_batch_set_reliable_mode(b, 0))
+    bool ru_soft = true;
     std::vector<double> last_dh;  // per chain, from the double-double energies
     std::map<int, GeoBuf> geo;  // key: 2 * rows + pair
     // CG chunks as HIP graphs (qed2d_batch_set_graphs): one executable graph per
@@ -428,7 +433,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
         for (int j = 0; j < len; ++j) {
             if (mixed) {
                 launch_b_cg_k1_f32(g32, b->rows32, b->Us, packed, b->rs, b->p0s, b->p1s, b->ts,
-                                   b->xs, b->S, b->P, s);
+                                   b->xs, b->S, b->P, s, b->ru_soft);
                 launch_b_cg_k2_f32(g32b, rows32b, b->Us, packed, b->ts, b->p0s, b->p1s, b->xs,
                                    b->rs, b->S, b->P, shifted, s);
             } else {
@@ -441,7 +446,8 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
     auto queue_chunk = [&](int len, bool ru_slot) -> int {
         ru_slot = ru_slot && mixed;
         if (b->graphs) {
-            const qed2d_batch::GraphKey key{len, ru_slot, mixed, packed, shifted, (const void *)x,
+            const qed2d_batch::GraphKey key{len, ru_slot + 2 * b->ru_soft, mixed, packed, shifted,
+                                            (const void *)x,
                                             (const void *)src, b->rows, rows64b, b->rows32,
                                             rows32b, b->rows_q};
             auto it = b->gexec.find(key);
@@ -509,7 +515,7 @@ int batch_cg(qed2d_batch *b, double2 *x, const double2 *src, int maxiter, double
             ev[c] = 0;
             if (h.status != 0) continue;
             running = true;
-            if (h.need_ru) {
+            if (h.need_ru || h.ru_req) {
                 ev[c] = pred[c].after_ru(h.err, h.tol, h.delta2);
                 ru_slot = true;
             } else {
@@ -870,6 +876,13 @@ int qed2d_batch_set_tiles_k2(qed2d_batch *b, int cg64, int cg32)
         return bfail(b, QED2D_EINVAL, "tile rows must be 0, 1, 2, 4, 8 or 16");
     b->rows_k2 = cg64;
     b->rows32_k2 = cg32;
+    return QED2D_OK;
+}
+
+int qed2d_batch_set_reliable_mode(qed2d_batch *b, int on_device)
+{
+    if (!b) return QED2D_EINVAL;
+    b->ru_soft = on_device != 0;
     return QED2D_OK;
 }
 

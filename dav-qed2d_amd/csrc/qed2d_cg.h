@@ -122,10 +122,11 @@ __device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, dou
 template <bool mixed>
 __device__ __forceinline__ bool cg_k1_scalars(SolverState *S, const double *Prr, const double *Pb,
                                               int nrr, int nbb, int k, int blk, double *lds,
-                                              double &beta, bool *fresh = nullptr)
+                                              double &beta, bool *fresh = nullptr,
+                                              bool soft = false)
 {
     const double rr = sum_prev_partials(Prr, nrr, 1, 0, lds);
-    return cg_k1_decide<mixed>(S, k1_in(S, k), rr, Pb, nbb, k, blk, lds, beta, fresh);
+    return cg_k1_decide<mixed>(S, k1_in(S, k), rr, Pb, nbb, k, blk, lds, beta, fresh, soft);
 }
 
 template <typename T> struct EpiK1 {

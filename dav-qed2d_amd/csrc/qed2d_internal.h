@@ -233,9 +233,11 @@ void launch_b_cg_k1_f64(const BGeo &g, int rows, const double2 *U, const double2
 void launch_b_cg_k2_f64(const BGeo &g, int rows, const double2 *U, const double2 *t,
                         const double2 *p0, const double2 *p1, double2 *x, double2 *r,
                         SolverState *S, BParts P, hipStream_t st);
+// soft: a K1 whose iterated residual has dropped by delta requests a reliable
+// update (ru_req) and the chain goes on (else it pauses, need_ru).
 void launch_b_cg_k1_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *r,
                         float2 *p0, float2 *p1, float2 *t, float2 *x, SolverState *S, BParts P,
-                        hipStream_t st);
+                        hipStream_t st, bool soft = false);
 void launch_b_cg_k2_f32(const BGeo &g, int rows, const void *U, bool packed, const float2 *t,
                         const float2 *p0, const float2 *p1, float2 *x, float2 *r,
                         SolverState *S, BParts P, bool mu, hipStream_t st);

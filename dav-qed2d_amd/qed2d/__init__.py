@@ -76,6 +76,7 @@ SIGNATURES = {
     "qed2d_batch_get_tiles": (_i, [_p, _p]),
     "qed2d_batch_autotune": (_i, [_p, _i, _p]),
     "qed2d_batch_set_graphs": (_i, [_p, _i]),
+    "qed2d_batch_set_reliable_mode": (_i, [_p, _i]),
     "qed2d_batch_hot_start": (_i, [_p, _p]),
     "qed2d_batch_cg": (_i, [_p, _p, _p, _p, _i, _d, _i, _p]),
     "qed2d_batch_create_rhs": (_i, [_p, _i, _i, _i, _d, _i, _p]),
@@ -901,6 +902,12 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def set_reliable_mode(self, on_device=True):
+        """Mixed precision: reliable updates requested on the way (True, the
+        default) or after a pause (False: each chain reproduces a single-lattice
+        solve with set_reliable_mode(prm, False) bit for bit)."""
+        self._check(lib.qed2d_batch_set_reliable_mode(self._b, 1 if on_device else 0))
 
     def set_graphs(self, on=True):
         """Queue each CG chunk as one HIP graph launch (same kernels, same bits)."""

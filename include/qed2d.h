@@ -319,6 +319,13 @@ int qed2d_batch_set_tiles(qed2d_batch *b, int q64, int cg64, int cg32);
 int qed2d_batch_set_tiles_k2(qed2d_batch *b, int cg64, int cg32);
 int qed2d_batch_get_tiles(qed2d_batch *b, int tiles[5]);
 int qed2d_batch_autotune(qed2d_batch *b, int iters, int tiles[5]);
+/* Mixed-precision CG of a batch: on_device != 0 (the default) lets a chain
+ * whose FP32 residual has dropped by the reliable-update factor request the
+ * update and iterate on until the device performs it at the next chunk start;
+ * 0 pauses the chain until then, which reproduces qed2d_cg with
+ * qed2d_ctx_set_reliable_mode(ctx, 0) on that chain bit for bit.  No
+ * counterpart in the reference. */
+int qed2d_batch_set_reliable_mode(qed2d_batch *b, int on_device);
 /* on != 0: the batch's CG solves queue each chunk of iterations (its reliable-
  * update slot and K1/K2 launches) as one HIP graph launch, captured on first
  * use per chunk shape and kept until qed2d_batch_destroy; the same kernels and

@@ -2,9 +2,11 @@
 single-lattice entry points on the same lattice, seeds and tiles.  The batch
 runs the same device code per chain with the same tiling, so the comparison
 is bit-exact (CG solutions, energies, accept/reject, links), FP64 and mixed
-precision (the single lattice with host-decided reliable updates, the batch's
-schedule).  The ensemble of 64 chains of uneven sizes is checked for mixed vs
-FP64 agreement (accept/reject identical, |dH_mixed - dH_fp64| < 1e-9)."""
+precision (the batch with its chains pausing for reliable updates, the
+single lattice with host-decided updates: the same schedule).  In its default
+mode a batch's chains request updates and iterate on; that mode is checked
+against FP64 solves and, in the ensemble of 64 chains of uneven sizes, against
+FP64 trajectories (accept/reject identical, |dH_mixed - dH_fp64| < 1e-9)."""
 import numpy as np
 import pytest
 
@@ -30,6 +32,7 @@ def test_batch_cg_bit_exact(Q, O, prec):
     betas = [2.0, 4.0, 6.0, 4.0]
     am0s = [0.1, 0.2, 0.15, 0.3]
     B = Q.Batch(sizes, betas)
+    B.set_reliable_mode(False)  # (the single lattice's host-decided schedule)
     B.set_tiles(TILES["q64"], TILES["cg64"], TILES["cg32"])
     B.hot_start([11, 12, 13, 14])
     src = B.arena_spinor()
@@ -51,6 +54,37 @@ def test_batch_cg_bit_exact(Q, O, prec):
         assert torch.equal(xs, B.field(x, c)), c
 
 
+def test_batch_cg_soft_reliable_updates(Q, O):
+    """Mixed-precision batched CG in its default mode (a chain requests a
+    reliable update and iterates on until the next chunk start) against FP64
+    single-lattice solves: the FP64 stopping rule holds on every chain's true
+    residual, the solutions agree to what the tolerance allows, and the
+    iteration counts stay close to FP64's."""
+    say("batch CG soft reliable updates")
+    sizes = [(16, 16), (32, 24), (64, 64), (30, 18), (128, 96)]
+    betas = [2.0, 4.0, 6.0, 4.0, 4.0]
+    am0s = [0.1, 0.2, 0.15, 0.3, 0.1]
+    B = Q.Batch(sizes, betas)
+    B.hot_start([51, 52, 53, 54, 55])
+    src = B.arena_spinor()
+    for c, (L1, L2) in enumerate(sizes):
+        prm = Q.LattParm((L1, L2), betas[c])
+        Q.fill_pseudofermion(B.field(src, c), prm, 700 + c, c, 0, 0)
+    x = B.arena_spinor()
+    its = B.CG(x, src, am0s, 10000, 1e-22, precision=1)
+    for c, (L1, L2) in enumerate(sizes):
+        prm = Q.LattParm((L1, L2), betas[c])
+        U = B.get_links(c)
+        b = B.field(src, c).contiguous()
+        xr = Q.spinor(prm)
+        it = Q.CG(xr, U, b, am0s[c], 10000, 1e-22, Q.gamm5Dw_sqr_msq, prm, None)
+        xc = B.field(x, c).contiguous()
+        res = O.norm2(host(b) - O.gamm5Dw_sqr_musq(host(U), host(xc), am0s[c], 0.0))
+        assert res < 1e-22 * O.norm2(host(b)), (c, res)
+        assert relerr(host(xc), host(xr)) < 1e-8, c
+        assert abs(its[c] - it) <= 0.1 * it + 8, (c, its[c], it)
+
+
 @pytest.mark.parametrize("prec,integ", [(0, "leapfrog"), (1, "leapfrog"), (1, "omf4")])
 def test_batch_hmc_bit_exact(Q, O, prec, integ):
     import torch
@@ -62,6 +96,7 @@ def test_batch_hmc_bit_exact(Q, O, prec, integ):
     integrator = Q.OMF4() if integ == "omf4" else Q.Leapfrog()
     ns = 3 if integ == "omf4" else 10
     B = Q.Batch(sizes, betas)
+    B.set_reliable_mode(False)  # (the single lattice's host-decided schedule)
     B.set_tiles(TILES["q64"], TILES["cg64"], TILES["cg32"])
     B.hot_start(seeds)
     singles = []
@@ -200,6 +235,7 @@ def test_batch_split_tiles_and_autotune_bit_exact(Q, O, prec):
     betas = [2.0, 4.0, 6.0, 4.0]
     am0s = [0.1, 0.2, 0.15, 0.3]
     B = Q.Batch(sizes, betas)
+    B.set_reliable_mode(False)  # (the single lattice's host-decided schedule)
     d = B.get_tiles()
     assert all(v in (1, 2, 4, 8, 16) for v in d.values())
     B.hot_start([41, 42, 43, 44])

@@ -544,15 +544,16 @@ def test_mixed_cg_packed_links_and_fallback(Q, O, unitary):
     say("iterations mixed", it1, "fp64", it0)
     assert it1 < 1.5 * it0 + 20
     # the same solve through a right-hand-side set of one, against the
-    # single-lattice solve on the set's schedule (host-decided updates)
+    # single-lattice solve, both with host-decided updates (a set follows its
+    # context's mode)
     Q.set_reliable_mode(prm, False)
     x2 = Q.spinor(prm)
     it2 = Q.CG(x2, dU, db, 0.3, 10000, 1e-22, Q.gamm5Dw_sqr_msq, prm, None, precision=1)
-    Q.set_reliable_mode(prm, True)
     assert relerr(host(x2), host(x0)) < 1e-9
     xs = torch.zeros((1, 2, L2, L1), dtype=torch.complex128, device="cuda")
     its = Q.CG_mrhs(xs, dU, db.reshape(1, 2, L2, L1).contiguous(), 0.3, 10000, 1e-22, prm, None,
                     precision=1)
+    Q.set_reliable_mode(prm, True)
     assert its[0] == it2 and torch.equal(xs[0], x2)
 
 

@@ -117,20 +117,20 @@ def test_cg_16rhs_1024_fp64_vs_mixed(Q, O):
         assert d < 1e-8, (j, d)
     say("iterations fp64", its[0], "mixed", its[1])
     assert max(i1 - 1.3 * i0 for i0, i1 in zip(its[0], its[1])) < 20
-    # the set against one-at-a-time solves on its schedule (mixed precision:
-    # reliable updates decided by the host)
+    # the set against one-at-a-time solves, both with reliable updates decided
+    # by the host (a set follows its context's mode)
     Q.set_reliable_mode(prm, False)
     ref = {0: (its[0], xs[0]), 1: ([], torch.zeros_like(si))}
     for j in range(nrhs):
         ref[1][0].append(Q.CG(ref[1][1][j], U, si[j], 0.1, 100000, 1e-12, Q.gamm5Dw_sqr_msq,
                               prm, None, precision=1))
-    Q.set_reliable_mode(prm, True)
     for prec in (0, 1):
         say("CG_mrhs 16 x 1024^2 prec", prec)
         so = torch.zeros_like(si)
         itm = Q.CG_mrhs(so, U, si, 0.1, 100000, 1e-12, prm, None, precision=prec)
         assert itm == ref[prec][0], prec
         assert torch.equal(so, ref[prec][1]), prec
+    Q.set_reliable_mode(prm, True)
 
 
 @pytest.mark.parametrize("L", [1024])
