@@ -692,6 +692,15 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     for (int R : {16, 8, 4, 2, 1})
         if (!rows_r && resid_blocks(L, R) <= nk2) rows_r = R;
     const bool soft = c->ru_mode && rows_r > 0;
+    // In soft mode the chunk predicted to end the solve carries the reliable
+    // update of its last K1 (which pauses where the iterated residual looks
+    // converged) and that K1 and K2 again, each run only if the pause came
+    // there: the host's look then finds the solve converged instead of paused.
+    // (Site-pair kernels, whose K1 and K2 take the guard, and grids whose
+    // partials are never folded: a folded slot must not be refolded by a
+    // guarded launch that did not run.)
+    const bool spec_ok = soft && pair_ok(L) && cg2_f32_blocks(L, c->rows32) <= kPartsUnfolded &&
+                         nk2 <= kPartsUnfolded;
     // The site-pair kernels read the links packed (one float each) when they
     // are U(1); the first look at the state tells, and a solve whose links are
     // not restarts with FP32 complex links (x is untouched until then).
@@ -723,8 +732,10 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
         plan = SolvePlan();
         return QED2D_OK;
     };
-    // iterations k .. k1, after the on-device update when ru (soft mode)
-    auto queue = [&](int k1, bool ru, bool last) -> int {
+    // iterations k .. k1, after the on-device update when ru (soft mode);
+    // spec: followed by the update of a pause at K1(k1) and K1(k1), K2(k1)
+    // again, each a no-op unless K1(k1) paused
+    auto queue = [&](int k1, bool ru, bool last, bool spec = false) -> int {
         const int k0 = k;
         if (ru) {
             hipEvent_t ev = prof_begin(c, QED2D_PROF_RELIABLE);
@@ -743,6 +754,21 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             PROF(c, QED2D_PROF_CG_K2_F32,
                  launch_cg_k2_f32(L, Us, packed, ts, P[0], P[1], xs, rs, am0f, a.mu2 != 0.0,
                                   c->state, c->P, k, rows32_k2, st));
+        }
+        if (spec && spec_ok && k > k0) {
+            const int kc = k - 1;
+            hipEvent_t ev = prof_begin(c, QED2D_PROF_RELIABLE);
+            launch_ru_accum(L, (double2 *)x, xs, pend0, P[1], c->state, st, 0, kc);
+            launch_q_apply_ru(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->state,
+                              c->rows_q, st, kc);
+            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
+                            nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, kc, 4, rows_r,
+                            st);
+            prof_end(c, QED2D_PROF_RELIABLE, ev);
+            launch_cg_k1_f32(L, Us, packed, rs, P[0], P[1], ts, xs, am0f, c->state, c->P, kc,
+                             c->rows32, st, soft, true);
+            launch_cg_k2_f32(L, Us, packed, ts, P[0], P[1], xs, rs, am0f, a.mu2 != 0.0,
+                             c->state, c->P, kc, rows32_k2, st, true);
         }
         TRY(launched(c));
         return cq_push(c, q, last || k > kmax, k0, ru);
@@ -846,7 +872,7 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             int k1;
             bool ru, last;
             plan_next(h, &k1, &ru, &last);
-            TRY(queue(k1, ru, last));
+            TRY(queue(k1, ru, last, last));
         }
     }
     q.n = 0;  // (launches still queued return at once)

@@ -109,9 +109,9 @@ __device__ __forceinline__ bool cg_k1_decide(SolverState *S, const K1In &in, dou
             S->status = stat;
             S->stop_k = k;
         }
-        // (a batched reliable update leaves need_ru set for the K1 it resumes
-        // to clear here)
-        S->need_ru = ru;
+        // need_ru: the iteration that paused (a batched or queued-ahead
+        // reliable update leaves it set for the K1 it resumes to clear here)
+        S->need_ru = ru ? k : 0;
         if (ru) S->stop_k = k;
     }
     if (stat || ru) return false;

@@ -320,15 +320,18 @@ void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const d
                       CGPartials &P, int k, int rows, hipStream_t st);
 // U: FP32 complex links, or (packed, site-pair kernels only: even L1 >= 4)
 // links packed by launch_link_pack.
-// (soft: on-device reliable updates, see SolverState::ru_req)
+// (soft: on-device reliable updates, see SolverState::ru_req; guard, site-pair
+// kernels only: K1(k) queued again behind a mode-4 reliable update, which it
+// runs only if that update ran)
 void launch_cg_k1_f32(const Latt &L, const void *U, bool packed, const float2 *r, float2 *p0,
                       float2 *p1, float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P,
-                      int k, int rows, hipStream_t st, bool soft = false);
+                      int k, int rows, hipStream_t st, bool soft = false, bool guard = false);
 // (mu: the operator has mu2 != 0.  On even L1 >= 4 the site-pair kernels
-// leave x += alpha p to the next K1, see SolverState::alpha_x.)
+// leave x += alpha p to the next K1, see SolverState::alpha_x; guard as K1's.)
 void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t,
                       const float2 *p0, const float2 *p1, float2 *x, float2 *r, float am0,
-                      bool mu, SolverState *s, CGPartials &P, int k, int rows, hipStream_t st);
+                      bool mu, SolverState *s, CGPartials &P, int k, int rows, hipStream_t st,
+                      bool guard = false);
 // n FP64 U(1) links -> one float each (the packed format of the FP32 site-pair
 // solver); sets S->bad_links if a link is not of unit modulus to 2^-23 (FP32 epsilon).
 void launch_link_pack(const double2 *U, float *out, long n, SolverState *S, hipStream_t st);
@@ -337,12 +340,14 @@ void launch_state_snap(const SolverState *S, SolverState *out, int *seq, int v, 
 // A reliable update's x += double(xs), xs = 0; with p0 / p1 given (the FP32 site-pair solver, whose
 // K1 was paused with an x update pending), xs + alpha_x p_{stop_k - 1} first.
 // kb > 0: the on-device update queued before K1(kb), run only while a soft
-// request is pending (pending update from p_{kb - 1}).
+// request is pending (pending update from p_{kb - 1}).  pk > 0: the update
+// queued behind K1(pk), run only if K1(pk) paused (need_ru == pk).
 void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const float2 *p0, const float2 *p1,
-                     const SolverState *S, hipStream_t st, int kb = 0);
-// so = Q si while a soft reliable-update request is pending (SolverState::soft).
+                     const SolverState *S, hipStream_t st, int kb = 0, int pk = 0);
+// so = Q si while a soft reliable-update request is pending (SolverState::ru_req),
+// or, pk > 0, while K1(pk) is paused for an update.
 void launch_q_apply_ru(const Latt &L, const double2 *U, const double2 *si, double2 *so,
-                       double am0, const SolverState *S, int rows, hipStream_t st);
+                       double am0, const SolverState *S, int rows, hipStream_t st, int pk = 0);
 // Partials a consumer sums without a fold first (kPartMax of the device code).
 constexpr int kPartsUnfolded = 1024;
 // Block counts of the residual kernel and of the FP32 solver's K2 at a tile height.

@@ -131,6 +131,9 @@ void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *r
 // which none of the update's kernels modify), and pads its |r|^2 partials with
 // zeros up to npad, the block count of the K2 whose slot it overwrites (adding
 // +0.0 leaves the consumer's sum bit for bit as over the real partials).
+// Mode 4 is the update queued behind the chunk predicted to end a solve: it
+// runs only if K1(k) paused there (need_ru == k, which the K1 that runs next
+// clears), pads like mode 3 and resumes the solve at K1(k) like mode 2.
 __device__ __forceinline__ bool soft_ru_pending(const SolverState *S)
 {
     return S->ru_req != 0 && S->stop_k == 0;
@@ -144,6 +147,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_resid(Latt L, const double2 *__re
                                                      int npad, int bx, int by, int nbx)
 {
     if (mode == 3 && !soft_ru_pending(S)) return;
+    if (mode == 4 && S->need_ru != k) return;
     MarchPos p = march_pos(L, bx, by, nbx, R);
     EpiResid epi = e0;
     epi.mu2 = S->mu2;
@@ -167,6 +171,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_resid(Latt L, const double2 *__re
             S->need_ru = 0;
             S->stop_k = 0;  // resume: K1(k) runs again
         }
+        if (mode == 4) S->stop_k = 0;  // resume (need_ru is left to K1(k))
     }
 }
 
@@ -184,9 +189,10 @@ void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const do
     // (which the caller checked is at least this grid's) zero padding up to it
     const int nslot = P.nrr[(k - 1) & 1];
     const bool fold3 = (mode == 3 && nslot == 1 && g.nblocks() > 1);
-    const int npad = (mode == 3 && !fold3) ? nslot : 0;
+    const int npad = ((mode == 3 && !fold3) || mode == 4) ? nslot : 0;
     DISPATCH_ROWS(rows, k_cg_resid, grid, blk, 0, st, L, U, t, e, 2.0 + am0, s, P.rr[(k - 1) & 1],
                   P.b, k, mode, npad, g.bx, g.by, g.nbx);
+    if (mode == 4) return;  // (the caller made sure nslot >= this grid, unfolded)
     if (mode == 3) {
         if (fold3)
             launch_k(k_fold, dim3(1), dim3(kBlock), 0, st, P.rr[(k - 1) & 1], g.nblocks(), 1,
@@ -199,14 +205,15 @@ void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const do
 
 // Predicated FP64 apply of the on-device reliable update: so = Q si while a
 // soft request is pending (see soft_ru_pending).
+// (pk > 0: instead while K1(pk) is paused for an update, need_ru == pk)
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_q_apply_ru(Latt L, const double2 *__restrict__ U,
                                                        const double2 *__restrict__ si,
                                                        double2 *__restrict__ so, double mass2,
                                                        const SolverState *S, int bx, int by,
-                                                       int nbx)
+                                                       int nbx, int pk)
 {
-    if (!soft_ru_pending(S)) return;
+    if (pk > 0 ? S->need_ru != pk : !soft_ru_pending(S)) return;
     MarchPos p = march_pos(L, bx, by, nbx, R);
     if (!p.active) return;
     SrcPlain<double> src{si, L.V};
@@ -214,11 +221,11 @@ __global__ void __launch_bounds__(kBlock) k_q_apply_ru(Latt L, const double2 *__
     q_march<double, R>(L, U, mass2, src, epi, p.i1, p.r0);
 }
 void launch_q_apply_ru(const Latt &L, const double2 *U, const double2 *si, double2 *so,
-                       double am0, const SolverState *S, int rows, hipStream_t st)
+                       double am0, const SolverState *S, int rows, hipStream_t st, int pk)
 {
     MarchGrid g = march_grid(L, rows);
     DISPATCH_ROWS(rows, k_q_apply_ru, dim3(g.nblocks()), dim3(kBlock), 0, st, L, U, si, so,
-                  2.0 + am0, S, g.bx, g.by, g.nbx);
+                  2.0 + am0, S, g.bx, g.by, g.nbx, pk);
 }
 int resid_blocks(const Latt &L, int rows) { return march_grid(L, rows).nblocks(); }
 int cg2_f32_blocks(const Latt &L, int rows)
@@ -231,13 +238,17 @@ int cg2_f32_blocks(const Latt &L, int rows)
 // run only while a soft request is pending)
 __global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs,
                            const float2 *__restrict__ P0, const float2 *__restrict__ P1,
-                           const SolverState *S, long n, int kb)
+                           const SolverState *S, long n, int kb, int pk)
 {
-    if (kb > 0 && !soft_ru_pending(S)) return;
+    if (pk > 0) {
+        if (S->need_ru != pk) return;
+    } else if (kb > 0 && !soft_ru_pending(S)) {
+        return;
+    }
     const float2 *pp = nullptr;
     float a = 0.0f;
     if (P0) {
-        pp = (((kb > 0 ? kb : S->stop_k) - 1) & 1) ? P1 : P0;
+        pp = (((pk > 0 ? pk : kb > 0 ? kb : S->stop_k) - 1) & 1) ? P1 : P0;
         a = (float)S->alpha_x;
     }
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
@@ -250,10 +261,10 @@ __global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs,
     }
 }
 void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const float2 *p0, const float2 *p1,
-                     const SolverState *S, hipStream_t st, int kb)
+                     const SolverState *S, hipStream_t st, int kb, int pk)
 {
     launch_k(k_ru_accum, dim3(ew_grid(2 * L.V)), dim3(kBlock), 0, st, x, xs, p0, p1, S,
-                       2 * L.V, kb);
+                       2 * L.V, kb, pk);
 }
 
 __global__ void k_to_f32(const double2 *__restrict__ in, float2 *__restrict__ out, long n)
@@ -364,7 +375,10 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
 }
 
 // ---- FP32 pair kernels (two sites per thread along i1) -------------------
-template <int R, bool PK>
+// G: K1(k) queued again behind a mode-4 reliable update, run only if that
+// update ran (K1(k) had paused and was resumed); a separate instance, so the
+// check costs the ordinary K1 nothing.
+template <int R, bool PK, bool G = false>
 __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
                                                  const float2 *__restrict__ r, float2 *P0,
                                                  float2 *P1, float2 *__restrict__ t,
@@ -389,8 +403,11 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
     float2 *pold = rd ? P1 : P0;
     EpiK1v epi{w ? P1 : P0, t, x, 0.0f, false, k != 1, L.V, 0.0, 0.0};
     SrcXpay2 src{r, pold, 0.0f, L.V};
+    int gate = 1;
+    if constexpr (G) gate = (Sv->need_ru == k && in.true_at == k) ? 1 : 0;
     auto ready = [&]() -> bool {
         if (sk != 0 && sk < k) return false;
+        if (G && !gate) return false;
         double beta;
         bool fresh;
         if (!cg_k1_decide<true>(S, in, prr.sum(), Pb, nbb, k, p.blk, lds, beta, &fresh,
@@ -415,14 +432,14 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
                                           const float2 *__restrict__ t, const float2 *P0,
                                           const float2 *P1, float2 *__restrict__ r, float mass2,
                                           SolverState *S, const double *Ptt, double *Prr, int ntt,
-                                          int k, int bx, int by, int nbx)
+                                          int k, int bx, int by, int nbx, int guard)
 {
     // The solver state's loads go out first, the partials' next, the first
     // rows' last; nothing waits for them before the rows' loads are in flight.
     // A solve that stopped before K2(k) returns before any store.
     const int w = k & 1;
     const SolverState *Sv = state_v(S);
-    const int sk = Sv->stop_k;
+    const int sk = Sv->stop_k, ta = Sv->true_at;
     const double nw = Sv->norm[w], mu2 = Sv->mu2;
     const MarchPos p = march_pos2(L, bx, by, nbx, R);
     PartialsAhead2 ptt;
@@ -431,6 +448,9 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
     double alpha = 0.0;
     auto ready = [&]() -> bool {
         if (sk != 0 && sk <= k) return false;
+        // (guard: K2(k) queued again behind a mode-4 reliable update runs only
+        // if that update ran, which made K1(k) see a true residual)
+        if (guard && ta != k) return false;
         double tt, pp;
         ptt.sum(tt, pp);
         alpha = nw / (tt + mu2 * pp);
@@ -452,17 +472,28 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
         NAME(Latt L, LinksF32 U, const float2 *__restrict__ t,                            \
              const float2 *P0, const float2 *P1, float2 *__restrict__ r, float mass2,      \
              SolverState *S, const double *Ptt, double *Prr, int ntt, int k, int bx, int by, \
-             int nbx)                                                                      \
+             int nbx, int guard)                                                           \
     {                                                                                      \
-        cg2v_body<R, MU, PK>(L, U, t, P0, P1, r, mass2, S, Ptt, Prr, ntt, k, bx, by, nbx); \
+        cg2v_body<R, MU, PK>(L, U, t, P0, P1, r, mass2, S, Ptt, Prr, ntt, k, bx, by, nbx,  \
+                             guard);                                                       \
     }
 CG2V_KERNEL(k_cg2v, false)
 CG2V_KERNEL(k_cg2vm, true)  // mu2 != 0
 
+// k_cg1v<R, PK, true> (the guarded K1, see k_cg1v)
+#define DISPATCH_K1G(rows, PK, ...)                                                   \
+    switch (rows) {                                                                   \
+    case 1: launch_k((k_cg1v<1, PK, true>), __VA_ARGS__); break;                      \
+    case 2: launch_k((k_cg1v<2, PK, true>), __VA_ARGS__); break;                      \
+    case 4: launch_k((k_cg1v<4, PK, true>), __VA_ARGS__); break;                      \
+    case 8: launch_k((k_cg1v<8, PK, true>), __VA_ARGS__); break;                      \
+    default: launch_k((k_cg1v<16, PK, true>), __VA_ARGS__); break;                    \
+    }
+
 template <typename T>
 static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c2<T> *p1,
                      c2<T> *t, c2<T> *x, T am0, SolverState *S, CGPartials &P, int k, int rows,
-                     hipStream_t st, int packed = 0, int soft = 0)
+                     hipStream_t st, int packed = 0, int soft = 0, bool guard = false)
 {
     const T m2 = T(2) + am0;
     double *ptt = P.tt[k & 1];
@@ -473,8 +504,16 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
             MarchGrid g = march_grid2(L, rows);
             dim3 grid(g.nblocks()), blk(kBlock);
             const LinksF32 lk{U, packed};
-            DISPATCH_ROWS_PK(rows, packed, k_cg1v, grid, blk, 0, st, L, lk, r, p0, p1, t, x, m2,
-                             S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft);
+            if (guard && packed) {
+                DISPATCH_K1G(rows, true, grid, blk, 0, st, L, lk, r, p0, p1, t, x, m2, S, ptt,
+                             prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft);
+            } else if (guard) {
+                DISPATCH_K1G(rows, false, grid, blk, 0, st, L, lk, r, p0, p1, t, x, m2, S, ptt,
+                             prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft);
+            } else {
+                DISPATCH_ROWS_PK(rows, packed, k_cg1v, grid, blk, 0, st, L, lk, r, p0, p1, t, x,
+                                 m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft);
+            }
             P.ntt[k & 1] = fold(ptt, g.nblocks(), 2, st);
             return;
         }
@@ -494,7 +533,8 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
 template <typename T>
 static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> *p0,
                      const c2<T> *p1, c2<T> *x, c2<T> *r, T am0, SolverState *S,
-                     CGPartials &P, int k, int rows, bool mu, hipStream_t st, int packed = 0)
+                     CGPartials &P, int k, int rows, bool mu, hipStream_t st, int packed = 0,
+                     int guard = 0)
 {
     const T m2 = T(2) + am0;
     const double *ptt = P.tt[k & 1];
@@ -508,10 +548,10 @@ static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> 
             const LinksF32 lk{U, packed};
             if (mu) {
                 DISPATCH_ROWS_PK(rows, packed, k_cg2vm, grid, blk, 0, st, L, lk, t, p0, p1, r, m2,
-                                 S, ptt, prr, ntt, k, g.bx, g.by, g.nbx);
+                                 S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, guard);
             } else {
                 DISPATCH_ROWS_PK(rows, packed, k_cg2v, grid, blk, 0, st, L, lk, t, p0, p1, r, m2,
-                                 S, ptt, prr, ntt, k, g.bx, g.by, g.nbx);
+                                 S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, guard);
             }
             P.nrr[k & 1] = fold(prr, g.nblocks(), 1, st);
             return;
@@ -543,16 +583,18 @@ void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const d
 }
 void launch_cg_k1_f32(const Latt &L, const void *U, bool packed, const float2 *r, float2 *p0,
                       float2 *p1, float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P,
-                      int k, int rows, hipStream_t st, bool soft)
+                      int k, int rows, hipStream_t st, bool soft, bool guard)
 {
     cg1_impl<float>(L, (const float2 *)U, r, p0, p1, t, x, am0, s, P, k, rows, st, packed,
-                    soft ? 1 : 0);
+                    soft ? 1 : 0, guard);
 }
 void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t,
                       const float2 *p0, const float2 *p1, float2 *x, float2 *r, float am0,
-                      bool mu, SolverState *s, CGPartials &P, int k, int rows, hipStream_t st)
+                      bool mu, SolverState *s, CGPartials &P, int k, int rows, hipStream_t st,
+                      bool guard)
 {
-    cg2_impl<float>(L, (const float2 *)U, t, p0, p1, x, r, am0, s, P, k, rows, mu, st, packed);
+    cg2_impl<float>(L, (const float2 *)U, t, p0, p1, x, r, am0, s, P, k, rows, mu, st, packed,
+                    guard ? 1 : 0);
 }
 
 // U(1) links packed into one float each (the format link_dec reads): the
