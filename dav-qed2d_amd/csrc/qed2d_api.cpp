@@ -494,9 +494,11 @@ struct ChunkPredictor {
 };
 
 // ---------------------------------------------------------------------------
-// Pipelined solves.  A queued chunk of CG iterations ends with a one-thread
-// kernel that copies the solver state into a host-coherent pinned slot of its
-// own and then writes the chunk's sequence number beside it; the host queues
+// Pipelined solves.  The last K2 of a queued chunk of CG iterations (its
+// block 0, after its last write to the state) copies the solver state into a
+// host-coherent pinned slot of the chunk's own and then writes the chunk's
+// sequence number beside it (a chunk without a K2 ends with a one-thread
+// kernel that does the same); the host queues
 // the next chunk before it waits (polling that number) for the previous one,
 // so the GPU keeps iterating while the host looks at the state.  Chunks are sized from
 // the observed convergence rate to end where the solve is predicted to end; a
@@ -513,16 +515,30 @@ struct ChunkQueue {
     bool back_last() const { return n > 0 && last[back()]; }
 };
 
-int cq_push(qed2d_ctx *c, ChunkQueue &q, bool last, int k0 = 0, bool ru = false)
+// The state-copy slot and sequence number of the next chunk; the chunk's last
+// K2 takes the copy (StateSnap).
+struct ChunkSnap {
+    int slot;
+    StateSnap sn;
+};
+ChunkSnap cq_reserve(qed2d_ctx *c)
 {
     const int s = c->ring_next;
     c->ring_next = (s + 1) % kRing;
     c->seq_no = (c->seq_no == 0x7fffffff) ? 1 : c->seq_no + 1;
-    launch_state_snap(c->state, c->hring + s, c->hseq + s, c->seq_no, c->stream);
+    return ChunkSnap{s, StateSnap{c->hring + s, c->hseq + s, c->seq_no}};
+}
+
+// Enters a queued chunk; taken: its last kernel took the state copy, else a
+// one-thread kernel takes it now.
+int cq_push(qed2d_ctx *c, ChunkQueue &q, const ChunkSnap &cs, bool taken, bool last, int k0 = 0,
+            bool ru = false)
+{
+    if (!taken) launch_state_snap(c->state, cs.sn.out, cs.sn.seq, cs.sn.v, c->stream);
     TRY(launched(c));
     const int i = (q.head + q.n) % kRing;
-    q.want[i] = c->seq_no;
-    q.slot[i] = s;
+    q.want[i] = cs.sn.v;
+    q.slot[i] = cs.slot;
     q.last[i] = last;
     q.k0[i] = k0;
     q.ru[i] = ru;
@@ -618,16 +634,19 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
     ChunkQueue q;
     SolvePlan plan;
     auto queue = [&](int k1, bool last) -> int {
-        for (; k <= k1 && k <= kmax; ++k) {
+        const ChunkSnap cs = cq_reserve(c);
+        const int kend = std::min(k1, kmax), k0 = k;
+        for (; k <= kend; ++k) {
             PROF(c, QED2D_PROF_CG_K1_F64,
                  launch_cg_k1_f64(L, (const double2 *)U, r, P[0], P[1], t, a.am0, c->state, c->P,
                                   k, c->rows, st));
             PROF(c, QED2D_PROF_CG_K2_F64,
                  launch_cg_k2_f64(L, (const double2 *)U, t, P[0], P[1], (double2 *)x, r, a.am0,
-                                  c->state, c->P, k, c->rows_k2 ? c->rows_k2 : c->rows, st));
+                                  c->state, c->P, k, c->rows_k2 ? c->rows_k2 : c->rows, st,
+                                  k == kend ? cs.sn : StateSnap{nullptr, nullptr, 0}));
         }
         TRY(launched(c));
-        return cq_push(c, q, last || k > kmax);
+        return cq_push(c, q, cs, k > k0, last || k > kmax);
     };
     TRY(queue(a.chunk0, false));
     // (a caller's guess may already satisfy the tolerance: no second chunk
@@ -736,7 +755,11 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     // spec: followed by the update of a pause at K1(k1) and K1(k1), K2(k1)
     // again, each a no-op unless K1(k1) paused
     auto queue = [&](int k1, bool ru, bool last, bool spec = false) -> int {
-        const int k0 = k;
+        const int k0 = k, kend = std::min(k1, kmax);
+        const ChunkSnap cs = cq_reserve(c);
+        const StateSnap none{nullptr, nullptr, 0};
+        // (the spec tail runs iff at least one iteration is queued)
+        const bool tail = spec && spec_ok && kend >= k0;
         if (ru) {
             hipEvent_t ev = prof_begin(c, QED2D_PROF_RELIABLE);
             launch_ru_accum(L, (double2 *)x, xs, pend0, P[1], c->state, st, k);
@@ -747,15 +770,16 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
                             st);
             prof_end(c, QED2D_PROF_RELIABLE, ev);
         }
-        for (; k <= k1 && k <= kmax; ++k) {
+        for (; k <= kend; ++k) {
             PROF(c, QED2D_PROF_CG_K1_F32,
                  launch_cg_k1_f32(L, Us, packed, rs, P[0], P[1], ts, xs, am0f, c->state, c->P, k,
                                   c->rows32, st, soft));
             PROF(c, QED2D_PROF_CG_K2_F32,
                  launch_cg_k2_f32(L, Us, packed, ts, P[0], P[1], xs, rs, am0f, a.mu2 != 0.0,
-                                  c->state, c->P, k, rows32_k2, st));
+                                  c->state, c->P, k, rows32_k2, st, false,
+                                  (k == kend && !tail) ? cs.sn : none));
         }
-        if (spec && spec_ok && k > k0) {
+        if (tail) {
             const int kc = k - 1;
             hipEvent_t ev = prof_begin(c, QED2D_PROF_RELIABLE);
             launch_ru_accum(L, (double2 *)x, xs, pend0, P[1], c->state, st, 0, kc);
@@ -768,10 +792,10 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             launch_cg_k1_f32(L, Us, packed, rs, P[0], P[1], ts, xs, am0f, c->state, c->P, kc,
                              c->rows32, st, soft, true);
             launch_cg_k2_f32(L, Us, packed, ts, P[0], P[1], xs, rs, am0f, a.mu2 != 0.0,
-                             c->state, c->P, kc, rows32_k2, st, true);
+                             c->state, c->P, kc, rows32_k2, st, true, cs.sn);
         }
         TRY(launched(c));
-        return cq_push(c, q, last || k > kmax, k0, ru);
+        return cq_push(c, q, cs, k > k0, last || k > kmax, k0, ru);
     };
     // The next chunk from k: its end, whether it starts with an update.  In soft
     // mode an update is planned at the chunk start after the K1 predicted to

@@ -57,6 +57,16 @@ __device__ __forceinline__ const SolverState *state_v(const SolverState *S)
     return (const SolverState *)((const char *)S + z);
 }
 
+// The state copy for the host (StateSnap), by one thread after its last write
+// to S: the copy, then the sequence number with a release at system scope, so
+// that the host that reads the number finds the copy complete.
+__device__ __forceinline__ void put_snap(const StateSnap &sn, const SolverState *S)
+{
+    if (sn.out == nullptr) return;
+    *sn.out = *S;
+    __hip_atomic_store(sn.seq, sn.v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // soft: the mixed solver's on-device reliable updates (SolverState::ru_req; a
 // drop 1e-4 below the trigger pauses the solve instead).
 template <bool mixed>

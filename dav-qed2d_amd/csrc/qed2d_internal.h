@@ -91,6 +91,16 @@ struct SolverState {
     double r0norm;  // |r_0|^2, committed by K1(1) (host information)
 };
 
+// A copy of the solver state for the host: *out = *S in pinned host memory,
+// then v into *seq (release, system scope).  Taken by the K2 that ends a
+// queued chunk (block 0, thread 0, after its last write to S); out == nullptr:
+// no copy.
+struct StateSnap {
+    SolverState *out;
+    int *seq;
+    int v;
+};
+
 // Multi-shift CG state (all shifts sigma_i = mu_i^2 of (Q^2 + sigma_i) x_i = b on
 // one Krylov sequence).  Slots as above: K1(k) handles Krylov index j = k - 1.
 constexpr int kMaxShift = 32;
@@ -315,9 +325,11 @@ void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const do
 void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2 *p0,
                       double2 *p1, double2 *t, double am0, SolverState *s, CGPartials &P,
                       int k, int rows, hipStream_t st);
+// (snap: the state copy of a chunk ending with this K2, see StateSnap)
 void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
                       const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
-                      CGPartials &P, int k, int rows, hipStream_t st);
+                      CGPartials &P, int k, int rows, hipStream_t st,
+                      StateSnap snap = StateSnap{nullptr, nullptr, 0});
 // U: FP32 complex links, or (packed, site-pair kernels only: even L1 >= 4)
 // links packed by launch_link_pack.
 // (soft: on-device reliable updates, see SolverState::ru_req; guard, site-pair
@@ -331,7 +343,7 @@ void launch_cg_k1_f32(const Latt &L, const void *U, bool packed, const float2 *r
 void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t,
                       const float2 *p0, const float2 *p1, float2 *x, float2 *r, float am0,
                       bool mu, SolverState *s, CGPartials &P, int k, int rows, hipStream_t st,
-                      bool guard = false);
+                      bool guard = false, StateSnap snap = StateSnap{nullptr, nullptr, 0});
 // n FP64 U(1) links -> one float each (the packed format of the FP32 site-pair
 // solver); sets S->bad_links if a link is not of unit modulus to 2^-23 (FP32 epsilon).
 void launch_link_pack(const double2 *U, float *out, long n, SolverState *S, hipStream_t st);

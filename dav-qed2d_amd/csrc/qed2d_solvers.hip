@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
                                                 const c2<T> *P1, c2<T> *__restrict__ x,
                                                 c2<T> *__restrict__ r, T mass2, SolverState *S,
                                                 const double *Ptt, double *Prr, int ntt, int k,
-                                                int bx, int by, int nbx)
+                                                int bx, int by, int nbx, StateSnap snap)
 {
     // The solver state's loads go out first, the partials' next, the first
     // rows' last; nothing waits for them before the rows' loads are in flight.
@@ -369,9 +369,11 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
         return true;
     };
     SrcPlain<T> src{t, L.V};
-    if (!q_march_r<T, R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) return;
-    const double a = block_sum_t0(epi.rr);
-    if (threadIdx.x == 0) Prr[p.blk] = a;
+    if (q_march_r<T, R>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) {
+        const double a = block_sum_t0(epi.rr);
+        if (threadIdx.x == 0) Prr[p.blk] = a;
+    }
+    if (p.blk == 0 && threadIdx.x == 0) put_snap(snap, S);
 }
 
 // ---- FP32 pair kernels (two sites per thread along i1) -------------------
@@ -432,7 +434,8 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
                                           const float2 *__restrict__ t, const float2 *P0,
                                           const float2 *P1, float2 *__restrict__ r, float mass2,
                                           SolverState *S, const double *Ptt, double *Prr, int ntt,
-                                          int k, int bx, int by, int nbx, int guard)
+                                          int k, int bx, int by, int nbx, int guard,
+                                          const StateSnap &snap)
 {
     // The solver state's loads go out first, the partials' next, the first
     // rows' last; nothing waits for them before the rows' loads are in flight.
@@ -459,12 +462,14 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
         return true;
     };
     SrcPlain2 src{t, L.V};  // alpha: src/QED2dSolver.jl:21-22
-    if (!q_march2t<R, PK>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) return;
-    const double a = block_sum_t0(epi.rr);
-    if (threadIdx.x == 0) {
-        Prr[p.blk] = a;
-        if (p.blk == 0) S->alpha_x = alpha;  // for K1(k + 1)
+    if (q_march2t<R, PK>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) {
+        const double a = block_sum_t0(epi.rr);
+        if (threadIdx.x == 0) {
+            Prr[p.blk] = a;
+            if (p.blk == 0) S->alpha_x = alpha;  // for K1(k + 1)
+        }
     }
+    if (p.blk == 0 && threadIdx.x == 0) put_snap(snap, S);
 }
 #define CG2V_KERNEL(NAME, MU)                                                              \
     template <int R, bool PK>                                                              \
@@ -472,10 +477,10 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
         NAME(Latt L, LinksF32 U, const float2 *__restrict__ t,                            \
              const float2 *P0, const float2 *P1, float2 *__restrict__ r, float mass2,      \
              SolverState *S, const double *Ptt, double *Prr, int ntt, int k, int bx, int by, \
-             int nbx, int guard)                                                           \
+             int nbx, int guard, StateSnap snap)                                           \
     {                                                                                      \
         cg2v_body<R, MU, PK>(L, U, t, P0, P1, r, mass2, S, Ptt, Prr, ntt, k, bx, by, nbx,  \
-                             guard);                                                       \
+                             guard, snap);                                                 \
     }
 CG2V_KERNEL(k_cg2v, false)
 CG2V_KERNEL(k_cg2vm, true)  // mu2 != 0
@@ -534,7 +539,7 @@ template <typename T>
 static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> *p0,
                      const c2<T> *p1, c2<T> *x, c2<T> *r, T am0, SolverState *S,
                      CGPartials &P, int k, int rows, bool mu, hipStream_t st, int packed = 0,
-                     int guard = 0)
+                     int guard = 0, StateSnap snap = StateSnap{nullptr, nullptr, 0})
 {
     const T m2 = T(2) + am0;
     const double *ptt = P.tt[k & 1];
@@ -548,10 +553,10 @@ static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> 
             const LinksF32 lk{U, packed};
             if (mu) {
                 DISPATCH_ROWS_PK(rows, packed, k_cg2vm, grid, blk, 0, st, L, lk, t, p0, p1, r, m2,
-                                 S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, guard);
+                                 S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, guard, snap);
             } else {
                 DISPATCH_ROWS_PK(rows, packed, k_cg2v, grid, blk, 0, st, L, lk, t, p0, p1, r, m2,
-                                 S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, guard);
+                                 S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, guard, snap);
             }
             P.nrr[k & 1] = fold(prr, g.nblocks(), 1, st);
             return;
@@ -560,11 +565,11 @@ static void cg2_impl(const Latt &L, const c2<T> *U, const c2<T> *t, const c2<T> 
     MarchGrid g = march_grid(L, rows);
     dim3 grid(g.nblocks()), blk(kBlock);
     switch (rows) {
-    case 1: launch_k((k_cg2<T, 1>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
-    case 2: launch_k((k_cg2<T, 2>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
-    case 4: launch_k((k_cg2<T, 4>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
-    case 8: launch_k((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
-    default: launch_k((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx); break;
+    case 1: launch_k((k_cg2<T, 1>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, snap); break;
+    case 2: launch_k((k_cg2<T, 2>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, snap); break;
+    case 4: launch_k((k_cg2<T, 4>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, snap); break;
+    case 8: launch_k((k_cg2<T, 8>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, snap); break;
+    default: launch_k((k_cg2<T, 16>), grid, blk, 0, st, L, U, t, p0, p1, x, r, m2, S, ptt, prr, ntt, k, g.bx, g.by, g.nbx, snap); break;
     }
     P.nrr[k & 1] = fold(prr, g.nblocks(), 1, st);
 }
@@ -577,9 +582,9 @@ void launch_cg_k1_f64(const Latt &L, const double2 *U, const double2 *r, double2
 }
 void launch_cg_k2_f64(const Latt &L, const double2 *U, const double2 *t, const double2 *p0,
                       const double2 *p1, double2 *x, double2 *r, double am0, SolverState *s,
-                      CGPartials &P, int k, int rows, hipStream_t st)
+                      CGPartials &P, int k, int rows, hipStream_t st, StateSnap snap)
 {
-    cg2_impl<double>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, false, st);
+    cg2_impl<double>(L, U, t, p0, p1, x, r, am0, s, P, k, rows, false, st, 0, 0, snap);
 }
 void launch_cg_k1_f32(const Latt &L, const void *U, bool packed, const float2 *r, float2 *p0,
                       float2 *p1, float2 *t, float2 *x, float am0, SolverState *s, CGPartials &P,
@@ -591,10 +596,10 @@ void launch_cg_k1_f32(const Latt &L, const void *U, bool packed, const float2 *r
 void launch_cg_k2_f32(const Latt &L, const void *U, bool packed, const float2 *t,
                       const float2 *p0, const float2 *p1, float2 *x, float2 *r, float am0,
                       bool mu, SolverState *s, CGPartials &P, int k, int rows, hipStream_t st,
-                      bool guard)
+                      bool guard, StateSnap snap)
 {
     cg2_impl<float>(L, (const float2 *)U, t, p0, p1, x, r, am0, s, P, k, rows, mu, st, packed,
-                    guard ? 1 : 0);
+                    guard ? 1 : 0, snap);
 }
 
 // U(1) links packed into one float each (the format link_dec reads): the
@@ -633,10 +638,7 @@ void launch_link_pack(const double2 *U, float *out, long n, SolverState *S, hipS
 // event between the chunks: the next chunk's kernels follow at once.)
 __global__ void k_state_snap(const SolverState *__restrict__ S, SolverState *out, int *seq, int v)
 {
-    if (threadIdx.x == 0) {
-        *out = *S;
-        __hip_atomic_store(seq, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    if (threadIdx.x == 0) put_snap(StateSnap{out, seq, v}, S);
 }
 void launch_state_snap(const SolverState *S, SolverState *out, int *seq, int v, hipStream_t st)
 {
