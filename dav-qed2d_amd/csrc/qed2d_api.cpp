@@ -7,6 +7,7 @@
 // and no synchronisation.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -553,11 +554,16 @@ int cq_pop(qed2d_ctx *c, ChunkQueue &q)
     const int s = q.slot[q.head], want = q.want[q.head];
     q.head = (q.head + 1) % kRing;
     q.n -= 1;
-    // poll the sequence number; now and then ask the stream whether a launch
-    // failed, or whether it drained without the copy arriving
+    // poll the sequence number; every 2 ms ask the stream whether a launch
+    // failed, or whether it drained without the copy arriving (a stream query
+    // is not free on the GPU side: asked every few microseconds, it put a gap
+    // of about 6 us before the chunk queued next)
+    auto t_query = std::chrono::steady_clock::now();
     for (long spin = 1;; ++spin) {
         if (__atomic_load_n(c->hseq + s, __ATOMIC_ACQUIRE) == want) break;
-        if ((spin & 255) == 0) {
+        if ((spin & 255) == 0 &&
+            std::chrono::steady_clock::now() - t_query >= std::chrono::milliseconds(2)) {
+            t_query = std::chrono::steady_clock::now();
             const hipError_t e = hipStreamQuery(c->stream);
             if (e == hipSuccess && __atomic_load_n(c->hseq + s, __ATOMIC_ACQUIRE) != want)
                 return fail(c, QED2D_EHIP, "internal: solver state copy %d did not arrive", want);
