@@ -104,6 +104,17 @@ struct qed2d_ctx {
     int ring_next = 0;
     int seq_no = 0;
     RedOut *hred = nullptr;         // pinned
+    // A Q apply (so = Q si) that the CG solvers queue behind each chunk
+    // predicted to end the solve, run on the device iff the solve has then
+    // converged (the Nf=2 force's Q X, which then needs no host round trip);
+    // tail_ran: whether it ran for the last solve.  Unset: U == nullptr.
+    struct SolveTail {
+        const double *U = nullptr;
+        const double *si = nullptr;
+        double *so = nullptr;
+        double am0 = 0.0;
+    } tail;
+    bool tail_ran = false;
     // kernel-class timing (qed2d_profile_*)
     int prof = 0;  // 0 off, N: time every N-th launch of each class
     long prof_seen[QED2D_PROF_NCLASS] = {};
@@ -511,6 +522,8 @@ struct ChunkQueue {
     int k0[kRing];   // first iteration of the chunk
     bool ru[kRing];  // the chunk starts with an on-device reliable update
     int want[kRing];  // the sequence number of its state copy
+    bool tail[kRing];  // the chunk is followed by the solve's tail (qed2d_ctx::tail)
+    bool popped_tail = false;  // the chunk popped last was
     int head = 0, n = 0;
     int back() const { return (head + n - 1) % kRing; }
     bool back_last() const { return n > 0 && last[back()]; }
@@ -533,11 +546,16 @@ ChunkSnap cq_reserve(qed2d_ctx *c)
 // Enters a queued chunk; taken: its last kernel took the state copy, else a
 // one-thread kernel takes it now.
 int cq_push(qed2d_ctx *c, ChunkQueue &q, const ChunkSnap &cs, bool taken, bool last, int k0 = 0,
-            bool ru = false)
+            bool ru = false, bool tail = false)
 {
     if (!taken) launch_state_snap(c->state, cs.sn.out, cs.sn.seq, cs.sn.v, c->stream);
+    // the tail sees the state the copy holds: it ran iff the copy says converged
+    if (tail)
+        launch_q_apply_ru(c->L, (const double2 *)c->tail.U, (const double2 *)c->tail.si,
+                          (double2 *)c->tail.so, c->tail.am0, c->state, c->rows_q, c->stream, -1);
     TRY(launched(c));
     const int i = (q.head + q.n) % kRing;
+    q.tail[i] = tail;
     q.want[i] = cs.sn.v;
     q.slot[i] = cs.slot;
     q.last[i] = last;
@@ -552,6 +570,7 @@ int cq_pop(qed2d_ctx *c, ChunkQueue &q)
 {
     if (q.n < 1) return fail(c, QED2D_EHIP, "internal: no queued CG chunk to wait for");
     const int s = q.slot[q.head], want = q.want[q.head];
+    q.popped_tail = q.tail[q.head];
     q.head = (q.head + 1) % kRing;
     q.n -= 1;
     // poll the sequence number; every 2 ms ask the stream whether a launch
@@ -574,6 +593,18 @@ int cq_pop(qed2d_ctx *c, ChunkQueue &q)
     }
     std::memcpy((void *)c->hstate, (const void *)(c->hring + s), sizeof(SolverState));
     return QED2D_OK;
+}
+
+// At the end of a solve (the chunk popped last reported it converged, or
+// not): the tail ran iff that chunk or one queued after it carried it (the
+// state stays converged once it is).  Forgets the queued chunks (their
+// launches return at once).
+void cq_finish(qed2d_ctx *c, ChunkQueue &q)
+{
+    bool any = q.popped_tail;
+    for (int j = 0; j < q.n; ++j) any = any || q.tail[(q.head + j) % kRing];
+    c->tail_ran = any && c->hstate->status == 1;
+    q.n = 0;
 }
 
 // What the observed states say about the rest of a solve.
@@ -630,8 +661,9 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
     hipStream_t st = c->stream;
     if (a.guess) {
         q64(c, U, (const double *)x, (double *)t, a.am0);
-        launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x, r,
-                        nullptr, P[0], nullptr, a.am0, c->state, c->P, 1, 0, c->rows, st);
+        launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t,
+                        a.mu2 != 0.0 ? (const double2 *)x : nullptr, r, nullptr, P[0], nullptr,
+                        a.am0, c->state, c->P, 1, 0, c->rows, st);
     } else {
         launch_cg_init_f64(L, (const double2 *)b, (double2 *)x, r, P[0], c->P, c->rows, st);
     }
@@ -639,6 +671,8 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
     int k = 1;
     ChunkQueue q;
     SolvePlan plan;
+    c->tail_ran = false;
+    const bool has_tail = c->tail.U != nullptr;
     auto queue = [&](int k1, bool last) -> int {
         const ChunkSnap cs = cq_reserve(c);
         const int kend = std::min(k1, kmax), k0 = k;
@@ -652,7 +686,7 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
                                   k == kend ? cs.sn : StateSnap{nullptr, nullptr, 0}));
         }
         TRY(launched(c));
-        return cq_push(c, q, cs, k > k0, last || k > kmax);
+        return cq_push(c, q, cs, k > k0, last || k > kmax, 0, false, has_tail && last);
     };
     TRY(queue(a.chunk0, false));
     // (a caller's guess may already satisfy the tolerance: no second chunk
@@ -676,7 +710,7 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
             TRY(queue(k1, last));
         }
     }
-    q.n = 0;  // (launches still queued return at once)
+    cq_finish(c, q);
     *iters = c->hstate->iter;
     if (c->hstate->status == 1) return QED2D_OK;
     return fail(c, QED2D_ENOCONV, "CG not converged after %d iterations (|r|^2 = %g, tol = %g)",
@@ -696,6 +730,8 @@ int cg_f64(qed2d_ctx *c, double *x, const double *U, const double *b, const Solv
 int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const SolveArgs &a,
              int *iters)
 {
+    // (the residual reads x only for its mu^2 x term)
+    const double2 *xmu = a.mu2 != 0.0 ? (const double2 *)x : nullptr;
     float2 *rs, *P[2], *ts, *xs, *Us;
     double2 *t;
     TRY(get_fspin(c, FS_R, &rs));
@@ -744,7 +780,7 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             launch_to_f32((const double2 *)U, Us, 2 * L.V, st);
         if (a.guess) {
             q64(c, U, x, (double *)t, a.am0);
-            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
+            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, xmu,
                             nullptr, rs, nullptr, P[0], a.am0, c->state, c->P, 1, 1, c->rows, st);
             HIPCHK(c, hipMemsetAsync(xs, 0, sizeof(float2) * 2 * L.V, st));
         } else {
@@ -760,7 +796,10 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     // iterations k .. k1, after the on-device update when ru (soft mode);
     // spec: followed by the update of a pause at K1(k1) and K1(k1), K2(k1)
     // again, each a no-op unless K1(k1) paused
-    auto queue = [&](int k1, bool ru, bool last, bool spec = false) -> int {
+    c->tail_ran = false;
+    const bool has_tail = c->tail.U != nullptr;
+    // (tl: the solve's tail follows the chunk)
+    auto queue = [&](int k1, bool ru, bool last, bool spec = false, bool tl = false) -> int {
         const int k0 = k, kend = std::min(k1, kmax);
         const ChunkSnap cs = cq_reserve(c);
         const StateSnap none{nullptr, nullptr, 0};
@@ -771,7 +810,7 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             launch_ru_accum(L, (double2 *)x, xs, pend0, P[1], c->state, st, k);
             launch_q_apply_ru(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->state,
                               c->rows_q, st);
-            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
+            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, xmu,
                             nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, k, 3, rows_r,
                             st);
             prof_end(c, QED2D_PROF_RELIABLE, ev);
@@ -791,7 +830,7 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             launch_ru_accum(L, (double2 *)x, xs, pend0, P[1], c->state, st, 0, kc);
             launch_q_apply_ru(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->state,
                               c->rows_q, st, kc);
-            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
+            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, xmu,
                             nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, kc, 4, rows_r,
                             st);
             prof_end(c, QED2D_PROF_RELIABLE, ev);
@@ -801,7 +840,7 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
                              c->state, c->P, kc, rows32_k2, st, true, cs.sn);
         }
         TRY(launched(c));
-        return cq_push(c, q, cs, k > k0, last || k > kmax, k0, ru);
+        return cq_push(c, q, cs, k > k0, last || k > kmax, k0, ru, has_tail && tl);
     };
     // The next chunk from k: its end, whether it starts with an update.  In soft
     // mode an update is planned at the chunk start after the K1 predicted to
@@ -887,14 +926,15 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             // (the site-pair kernels pause with K2(k-1)'s x update pending)
             launch_ru_accum(L, (double2 *)x, xs, pend0, P[1], c->state, st);
             launch_q_apply_f64(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->rows_q, st);
-            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, (const double2 *)x,
+            launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, xmu,
                             nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, k, 2, c->rows,
                             st);
             prof_end(c, QED2D_PROF_RELIABLE, ev);
             kb_last = k;
             plan.observe(h);
-            // K1(k) decides at once whether the true residual converged
-            TRY(queue(k, false, false));
+            // K1(k) decides at once whether the true residual converged (the
+            // tail follows when the pause came from a residual that looks so)
+            TRY(queue(k, false, false, false, h.err < h.tol));
             continue;
         }
         plan.observe(h);
@@ -902,10 +942,10 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             int k1;
             bool ru, last;
             plan_next(h, &k1, &ru, &last);
-            TRY(queue(k1, ru, last, last));
+            TRY(queue(k1, ru, last, last, last));
         }
     }
-    q.n = 0;  // (launches still queued return at once)
+    cq_finish(c, q);
     *iters = c->hstate->iter;
     if (c->hstate->status == 1) return QED2D_OK;
     return fail(c, QED2D_ENOCONV,
@@ -1240,9 +1280,13 @@ int update_momenta_nf2(qed2d_ctx *c, double *mom, const double *U, double *X, co
                        int precision, int *iters)
 {
     int it = 0;
-    TRY(cg_any(c, X, U, F, am0, 0.0, QED2D_OP_SQR_MSQ, maxiter, tol, 0, precision, &it));
+    // g5DX = Q X, queued by the solver behind its last chunk where it can be
+    c->tail = qed2d_ctx::SolveTail{U, X, g5DX, am0};
+    const int rc = cg_any(c, X, U, F, am0, 0.0, QED2D_OP_SQR_MSQ, maxiter, tol, 0, precision, &it);
+    c->tail = qed2d_ctx::SolveTail{};
+    TRY(rc);
     *iters += it;
-    q64(c, U, X, g5DX, am0);
+    if (!c->tail_ran) q64(c, U, X, g5DX, am0);
     PROF(c, QED2D_PROF_FORCE_MOM,
          launch_force_mom(c->L, (const double2 *)U, (const double2 *)X, (const double2 *)g5DX,
                           nullptr, mom, c->beta, eps, 1.0, c->stream));

@@ -297,7 +297,12 @@ struct EpiResid {
     struct Pre {
         double2 b1, b2, x1, x2;
     };
-    __device__ __forceinline__ Pre pre(long s) const { return Pre{b[s], b[V + s], x[s], x[V + s]}; }
+    // (x == nullptr: x = 0, for mu2 == 0, so that x is not read)
+    __device__ __forceinline__ Pre pre(long s) const
+    {
+        if (x == nullptr) return Pre{b[s], b[V + s], double2{}, double2{}};
+        return Pre{b[s], b[V + s], x[s], x[V + s]};
+    }
     __device__ __forceinline__ void one(long i, double2 o, double2 bv, double2 xv)
     {
         const double2 rv = mk<double>(bv.x - (o.x + mu2 * xv.x), bv.y - (o.y + mu2 * xv.y));

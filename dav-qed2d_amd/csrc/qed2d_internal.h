@@ -310,7 +310,8 @@ void launch_cg_init_f64(const Latt &L, const double2 *b, double2 *x, double2 *r,
 // Mixed precision start from x = 0: x = 0, rs = float(b), xs = 0, p0 = 0.
 void launch_cg_init_mixed(const Latt &L, const double2 *b, double2 *x, float2 *rs, float2 *xs,
                           float2 *p0, CGPartials &P, int rows, hipStream_t st);
-// Residual from x with t = Q x already computed: r = b - (Q t + mu^2 x).
+// Residual from x with t = Q x already computed: r = b - (Q t + mu^2 x)
+// (x may be null when mu^2 = 0: it is then not read).
 // mode 0: FP64 start from a guess (writes r, zeroes p0, |b|^2 partials);
 // mode 1: mixed start from a guess (writes rs, zeroes p0s, |b|^2 partials);
 // mode 2: mixed reliable update before K1(k) (writes rs);
@@ -357,7 +358,8 @@ void launch_state_snap(const SolverState *S, SolverState *out, int *seq, int v, 
 void launch_ru_accum(const Latt &L, double2 *x, float2 *xs, const float2 *p0, const float2 *p1,
                      const SolverState *S, hipStream_t st, int kb = 0, int pk = 0);
 // so = Q si while a soft reliable-update request is pending (SolverState::ru_req),
-// or, pk > 0, while K1(pk) is paused for an update.
+// or, pk > 0, while K1(pk) is paused for an update, or, pk < 0, once the solve
+// has converged (status == 1).
 void launch_q_apply_ru(const Latt &L, const double2 *U, const double2 *si, double2 *so,
                        double am0, const SolverState *S, int rows, hipStream_t st, int pk = 0);
 // Partials a consumer sums without a fold first (kPartMax of the device code).

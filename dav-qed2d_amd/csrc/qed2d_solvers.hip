@@ -205,7 +205,8 @@ void launch_cg_resid(const Latt &L, const double2 *U, const double2 *b, const do
 
 // Predicated FP64 apply of the on-device reliable update: so = Q si while a
 // soft request is pending (see soft_ru_pending).
-// (pk > 0: instead while K1(pk) is paused for an update, need_ru == pk)
+// (pk > 0: instead while K1(pk) is paused for an update, need_ru == pk;
+// pk < 0: once the solve has converged, status == 1)
 template <int R>
 __global__ void __launch_bounds__(kBlock) k_q_apply_ru(Latt L, const double2 *__restrict__ U,
                                                        const double2 *__restrict__ si,
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_q_apply_ru(Latt L, const double2 *__
                                                        const SolverState *S, int bx, int by,
                                                        int nbx, int pk)
 {
-    if (pk > 0 ? S->need_ru != pk : !soft_ru_pending(S)) return;
+    if (pk > 0 ? S->need_ru != pk : pk < 0 ? S->status != 1 : !soft_ru_pending(S)) return;
     MarchPos p = march_pos(L, bx, by, nbx, R);
     if (!p.active) return;
     SrcPlain<double> src{si, L.V};
