@@ -1018,10 +1018,11 @@ void launch_force_mom(const Latt &L, const double2 *U, const double2 *X, const d
                       mom, beta, eps, g.bx, g.by, g.nbx);
         return;
     }
-    // with the fermion force: 16 rows where the lattice leaves a block per CU
+    // with the fermion force: 8 rows where the lattice leaves a block per CU
     // (the halo rows every tile re-reads are 1/rows of its X, Y and link
-    // loads: at 16 rows the counters read within 6 % of 128 B/site at 1024^2)
-    int rows = 16;
+    // loads, mostly from L2; at 1024^2, 16 rows leave one block per CU and ran
+    // 30.4 us, 8 rows 25.9-27.3 us, 4 rows 28.6-30.1 us)
+    int rows = 8;
     while (rows > 1 && march_grid(L, rows).nblocks() < 256) rows >>= 1;
     const MarchGrid g = march_grid(L, rows);
     DISPATCH_ROWS(rows, k_force_mom, dim3(g.nblocks()), dim3(kBlock), 0, st, L, U, X, Y, facc,
