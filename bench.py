@@ -34,9 +34,11 @@ HBM_PEAK_GBS = 8000.0
 # components or 2 link directions per site).
 BYTES_PER_SITE = {
     # p = r + beta p_old (reads r, p_old), t = Q p (reads U), writes p, t; the
-    # FP32 site-pair K1 also applies the previous iteration's x += a p_old
-    # (x read+write) and reads its U(1) links packed, one float each (8 B/site)
-    "cg_k1_f64": 32 * 5, "cg_k1_f32": 16 * 6 + 8,
+    # FP32 site-pair K1 reads its U(1) links packed, one float each (8 B/site),
+    # and every other one (even k) applies the two pending x updates
+    # x += a p_{k-2} + a p_old (x read+write, p_{k-2} read): 72 and 120 B/site,
+    # 96 on average over the alternating launches
+    "cg_k1_f64": 32 * 5, "cg_k1_f32": (16 * 4 + 8 + 16 * 7 + 8) // 2,
     # Ap = Q t (+ mu^2 p) (reads t, U), r -= a Ap (r read+write); FP64 also
     # x += a p (reads p, x read+write); FP32: packed links
     "cg_k2_f64": 32 * 7, "cg_k2_f32": 16 * 3 + 8,
@@ -154,7 +156,7 @@ def main():
             Q.set_tiles(prm, cg64_k2=tv[4], cg32_k2=tv[5])
     elif not (args.no_autotune or args.tile or args.tile32):
         t0 = time.time()
-        tiles = Q.autotune(prm, 40)
+        tiles = Q.autotune(prm, 160)
         log(f"autotuned tiles {tiles} in {time.time() - t0:.2f}s")
     tiles = Q.get_tiles(prm)
 
@@ -1093,7 +1095,10 @@ def rocprof_summary(argv):
     print("| kernel | dispatches | FETCH_SIZE KiB (median, dispatches > 1 MiB) | "
           "WRITE_SIZE KiB (median) | HBM MB per launch (scaled) |")
     print("|---|---|---|---|---|")
-    traffic = {}
+    # a class's traffic: the mean over its kernels' medians, weighted by their
+    # working dispatches (instances of one class, such as the FP32 K1 of even
+    # and of odd k, alternate)
+    acc = {}
     for n in sorted(set(F) | set(W), key=lambda n: -np.sum(F.get(n, [0]))):
         f = F.get(n, np.zeros(1))
         w = W.get(n, np.zeros(1))
@@ -1103,10 +1108,30 @@ def rocprof_summary(argv):
         mb = (fm * fcal + wm * wcal) * 1024 / 1e6
         print(f"| {n} | {len(f)} | {fm:.0f} | {wm:.0f} | {mb:.1f} |")
         for pat, cls in KERNEL_CLASS:
-            if n.startswith(pat) and cls not in traffic:
-                traffic[cls] = {"kernel": n, "lattice": [a.L, a.L], "fetch_kib": fm,
-                                "write_kib": wm, "fetch_scale": fcal, "write_scale": wcal,
-                                "hbm_bytes_per_launch": int(round((fm * fcal + wm * wcal) * 1024))}
+            if n.startswith(pat):
+                # (a class is the first pattern's kernels: k_cg1v before k_cg1<float)
+                if cls in acc and not acc[cls]["pat"] == pat:
+                    break
+                e = acc.setdefault(cls, {"pat": pat, "names": [], "nw": 0, "f": 0.0, "w": 0.0})
+                nw = max(len(fb), 1)
+                e["names"].append(n)
+                e["nw"] += nw
+                e["f"] += fm * nw
+                e["w"] += wm * nw
+                break
+    traffic = {}
+    for cls, e in acc.items():
+        fm, wm = e["f"] / e["nw"], e["w"] / e["nw"]
+        traffic[cls] = {"kernel": " + ".join(e["names"]), "lattice": [a.L, a.L],
+                        "fetch_kib": fm, "write_kib": wm, "fetch_scale": fcal,
+                        "write_scale": wcal,
+                        "hbm_bytes_per_launch": int(round((fm * fcal + wm * wcal) * 1024))}
+    if len(acc):
+        print()
+        print("| class | kernels | HBM MB per launch (dispatch-weighted) |")
+        print("|---|---|---|")
+        for cls, t in traffic.items():
+            print(f"| {cls} | {t['kernel']} | {t['hbm_bytes_per_launch'] / 1e6:.1f} |")
     if a.json:
         with open(a.json, "w") as fo:
             json.dump({"source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py",

@@ -1904,28 +1904,38 @@ int autotune_impl(qed2d_ctx *c, int iters)
     c->ru_delta = 1e-30;
     int it;
     // (the two kernels of an iteration are timed together, every pair of
-    // heights: K1 and K2 may prefer different ones)
+    // heights: K1 and K2 may prefer different ones, and one's height changes
+    // the other's time through the caches.  Two passes over all pairs, the
+    // times of a pair added: the pairs lie within a few per cent of each
+    // other, and a drift of the clocks during one pass would otherwise pick.)
     for (int prec = 0; prec < 2 && !rc; ++prec) {
         int &tile = prec ? c->rows32 : c->rows;
         int &tile2 = prec ? c->rows32_k2 : c->rows_k2;
         const int keep = tile, keep2 = tile2;
-        int pick2 = keep2;
-        best = 1e30, pick = keep;
-        for (int R : cand) {
-            if (prec == 0 && R == 1) continue;
-            for (int R2 : cand) {
-                if (prec == 0 && R2 == 1) continue;
-                tile = R;
-                tile2 = R2;
-                rc = time_runs(c, 3, [&] {
-                    if (prec) (void)cg_mixed(c, (double *)x, (const double *)U, (const double *)b, a, &it);
-                    else (void)cg_f64(c, (double *)x, (const double *)U, (const double *)b, a, &it);
-                }, &us);
-                if (rc) break;
-                if (us < best) best = us, pick = R, pick2 = R2;
+        double tsum[5][5] = {};
+        for (int pass = 0; pass < 2 && !rc; ++pass) {
+            for (int i = 0; i < 5 && !rc; ++i) {
+                if (prec == 0 && cand[i] == 1) continue;
+                for (int j = 0; j < 5; ++j) {
+                    if (prec == 0 && cand[j] == 1) continue;
+                    tile = cand[i];
+                    tile2 = cand[j];
+                    rc = time_runs(c, 3, [&] {
+                        if (prec) (void)cg_mixed(c, (double *)x, (const double *)U, (const double *)b, a, &it);
+                        else (void)cg_f64(c, (double *)x, (const double *)U, (const double *)b, a, &it);
+                    }, &us);
+                    if (rc) break;
+                    tsum[i][j] += us;
+                }
             }
-            if (rc) break;
         }
+        best = 1e30;
+        int pick2 = keep2;
+        pick = keep;
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j)
+                if (tsum[i][j] > 0.0 && tsum[i][j] < best)
+                    best = tsum[i][j], pick = cand[i], pick2 = cand[j];
         tile = rc ? keep : pick;
         tile2 = rc ? keep2 : (pick2 == pick ? 0 : pick2);
     }

@@ -207,17 +207,32 @@ struct EpiK1v {
     bool ld;      // load x with the rows (set wherever upd may end up set)
     long V;
     double tt, pp;
+    // Two pending updates (see SolverState::alpha_xk): p_{k-2} is read from
+    // pnew at the thread's own sites before p_k overwrites it there (ldp2:
+    // load it; two: apply alpha2 p_{k-2} first).
+    float alpha2 = 0.0f;
+    bool two = false;
+    bool ldp2 = false;
     struct Pre {
-        F2x2 x1, x2, p1, p2;
+        F2x2 x1, x2, p1, p2, q1, q2;
     };
     // p_{k-1} of the row comes with the source's raw loads (SrcXpay2::Raw2)
     template <typename Raw> __device__ __forceinline__ Pre pre_raw(long s, const Raw &w) const
     {
         if (!ld) return Pre{};
-        return Pre{ld2(x, s), ld2(x, V + s), w.p0, w.p1};
+        Pre q{ld2(x, s), ld2(x, V + s), w.p0, w.p1, F2x2{}, F2x2{}};
+        if (ldp2) {
+            q.q1 = ld2(pnew, s);
+            q.q2 = ld2(pnew, V + s);
+        }
+        return q;
     }
-    __device__ __forceinline__ void xupd(long i, F2x2 xv, F2x2 pv)
+    __device__ __forceinline__ void xupd(long i, F2x2 xv, F2x2 pv, F2x2 qv)
     {
+        if (two) {
+            xv.a0 = xpa(xv.a0, alpha2, qv.a0);
+            xv.a1 = xpa(xv.a1, alpha2, qv.a1);
+        }
         st2(x, i, xpa(xv.a0, alpha, pv.a0), xpa(xv.a1, alpha, pv.a1));
     }
     __device__ __forceinline__ void pair(long s, const Pre &q, float2 oa1, float2 oa2, float2 ob1,
@@ -228,8 +243,8 @@ struct EpiK1v {
         st2(t, s, oa1, ob1);
         st2(t, V + s, oa2, ob2);
         if (upd) {
-            xupd(s, q.x1, q.p1);
-            xupd(V + s, q.x2, q.p2);
+            xupd(s, q.x1, q.p1, q.q1);
+            xupd(V + s, q.x2, q.p2, q.q2);
         }
         tt += (double)(cabs2<float>(oa1) + cabs2<float>(ob1)) +
               (double)(cabs2<float>(oa2) + cabs2<float>(ob2));

@@ -89,6 +89,11 @@ struct SolverState {
     // and the iteration limit pause the solve for the host instead.
     int ru_req;
     double r0norm;  // |r_0|^2, committed by K1(1) (host information)
+    // FP32 site-pair iterations: alpha_j in slot j & 1 (written by K2(j)'s
+    // block 0).  Only the K1 of an even k applies the pending x updates, those
+    // of p_{k-2} (if K2(k-2) ran after the last true residual) and p_{k-1}, in
+    // that order; the K1 of an odd k leaves its one pending update alone.
+    double alpha_xk[2];
 };
 
 // A copy of the solver state for the host: *out = *S in pinned host memory,

@@ -234,8 +234,10 @@ int cg2_f32_blocks(const Latt &L, int rows)
     return pair_ok(L) ? march_grid2(L, rows).nblocks() : march_grid(L, rows).nblocks();
 }
 
-// x += double(xs); xs = 0 (with the pending xs += alpha_x p_{k - 1} first when
-// P0 is given, k = stop_k; kb > 0: the on-device update before K1(kb), k = kb,
+// x += double(xs); xs = 0 (with the pending xs updates first when P0 is
+// given, before K1(k), k = stop_k: that of p_{k-2} if k is even and K2(k-2)
+// ran after the last true residual, then that of p_{k-1}, see
+// SolverState::alpha_xk; kb > 0: the on-device update before K1(kb), k = kb,
 // run only while a soft request is pending)
 __global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs,
                            const float2 *__restrict__ P0, const float2 *__restrict__ P1,
@@ -246,15 +248,22 @@ __global__ void k_ru_accum(double2 *__restrict__ x, float2 *__restrict__ xs,
     } else if (kb > 0 && !soft_ru_pending(S)) {
         return;
     }
-    const float2 *pp = nullptr;
-    float a = 0.0f;
+    const float2 *pp = nullptr, *pq = nullptr;
+    float a = 0.0f, a2 = 0.0f;
     if (P0) {
-        pp = (((pk > 0 ? pk : kb > 0 ? kb : S->stop_k) - 1) & 1) ? P1 : P0;
+        const int k = pk > 0 ? pk : kb > 0 ? kb : S->stop_k;
+        pp = ((k - 1) & 1) ? P1 : P0;
         a = (float)S->alpha_x;
+        const int t0 = S->true_at > 1 ? S->true_at : 1;
+        if ((k & 1) == 0 && k - 2 >= t0) {
+            pq = (k & 1) ? P1 : P0;
+            a2 = (float)S->alpha_xk[k & 1];
+        }
     }
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
          i += (long)gridDim.x * blockDim.x) {
         float2 d = xs[i];
+        if (pq) d = xpa(d, a2, pq[i]);
         if (pp) d = xpa(d, a, pp[i]);
         const double2 v = x[i];
         x[i] = mk<double>(v.x + (double)d.x, v.y + (double)d.y);
@@ -380,8 +389,10 @@ __global__ void __launch_bounds__(kBlock) k_cg2(Latt L, const c2<T> *__restrict_
 // ---- FP32 pair kernels (two sites per thread along i1) -------------------
 // G: K1(k) queued again behind a mode-4 reliable update, run only if that
 // update ran (K1(k) had paused and was resumed); a separate instance, so the
-// check costs the ordinary K1 nothing.
-template <int R, bool PK, bool G = false>
+// check costs the ordinary K1 nothing.  F: k is even, the K1 that applies the
+// pending x updates (SolverState::alpha_xk); the K1 of an odd k neither reads
+// nor writes x.
+template <int R, bool PK, bool G, bool F>
 __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
                                                  const float2 *__restrict__ r, float2 *P0,
                                                  float2 *P1, float2 *__restrict__ t,
@@ -400,11 +411,15 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
     const K1In in = k1_in(Sv, k);
     const float ax = (float)Sv->alpha_x;
     const int w = k & 1, rd = (k - 1) & 1;
+    // (only an even k applies the pending x updates, see SolverState::alpha_xk)
+    constexpr bool flush = F;
+    const float ax2 = flush ? (float)Sv->alpha_xk[w] : 0.0f;
     const MarchPos p = march_pos2(L, bx, by, nbx, R);
     PartialsAhead prr;
     prr.load(Prr, nrr, 1, 0);
     float2 *pold = rd ? P1 : P0;
-    EpiK1v epi{w ? P1 : P0, t, x, 0.0f, false, k != 1, L.V, 0.0, 0.0};
+    EpiK1v epi{w ? P1 : P0, t, x, 0.0f, false, flush, L.V, 0.0, 0.0};
+    epi.ldp2 = flush;
     SrcXpay2 src{r, pold, 0.0f, L.V};
     int gate = 1;
     if constexpr (G) gate = (Sv->need_ru == k && in.true_at == k) ? 1 : 0;
@@ -418,7 +433,11 @@ __global__ void __launch_bounds__(kBlock) k_cg1v(Latt L, LinksF32 U,
             return false;
         src.beta = (float)beta;
         epi.alpha = ax;
-        epi.upd = !fresh;
+        epi.alpha2 = ax2;
+        epi.upd = flush && !fresh;
+        // p_{k-2}'s update is pending if K2(k-2) ran after the last true
+        // residual (K1(true_at), or K1(1))
+        epi.two = k - 2 >= (in.true_at > 1 ? in.true_at : 1);
         return true;
     };
     if (!q_march2t<R, PK>(L, U, mass2, src, epi, p.i1, p.r0, p.active, ready)) return;
@@ -467,7 +486,10 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
         const double a = block_sum_t0(epi.rr);
         if (threadIdx.x == 0) {
             Prr[p.blk] = a;
-            if (p.blk == 0) S->alpha_x = alpha;  // for K1(k + 1)
+            if (p.blk == 0) {
+                S->alpha_x = alpha;  // for K1(k + 1)
+                S->alpha_xk[k & 1] = alpha;  // for K1(k + 1), K1(k + 2)
+            }
         }
     }
     if (p.blk == 0 && threadIdx.x == 0) put_snap(snap, S);
@@ -486,15 +508,26 @@ __device__ __forceinline__ void cg2v_body(const Latt &L, const LinksF32 &U,
 CG2V_KERNEL(k_cg2v, false)
 CG2V_KERNEL(k_cg2vm, true)  // mu2 != 0
 
-// k_cg1v<R, PK, true> (the guarded K1, see k_cg1v)
-#define DISPATCH_K1G(rows, PK, ...)                                                   \
-    switch (rows) {                                                                   \
-    case 1: launch_k((k_cg1v<1, PK, true>), __VA_ARGS__); break;                      \
-    case 2: launch_k((k_cg1v<2, PK, true>), __VA_ARGS__); break;                      \
-    case 4: launch_k((k_cg1v<4, PK, true>), __VA_ARGS__); break;                      \
-    case 8: launch_k((k_cg1v<8, PK, true>), __VA_ARGS__); break;                      \
-    default: launch_k((k_cg1v<16, PK, true>), __VA_ARGS__); break;                    \
+// k_cg1v<R, PK, G, F> by rows
+template <bool PK, bool G, bool F, typename... A>
+static void k1v_rows(int rows, dim3 grid, dim3 blk, hipStream_t st, A... args)
+{
+    switch (rows) {
+    case 1: launch_k((k_cg1v<1, PK, G, F>), grid, blk, 0, st, args...); break;
+    case 2: launch_k((k_cg1v<2, PK, G, F>), grid, blk, 0, st, args...); break;
+    case 4: launch_k((k_cg1v<4, PK, G, F>), grid, blk, 0, st, args...); break;
+    case 8: launch_k((k_cg1v<8, PK, G, F>), grid, blk, 0, st, args...); break;
+    default: launch_k((k_cg1v<16, PK, G, F>), grid, blk, 0, st, args...); break;
     }
+}
+template <bool PK, bool G, typename... A>
+static void k1v_launch(bool even, int rows, dim3 grid, dim3 blk, hipStream_t st, A... args)
+{
+    if (even)
+        k1v_rows<PK, G, true>(rows, grid, blk, st, args...);
+    else
+        k1v_rows<PK, G, false>(rows, grid, blk, st, args...);
+}
 
 template <typename T>
 static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c2<T> *p1,
@@ -510,16 +543,18 @@ static void cg1_impl(const Latt &L, const c2<T> *U, const c2<T> *r, c2<T> *p0, c
             MarchGrid g = march_grid2(L, rows);
             dim3 grid(g.nblocks()), blk(kBlock);
             const LinksF32 lk{U, packed};
-            if (guard && packed) {
-                DISPATCH_K1G(rows, true, grid, blk, 0, st, L, lk, r, p0, p1, t, x, m2, S, ptt,
-                             prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft);
-            } else if (guard) {
-                DISPATCH_K1G(rows, false, grid, blk, 0, st, L, lk, r, p0, p1, t, x, m2, S, ptt,
-                             prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft);
-            } else {
-                DISPATCH_ROWS_PK(rows, packed, k_cg1v, grid, blk, 0, st, L, lk, r, p0, p1, t, x,
-                                 m2, S, ptt, prr, P.b, nrr, nbb, k, g.bx, g.by, g.nbx, soft);
-            }
+            const bool even = (k & 1) == 0;
+#define K1V_ARGS L, lk, r, p0, p1, t, x, m2, S, ptt, prr, (const double *)P.b, nrr, nbb, k, g.bx, \
+                 g.by, g.nbx, soft
+            if (packed && guard)
+                k1v_launch<true, true>(even, rows, grid, blk, st, K1V_ARGS);
+            else if (packed)
+                k1v_launch<true, false>(even, rows, grid, blk, st, K1V_ARGS);
+            else if (guard)
+                k1v_launch<false, true>(even, rows, grid, blk, st, K1V_ARGS);
+            else
+                k1v_launch<false, false>(even, rows, grid, blk, st, K1V_ARGS);
+#undef K1V_ARGS
             P.ntt[k & 1] = fold(ptt, g.nblocks(), 2, st);
             return;
         }
