@@ -747,10 +747,12 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
     const int rows32_k2 = c->rows32_k2 ? c->rows32_k2 : c->rows32;
     // soft mode: the on-device update's residual partials take the count K2
     // leaves in the slot: folded to one (K2 grids past kPartsUnfolded), or
-    // padded, which needs a residual grid no larger than K2's
+    // padded, which needs a residual grid no larger than K2's (the largest
+    // such grid: at 1024^2 with K2 at 4 rows, 8 rows per thread ran the
+    // residual in less time than 16, which leave one block per CU)
     const int nk2 = cg2_f32_blocks(L, rows32_k2);
     int rows_r = nk2 > kPartsUnfolded ? c->rows : 0;
-    for (int R : {16, 8, 4, 2, 1})
+    for (int R : {1, 2, 4, 8, 16})
         if (!rows_r && resid_blocks(L, R) <= nk2) rows_r = R;
     const bool soft = c->ru_mode && rows_r > 0;
     // In soft mode the chunk predicted to end the solve carries the reliable
@@ -927,8 +929,8 @@ int cg_mixed(qed2d_ctx *c, double *x, const double *U, const double *b, const So
             launch_ru_accum(L, (double2 *)x, xs, pend0, P[1], c->state, st);
             launch_q_apply_f64(L, (const double2 *)U, (const double2 *)x, t, a.am0, c->rows_q, st);
             launch_cg_resid(L, (const double2 *)U, (const double2 *)b, t, xmu,
-                            nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, k, 2, c->rows,
-                            st);
+                            nullptr, rs, nullptr, nullptr, a.am0, c->state, c->P, k, 2,
+                            rows_r ? rows_r : c->rows, st);
             prof_end(c, QED2D_PROF_RELIABLE, ev);
             kb_last = k;
             plan.observe(h);
