@@ -180,9 +180,10 @@ def get_tiles(prm):
     return dict(zip(("q64", "q32", "cg64", "cg32", "cg64_k2", "cg32_k2"), list(t) + list(t2)))
 
 
-def autotune(prm, iters=40):
+def autotune(prm, iters=160):
     """Times every tile height of each kernel family on this lattice and keeps
-    the fastest (returns the choice, as get_tiles)."""
+    the fastest (returns the choice, as get_tiles); the CG families over
+    `iters` iterations, every pair of the two kernels' heights in two passes."""
     t = (_i * 4)()
     _check(prm, lib.qed2d_ctx_autotune(prm.ctx(), int(iters), t))
     return get_tiles(prm)

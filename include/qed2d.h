@@ -96,7 +96,8 @@ int qed2d_ctx_get_tiles_k2(qed2d_ctx *ctx, int tiles[2]);
 /* The autotuner: times each tile height of the four families on this
  * context's lattice (scratch fields; CG families over `iters` iterations of a
  * solve that cannot converge, every pair of heights of the iteration's two
- * kernels) and keeps the fastest.  tiles (may be NULL)
+ * kernels, in two passes) and keeps the fastest; the pairs differ by a few per
+ * cent, which 100 or more iterations resolve.  tiles (may be NULL)
  * receives the choice.  Tile heights change the order of the solvers' global
  * sums, so a run that must reproduce bits fixes its tiles instead. */
 int qed2d_ctx_autotune(qed2d_ctx *ctx, int iters, int tiles[4]);
