@@ -622,20 +622,28 @@ int batch_autotune(qed2d_batch *b, int iters)
         int &t2 = prec ? b->rows32_k2 : b->rows_k2;
         const int keep1 = t1, keep2 = t2;
         int p1 = keep1, p2 = keep2;
-        best = 1e30;
-        for (int R1 : cand) {
-            for (int R2 : cand) {
-                t1 = R1;
-                t2 = R2;
-                rc = timed([&] {
-                    return batch_cg(b, b->X, b->F, iters, 0.0,
-                                    prec ? QED2D_PREC_MIXED : QED2D_PREC_F64, it, iters + 1);
-                }, &us);
-                if (rc) break;
-                if (us < best) best = us, p1 = R1, p2 = R2;
+        // (two passes over all pairs, the times of a pair added: the pairs lie
+        // within a few per cent of each other)
+        double tsum[5][5] = {};
+        for (int pass = 0; pass < 2 && !rc; ++pass) {
+            for (int i = 0; i < 5 && !rc; ++i) {
+                for (int j = 0; j < 5; ++j) {
+                    t1 = cand[i];
+                    t2 = cand[j];
+                    rc = timed([&] {
+                        return batch_cg(b, b->X, b->F, iters, 0.0,
+                                        prec ? QED2D_PREC_MIXED : QED2D_PREC_F64, it, iters + 1);
+                    }, &us);
+                    if (rc) break;
+                    tsum[i][j] += us;
+                }
             }
-            if (rc) break;
         }
+        best = 1e30;
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j)
+                if (tsum[i][j] > 0.0 && tsum[i][j] < best)
+                    best = tsum[i][j], p1 = cand[i], p2 = cand[j];
         t1 = rc ? keep1 : p1;
         t2 = rc ? keep2 : (p2 == p1 ? 0 : p2);
     }

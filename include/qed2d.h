@@ -312,9 +312,10 @@ int qed2d_batch_set_links(qed2d_batch *b, int chain, const double *src);
  * rows) that leave a solver launch of at least 1024 blocks over all chains.
  * set_tiles_k2: the second CG kernel's heights (0: the family's).  get_tiles:
  * q64, cg64, cg32, cg64 of K2, cg32 of K2 in use.  autotune: times every apply
- * height and every (K1, K2) pair of the FP64 and FP32 CG on the batch's own
- * lattices and links over `iters` iterations (the pseudofermion fields serve
- * as scratch) and keeps the fastest.  No counterpart in the reference
+ * height and every (K1, K2) pair of the FP64 and FP32 CG (the pairs in two
+ * passes) on the batch's own lattices and links over `iters` iterations (the
+ * pseudofermion fields serve as scratch) and keeps the fastest.  No
+ * counterpart in the reference
  * (KernelParm, src/QED2d.jl:10-13). */
 int qed2d_batch_set_tiles(qed2d_batch *b, int q64, int cg64, int cg32);
 int qed2d_batch_set_tiles_k2(qed2d_batch *b, int cg64, int cg32);
